@@ -1,0 +1,157 @@
+"""ctypes binding of ``libfss_hip.so`` (the C ABI declared in ``include/fss.h``).
+
+The library is built in-tree by :func:`fast_shepherd_score_amd.build.build_hip`.  Loading fails
+loudly when it is missing: there is no CPU fallback anywhere in this package.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import threading
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libfss_hip.so")
+
+# Mode / similarity ids (must match include/fss.h).
+MODE_SHAPE = 0
+MODE_SHAPE_ESP = 1
+MODE_PHARM = 2
+MODE_SHAPE_PHARM = 3
+SIM_TANIMOTO = 0
+SIM_TVERSKY = 1
+N_PTYPES = 9
+
+E_INVALID = 9001
+E_TOO_LARGE = 9002
+
+_i32p = ctypes.POINTER(ctypes.c_int32)
+_f32p = ctypes.POINTER(ctypes.c_float)
+_f64p = ctypes.POINTER(ctypes.c_double)
+
+
+class FssPairBatch(ctypes.Structure):
+    _fields_ = [
+        ("n_pairs", ctypes.c_int32),
+        ("ref_pts", ctypes.c_void_p), ("fit_pts", ctypes.c_void_p),
+        ("ref_rng", ctypes.c_void_p), ("fit_rng", ctypes.c_void_p),
+        ("ref_anc", ctypes.c_void_p), ("ref_vec", ctypes.c_void_p),
+        ("ref_typ", ctypes.c_void_p), ("ref_prng", ctypes.c_void_p),
+        ("fit_anc", ctypes.c_void_p), ("fit_vec", ctypes.c_void_p),
+        ("fit_typ", ctypes.c_void_p), ("fit_prng", ctypes.c_void_p),
+        ("max_ref", ctypes.c_int32), ("max_fit", ctypes.c_int32),
+        ("max_ref_ph", ctypes.c_int32), ("max_fit_ph", ctypes.c_int32),
+    ]
+
+
+class FssScoreParams(ctypes.Structure):
+    _fields_ = [
+        ("mode", ctypes.c_int32),
+        ("alpha", ctypes.c_float),
+        ("lam", ctypes.c_float),
+        ("color_weight", ctypes.c_float),
+        ("similarity", ctypes.c_int32),
+        ("sigma", ctypes.c_float),
+        ("ph_alpha", ctypes.c_float * N_PTYPES),
+        ("ph_k", ctypes.c_float * N_PTYPES),
+        ("ph_cat", ctypes.c_int32 * N_PTYPES),
+    ]
+
+
+class FssAlignParams(ctypes.Structure):
+    _fields_ = [
+        ("n_restarts", ctypes.c_int32),
+        ("max_steps", ctypes.c_int32),
+        ("lr", ctypes.c_float), ("beta1", ctypes.c_float),
+        ("beta2", ctypes.c_float), ("eps", ctypes.c_float),
+        ("early_stop", ctypes.c_int32),
+        ("es_tol", ctypes.c_float),
+        ("es_patience", ctypes.c_int32),
+        ("lanes_per_pose", ctypes.c_int32),
+        ("restarts_per_wg", ctypes.c_int32),
+        ("pair_order", ctypes.c_void_p),
+        ("skip_finish", ctypes.c_int32),
+    ]
+
+
+class FssAlignOut(ctypes.Structure):
+    _fields_ = [
+        ("q", ctypes.c_void_p), ("t", ctypes.c_void_p),
+        ("pose_score", ctypes.c_void_p), ("best_score", ctypes.c_void_p),
+        ("best_idx", ctypes.c_void_p), ("best_tf", ctypes.c_void_p),
+        ("n_steps", ctypes.c_void_p),
+    ]
+
+
+# name -> (restype, argtypes); every symbol include/fss.h declares.
+_P = ctypes.c_void_p
+SIGNATURES = {
+    "fss_version": (ctypes.c_char_p, []),
+    "fss_last_error": (ctypes.c_char_p, []),
+    "fss_device_count": (ctypes.c_int, [ctypes.POINTER(ctypes.c_int)]),
+    "fss_self_overlap": (ctypes.c_int, [_P, _P, _P, _P]),
+    "fss_overlap_grad": (ctypes.c_int, [_P, _P, ctypes.c_int32, _P, _P, _P, ctypes.c_int32,
+                                        _P, _P, _P, _P, _P]),
+    "fss_pose_loss_grad": (ctypes.c_int, [_P, _P, ctypes.c_int32, _P, ctypes.c_int32, _P,
+                                          _P, _P, _P, _P, _P]),
+    "fss_adam_step": (ctypes.c_int, [ctypes.c_int32, _P, _P, _P, _P, _P, ctypes.c_int32,
+                                     ctypes.c_float, ctypes.c_float, ctypes.c_float,
+                                     ctypes.c_float, _P, ctypes.c_int32, _P]),
+    "fss_early_stop_update": (ctypes.c_int, [ctypes.c_int32, ctypes.c_int32, _P, _P, _P, _P, _P,
+                                             ctypes.c_int32, ctypes.c_float, ctypes.c_int32, _P]),
+    "fss_pose_score": (ctypes.c_int, [_P, _P, ctypes.c_int32, _P, ctypes.c_int32, _P, _P, _P,
+                                      _P, _P]),
+    "fss_align": (ctypes.c_int, [_P, _P, _P, _P, _P, _P]),
+    "fss_align_finish": (ctypes.c_int, [_P, _P, ctypes.c_int32, _P, _P, _P]),
+    "fss_score_fwd": (ctypes.c_int, [_P, _P, _P, _P, _P]),
+    "fss_score_bwd": (ctypes.c_int, [_P, _P, _P, _P, _P, _P, _P]),
+    "fss_score_fwd_f64": (ctypes.c_int, [_P, _P, _P, _P, _P, _P, _P]),
+    "fss_score_bwd_f64": (ctypes.c_int, [_P, _P, _P, _P, _P, _P, _P, _P, _P]),
+}
+
+_lock = threading.Lock()
+_lib = None
+
+
+class FssError(RuntimeError):
+    """A nonzero return code from libfss_hip.so, with the library's error text."""
+
+    def __init__(self, fn: str, code: int, text: str):
+        super().__init__(f"{fn} failed with code {code}: {text}")
+        self.code = code
+
+
+def load(path: str = LIB_PATH) -> ctypes.CDLL:
+    """Load libfss_hip.so once (raises if it was never built)."""
+    global _lib
+    with _lock:
+        if _lib is not None:
+            return _lib
+        if not os.path.exists(path):
+            raise ImportError(
+                f"{path} is missing: build it first with "
+                "`python -c 'import __graft_entry__ as g; g.build()'` (hipcc --offload-arch=gfx950)")
+        lib = ctypes.CDLL(path, mode=ctypes.RTLD_GLOBAL)
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(lib, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = lib
+        return lib
+
+
+def call(name: str, *args) -> None:
+    """Call ``name`` and raise :class:`FssError` on a nonzero return code."""
+    lib = load()
+    rc = getattr(lib, name)(*args)
+    if rc != 0:
+        text = lib.fss_last_error().decode(errors="replace")
+        raise FssError(name, rc, text)
+
+
+def ptr(t) -> int:
+    """Device address of a tensor (0 for None)."""
+    return 0 if t is None else int(t.data_ptr())
+
+
+def byref(s) -> ctypes.c_void_p:
+    return ctypes.cast(ctypes.pointer(s), ctypes.c_void_p)

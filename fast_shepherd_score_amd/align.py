@@ -1,0 +1,385 @@
+"""Batched multi-restart SE(3) alignment on libfss_hip.so.
+
+Three execution paths over the same kernels and the same optimiser semantics:
+
+* :func:`align` -- the persistent path: one ``fss_align`` launch runs every optimiser step of
+  every (pair, restart-block) workgroup, then FP64 re-scoring and best-restart selection.
+* :func:`align_stepped` -- all pairs advance one step per launch pair (pose loss + gradient,
+  Adam, on-device early-stop bookkeeping); the step sequence can be captured once in a HIP
+  graph and replayed, so there is no host synchronisation inside the loop.
+* :func:`align_plain` -- one pair at a time, one launch per step, with a host read of the mean
+  loss every step (the control flow of the reference's per-pair optimisers on HIP kernels).
+
+Optimiser semantics (all paths) restate alignment/_torch_analytical.py:23-478:
+loss L_s = 1 - S_s per restart, gradient [dL/dq_raw, dL/dt] / B handed to torch-style Adam
+(bias corrected, eps outside the sqrt) on raw quaternions, per-pair early stop once
+|mean L_k - mean L_{k-1}| <= 1e-5 has held for more than 10 consecutive steps, then the
+similarity of every final pose and the argmax over restarts.
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass, field
+from typing import Optional, Tuple
+
+import numpy as np
+import torch
+
+from . import _lib
+from .constants import LAM_SCALING, SIGMA_MAP, pharm_tables
+from .packing import PairBatch
+
+MODES = {"shape": _lib.MODE_SHAPE, "esp": _lib.MODE_SHAPE_ESP, "pharm": _lib.MODE_PHARM,
+         "shape_pharm": _lib.MODE_SHAPE_PHARM}
+
+
+@dataclass
+class ScoreSpec:
+    """What is scored: objective mode and its constants."""
+    mode: int = _lib.MODE_SHAPE
+    alpha: float = 0.81
+    lam: float = 0.3 * LAM_SCALING          # already scaled
+    color_weight: float = 0.5
+    similarity: str = "tanimoto"
+    directionless: bool = False
+
+    def struct(self) -> _lib.FssScoreParams:
+        s = _lib.FssScoreParams()
+        s.mode = int(self.mode)
+        s.alpha = float(self.alpha)
+        s.lam = float(self.lam)
+        s.color_weight = float(self.color_weight)
+        sim = self.similarity.lower()
+        if sim == "tanimoto":
+            s.similarity, s.sigma = _lib.SIM_TANIMOTO, 0.5
+        elif sim in SIGMA_MAP:
+            s.similarity, s.sigma = _lib.SIM_TVERSKY, SIGMA_MAP[sim]
+        else:
+            raise ValueError("similarity must be one of (tanimoto, tversky, tversky_ref, tversky_fit)")
+        a, k, c = pharm_tables(self.directionless)
+        for i in range(_lib.N_PTYPES):
+            s.ph_alpha[i], s.ph_k[i], s.ph_cat[i] = a[i], k[i], c[i]
+        return s
+
+
+@dataclass
+class OptimSpec:
+    """How the pose is optimised."""
+    max_steps: int = 200
+    lr: float = 0.1
+    beta1: float = 0.9
+    beta2: float = 0.999
+    eps: float = 1e-8
+    early_stop: bool = True
+    es_tol: float = 1e-5
+    es_patience: int = 10
+
+
+@dataclass
+class AlignResult:
+    score: torch.Tensor         # (P,) float64 best similarity per pair
+    best_idx: torch.Tensor      # (P,) int32
+    transform: torch.Tensor     # (P, 4, 4) float32
+    q: torch.Tensor             # (P, B, 4) float32 final raw quaternions
+    t: torch.Tensor             # (P, B, 3) float32 final translations
+    pose_score: torch.Tensor    # (P, B) float64
+    n_steps: torch.Tensor       # (P,) int32
+    extra: dict = field(default_factory=dict)
+
+
+def _stream(stream=None) -> int:
+    s = stream if stream is not None else torch.cuda.current_stream()
+    return int(s.cuda_stream)
+
+
+def self_overlaps(pb: PairBatch, spec: ScoreSpec, stream=None) -> torch.Tensor:
+    """(P, 4) float64 [VAA_shape, VBB_shape, VAA_pharm, VBB_pharm] at the identity pose."""
+    out = torch.empty((pb.n_pairs, 4), dtype=torch.float64, device=pb.device)
+    bs, sp = pb.struct(), spec.struct()
+    _lib.call("fss_self_overlap", _lib.byref(bs), _lib.byref(sp), _lib.ptr(out), _stream(stream))
+    return out
+
+
+def _next_pow2(x: int) -> int:
+    return 1 << max(0, int(np.ceil(np.log2(max(1, x)))))
+
+
+def default_tiling(pb: PairBatch, mode: int, B: int, early_stop: bool) -> Tuple[int, int]:
+    """(lanes_per_pose, restarts_per_wg) heuristic: ~4 fit rows per lane, groups inside a wave."""
+    cnt = pb.fit_pcounts if mode == _lib.MODE_PHARM else pb.fit_counts
+    m = float(np.median(cnt)) if cnt is not None and len(cnt) else 1.0
+    lpp = int(min(64, max(1, _next_pow2(int(np.ceil(m / 4.0))))))
+    if early_stop:
+        while lpp > 1 and B * lpp > 1024:
+            lpp //= 2
+        rpw = B
+    else:
+        rpw = max(1, min(B, 512 // lpp))
+    return lpp, rpw
+
+
+def check_tiling(B: int, lpp: int, rpw: int, early_stop: bool) -> None:
+    if lpp < 1 or lpp > 64 or lpp & (lpp - 1):
+        raise ValueError("lanes_per_pose must be a power of two in [1, 64]")
+    if early_stop and rpw < B:
+        raise ValueError("per-pair early stop needs every restart of a pair in one workgroup")
+    if rpw * lpp > 1024:
+        raise ValueError("restarts_per_wg * lanes_per_pose exceeds 1024 threads")
+
+
+def _prep_seeds(pb: PairBatch, q0: torch.Tensor, t0: torch.Tensor):
+    P = pb.n_pairs
+    if q0.dim() != 3 or q0.shape[0] != P or q0.shape[2] != 4:
+        raise ValueError(f"q0 must be (n_pairs={P}, B, 4), got {tuple(q0.shape)}")
+    B = int(q0.shape[1])
+    if tuple(t0.shape) != (P, B, 3):
+        raise ValueError(f"t0 must be ({P}, {B}, 3), got {tuple(t0.shape)}")
+    q = q0.to(device=pb.device, dtype=torch.float32).contiguous().clone()
+    t = t0.to(device=pb.device, dtype=torch.float32).contiguous().clone()
+    return q, t, B
+
+
+def align(pb: PairBatch, spec: ScoreSpec, q0: torch.Tensor, t0: torch.Tensor,
+          opt: OptimSpec = OptimSpec(), *, lanes_per_pose: Optional[int] = None,
+          restarts_per_wg: Optional[int] = None, order: str = "cost",
+          self_ov: Optional[torch.Tensor] = None, stream=None, events: Optional[list] = None) -> AlignResult:
+    """Persistent path: every step of every pair in one launch.
+
+    ``events``: if a list is given, HIP events bracketing the optimiser kernel and the finishing
+    kernels are appended to it as (start, optimiser_end, finish_end).
+    """
+    q, t, B = _prep_seeds(pb, q0, t0)
+    P = pb.n_pairs
+    dev = pb.device
+    if self_ov is None:
+        self_ov = self_overlaps(pb, spec, stream)
+    lpp0, rpw0 = default_tiling(pb, spec.mode, B, opt.early_stop)
+    lpp = lanes_per_pose or lpp0
+    rpw = restarts_per_wg or (B if opt.early_stop else rpw0)
+    check_tiling(B, lpp, rpw, opt.early_stop)
+    order_t = None
+    if order == "cost" and P > 1:
+        c = pb.cost(spec.mode)
+        if c.min() != c.max():
+            order_t = torch.from_numpy(np.argsort(-c, kind="stable").astype(np.int32)).to(dev)
+    pose_score = torch.empty((P, B), dtype=torch.float64, device=dev)
+    best = torch.empty(P, dtype=torch.float64, device=dev)
+    bidx = torch.empty(P, dtype=torch.int32, device=dev)
+    tf = torch.empty((P, 4, 4), dtype=torch.float32, device=dev)
+    nst = torch.empty(P, dtype=torch.int32, device=dev)
+    ap = _lib.FssAlignParams()
+    ap.n_restarts, ap.max_steps = B, int(opt.max_steps)
+    ap.lr, ap.beta1, ap.beta2, ap.eps = opt.lr, opt.beta1, opt.beta2, opt.eps
+    ap.early_stop, ap.es_tol, ap.es_patience = int(bool(opt.early_stop)), opt.es_tol, int(opt.es_patience)
+    ap.lanes_per_pose, ap.restarts_per_wg = int(lpp), int(rpw)
+    ap.pair_order = _lib.ptr(order_t)
+    out = _lib.FssAlignOut()
+    out.q, out.t = _lib.ptr(q), _lib.ptr(t)
+    out.pose_score, out.best_score, out.best_idx = _lib.ptr(pose_score), _lib.ptr(best), _lib.ptr(bidx)
+    out.best_tf, out.n_steps = _lib.ptr(tf), _lib.ptr(nst)
+    bs, sp = pb.struct(), spec.struct()
+    if events is None:
+        _lib.call("fss_align", _lib.byref(bs), _lib.byref(sp), _lib.byref(ap), _lib.ptr(self_ov),
+                  _lib.byref(out), _stream(stream))
+    else:
+        ts = stream if stream is not None else torch.cuda.current_stream()
+        ev = [torch.cuda.Event(enable_timing=True) for _ in range(3)]
+        ap.skip_finish = 1
+        ev[0].record(ts)
+        _lib.call("fss_align", _lib.byref(bs), _lib.byref(sp), _lib.byref(ap), _lib.ptr(self_ov),
+                  _lib.byref(out), _stream(ts))
+        ev[1].record(ts)
+        _lib.call("fss_align_finish", _lib.byref(bs), _lib.byref(sp), B, _lib.ptr(self_ov),
+                  _lib.byref(out), _stream(ts))
+        ev[2].record(ts)
+        events.append(tuple(ev))
+    return AlignResult(best, bidx, tf, q, t, pose_score, nst,
+                       extra={"lanes_per_pose": lpp, "restarts_per_wg": rpw, "order": order_t})
+
+
+class SteppedAligner:
+    """All pairs advance one optimiser step per (loss+grad, Adam, early-stop) launch triple.
+
+    The step sequence touches only buffers allocated here, so ``capture()`` records it once in
+    a HIP graph (torch.cuda.CUDAGraph) and ``run`` replays it ``max_steps`` times.
+    """
+
+    def __init__(self, pb: PairBatch, spec: ScoreSpec, q0: torch.Tensor, t0: torch.Tensor,
+                 opt: OptimSpec = OptimSpec(), self_ov: Optional[torch.Tensor] = None):
+        self.pb, self.spec, self.opt = pb, spec, opt
+        self.q, self.t, self.B = _prep_seeds(pb, q0, t0)
+        P, B, dev = pb.n_pairs, self.B, pb.device
+        self.P = P
+        self.self_ov = self_ov if self_ov is not None else self_overlaps(pb, spec)
+        self.pose_pair = torch.arange(P, device=dev, dtype=torch.int32).repeat_interleave(B)
+        self.loss = torch.zeros(P * B, dtype=torch.float32, device=dev)
+        self.grad = torch.zeros((P * B, 7), dtype=torch.float32, device=dev)
+        self.m = torch.zeros((P * B, 7), dtype=torch.float32, device=dev)
+        self.v = torch.zeros((P * B, 7), dtype=torch.float32, device=dev)
+        self.last = torch.ones(P, dtype=torch.float64, device=dev)
+        self.counter = torch.zeros(P, dtype=torch.int32, device=dev)
+        self.active = torch.ones(P, dtype=torch.int32, device=dev)
+        self.n_steps = torch.zeros(P, dtype=torch.int32, device=dev)
+        self._bs, self._sp = pb.struct(), spec.struct()
+        self.graph = None
+
+    def step(self, k: int, stream=None) -> None:
+        """Optimiser step k (1-based) for every still-active pair."""
+        s = _stream(stream)
+        P, B = self.P, self.B
+        _lib.call("fss_pose_loss_grad", _lib.byref(self._bs), _lib.byref(self._sp), P * B,
+                  _lib.ptr(self.pose_pair), B, _lib.ptr(self.self_ov), _lib.ptr(self.q),
+                  _lib.ptr(self.t), _lib.ptr(self.loss), _lib.ptr(self.grad), s)
+        act = self.active if self.opt.early_stop else None
+        _lib.call("fss_adam_step", P * B, _lib.ptr(self.q), _lib.ptr(self.t), _lib.ptr(self.grad),
+                  _lib.ptr(self.m), _lib.ptr(self.v), int(k), self.opt.lr, self.opt.beta1,
+                  self.opt.beta2, self.opt.eps, _lib.ptr(act), B, s)
+        if self.opt.early_stop:
+            _lib.call("fss_early_stop_update", P, B, _lib.ptr(self.loss), _lib.ptr(self.last),
+                      _lib.ptr(self.counter), _lib.ptr(self.active), _lib.ptr(self.n_steps),
+                      int(k), self.opt.es_tol, int(self.opt.es_patience), s)
+
+    def capture(self) -> None:
+        """Record all max_steps steps as one HIP graph (Adam's step count is baked per node)."""
+        stream = torch.cuda.Stream()
+        stream.wait_stream(torch.cuda.current_stream())
+        g = torch.cuda.CUDAGraph()
+        with torch.cuda.stream(stream):
+            with torch.cuda.graph(g, stream=stream):
+                for k in range(1, self.opt.max_steps + 1):
+                    self.step(k, stream)
+        torch.cuda.current_stream().wait_stream(stream)
+        self.graph = g
+
+    def run(self, use_graph: bool = False) -> AlignResult:
+        if use_graph:
+            if self.graph is None:
+                self.capture()
+            self.graph.replay()
+        else:
+            for k in range(1, self.opt.max_steps + 1):
+                self.step(k)
+        if not self.opt.early_stop:
+            self.n_steps.fill_(self.opt.max_steps)
+        return finish(self.pb, self.spec, self.q.view(self.P, self.B, 4),
+                      self.t.view(self.P, self.B, 3), self.self_ov, self.n_steps)
+
+
+def finish(pb: PairBatch, spec: ScoreSpec, q: torch.Tensor, t: torch.Tensor,
+           self_ov: torch.Tensor, n_steps: torch.Tensor, stream=None) -> AlignResult:
+    """FP64 similarity of every final pose, argmax over restarts, SE(3) matrix of the best."""
+    P, B = q.shape[0], q.shape[1]
+    ps = torch.empty((P, B), dtype=torch.float64, device=pb.device)
+    bs, sp = pb.struct(), spec.struct()
+    _lib.call("fss_pose_score", _lib.byref(bs), _lib.byref(sp), P * B, 0, B, _lib.ptr(self_ov),
+              _lib.ptr(q), _lib.ptr(t), _lib.ptr(ps), _stream(stream))
+    bidx = torch.argmax(ps, dim=1)
+    best = ps.gather(1, bidx[:, None])[:, 0]
+    qb = q[torch.arange(P, device=q.device), bidx].double()
+    tb = t[torch.arange(P, device=q.device), bidx]
+    tf = se3_matrix(qb, tb.double()).float()
+    return AlignResult(best, bidx.int(), tf, q, t, ps, n_steps)
+
+
+def align_stepped(pb, spec, q0, t0, opt: OptimSpec = OptimSpec(), use_graph: bool = True,
+                  self_ov=None) -> AlignResult:
+    return SteppedAligner(pb, spec, q0, t0, opt, self_ov).run(use_graph=use_graph)
+
+
+def align_plain(pb: PairBatch, spec: ScoreSpec, q0: torch.Tensor, t0: torch.Tensor,
+                opt: OptimSpec = OptimSpec(), self_ov=None) -> AlignResult:
+    """One pair per launch sequence, one launch pair per step, host early-stop test per step."""
+    q, t, B = _prep_seeds(pb, q0, t0)
+    P, dev = pb.n_pairs, pb.device
+    if self_ov is None:
+        self_ov = self_overlaps(pb, spec)
+    bs, sp = pb.struct(), spec.struct()
+    s = _stream()
+    loss = torch.zeros(B, dtype=torch.float32, device=dev)
+    grad = torch.zeros((B, 7), dtype=torch.float32, device=dev)
+    m = torch.zeros((B, 7), dtype=torch.float32, device=dev)
+    v = torch.zeros((B, 7), dtype=torch.float32, device=dev)
+    n_steps = np.zeros(P, dtype=np.int32)
+    for p in range(P):
+        pose_pair = torch.full((B,), p, dtype=torch.int32, device=dev)
+        qp, tp = q[p], t[p]
+        m.zero_()
+        v.zero_()
+        last, counter, done = 1.0, 0, opt.max_steps
+        for k in range(1, opt.max_steps + 1):
+            _lib.call("fss_pose_loss_grad", _lib.byref(bs), _lib.byref(sp), B, _lib.ptr(pose_pair),
+                      B, _lib.ptr(self_ov), _lib.ptr(qp), _lib.ptr(tp), _lib.ptr(loss),
+                      _lib.ptr(grad), s)
+            _lib.call("fss_adam_step", B, _lib.ptr(qp), _lib.ptr(tp), _lib.ptr(grad), _lib.ptr(m),
+                      _lib.ptr(v), k, opt.lr, opt.beta1, opt.beta2, opt.eps, 0, B, s)
+            if opt.early_stop:
+                L = float(loss.double().mean().item())
+                counter = 0 if abs(L - last) > opt.es_tol else counter + 1
+                last = L
+                if counter > opt.es_patience:
+                    done = k
+                    break
+        n_steps[p] = done
+    return finish(pb, spec, q, t, self_ov, torch.from_numpy(n_steps).to(dev))
+
+
+def se3_matrix(q: torch.Tensor, t: torch.Tensor) -> torch.Tensor:
+    """(K,4) quaternion (normalised here) + (K,3) translation -> (K,4,4) [[R, t],[0, 1]]."""
+    qn = q / q.norm(dim=1, keepdim=True)
+    w, x, y, z = qn.unbind(1)
+    R = torch.stack([1 - 2 * (y * y + z * z), 2 * (x * y - z * w), 2 * (x * z + y * w),
+                     2 * (x * y + z * w), 1 - 2 * (x * x + z * z), 2 * (y * z - x * w),
+                     2 * (x * z - y * w), 2 * (y * z + x * w), 1 - 2 * (x * x + y * y)],
+                    dim=1).view(-1, 3, 3)
+    T = torch.zeros((q.shape[0], 4, 4), dtype=q.dtype, device=q.device)
+    T[:, :3, :3] = R
+    T[:, :3, 3] = t
+    T[:, 3, 3] = 1
+    return T
+
+
+def overlap_grad(pb: PairBatch, spec: ScoreSpec, pose_pair: torch.Tensor, q: torch.Tensor,
+                 t: torch.Tensor, need_grad: bool = True, stream=None):
+    """Per-pose channel overlaps (and gradients): V (n,2), gR (n,2,3,3), gT (n,2,3), gQ (n,2,4)."""
+    n = int(q.shape[0])
+    dev = pb.device
+    V = torch.zeros((n, 2), dtype=torch.float32, device=dev)
+    gR = torch.zeros((n, 2, 3, 3), dtype=torch.float32, device=dev) if need_grad else None
+    gT = torch.zeros((n, 2, 3), dtype=torch.float32, device=dev) if need_grad else None
+    gQ = torch.zeros((n, 2, 4), dtype=torch.float32, device=dev) if need_grad else None
+    pp = pose_pair.to(device=dev, dtype=torch.int32).contiguous()
+    qq = q.to(device=dev, dtype=torch.float32).contiguous()
+    tt = t.to(device=dev, dtype=torch.float32).contiguous()
+    bs, sp = pb.struct(), spec.struct()
+    _lib.call("fss_overlap_grad", _lib.byref(bs), _lib.byref(sp), n, _lib.ptr(pp), _lib.ptr(qq),
+              _lib.ptr(tt), int(need_grad), _lib.ptr(V), _lib.ptr(gR), _lib.ptr(gT), _lib.ptr(gQ),
+              _stream(stream))
+    return V, gR, gT, gQ
+
+
+def pose_loss_grad(pb: PairBatch, spec: ScoreSpec, pose_pair, B: int, q, t, self_ov, stream=None):
+    """Per-pose loss (n,) and SE(3) gradient (n, 7) of one optimiser step."""
+    n = int(q.shape[0])
+    dev = pb.device
+    loss = torch.zeros(n, dtype=torch.float32, device=dev)
+    grad = torch.zeros((n, 7), dtype=torch.float32, device=dev)
+    pp = pose_pair.to(device=dev, dtype=torch.int32).contiguous()
+    qq = q.to(device=dev, dtype=torch.float32).contiguous()
+    tt = t.to(device=dev, dtype=torch.float32).contiguous()
+    bs, sp = pb.struct(), spec.struct()
+    _lib.call("fss_pose_loss_grad", _lib.byref(bs), _lib.byref(sp), n, _lib.ptr(pp), int(B),
+              _lib.ptr(self_ov), _lib.ptr(qq), _lib.ptr(tt), _lib.ptr(loss), _lib.ptr(grad),
+              _stream(stream))
+    return loss, grad
+
+
+def pose_score(pb: PairBatch, spec: ScoreSpec, pose_pair, q, t, self_ov, stream=None):
+    """FP64 similarity of arbitrary poses (n,)."""
+    n = int(q.shape[0])
+    out = torch.empty(n, dtype=torch.float64, device=pb.device)
+    pp = pose_pair.to(device=pb.device, dtype=torch.int32).contiguous()
+    qq = q.to(device=pb.device, dtype=torch.float32).contiguous()
+    tt = t.to(device=pb.device, dtype=torch.float32).contiguous()
+    bs, sp = pb.struct(), spec.struct()
+    _lib.call("fss_pose_score", _lib.byref(bs), _lib.byref(sp), n, _lib.ptr(pp), 1,
+              _lib.ptr(self_ov), _lib.ptr(qq), _lib.ptr(tt), _lib.ptr(out), _stream(stream))
+    return out

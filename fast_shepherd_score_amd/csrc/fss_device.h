@@ -1,0 +1,308 @@
+// Device-side building blocks shared by every gfx950 kernel of libfss_hip.so.
+//
+// Math contract (restated from score/analytical_gradients/_torch.py):
+//   overlap  O = K * sum_a sum_b w_ab * exp(-alpha/2 |a - b'|^2),  b' = R b + t,  K = (pi/(2 alpha))^1.5
+//   dO/dt    = K alpha sum w E (a - b')                         (:495-501)
+//   dO/dR    = K alpha sum w E (a - b') b^T                     (:503-512)
+//   ESP      w_ab = exp(-(c_a - c_b)^2 / lam)                   (:544-566)
+//   pharm    same-type pairs only, alpha/K/category from the type; directional weight
+//            w = (clamp(D,0,1)+2)/3 (cat 1) or (|D|+2)/3 (cat 2), D = (R v_fit).v_ref, with the
+//            weight gradient (1/3) K E c v_ref v_fit^T          (:223-361)
+//   rotation R(q_hat) in (w,x,y,z) order and its four Jacobians (:28-139)
+//
+// Shape channel arithmetic runs on pre-scaled coordinates: with s = sqrt(alpha/2 * log2(e)) every
+// coordinate (and the translation) is multiplied by s while it is staged in LDS, so a pair term
+// costs 3 sub + 3 mul/fma + 1 v_exp_f32 + 4 accumulations and no extra scale; the ESP value is
+// pre-scaled by sqrt(log2(e)/lam) so the ESP weight folds into the same exp2.  The sums are
+// rescaled once per pose: dO/dt = K alpha / s * sum(f~), dO/dR = K alpha / s^2 * G~.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include "../../include/fss.h"
+
+namespace fss {
+
+constexpr float kLog2e = 1.4426950408889634f;
+constexpr double kLog2eD = 1.4426950408889634;
+
+__device__ __forceinline__ float fast_exp2(float x) { return __builtin_amdgcn_exp2f(x); }
+
+// Rotation from a raw quaternion in FP64: qn = |q|, qh = q / qn, R(qh) row-major.
+__device__ __forceinline__ void rotation_fp64(const float q[4], double qh[4], double& qn,
+                                              double Rd[9]) {
+  const double w = q[0], x = q[1], y = q[2], z = q[3];
+  qn = sqrt(w * w + x * x + y * y + z * z);
+  const double inv = 1.0 / qn;
+  qh[0] = w * inv; qh[1] = x * inv; qh[2] = y * inv; qh[3] = z * inv;
+  const double a = qh[0], b = qh[1], c = qh[2], d = qh[3];
+  Rd[0] = 1.0 - 2.0 * (c * c + d * d); Rd[1] = 2.0 * (b * c - d * a);       Rd[2] = 2.0 * (b * d + c * a);
+  Rd[3] = 2.0 * (b * c + d * a);       Rd[4] = 1.0 - 2.0 * (b * b + d * d); Rd[5] = 2.0 * (c * d - b * a);
+  Rd[6] = 2.0 * (b * d - c * a);       Rd[7] = 2.0 * (c * d + b * a);       Rd[8] = 1.0 - 2.0 * (b * b + c * c);
+}
+
+// g_k = sum_ij G_ij (dR/dq_k)_ij at a unit quaternion qh (ag/_torch.py:62-173).
+template <typename T>
+__device__ __forceinline__ void project_to_quat(const T G[9], const double qh[4], double g[4]) {
+  const double w = qh[0], x = qh[1], y = qh[2], z = qh[3];
+  const double G0 = G[0], G1 = G[1], G2 = G[2], G3 = G[3], G4 = G[4], G5 = G[5], G6 = G[6],
+               G7 = G[7], G8 = G[8];
+  // dR/dw = [[0,-2z,2y],[2z,0,-2x],[-2y,2x,0]]
+  g[0] = -2 * z * G1 + 2 * y * G2 + 2 * z * G3 - 2 * x * G5 - 2 * y * G6 + 2 * x * G7;
+  // dR/dx = [[0,2y,2z],[2y,-4x,-2w],[2z,2w,-4x]]
+  g[1] = 2 * y * G1 + 2 * z * G2 + 2 * y * G3 - 4 * x * G4 - 2 * w * G5 + 2 * z * G6 + 2 * w * G7 - 4 * x * G8;
+  // dR/dy = [[-4y,2x,2w],[2x,0,2z],[-2w,2z,-4y]]
+  g[2] = -4 * y * G0 + 2 * x * G1 + 2 * w * G2 + 2 * x * G3 + 2 * z * G5 - 2 * w * G6 + 2 * z * G7 - 4 * y * G8;
+  // dR/dz = [[-4z,-2w,2x],[2w,-4z,2y],[2x,2y,0]]
+  g[3] = -4 * z * G0 - 2 * w * G1 + 2 * x * G2 + 2 * w * G3 - 4 * z * G4 + 2 * y * G5 + 2 * x * G6 + 2 * y * G7;
+}
+
+// Per-lane partial sums of one channel for one pose.
+struct Acc {
+  float V;
+  float gt[3];
+  float G[9];
+  __device__ __forceinline__ void zero() {
+    V = 0.f;
+#pragma unroll
+    for (int i = 0; i < 3; ++i) gt[i] = 0.f;
+#pragma unroll
+    for (int i = 0; i < 9; ++i) G[i] = 0.f;
+  }
+};
+
+// Butterfly sum over `width` consecutive lanes (width a power of two <= 64); every lane of the
+// group ends with the bit-identical total (fp add is commutative).
+__device__ __forceinline__ float group_sum(float v, int width) {
+  for (int m = 1; m < width; m <<= 1) v += __shfl_xor(v, m, 64);
+  return v;
+}
+__device__ __forceinline__ double group_sum(double v, int width) {
+  for (int m = 1; m < width; m <<= 1) v += __shfl_xor(v, m, 64);
+  return v;
+}
+__device__ __forceinline__ void group_sum(Acc& a, int width, bool grad) {
+  a.V = group_sum(a.V, width);
+  if (grad) {
+#pragma unroll
+    for (int i = 0; i < 3; ++i) a.gt[i] = group_sum(a.gt[i], width);
+#pragma unroll
+    for (int i = 0; i < 9; ++i) a.G[i] = group_sum(a.G[i], width);
+  }
+}
+
+// ---------------------------------------------------------------------------------------------
+// Shape channel, FP32, scaled coordinates in LDS.
+//   ref[a] = (s a, sc c_a), fit[b] = (s b, sc c_b);  R unscaled; ts = s t.
+// Lane j of a group of `lpp` lanes owns fit rows b = j, j+lpp, ...; two rows are processed per
+// pass over the ref cloud so one LDS broadcast read feeds two pair terms.
+// ---------------------------------------------------------------------------------------------
+template <bool ESP, bool GRAD>
+__device__ __forceinline__ void shape_accum(const float4* __restrict__ ref, int N,
+                                            const float4* __restrict__ fit, int M, int j, int lpp,
+                                            const float R[9], const float ts[3], Acc& acc) {
+  for (int b0 = j; b0 < M; b0 += 2 * lpp) {
+    const int b1r = b0 + lpp;
+    const bool has1 = b1r < M;
+    const int b1 = has1 ? b1r : b0;
+    const float4 p0 = fit[b0];
+    const float4 p1 = fit[b1];
+    const float x0 = fmaf(R[0], p0.x, fmaf(R[1], p0.y, fmaf(R[2], p0.z, ts[0])));
+    const float y0 = fmaf(R[3], p0.x, fmaf(R[4], p0.y, fmaf(R[5], p0.z, ts[1])));
+    const float z0 = fmaf(R[6], p0.x, fmaf(R[7], p0.y, fmaf(R[8], p0.z, ts[2])));
+    const float x1 = fmaf(R[0], p1.x, fmaf(R[1], p1.y, fmaf(R[2], p1.z, ts[0])));
+    const float y1 = fmaf(R[3], p1.x, fmaf(R[4], p1.y, fmaf(R[5], p1.z, ts[1])));
+    const float z1 = fmaf(R[6], p1.x, fmaf(R[7], p1.y, fmaf(R[8], p1.z, ts[2])));
+    float v0 = 0.f, v1 = 0.f;
+    float fx0 = 0.f, fy0 = 0.f, fz0 = 0.f, fx1 = 0.f, fy1 = 0.f, fz1 = 0.f;
+#pragma unroll 4
+    for (int a = 0; a < N; ++a) {
+      const float4 r = ref[a];
+      const float dx0 = r.x - x0, dy0 = r.y - y0, dz0 = r.z - z0;
+      const float dx1 = r.x - x1, dy1 = r.y - y1, dz1 = r.z - z1;
+      float s0 = fmaf(dx0, dx0, fmaf(dy0, dy0, dz0 * dz0));
+      float s1 = fmaf(dx1, dx1, fmaf(dy1, dy1, dz1 * dz1));
+      if (ESP) {
+        const float c0 = r.w - p0.w, c1 = r.w - p1.w;
+        s0 = fmaf(c0, c0, s0);
+        s1 = fmaf(c1, c1, s1);
+      }
+      const float e0 = fast_exp2(-s0);
+      const float e1 = fast_exp2(-s1);
+      v0 += e0;
+      v1 += e1;
+      if (GRAD) {
+        fx0 = fmaf(e0, dx0, fx0); fy0 = fmaf(e0, dy0, fy0); fz0 = fmaf(e0, dz0, fz0);
+        fx1 = fmaf(e1, dx1, fx1); fy1 = fmaf(e1, dy1, fy1); fz1 = fmaf(e1, dz1, fz1);
+      }
+    }
+    if (!has1) { v1 = 0.f; fx1 = fy1 = fz1 = 0.f; }
+    acc.V += v0 + v1;
+    if (GRAD) {
+      acc.gt[0] += fx0 + fx1; acc.gt[1] += fy0 + fy1; acc.gt[2] += fz0 + fz1;
+      acc.G[0] = fmaf(fx0, p0.x, fmaf(fx1, p1.x, acc.G[0]));
+      acc.G[1] = fmaf(fx0, p0.y, fmaf(fx1, p1.y, acc.G[1]));
+      acc.G[2] = fmaf(fx0, p0.z, fmaf(fx1, p1.z, acc.G[2]));
+      acc.G[3] = fmaf(fy0, p0.x, fmaf(fy1, p1.x, acc.G[3]));
+      acc.G[4] = fmaf(fy0, p0.y, fmaf(fy1, p1.y, acc.G[4]));
+      acc.G[5] = fmaf(fy0, p0.z, fmaf(fy1, p1.z, acc.G[5]));
+      acc.G[6] = fmaf(fz0, p0.x, fmaf(fz1, p1.x, acc.G[6]));
+      acc.G[7] = fmaf(fz0, p0.y, fmaf(fz1, p1.y, acc.G[7]));
+      acc.G[8] = fmaf(fz0, p0.z, fmaf(fz1, p1.z, acc.G[8]));
+    }
+  }
+}
+
+// Convert scaled shape sums into O, dO/dt, dO/dR (in place).
+__device__ __forceinline__ void shape_finish(Acc& a, float K, float alpha, float s, bool grad) {
+  a.V *= K;
+  if (grad) {
+    const float ct = K * alpha / s, cr = K * alpha / (s * s);
+#pragma unroll
+    for (int i = 0; i < 3; ++i) a.gt[i] *= ct;
+#pragma unroll
+    for (int i = 0; i < 9; ++i) a.G[i] *= cr;
+  }
+}
+
+// Pharmacophore tables in registers/LDS.
+struct PhTab {
+  float kap[FSS_N_PTYPES];   // alpha/2 * log2(e)
+  float ak[FSS_N_PTYPES];    // alpha * K
+  float k[FSS_N_PTYPES];     // K
+  int cat[FSS_N_PTYPES];
+};
+
+// ---------------------------------------------------------------------------------------------
+// Pharmacophore channel, FP32.  anc[i] = (x, y, z, type bits), vec[i] = (unit x, y, z, 0).
+// ---------------------------------------------------------------------------------------------
+template <bool GRAD>
+__device__ __forceinline__ void pharm_accum(const float4* __restrict__ ranc,
+                                            const float4* __restrict__ rvec, int N,
+                                            const float4* __restrict__ fanc,
+                                            const float4* __restrict__ fvec, int M, int j, int lpp,
+                                            const float R[9], const float t[3], const PhTab& tab,
+                                            Acc& acc) {
+  for (int b = j; b < M; b += lpp) {
+    const float4 p = fanc[b];
+    const float4 u = fvec[b];
+    const int tb = __float_as_int(p.w);
+    if (tb < 0 || tb >= FSS_N_PTYPES) continue;
+    const int cat = tab.cat[tb];
+    if (cat == 3) continue;
+    const float kap = tab.kap[tb], ak = tab.ak[tb], kk = tab.k[tb];
+    const float bx = fmaf(R[0], p.x, fmaf(R[1], p.y, fmaf(R[2], p.z, t[0])));
+    const float by = fmaf(R[3], p.x, fmaf(R[4], p.y, fmaf(R[5], p.z, t[1])));
+    const float bz = fmaf(R[6], p.x, fmaf(R[7], p.y, fmaf(R[8], p.z, t[2])));
+    const float ux = R[0] * u.x + R[1] * u.y + R[2] * u.z;
+    const float uy = R[3] * u.x + R[4] * u.y + R[5] * u.z;
+    const float uz = R[6] * u.x + R[7] * u.y + R[8] * u.z;
+    float fx = 0.f, fy = 0.f, fz = 0.f, hx = 0.f, hy = 0.f, hz = 0.f, v = 0.f;
+    for (int a = 0; a < N; ++a) {
+      const float4 r = ranc[a];
+      if (__float_as_int(r.w) != tb) continue;
+      const float4 rv = rvec[a];
+      const float dx = r.x - bx, dy = r.y - by, dz = r.z - bz;
+      const float e = fast_exp2(-kap * fmaf(dx, dx, fmaf(dy, dy, dz * dz)));
+      const float D = ux * rv.x + uy * rv.y + uz * rv.z;
+      float w = 1.f, c = 0.f;
+      if (cat == 1) {
+        w = (fminf(fmaxf(D, 0.f), 1.f) + 2.f) * (1.f / 3.f);
+        c = (D > 0.f && D < 1.f) ? 1.f : 0.f;
+      } else if (cat == 2) {
+        w = (fabsf(D) + 2.f) * (1.f / 3.f);
+        c = D > 0.f ? 1.f : (D < 0.f ? -1.f : 0.f);
+      }
+      const float we = w * e;
+      v = fmaf(kk, we, v);
+      if (GRAD) {
+        const float fw = ak * we;
+        fx = fmaf(fw, dx, fx); fy = fmaf(fw, dy, fy); fz = fmaf(fw, dz, fz);
+        const float hc = kk * (1.f / 3.f) * e * c;
+        hx = fmaf(hc, rv.x, hx); hy = fmaf(hc, rv.y, hy); hz = fmaf(hc, rv.z, hz);
+      }
+    }
+    acc.V += v;
+    if (GRAD) {
+      acc.gt[0] += fx; acc.gt[1] += fy; acc.gt[2] += fz;
+      acc.G[0] += fx * p.x + hx * u.x; acc.G[1] += fx * p.y + hx * u.y; acc.G[2] += fx * p.z + hx * u.z;
+      acc.G[3] += fy * p.x + hy * u.x; acc.G[4] += fy * p.y + hy * u.y; acc.G[5] += fy * p.z + hy * u.z;
+      acc.G[6] += fz * p.x + hz * u.x; acc.G[7] += fz * p.y + hz * u.y; acc.G[8] += fz * p.z + hz * u.z;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------------------------
+// FP64 value-only evaluation straight from the packed (unscaled) device arrays; used for the
+// final re-scoring of every optimised pose.
+// ---------------------------------------------------------------------------------------------
+template <bool ESP>
+__device__ __forceinline__ double shape_value_f64(const float4* __restrict__ ref, int N,
+                                                  const float4* __restrict__ fit, int M, int j,
+                                                  int lpp, const double R[9], const double t[3],
+                                                  double half_alpha, double inv_lam) {
+  double v = 0.0;
+  for (int b = j; b < M; b += lpp) {
+    const float4 p = fit[b];
+    const double bx = R[0] * p.x + R[1] * p.y + R[2] * p.z + t[0];
+    const double by = R[3] * p.x + R[4] * p.y + R[5] * p.z + t[1];
+    const double bz = R[6] * p.x + R[7] * p.y + R[8] * p.z + t[2];
+    for (int a = 0; a < N; ++a) {
+      const float4 r = ref[a];
+      const double dx = r.x - bx, dy = r.y - by, dz = r.z - bz;
+      double arg = -half_alpha * (dx * dx + dy * dy + dz * dz);
+      if (ESP) {
+        const double dc = (double)r.w - (double)p.w;
+        arg -= dc * dc * inv_lam;
+      }
+      v += exp(arg);
+    }
+  }
+  return v;
+}
+
+__device__ __forceinline__ double pharm_value_f64(const float4* __restrict__ ranc,
+                                                  const float4* __restrict__ rvec,
+                                                  const int32_t* __restrict__ rtyp, int N,
+                                                  const float4* __restrict__ fanc,
+                                                  const float4* __restrict__ fvec,
+                                                  const int32_t* __restrict__ ftyp, int M, int j,
+                                                  int lpp, const double R[9], const double t[3],
+                                                  const float* alpha, const float* kk,
+                                                  const int32_t* cats) {
+  double v = 0.0;
+  for (int b = j; b < M; b += lpp) {
+    const int tb = ftyp[b];
+    if (tb < 0 || tb >= FSS_N_PTYPES) continue;
+    const int cat = cats[tb];
+    if (cat == 3) continue;
+    const float4 p = fanc[b];
+    const float4 u0 = fvec[b];
+    double un = sqrt((double)u0.x * u0.x + (double)u0.y * u0.y + (double)u0.z * u0.z);
+    un = un > 1e-12 ? un : 1e-12;
+    const double ux0 = u0.x / un, uy0 = u0.y / un, uz0 = u0.z / un;
+    const double bx = R[0] * p.x + R[1] * p.y + R[2] * p.z + t[0];
+    const double by = R[3] * p.x + R[4] * p.y + R[5] * p.z + t[1];
+    const double bz = R[6] * p.x + R[7] * p.y + R[8] * p.z + t[2];
+    const double ux = R[0] * ux0 + R[1] * uy0 + R[2] * uz0;
+    const double uy = R[3] * ux0 + R[4] * uy0 + R[5] * uz0;
+    const double uz = R[6] * ux0 + R[7] * uy0 + R[8] * uz0;
+    const double ha = 0.5 * (double)alpha[tb], K = kk[tb];
+    for (int a = 0; a < N; ++a) {
+      if (rtyp[a] != tb) continue;
+      const float4 r = ranc[a];
+      const float4 rv0 = rvec[a];
+      double rn = sqrt((double)rv0.x * rv0.x + (double)rv0.y * rv0.y + (double)rv0.z * rv0.z);
+      rn = rn > 1e-12 ? rn : 1e-12;
+      const double dx = r.x - bx, dy = r.y - by, dz = r.z - bz;
+      const double e = exp(-ha * (dx * dx + dy * dy + dz * dz));
+      const double D = (ux * rv0.x + uy * rv0.y + uz * rv0.z) / rn;
+      double w = 1.0;
+      if (cat == 1) w = (fmin(fmax(D, 0.0), 1.0) + 2.0) / 3.0;
+      else if (cat == 2) w = (fabs(D) + 2.0) / 3.0;
+      v += K * w * e;
+    }
+  }
+  return v;
+}
+
+}  // namespace fss

@@ -1,0 +1,1028 @@
+// libfss_hip.so: gfx950 kernels and the C ABI declared in include/fss.h.
+//
+// Kernels
+//   k_self_overlap   one workgroup per (pair, side): VAA / VBB in FP64 at the identity pose
+//   k_pose_grad      one workgroup per pose: O_AB + dO/dR, dO/dt, dO/dq_hat (per channel), or the
+//                    full per-pose loss and SE(3) gradient of one optimiser step
+//   k_adam           torch.optim.Adam step over (n, 7) parameter rows
+//   k_align          persistent optimiser: one workgroup per (pair, restart-block); the pair's
+//                    clouds are staged once in LDS, every lane group of `lpp` lanes owns one
+//                    restart for all steps (parameters and Adam moments live in registers), the
+//                    N x M sum of a pose is split over the group and reduced with butterflies,
+//                    and per-pair early stop is one LDS exchange per step
+//   k_align_finish   best restart per pair + SE(3) matrix
+//   k_score_fwd/bwd  differentiable similarity of fixed clouds (FP32 and FP64 instances)
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <stdarg.h>
+#include <stdio.h>
+#include <string.h>
+#include <string>
+
+#include "../../include/fss.h"
+#include "fss_device.h"
+
+using namespace fss;
+
+namespace {
+
+thread_local std::string g_last_error;
+
+int set_error(int code, const char* fmt, ...) __attribute__((format(printf, 2, 3)));
+int set_error(int code, const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof(buf), fmt, ap);
+  va_end(ap);
+  g_last_error = buf;
+  return code;
+}
+
+int check_launch(const char* what) {
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return set_error((int)e, "%s: launch failed: %s", what, hipGetErrorString(e));
+  return FSS_OK;
+}
+
+constexpr int kPoseBlock = 256;  // threads of the one-workgroup-per-pose kernels
+
+bool uses_shape(int mode) { return mode == FSS_MODE_SHAPE || mode == FSS_MODE_SHAPE_ESP || mode == FSS_MODE_SHAPE_PHARM; }
+bool uses_pharm(int mode) { return mode == FSS_MODE_PHARM || mode == FSS_MODE_SHAPE_PHARM; }
+
+// Device-side scoring constants derived once per call on the host.
+struct KScore {
+  int mode;
+  float alpha, K, s, sc;      // shape: K = (pi/(2 alpha))^1.5, s = sqrt(alpha/2 log2 e), sc = sqrt(log2 e / lam)
+  double alpha_d, lam_d;
+  float w;
+  int sim;
+  float sigma;
+  PhTab tab;
+  float ph_alpha[FSS_N_PTYPES];
+  float ph_k[FSS_N_PTYPES];
+  int32_t ph_cat[FSS_N_PTYPES];
+};
+
+int make_kscore(const FssScoreParams* sp, KScore& ks) {
+  if (!sp) return set_error(FSS_E_INVALID, "score params are NULL");
+  if (sp->mode < 0 || sp->mode > 3) return set_error(FSS_E_INVALID, "unknown mode %d", sp->mode);
+  if (uses_shape(sp->mode) && !(sp->alpha > 0.f)) return set_error(FSS_E_INVALID, "alpha must be > 0");
+  if (sp->mode == FSS_MODE_SHAPE_ESP && !(sp->lam > 0.f)) return set_error(FSS_E_INVALID, "lam must be > 0");
+  if (sp->similarity != FSS_SIM_TANIMOTO && sp->similarity != FSS_SIM_TVERSKY)
+    return set_error(FSS_E_INVALID, "unknown similarity %d", sp->similarity);
+  memset(&ks, 0, sizeof(ks));
+  ks.mode = sp->mode;
+  ks.alpha = sp->alpha;
+  ks.alpha_d = sp->alpha;
+  ks.K = uses_shape(sp->mode) ? (float)pow(M_PI / (2.0 * (double)sp->alpha), 1.5) : 0.f;
+  ks.s = uses_shape(sp->mode) ? (float)sqrt(0.5 * (double)sp->alpha * kLog2eD) : 1.f;
+  ks.lam_d = sp->lam > 0.f ? (double)sp->lam : 1.0;
+  ks.sc = sp->mode == FSS_MODE_SHAPE_ESP ? (float)sqrt(kLog2eD / (double)sp->lam) : 0.f;
+  ks.w = sp->color_weight;
+  ks.sim = sp->similarity;
+  ks.sigma = sp->sigma;
+  for (int i = 0; i < FSS_N_PTYPES; ++i) {
+    ks.ph_alpha[i] = sp->ph_alpha[i];
+    ks.ph_k[i] = sp->ph_k[i];
+    ks.ph_cat[i] = sp->ph_cat[i];
+    ks.tab.kap[i] = (float)(0.5 * (double)sp->ph_alpha[i] * kLog2eD);
+    ks.tab.ak[i] = (float)((double)sp->ph_alpha[i] * (double)sp->ph_k[i]);
+    ks.tab.k[i] = sp->ph_k[i];
+    ks.tab.cat[i] = sp->ph_cat[i];
+  }
+  return FSS_OK;
+}
+
+int check_batch(const FssPairBatch* pb, int mode) {
+  if (!pb) return set_error(FSS_E_INVALID, "pair batch is NULL");
+  if (pb->n_pairs < 0) return set_error(FSS_E_INVALID, "n_pairs < 0");
+  if (uses_shape(mode) && (!pb->ref_pts || !pb->fit_pts || !pb->ref_rng || !pb->fit_rng))
+    return set_error(FSS_E_INVALID, "shape channel arrays are NULL");
+  if (uses_pharm(mode) && (!pb->ref_anc || !pb->ref_vec || !pb->ref_typ || !pb->ref_prng ||
+                           !pb->fit_anc || !pb->fit_vec || !pb->fit_typ || !pb->fit_prng))
+    return set_error(FSS_E_INVALID, "pharmacophore channel arrays are NULL");
+  if (pb->max_ref < 0 || pb->max_fit < 0 || pb->max_ref_ph < 0 || pb->max_fit_ph < 0)
+    return set_error(FSS_E_INVALID, "negative cloud maxima");
+  return FSS_OK;
+}
+
+// LDS plan for one workgroup: shape ref/fit rows then pharm ref anc/vec, fit anc/vec.
+struct LdsPlan {
+  int ref, fit, ranc, rvec, fanc, fvec, end;  // float4 offsets
+};
+LdsPlan plan_lds(const FssPairBatch* pb, int mode) {
+  LdsPlan p;
+  int o = 0;
+  const bool sh = uses_shape(mode), ph = uses_pharm(mode);
+  p.ref = o; o += sh ? pb->max_ref : 0;
+  p.fit = o; o += sh ? pb->max_fit : 0;
+  p.ranc = o; o += ph ? pb->max_ref_ph : 0;
+  p.rvec = o; o += ph ? pb->max_ref_ph : 0;
+  p.fanc = o; o += ph ? pb->max_fit_ph : 0;
+  p.fvec = o; o += ph ? pb->max_fit_ph : 0;
+  p.end = o;
+  return p;
+}
+
+size_t max_lds_bytes() {
+  int dev = 0, v = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return 65536;
+  if (hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess) return 65536;
+  return (size_t)v;
+}
+
+struct Staged {
+  const float4* ref; const float4* fit; int N, M;
+  const float4* ranc; const float4* rvec; const float4* fanc; const float4* fvec; int Np, Mp;
+};
+
+// Cooperative copy of one pair's clouds into LDS (shape rows scaled, pharm vectors normalised,
+// pharm type folded into the anchor's w as integer bits).  Caller syncs afterwards.
+template <bool SH, bool ESP, bool PH>
+__device__ __forceinline__ Staged stage_pair(const FssPairBatch& pb, const KScore& ks, LdsPlan lp,
+                                             int pair, int ref_side_a, int fit_side_b,
+                                             float4* lds) {
+  // ref_side_a / fit_side_b select which cloud feeds the "ref" and "fit" slots:
+  // 0 = the pair's ref cloud, 1 = the pair's fit cloud (self-overlaps use (0,0) and (1,1)).
+  Staged st{};
+  const int tid = threadIdx.x, nt = blockDim.x;
+  if (SH) {
+    const int* ra = ref_side_a ? pb.fit_rng : pb.ref_rng;
+    const float* pa = ref_side_a ? pb.fit_pts : pb.ref_pts;
+    const int* rb = fit_side_b ? pb.fit_rng : pb.ref_rng;
+    const float* pbp = fit_side_b ? pb.fit_pts : pb.ref_pts;
+    const int a0 = ra[2 * pair], N = ra[2 * pair + 1];
+    const int b0 = rb[2 * pair], M = rb[2 * pair + 1];
+    const float4* ga = reinterpret_cast<const float4*>(pa) + a0;
+    const float4* gb = reinterpret_cast<const float4*>(pbp) + b0;
+    float4* la = lds + lp.ref;
+    float4* lb = lds + lp.fit;
+    const float s = ks.s, sc = ks.sc;
+    for (int i = tid; i < N; i += nt) {
+      const float4 v = ga[i];
+      la[i] = make_float4(v.x * s, v.y * s, v.z * s, ESP ? v.w * sc : 0.f);
+    }
+    for (int i = tid; i < M; i += nt) {
+      const float4 v = gb[i];
+      lb[i] = make_float4(v.x * s, v.y * s, v.z * s, ESP ? v.w * sc : 0.f);
+    }
+    st.ref = la; st.fit = lb; st.N = N; st.M = M;
+  }
+  if (PH) {
+    const int* ra = ref_side_a ? pb.fit_prng : pb.ref_prng;
+    const int* rb = fit_side_b ? pb.fit_prng : pb.ref_prng;
+    const float4* aa = reinterpret_cast<const float4*>(ref_side_a ? pb.fit_anc : pb.ref_anc);
+    const float4* av = reinterpret_cast<const float4*>(ref_side_a ? pb.fit_vec : pb.ref_vec);
+    const int32_t* at = ref_side_a ? pb.fit_typ : pb.ref_typ;
+    const float4* ba = reinterpret_cast<const float4*>(fit_side_b ? pb.fit_anc : pb.ref_anc);
+    const float4* bv = reinterpret_cast<const float4*>(fit_side_b ? pb.fit_vec : pb.ref_vec);
+    const int32_t* bt = fit_side_b ? pb.fit_typ : pb.ref_typ;
+    const int a0 = ra[2 * pair], N = ra[2 * pair + 1];
+    const int b0 = rb[2 * pair], M = rb[2 * pair + 1];
+    for (int i = tid; i < N + M; i += nt) {
+      const bool isref = i < N;
+      const int k = isref ? a0 + i : b0 + (i - N);
+      const float4 p = isref ? aa[k] : ba[k];
+      const float4 v = isref ? av[k] : bv[k];
+      const int ty = isref ? at[k] : bt[k];
+      float n = sqrtf(v.x * v.x + v.y * v.y + v.z * v.z);
+      n = n > 1e-12f ? n : 1e-12f;
+      const float4 pa4 = make_float4(p.x, p.y, p.z, __int_as_float(ty));
+      const float4 va4 = make_float4(v.x / n, v.y / n, v.z / n, 0.f);
+      if (isref) { lds[lp.ranc + i] = pa4; lds[lp.rvec + i] = va4; }
+      else { lds[lp.fanc + (i - N)] = pa4; lds[lp.fvec + (i - N)] = va4; }
+    }
+    st.ranc = lds + lp.ranc; st.rvec = lds + lp.rvec; st.fanc = lds + lp.fanc; st.fvec = lds + lp.fvec;
+    st.Np = N; st.Mp = M;
+  }
+  return st;
+}
+
+// Similarity of one pose from its channel overlaps and, with grad, dS/dR and dS/dt.
+template <typename T>
+__device__ __forceinline__ T similarity(const KScore& ks, T Os, T Op, const double* so, bool has_sh,
+                                        bool has_ph, T& ws, T& wp) {
+  // ws, wp receive dS/dO of the shape and pharm channel.
+  T S = 0;
+  ws = 0; wp = 0;
+  const T chs = ks.mode == FSS_MODE_SHAPE_PHARM ? (T)(1.0f - ks.w) : (T)1;
+  const T chp = ks.mode == FSS_MODE_SHAPE_PHARM ? (T)ks.w : (T)1;
+  if (has_sh) {
+    const T U = (T)so[0] + (T)so[1];
+    const T D = U - Os;
+    S += chs * (Os / D);
+    ws = chs * U / (D * D);
+  }
+  if (has_ph) {
+    if (ks.sim == FSS_SIM_TANIMOTO) {
+      const T U = (T)so[2] + (T)so[3];
+      const T D = U - Op;
+      S += chp * (Op / D);
+      wp = chp * U / (D * D);
+    } else {
+      const T D = (T)ks.sigma * (T)so[2] + (T)(1.0f - ks.sigma) * (T)so[3];
+      const T r = Op / D;
+      S += chp * (r < (T)1 ? r : (T)1);
+      wp = Op < D ? chp / D : (T)0;
+    }
+  }
+  return S;
+}
+
+// Per-pose loss gradient tail: dL/dR, dL/dt -> [dL/dq_raw, dL/dt] / B (projection in FP64).
+__device__ __forceinline__ void se3_grad(const float gR[9], const float gT[3], const double qh[4],
+                                         double qn, float invB, float g[7]) {
+  double gq[4];
+  project_to_quat(gR, qh, gq);
+  const double dot = qh[0] * gq[0] + qh[1] * gq[1] + qh[2] * gq[2] + qh[3] * gq[3];
+#pragma unroll
+  for (int k = 0; k < 4; ++k) g[k] = (float)((gq[k] - qh[k] * dot) / qn) * invB;
+#pragma unroll
+  for (int k = 0; k < 3; ++k) g[4 + k] = gT[k] * invB;
+}
+
+// Sum an Acc over the workgroup (one value set per wave through LDS); result in thread 0.
+__device__ __forceinline__ void block_sum(Acc& a, float* red, bool grad) {
+  group_sum(a, 64, grad);
+  const int w = threadIdx.x >> 6, l = threadIdx.x & 63, nw = blockDim.x >> 6;
+  __syncthreads();
+  if (l == 0) {
+    red[w * 13] = a.V;
+    for (int i = 0; i < 3; ++i) red[w * 13 + 1 + i] = a.gt[i];
+    for (int i = 0; i < 9; ++i) red[w * 13 + 4 + i] = a.G[i];
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    Acc s; s.zero();
+    for (int k = 0; k < nw; ++k) {
+      s.V += red[k * 13];
+      for (int i = 0; i < 3; ++i) s.gt[i] += red[k * 13 + 1 + i];
+      for (int i = 0; i < 9; ++i) s.G[i] += red[k * 13 + 4 + i];
+    }
+    a = s;
+  }
+}
+
+__device__ __forceinline__ double block_sum_d(double v, double* red) {
+  v = group_sum(v, 64);
+  const int w = threadIdx.x >> 6, l = threadIdx.x & 63, nw = blockDim.x >> 6;
+  __syncthreads();
+  if (l == 0) red[w] = v;
+  __syncthreads();
+  double s = 0.0;
+  if (threadIdx.x == 0)
+    for (int k = 0; k < nw; ++k) s += red[k];
+  return s;
+}
+
+// ------------------------------------------------------------------------------------------------
+// Self-overlaps, FP64, one workgroup per (pair, side).
+// ------------------------------------------------------------------------------------------------
+template <int MODE>
+__global__ __launch_bounds__(kPoseBlock) void k_self_overlap(FssPairBatch pb, KScore ks, double* out) {
+  constexpr bool SH = MODE != FSS_MODE_PHARM;
+  constexpr bool ESP = MODE == FSS_MODE_SHAPE_ESP;
+  constexpr bool PH = MODE == FSS_MODE_PHARM || MODE == FSS_MODE_SHAPE_PHARM;
+  __shared__ double red[kPoseBlock / 64];
+  const int pair = blockIdx.x, side = blockIdx.y;
+  const double I[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
+  const double z[3] = {0, 0, 0};
+  double vs = 0.0, vp = 0.0;
+  if (SH) {
+    const int* rg = side ? pb.fit_rng : pb.ref_rng;
+    const float4* p = reinterpret_cast<const float4*>(side ? pb.fit_pts : pb.ref_pts) + rg[2 * pair];
+    const int n = rg[2 * pair + 1];
+    vs = shape_value_f64<ESP>(p, n, p, n, threadIdx.x, blockDim.x, I, z, 0.5 * ks.alpha_d, 1.0 / ks.lam_d);
+    vs = block_sum_d(vs, red) * pow(M_PI / (2.0 * ks.alpha_d), 1.5);
+  }
+  if (PH) {
+    const int* rg = side ? pb.fit_prng : pb.ref_prng;
+    const int o = rg[2 * pair], n = rg[2 * pair + 1];
+    const float4* a = reinterpret_cast<const float4*>(side ? pb.fit_anc : pb.ref_anc) + o;
+    const float4* v = reinterpret_cast<const float4*>(side ? pb.fit_vec : pb.ref_vec) + o;
+    const int32_t* ty = (side ? pb.fit_typ : pb.ref_typ) + o;
+    vp = pharm_value_f64(a, v, ty, n, a, v, ty, n, threadIdx.x, blockDim.x, I, z, ks.ph_alpha, ks.ph_k, ks.ph_cat);
+    vp = block_sum_d(vp, red);
+  }
+  if (threadIdx.x == 0) {
+    out[4 * pair + side] = SH ? vs : 0.0;
+    out[4 * pair + 2 + side] = PH ? vp : 0.0;
+  }
+}
+
+// ------------------------------------------------------------------------------------------------
+// One workgroup per pose.  OUT == 0: channel overlaps and gradients; OUT == 1: loss + SE(3) grad.
+// ------------------------------------------------------------------------------------------------
+template <int MODE, bool GRAD, int OUT>
+__global__ __launch_bounds__(kPoseBlock) void k_pose_grad(FssPairBatch pb, KScore ks, LdsPlan lp,
+                                                          const int32_t* __restrict__ pose_pair,
+                                                          const float* __restrict__ qin,
+                                                          const float* __restrict__ tin,
+                                                          const double* __restrict__ self_ov,
+                                                          float invB, float* V, float* gR,
+                                                          float* gT, float* gQ, float* loss,
+                                                          float* grad) {
+  constexpr bool SH = MODE != FSS_MODE_PHARM;
+  constexpr bool ESP = MODE == FSS_MODE_SHAPE_ESP;
+  constexpr bool PH = MODE == FSS_MODE_PHARM || MODE == FSS_MODE_SHAPE_PHARM;
+  extern __shared__ float4 lds[];
+  __shared__ float red[2][(kPoseBlock / 64) * 13];
+  const int pose = blockIdx.x;
+  const int pair = pose_pair[pose];
+  const Staged st = stage_pair<SH, ESP, PH>(pb, ks, lp, pair, 0, 1, lds);
+  float q[4] = {qin[4 * pose], qin[4 * pose + 1], qin[4 * pose + 2], qin[4 * pose + 3]};
+  const float t[3] = {tin[3 * pose], tin[3 * pose + 1], tin[3 * pose + 2]};
+  double qh[4], qn, Rd[9];
+  rotation_fp64(q, qh, qn, Rd);
+  float R[9];
+#pragma unroll
+  for (int i = 0; i < 9; ++i) R[i] = (float)Rd[i];
+  __syncthreads();
+  Acc sh, ph;
+  sh.zero();
+  ph.zero();
+  if (SH) {
+    const float ts[3] = {t[0] * ks.s, t[1] * ks.s, t[2] * ks.s};
+    shape_accum<ESP, GRAD>(st.ref, st.N, st.fit, st.M, threadIdx.x, blockDim.x, R, ts, sh);
+    block_sum(sh, red[0], GRAD);
+  }
+  if (PH) {
+    pharm_accum<GRAD>(st.ranc, st.rvec, st.Np, st.fanc, st.fvec, st.Mp, threadIdx.x, blockDim.x, R, t, ks.tab, ph);
+    block_sum(ph, red[1], GRAD);
+  }
+  if (threadIdx.x != 0) return;
+  if (SH) shape_finish(sh, ks.K, ks.alpha, ks.s, GRAD);
+  if (OUT == 0) {
+    const Acc* ch[2] = {&sh, &ph};
+    for (int c = 0; c < 2; ++c) {
+      V[2 * pose + c] = ch[c]->V;
+      if (GRAD) {
+        for (int i = 0; i < 9; ++i) gR[18 * pose + 9 * c + i] = ch[c]->G[i];
+        for (int i = 0; i < 3; ++i) gT[6 * pose + 3 * c + i] = ch[c]->gt[i];
+        double g4[4];
+        project_to_quat(ch[c]->G, qh, g4);
+        for (int i = 0; i < 4; ++i) gQ[8 * pose + 4 * c + i] = (float)g4[i];
+      }
+    }
+  } else {
+    float ws, wp;
+    const float S = similarity<float>(ks, sh.V, ph.V, self_ov + 4 * pair, SH, PH, ws, wp);
+    float lR[9], lT[3];
+    for (int i = 0; i < 9; ++i) lR[i] = -(ws * sh.G[i] + wp * ph.G[i]);
+    for (int i = 0; i < 3; ++i) lT[i] = -(ws * sh.gt[i] + wp * ph.gt[i]);
+    float g[7];
+    se3_grad(lR, lT, qh, qn, invB, g);
+    loss[pose] = 1.f - S;
+    for (int i = 0; i < 7; ++i) grad[7 * pose + i] = g[i];
+  }
+}
+
+// ------------------------------------------------------------------------------------------------
+// torch.optim.Adam step (single-tensor form: lerp for m, bias corrections in double).
+// ------------------------------------------------------------------------------------------------
+__global__ void k_adam(int n, float* q, float* t, const float* __restrict__ g, float* m, float* v,
+                       int step, float lr, float b1, float b2, float eps,
+                       const int32_t* __restrict__ active, int B) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n * 7) return;
+  const int row = i / 7, c = i % 7;
+  if (active && !active[row / B]) return;
+  float* p = c < 4 ? q + 4 * row + c : t + 3 * row + (c - 4);
+  const double bc1 = 1.0 - pow((double)b1, (double)step);
+  const double bc2 = 1.0 - pow((double)b2, (double)step);
+  const float step_size = (float)((double)lr / bc1);
+  const float bc2s = (float)sqrt(bc2);
+  const float gi = g[i];
+  float mi = m[i], vi = v[i];
+  mi = fmaf(1.f - b1, gi - mi, mi);
+  vi = fmaf(1.f - b2, gi * gi, vi * b2);
+  m[i] = mi;
+  v[i] = vi;
+  const float den = sqrtf(vi) / bc2s + eps;
+  *p = *p - step_size * (mi / den);
+}
+
+// ------------------------------------------------------------------------------------------------
+// Persistent optimiser.
+// ------------------------------------------------------------------------------------------------
+struct AlignK {
+  int B, max_steps;
+  float lr, b1, b2, eps;
+  float es_tol;
+  int es_pat;
+  int lpp, rpw, wg_per_pair;
+  const int32_t* order;
+  const double* self_ov;
+  float* q;
+  float* t;
+  double* pose_score;
+  int32_t* n_steps;
+};
+
+template <int MODE, bool ES, int MAXT>
+__global__ __launch_bounds__(MAXT) void k_align(FssPairBatch pb, KScore ks, LdsPlan lp, AlignK ak) {
+  constexpr bool SH = MODE != FSS_MODE_PHARM;
+  constexpr bool ESP = MODE == FSS_MODE_SHAPE_ESP;
+  constexpr bool PH = MODE == FSS_MODE_PHARM || MODE == FSS_MODE_SHAPE_PHARM;
+  extern __shared__ float4 lds[];
+  const int slot = blockIdx.x / ak.wg_per_pair;
+  const int blk = blockIdx.x - slot * ak.wg_per_pair;
+  const int pair = ak.order ? ak.order[slot] : slot;
+  const Staged st = stage_pair<SH, ESP, PH>(pb, ks, lp, pair, 0, 1, lds);
+  double* lossbuf = reinterpret_cast<double*>(lds + lp.end);  // 2 * rpw doubles (ES only)
+
+  const int lpp = ak.lpp;
+  const int grp = threadIdx.x / lpp, j = threadIdx.x - grp * lpp;
+  const int r = blk * ak.rpw + grp;
+  const bool active = grp < ak.rpw && r < ak.B;
+  const long pose = (long)pair * ak.B + (active ? r : 0);
+  const double* so = ak.self_ov + 4 * pair;
+  const float invB = 1.0f / (float)ak.B;
+
+  float p[7];
+  float m[7], v[7];
+#pragma unroll
+  for (int k = 0; k < 7; ++k) { m[k] = 0.f; v[k] = 0.f; p[k] = 0.f; }
+  if (active) {
+#pragma unroll
+    for (int k = 0; k < 4; ++k) p[k] = ak.q[4 * pose + k];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) p[4 + k] = ak.t[3 * pose + k];
+  }
+  __syncthreads();
+
+  double b1p = 1.0, b2p = 1.0, last = 1.0;
+  int counter = 0, done = ak.max_steps;
+  for (int step = 1; step <= ak.max_steps; ++step) {
+    float loss = 0.f;
+    if (active) {
+      double qh[4], qn, Rd[9];
+      rotation_fp64(p, qh, qn, Rd);
+      float R[9];
+#pragma unroll
+      for (int i = 0; i < 9; ++i) R[i] = (float)Rd[i];
+      Acc sh, ph;
+      sh.zero();
+      ph.zero();
+      if (SH) {
+        const float ts[3] = {p[4] * ks.s, p[5] * ks.s, p[6] * ks.s};
+        shape_accum<ESP, true>(st.ref, st.N, st.fit, st.M, j, lpp, R, ts, sh);
+        group_sum(sh, lpp, true);
+        shape_finish(sh, ks.K, ks.alpha, ks.s, true);
+      }
+      if (PH) {
+        pharm_accum<true>(st.ranc, st.rvec, st.Np, st.fanc, st.fvec, st.Mp, j, lpp, R, p + 4, ks.tab, ph);
+        group_sum(ph, lpp, true);
+      }
+      float ws, wp;
+      const float S = similarity<float>(ks, sh.V, ph.V, so, SH, PH, ws, wp);
+      loss = 1.f - S;
+      float lR[9], lT[3];
+#pragma unroll
+      for (int i = 0; i < 9; ++i) lR[i] = -(ws * sh.G[i] + wp * ph.G[i]);
+#pragma unroll
+      for (int i = 0; i < 3; ++i) lT[i] = -(ws * sh.gt[i] + wp * ph.gt[i]);
+      float g[7];
+      se3_grad(lR, lT, qh, qn, invB, g);
+      b1p *= (double)ak.b1;
+      b2p *= (double)ak.b2;
+      const float step_size = (float)((double)ak.lr / (1.0 - b1p));
+      const float bc2s = (float)sqrt(1.0 - b2p);
+#pragma unroll
+      for (int k = 0; k < 7; ++k) {
+        m[k] = fmaf(1.f - ak.b1, g[k] - m[k], m[k]);
+        v[k] = fmaf(1.f - ak.b2, g[k] * g[k], v[k] * ak.b2);
+        const float den = sqrtf(v[k]) / bc2s + ak.eps;
+        p[k] = p[k] - step_size * (m[k] / den);
+      }
+    }
+    if (ES) {
+      double* buf = lossbuf + (step & 1) * ak.rpw;
+      if (active && j == 0) buf[grp] = (double)loss;
+      __syncthreads();
+      double L = 0.0;
+      for (int i = 0; i < ak.B; ++i) L += buf[i];
+      L /= (double)ak.B;
+      if (fabs(L - last) > (double)ak.es_tol) counter = 0;
+      else ++counter;
+      last = L;
+      if (counter > ak.es_pat) { done = step; break; }
+    }
+  }
+
+  if (active && j == 0) {
+#pragma unroll
+    for (int k = 0; k < 4; ++k) ak.q[4 * pose + k] = p[k];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) ak.t[3 * pose + k] = p[4 + k];
+  }
+  if (threadIdx.x == 0 && blk == 0) ak.n_steps[pair] = done;
+}
+
+// ------------------------------------------------------------------------------------------------
+// FP64 similarity of every pose, one wave per pose (4 poses per workgroup), straight from the
+// packed unscaled arrays.  pose_pair == NULL: pair = pose / B.
+// ------------------------------------------------------------------------------------------------
+template <int MODE>
+__global__ __launch_bounds__(256) void k_pose_score(FssPairBatch pb, KScore ks, int n_poses, int B,
+                                                    const int32_t* __restrict__ pose_pair,
+                                                    const double* __restrict__ self_ov,
+                                                    const float* __restrict__ qin,
+                                                    const float* __restrict__ tin, double* score) {
+  constexpr bool SH = MODE != FSS_MODE_PHARM;
+  constexpr bool ESP = MODE == FSS_MODE_SHAPE_ESP;
+  constexpr bool PH = MODE == FSS_MODE_PHARM || MODE == FSS_MODE_SHAPE_PHARM;
+  const int pose = blockIdx.x * 4 + (threadIdx.x >> 6);
+  const int j = threadIdx.x & 63;
+  if (pose >= n_poses) return;
+  const int pair = pose_pair ? pose_pair[pose] : pose / B;
+  const float q[4] = {qin[4 * (long)pose], qin[4 * (long)pose + 1], qin[4 * (long)pose + 2], qin[4 * (long)pose + 3]};
+  double qh[4], qn, Rd[9];
+  rotation_fp64(q, qh, qn, Rd);
+  const double td[3] = {tin[3 * (long)pose], tin[3 * (long)pose + 1], tin[3 * (long)pose + 2]};
+  double vs = 0.0, vp = 0.0;
+  if (SH) {
+    const float4* gref = reinterpret_cast<const float4*>(pb.ref_pts) + pb.ref_rng[2 * pair];
+    const float4* gfit = reinterpret_cast<const float4*>(pb.fit_pts) + pb.fit_rng[2 * pair];
+    vs = shape_value_f64<ESP>(gref, pb.ref_rng[2 * pair + 1], gfit, pb.fit_rng[2 * pair + 1], j, 64, Rd, td,
+                              0.5 * ks.alpha_d, 1.0 / ks.lam_d);
+    vs = group_sum(vs, 64) * pow(M_PI / (2.0 * ks.alpha_d), 1.5);
+  }
+  if (PH) {
+    const int ro = pb.ref_prng[2 * pair], fo = pb.fit_prng[2 * pair];
+    vp = pharm_value_f64(reinterpret_cast<const float4*>(pb.ref_anc) + ro,
+                         reinterpret_cast<const float4*>(pb.ref_vec) + ro, pb.ref_typ + ro, pb.ref_prng[2 * pair + 1],
+                         reinterpret_cast<const float4*>(pb.fit_anc) + fo,
+                         reinterpret_cast<const float4*>(pb.fit_vec) + fo, pb.fit_typ + fo, pb.fit_prng[2 * pair + 1],
+                         j, 64, Rd, td, ks.ph_alpha, ks.ph_k, ks.ph_cat);
+    vp = group_sum(vp, 64);
+  }
+  double ws, wp;
+  const double S = similarity<double>(ks, vs, vp, self_ov + 4 * pair, SH, PH, ws, wp);
+  if (j == 0) score[pose] = S;
+}
+
+// Per-pair early-stop bookkeeping of the stepped optimiser: mean loss over the pair's B poses,
+// the |dL| <= tol counter, and deactivation (recording the step count) once it exceeds the
+// patience.  State: last (n_pairs) double, counter / active / n_steps (n_pairs) int32.
+__global__ void k_es_update(int n_pairs, int B, const float* __restrict__ loss, double* last,
+                            int32_t* counter, int32_t* active, int32_t* n_steps, int step, double tol,
+                            int patience) {
+  const int pair = blockIdx.x * blockDim.x + threadIdx.x;
+  if (pair >= n_pairs || !active[pair]) return;
+  double L = 0.0;
+  for (int r = 0; r < B; ++r) L += (double)loss[(long)pair * B + r];
+  L /= (double)B;
+  int c = counter[pair];
+  c = fabs(L - last[pair]) > tol ? 0 : c + 1;
+  counter[pair] = c;
+  last[pair] = L;
+  n_steps[pair] = step;
+  if (c > patience) active[pair] = 0;
+}
+
+__global__ void k_align_finish(int n_pairs, int B, const float* __restrict__ q,
+                               const float* __restrict__ t, const double* __restrict__ ps,
+                               double* best_score, int32_t* best_idx, float* best_tf) {
+  const int pair = blockIdx.x * blockDim.x + threadIdx.x;
+  if (pair >= n_pairs) return;
+  int bi = 0;
+  double bs = ps[(long)pair * B];
+  for (int r = 1; r < B; ++r) {
+    const double s = ps[(long)pair * B + r];
+    if (s > bs || (isnan(s) && !isnan(bs))) { bs = s; bi = r; }
+  }
+  const long pose = (long)pair * B + bi;
+  const float qq[4] = {q[4 * pose], q[4 * pose + 1], q[4 * pose + 2], q[4 * pose + 3]};
+  double qh[4], qn, Rd[9];
+  rotation_fp64(qq, qh, qn, Rd);
+  float* T = best_tf + 16 * (long)pair;
+  for (int i = 0; i < 3; ++i) {
+    for (int k = 0; k < 3; ++k) T[4 * i + k] = (float)Rd[3 * i + k];
+    T[4 * i + 3] = t[3 * pose + i];
+  }
+  T[12] = 0.f; T[13] = 0.f; T[14] = 0.f; T[15] = 1.f;
+  best_score[pair] = bs;
+  best_idx[pair] = bi;
+}
+
+// ------------------------------------------------------------------------------------------------
+// Differentiable similarity of fixed clouds (shape or ESP shape), FP32 or FP64.
+// ------------------------------------------------------------------------------------------------
+template <typename T>
+__device__ __forceinline__ T gauss(T x);
+template <>
+__device__ __forceinline__ float gauss<float>(float x) { return fast_exp2(x * kLog2e); }
+template <>
+__device__ __forceinline__ double gauss<double>(double x) { return exp(x); }
+
+template <typename T, bool ESP>
+__device__ __forceinline__ T pair_sum(const T* __restrict__ A, int N, const T* __restrict__ Bp, int M,
+                                      int j, int nt, T ha, T il) {
+  T v = 0;
+  for (int b = j; b < M; b += nt) {
+    const T bx = Bp[4 * b], by = Bp[4 * b + 1], bz = Bp[4 * b + 2], bw = Bp[4 * b + 3];
+    for (int a = 0; a < N; ++a) {
+      const T dx = A[4 * a] - bx, dy = A[4 * a + 1] - by, dz = A[4 * a + 2] - bz;
+      T arg = -ha * (dx * dx + dy * dy + dz * dz);
+      if (ESP) { const T dc = A[4 * a + 3] - bw; arg -= dc * dc * il; }
+      v += gauss<T>(arg);
+    }
+  }
+  return v;
+}
+
+template <typename T>
+__device__ __forceinline__ T block_sum_t(T v, T* red) {
+  v = group_sum(v, 64);
+  const int w = threadIdx.x >> 6, l = threadIdx.x & 63, nw = blockDim.x >> 6;
+  __syncthreads();
+  if (l == 0) red[w] = v;
+  __syncthreads();
+  T s = 0;
+  for (int k = 0; k < nw; ++k) s += red[k];
+  __syncthreads();
+  return s;
+}
+
+template <typename T, bool ESP>
+__global__ __launch_bounds__(kPoseBlock) void k_score_fwd(const T* __restrict__ ref, const T* __restrict__ fit,
+                                                          const int32_t* __restrict__ rr,
+                                                          const int32_t* __restrict__ fr, T alpha, T lam,
+                                                          T* S, T* ov) {
+  __shared__ T red[kPoseBlock / 64];
+  const int pair = blockIdx.x;
+  const T* A = ref + 4 * (long)rr[2 * pair];
+  const T* Bp = fit + 4 * (long)fr[2 * pair];
+  const int N = rr[2 * pair + 1], M = fr[2 * pair + 1];
+  const T ha = alpha / 2, il = (T)1 / lam;
+  const T K = (T)pow(M_PI / (2.0 * (double)alpha), 1.5);
+  const T vab = block_sum_t(pair_sum<T, ESP>(A, N, Bp, M, threadIdx.x, blockDim.x, ha, il), red) * K;
+  const T vaa = block_sum_t(pair_sum<T, ESP>(A, N, A, N, threadIdx.x, blockDim.x, ha, il), red) * K;
+  const T vbb = block_sum_t(pair_sum<T, ESP>(Bp, M, Bp, M, threadIdx.x, blockDim.x, ha, il), red) * K;
+  if (threadIdx.x == 0) {
+    S[pair] = vab / (vaa + vbb - vab);
+    ov[4 * pair] = vab; ov[4 * pair + 1] = vaa; ov[4 * pair + 2] = vbb; ov[4 * pair + 3] = 0;
+  }
+}
+
+// d/dx_b of sum_{a in A} w E(a, x_b) = K alpha sum_a w E (a - x_b)
+template <typename T, bool ESP>
+__device__ __forceinline__ void point_force(const T* __restrict__ A, int N, T bx, T by, T bz, T bw,
+                                            T ha, T il, T f[3]) {
+  f[0] = f[1] = f[2] = 0;
+  for (int a = 0; a < N; ++a) {
+    const T dx = A[4 * a] - bx, dy = A[4 * a + 1] - by, dz = A[4 * a + 2] - bz;
+    T arg = -ha * (dx * dx + dy * dy + dz * dz);
+    if (ESP) { const T dc = A[4 * a + 3] - bw; arg -= dc * dc * il; }
+    const T e = gauss<T>(arg);
+    f[0] += e * dx; f[1] += e * dy; f[2] += e * dz;
+  }
+}
+
+template <typename T, bool ESP>
+__global__ __launch_bounds__(kPoseBlock) void k_score_bwd(const T* __restrict__ ref, const T* __restrict__ fit,
+                                                          const int32_t* __restrict__ rr,
+                                                          const int32_t* __restrict__ fr, T alpha, T lam,
+                                                          const T* __restrict__ ov, const T* __restrict__ gS,
+                                                          T* d_fit, T* d_ref) {
+  const int pair = blockIdx.x;
+  const T* A = ref + 4 * (long)rr[2 * pair];
+  const T* Bp = fit + 4 * (long)fr[2 * pair];
+  const int N = rr[2 * pair + 1], M = fr[2 * pair + 1];
+  const T ha = alpha / 2, il = (T)1 / lam;
+  const T K = (T)pow(M_PI / (2.0 * (double)alpha), 1.5);
+  const T vab = ov[4 * pair], vaa = ov[4 * pair + 1], vbb = ov[4 * pair + 2];
+  const T U = vaa + vbb, D = U - vab;
+  const T g = gS[pair] * K * alpha / (D * D);
+  // dS/dx_b = (U dVAB/dx_b - VAB dVBB/dx_b) / D^2, dVBB/dx_b = 2 K alpha sum_b' w E (b' - b)
+  for (int b = threadIdx.x; b < M; b += blockDim.x) {
+    const T bx = Bp[4 * b], by = Bp[4 * b + 1], bz = Bp[4 * b + 2], bw = Bp[4 * b + 3];
+    T fa[3], fb[3];
+    point_force<T, ESP>(A, N, bx, by, bz, bw, ha, il, fa);
+    point_force<T, ESP>(Bp, M, bx, by, bz, bw, ha, il, fb);
+    T* o = d_fit + 4 * ((long)fr[2 * pair] + b);
+    for (int i = 0; i < 3; ++i) o[i] = g * (U * fa[i] - 2 * vab * fb[i]);
+    o[3] = 0;
+  }
+  if (d_ref) {
+    for (int a = threadIdx.x; a < N; a += blockDim.x) {
+      const T ax = A[4 * a], ay = A[4 * a + 1], az = A[4 * a + 2], aw = A[4 * a + 3];
+      T fa[3], fb[3];
+      point_force<T, ESP>(Bp, M, ax, ay, az, aw, ha, il, fb);
+      point_force<T, ESP>(A, N, ax, ay, az, aw, ha, il, fa);
+      T* o = d_ref + 4 * ((long)rr[2 * pair] + a);
+      for (int i = 0; i < 3; ++i) o[i] = g * (U * fb[i] - 2 * vab * fa[i]);
+      o[3] = 0;
+    }
+  }
+}
+
+template <int MODE>
+void launch_self(const FssPairBatch* pb, const KScore& ks, double* out, hipStream_t s) {
+  hipLaunchKernelGGL(k_self_overlap<MODE>, dim3(pb->n_pairs, 2), dim3(kPoseBlock), 0, s, *pb, ks, out);
+}
+
+template <int MODE, bool GRAD, int OUT>
+void launch_pose(const FssPairBatch* pb, const KScore& ks, LdsPlan lp, int n, const int32_t* pp,
+                 const float* q, const float* t, const double* so, float invB, float* V, float* gR,
+                 float* gT, float* gQ, float* loss, float* grad, hipStream_t s) {
+  hipLaunchKernelGGL((k_pose_grad<MODE, GRAD, OUT>), dim3(n), dim3(kPoseBlock), lp.end * sizeof(float4), s,
+                     *pb, ks, lp, pp, q, t, so, invB, V, gR, gT, gQ, loss, grad);
+}
+
+// Workgroups of up to 512 threads get a 256-VGPR budget, larger ones 128.
+template <int MODE, bool ES>
+void launch_align(const FssPairBatch* pb, const KScore& ks, LdsPlan lp, const AlignK& ak, int grid,
+                  int block, size_t lds, hipStream_t s) {
+  if (block <= 512)
+    hipLaunchKernelGGL((k_align<MODE, ES, 512>), dim3(grid), dim3(block), lds, s, *pb, ks, lp, ak);
+  else
+    hipLaunchKernelGGL((k_align<MODE, ES, 1024>), dim3(grid), dim3(block), lds, s, *pb, ks, lp, ak);
+}
+
+int score_check(const FssPairBatch* batch, const FssScoreParams* sp) {
+  if (!sp) return set_error(FSS_E_INVALID, "score params are NULL");
+  if (sp->mode != FSS_MODE_SHAPE && sp->mode != FSS_MODE_SHAPE_ESP)
+    return set_error(FSS_E_INVALID, "score fwd/bwd supports the shape and ESP modes only");
+  if (!(sp->alpha > 0.f)) return set_error(FSS_E_INVALID, "alpha must be > 0");
+  if (sp->mode == FSS_MODE_SHAPE_ESP && !(sp->lam > 0.f)) return set_error(FSS_E_INVALID, "lam must be > 0");
+  if (!batch || !batch->ref_rng || !batch->fit_rng) return set_error(FSS_E_INVALID, "NULL ranges");
+  return FSS_OK;
+}
+
+template <typename T>
+int score_fwd_t(const FssPairBatch* batch, const FssScoreParams* sp, const T* ref, const T* fit,
+                       T* S, T* ov, hipStream_t s) {
+  int rc = score_check(batch, sp);
+  if (rc) return rc;
+  if (!ref || !fit || !S || !ov) return set_error(FSS_E_INVALID, "NULL score arrays");
+  if (batch->n_pairs == 0) return FSS_OK;
+  const T lam = sp->mode == FSS_MODE_SHAPE_ESP ? (T)sp->lam : (T)1;
+  if (sp->mode == FSS_MODE_SHAPE_ESP)
+    hipLaunchKernelGGL((k_score_fwd<T, true>), dim3(batch->n_pairs), dim3(kPoseBlock), 0, s, ref, fit, batch->ref_rng, batch->fit_rng, (T)sp->alpha, lam, S, ov);
+  else
+    hipLaunchKernelGGL((k_score_fwd<T, false>), dim3(batch->n_pairs), dim3(kPoseBlock), 0, s, ref, fit, batch->ref_rng, batch->fit_rng, (T)sp->alpha, lam, S, ov);
+  return check_launch("fss_score_fwd");
+}
+
+template <typename T>
+int score_bwd_t(const FssPairBatch* batch, const FssScoreParams* sp, const T* ref, const T* fit,
+                       const T* ov, const T* gS, T* d_fit, T* d_ref, hipStream_t s) {
+  int rc = score_check(batch, sp);
+  if (rc) return rc;
+  if (!ref || !fit || !ov || !gS || !d_fit) return set_error(FSS_E_INVALID, "NULL score arrays");
+  if (batch->n_pairs == 0) return FSS_OK;
+  const T lam = sp->mode == FSS_MODE_SHAPE_ESP ? (T)sp->lam : (T)1;
+  if (sp->mode == FSS_MODE_SHAPE_ESP)
+    hipLaunchKernelGGL((k_score_bwd<T, true>), dim3(batch->n_pairs), dim3(kPoseBlock), 0, s, ref, fit, batch->ref_rng, batch->fit_rng, (T)sp->alpha, lam, ov, gS, d_fit, d_ref);
+  else
+    hipLaunchKernelGGL((k_score_bwd<T, false>), dim3(batch->n_pairs), dim3(kPoseBlock), 0, s, ref, fit, batch->ref_rng, batch->fit_rng, (T)sp->alpha, lam, ov, gS, d_fit, d_ref);
+  return check_launch("fss_score_bwd");
+}
+
+template <int MODE>
+void launch_pose_score(const FssPairBatch* pb, const KScore& ks, int n, int B, const int32_t* pp,
+                              const double* so, const float* q, const float* t, double* out, hipStream_t s) {
+  hipLaunchKernelGGL(k_pose_score<MODE>, dim3((n + 3) / 4), dim3(256), 0, s, *pb, ks, n, B, pp, so, q, t, out);
+}
+
+int pose_score_impl(const FssPairBatch* batch, const KScore& ks, int32_t n_poses,
+                           const int32_t* pose_pair, int32_t B, const double* self_ov, const float* q,
+                           const float* t, double* score, hipStream_t s) {
+  switch (ks.mode) {
+    case FSS_MODE_SHAPE: launch_pose_score<FSS_MODE_SHAPE>(batch, ks, n_poses, B, pose_pair, self_ov, q, t, score, s); break;
+    case FSS_MODE_SHAPE_ESP: launch_pose_score<FSS_MODE_SHAPE_ESP>(batch, ks, n_poses, B, pose_pair, self_ov, q, t, score, s); break;
+    case FSS_MODE_PHARM: launch_pose_score<FSS_MODE_PHARM>(batch, ks, n_poses, B, pose_pair, self_ov, q, t, score, s); break;
+    default: launch_pose_score<FSS_MODE_SHAPE_PHARM>(batch, ks, n_poses, B, pose_pair, self_ov, q, t, score, s); break;
+  }
+  return check_launch("fss_pose_score");
+}
+
+int align_finish_impl(const FssPairBatch* batch, const KScore& ks, int B, const double* self_ov,
+                      FssAlignOut* out, hipStream_t s) {
+  int rc = pose_score_impl(batch, ks, batch->n_pairs * B, nullptr, B, self_ov, out->q, out->t, out->pose_score, s);
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_align_finish, dim3((batch->n_pairs + 255) / 256), dim3(256), 0, s, batch->n_pairs, B,
+                     out->q, out->t, out->pose_score, out->best_score, out->best_idx, out->best_tf);
+  return check_launch("fss_align (finish)");
+}
+
+}  // namespace
+
+// ================================================================================================
+// C ABI
+// ================================================================================================
+extern "C" {
+
+const char* fss_version(void) { return "fss_hip 0.1.0 gfx950"; }
+
+const char* fss_last_error(void) { return g_last_error.c_str(); }
+
+int fss_device_count(int* count) {
+  if (!count) return set_error(FSS_E_INVALID, "count is NULL");
+  hipError_t e = hipGetDeviceCount(count);
+  if (e != hipSuccess) { *count = 0; return set_error((int)e, "hipGetDeviceCount: %s", hipGetErrorString(e)); }
+  return FSS_OK;
+}
+
+int fss_self_overlap(const FssPairBatch* batch, const FssScoreParams* sp, double* out, void* stream) {
+  KScore ks;
+  int rc = make_kscore(sp, ks);
+  if (rc) return rc;
+  if ((rc = check_batch(batch, ks.mode))) return rc;
+  if (!out) return set_error(FSS_E_INVALID, "out is NULL");
+  if (batch->n_pairs == 0) return FSS_OK;
+  hipStream_t s = (hipStream_t)stream;
+  switch (ks.mode) {
+    case FSS_MODE_SHAPE: launch_self<FSS_MODE_SHAPE>(batch, ks, out, s); break;
+    case FSS_MODE_SHAPE_ESP: launch_self<FSS_MODE_SHAPE_ESP>(batch, ks, out, s); break;
+    case FSS_MODE_PHARM: launch_self<FSS_MODE_PHARM>(batch, ks, out, s); break;
+    default: launch_self<FSS_MODE_SHAPE_PHARM>(batch, ks, out, s); break;
+  }
+  return check_launch("fss_self_overlap");
+}
+
+static int check_pose_lds(const FssPairBatch* batch, const LdsPlan& lp) {
+  const size_t need = (size_t)lp.end * sizeof(float4);
+  const size_t cap = max_lds_bytes() - 2 * (kPoseBlock / 64) * 13 * sizeof(float);
+  if (need > cap)
+    return set_error(FSS_E_TOO_LARGE, "clouds need %zu B of LDS, the device allows %zu B", need, cap);
+  (void)batch;
+  return FSS_OK;
+}
+
+int fss_overlap_grad(const FssPairBatch* batch, const FssScoreParams* sp, int32_t n_poses,
+                     const int32_t* pose_pair, const float* q, const float* t, int32_t need_grad,
+                     float* V, float* gR, float* gT, float* gQ, void* stream) {
+  KScore ks;
+  int rc = make_kscore(sp, ks);
+  if (rc) return rc;
+  if ((rc = check_batch(batch, ks.mode))) return rc;
+  if (n_poses < 0 || !pose_pair || !q || !t || !V) return set_error(FSS_E_INVALID, "NULL pose arrays");
+  if (need_grad && (!gR || !gT || !gQ)) return set_error(FSS_E_INVALID, "NULL gradient outputs");
+  if (n_poses == 0) return FSS_OK;
+  const LdsPlan lp = plan_lds(batch, ks.mode);
+  if ((rc = check_pose_lds(batch, lp))) return rc;
+  hipStream_t s = (hipStream_t)stream;
+#define FSS_POSE0(M)                                                                                      \
+  do {                                                                                                    \
+    if (need_grad) launch_pose<M, true, 0>(batch, ks, lp, n_poses, pose_pair, q, t, nullptr, 1.f, V, gR, gT, gQ, nullptr, nullptr, s); \
+    else launch_pose<M, false, 0>(batch, ks, lp, n_poses, pose_pair, q, t, nullptr, 1.f, V, gR, gT, gQ, nullptr, nullptr, s);         \
+  } while (0)
+  switch (ks.mode) {
+    case FSS_MODE_SHAPE: FSS_POSE0(FSS_MODE_SHAPE); break;
+    case FSS_MODE_SHAPE_ESP: FSS_POSE0(FSS_MODE_SHAPE_ESP); break;
+    case FSS_MODE_PHARM: FSS_POSE0(FSS_MODE_PHARM); break;
+    default: FSS_POSE0(FSS_MODE_SHAPE_PHARM); break;
+  }
+#undef FSS_POSE0
+  return check_launch("fss_overlap_grad");
+}
+
+int fss_pose_loss_grad(const FssPairBatch* batch, const FssScoreParams* sp, int32_t n_poses,
+                       const int32_t* pose_pair, int32_t n_restarts, const double* self_ov,
+                       const float* q, const float* t, float* loss, float* grad, void* stream) {
+  KScore ks;
+  int rc = make_kscore(sp, ks);
+  if (rc) return rc;
+  if ((rc = check_batch(batch, ks.mode))) return rc;
+  if (n_poses < 0 || !pose_pair || !q || !t || !loss || !grad || !self_ov)
+    return set_error(FSS_E_INVALID, "NULL pose arrays");
+  if (n_restarts <= 0) return set_error(FSS_E_INVALID, "n_restarts must be > 0");
+  if (n_poses == 0) return FSS_OK;
+  const LdsPlan lp = plan_lds(batch, ks.mode);
+  if ((rc = check_pose_lds(batch, lp))) return rc;
+  hipStream_t s = (hipStream_t)stream;
+  const float invB = 1.0f / (float)n_restarts;
+  switch (ks.mode) {
+    case FSS_MODE_SHAPE: launch_pose<FSS_MODE_SHAPE, true, 1>(batch, ks, lp, n_poses, pose_pair, q, t, self_ov, invB, nullptr, nullptr, nullptr, nullptr, loss, grad, s); break;
+    case FSS_MODE_SHAPE_ESP: launch_pose<FSS_MODE_SHAPE_ESP, true, 1>(batch, ks, lp, n_poses, pose_pair, q, t, self_ov, invB, nullptr, nullptr, nullptr, nullptr, loss, grad, s); break;
+    case FSS_MODE_PHARM: launch_pose<FSS_MODE_PHARM, true, 1>(batch, ks, lp, n_poses, pose_pair, q, t, self_ov, invB, nullptr, nullptr, nullptr, nullptr, loss, grad, s); break;
+    default: launch_pose<FSS_MODE_SHAPE_PHARM, true, 1>(batch, ks, lp, n_poses, pose_pair, q, t, self_ov, invB, nullptr, nullptr, nullptr, nullptr, loss, grad, s); break;
+  }
+  return check_launch("fss_pose_loss_grad");
+}
+
+int fss_adam_step(int32_t n, float* q, float* t, const float* grad, float* m, float* v, int32_t step,
+                  float lr, float beta1, float beta2, float eps, const int32_t* pair_active,
+                  int32_t n_restarts, void* stream) {
+  if (n < 0 || !q || !t || !grad || !m || !v) return set_error(FSS_E_INVALID, "NULL Adam arrays");
+  if (step < 1) return set_error(FSS_E_INVALID, "Adam step count is 1-based");
+  if (pair_active && n_restarts <= 0) return set_error(FSS_E_INVALID, "n_restarts must be > 0");
+  if (n == 0) return FSS_OK;
+  const int tot = n * 7;
+  hipLaunchKernelGGL(k_adam, dim3((tot + 255) / 256), dim3(256), 0, (hipStream_t)stream, n, q, t, grad, m, v,
+                     step, lr, beta1, beta2, eps, pair_active, pair_active ? n_restarts : 1);
+  return check_launch("fss_adam_step");
+}
+
+int fss_early_stop_update(int32_t n_pairs, int32_t n_restarts, const float* loss, double* last,
+                          int32_t* counter, int32_t* active, int32_t* n_steps, int32_t step, float tol,
+                          int32_t patience, void* stream) {
+  if (n_pairs < 0 || n_restarts <= 0 || !loss || !last || !counter || !active || !n_steps)
+    return set_error(FSS_E_INVALID, "bad early-stop arguments");
+  if (n_pairs == 0) return FSS_OK;
+  hipLaunchKernelGGL(k_es_update, dim3((n_pairs + 255) / 256), dim3(256), 0, (hipStream_t)stream, n_pairs,
+                     n_restarts, loss, last, counter, active, n_steps, step, (double)tol, patience);
+  return check_launch("fss_early_stop_update");
+}
+
+int fss_pose_score(const FssPairBatch* batch, const FssScoreParams* sp, int32_t n_poses,
+                   const int32_t* pose_pair, int32_t n_restarts, const double* self_ov, const float* q,
+                   const float* t, double* score, void* stream) {
+  KScore ks;
+  int rc = make_kscore(sp, ks);
+  if (rc) return rc;
+  if ((rc = check_batch(batch, ks.mode))) return rc;
+  if (n_poses < 0 || !self_ov || !q || !t || !score) return set_error(FSS_E_INVALID, "NULL pose arrays");
+  if (!pose_pair && n_restarts <= 0) return set_error(FSS_E_INVALID, "n_restarts must be > 0 without pose_pair");
+  if (n_poses == 0) return FSS_OK;
+  return pose_score_impl(batch, ks, n_poses, pose_pair, n_restarts, self_ov, q, t, score, (hipStream_t)stream);
+}
+
+int fss_align(const FssPairBatch* batch, const FssScoreParams* sp, const FssAlignParams* ap,
+              const double* self_ov, FssAlignOut* out, void* stream) {
+  KScore ks;
+  int rc = make_kscore(sp, ks);
+  if (rc) return rc;
+  if ((rc = check_batch(batch, ks.mode))) return rc;
+  if (!ap || !out || !self_ov) return set_error(FSS_E_INVALID, "NULL align arguments");
+  if (!out->q || !out->t || !out->pose_score || !out->best_score || !out->best_idx || !out->best_tf || !out->n_steps)
+    return set_error(FSS_E_INVALID, "NULL align outputs");
+  const int B = ap->n_restarts, lpp = ap->lanes_per_pose, rpw = ap->restarts_per_wg;
+  if (B <= 0) return set_error(FSS_E_INVALID, "n_restarts must be > 0");
+  if (ap->max_steps < 0) return set_error(FSS_E_INVALID, "max_steps must be >= 0");
+  if (lpp < 1 || lpp > 64 || (lpp & (lpp - 1))) return set_error(FSS_E_INVALID, "lanes_per_pose must be a power of two in [1, 64]");
+  if (rpw < 1) return set_error(FSS_E_INVALID, "restarts_per_wg must be >= 1");
+  if (ap->early_stop && rpw < B)
+    return set_error(FSS_E_INVALID, "early stop couples all %d restarts of a pair: restarts_per_wg (%d) must be >= n_restarts", B, rpw);
+  const int block = ((rpw * lpp + 63) / 64) * 64;
+  if (block > 1024) return set_error(FSS_E_INVALID, "restarts_per_wg * lanes_per_pose = %d exceeds 1024 threads", rpw * lpp);
+  if (batch->n_pairs == 0) return FSS_OK;
+  const LdsPlan lp = plan_lds(batch, ks.mode);
+  const size_t lds = (size_t)lp.end * sizeof(float4) + (ap->early_stop ? 2 * (size_t)rpw * sizeof(double) : 0);
+  const size_t cap = max_lds_bytes();
+  if (lds > cap) return set_error(FSS_E_TOO_LARGE, "clouds need %zu B of LDS, the device allows %zu B", lds, cap);
+  AlignK ak;
+  ak.B = B; ak.max_steps = ap->max_steps;
+  ak.lr = ap->lr; ak.b1 = ap->beta1; ak.b2 = ap->beta2; ak.eps = ap->eps;
+  ak.es_tol = ap->es_tol; ak.es_pat = ap->es_patience;
+  ak.lpp = lpp; ak.rpw = rpw; ak.wg_per_pair = (B + rpw - 1) / rpw;
+  ak.order = ap->pair_order; ak.self_ov = self_ov;
+  ak.q = out->q; ak.t = out->t; ak.pose_score = out->pose_score; ak.n_steps = out->n_steps;
+  const long grid = (long)batch->n_pairs * ak.wg_per_pair;
+  if (grid > 0x7fffffffL) return set_error(FSS_E_INVALID, "grid too large");
+  hipStream_t s = (hipStream_t)stream;
+  const bool es = ap->early_stop != 0;
+#define FSS_ALIGN(M) \
+  (es ? launch_align<M, true>(batch, ks, lp, ak, (int)grid, block, lds, s) : launch_align<M, false>(batch, ks, lp, ak, (int)grid, block, lds, s))
+  switch (ks.mode) {
+    case FSS_MODE_SHAPE: FSS_ALIGN(FSS_MODE_SHAPE); break;
+    case FSS_MODE_SHAPE_ESP: FSS_ALIGN(FSS_MODE_SHAPE_ESP); break;
+    case FSS_MODE_PHARM: FSS_ALIGN(FSS_MODE_PHARM); break;
+    default: FSS_ALIGN(FSS_MODE_SHAPE_PHARM); break;
+  }
+#undef FSS_ALIGN
+  if ((rc = check_launch("fss_align"))) return rc;
+  if (ap->skip_finish) return FSS_OK;
+  return align_finish_impl(batch, ks, B, self_ov, out, s);
+}
+
+int fss_align_finish(const FssPairBatch* batch, const FssScoreParams* sp, int32_t n_restarts,
+                     const double* self_ov, FssAlignOut* out, void* stream) {
+  KScore ks;
+  int rc = make_kscore(sp, ks);
+  if (rc) return rc;
+  if ((rc = check_batch(batch, ks.mode))) return rc;
+  if (!out || !self_ov || !out->q || !out->t || !out->pose_score || !out->best_score || !out->best_idx || !out->best_tf)
+    return set_error(FSS_E_INVALID, "NULL align outputs");
+  if (n_restarts <= 0) return set_error(FSS_E_INVALID, "n_restarts must be > 0");
+  if (batch->n_pairs == 0) return FSS_OK;
+  return align_finish_impl(batch, ks, n_restarts, self_ov, out, (hipStream_t)stream);
+}
+
+int fss_score_fwd(const FssPairBatch* batch, const FssScoreParams* sp, float* S, float* ov, void* stream) {
+  if (!batch) return set_error(FSS_E_INVALID, "pair batch is NULL");
+  return score_fwd_t<float>(batch, sp, batch->ref_pts, batch->fit_pts, S, ov, (hipStream_t)stream);
+}
+
+int fss_score_bwd(const FssPairBatch* batch, const FssScoreParams* sp, const float* ov, const float* gS,
+                  float* d_fit, float* d_ref, void* stream) {
+  if (!batch) return set_error(FSS_E_INVALID, "pair batch is NULL");
+  return score_bwd_t<float>(batch, sp, batch->ref_pts, batch->fit_pts, ov, gS, d_fit, d_ref, (hipStream_t)stream);
+}
+
+int fss_score_fwd_f64(const FssPairBatch* batch, const FssScoreParams* sp, const double* ref_pts,
+                      const double* fit_pts, double* S, double* ov, void* stream) {
+  return score_fwd_t<double>(batch, sp, ref_pts, fit_pts, S, ov, (hipStream_t)stream);
+}
+
+int fss_score_bwd_f64(const FssPairBatch* batch, const FssScoreParams* sp, const double* ref_pts,
+                      const double* fit_pts, const double* ov, const double* gS, double* d_fit,
+                      double* d_ref, void* stream) {
+  return score_bwd_t<double>(batch, sp, ref_pts, fit_pts, ov, gS, d_fit, d_ref, (hipStream_t)stream);
+}
+
+}  // extern "C"

@@ -1,0 +1,126 @@
+"""Seeded synthetic molecules for the benchmark workloads (numpy, deterministic).
+
+* atoms: a 3D random walk from the origin with 1.5 A steps in uniform random directions; a
+  direction is redrawn (up to 100 tries) until the new atom is >= 1.2 A from every earlier
+  atom; the walk is then centred on its mean.
+* surface: points drawn uniformly on the exposed part of the union of spheres of radius
+  1.2 * 1.7 A around the atoms (a draw inside any other sphere is rejected).
+* ESP values: N(0, 0.1) clipped to +-0.3.
+* pharmacophores: types drawn uniformly from the 8 real types (0..7), anchor = a random atom
+  + N(0, 0.5 A), unit direction vectors for acceptor / donor / aromatic / halogen and zero
+  vectors for the others.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+STEP = 1.5
+MIN_SEP = 1.2
+VDW = 1.7
+SURF_SCALE = 1.2
+DIRECTIONAL_TYPE_IDS = (0, 1, 2, 4)   # acceptor, donor, aromatic, halogen
+
+
+def _unit(rng, shape):
+    v = rng.normal(size=shape + (3,))
+    return v / np.linalg.norm(v, axis=-1, keepdims=True)
+
+
+def random_walks(rng: np.random.Generator, counts) -> list:
+    """Random-walk heavy-atom clouds, one per entry of ``counts`` (vectorised over molecules)."""
+    counts = np.asarray(counts, dtype=np.int64)
+    n = len(counts)
+    L = int(counts.max()) if n else 0
+    X = np.zeros((n, max(L, 1), 3))
+    for k in range(1, L):
+        live = np.nonzero(counts > k)[0]
+        prev = X[live, k - 1]
+        pos = prev + STEP * _unit(rng, (len(live),))
+        todo = np.ones(len(live), dtype=bool)
+        for _ in range(100):
+            d = np.linalg.norm(X[live, :k] - pos[:, None, :], axis=-1)
+            ok = (d >= MIN_SEP).all(axis=1)
+            todo &= ~ok
+            if not todo.any():
+                break
+            idx = np.nonzero(todo)[0]
+            pos[idx] = prev[idx] + STEP * _unit(rng, (len(idx),))
+        X[live, k] = pos
+    out = []
+    for i in range(n):
+        a = X[i, :counts[i]]
+        out.append((a - a.mean(axis=0)).astype(np.float32))
+    return out
+
+
+def surface_points(rng: np.random.Generator, atoms: np.ndarray, n_points: int) -> np.ndarray:
+    """n_points uniform points on the exposed union of 1.2*vdW spheres around ``atoms``."""
+    r = SURF_SCALE * VDW
+    atoms = np.asarray(atoms, np.float64)
+    got = []
+    need = n_points
+    while need > 0:
+        m = max(2 * need, 64)
+        which = rng.integers(0, len(atoms), size=m)
+        pts = atoms[which] + r * _unit(rng, (m,))
+        d = np.linalg.norm(pts[:, None, :] - atoms[None, :, :], axis=-1)
+        d[np.arange(m), which] = np.inf
+        keep = pts[(d >= r - 1e-9).all(axis=1)]
+        got.append(keep[:need])
+        need -= len(got[-1])
+    return np.concatenate(got, axis=0).astype(np.float32)
+
+
+def esp_values(rng: np.random.Generator, n: int) -> np.ndarray:
+    return np.clip(rng.normal(0.0, 0.1, size=n), -0.3, 0.3).astype(np.float32)
+
+
+def pharmacophores(rng: np.random.Generator, atoms: np.ndarray, n_ph: int):
+    """(types int64 (n_ph,), anchors (n_ph, 3), vectors (n_ph, 3))."""
+    types = rng.integers(0, 8, size=n_ph)
+    anchors = atoms[rng.integers(0, len(atoms), size=n_ph)] + rng.normal(0.0, 0.5, size=(n_ph, 3))
+    vecs = _unit(rng, (n_ph,))
+    vecs[~np.isin(types, DIRECTIONAL_TYPE_IDS)] = 0.0
+    return types.astype(np.int64), anchors.astype(np.float32), vecs.astype(np.float32)
+
+
+def random_quaternions(rng: np.random.Generator, n: int) -> np.ndarray:
+    """Uniform random unit quaternions (normalised 4-D Gaussian), (n, 4) float32."""
+    q = rng.normal(size=(n, 4))
+    return (q / np.linalg.norm(q, axis=1, keepdims=True)).astype(np.float32)
+
+
+# ---------------------------------------------------------------------------------------------
+# Workloads
+# ---------------------------------------------------------------------------------------------
+def vol_pairs(n_pairs: int, lo: int = 20, hi: int = 40, seed: int = 0):
+    """Atom-cloud pairs (ref list, fit list) with U{lo..hi} atoms each."""
+    rng = np.random.default_rng(seed)
+    ref = random_walks(rng, rng.integers(lo, hi + 1, size=n_pairs))
+    fit = random_walks(rng, rng.integers(lo, hi + 1, size=n_pairs))
+    return ref, fit
+
+
+def surf_esp_pairs(n_pairs: int, atoms=(15, 50), points=(200, 400), seed: int = 1):
+    """Surface + ESP pairs: (ref_pts, ref_esp, fit_pts, fit_esp) lists."""
+    rng = np.random.default_rng(seed)
+    mols = random_walks(rng, rng.integers(atoms[0], atoms[1] + 1, size=2 * n_pairs))
+    npts = rng.integers(points[0], points[1] + 1, size=2 * n_pairs)
+    pts = [surface_points(rng, m, int(k)) for m, k in zip(mols, npts)]
+    esp = [esp_values(rng, len(p)) for p in pts]
+    return pts[:n_pairs], esp[:n_pairs], pts[n_pairs:], esp[n_pairs:]
+
+
+def pharm_shape_pairs(n_pairs: int, atoms=(5, 60), pharms=(2, 20), seed: int = 2):
+    """Uneven atom + pharmacophore pairs: dict of ref/fit atoms, types, anchors, vectors."""
+    rng = np.random.default_rng(seed)
+    mols = random_walks(rng, rng.integers(atoms[0], atoms[1] + 1, size=2 * n_pairs))
+    nph = rng.integers(pharms[0], pharms[1] + 1, size=2 * n_pairs)
+    ph = [pharmacophores(rng, m, int(k)) for m, k in zip(mols, nph)]
+    out = {}
+    for side, sl in (("ref", slice(0, n_pairs)), ("fit", slice(n_pairs, 2 * n_pairs))):
+        out[f"{side}_atoms"] = mols[sl]
+        out[f"{side}_types"] = [p[0] for p in ph[sl]]
+        out[f"{side}_anchors"] = [p[1] for p in ph[sl]]
+        out[f"{side}_vectors"] = [p[2] for p in ph[sl]]
+    return out

@@ -1,0 +1,227 @@
+/*
+ * fss.h -- C ABI of libfss_hip.so, the MI355X (gfx950) kernels for ShEPhERD Gaussian-overlap
+ * scoring and multi-restart SE(3) alignment.
+ *
+ * Conventions (every entry point):
+ *   - all array pointers are caller-owned DEVICE memory, densely packed, naturally aligned
+ *     (float4 rows are 16-byte aligned);
+ *   - nothing is allocated inside any call; every call is asynchronous on `stream`
+ *     (a hipStream_t passed as void*; NULL = the legacy default stream);
+ *   - the return value is 0 on success or a nonzero FSS_E* / hipError_t code; the text of the
+ *     last error on the calling thread is available from fss_last_error();
+ *   - calls on distinct streams are thread-safe.
+ *
+ * Point layout.  A "cloud" is a run of float4 rows (x, y, z, w) inside one packed array.
+ *   - shape channel (atoms or surface points): w = the point's ESP value (0 when unused);
+ *   - pharmacophore channel: anchors (x, y, z, 0) and direction vectors (x, y, z, 0) in two
+ *     parallel float4 arrays plus one int32 type index (order of P_TYPES) per row.
+ * A pair names its ref and fit clouds by (start, count) rows, so several pairs may share one
+ * ref cloud (a screening query) without copies.
+ *
+ * Pose layout.  Pose k of pair p is row p*B + k (B = restarts per pair) of
+ *   q (n_pairs*B, 4) float32 -- raw quaternion (w, x, y, z), NOT required to be unit;
+ *   t (n_pairs*B, 3) float32 -- translation.  The fit cloud moves: b' = R(q/|q|) b + t.
+ */
+#ifndef FSS_H
+#define FSS_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Objective modes. */
+#define FSS_MODE_SHAPE        0  /* Gaussian shape Tanimoto (vol / surf)                    */
+#define FSS_MODE_SHAPE_ESP    1  /* ESP-weighted shape Tanimoto (surf_esp / vol_esp)        */
+#define FSS_MODE_PHARM        2  /* typed, directional pharmacophore Tanimoto / Tversky     */
+#define FSS_MODE_SHAPE_PHARM  3  /* (1-w)*shape Tanimoto + w*pharm similarity (vol_color)   */
+
+/* Pharmacophore similarity. */
+#define FSS_SIM_TANIMOTO 0
+#define FSS_SIM_TVERSKY  1
+
+/* Error codes (beside hipError_t values). */
+#define FSS_OK           0
+#define FSS_E_INVALID    9001  /* bad argument (null pointer, size, mode)                   */
+#define FSS_E_TOO_LARGE  9002  /* clouds do not fit the kernel's LDS staging               */
+#define FSS_E_NO_DEVICE  9003
+
+#define FSS_N_PTYPES 9         /* Acceptor Donor Aromatic Hydrophobe Halogen Cation Anion ZnBinder Dummy */
+
+typedef struct FssPairBatch {
+  int32_t n_pairs;
+  /* shape channel */
+  const float*   ref_pts;    /* (rows, 4) */
+  const float*   fit_pts;    /* (rows, 4) */
+  const int32_t* ref_rng;    /* (n_pairs, 2): first row, row count */
+  const int32_t* fit_rng;    /* (n_pairs, 2) */
+  /* pharmacophore channel (may be NULL unless the mode uses it) */
+  const float*   ref_anc;    /* (rows, 4) */
+  const float*   ref_vec;    /* (rows, 4) */
+  const int32_t* ref_typ;    /* (rows,)   */
+  const int32_t* ref_prng;   /* (n_pairs, 2) */
+  const float*   fit_anc;
+  const float*   fit_vec;
+  const int32_t* fit_typ;
+  const int32_t* fit_prng;
+  /* host-side maxima over the pairs (size the LDS staging); must be >= every count */
+  int32_t max_ref, max_fit, max_ref_ph, max_fit_ph;
+} FssPairBatch;
+
+/* Scoring constants shared by every entry point. */
+typedef struct FssScoreParams {
+  int32_t mode;              /* FSS_MODE_*                                                   */
+  float   alpha;             /* shape Gaussian width (0.81 volumetric)                       */
+  float   lam;               /* ESP lambda, already scaled (surface: 0.3*LAM_SCALING)        */
+  float   color_weight;      /* w of FSS_MODE_SHAPE_PHARM                                    */
+  int32_t similarity;        /* FSS_SIM_* for the pharmacophore channel                     */
+  float   sigma;             /* Tversky sigma                                                */
+  float   ph_alpha[FSS_N_PTYPES];   /* per-type pharmacophore alpha                          */
+  float   ph_k[FSS_N_PTYPES];       /* per-type (pi/(2 alpha))^1.5                           */
+  int32_t ph_cat[FSS_N_PTYPES];     /* 0 isotropic, 1 directional, 2 aromatic, 3 ignored     */
+} FssScoreParams;
+
+typedef struct FssAlignParams {
+  int32_t n_restarts;        /* B, poses per pair                                            */
+  int32_t max_steps;
+  float   lr, beta1, beta2, eps;    /* torch.optim.Adam semantics (bias-corrected)           */
+  int32_t early_stop;        /* 1: stop a pair once |dL| <= es_tol held > es_patience steps  */
+  float   es_tol;
+  int32_t es_patience;
+  int32_t lanes_per_pose;    /* 1..64, power of two: lanes sharing one pose's N x M sum      */
+  int32_t restarts_per_wg;   /* poses per workgroup; >= B when early_stop is set             */
+  const int32_t* pair_order; /* device (n_pairs,) launch order of the pairs (costliest first),
+                                or NULL for 0..n_pairs-1; results are always indexed by pair  */
+  int32_t skip_finish;       /* 1: run the optimiser only; call fss_align_finish afterwards    */
+} FssAlignParams;
+
+typedef struct FssAlignOut {
+  float*   q;                /* (n_pairs*B, 4) in: seeds, out: final raw quaternions         */
+  float*   t;                /* (n_pairs*B, 3) in: seeds, out: final translations            */
+  double*  pose_score;       /* (n_pairs*B) similarity of every final pose (FP64 sums)       */
+  double*  best_score;       /* (n_pairs)                                                    */
+  int32_t* best_idx;         /* (n_pairs) restart index of the best final pose               */
+  float*   best_tf;          /* (n_pairs, 4, 4) SE(3) matrix of the best pose, row-major      */
+  int32_t* n_steps;          /* (n_pairs) optimiser steps executed                           */
+} FssAlignOut;
+
+/* Library identity and errors. */
+const char* fss_version(void);
+const char* fss_last_error(void);
+int fss_device_count(int* count);
+
+/*
+ * Self-overlaps VAA and VBB of every pair at the identity pose (value only).
+ * out: (n_pairs, 4) float64 = [VAA_shape, VBB_shape, VAA_pharm, VBB_pharm], evaluated with FP64
+ * arithmetic; channels the mode does not use are written as 0.
+ * Replaces: accel/kernels/shape_triton.py:436-448 (_batch_self_overlap),
+ *           accel/kernels/esp_triton.py:393-425 (_batch_self_overlap_esp),
+ *           score/analytical_gradients/_torch.py:364-389, 519-541, 569-599.
+ */
+int fss_self_overlap(const FssPairBatch* batch, const FssScoreParams* sp, double* out,
+                     void* stream);
+
+/*
+ * Overlap value and analytic gradient of every pose (one workgroup per pose).
+ * pose_pair: (n_poses,) int32 pair index of each pose; q (n_poses,4), t (n_poses,3).
+ * Per pose and per channel c (0 shape, 1 pharm):
+ *   V  (n_poses, 2)      overlap O_AB
+ *   gR (n_poses, 2, 9)   dO/dR, row-major (NULL when need_grad == 0)
+ *   gT (n_poses, 2, 3)   dO/dt
+ *   gQ (n_poses, 2, 4)   dO/dq_hat through the unit-quaternion Jacobians
+ * Replaces: accel/kernels/dispatch.py:96-112 table entries overlap_score_grad_se3_batch
+ *   (shape_triton.py:191-249), overlap_score_grad_esp_se3_batch (esp_triton.py:175-246),
+ *   pharm_grad_dq_se3_batch (pharm_triton.py:403-430), pharm_score_grad_se3_batch
+ *   (pharm_triton.py:433-467); arithmetic of ag/_torch.py:223-361 and :441-516.
+ */
+int fss_overlap_grad(const FssPairBatch* batch, const FssScoreParams* sp, int32_t n_poses,
+                     const int32_t* pose_pair, const float* q, const float* t, int32_t need_grad,
+                     float* V, float* gR, float* gT, float* gQ, void* stream);
+
+/*
+ * Loss and SE(3) gradient of every pose for one optimiser step, with the Tanimoto / Tversky
+ * chain rule, the unit-quaternion projection and the 1/B scaling of the reference's batched
+ * objective.  loss (n_poses,) = 1 - S;  grad (n_poses, 7) = [dL/dq_raw, dL/dt] / B.
+ * self_ov: (n_pairs, 4) from fss_self_overlap.
+ * Replaces: ag/_torch.py:602-693 (compute_analytical_grad_se3_esp), :696-772
+ *   (compute_analytical_grad_se3_shape), :939-1017 (compute_analytical_grad_se3).
+ */
+int fss_pose_loss_grad(const FssPairBatch* batch, const FssScoreParams* sp, int32_t n_poses,
+                       const int32_t* pose_pair, int32_t n_restarts, const double* self_ov,
+                       const float* q, const float* t, float* loss, float* grad, void* stream);
+
+/*
+ * One torch.optim.Adam step (bias-corrected, eps outside the square root) over n rows of
+ * 7 parameters [q_raw, t]; m, v are the moment buffers and `step` the 1-based step count.
+ * pair_active (n / n_restarts,) int32 or NULL: rows of pairs whose flag is 0 are left untouched
+ * (a pair that stopped early keeps its pose).
+ * Replaces: the torch.optim.Adam call of alignment/_torch_analytical.py:108,281,418
+ *   (the reference's own accel Adam is accel/kernels/shape_triton.py:253-406).
+ */
+int fss_adam_step(int32_t n, float* q, float* t, const float* grad, float* m, float* v,
+                  int32_t step, float lr, float beta1, float beta2, float eps,
+                  const int32_t* pair_active, int32_t n_restarts, void* stream);
+
+/*
+ * Early-stop bookkeeping of a stepped optimiser, on device (no host synchronisation):
+ * L = mean of the pair's n_restarts losses; counter = |L - last| > tol ? 0 : counter + 1;
+ * last = L; n_steps = step; once counter > patience the pair's active flag drops to 0.
+ * last (n_pairs) float64 starts at 1.0; counter, n_steps start at 0; active starts at 1.
+ * Replaces: the per-step loss.item() test of alignment/_torch_analytical.py:132-139.
+ */
+int fss_early_stop_update(int32_t n_pairs, int32_t n_restarts, const float* loss, double* last,
+                          int32_t* counter, int32_t* active, int32_t* n_steps, int32_t step,
+                          float tol, int32_t patience, void* stream);
+
+/*
+ * FP64 similarity of every pose (one wave per pose): score (n_poses,) float64.
+ * pose_pair (n_poses,) or NULL for pair = pose / n_restarts.
+ * Replaces: the final re-scoring of alignment/_torch_analytical.py:141-158, 445-462, 308-322
+ *   (get_SE3_transform + apply_SE3_transform + get_overlap / get_overlap_esp / get_overlap_pharm).
+ */
+int fss_pose_score(const FssPairBatch* batch, const FssScoreParams* sp, int32_t n_poses,
+                   const int32_t* pose_pair, int32_t n_restarts, const double* self_ov,
+                   const float* q, const float* t, double* score, void* stream);
+
+/*
+ * Persistent multi-restart alignment: seeds in q/t, every optimiser step, per-pair early stop,
+ * final re-scoring of every pose and the best-restart selection, in one launch (plus a small
+ * finishing launch), one workgroup per (pair, restart-block).
+ * Replaces: alignment/_torch_analytical.py:23-478 (the oracle optimisers) and the accel
+ *   drivers accel/drivers/shape.py:155-313, esp.py:139-367, pharm.py:123-519.
+ */
+int fss_align(const FssPairBatch* batch, const FssScoreParams* sp, const FssAlignParams* ap,
+              const double* self_ov, FssAlignOut* out, void* stream);
+
+/*
+ * The finishing half of fss_align (FP64 re-scoring of every final pose, best restart, SE(3)
+ * matrix) for a call made with skip_finish = 1; reads out->q, out->t.
+ */
+int fss_align_finish(const FssPairBatch* batch, const FssScoreParams* sp, int32_t n_restarts,
+                     const double* self_ov, FssAlignOut* out, void* stream);
+
+/*
+ * Differentiable similarity of fixed clouds (score-as-loss).  fit rows are the moved points.
+ *   fwd: S (n_pairs,) and, for reuse by bwd, ov (n_pairs, 4) = [VAB, VAA, VBB, 0].
+ *   bwd: dS/dfit (fit rows, 4 floats: x, y, z, 0) scaled by gS (n_pairs,), including the
+ *        dVBB/dfit term; dS/dref likewise when d_ref != NULL.
+ * Modes FSS_MODE_SHAPE and FSS_MODE_SHAPE_ESP.  The *_f64 variants take double clouds.
+ * No reference counterpart for bwd w.r.t. coordinates; forward follows
+ *   score/gaussian_overlap.py:16-106 and score/electrostatic_scoring.py:17-125.
+ */
+int fss_score_fwd(const FssPairBatch* batch, const FssScoreParams* sp, float* S, float* ov,
+                  void* stream);
+int fss_score_bwd(const FssPairBatch* batch, const FssScoreParams* sp, const float* ov,
+                  const float* gS, float* d_fit, float* d_ref, void* stream);
+int fss_score_fwd_f64(const FssPairBatch* batch, const FssScoreParams* sp, const double* ref_pts,
+                      const double* fit_pts, double* S, double* ov, void* stream);
+int fss_score_bwd_f64(const FssPairBatch* batch, const FssScoreParams* sp, const double* ref_pts,
+                      const double* fit_pts, const double* ov, const double* gS, double* d_fit,
+                      double* d_ref, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* FSS_H */

@@ -1,0 +1,293 @@
+// CPU baseline of the ShEPhERD alignment hot path: C++17 / OpenMP.
+//
+// TEST INFRASTRUCTURE ONLY (the oracle package): timed beside the GPU by bench.py's cpu_baseline
+// leg and checked against the FP64 oracle by tests; never part of the product path.
+//
+// It restates the reference's numba CPU backend structure -- accel/kernels/cpu.py:38-87
+// (_overlap_grad_kernel, fp64 accumulation, fastmath), :158-200 (_overlap_grad_esp_kernel, one
+// fused exp), :500-582 (_pharm_grad_dq_kernel) -- driving the same optimiser semantics as the HIP
+// path (alignment/_torch_analytical.py:23-478: torch-style Adam on raw (q, t), gradient / B,
+// per-pair early stop on the restart-mean loss, FP64 re-scoring of the final poses, argmax).
+// Parallelism: one OpenMP task per pair (the numba backend's prange over poses, coarsened so a
+// thread keeps a pair's restarts for all steps).
+#include <math.h>
+#include <omp.h>
+#include <stdint.h>
+#include <string.h>
+
+#include <vector>
+
+#include "../include/fss.h"
+
+namespace {
+
+struct Cloud { const float* p; int n; };
+struct PhCloud { const float* anc; const float* vec; const int32_t* typ; int n; };
+
+void rotation(const float* q, double qh[4], double& qn, double R[9]) {
+  const double w = q[0], x = q[1], y = q[2], z = q[3];
+  qn = sqrt(w * w + x * x + y * y + z * z);
+  qh[0] = w / qn; qh[1] = x / qn; qh[2] = y / qn; qh[3] = z / qn;
+  const double a = qh[0], b = qh[1], c = qh[2], d = qh[3];
+  R[0] = 1 - 2 * (c * c + d * d); R[1] = 2 * (b * c - d * a);     R[2] = 2 * (b * d + c * a);
+  R[3] = 2 * (b * c + d * a);     R[4] = 1 - 2 * (b * b + d * d); R[5] = 2 * (c * d - b * a);
+  R[6] = 2 * (b * d - c * a);     R[7] = 2 * (c * d + b * a);     R[8] = 1 - 2 * (b * b + c * c);
+}
+
+void project(const double G[9], const double qh[4], double g[4]) {
+  const double w = qh[0], x = qh[1], y = qh[2], z = qh[3];
+  g[0] = -2 * z * G[1] + 2 * y * G[2] + 2 * z * G[3] - 2 * x * G[5] - 2 * y * G[6] + 2 * x * G[7];
+  g[1] = 2 * y * G[1] + 2 * z * G[2] + 2 * y * G[3] - 4 * x * G[4] - 2 * w * G[5] + 2 * z * G[6] + 2 * w * G[7] - 4 * x * G[8];
+  g[2] = -4 * y * G[0] + 2 * x * G[1] + 2 * w * G[2] + 2 * x * G[3] + 2 * z * G[5] - 2 * w * G[6] + 2 * z * G[7] - 4 * y * G[8];
+  g[3] = -4 * z * G[0] - 2 * w * G[1] + 2 * x * G[2] + 2 * w * G[3] - 4 * z * G[4] + 2 * y * G[5] + 2 * x * G[6] + 2 * y * G[7];
+}
+
+// O, dO/dt, dO/dR of the shape channel (float terms, double accumulation: numba's kernel).
+void shape_grad(const Cloud& A, const Cloud& Bc, const float R[9], const float t[3], float alpha,
+                bool esp, float inv_lam, double& O, double gt[3], double G[9]) {
+  O = 0; for (int i = 0; i < 3; ++i) gt[i] = 0; for (int i = 0; i < 9; ++i) G[i] = 0;
+  const float ha = 0.5f * alpha;
+  for (int b = 0; b < Bc.n; ++b) {
+    const float* p = Bc.p + 4 * b;
+    const float bx = R[0] * p[0] + R[1] * p[1] + R[2] * p[2] + t[0];
+    const float by = R[3] * p[0] + R[4] * p[1] + R[5] * p[2] + t[1];
+    const float bz = R[6] * p[0] + R[7] * p[1] + R[8] * p[2] + t[2];
+    double v = 0, fx = 0, fy = 0, fz = 0;
+#pragma omp simd reduction(+ : v, fx, fy, fz)
+    for (int a = 0; a < A.n; ++a) {
+      const float* r = A.p + 4 * a;
+      const float dx = r[0] - bx, dy = r[1] - by, dz = r[2] - bz;
+      float arg = -ha * (dx * dx + dy * dy + dz * dz);
+      if (esp) { const float dc = r[3] - p[3]; arg -= dc * dc * inv_lam; }
+      const float e = expf(arg);
+      v += e; fx += e * dx; fy += e * dy; fz += e * dz;
+    }
+    O += v; gt[0] += fx; gt[1] += fy; gt[2] += fz;
+    G[0] += fx * p[0]; G[1] += fx * p[1]; G[2] += fx * p[2];
+    G[3] += fy * p[0]; G[4] += fy * p[1]; G[5] += fy * p[2];
+    G[6] += fz * p[0]; G[7] += fz * p[1]; G[8] += fz * p[2];
+  }
+  const double K = pow(M_PI / (2.0 * alpha), 1.5);
+  O *= K;
+  for (int i = 0; i < 3; ++i) gt[i] *= K * alpha;
+  for (int i = 0; i < 9; ++i) G[i] *= K * alpha;
+}
+
+void pharm_grad(const PhCloud& A, const PhCloud& Bc, const float R[9], const float t[3],
+                const FssScoreParams& sp, double& O, double gt[3], double G[9]) {
+  O = 0; for (int i = 0; i < 3; ++i) gt[i] = 0; for (int i = 0; i < 9; ++i) G[i] = 0;
+  for (int b = 0; b < Bc.n; ++b) {
+    const int tb = Bc.typ[b];
+    if (tb < 0 || tb >= FSS_N_PTYPES || sp.ph_cat[tb] == 3) continue;
+    const int cat = sp.ph_cat[tb];
+    const double al = sp.ph_alpha[tb], K = sp.ph_k[tb];
+    const float* p = Bc.anc + 4 * b;
+    const float* u0 = Bc.vec + 4 * b;
+    double un = sqrt((double)u0[0] * u0[0] + (double)u0[1] * u0[1] + (double)u0[2] * u0[2]);
+    un = un > 1e-12 ? un : 1e-12;
+    const double ux0 = u0[0] / un, uy0 = u0[1] / un, uz0 = u0[2] / un;
+    const double bx = R[0] * p[0] + R[1] * p[1] + R[2] * p[2] + t[0];
+    const double by = R[3] * p[0] + R[4] * p[1] + R[5] * p[2] + t[1];
+    const double bz = R[6] * p[0] + R[7] * p[1] + R[8] * p[2] + t[2];
+    const double ux = R[0] * ux0 + R[1] * uy0 + R[2] * uz0;
+    const double uy = R[3] * ux0 + R[4] * uy0 + R[5] * uz0;
+    const double uz = R[6] * ux0 + R[7] * uy0 + R[8] * uz0;
+    double fx = 0, fy = 0, fz = 0, hx = 0, hy = 0, hz = 0;
+    for (int a = 0; a < A.n; ++a) {
+      if (A.typ[a] != tb) continue;
+      const float* r = A.anc + 4 * a;
+      const float* rv = A.vec + 4 * a;
+      double rn = sqrt((double)rv[0] * rv[0] + (double)rv[1] * rv[1] + (double)rv[2] * rv[2]);
+      rn = rn > 1e-12 ? rn : 1e-12;
+      const double vx = rv[0] / rn, vy = rv[1] / rn, vz = rv[2] / rn;
+      const double dx = r[0] - bx, dy = r[1] - by, dz = r[2] - bz;
+      const double e = exp(-0.5 * al * (dx * dx + dy * dy + dz * dz));
+      const double D = ux * vx + uy * vy + uz * vz;
+      double w = 1, c = 0;
+      if (cat == 1) { w = (fmin(fmax(D, 0.0), 1.0) + 2) / 3; c = (D > 0 && D < 1) ? 1 : 0; }
+      else if (cat == 2) { w = (fabs(D) + 2) / 3; c = D > 0 ? 1 : (D < 0 ? -1 : 0); }
+      O += K * w * e;
+      const double fw = al * K * w * e;
+      fx += fw * dx; fy += fw * dy; fz += fw * dz;
+      const double hc = K / 3.0 * e * c;
+      hx += hc * vx; hy += hc * vy; hz += hc * vz;
+    }
+    gt[0] += fx; gt[1] += fy; gt[2] += fz;
+    G[0] += fx * p[0] + hx * ux0; G[1] += fx * p[1] + hx * uy0; G[2] += fx * p[2] + hx * uz0;
+    G[3] += fy * p[0] + hy * ux0; G[4] += fy * p[1] + hy * uy0; G[5] += fy * p[2] + hy * uz0;
+    G[6] += fz * p[0] + hz * ux0; G[7] += fz * p[1] + hz * uy0; G[8] += fz * p[2] + hz * uz0;
+  }
+}
+
+double shape_value64(const Cloud& A, const Cloud& Bc, const double R[9], const double t[3],
+                     double alpha, bool esp, double lam) {
+  double v = 0;
+  for (int b = 0; b < Bc.n; ++b) {
+    const float* p = Bc.p + 4 * b;
+    const double bx = R[0] * p[0] + R[1] * p[1] + R[2] * p[2] + t[0];
+    const double by = R[3] * p[0] + R[4] * p[1] + R[5] * p[2] + t[1];
+    const double bz = R[6] * p[0] + R[7] * p[1] + R[8] * p[2] + t[2];
+    for (int a = 0; a < A.n; ++a) {
+      const float* r = A.p + 4 * a;
+      const double dx = r[0] - bx, dy = r[1] - by, dz = r[2] - bz;
+      double arg = -0.5 * alpha * (dx * dx + dy * dy + dz * dz);
+      if (esp) { const double dc = (double)r[3] - (double)p[3]; arg -= dc * dc / lam; }
+      v += exp(arg);
+    }
+  }
+  return v * pow(M_PI / (2.0 * alpha), 1.5);
+}
+
+double pharm_value64(const PhCloud& A, const PhCloud& Bc, const double R[9], const double t[3],
+                     const FssScoreParams& sp) {
+  float Rf[9], tf[3];
+  for (int i = 0; i < 9; ++i) Rf[i] = (float)R[i];
+  for (int i = 0; i < 3; ++i) tf[i] = (float)t[i];
+  double O, gt[3], G[9];
+  pharm_grad(A, Bc, Rf, tf, sp, O, gt, G);
+  return O;
+}
+
+struct Sim { double S, ws, wp; };
+
+Sim similarity(const FssScoreParams& sp, double Os, double Op, const double so[4], bool sh, bool ph) {
+  Sim r{0, 0, 0};
+  const double cs = sp.mode == FSS_MODE_SHAPE_PHARM ? 1.0 - sp.color_weight : 1.0;
+  const double cp = sp.mode == FSS_MODE_SHAPE_PHARM ? sp.color_weight : 1.0;
+  if (sh) {
+    const double U = so[0] + so[1], D = U - Os;
+    r.S += cs * Os / D; r.ws = cs * U / (D * D);
+  }
+  if (ph) {
+    if (sp.similarity == FSS_SIM_TANIMOTO) {
+      const double U = so[2] + so[3], D = U - Op;
+      r.S += cp * Op / D; r.wp = cp * U / (D * D);
+    } else {
+      const double D = sp.sigma * so[2] + (1.0 - sp.sigma) * so[3];
+      r.S += cp * fmin(Op / D, 1.0); r.wp = Op < D ? cp / D : 0.0;
+    }
+  }
+  return r;
+}
+
+}  // namespace
+
+extern "C" {
+
+// Self-overlaps (n_pairs, 4) in FP64.
+int fss_cpu_self_overlap(const FssPairBatch* pb, const FssScoreParams* sp, double* out, int n_threads) {
+  const bool sh = sp->mode != FSS_MODE_PHARM, esp = sp->mode == FSS_MODE_SHAPE_ESP;
+  const bool ph = sp->mode == FSS_MODE_PHARM || sp->mode == FSS_MODE_SHAPE_PHARM;
+  const double I[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1}, z[3] = {0, 0, 0};
+#pragma omp parallel for schedule(dynamic) num_threads(n_threads)
+  for (int p = 0; p < pb->n_pairs; ++p) {
+    for (int side = 0; side < 2; ++side) {
+      double vs = 0, vp = 0;
+      if (sh) {
+        const int32_t* rg = side ? pb->fit_rng : pb->ref_rng;
+        Cloud c{(side ? pb->fit_pts : pb->ref_pts) + 4 * (long)rg[2 * p], rg[2 * p + 1]};
+        vs = shape_value64(c, c, I, z, sp->alpha, esp, sp->lam);
+      }
+      if (ph) {
+        const int32_t* rg = side ? pb->fit_prng : pb->ref_prng;
+        const long o = rg[2 * p];
+        PhCloud c{(side ? pb->fit_anc : pb->ref_anc) + 4 * o, (side ? pb->fit_vec : pb->ref_vec) + 4 * o,
+                  (side ? pb->fit_typ : pb->ref_typ) + o, rg[2 * p + 1]};
+        vp = pharm_value64(c, c, I, z, *sp);
+      }
+      out[4 * p + side] = vs;
+      out[4 * p + 2 + side] = vp;
+    }
+  }
+  return 0;
+}
+
+// Full multi-restart alignment of every pair (host pointers; same layout as fss_align).
+int fss_cpu_align(const FssPairBatch* pb, const FssScoreParams* sp, const FssAlignParams* ap,
+                  const double* self_ov, float* q, float* t, double* pose_score, double* best_score,
+                  int32_t* best_idx, int32_t* n_steps, int n_threads) {
+  const bool sh = sp->mode != FSS_MODE_PHARM, esp = sp->mode == FSS_MODE_SHAPE_ESP;
+  const bool ph = sp->mode == FSS_MODE_PHARM || sp->mode == FSS_MODE_SHAPE_PHARM;
+  const int B = ap->n_restarts;
+  const float inv_lam = esp ? 1.0f / sp->lam : 0.f;
+#pragma omp parallel for schedule(dynamic) num_threads(n_threads)
+  for (int p = 0; p < pb->n_pairs; ++p) {
+    Cloud A{}, Bc{};
+    PhCloud PA{}, PB{};
+    if (sh) {
+      A = Cloud{pb->ref_pts + 4 * (long)pb->ref_rng[2 * p], pb->ref_rng[2 * p + 1]};
+      Bc = Cloud{pb->fit_pts + 4 * (long)pb->fit_rng[2 * p], pb->fit_rng[2 * p + 1]};
+    }
+    if (ph) {
+      const long ro = pb->ref_prng[2 * p], fo = pb->fit_prng[2 * p];
+      PA = PhCloud{pb->ref_anc + 4 * ro, pb->ref_vec + 4 * ro, pb->ref_typ + ro, pb->ref_prng[2 * p + 1]};
+      PB = PhCloud{pb->fit_anc + 4 * fo, pb->fit_vec + 4 * fo, pb->fit_typ + fo, pb->fit_prng[2 * p + 1]};
+    }
+    const double* so = self_ov + 4 * p;
+    std::vector<float> m(7 * B, 0.f), v(7 * B, 0.f);
+    std::vector<double> loss(B);
+    float* qp = q + 4 * (long)p * B;
+    float* tp = t + 3 * (long)p * B;
+    double b1p = 1, b2p = 1, last = 1.0;
+    int counter = 0, done = ap->max_steps;
+    for (int step = 1; step <= ap->max_steps; ++step) {
+      b1p *= ap->beta1; b2p *= ap->beta2;
+      const float step_size = (float)(ap->lr / (1.0 - b1p));
+      const float bc2s = (float)sqrt(1.0 - b2p);
+      for (int r = 0; r < B; ++r) {
+        double qh[4], qn, Rd[9];
+        rotation(qp + 4 * r, qh, qn, Rd);
+        float R[9];
+        for (int i = 0; i < 9; ++i) R[i] = (float)Rd[i];
+        double Os = 0, Op = 0, gts[3] = {0, 0, 0}, Gs[9] = {0}, gtp[3] = {0, 0, 0}, Gp[9] = {0};
+        if (sh) shape_grad(A, Bc, R, tp + 3 * r, sp->alpha, esp, inv_lam, Os, gts, Gs);
+        if (ph) pharm_grad(PA, PB, R, tp + 3 * r, *sp, Op, gtp, Gp);
+        const Sim s = similarity(*sp, Os, Op, so, sh, ph);
+        loss[r] = 1.0 - s.S;
+        double lR[9], lT[3];
+        for (int i = 0; i < 9; ++i) lR[i] = -(s.ws * Gs[i] + s.wp * Gp[i]);
+        for (int i = 0; i < 3; ++i) lT[i] = -(s.ws * gts[i] + s.wp * gtp[i]);
+        double gq[4];
+        project(lR, qh, gq);
+        const double dot = qh[0] * gq[0] + qh[1] * gq[1] + qh[2] * gq[2] + qh[3] * gq[3];
+        float g[7];
+        for (int k = 0; k < 4; ++k) g[k] = (float)((gq[k] - qh[k] * dot) / qn / B);
+        for (int k = 0; k < 3; ++k) g[4 + k] = (float)(lT[k] / B);
+        float* mm = &m[7 * r];
+        float* vv = &v[7 * r];
+        for (int k = 0; k < 7; ++k) {
+          float* par = k < 4 ? qp + 4 * r + k : tp + 3 * r + (k - 4);
+          mm[k] = mm[k] + (1.f - ap->beta1) * (g[k] - mm[k]);
+          vv[k] = vv[k] * ap->beta2 + (1.f - ap->beta2) * g[k] * g[k];
+          *par = *par - step_size * (mm[k] / (sqrtf(vv[k]) / bc2s + ap->eps));
+        }
+      }
+      if (ap->early_stop) {
+        double L = 0;
+        for (int r = 0; r < B; ++r) L += loss[r];
+        L /= B;
+        counter = fabs(L - last) > ap->es_tol ? 0 : counter + 1;
+        last = L;
+        if (counter > ap->es_patience) { done = step; break; }
+      }
+    }
+    int bi = 0;
+    double bs = -1e300;
+    for (int r = 0; r < B; ++r) {
+      double qh[4], qn, Rd[9];
+      rotation(qp + 4 * r, qh, qn, Rd);
+      const double td[3] = {tp[3 * r], tp[3 * r + 1], tp[3 * r + 2]};
+      const double vs = sh ? shape_value64(A, Bc, Rd, td, sp->alpha, esp, sp->lam) : 0.0;
+      const double vp = ph ? pharm_value64(PA, PB, Rd, td, *sp) : 0.0;
+      const double S = similarity(*sp, vs, vp, so, sh, ph).S;
+      pose_score[(long)p * B + r] = S;
+      if (S > bs) { bs = S; bi = r; }
+    }
+    best_score[p] = bs;
+    best_idx[p] = bi;
+    n_steps[p] = done;
+  }
+  return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,81 @@
+"""C ABI checks that need no GPU: the library loads, exports every declared symbol, the ctypes
+structures match the C layout, and argument validation rejects bad calls before any launch."""
+import ctypes
+import os
+import re
+import subprocess
+import tempfile
+
+import pytest
+
+from fast_shepherd_score_amd import _lib
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "fss.h")
+
+
+def declared_functions():
+    text = open(HEADER).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(fss_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_library_exports_every_declared_symbol():
+    lib = _lib.load()
+    names = declared_functions()
+    assert len(names) >= 10
+    for n in names:
+        assert hasattr(lib, n), n
+    assert set(names) == set(_lib.SIGNATURES), set(names) ^ set(_lib.SIGNATURES)
+    assert lib.fss_version().decode().startswith("fss_hip")
+
+
+def test_struct_layout_matches_c():
+    src = r'''
+#include <stdio.h>
+#include <stddef.h>
+#include "fss.h"
+int main(void) {
+  printf("%zu %zu %zu %zu\n", sizeof(FssPairBatch), sizeof(FssScoreParams), sizeof(FssAlignParams), sizeof(FssAlignOut));
+  printf("%zu %zu %zu %zu\n", offsetof(FssPairBatch, max_ref), offsetof(FssScoreParams, ph_cat),
+         offsetof(FssAlignParams, pair_order), offsetof(FssAlignOut, n_steps));
+  return 0;
+}
+'''
+    with tempfile.TemporaryDirectory() as d:
+        c = os.path.join(d, "l.c")
+        exe = os.path.join(d, "l")
+        open(c, "w").write(src)
+        subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), c, "-o", exe], check=True)
+        out = subprocess.run([exe], capture_output=True, text=True, check=True).stdout.split()
+    sizes = [int(x) for x in out]
+    assert sizes[:4] == [ctypes.sizeof(_lib.FssPairBatch), ctypes.sizeof(_lib.FssScoreParams),
+                         ctypes.sizeof(_lib.FssAlignParams), ctypes.sizeof(_lib.FssAlignOut)]
+    assert sizes[4:] == [_lib.FssPairBatch.max_ref.offset, _lib.FssScoreParams.ph_cat.offset,
+                         _lib.FssAlignParams.pair_order.offset, _lib.FssAlignOut.n_steps.offset]
+
+
+def test_validation_rejects_bad_arguments_without_launching():
+    """These calls fail in host-side validation, before any device work."""
+    lib = _lib.load()
+    sp = _lib.FssScoreParams()
+    sp.mode = 7
+    pb = _lib.FssPairBatch()
+    rc = lib.fss_self_overlap(_lib.byref(pb), _lib.byref(sp), None, None)
+    assert rc == _lib.E_INVALID and b"unknown mode" in lib.fss_last_error()
+    sp.mode, sp.alpha = _lib.MODE_SHAPE, 0.81
+    rc = lib.fss_self_overlap(_lib.byref(pb), _lib.byref(sp), None, None)
+    assert rc == _lib.E_INVALID and b"NULL" in lib.fss_last_error()
+    ap = _lib.FssAlignParams()
+    ap.n_restarts, ap.lanes_per_pose, ap.restarts_per_wg, ap.early_stop = 50, 3, 50, 1
+    out = _lib.FssAlignOut()
+    pb.n_pairs = 1
+    pb.ref_pts = pb.fit_pts = pb.ref_rng = pb.fit_rng = 8
+    out.q = out.t = out.pose_score = out.best_score = out.best_idx = out.best_tf = out.n_steps = 8
+    rc = lib.fss_align(_lib.byref(pb), _lib.byref(sp), _lib.byref(ap), ctypes.c_void_p(8), _lib.byref(out), None)
+    assert rc == _lib.E_INVALID and b"power of two" in lib.fss_last_error()
+    ap.lanes_per_pose, ap.restarts_per_wg = 16, 10
+    rc = lib.fss_align(_lib.byref(pb), _lib.byref(sp), _lib.byref(ap), ctypes.c_void_p(8), _lib.byref(out), None)
+    assert rc == _lib.E_INVALID and b"early stop" in lib.fss_last_error()
+    with pytest.raises(_lib.FssError):
+        _lib.call("fss_adam_step", 1, 0, 0, 0, 0, 0, 1, 0.1, 0.9, 0.999, 1e-8, 0, 1, 0)

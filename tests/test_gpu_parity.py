@@ -1,0 +1,178 @@
+"""HIP kernels vs the FP64 oracle on identical seeded inputs (needs an MI355X).
+
+Tolerances (stated by the north star): per-pose scores within 1e-4 absolute; best-restart scores
+within 1e-3 under identical restart seeds.  Gradients of the FP32 kernels: 1e-4 relative to the
+largest component.
+"""
+import numpy as np
+import pytest
+import torch
+
+from fast_shepherd_score_amd import _lib
+from fast_shepherd_score_amd.align import (OptimSpec, align, align_plain, align_stepped, overlap_grad,
+                                           pose_loss_grad, pose_score, self_overlaps)
+from oracle import fss_oracle as orc
+from tests._cases import make_case, random_seeds, to_dev
+
+pytestmark = pytest.mark.gpu
+MODES = [_lib.MODE_SHAPE, _lib.MODE_SHAPE_ESP, _lib.MODE_PHARM, _lib.MODE_SHAPE_PHARM]
+DT = torch.float64
+
+
+def _se3(q, t):
+    return torch.cat([torch.as_tensor(q, dtype=DT), torch.as_tensor(t, dtype=DT)], dim=-1)
+
+
+@pytest.mark.parametrize("mode", MODES)
+def test_self_overlaps_fp64(dev, mode):
+    pb, spec, pairs = make_case(mode, 5, dev, seed=1)
+    so = self_overlaps(pb, spec).cpu().numpy()
+    for i, p in enumerate(pairs):
+        o = p.self_overlaps()
+        if p.has_shape:
+            np.testing.assert_allclose(so[i, :2], [float(o["VAA_s"]), float(o["VBB_s"])], rtol=1e-12)
+        if p.has_pharm:
+            np.testing.assert_allclose(so[i, 2:], [float(o["VAA_p"]), float(o["VBB_p"])], rtol=1e-6)
+
+
+@pytest.mark.parametrize("mode", MODES)
+def test_overlap_and_gradients_per_pose(dev, mode):
+    print(f"case overlap_grad mode={mode}", flush=True)
+    pb, spec, pairs = make_case(mode, 4, dev, seed=2)
+    B = 6
+    q, t = random_seeds(4, B, seed=3, t_scale=0.5)
+    pp = torch.arange(4, dtype=torch.int32).repeat_interleave(B)
+    V, gR, gT, gQ = overlap_grad(pb, spec, pp, to_dev(q.reshape(-1, 4), dev), to_dev(t.reshape(-1, 3), dev))
+    V, gR, gT = V.cpu().numpy(), gR.cpu().numpy(), gT.cpu().numpy()
+    for i, p in enumerate(pairs):
+        qn = torch.tensor(q[i], dtype=DT)
+        qn = qn / qn.norm(dim=1, keepdim=True)
+        R = orc.rot_from_unit_quat(qn)
+        ch = p.channel_overlaps(R, torch.tensor(t[i], dtype=DT))
+        for c, key in enumerate(("shape", "pharm")):
+            if key not in ch:
+                continue
+            O, GR, GT = (x.numpy() for x in ch[key])
+            sl = slice(i * B, (i + 1) * B)
+            np.testing.assert_allclose(V[sl, c], O, rtol=2e-5, atol=1e-6)
+            scale = max(np.abs(GR).max(), np.abs(GT).max(), 1e-6)
+            np.testing.assert_allclose(gR[sl, c], GR, atol=1e-4 * scale)
+            np.testing.assert_allclose(gT[sl, c], GT, atol=1e-4 * scale)
+
+
+@pytest.mark.parametrize("mode", MODES)
+def test_pose_loss_grad_and_score(dev, mode):
+    print(f"case pose_loss_grad mode={mode}", flush=True)
+    pb, spec, pairs = make_case(mode, 3, dev, seed=4)
+    B = 5
+    q, t = random_seeds(3, B, seed=5, t_scale=0.5)
+    q = q * 1.3  # raw (non-unit) quaternions, as the optimiser carries them
+    so = self_overlaps(pb, spec)
+    pp = torch.arange(3, dtype=torch.int32).repeat_interleave(B)
+    qd, td = to_dev(q.reshape(-1, 4), dev), to_dev(t.reshape(-1, 3), dev)
+    loss, grad = pose_loss_grad(pb, spec, pp, B, qd, td, so)
+    S = pose_score(pb, spec, pp, qd, td, so).cpu().numpy()
+    loss, grad = loss.cpu().numpy(), grad.cpu().numpy()
+    for i, p in enumerate(pairs):
+        se3 = _se3(q[i], t[i])
+        _, g, per = p.loss_grad_se3(se3, p.self_overlaps())
+        sl = slice(i * B, (i + 1) * B)
+        np.testing.assert_allclose(loss[sl], per.numpy(), atol=1e-4)
+        scale = max(np.abs(g.numpy()).max(), 1e-8)
+        np.testing.assert_allclose(grad[sl], g.numpy(), atol=2e-4 * scale)
+        np.testing.assert_allclose(S[sl], p.score_poses(se3).numpy(), atol=1e-9)
+
+
+@pytest.mark.parametrize("mode", MODES)
+@pytest.mark.parametrize("early_stop", [True, False])
+def test_persistent_align_matches_oracle(dev, mode, early_stop):
+    print(f"case align mode={mode} es={early_stop}", flush=True)
+    n, B, steps = 6, 10, 120
+    pb, spec, pairs = make_case(mode, n, dev, seed=6 + mode)
+    q, t = random_seeds(n, B, seed=7)
+    opt = OptimSpec(max_steps=steps, lr=0.1, early_stop=early_stop)
+    res = align(pb, spec, to_dev(q, dev), to_dev(t, dev), opt)
+    torch.cuda.synchronize()
+    score = res.score.cpu().numpy()
+    pose = res.pose_score.cpu().numpy()
+    nst = res.n_steps.cpu().numpy()
+    for i, p in enumerate(pairs):
+        o = orc.optimize(p, q[i], t[i], lr=0.1, max_steps=steps, early_stop=early_stop)
+        assert abs(score[i] - o["score"]) < 1e-3, (i, score[i], o["score"])
+        # the HIP pose scores are FP64 re-scorings of the HIP poses: consistent with its argmax
+        assert score[i] == pose[i].max()
+        if not early_stop:
+            assert nst[i] == steps
+
+
+def test_paths_agree(dev):
+    """Persistent, graph-captured stepped and plain per-pair paths: same optimiser."""
+    n, B = 4, 8
+    pb, spec, pairs = make_case(_lib.MODE_SHAPE, n, dev, seed=9)
+    q, t = random_seeds(n, B, seed=10)
+    opt = OptimSpec(max_steps=80, early_stop=True)
+    a = align(pb, spec, to_dev(q, dev), to_dev(t, dev), opt)
+    b = align_stepped(pb, spec, to_dev(q, dev), to_dev(t, dev), opt, use_graph=True)
+    c = align_plain(pb, spec, to_dev(q, dev), to_dev(t, dev), opt)
+    torch.cuda.synchronize()
+    for r in (b, c):
+        np.testing.assert_allclose(r.score.cpu().numpy(), a.score.cpu().numpy(), atol=1e-3)
+
+
+def test_alone_equals_in_mix_bitwise(dev):
+    """A pair's result does not depend on the other pairs of the launch."""
+    n, B = 8, 10
+    pb, spec, _ = make_case(_lib.MODE_SHAPE, n, dev, seed=11)
+    q, t = random_seeds(n, B, seed=12)
+    opt = OptimSpec(max_steps=60, early_stop=True)
+    full = align(pb, spec, to_dev(q, dev), to_dev(t, dev), opt)
+    from fast_shepherd_score_amd.packing import PairBatch
+    for i in (0, 5):
+        one = PairBatch(pb.ref, pb.fit, fit_idx=[i], ref_idx=[i])
+        r = align(one, spec, to_dev(q[i:i + 1], dev), to_dev(t[i:i + 1], dev), opt)
+        assert float(r.score[0]) == float(full.score[i])
+        assert torch.equal(r.q[0], full.q[i])
+
+
+def test_deterministic_repeat(dev):
+    pb, spec, _ = make_case(_lib.MODE_SHAPE_ESP, 4, dev, seed=13)
+    q, t = random_seeds(4, 8, seed=14)
+    opt = OptimSpec(max_steps=40)
+    a = align(pb, spec, to_dev(q, dev), to_dev(t, dev), opt)
+    b = align(pb, spec, to_dev(q, dev), to_dev(t, dev), opt)
+    assert torch.equal(a.pose_score, b.pose_score) and torch.equal(a.q, b.q)
+
+
+@pytest.mark.parametrize("lpp,rpw", [(1, 10), (4, 10), (16, 10), (64, 10), (8, 3), (32, 1)])
+def test_tilings_agree(dev, lpp, rpw):
+    """Every legal (lanes_per_pose, restarts_per_wg) computes the same optimisation."""
+    n, B = 3, 10
+    pb, spec, _ = make_case(_lib.MODE_SHAPE, n, dev, seed=15)
+    q, t = random_seeds(n, B, seed=16)
+    es = rpw >= B
+    opt = OptimSpec(max_steps=50, early_stop=es)
+    ref = align(pb, spec, to_dev(q, dev), to_dev(t, dev), opt, lanes_per_pose=16, restarts_per_wg=B)
+    r = align(pb, spec, to_dev(q, dev), to_dev(t, dev), opt, lanes_per_pose=lpp, restarts_per_wg=rpw)
+    np.testing.assert_allclose(r.pose_score.cpu().numpy(), ref.pose_score.cpu().numpy(), atol=1e-4)
+
+
+def test_edge_cases(dev):
+    from fast_shepherd_score_amd.align import ScoreSpec
+    from fast_shepherd_score_amd.packing import CloudSet, PairBatch
+    # single-atom fit, single restart, uneven sizes in one launch
+    ref = [np.zeros((1, 3), np.float32), np.random.default_rng(0).normal(size=(37, 3)).astype(np.float32)]
+    fit = [np.array([[0.5, 0.0, 0.0]], np.float32), np.random.default_rng(1).normal(size=(5, 3)).astype(np.float32)]
+    pb = PairBatch(CloudSet.from_arrays(ref, device=dev), CloudSet.from_arrays(fit, device=dev))
+    q = np.tile(np.array([1, 0, 0, 0], np.float32), (2, 1, 1))
+    t = np.zeros((2, 1, 3), np.float32)
+    r = align(pb, ScoreSpec(), to_dev(q, dev), to_dev(t, dev), OptimSpec(max_steps=30))
+    torch.cuda.synchronize()
+    for i in range(2):
+        o = orc.optimize(orc.Pair("shape", ref=ref[i], fit=fit[i]), q[i], t[i], max_steps=30)
+        assert abs(float(r.score[i]) - o["score"]) < 1e-3
+    # oversized clouds are rejected with a clear error
+    big = [np.zeros((20000, 3), np.float32)]
+    pbb = PairBatch(CloudSet.from_arrays(big, device=dev), CloudSet.from_arrays(big, device=dev))
+    so = torch.ones((1, 4), dtype=torch.float64, device=dev)
+    with pytest.raises(_lib.FssError, match="LDS"):
+        align(pbb, ScoreSpec(), to_dev(q[:1], dev), to_dev(t[:1], dev), OptimSpec(max_steps=1), self_ov=so)

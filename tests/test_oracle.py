@@ -1,0 +1,259 @@
+"""Pins of the FP64 oracle (CPU): golden vectors, closed forms, brute force, FD, autograd.
+
+Each check is chosen so a plausible slip (a dropped term, a wrong sign or index, a transposed
+operand) fails it; none of them re-types the oracle's own formula.
+"""
+import json
+import math
+import os
+
+import numpy as np
+import pytest
+import torch
+
+from oracle import fss_oracle as orc
+from fast_shepherd_score_amd import synth
+
+GOLD = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "reference_tables.json")))
+DT = torch.float64
+
+
+def brute_overlap(ref, fit, alpha, wref=None, wfit=None, lam=1.0):
+    """Plain double loop: sum_a sum_b K w exp(-alpha/2 |a-b|^2)."""
+    K = (math.pi / (2 * alpha)) ** 1.5
+    s = 0.0
+    for i, a in enumerate(ref):
+        for j, b in enumerate(fit):
+            w = 1.0 if wref is None else math.exp(-((wref[i] - wfit[j]) ** 2) / lam)
+            s += K * w * math.exp(-alpha / 2 * float(((np.asarray(a) - np.asarray(b)) ** 2).sum()))
+    return s
+
+
+def se3_from(q, t):
+    return torch.cat([torch.as_tensor(q, dtype=DT), torch.as_tensor(t, dtype=DT)], dim=-1)
+
+
+# ---------------------------------------------------------------------------------------------
+# Rotations
+# ---------------------------------------------------------------------------------------------
+def test_rotation_known_answer():
+    """Reference-held known answer (tests/test_alignment_utils.py:44-71)."""
+    ka = GOLD["se3_known_answer"]
+    q = torch.tensor([ka["quaternion"]], dtype=DT)
+    R = orc.rot_from_unit_quat(q / q.norm(dim=1, keepdim=True))[0].numpy()
+    np.testing.assert_allclose(R, np.array(ka["rotation"]), atol=1e-7)
+    T = orc.se3_matrix(torch.tensor([ka["se3_params"]], dtype=DT))[0].numpy()
+    np.testing.assert_allclose(T, np.array(ka["se3_transform"]), atol=1e-7)
+    pts = torch.tensor([ka["points"]], dtype=DT)
+    moved = orc.apply_se3(pts, torch.tensor(T)[None])[0].numpy()
+    sol = (np.array(ka["se3_transform"])[:3, :3] @ np.array(ka["points"]).T).T + np.array(ka["se3_transform"])[:3, 3]
+    np.testing.assert_allclose(moved, sol, atol=1e-6)
+
+
+def test_rotation_jacobians_finite_difference():
+    rng = np.random.default_rng(0)
+    q = torch.tensor(synth.random_quaternions(rng, 4), dtype=DT)
+    J = orc.rot_jacobians(q)
+    h = 1e-6
+    for k in range(4):
+        dq = torch.zeros_like(q)
+        dq[:, k] = h
+        # unnormalised formula (the Jacobians are of R(q) as written, evaluated at unit q)
+        fd = (orc.rot_from_unit_quat(q + dq) - orc.rot_from_unit_quat(q - dq)) / (2 * h)
+        np.testing.assert_allclose(J[k].numpy(), fd.numpy(), atol=1e-8)
+
+
+def test_rotation_is_orthonormal():
+    rng = np.random.default_rng(1)
+    q = torch.tensor(synth.random_quaternions(rng, 8), dtype=DT)
+    q = q / q.norm(dim=1, keepdim=True)
+    R = orc.rot_from_unit_quat(q)
+    np.testing.assert_allclose((R @ R.transpose(1, 2)).numpy(), np.broadcast_to(np.eye(3), (8, 3, 3)), atol=1e-12)
+    np.testing.assert_allclose(torch.det(R).numpy(), np.ones(8), atol=1e-12)
+
+
+# ---------------------------------------------------------------------------------------------
+# Overlaps
+# ---------------------------------------------------------------------------------------------
+def test_shape_overlap_brute_force_and_special_cases():
+    rng = np.random.default_rng(2)
+    ref = rng.normal(size=(5, 3))
+    fit = rng.normal(size=(4, 3))
+    I = torch.eye(3, dtype=DT)[None]
+    z = torch.zeros((1, 3), dtype=DT)
+    O, _, _ = orc.overlap_grad_shape(I, z, torch.tensor(ref)[None], torch.tensor(fit)[None], 0.81)
+    assert float(O) == pytest.approx(brute_overlap(ref, fit, 0.81), rel=1e-12)
+    # 1 x 1 closed form
+    a, b = np.array([[0.0, 0.0, 0.0]]), np.array([[1.0, 2.0, -0.5]])
+    O1, _, _ = orc.overlap_grad_shape(I, z, torch.tensor(a)[None], torch.tensor(b)[None], 0.7)
+    assert float(O1) == pytest.approx((math.pi / 1.4) ** 1.5 * math.exp(-0.35 * 5.25), rel=1e-12)
+    # coincident clouds: K * N * M
+    P = np.zeros((3, 3))
+    Q = np.zeros((6, 3))
+    Oc, _, _ = orc.overlap_grad_shape(I, z, torch.tensor(P)[None], torch.tensor(Q)[None], 1.0)
+    assert float(Oc) == pytest.approx(orc.K_of(1.0) * 18, rel=1e-12)
+    # K(0.81) = 2.70055 (docs/theory.md:43 quotes w = 2.7)
+    assert orc.K_of(0.81) == pytest.approx(2.70055, abs=1e-5)
+
+
+def test_esp_overlap_brute_force_and_equal_charges():
+    rng = np.random.default_rng(3)
+    ref, fit = rng.normal(size=(4, 3)), rng.normal(size=(5, 3))
+    cr, cf = rng.normal(0, 0.1, 4), rng.normal(0, 0.1, 5)
+    lam = 0.3 * orc.LAM_SCALING
+    I = torch.eye(3, dtype=DT)[None]
+    z = torch.zeros((1, 3), dtype=DT)
+    w = orc.esp_pair_weights(torch.tensor(cf), torch.tensor(cr), lam)
+    O, _, _ = orc.overlap_grad_shape(I, z, torch.tensor(ref)[None], torch.tensor(fit)[None], 0.81, w)
+    assert float(O) == pytest.approx(brute_overlap(ref, fit, 0.81, cr, cf, lam), rel=1e-12)
+    # equal charges: ESP overlap == shape overlap; lambda scaling constant (constants.py:16-17)
+    assert orc.LAM_SCALING == pytest.approx(207.3535, rel=1e-6)
+    w1 = orc.esp_pair_weights(torch.zeros(5, dtype=DT), torch.zeros(4, dtype=DT), lam)
+    O1, _, _ = orc.overlap_grad_shape(I, z, torch.tensor(ref)[None], torch.tensor(fit)[None], 0.81, w1)
+    O0, _, _ = orc.overlap_grad_shape(I, z, torch.tensor(ref)[None], torch.tensor(fit)[None], 0.81)
+    assert float(O1) == pytest.approx(float(O0), rel=1e-14)
+
+
+def test_pharm_overlap_special_cases():
+    I = torch.eye(3, dtype=DT)[None]
+    z = torch.zeros((1, 3), dtype=DT)
+    a = torch.zeros((1, 1, 3), dtype=DT)
+    up = torch.tensor([[[0.0, 0.0, 1.0]]], dtype=DT)
+    down = -up
+    def ov(tr, tf, vr, vf):
+        return float(orc.overlap_grad_pharm(I, z, torch.tensor([[tr]]), torch.tensor([[tf]]), a, a, vr, vf)[0])
+    K1, K07 = orc.K_of(1.0), orc.K_of(0.7)
+    assert ov(0, 0, up, up) == pytest.approx(K1)              # acceptor parallel: w = 1
+    assert ov(0, 0, up, down) == pytest.approx(K1 * 2 / 3)    # acceptor antiparallel: clamp -> 2/3
+    assert ov(2, 2, up, down) == pytest.approx(K07)           # aromatic: |cos| -> 1
+    assert ov(3, 3, up, down) == pytest.approx(K07)           # hydrophobe: w = 1
+    assert ov(0, 1, up, up) == 0.0                            # different types never pair
+    assert ov(8, 8, up, up) == 0.0                            # Dummy is ignored
+    # directionless: acceptor antiparallel is isotropic
+    O = orc.overlap_grad_pharm(I, z, torch.tensor([[0]]), torch.tensor([[0]]), a, a, up, down, directionless=True)[0]
+    assert float(O) == pytest.approx(K1)
+
+
+def _finite_difference(pair, se3, h=1e-6):
+    so = pair.self_overlaps()
+    g = torch.zeros_like(se3)
+    for i in range(se3.shape[0]):
+        for k in range(7):
+            d = torch.zeros_like(se3)
+            d[i, k] = h
+            lp, _, _ = pair.loss_grad_se3(se3 + d, so)
+            lm, _, _ = pair.loss_grad_se3(se3 - d, so)
+            g[i, k] = (lp - lm) / (2 * h)
+    return g
+
+
+def _pairs():
+    re, rw = synth.vol_pairs(1, 8, 12, seed=11)
+    se, sez, fe, fez = synth.surf_esp_pairs(1, atoms=(6, 9), points=(20, 30), seed=12)
+    d = synth.pharm_shape_pairs(1, atoms=(8, 14), pharms=(6, 10), seed=13)
+    ph = dict(ref=d["ref_atoms"][0], fit=d["fit_atoms"][0], ref_types=d["ref_types"][0],
+              fit_types=d["fit_types"][0], ref_anc=d["ref_anchors"][0], fit_anc=d["fit_anchors"][0],
+              ref_vec=d["ref_vectors"][0], fit_vec=d["fit_vectors"][0])
+    # give every fit pharmacophore a matching-type partner so gradients are non-trivial
+    ph["fit_types"] = ph["ref_types"][: len(ph["fit_types"])].copy() if len(ph["ref_types"]) >= len(ph["fit_types"]) else ph["fit_types"]
+    return [
+        orc.Pair("shape", ref=re[0], fit=rw[0]),
+        orc.Pair("esp", ref=se[0], fit=fe[0], ref_esp=sez[0], fit_esp=fez[0]),
+        orc.Pair("pharm", **ph),
+        orc.Pair("pharm", similarity="tversky", **ph),
+        orc.Pair("shape_pharm", **ph),
+        orc.Pair("shape_pharm", directionless=True, **ph),
+    ]
+
+
+@pytest.mark.parametrize("k", range(6))
+def test_se3_gradient_finite_difference(k):
+    pair = _pairs()[k]
+    rng = np.random.default_rng(20 + k)
+    se3 = se3_from(synth.random_quaternions(rng, 3) * 1.2, rng.normal(0, 0.3, (3, 3)))
+    _, g, _ = pair.loss_grad_se3(se3, pair.self_overlaps())
+    fd = _finite_difference(pair, se3)
+    np.testing.assert_allclose(g.numpy(), fd.numpy(), atol=1e-8, rtol=1e-5)
+
+
+@pytest.mark.parametrize("k", range(6))
+def test_se3_gradient_vs_autograd_of_scoring(k):
+    """Analytic gradient == autograd of the independently restated scoring functions."""
+    pair = _pairs()[k]
+    rng = np.random.default_rng(40 + k)
+    se3 = se3_from(synth.random_quaternions(rng, 4) * 0.8, rng.normal(0, 0.5, (4, 3)))
+    L, g, per = pair.loss_grad_se3(se3, pair.self_overlaps())
+    x = se3.clone().requires_grad_(True)
+    La = 1 - pair.score_poses(x).mean()
+    La.backward()
+    assert float(L) == pytest.approx(float(La.detach()), abs=1e-12)
+    np.testing.assert_allclose(g.numpy(), x.grad.numpy(), atol=1e-12)
+
+
+def test_brute_force_loss_tiny():
+    """Per-restart loss of the analytic path == 1 - brute-force Tanimoto."""
+    rng = np.random.default_rng(5)
+    ref, fit = rng.normal(size=(3, 3)), rng.normal(size=(4, 3))
+    pair = orc.Pair("shape", ref=ref, fit=fit)
+    q = synth.random_quaternions(rng, 1).astype(np.float64)
+    q /= np.linalg.norm(q)
+    t = rng.normal(size=(1, 3))
+    _, _, per = pair.loss_grad_se3(se3_from(q, t), pair.self_overlaps())
+    R = orc.rot_from_unit_quat(torch.tensor(q, dtype=DT))[0].numpy()
+    moved = fit @ R.T + t
+    vab = brute_overlap(ref, moved, 0.81)
+    S = vab / (brute_overlap(ref, ref, 0.81) + brute_overlap(fit, fit, 0.81) - vab)
+    assert float(per[0]) == pytest.approx(1 - S, abs=1e-12)
+
+
+def test_invariances_and_self_copy():
+    rng = np.random.default_rng(6)
+    ref, = synth.vol_pairs(1, 15, 15, seed=3)[0:1]
+    ref = ref[0].astype(np.float64)
+    pair = orc.Pair("shape", ref=ref, fit=ref)
+    ident = se3_from([[1.0, 0, 0, 0]], [[0.0, 0, 0]])
+    assert float(pair.score_poses(ident)[0]) == pytest.approx(1.0, abs=1e-14)
+    # a rigid motion of the fit cloud changes neither VBB nor, after undoing it, the score
+    q = synth.random_quaternions(rng, 1).astype(np.float64)
+    q /= np.linalg.norm(q)
+    R = orc.rot_from_unit_quat(torch.tensor(q))[0].numpy()
+    t = rng.normal(size=3)
+    moved = ref @ R.T + t
+    p2 = orc.Pair("shape", ref=ref, fit=moved)
+    assert float(p2.self_overlaps()["VBB_s"]) == pytest.approx(float(pair.self_overlaps()["VBB_s"]), rel=1e-12)
+    # inverse motion as a pose: R^T, -R^T t
+    qi = q.copy()
+    qi[:, 1:] *= -1
+    back = se3_from(qi, (-(R.T @ t))[None])
+    assert float(p2.score_poses(back)[0]) == pytest.approx(1.0, abs=1e-12)
+
+
+def test_tversky_clamp():
+    I = torch.eye(3, dtype=DT)[None]
+    O = torch.tensor([2.0, 0.5], dtype=DT)
+    l, gR, gt = orc.tversky_chain(O, torch.tensor([1.0, 1.0], dtype=DT), I.repeat(2, 1, 1), torch.ones((2, 3), dtype=DT))
+    assert l.tolist() == [0.0, 0.5]
+    assert float(gR[0].abs().sum()) == 0.0 and float(gt[1, 0]) == pytest.approx(-1.0)
+
+
+def test_optimizer_recovers_rigid_copy():
+    """Aligning a rotated, shifted copy reaches the known optimum S = 1."""
+    rng = np.random.default_rng(7)
+    ref = synth.vol_pairs(1, 20, 20, seed=9)[0][0].astype(np.float64)
+    q = synth.random_quaternions(rng, 1).astype(np.float64)
+    q /= np.linalg.norm(q)
+    R = orc.rot_from_unit_quat(torch.tensor(q))[0].numpy()
+    fit = ref @ R.T + np.array([0.4, -0.3, 0.2])
+    pair = orc.Pair("shape", ref=ref, fit=fit)
+    B = 12
+    q0 = synth.random_quaternions(np.random.default_rng(8), B)
+    q0[0] = np.concatenate([q[0, :1], -q[0, 1:]])  # near the inverse
+    t0 = np.tile(ref.mean(0) - fit.mean(0), (B, 1))
+    out = orc.optimize(pair, q0, t0, lr=0.05, max_steps=300)
+    assert out["score"] > 0.999
+
+
+def test_fibo45_table_is_fibonacci_construction():
+    """The reference's 45-seed table equals the Fibonacci-sphere construction (to float32)."""
+    from fast_shepherd_score_amd.seeds import quats_from_fibo
+    np.testing.assert_allclose(quats_from_fibo(45), np.array(GOLD["fibo45"]), atol=2e-7)
