@@ -46,13 +46,13 @@ class FssPairBatch(ctypes.Structure):
 class FssScoreParams(ctypes.Structure):
     _fields_ = [
         ("mode", ctypes.c_int32),
-        ("alpha", ctypes.c_float),
-        ("lam", ctypes.c_float),
-        ("color_weight", ctypes.c_float),
         ("similarity", ctypes.c_int32),
-        ("sigma", ctypes.c_float),
-        ("ph_alpha", ctypes.c_float * N_PTYPES),
-        ("ph_k", ctypes.c_float * N_PTYPES),
+        ("alpha", ctypes.c_double),
+        ("lam", ctypes.c_double),
+        ("color_weight", ctypes.c_double),
+        ("sigma", ctypes.c_double),
+        ("ph_alpha", ctypes.c_double * N_PTYPES),
+        ("ph_k", ctypes.c_double * N_PTYPES),
         ("ph_cat", ctypes.c_int32 * N_PTYPES),
     ]
 

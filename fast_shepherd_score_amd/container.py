@@ -1,0 +1,277 @@
+"""Drop-in pair containers with ``backend="hip"`` (rdkit-free).
+
+``MoleculePairBatch.align_with_<mode>`` keeps the reference's method names, keyword arguments,
+defaults, in-place result attributes and return contract
+(shepherd_score/container/_batch.py:274-1515; attributes accel/_modes.py:18-26):
+  * results land on each pair as ``transform_<mode>`` (4x4 float32 torch tensor, fit -> ref,
+    applied as ``fit @ R.T + t``) and ``sim_aligned_<mode>`` (Python float);
+  * the call returns ``(scores np.float64 (K,), aligned list or [None] * K)``; pharm returns
+    ``(scores, anchors list, vectors list)``.
+The backend dispatch mirrors ``_run_fast_or_fallthrough`` (_batch.py:127-151): ``None``, ``"hip"``
+and the accelerated names ``"triton"``, ``"cuda"``, ``"gpu"`` run the gfx950 kernels; names of
+backends this package does not ship (``"numba"``, ``"cpu"``, ``"jax"``) raise ValueError.
+
+Semantics of the HIP backend: ``num_repeats`` restarts from the reference's multi-start seeds
+(identity, four principal-axis alignments, Fibonacci rotations; alignment/_torch.py:253-326),
+``lr`` and ``max_num_steps`` are honoured, torch-style Adam, per-pair early stop
+(alignment/_torch_analytical.py:23-478), FP64 re-scoring and best-restart selection.
+"""
+from __future__ import annotations
+
+from typing import List, Optional, Sequence
+
+import numpy as np
+import torch
+
+from . import _lib
+from .align import OptimSpec, ScoreSpec, align
+from .constants import LAM_SCALING
+from .modes import HIP_MODES, MODE_ATTRS, MODE_SEEDS, MODE_STEPS, canonical
+from .packing import CloudSet, PairBatch, PharmSet
+from .seeds import initial_seeds, pad
+
+HIP_BACKENDS = ("hip", "rocm", "triton", "cuda", "gpu")
+
+
+def resolve_backend(backend: Optional[str]) -> str:
+    if backend is None or backend in HIP_BACKENDS:
+        return "hip"
+    raise ValueError(f"unknown backend {backend!r}; fast_shepherd_score_amd runs backend='hip' "
+                     f"(aliases {HIP_BACKENDS[1:]})")
+
+
+class Molecule:
+    """Array-only molecule: heavy-atom positions and optional surface / ESP / pharmacophores.
+
+    Mirrors the attributes the reference's aligners read from a ``Molecule``
+    (container/_core.py; screen.py:650-668): ``atom_pos``, ``surf_pos``, ``surf_esp``,
+    ``pharm_types``, ``pharm_ancs``, ``pharm_vecs`` and per-heavy-atom ``atom_charges``.
+    """
+
+    def __init__(self, atom_pos, surf_pos=None, surf_esp=None, pharm_types=None,
+                 pharm_ancs=None, pharm_vecs=None, atom_charges=None):
+        f = lambda a: None if a is None else np.ascontiguousarray(np.asarray(a, np.float32))
+        self.atom_pos = f(atom_pos)
+        self.surf_pos = f(surf_pos)
+        self.surf_esp = f(surf_esp)
+        self.pharm_types = None if pharm_types is None else np.asarray(pharm_types, np.int64)
+        self.pharm_ancs = f(pharm_ancs)
+        self.pharm_vecs = f(pharm_vecs)
+        self.atom_charges = f(atom_charges)
+
+    def centered(self) -> "Molecule":
+        c = self.atom_pos.mean(0)
+        sub = lambda a: None if a is None else a - c
+        return Molecule(self.atom_pos - c, sub(self.surf_pos), self.surf_esp, self.pharm_types,
+                        sub(self.pharm_ancs), self.pharm_vecs, self.atom_charges)
+
+
+class MoleculePair:
+    """A (ref, fit) pair; the fit molecule is moved onto the ref by the alignment."""
+
+    def __init__(self, ref_mol: Molecule, fit_mol: Molecule, num_surf_points: Optional[int] = None,
+                 do_center: bool = False, device=None):
+        self.ref_molec = ref_mol.centered() if do_center else ref_mol
+        self.fit_molec = fit_mol.centered() if do_center else fit_mol
+        self.num_surf_points = num_surf_points
+        self.device = torch.device(device) if device is not None else torch.device("cuda")
+        for _, (tf, sc) in MODE_ATTRS.items():
+            setattr(self, tf, None)
+            setattr(self, sc, None)
+
+
+def _mode_inputs(pairs: Sequence[MoleculePair], mode: str):
+    """Host arrays (ref clouds, fit clouds, ref w, fit w, pharm dict) a mode consumes."""
+    R = [p.ref_molec for p in pairs]
+    F = [p.fit_molec for p in pairs]
+    need = lambda ms, a: [getattr(m, a) for m in ms]
+    def check(vals, what):
+        if any(v is None for v in vals):
+            raise ValueError(f"mode needs {what} on every molecule")
+        return vals
+    ph = None
+    rw = fw = None
+    if mode in ("vol", "vol_color"):
+        rc, fc = check(need(R, "atom_pos"), "atom_pos"), check(need(F, "atom_pos"), "atom_pos")
+    elif mode == "vol_esp":
+        rc, fc = check(need(R, "atom_pos"), "atom_pos"), check(need(F, "atom_pos"), "atom_pos")
+        rw, fw = check(need(R, "atom_charges"), "atom_charges"), check(need(F, "atom_charges"), "atom_charges")
+    elif mode == "surf":
+        rc, fc = check(need(R, "surf_pos"), "surf_pos"), check(need(F, "surf_pos"), "surf_pos")
+    elif mode == "surf_esp":
+        rc, fc = check(need(R, "surf_pos"), "surf_pos"), check(need(F, "surf_pos"), "surf_pos")
+        rw, fw = check(need(R, "surf_esp"), "surf_esp"), check(need(F, "surf_esp"), "surf_esp")
+    elif mode == "pharm":
+        rc, fc = None, None
+    else:
+        raise ValueError(mode)
+    if mode in ("pharm", "vol_color"):
+        ph = {}
+        for side, ms in (("ref", R), ("fit", F)):
+            ph[f"{side}_types"] = check(need(ms, "pharm_types"), "pharm_types")
+            ph[f"{side}_anchors"] = check(need(ms, "pharm_ancs"), "pharm_ancs")
+            ph[f"{side}_vectors"] = check(need(ms, "pharm_vecs"), "pharm_vecs")
+    return rc, fc, rw, fw, ph
+
+
+def align_pairs(pairs: Sequence[MoleculePair], mode: str, *, alpha=0.81, lam=None,
+                color_weight=0.5, similarity="tanimoto", directionless=True,
+                num_repeats: int = 50, lr: float = 0.1, max_num_steps: int = 200,
+                early_stop: bool = True, seeds=None, device=None, **tiling):
+    """Align every pair in one persistent launch; write results in place; return the result."""
+    mode = canonical(mode)
+    if mode not in HIP_MODES:
+        raise NotImplementedError(f"mode {mode!r} is not implemented by the HIP backend")
+    if not pairs:
+        return None
+    dev = torch.device(device) if device is not None else pairs[0].device
+    rc, fc, rw, fw, ph = _mode_inputs(pairs, mode)
+    kind = {"vol": _lib.MODE_SHAPE, "surf": _lib.MODE_SHAPE, "vol_esp": _lib.MODE_SHAPE_ESP,
+            "surf_esp": _lib.MODE_SHAPE_ESP, "pharm": _lib.MODE_PHARM,
+            "vol_color": _lib.MODE_SHAPE_PHARM}[mode]
+    if lam is None:
+        lam = 0.3 * LAM_SCALING if mode == "surf_esp" else 0.1
+    spec = ScoreSpec(mode=kind, alpha=alpha, lam=lam, color_weight=color_weight,
+                     similarity=similarity, directionless=directionless)
+    ref = CloudSet.from_arrays(rc, rw, device=dev) if rc is not None else None
+    fit = CloudSet.from_arrays(fc, fw, device=dev) if fc is not None else None
+    rph = fph = None
+    if ph is not None:
+        rph = PharmSet.from_arrays(ph["ref_types"], ph["ref_anchors"], ph["ref_vectors"], device=dev)
+        fph = PharmSet.from_arrays(ph["fit_types"], ph["fit_anchors"], ph["fit_vectors"], device=dev)
+    pb = PairBatch(ref, fit, rph, fph)
+    if seeds is None:
+        # Seeds come from the clouds the objective moves: pharm anchors for pharm, else shape.
+        sref = ph["ref_anchors"] if mode == "pharm" else rc
+        sfit = ph["fit_anchors"] if mode == "pharm" else fc
+        (R, rn), (Fp, fn) = pad(sref), pad(sfit)
+        q0, t0 = initial_seeds(R, rn, Fp, fn, num_repeats)
+    else:
+        q0, t0 = seeds
+    q0 = torch.as_tensor(np.asarray(q0, np.float32)).to(dev)
+    t0 = torch.as_tensor(np.asarray(t0, np.float32)).to(dev)
+    opt = OptimSpec(max_steps=int(max_num_steps), lr=float(lr), early_stop=bool(early_stop))
+    res = align(pb, spec, q0, t0, opt, **tiling)
+    scores = res.score.cpu().numpy()
+    tfs = res.transform.cpu()
+    tf_attr, sc_attr = MODE_ATTRS[mode]
+    for i, p in enumerate(pairs):
+        setattr(p, tf_attr, tfs[i].clone())
+        setattr(p, sc_attr, float(scores[i]))
+    return res
+
+
+class MoleculePairBatch:
+    """Batch of MoleculePair objects aligned together on the GPU."""
+
+    _TRITON_BACKENDS = ("triton", "cuda", "gpu")
+    _HIP_BACKENDS = HIP_BACKENDS
+
+    def __init__(self, pairs: List[MoleculePair]):
+        self.pairs = pairs
+
+    # -- shared plumbing -------------------------------------------------------------------
+    def _run(self, mode, backend, return_aligned, fit_cloud, num_repeats, max_num_steps, **kw):
+        resolve_backend(backend)
+        if kw.pop("trans_init", False):
+            raise NotImplementedError("trans_init=True seeds are not implemented by the HIP backend")
+        if num_repeats is None:
+            num_repeats = MODE_SEEDS[mode]
+        if max_num_steps is None:
+            max_num_steps = MODE_STEPS[mode]
+        align_pairs(self.pairs, mode, num_repeats=num_repeats, max_num_steps=max_num_steps, **kw)
+        scores = np.array([float(getattr(p, MODE_ATTRS[mode][1])) for p in self.pairs])
+        if not return_aligned:
+            return scores, [None] * len(self.pairs)
+        aligned = []
+        for p in self.pairs:
+            T = getattr(p, MODE_ATTRS[mode][0]).numpy().astype(np.float64)
+            X = np.asarray(getattr(p.fit_molec, fit_cloud), np.float64)
+            aligned.append((X @ T[:3, :3].T + T[:3, 3]).astype(np.float32))
+        return scores, aligned
+
+    @staticmethod
+    def _hip_kw(kw):
+        keep = ("early_stop", "seeds", "lanes_per_pose", "restarts_per_wg", "device")
+        return {k: v for k, v in kw.items() if k in keep}
+
+    # -- public API (reference signatures) --------------------------------------------------
+    def align_with_vol(self, no_H: bool = True, num_repeats: int = None, trans_init: bool = False,
+                       lr: float = 0.1, max_num_steps: int = None, num_workers: int = 1,
+                       use_shmap: bool = True, num_buckets: int = 1, verbose: bool = False,
+                       backend: Optional[str] = None, alpha: float = 0.81,
+                       return_aligned: bool = False, **hip_kw):
+        if not no_H:
+            raise ValueError("the HIP vol backend aligns heavy atoms only (no_H=True)")
+        return self._run("vol", backend, return_aligned, "atom_pos", num_repeats, max_num_steps,
+                         trans_init=trans_init, lr=lr, alpha=alpha, **self._hip_kw(hip_kw))
+
+    def align_with_vol_esp(self, lam: float = 0.1, no_H: bool = True, num_repeats: int = None,
+                           trans_init: bool = False, lr: float = 0.1, max_num_steps: int = None,
+                           num_workers: int = 1, use_shmap: bool = True, num_buckets: int = 1,
+                           verbose: bool = False, backend: Optional[str] = None,
+                           alpha: float = 0.81, return_aligned: bool = False, **hip_kw):
+        if not no_H:
+            raise ValueError("the HIP vol_esp backend aligns heavy atoms only (no_H=True)")
+        return self._run("vol_esp", backend, return_aligned, "atom_pos", num_repeats,
+                         max_num_steps, trans_init=trans_init, lr=lr, alpha=alpha, lam=lam,
+                         **self._hip_kw(hip_kw))
+
+    def align_with_surf(self, alpha: float, num_repeats: int = None, trans_init: bool = False,
+                        lr: float = 0.1, max_num_steps: int = None, use_jax: bool = True,
+                        use_analytical: bool = True, num_workers: int = 1,
+                        use_shmap: bool = False, verbose: bool = False,
+                        backend: Optional[str] = None, return_aligned: bool = False, **hip_kw):
+        return self._run("surf", backend, return_aligned, "surf_pos", num_repeats, max_num_steps,
+                         trans_init=trans_init, lr=lr, alpha=alpha, **self._hip_kw(hip_kw))
+
+    def align_with_surf_esp(self, alpha: float, lam: float = 0.3, num_repeats: int = None,
+                            trans_init: bool = False, lr: float = 0.1, max_num_steps: int = None,
+                            use_jax: bool = True, use_analytical: bool = True,
+                            num_workers: int = 1, use_shmap: bool = False, verbose: bool = False,
+                            backend: Optional[str] = None, return_aligned: bool = False, **hip_kw):
+        # lam is scaled by LAM_SCALING internally, as in the reference (_batch.py:944-948).
+        return self._run("surf_esp", backend, return_aligned, "surf_pos", num_repeats,
+                         max_num_steps, trans_init=trans_init, lr=lr, alpha=alpha,
+                         lam=LAM_SCALING * lam, **self._hip_kw(hip_kw))
+
+    align_with_esp = align_with_surf_esp   # legacy alias (accel/_modes.py:35)
+
+    def align_with_pharm(self, similarity: str = "tanimoto", extended_points: bool = False,
+                         only_extended: bool = False, num_repeats: int = None,
+                         trans_init: bool = False, lr: float = 0.1, max_num_steps: int = None,
+                         num_workers: int = 1, use_shmap: bool = True, num_buckets: int = 1,
+                         verbose: bool = False, backend: Optional[str] = None,
+                         return_aligned: bool = False, **hip_kw):
+        if extended_points:
+            raise NotImplementedError("extended_points scoring is not implemented by the HIP backend")
+        scores, _ = self._run("pharm", backend, False, None, num_repeats, max_num_steps,
+                              trans_init=trans_init, lr=lr, similarity=similarity,
+                              **self._hip_kw(hip_kw))
+        n = len(self.pairs)
+        if not return_aligned:
+            return scores, [None] * n, [None] * n
+        anchors, vectors = [], []
+        for p in self.pairs:
+            T = p.transform_pharm.numpy().astype(np.float64)
+            anchors.append((np.asarray(p.fit_molec.pharm_ancs, np.float64) @ T[:3, :3].T + T[:3, 3]).astype(np.float32))
+            vectors.append((np.asarray(p.fit_molec.pharm_vecs, np.float64) @ T[:3, :3].T).astype(np.float32))
+        return scores, anchors, vectors
+
+    def align_with_vol_color(self, color_weight: float = 0.5, alpha: float = 0.81,
+                             num_repeats: int = None, trans_init: bool = False, lr: float = 0.1,
+                             max_num_steps: int = None, verbose: bool = False,
+                             backend: Optional[str] = None, return_aligned: bool = False,
+                             directionless: bool = True, similarity: str = "tanimoto", **hip_kw):
+        """Shape + pharmacophore colour.  ``directionless=False`` keeps the cosine-weighted
+        directional pharmacophore channel (alignment/_torch.py:1376-1548)."""
+        return self._run("vol_color", backend, return_aligned, "atom_pos", num_repeats,
+                         max_num_steps, trans_init=trans_init, lr=lr, alpha=alpha,
+                         color_weight=color_weight, directionless=directionless,
+                         similarity=similarity, **self._hip_kw(hip_kw))
+
+    def align_with_vol_and_surf_esp(self, *a, **k):
+        raise NotImplementedError("vol_and_surf_esp (ShaEP ESP comparison) is not implemented by "
+                                  "the HIP backend")
+
+    align_with_esp_combo = align_with_vol_and_surf_esp

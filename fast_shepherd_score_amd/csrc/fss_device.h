@@ -267,7 +267,7 @@ __device__ __forceinline__ double pharm_value_f64(const float4* __restrict__ ran
                                                   const float4* __restrict__ fvec,
                                                   const int32_t* __restrict__ ftyp, int M, int j,
                                                   int lpp, const double R[9], const double t[3],
-                                                  const float* alpha, const float* kk,
+                                                  const double* alpha, const double* kk,
                                                   const int32_t* cats) {
   double v = 0.0;
   for (int b = j; b < M; b += lpp) {
@@ -286,7 +286,7 @@ __device__ __forceinline__ double pharm_value_f64(const float4* __restrict__ ran
     const double ux = R[0] * ux0 + R[1] * uy0 + R[2] * uz0;
     const double uy = R[3] * ux0 + R[4] * uy0 + R[5] * uz0;
     const double uz = R[6] * ux0 + R[7] * uy0 + R[8] * uz0;
-    const double ha = 0.5 * (double)alpha[tb], K = kk[tb];
+    const double ha = 0.5 * alpha[tb], K = kk[tb];
     for (int a = 0; a < N; ++a) {
       if (rtyp[a] != tb) continue;
       const float4 r = ranc[a];

@@ -58,37 +58,40 @@ struct KScore {
   float w;
   int sim;
   float sigma;
+  double w_d, sigma_d;
   PhTab tab;
-  float ph_alpha[FSS_N_PTYPES];
-  float ph_k[FSS_N_PTYPES];
+  double ph_alpha[FSS_N_PTYPES];
+  double ph_k[FSS_N_PTYPES];
   int32_t ph_cat[FSS_N_PTYPES];
 };
 
 int make_kscore(const FssScoreParams* sp, KScore& ks) {
   if (!sp) return set_error(FSS_E_INVALID, "score params are NULL");
   if (sp->mode < 0 || sp->mode > 3) return set_error(FSS_E_INVALID, "unknown mode %d", sp->mode);
-  if (uses_shape(sp->mode) && !(sp->alpha > 0.f)) return set_error(FSS_E_INVALID, "alpha must be > 0");
-  if (sp->mode == FSS_MODE_SHAPE_ESP && !(sp->lam > 0.f)) return set_error(FSS_E_INVALID, "lam must be > 0");
+  if (uses_shape(sp->mode) && !(sp->alpha > 0.0)) return set_error(FSS_E_INVALID, "alpha must be > 0");
+  if (sp->mode == FSS_MODE_SHAPE_ESP && !(sp->lam > 0.0)) return set_error(FSS_E_INVALID, "lam must be > 0");
   if (sp->similarity != FSS_SIM_TANIMOTO && sp->similarity != FSS_SIM_TVERSKY)
     return set_error(FSS_E_INVALID, "unknown similarity %d", sp->similarity);
   memset(&ks, 0, sizeof(ks));
   ks.mode = sp->mode;
-  ks.alpha = sp->alpha;
+  ks.alpha = (float)sp->alpha;
   ks.alpha_d = sp->alpha;
-  ks.K = uses_shape(sp->mode) ? (float)pow(M_PI / (2.0 * (double)sp->alpha), 1.5) : 0.f;
-  ks.s = uses_shape(sp->mode) ? (float)sqrt(0.5 * (double)sp->alpha * kLog2eD) : 1.f;
-  ks.lam_d = sp->lam > 0.f ? (double)sp->lam : 1.0;
-  ks.sc = sp->mode == FSS_MODE_SHAPE_ESP ? (float)sqrt(kLog2eD / (double)sp->lam) : 0.f;
-  ks.w = sp->color_weight;
+  ks.K = uses_shape(sp->mode) ? (float)pow(M_PI / (2.0 * sp->alpha), 1.5) : 0.f;
+  ks.s = uses_shape(sp->mode) ? (float)sqrt(0.5 * sp->alpha * kLog2eD) : 1.f;
+  ks.lam_d = sp->lam > 0.0 ? sp->lam : 1.0;
+  ks.sc = sp->mode == FSS_MODE_SHAPE_ESP ? (float)sqrt(kLog2eD / sp->lam) : 0.f;
+  ks.w = (float)sp->color_weight;
+  ks.w_d = sp->color_weight;
   ks.sim = sp->similarity;
-  ks.sigma = sp->sigma;
+  ks.sigma = (float)sp->sigma;
+  ks.sigma_d = sp->sigma;
   for (int i = 0; i < FSS_N_PTYPES; ++i) {
     ks.ph_alpha[i] = sp->ph_alpha[i];
     ks.ph_k[i] = sp->ph_k[i];
     ks.ph_cat[i] = sp->ph_cat[i];
-    ks.tab.kap[i] = (float)(0.5 * (double)sp->ph_alpha[i] * kLog2eD);
-    ks.tab.ak[i] = (float)((double)sp->ph_alpha[i] * (double)sp->ph_k[i]);
-    ks.tab.k[i] = sp->ph_k[i];
+    ks.tab.kap[i] = (float)(0.5 * sp->ph_alpha[i] * kLog2eD);
+    ks.tab.ak[i] = (float)(sp->ph_alpha[i] * sp->ph_k[i]);
+    ks.tab.k[i] = (float)sp->ph_k[i];
     ks.tab.cat[i] = sp->ph_cat[i];
   }
   return FSS_OK;
@@ -206,8 +209,10 @@ __device__ __forceinline__ T similarity(const KScore& ks, T Os, T Op, const doub
   // ws, wp receive dS/dO of the shape and pharm channel.
   T S = 0;
   ws = 0; wp = 0;
-  const T chs = ks.mode == FSS_MODE_SHAPE_PHARM ? (T)(1.0f - ks.w) : (T)1;
-  const T chp = ks.mode == FSS_MODE_SHAPE_PHARM ? (T)ks.w : (T)1;
+  const T wc = sizeof(T) == 8 ? (T)ks.w_d : (T)ks.w;
+  const T sg = sizeof(T) == 8 ? (T)ks.sigma_d : (T)ks.sigma;
+  const T chs = ks.mode == FSS_MODE_SHAPE_PHARM ? (T)1 - wc : (T)1;
+  const T chp = ks.mode == FSS_MODE_SHAPE_PHARM ? wc : (T)1;
   if (has_sh) {
     const T U = (T)so[0] + (T)so[1];
     const T D = U - Os;
@@ -221,7 +226,7 @@ __device__ __forceinline__ T similarity(const KScore& ks, T Os, T Op, const doub
       S += chp * (Op / D);
       wp = chp * U / (D * D);
     } else {
-      const T D = (T)ks.sigma * (T)so[2] + (T)(1.0f - ks.sigma) * (T)so[3];
+      const T D = sg * (T)so[2] + ((T)1 - sg) * (T)so[3];
       const T r = Op / D;
       S += chp * (r < (T)1 ? r : (T)1);
       wp = Op < D ? chp / D : (T)0;
@@ -746,8 +751,8 @@ int score_check(const FssPairBatch* batch, const FssScoreParams* sp) {
   if (!sp) return set_error(FSS_E_INVALID, "score params are NULL");
   if (sp->mode != FSS_MODE_SHAPE && sp->mode != FSS_MODE_SHAPE_ESP)
     return set_error(FSS_E_INVALID, "score fwd/bwd supports the shape and ESP modes only");
-  if (!(sp->alpha > 0.f)) return set_error(FSS_E_INVALID, "alpha must be > 0");
-  if (sp->mode == FSS_MODE_SHAPE_ESP && !(sp->lam > 0.f)) return set_error(FSS_E_INVALID, "lam must be > 0");
+  if (!(sp->alpha > 0.0)) return set_error(FSS_E_INVALID, "alpha must be > 0");
+  if (sp->mode == FSS_MODE_SHAPE_ESP && !(sp->lam > 0.0)) return set_error(FSS_E_INVALID, "lam must be > 0");
   if (!batch || !batch->ref_rng || !batch->fit_rng) return set_error(FSS_E_INVALID, "NULL ranges");
   return FSS_OK;
 }

@@ -72,13 +72,13 @@ typedef struct FssPairBatch {
 /* Scoring constants shared by every entry point. */
 typedef struct FssScoreParams {
   int32_t mode;              /* FSS_MODE_*                                                   */
-  float   alpha;             /* shape Gaussian width (0.81 volumetric)                       */
-  float   lam;               /* ESP lambda, already scaled (surface: 0.3*LAM_SCALING)        */
-  float   color_weight;      /* w of FSS_MODE_SHAPE_PHARM                                    */
   int32_t similarity;        /* FSS_SIM_* for the pharmacophore channel                     */
-  float   sigma;             /* Tversky sigma                                                */
-  float   ph_alpha[FSS_N_PTYPES];   /* per-type pharmacophore alpha                          */
-  float   ph_k[FSS_N_PTYPES];       /* per-type (pi/(2 alpha))^1.5                           */
+  double  alpha;             /* shape Gaussian width (0.81 volumetric)                       */
+  double  lam;               /* ESP lambda, already scaled (surface: 0.3*LAM_SCALING)        */
+  double  color_weight;      /* w of FSS_MODE_SHAPE_PHARM                                    */
+  double  sigma;             /* Tversky sigma                                                */
+  double  ph_alpha[FSS_N_PTYPES];   /* per-type pharmacophore alpha                          */
+  double  ph_k[FSS_N_PTYPES];       /* per-type (pi/(2 alpha))^1.5                           */
   int32_t ph_cat[FSS_N_PTYPES];     /* 0 isotropic, 1 directional, 2 aromatic, 3 ignored     */
 } FssScoreParams;
 

@@ -12,8 +12,8 @@ import tempfile
 HERE = os.path.dirname(os.path.abspath(__file__))
 SRC = os.path.join(HERE, "cpu_baseline.cpp")
 OUT = os.path.join(HERE, "libfss_cpu.so")
-FLAGS = ["-O3", "-std=c++17", "-march=x86-64-v3", "-fopenmp", "-fPIC", "-shared",
-         "-fno-math-errno", "-ffp-contract=fast"]
+FLAGS = ["-O3", "-std=c++17", "-march=x86-64-v3", "-pthread", "-fPIC", "-shared",
+         "-fno-math-errno", "-ffp-contract=fast", "-fopenmp-simd"]
 
 
 def build_oracle(verbose: bool = False) -> str:

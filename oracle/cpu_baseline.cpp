@@ -1,4 +1,4 @@
-// CPU baseline of the ShEPhERD alignment hot path: C++17 / OpenMP.
+// CPU baseline of the ShEPhERD alignment hot path: C++17, std::thread pool.
 //
 // TEST INFRASTRUCTURE ONLY (the oracle package): timed beside the GPU by bench.py's cpu_baseline
 // leg and checked against the FP64 oracle by tests; never part of the product path.
@@ -8,18 +8,33 @@
 // fused exp), :500-582 (_pharm_grad_dq_kernel) -- driving the same optimiser semantics as the HIP
 // path (alignment/_torch_analytical.py:23-478: torch-style Adam on raw (q, t), gradient / B,
 // per-pair early stop on the restart-mean loss, FP64 re-scoring of the final poses, argmax).
-// Parallelism: one OpenMP task per pair (the numba backend's prange over poses, coarsened so a
-// thread keeps a pair's restarts for all steps).
+// Parallelism: a pool of std::threads pulling pairs from an atomic counter (the numba backend's
+// prange over poses, coarsened so a thread keeps a pair's restarts for all steps).  No OpenMP
+// runtime is linked, so the library can share a process with torch's own.
 #include <math.h>
-#include <omp.h>
 #include <stdint.h>
 #include <string.h>
 
+#include <atomic>
+#include <thread>
 #include <vector>
 
 #include "../include/fss.h"
 
 namespace {
+
+template <typename F>
+void parallel_for(int n, int n_threads, F&& fn) {
+  std::atomic<int> next{0};
+  auto work = [&]() {
+    for (int i = next.fetch_add(1); i < n; i = next.fetch_add(1)) fn(i);
+  };
+  const int nt = n_threads < 1 ? 1 : (n_threads > n ? (n > 0 ? n : 1) : n_threads);
+  std::vector<std::thread> pool;
+  for (int k = 1; k < nt; ++k) pool.emplace_back(work);
+  work();
+  for (auto& th : pool) th.join();
+}
 
 struct Cloud { const float* p; int n; };
 struct PhCloud { const float* anc; const float* vec; const int32_t* typ; int n; };
@@ -53,7 +68,6 @@ void shape_grad(const Cloud& A, const Cloud& Bc, const float R[9], const float t
     const float by = R[3] * p[0] + R[4] * p[1] + R[5] * p[2] + t[1];
     const float bz = R[6] * p[0] + R[7] * p[1] + R[8] * p[2] + t[2];
     double v = 0, fx = 0, fy = 0, fz = 0;
-#pragma omp simd reduction(+ : v, fx, fy, fz)
     for (int a = 0; a < A.n; ++a) {
       const float* r = A.p + 4 * a;
       const float dx = r[0] - bx, dy = r[1] - by, dz = r[2] - bz;
@@ -179,8 +193,7 @@ int fss_cpu_self_overlap(const FssPairBatch* pb, const FssScoreParams* sp, doubl
   const bool sh = sp->mode != FSS_MODE_PHARM, esp = sp->mode == FSS_MODE_SHAPE_ESP;
   const bool ph = sp->mode == FSS_MODE_PHARM || sp->mode == FSS_MODE_SHAPE_PHARM;
   const double I[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1}, z[3] = {0, 0, 0};
-#pragma omp parallel for schedule(dynamic) num_threads(n_threads)
-  for (int p = 0; p < pb->n_pairs; ++p) {
+  parallel_for(pb->n_pairs, n_threads, [&](int p) {
     for (int side = 0; side < 2; ++side) {
       double vs = 0, vp = 0;
       if (sh) {
@@ -198,7 +211,7 @@ int fss_cpu_self_overlap(const FssPairBatch* pb, const FssScoreParams* sp, doubl
       out[4 * p + side] = vs;
       out[4 * p + 2 + side] = vp;
     }
-  }
+  });
   return 0;
 }
 
@@ -209,9 +222,8 @@ int fss_cpu_align(const FssPairBatch* pb, const FssScoreParams* sp, const FssAli
   const bool sh = sp->mode != FSS_MODE_PHARM, esp = sp->mode == FSS_MODE_SHAPE_ESP;
   const bool ph = sp->mode == FSS_MODE_PHARM || sp->mode == FSS_MODE_SHAPE_PHARM;
   const int B = ap->n_restarts;
-  const float inv_lam = esp ? 1.0f / sp->lam : 0.f;
-#pragma omp parallel for schedule(dynamic) num_threads(n_threads)
-  for (int p = 0; p < pb->n_pairs; ++p) {
+  const float inv_lam = esp ? (float)(1.0 / sp->lam) : 0.f;
+  parallel_for(pb->n_pairs, n_threads, [&](int p) {
     Cloud A{}, Bc{};
     PhCloud PA{}, PB{};
     if (sh) {
@@ -240,7 +252,7 @@ int fss_cpu_align(const FssPairBatch* pb, const FssScoreParams* sp, const FssAli
         float R[9];
         for (int i = 0; i < 9; ++i) R[i] = (float)Rd[i];
         double Os = 0, Op = 0, gts[3] = {0, 0, 0}, Gs[9] = {0}, gtp[3] = {0, 0, 0}, Gp[9] = {0};
-        if (sh) shape_grad(A, Bc, R, tp + 3 * r, sp->alpha, esp, inv_lam, Os, gts, Gs);
+        if (sh) shape_grad(A, Bc, R, tp + 3 * r, (float)sp->alpha, esp, inv_lam, Os, gts, Gs);
         if (ph) pharm_grad(PA, PB, R, tp + 3 * r, *sp, Op, gtp, Gp);
         const Sim s = similarity(*sp, Os, Op, so, sh, ph);
         loss[r] = 1.0 - s.S;
@@ -286,7 +298,7 @@ int fss_cpu_align(const FssPairBatch* pb, const FssScoreParams* sp, const FssAli
     best_score[p] = bs;
     best_idx[p] = bi;
     n_steps[p] = done;
-  }
+  });
   return 0;
 }
 

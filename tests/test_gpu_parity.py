@@ -120,16 +120,17 @@ def test_paths_agree(dev):
 
 
 def test_alone_equals_in_mix_bitwise(dev):
-    """A pair's result does not depend on the other pairs of the launch."""
+    """A pair's result does not depend on the other pairs of the launch (fixed tiling)."""
     n, B = 8, 10
     pb, spec, _ = make_case(_lib.MODE_SHAPE, n, dev, seed=11)
     q, t = random_seeds(n, B, seed=12)
     opt = OptimSpec(max_steps=60, early_stop=True)
-    full = align(pb, spec, to_dev(q, dev), to_dev(t, dev), opt)
+    tile = dict(lanes_per_pose=8, restarts_per_wg=B)
+    full = align(pb, spec, to_dev(q, dev), to_dev(t, dev), opt, **tile)
     from fast_shepherd_score_amd.packing import PairBatch
     for i in (0, 5):
         one = PairBatch(pb.ref, pb.fit, fit_idx=[i], ref_idx=[i])
-        r = align(one, spec, to_dev(q[i:i + 1], dev), to_dev(t[i:i + 1], dev), opt)
+        r = align(one, spec, to_dev(q[i:i + 1], dev), to_dev(t[i:i + 1], dev), opt, **tile)
         assert float(r.score[0]) == float(full.score[i])
         assert torch.equal(r.q[0], full.q[i])
 
