@@ -15,7 +15,6 @@ from __future__ import annotations
 import argparse
 import json
 import os
-import platform
 import sys
 import time
 
@@ -68,28 +67,31 @@ def pair_terms(wl, B: int, steps: int) -> float:
 
 
 def cpu_baseline(wl, q, t, B, steps, sample: int, spec, opt):
-    """The oracle package's C++/OpenMP restatement timed on this host's cores."""
-    from oracle import cpu_ref
-    cores = len(os.sched_getaffinity(0))
+    """The oracle package's C++ restatement, timed on this host's cores in a child process."""
+    import subprocess
+    import tempfile
     idx = np.arange(min(sample, len(wl["fit"])))
-    sub = {k: ([v[i] for i in idx] if isinstance(v, list) else v) for k, v in wl.items()}
-    hb = cpu_ref.HostBatch(sub["ref"], sub["fit"], sub.get("ref_w"), sub.get("fit_w"))
-    cpu_ref.align(hb, spec, q[idx[:1]], t[idx[:1]], type(opt)(**{**opt.__dict__, "max_steps": 2}), cores)
-    t0 = time.perf_counter()
-    out = cpu_ref.align(hb, spec, q[idx], t[idx], opt, cores)
-    dt = time.perf_counter() - t0
-    model = ""
-    try:
-        for line in open("/proc/cpuinfo"):
-            if line.startswith("model name"):
-                model = line.split(":", 1)[1].strip()
-                break
-    except OSError:
-        pass
-    return {"value": len(idx) / dt, "unit": "aligned pairs/s", "cores": cores, "kind": "port",
+    cat = lambda xs: np.concatenate([np.asarray(xs[i], np.float32) for i in idx])
+    arrs = dict(n=len(idx), ref=cat(wl["ref"]), fit=cat(wl["fit"]),
+                ref_n=np.array([len(wl["ref"][i]) for i in idx]),
+                fit_n=np.array([len(wl["fit"][i]) for i in idx]),
+                q=q[idx], t=t[idx], mode=spec.mode, alpha=spec.alpha, lam=spec.lam,
+                steps=opt.max_steps, lr=opt.lr, early_stop=opt.early_stop)
+    if wl.get("ref_w") is not None:
+        arrs["ref_w"], arrs["fit_w"] = cat(wl["ref_w"]), cat(wl["fit_w"])
+    with tempfile.TemporaryDirectory() as d:
+        path = os.path.join(d, "sample.npz")
+        np.savez(path, **arrs)
+        res = subprocess.run([sys.executable, "-m", "oracle.cpu_bench", path], cwd=ROOT,
+                             capture_output=True, text=True, timeout=600)
+    if res.returncode != 0:
+        raise RuntimeError(f"cpu baseline failed: {res.stderr[-2000:]}")
+    r = json.loads(res.stdout.strip().splitlines()[-1])
+    dt = r["seconds"]
+    return {"value": len(idx) / dt, "unit": "aligned pairs/s", "cores": r["cores"], "kind": "port",
             "sample": f"first {len(idx)} pairs of the same workload, same seeds, {B} restarts x "
-                      f"{steps} steps, {dt:.1f} s on {cores} threads ({model or platform.processor()})",
-            "seconds": dt}, out
+                      f"{steps} steps, {dt:.1f} s on {r['cores']} threads ({r['cpu_model']})",
+            "seconds": dt}, np.array(r["score"])
 
 
 def main():
@@ -101,7 +103,7 @@ def main():
     ap.add_argument("--pairs", type=int, default=4096)
     ap.add_argument("--restarts", type=int, default=50)
     ap.add_argument("--opt-steps", type=int, default=200)
-    ap.add_argument("--cpu-sample", type=int, default=16)
+    ap.add_argument("--cpu-sample", type=int, default=48)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--lpp", type=int, default=0)
     ap.add_argument("--rpw", type=int, default=0)
@@ -182,8 +184,9 @@ def main():
         "score_mean": float(score.mean()),
     }
     if not args.no_cpu:
-        cb, _ = cpu_baseline(wl, q, t, B, steps, args.cpu_sample, spec, opt)
+        cb, cpu_scores = cpu_baseline(wl, q, t, B, steps, args.cpu_sample, spec, opt)
         cb.pop("seconds")
+        cb["max_abs_score_diff_vs_gpu"] = float(np.abs(cpu_scores - score[:len(cpu_scores)]).max())
         out["cpu_baseline"] = cb
     print(json.dumps(out), flush=True)
 

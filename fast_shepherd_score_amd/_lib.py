@@ -130,7 +130,7 @@ def load(path: str = LIB_PATH) -> ctypes.CDLL:
             raise ImportError(
                 f"{path} is missing: build it first with "
                 "`python -c 'import __graft_entry__ as g; g.build()'` (hipcc --offload-arch=gfx950)")
-        lib = ctypes.CDLL(path, mode=ctypes.RTLD_GLOBAL)
+        lib = ctypes.CDLL(path, mode=ctypes.RTLD_LOCAL)
         for name, (res, args) in SIGNATURES.items():
             fn = getattr(lib, name)
             fn.restype = res

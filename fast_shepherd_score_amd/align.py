@@ -103,18 +103,58 @@ def _next_pow2(x: int) -> int:
     return 1 << max(0, int(np.ceil(np.log2(max(1, x)))))
 
 
+def tiling_efficiency(counts: np.ndarray, B: int, lpp: int, rpw: int) -> float:
+    """Fraction of issued lane-work that is useful for a (lanes_per_pose, restarts_per_wg) tiling:
+    restart slots used x lanes of the rounded-up wavefronts used x fit rows of the two-row passes
+    used (averaged over the pairs' fit counts)."""
+    restart_util = B / (np.ceil(B / rpw) * rpw)
+    threads = rpw * lpp
+    lane_util = threads / (np.ceil(threads / 64.0) * 64.0)
+    m = np.maximum(np.asarray(counts, np.float64), 1.0)
+    tail_util = float(np.mean(m / (2.0 * lpp * np.ceil(m / (2.0 * lpp)))))
+    return float(restart_util * lane_util * tail_util)
+
+
+def candidate_tilings(B: int, early_stop: bool):
+    for lpp in (1, 2, 4, 8, 16, 32, 64):
+        rpws = [B] if early_stop else sorted({r for r in (B, (B + 1) // 2, (B + 2) // 3, (B + 3) // 4,
+                                                         8, 16, 32) if r >= 1})
+        for rpw in rpws:
+            if rpw * lpp <= 1024 and rpw * lpp >= 64 or (rpw * lpp < 64 and rpw == B):
+                yield lpp, min(rpw, B)
+
+
+N_SIMD = 256 * 4          # MI355X: 256 CUs x 4 SIMDs
+
+
+def tiling_cost(ref_counts, fit_counts, B: int, lpp: int, rpw: int) -> float:
+    """Modelled optimiser-step time of a tiling (arbitrary units).
+
+    A wave's serial work is its lanes' two-row passes over the ref cloud (about 12 issue slots
+    per pair term) plus the butterfly reduction (13 sums x log2(lpp) shuffles) and the per-pose
+    tail; the chip retires the waves of the launch N_SIMD at a time, and a SIMD needs two waves
+    to hide latency, so the step takes per-wave-work x max(waves / N_SIMD, 1/2)."""
+    n = np.maximum(np.asarray(ref_counts, np.float64), 1.0)
+    m = np.maximum(np.asarray(fit_counts, np.float64), 1.0)
+    passes = np.ceil(m / (2.0 * lpp))
+    per_wave = passes * 2.0 * n * 12.0 + 26.0 * np.log2(max(lpp, 1)) + 150.0
+    waves_per_pair = np.ceil(B / rpw) * np.ceil(rpw * lpp / 64.0)
+    waves = float(waves_per_pair * len(m))
+    return float(np.mean(per_wave)) * max(waves / N_SIMD, 0.5)
+
+
 def default_tiling(pb: PairBatch, mode: int, B: int, early_stop: bool) -> Tuple[int, int]:
-    """(lanes_per_pose, restarts_per_wg) heuristic: ~4 fit rows per lane, groups inside a wave."""
-    cnt = pb.fit_pcounts if mode == _lib.MODE_PHARM else pb.fit_counts
-    m = float(np.median(cnt)) if cnt is not None and len(cnt) else 1.0
-    lpp = int(min(64, max(1, _next_pow2(int(np.ceil(m / 4.0))))))
-    if early_stop:
-        while lpp > 1 and B * lpp > 1024:
-            lpp //= 2
-        rpw = B
+    """(lanes_per_pose, restarts_per_wg) minimising :func:`tiling_cost` over the legal tilings."""
+    if mode == _lib.MODE_PHARM:
+        rc, fc = pb.ref_pcounts, pb.fit_pcounts
     else:
-        rpw = max(1, min(B, 512 // lpp))
-    return lpp, rpw
+        rc, fc = pb.ref_counts, pb.fit_counts
+    rc = rc if rc is not None and len(rc) else np.ones(1)
+    fc = fc if fc is not None and len(fc) else np.ones(1)
+    cands = [(tiling_cost(rc, fc, B, l, r), -tiling_efficiency(fc, B, l, r), l, r)
+             for l, r in candidate_tilings(B, early_stop)]
+    cands.sort()
+    return cands[0][2], cands[0][3]
 
 
 def check_tiling(B: int, lpp: int, rpw: int, early_stop: bool) -> None:

@@ -11,10 +11,10 @@
 //   rotation R(q_hat) in (w,x,y,z) order and its four Jacobians (:28-139)
 //
 // Shape channel arithmetic runs on pre-scaled coordinates: with s = sqrt(alpha/2 * log2(e)) every
-// coordinate (and the translation) is multiplied by s while it is staged in LDS, so a pair term
-// costs 3 sub + 3 mul/fma + 1 v_exp_f32 + 4 accumulations and no extra scale; the ESP value is
-// pre-scaled by sqrt(log2(e)/lam) so the ESP weight folds into the same exp2.  The sums are
-// rescaled once per pose: dO/dt = K alpha / s * sum(f~), dO/dR = K alpha / s^2 * G~.
+// coordinate (and the translation) is multiplied by s while it is staged in LDS and the ESP value
+// by sqrt(log2(e)/lam), so the Gaussian and the ESP weight are one exp2 of an expanded quadratic
+// form (see shape_accum).  The sums are rescaled once per pose: dO/dt = K alpha / s * sum(f~),
+// dO/dR = K alpha / s^2 * G~.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -91,14 +91,44 @@ __device__ __forceinline__ void group_sum(Acc& a, int width, bool grad) {
 }
 
 // ---------------------------------------------------------------------------------------------
-// Shape channel, FP32, scaled coordinates in LDS.
-//   ref[a] = (s a, sc c_a), fit[b] = (s b, sc c_b);  R unscaled; ts = s t.
-// Lane j of a group of `lpp` lanes owns fit rows b = j, j+lpp, ...; two rows are processed per
-// pass over the ref cloud so one LDS broadcast read feeds two pair terms.
+// Shape channel, FP32, expanded form on scaled coordinates staged in LDS.
+//
+// With s = sqrt(alpha/2 log2 e), sc = sqrt(log2 e / lam) and the ref centroid c as origin:
+//   a~ = s (a - c),  b~' = R (s b) + s t - s c,  c~ = sc * esp
+//   E_ab = exp2(-|a~ - b~'|^2 - (c~_a - c~_b)^2) = exp2(g_ab) * exp2(-h_b)
+//   g_ab = W_a + 2 a~ . b~' + 2 c~_a c~_b,   W_a = -|a~|^2 - c~_a^2,   h_b = |b~'|^2 + c~_b^2
+// LDS rows: ref4[a] = (2 a~, W_a), refc[a] = 2 c~_a (ESP only), fit4[b] = (s b, c~_b).
+// A pair term is then 3-4 FMA + v_exp_f32 + 1 add + 3 FMA; exp2(-h_b) is applied once per fit
+// row.  Since g_ab <= h_b, exp2(g_ab) cannot overflow while h_b <= kHMax; fit rows beyond that
+// radius (about 11.7 A from the ref centroid at alpha 0.81) take the direct form.
+// Lane j of a group of `lpp` lanes owns fit rows b = j, j+lpp, ...; two rows per pass over the
+// ref cloud so one LDS broadcast read feeds two pair terms.
+// Output (per lane, scaled units): V = sum E, gt = sum_b f_b, G = sum_b f_b (s b)^T with
+// f_b = sum_a E_ab (a~ - b~'_b).
 // ---------------------------------------------------------------------------------------------
+constexpr float kHMax = 80.f;
+
+template <bool ESP>
+__device__ __forceinline__ void direct_pair(const float4* __restrict__ ref, const float* __restrict__ refc,
+                                            int N, float x, float y, float z, float cb, float& v,
+                                            float& fx, float& fy, float& fz) {
+  for (int a = 0; a < N; ++a) {
+    const float4 A = ref[a];
+    const float dx = 0.5f * A.x - x, dy = 0.5f * A.y - y, dz = 0.5f * A.z - z;
+    float s2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
+    if (ESP) {
+      const float dc = 0.5f * refc[a] - cb;
+      s2 = fmaf(dc, dc, s2);
+    }
+    const float e = fast_exp2(-s2);
+    v += e;
+    fx = fmaf(e, dx, fx); fy = fmaf(e, dy, fy); fz = fmaf(e, dz, fz);
+  }
+}
+
 template <bool ESP, bool GRAD>
-__device__ __forceinline__ void shape_accum(const float4* __restrict__ ref, int N,
-                                            const float4* __restrict__ fit, int M, int j, int lpp,
+__device__ __forceinline__ void shape_accum(const float4* __restrict__ ref, const float* __restrict__ refc,
+                                            int N, const float4* __restrict__ fit, int M, int j, int lpp,
                                             const float R[9], const float ts[3], Acc& acc) {
   for (int b0 = j; b0 < M; b0 += 2 * lpp) {
     const int b1r = b0 + lpp;
@@ -112,28 +142,43 @@ __device__ __forceinline__ void shape_accum(const float4* __restrict__ ref, int 
     const float x1 = fmaf(R[0], p1.x, fmaf(R[1], p1.y, fmaf(R[2], p1.z, ts[0])));
     const float y1 = fmaf(R[3], p1.x, fmaf(R[4], p1.y, fmaf(R[5], p1.z, ts[1])));
     const float z1 = fmaf(R[6], p1.x, fmaf(R[7], p1.y, fmaf(R[8], p1.z, ts[2])));
+    const float c0 = ESP ? p0.w : 0.f, c1 = ESP ? p1.w : 0.f;
+    const float h0 = fmaf(x0, x0, fmaf(y0, y0, fmaf(z0, z0, c0 * c0)));
+    const float h1 = fmaf(x1, x1, fmaf(y1, y1, fmaf(z1, z1, c1 * c1)));
     float v0 = 0.f, v1 = 0.f;
     float fx0 = 0.f, fy0 = 0.f, fz0 = 0.f, fx1 = 0.f, fy1 = 0.f, fz1 = 0.f;
+    if (fmaxf(h0, h1) <= kHMax) {
+      float sx0 = 0.f, sy0 = 0.f, sz0 = 0.f, sx1 = 0.f, sy1 = 0.f, sz1 = 0.f;
 #pragma unroll 4
-    for (int a = 0; a < N; ++a) {
-      const float4 r = ref[a];
-      const float dx0 = r.x - x0, dy0 = r.y - y0, dz0 = r.z - z0;
-      const float dx1 = r.x - x1, dy1 = r.y - y1, dz1 = r.z - z1;
-      float s0 = fmaf(dx0, dx0, fmaf(dy0, dy0, dz0 * dz0));
-      float s1 = fmaf(dx1, dx1, fmaf(dy1, dy1, dz1 * dz1));
-      if (ESP) {
-        const float c0 = r.w - p0.w, c1 = r.w - p1.w;
-        s0 = fmaf(c0, c0, s0);
-        s1 = fmaf(c1, c1, s1);
+      for (int a = 0; a < N; ++a) {
+        const float4 A = ref[a];
+        float g0 = A.w, g1 = A.w;
+        if (ESP) {
+          const float C = refc[a];
+          g0 = fmaf(C, c0, g0);
+          g1 = fmaf(C, c1, g1);
+        }
+        g0 = fmaf(A.x, x0, fmaf(A.y, y0, fmaf(A.z, z0, g0)));
+        g1 = fmaf(A.x, x1, fmaf(A.y, y1, fmaf(A.z, z1, g1)));
+        const float e0 = fast_exp2(g0);
+        const float e1 = fast_exp2(g1);
+        v0 += e0;
+        v1 += e1;
+        if (GRAD) {
+          sx0 = fmaf(e0, A.x, sx0); sy0 = fmaf(e0, A.y, sy0); sz0 = fmaf(e0, A.z, sz0);
+          sx1 = fmaf(e1, A.x, sx1); sy1 = fmaf(e1, A.y, sy1); sz1 = fmaf(e1, A.z, sz1);
+        }
       }
-      const float e0 = fast_exp2(-s0);
-      const float e1 = fast_exp2(-s1);
-      v0 += e0;
-      v1 += e1;
+      const float F0 = fast_exp2(-h0), F1 = fast_exp2(-h1);
+      v0 *= F0;
+      v1 *= F1;
       if (GRAD) {
-        fx0 = fmaf(e0, dx0, fx0); fy0 = fmaf(e0, dy0, fy0); fz0 = fmaf(e0, dz0, fz0);
-        fx1 = fmaf(e1, dx1, fx1); fy1 = fmaf(e1, dy1, fy1); fz1 = fmaf(e1, dz1, fz1);
+        fx0 = fmaf(0.5f * F0, sx0, -x0 * v0); fy0 = fmaf(0.5f * F0, sy0, -y0 * v0); fz0 = fmaf(0.5f * F0, sz0, -z0 * v0);
+        fx1 = fmaf(0.5f * F1, sx1, -x1 * v1); fy1 = fmaf(0.5f * F1, sy1, -y1 * v1); fz1 = fmaf(0.5f * F1, sz1, -z1 * v1);
       }
+    } else {
+      direct_pair<ESP>(ref, refc, N, x0, y0, z0, c0, v0, fx0, fy0, fz0);
+      direct_pair<ESP>(ref, refc, N, x1, y1, z1, c1, v1, fx1, fy1, fz1);
     }
     if (!has1) { v1 = 0.f; fx1 = fy1 = fz1 = 0.f; }
     acc.V += v0 + v1;

@@ -110,9 +110,10 @@ int check_batch(const FssPairBatch* pb, int mode) {
   return FSS_OK;
 }
 
-// LDS plan for one workgroup: shape ref/fit rows then pharm ref anc/vec, fit anc/vec.
+// LDS plan for one workgroup (float4 offsets): shape ref rows, fit rows, ref ESP column,
+// a 16-float4 reduction scratch, then pharm ref anchors / vectors, fit anchors / vectors.
 struct LdsPlan {
-  int ref, fit, ranc, rvec, fanc, fvec, end;  // float4 offsets
+  int ref, fit, refc, red, ranc, rvec, fanc, fvec, end;
 };
 LdsPlan plan_lds(const FssPairBatch* pb, int mode) {
   LdsPlan p;
@@ -120,6 +121,8 @@ LdsPlan plan_lds(const FssPairBatch* pb, int mode) {
   const bool sh = uses_shape(mode), ph = uses_pharm(mode);
   p.ref = o; o += sh ? pb->max_ref : 0;
   p.fit = o; o += sh ? pb->max_fit : 0;
+  p.refc = o; o += mode == FSS_MODE_SHAPE_ESP ? (pb->max_ref + 3) / 4 : 0;
+  p.red = o; o += 16;
   p.ranc = o; o += ph ? pb->max_ref_ph : 0;
   p.rvec = o; o += ph ? pb->max_ref_ph : 0;
   p.fanc = o; o += ph ? pb->max_fit_ph : 0;
@@ -136,7 +139,8 @@ size_t max_lds_bytes() {
 }
 
 struct Staged {
-  const float4* ref; const float4* fit; int N, M;
+  const float4* ref; const float* refc; const float4* fit; int N, M;
+  float cs[3];  // scaled ref centroid: the origin of the expanded shape form
   const float4* ranc; const float4* rvec; const float4* fanc; const float4* fvec; int Np, Mp;
 };
 
@@ -161,16 +165,35 @@ __device__ __forceinline__ Staged stage_pair(const FssPairBatch& pb, const KScor
     const float4* gb = reinterpret_cast<const float4*>(pbp) + b0;
     float4* la = lds + lp.ref;
     float4* lb = lds + lp.fit;
+    float* lc = reinterpret_cast<float*>(lds + lp.refc);
+    float* red = reinterpret_cast<float*>(lds + lp.red);
     const float s = ks.s, sc = ks.sc;
+    // ref centroid (wave butterflies, then one value set per wave through LDS)
+    float cx = 0.f, cy = 0.f, cz = 0.f;
     for (int i = tid; i < N; i += nt) {
       const float4 v = ga[i];
-      la[i] = make_float4(v.x * s, v.y * s, v.z * s, ESP ? v.w * sc : 0.f);
+      cx += v.x; cy += v.y; cz += v.z;
+    }
+    cx = group_sum(cx, 64); cy = group_sum(cy, 64); cz = group_sum(cz, 64);
+    if ((tid & 63) == 0) { red[3 * (tid >> 6)] = cx; red[3 * (tid >> 6) + 1] = cy; red[3 * (tid >> 6) + 2] = cz; }
+    __syncthreads();
+    cx = cy = cz = 0.f;
+    for (int w = 0; w < (nt >> 6); ++w) { cx += red[3 * w]; cy += red[3 * w + 1]; cz += red[3 * w + 2]; }
+    const float invN = N > 0 ? 1.f / (float)N : 0.f;
+    cx *= invN; cy *= invN; cz *= invN;
+    for (int i = tid; i < N; i += nt) {
+      const float4 v = ga[i];
+      const float ax = s * (v.x - cx), ay = s * (v.y - cy), az = s * (v.z - cz);
+      const float cc = ESP ? v.w * sc : 0.f;
+      la[i] = make_float4(2.f * ax, 2.f * ay, 2.f * az, -fmaf(ax, ax, fmaf(ay, ay, fmaf(az, az, cc * cc))));
+      if (ESP) lc[i] = 2.f * cc;
     }
     for (int i = tid; i < M; i += nt) {
       const float4 v = gb[i];
       lb[i] = make_float4(v.x * s, v.y * s, v.z * s, ESP ? v.w * sc : 0.f);
     }
-    st.ref = la; st.fit = lb; st.N = N; st.M = M;
+    st.ref = la; st.refc = lc; st.fit = lb; st.N = N; st.M = M;
+    st.cs[0] = s * cx; st.cs[1] = s * cy; st.cs[2] = s * cz;
   }
   if (PH) {
     const int* ra = ref_side_a ? pb.fit_prng : pb.ref_prng;
@@ -348,8 +371,8 @@ __global__ __launch_bounds__(kPoseBlock) void k_pose_grad(FssPairBatch pb, KScor
   sh.zero();
   ph.zero();
   if (SH) {
-    const float ts[3] = {t[0] * ks.s, t[1] * ks.s, t[2] * ks.s};
-    shape_accum<ESP, GRAD>(st.ref, st.N, st.fit, st.M, threadIdx.x, blockDim.x, R, ts, sh);
+    const float ts[3] = {fmaf(t[0], ks.s, -st.cs[0]), fmaf(t[1], ks.s, -st.cs[1]), fmaf(t[2], ks.s, -st.cs[2])};
+    shape_accum<ESP, GRAD>(st.ref, st.refc, st.N, st.fit, st.M, threadIdx.x, blockDim.x, R, ts, sh);
     block_sum(sh, red[0], GRAD);
   }
   if (PH) {
@@ -471,8 +494,8 @@ __global__ __launch_bounds__(MAXT) void k_align(FssPairBatch pb, KScore ks, LdsP
       sh.zero();
       ph.zero();
       if (SH) {
-        const float ts[3] = {p[4] * ks.s, p[5] * ks.s, p[6] * ks.s};
-        shape_accum<ESP, true>(st.ref, st.N, st.fit, st.M, j, lpp, R, ts, sh);
+        const float ts[3] = {fmaf(p[4], ks.s, -st.cs[0]), fmaf(p[5], ks.s, -st.cs[1]), fmaf(p[6], ks.s, -st.cs[2])};
+        shape_accum<ESP, true>(st.ref, st.refc, st.N, st.fit, st.M, j, lpp, R, ts, sh);
         group_sum(sh, lpp, true);
         shape_finish(sh, ks.K, ks.alpha, ks.s, true);
       }
