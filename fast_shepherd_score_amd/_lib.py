@@ -102,6 +102,7 @@ SIGNATURES = {
                                       _P, _P]),
     "fss_align": (ctypes.c_int, [_P, _P, _P, _P, _P, _P]),
     "fss_align_finish": (ctypes.c_int, [_P, _P, ctypes.c_int32, _P, _P, _P]),
+    "fss_seed_poses": (ctypes.c_int, [_P, ctypes.c_int32, ctypes.c_int32, _P, _P, _P, _P]),
     "fss_score_fwd": (ctypes.c_int, [_P, _P, _P, _P, _P]),
     "fss_score_bwd": (ctypes.c_int, [_P, _P, _P, _P, _P, _P, _P]),
     "fss_score_fwd_f64": (ctypes.c_int, [_P, _P, _P, _P, _P, _P, _P]),

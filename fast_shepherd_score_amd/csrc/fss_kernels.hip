@@ -828,6 +828,32 @@ int pose_score_impl(const FssPairBatch* batch, const KScore& ks, int32_t n_poses
   return check_launch("fss_pose_score");
 }
 
+// Restart seeds on device: q_r = table[r], t_r = com(ref) - R(q_r) com(fit) (one wave per pair).
+// use_pharm: centroids of the pharmacophore anchors instead of the shape clouds.
+__global__ __launch_bounds__(64) void k_seed_poses(FssPairBatch pb, int use_pharm, int B,
+                                                   const float* __restrict__ table, float* q, float* t) {
+  const int pair = blockIdx.x, j = threadIdx.x;
+  const int* rr = use_pharm ? pb.ref_prng : pb.ref_rng;
+  const int* fr = use_pharm ? pb.fit_prng : pb.fit_rng;
+  const float4* rp = reinterpret_cast<const float4*>(use_pharm ? pb.ref_anc : pb.ref_pts) + rr[2 * pair];
+  const float4* fp = reinterpret_cast<const float4*>(use_pharm ? pb.fit_anc : pb.fit_pts) + fr[2 * pair];
+  const int N = rr[2 * pair + 1], M = fr[2 * pair + 1];
+  double c[6] = {0, 0, 0, 0, 0, 0};
+  for (int i = j; i < N; i += 64) { const float4 v = rp[i]; c[0] += v.x; c[1] += v.y; c[2] += v.z; }
+  for (int i = j; i < M; i += 64) { const float4 v = fp[i]; c[3] += v.x; c[4] += v.y; c[5] += v.z; }
+  for (int k = 0; k < 6; ++k) c[k] = group_sum(c[k], 64);
+  const double iN = N > 0 ? 1.0 / N : 0.0, iM = M > 0 ? 1.0 / M : 0.0;
+  for (int r = j; r < B; r += 64) {
+    const float qq[4] = {table[4 * r], table[4 * r + 1], table[4 * r + 2], table[4 * r + 3]};
+    double qh[4], qn, R[9];
+    rotation_fp64(qq, qh, qn, R);
+    const long pose = (long)pair * B + r;
+    for (int k = 0; k < 4; ++k) q[4 * pose + k] = qq[k];
+    for (int i = 0; i < 3; ++i)
+      t[3 * pose + i] = (float)(c[i] * iN - (R[3 * i] * c[3] + R[3 * i + 1] * c[4] + R[3 * i + 2] * c[5]) * iM);
+  }
+}
+
 int align_finish_impl(const FssPairBatch* batch, const KScore& ks, int B, const double* self_ov,
                       FssAlignOut* out, hipStream_t s) {
   int rc = pose_score_impl(batch, ks, batch->n_pairs * B, nullptr, B, self_ov, out->q, out->t, out->pose_score, s);
@@ -1029,6 +1055,19 @@ int fss_align_finish(const FssPairBatch* batch, const FssScoreParams* sp, int32_
   if (n_restarts <= 0) return set_error(FSS_E_INVALID, "n_restarts must be > 0");
   if (batch->n_pairs == 0) return FSS_OK;
   return align_finish_impl(batch, ks, n_restarts, self_ov, out, (hipStream_t)stream);
+}
+
+int fss_seed_poses(const FssPairBatch* batch, int32_t use_pharm, int32_t n_restarts,
+                   const float* quat_table, float* q, float* t, void* stream) {
+  if (!batch || !quat_table || !q || !t) return set_error(FSS_E_INVALID, "NULL seed arguments");
+  if (n_restarts <= 0) return set_error(FSS_E_INVALID, "n_restarts must be > 0");
+  if (use_pharm ? (!batch->ref_prng || !batch->fit_prng || !batch->ref_anc || !batch->fit_anc)
+                : (!batch->ref_rng || !batch->fit_rng || !batch->ref_pts || !batch->fit_pts))
+    return set_error(FSS_E_INVALID, "cloud arrays are NULL");
+  if (batch->n_pairs <= 0) return FSS_OK;
+  hipLaunchKernelGGL(k_seed_poses, dim3(batch->n_pairs), dim3(64), 0, (hipStream_t)stream, *batch,
+                     use_pharm, n_restarts, quat_table, q, t);
+  return check_launch("fss_seed_poses");
 }
 
 int fss_score_fwd(const FssPairBatch* batch, const FssScoreParams* sp, float* S, float* ov, void* stream) {

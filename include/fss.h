@@ -202,6 +202,15 @@ int fss_align_finish(const FssPairBatch* batch, const FssScoreParams* sp, int32_
                      const double* self_ov, FssAlignOut* out, void* stream);
 
 /*
+ * Restart seeds on device: pose r of every pair gets quat_table[r] (n_restarts x 4, w x y z) and
+ * t = centroid(ref) - R(q) centroid(fit), the centre-of-mass alignment of every seed of
+ * alignment/_torch.py:253-326 (centroids of the pharmacophore anchors when use_pharm != 0).
+ * q (n_pairs*B, 4), t (n_pairs*B, 3).
+ */
+int fss_seed_poses(const FssPairBatch* batch, int32_t use_pharm, int32_t n_restarts,
+                   const float* quat_table, float* q, float* t, void* stream);
+
+/*
  * Differentiable similarity of fixed clouds (score-as-loss).  fit rows are the moved points.
  *   fwd: S (n_pairs,) and, for reuse by bwd, ov (n_pairs, 4) = [VAB, VAA, VBB, 0].
  *   bwd: dS/dfit (fit rows, 4 floats: x, y, z, 0) scaled by gS (n_pairs,), including the

@@ -1,0 +1,85 @@
+"""Score-as-loss: the hand-written HIP backward vs finite differences (gradcheck, FP64), vs
+autograd of the FP64 oracle's scoring restatement, and contracted to SE(3) vs the oracle's
+analytic gradient (needs an MI355X)."""
+import numpy as np
+import pytest
+import torch
+
+from fast_shepherd_score_amd import synth
+from fast_shepherd_score_amd.autograd import gaussian_similarity
+from fast_shepherd_score_amd.constants import LAM_SCALING
+from oracle import fss_oracle as orc
+
+pytestmark = pytest.mark.gpu
+LAM = 0.3 * LAM_SCALING
+
+
+def _clouds(P, N, M, seed, dtype):
+    rng = np.random.default_rng(seed)
+    ref = rng.normal(size=(P, N, 3)) * 1.5
+    fit = rng.normal(size=(P, M, 3)) * 1.5
+    re = np.clip(rng.normal(0, 0.1, (P, N)), -0.3, 0.3)
+    fe = np.clip(rng.normal(0, 0.1, (P, M)), -0.3, 0.3)
+    t = lambda a: torch.tensor(a, dtype=dtype)
+    return t(ref), t(fit), t(re), t(fe)
+
+
+@pytest.mark.parametrize("esp", [False, True])
+def test_gradcheck_fp64(dev, esp):
+    ref, fit, re, fe = (x.to(dev) for x in _clouds(3, 8, 7, 0, torch.float64))
+    ref.requires_grad_(True)
+    fit.requires_grad_(True)
+    # a small lam makes the ESP weights matter at these charge scales
+    lam = 0.05 if esp else LAM
+    f = lambda r, x: gaussian_similarity(r, x, re if esp else None, fe if esp else None, 0.81, lam)
+    assert torch.autograd.gradcheck(f, (ref, fit), eps=1e-6, atol=1e-8, rtol=1e-6)
+
+
+@pytest.mark.parametrize("esp", [False, True])
+def test_fp32_matches_oracle_autograd(dev, esp):
+    P, N, M = 16, 300, 280
+    ref64, fit64, re64, fe64 = _clouds(P, N, M, 1, torch.float64)
+    lam = 0.5 if esp else LAM
+    x = fit64.clone().requires_grad_(True)
+    r = ref64.clone().requires_grad_(True)
+    if esp:
+        S_ref = orc.tanimoto_esp(r, x, re64, fe64, 0.81, lam)
+    else:
+        S_ref = orc.tanimoto_shape(r, x, 0.81)
+    S_ref.sum().backward()
+    ref, fit = ref64.float().to(dev).requires_grad_(True), fit64.float().to(dev).requires_grad_(True)
+    S = gaussian_similarity(ref, fit, re64.float().to(dev) if esp else None,
+                            fe64.float().to(dev) if esp else None, 0.81, lam)
+    S.sum().backward()
+    np.testing.assert_allclose(S.detach().cpu().numpy(), S_ref.detach().numpy(), atol=1e-5)
+    for got, want in ((fit.grad, x.grad), (ref.grad, r.grad)):
+        g, w = got.cpu().numpy(), want.numpy()
+        np.testing.assert_allclose(g, w, atol=1e-4 * np.abs(w).max())
+
+
+def test_contraction_to_se3_matches_oracle(dev):
+    """Moving the fit rigidly: the coordinate gradient contracted through b' = R b + t equals the
+    oracle's SE(3) gradient (after the tangent projection both apply)."""
+    rs, rse, fs, fse = synth.surf_esp_pairs(1, atoms=(8, 12), points=(40, 60), seed=4)
+    pair = orc.Pair("esp", ref=rs[0], fit=fs[0], ref_esp=rse[0], fit_esp=fse[0], lam=LAM)
+    rng = np.random.default_rng(5)
+    q = synth.random_quaternions(rng, 1).astype(np.float64)
+    q /= np.linalg.norm(q)
+    t = rng.normal(0, 0.4, (1, 3))
+    se3 = torch.tensor(np.concatenate([q, t], 1))
+    _, g_or, _ = pair.loss_grad_se3(se3, pair.self_overlaps())   # d(1-S)/d(q,t), B = 1
+    R = orc.rot_from_unit_quat(torch.tensor(q))[0]
+    fit0 = torch.tensor(fs[0], dtype=torch.float64)
+    moved = (fit0 @ R.T + torch.tensor(t[0])).unsqueeze(0).to(dev).requires_grad_(True)
+    S = gaussian_similarity(torch.tensor(rs[0], dtype=torch.float64)[None].to(dev), moved,
+                            torch.tensor(rse[0], dtype=torch.float64)[None].to(dev),
+                            torch.tensor(fse[0], dtype=torch.float64)[None].to(dev), 0.81, LAM)
+    (1 - S).sum().backward()
+    gb = moved.grad[0].cpu()                       # dL/db' (M, 3)
+    gt = gb.sum(0)
+    GR = gb.T @ fit0                               # dL/dR as a general matrix
+    qh = torch.tensor(q)
+    gq = orc.project_grad_R(GR[None], qh)[0]
+    gq = gq - qh[0] * (qh[0] @ gq)                 # tangent part (unit q: |q| = 1)
+    np.testing.assert_allclose(gt.numpy(), g_or[0, 4:].numpy(), atol=1e-9)
+    np.testing.assert_allclose(gq.numpy(), g_or[0, :4].numpy(), atol=1e-9)

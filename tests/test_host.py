@@ -1,0 +1,154 @@
+"""Host-side logic (no GPU): seeds, packing, tiling model, synthetic generators, container API."""
+import numpy as np
+import pytest
+import torch
+
+from fast_shepherd_score_amd import _lib, synth
+from fast_shepherd_score_amd.align import (ScoreSpec, candidate_tilings, check_tiling, default_tiling,
+                                           tiling_efficiency)
+from fast_shepherd_score_amd.constants import ALPHA, LAM_SCALING, P_TYPES, pharm_tables
+from fast_shepherd_score_amd.container import MoleculePairBatch, resolve_backend
+from fast_shepherd_score_amd.modes import MODE_ATTRS, MODE_SEEDS, MODE_STEPS, canonical
+from fast_shepherd_score_amd.packing import CloudSet, PairBatch, PharmSet
+from fast_shepherd_score_amd.seeds import _principal_axes, initial_seeds, pad, quats_from_fibo, rotmat
+
+
+# ------------------------------------------------------------------------------- seeds
+def test_seed_zero_is_identity_with_com_alignment():
+    ref, fit = synth.vol_pairs(3, 10, 20, seed=1)
+    fit = [f + np.array([1.0, -2.0, 0.5], np.float32) for f in fit]
+    (R, rn), (F, fn) = pad(ref), pad(fit)
+    q, t = initial_seeds(R, rn, F, fn, 50)
+    assert q.shape == (3, 50, 4) and t.shape == (3, 50, 3)
+    np.testing.assert_allclose(q[:, 0], np.tile([1, 0, 0, 0], (3, 1)))
+    for i in range(3):
+        np.testing.assert_allclose(t[i, 0], ref[i].mean(0) - fit[i].mean(0), atol=1e-5)
+        # every seed puts the fit centroid on the ref centroid
+        moved = np.einsum("sij,j->si", rotmat(q[i].astype(np.float64)), fit[i].mean(0)) + t[i]
+        np.testing.assert_allclose(moved, np.tile(ref[i].mean(0), (50, 1)), atol=1e-4)
+
+
+def test_pca_seed_aligns_principal_axes():
+    """Seed 1 rotates the fit's principal axes onto the ref's (up to the axis sign flips)."""
+    rng = np.random.default_rng(2)
+    ref = rng.normal(size=(40, 3)) * np.array([4.0, 2.0, 0.7])
+    q = synth.random_quaternions(rng, 1).astype(np.float64)
+    fit = (ref - ref.mean(0)) @ rotmat(q)[0].T
+    (R, rn), (F, fn) = pad([ref]), pad([fit])
+    qs, _ = initial_seeds(R, rn, F, fn, 5)
+    ax_ref = _principal_axes(R, rn)[0]
+    for s in range(1, 5):
+        Rs = rotmat(qs[0, s].astype(np.float64))
+        moved = fit @ Rs.T
+        ax_fit = _principal_axes(moved[None], fn)[0]
+        for k in range(3):
+            assert abs(abs(ax_fit[k] @ ax_ref[k]) - 1.0) < 1e-4
+
+
+def test_seed_count_rules():
+    ref, fit = synth.vol_pairs(1, 8, 8, seed=3)
+    (R, rn), (F, fn) = pad(ref), pad(fit)
+    assert initial_seeds(R, rn, F, fn, 1)[0].shape[1] == 1
+    assert initial_seeds(R, rn, F, fn, 3)[0].shape[1] == 5       # 1 < S < 5 is raised to 5
+    q, _ = initial_seeds(R, rn, F, fn, 12)
+    np.testing.assert_allclose(q[0, 5:], quats_from_fibo(7), atol=1e-7)
+
+
+# ------------------------------------------------------------------------------- packing
+def test_cloudset_and_pairbatch_ranges():
+    xyz = [np.ones((3, 3)), np.zeros((0, 3)), 2 * np.ones((5, 3))]
+    cs = CloudSet.from_arrays(xyz, [np.arange(3), None, np.arange(5)], device="cpu")
+    assert cs.rng.tolist() == [[0, 3], [3, 0], [3, 5]]
+    assert cs.pts.shape == (8, 4) and cs.pts[4, 3] == 1.0
+    pb = PairBatch(cs, cs, ref_idx=[2, 0], fit_idx=[0, 2])
+    assert pb.n_pairs == 2 and pb.ref_rng.tolist() == [[3, 5], [0, 3]]
+    assert pb.cost(_lib.MODE_SHAPE).tolist() == [15.0, 15.0]
+    s = pb.struct()
+    assert s.max_ref == 5 and s.max_fit == 5 and s.n_pairs == 2
+    ph = PharmSet.from_arrays([np.array([0, 2]), np.array([7])], [np.zeros((2, 3)), np.ones((1, 3))],
+                              [np.eye(3)[:2], np.zeros((1, 3))], device="cpu")
+    assert ph.typ.tolist() == [0, 2, 7] and ph.rng.tolist() == [[0, 2], [2, 1]]
+
+
+def test_padded_packing_equals_list_packing():
+    rng = np.random.default_rng(0)
+    counts = np.array([3, 1, 4])
+    P = rng.normal(size=(3, 4, 3)).astype(np.float32)
+    W = rng.normal(size=(3, 4)).astype(np.float32)
+    a = CloudSet.from_padded(P, counts, W, device="cpu")
+    b = CloudSet.from_arrays([P[i, :counts[i]] for i in range(3)], [W[i, :counts[i]] for i in range(3)], device="cpu")
+    assert torch.equal(a.pts, b.pts) and torch.equal(a.rng, b.rng)
+
+
+# ------------------------------------------------------------------------------- tiling
+def test_tiling_choices_are_legal():
+    rng = np.random.default_rng(0)
+    for n, lo, hi, B, es in [(64, 20, 40, 50, True), (4096, 200, 400, 50, False), (3, 1, 5, 1, True),
+                             (1000, 10, 50, 10, True), (10, 2, 20, 32, False)]:
+        fit = [np.zeros((int(k), 3)) for k in rng.integers(lo, hi + 1, n)]
+        ref = [np.zeros((int(k), 3)) for k in rng.integers(lo, hi + 1, n)]
+        pb = PairBatch(CloudSet.from_arrays(ref, device="cpu"), CloudSet.from_arrays(fit, device="cpu"))
+        lpp, rpw = default_tiling(pb, _lib.MODE_SHAPE, B, es)
+        check_tiling(B, lpp, rpw, es)
+        assert 0 < tiling_efficiency(pb.fit_counts, B, lpp, rpw) <= 1
+    with pytest.raises(ValueError):
+        check_tiling(50, 3, 50, True)
+    with pytest.raises(ValueError):
+        check_tiling(50, 16, 10, True)
+    assert all(l * r <= 1024 for l, r in candidate_tilings(50, False))
+
+
+# ------------------------------------------------------------------------------- synth
+def test_random_walk_geometry():
+    rng = np.random.default_rng(0)
+    mols = synth.random_walks(rng, [30, 5, 1])
+    for m in mols:
+        np.testing.assert_allclose(m.mean(0), 0, atol=1e-5)
+        if len(m) > 1:
+            steps = np.linalg.norm(np.diff(m, axis=0), axis=1)
+            np.testing.assert_allclose(steps, 1.5, atol=1e-4)
+            d = np.linalg.norm(m[:, None] - m[None], axis=-1) + 10 * np.eye(len(m))
+            assert d.min() >= 1.2 - 1e-4
+
+
+def test_surface_and_esp_and_pharm():
+    rng = np.random.default_rng(1)
+    atoms = synth.random_walks(rng, [12])[0].astype(np.float64)
+    pts = synth.surface_points(rng, atoms, 150).astype(np.float64)
+    d = np.linalg.norm(pts[:, None] - atoms[None], axis=-1)
+    r = 1.2 * 1.7
+    assert (d.min(1) >= r - 1e-4).all() and np.isclose(d.min(1), r, atol=1e-4).all()
+    e = synth.esp_values(rng, 1000)
+    assert e.min() >= -0.3 and e.max() <= 0.3 and abs(e.std() - 0.1) < 0.02
+    t, a, v = synth.pharmacophores(rng, atoms.astype(np.float32), 50)
+    assert t.min() >= 0 and t.max() <= 7
+    directional = np.isin(t, synth.DIRECTIONAL_TYPE_IDS)
+    np.testing.assert_allclose(np.linalg.norm(v[directional], axis=1), 1, atol=1e-5)
+    assert (v[~directional] == 0).all()
+
+
+# ------------------------------------------------------------------------------- constants / API
+def test_constants_and_tables():
+    assert P_TYPES[-1] == "Dummy" and len(P_TYPES) == 9
+    assert LAM_SCALING == pytest.approx(207.3535, rel=1e-6)
+    a, k, c = pharm_tables()
+    assert c == [1, 1, 2, 0, 1, 0, 0, 0, 3]
+    assert pharm_tables(directionless=True)[2] == [0, 0, 0, 0, 0, 0, 0, 0, 3]
+    assert a[2] == 0.7 and a[3] == 0.7 and a[0] == 1.0
+    assert ALPHA(200) == pytest.approx(1.118, abs=1e-9)
+    s = ScoreSpec(similarity="tversky_fit").struct()
+    assert s.similarity == _lib.SIM_TVERSKY and s.sigma == pytest.approx(0.05)
+    with pytest.raises(ValueError):
+        ScoreSpec(similarity="dice").struct()
+
+
+def test_mode_registry_and_backend_names():
+    assert canonical("esp") == "surf_esp" and canonical("esp_combo") == "vol_and_surf_esp"
+    assert set(MODE_ATTRS) == set(MODE_SEEDS) == set(MODE_STEPS)
+    for b in (None, "hip", "triton", "cuda", "gpu"):
+        assert resolve_backend(b) == "hip"
+    for b in ("numba", "cpu", "jax", "nope"):
+        with pytest.raises(ValueError):
+            resolve_backend(b)
+    with pytest.raises(NotImplementedError):
+        MoleculePairBatch([]).align_with_vol_and_surf_esp()
