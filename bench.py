@@ -94,6 +94,120 @@ def cpu_baseline(wl, q, t, B, steps, sample: int, spec, opt):
             "seconds": dt}, np.array(r["score"])
 
 
+def bench_c1(dev, reps: int = 3):
+    """64 atom pairs (20-40 atoms), 50 restarts x 200 steps: persistent vs graph-stepped vs plain."""
+    import torch
+    from fast_shepherd_score_amd.align import OptimSpec, ScoreSpec, align, align_plain, SteppedAligner
+    from fast_shepherd_score_amd.packing import CloudSet, PairBatch
+    wl = build_workload("vol", 64, 3)
+    q, t = seeds_for(wl, 50, 3)
+    pb = PairBatch(CloudSet.from_arrays(wl["ref"], device=dev), CloudSet.from_arrays(wl["fit"], device=dev))
+    spec, opt = ScoreSpec(), OptimSpec(max_steps=200, early_stop=False)
+    qd, td = torch.from_numpy(q).to(dev), torch.from_numpy(t).to(dev)
+
+    def timed(fn, n):
+        fn()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(n):
+            r = fn()
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t0) / n, r
+
+    tp, rp = timed(lambda: align(pb, spec, qd, td, opt), reps)
+    st = SteppedAligner(pb, spec, qd, td, opt)
+    st.capture()
+
+    def graph_run():
+        st.q.copy_(qd.reshape(-1, 4))
+        st.t.copy_(td.reshape(-1, 3))
+        st.m.zero_()
+        st.v.zero_()
+        return st.run(use_graph=True)
+
+    tg, rg = timed(graph_run, reps)
+    tl, rl = timed(lambda: align_plain(pb, spec, qd, td, opt), 1)
+    sp, sg, sl = (r.score.cpu().numpy() for r in (rp, rg, rl))
+    return {"workload": "64 atom pairs (20-40 heavy atoms), 50 restarts x 200 steps (all executed)",
+            "persistent_pairs_per_s": 64 / tp, "graph_stepped_pairs_per_s": 64 / tg,
+            "plain_per_pair_launch_pairs_per_s": 64 / tl, "persistent_vs_plain": tl / tp,
+            "persistent_vs_graph_stepped": tg / tp,
+            "max_abs_score_diff_between_paths": float(max(np.abs(sp - sg).max(), np.abs(sp - sl).max()))}
+
+
+def bench_c3(dev, n_pairs: int = 16384):
+    """Uneven shape + directional pharmacophore pairs in one persistent launch."""
+    import torch
+    from fast_shepherd_score_amd import _lib, synth
+    from fast_shepherd_score_amd.align import OptimSpec, ScoreSpec, align, self_overlaps
+    from fast_shepherd_score_amd.packing import CloudSet, PairBatch, PharmSet
+    from fast_shepherd_score_amd.seeds import rotmat
+    d = synth.pharm_shape_pairs(n_pairs, seed=4)
+    pb = PairBatch(CloudSet.from_arrays(d["ref_atoms"], device=dev), CloudSet.from_arrays(d["fit_atoms"], device=dev),
+                   PharmSet.from_arrays(d["ref_types"], d["ref_anchors"], d["ref_vectors"], device=dev),
+                   PharmSet.from_arrays(d["fit_types"], d["fit_anchors"], d["fit_vectors"], device=dev))
+    B = 50
+    rng = np.random.default_rng(44)
+    q = synth.random_quaternions(rng, n_pairs * B).reshape(n_pairs, B, 4)
+    rc = np.stack([a.mean(0) for a in d["ref_atoms"]]).astype(np.float64)
+    fc = np.stack([a.mean(0) for a in d["fit_atoms"]]).astype(np.float64)
+    t = (rc[:, None] - np.einsum("nbij,nj->nbi", rotmat(q.astype(np.float64)), fc)).astype(np.float32)
+    spec = ScoreSpec(mode=_lib.MODE_SHAPE_PHARM, color_weight=0.5, directionless=False)
+    opt = OptimSpec(max_steps=200, early_stop=False)
+    qd, td = torch.from_numpy(q).to(dev), torch.from_numpy(t).to(dev)
+    align(pb, spec, qd, td, opt)
+    torch.cuda.synchronize()
+    ev = []
+    t0 = time.perf_counter()
+    res = align(pb, spec, qd, td, opt, self_ov=self_overlaps(pb, spec), events=ev)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    shape_terms = float((pb.ref_counts * pb.fit_counts).sum()) * B * 200
+    ph_terms = float((pb.ref_pcounts * pb.fit_pcounts).sum()) * B * 200
+    kms = ev[0][0].elapsed_time(ev[0][1])
+    return {"workload": f"{n_pairs} uneven pairs: 5-60 atoms, 2-20 pharmacophores (8 types, directional), "
+                        "(1-w) shape + w pharm Tanimoto, w=0.5, 50 restarts x 200 steps, one persistent launch",
+            "pairs_per_s": n_pairs / dt, "kernel_ms": kms, "lanes_per_pose": res.extra["lanes_per_pose"],
+            "restarts_per_wg": res.extra["restarts_per_wg"],
+            "achieved_tflops": (shape_terms * 16 + ph_terms * 36) / (kms * 1e-3) / 1e12,
+            "score_mean": float(res.score.mean())}
+
+
+def bench_screen(dev, n_lib: int = 1_000_000, n_queries: int = 24, load: float = 0.7, seed: int = 6):
+    """Screening service: 1M-conformer library in HBM, Poisson query stream, p50/p99 latency."""
+    from fast_shepherd_score_amd import synth
+    from fast_shepherd_score_amd.packing import CloudSet
+    from fast_shepherd_score_amd.screen import ConformerLibrary, ScreenService
+    t0 = time.perf_counter()
+    flat, counts = synth.conformer_library(n_lib, seed=seed)
+    lib = ConformerLibrary(CloudSet.from_flat(flat, counts, device=dev))
+    rng = np.random.default_rng(seed + 1)
+    qmols = synth.random_walks(rng, rng.integers(10, 51, size=n_queries + 1))
+    log(f"screen library {n_lib} built in {time.perf_counter() - t0:.1f} s")
+    with ScreenService(lib, "vol", num_repeats=10, max_num_steps=100, top_k=100,
+                       launch_pairs=262144) as svc:
+        t1 = time.perf_counter()
+        svc.submit({"points": qmols[-1]}).result()
+        solo = time.perf_counter() - t1
+        rate = load / solo
+        arrivals = np.cumsum(rng.exponential(1.0 / rate, size=n_queries))
+        futs = []
+        start = time.perf_counter()
+        for k in range(n_queries):
+            while time.perf_counter() - start < arrivals[k]:
+                time.sleep(0.0005)
+            futs.append(svc.submit({"points": qmols[k]}))
+        res = [f.result() for f in futs]
+        total = time.perf_counter() - start
+    lat = np.array([r.latency_ms for r in res])
+    return {"workload": f"{n_lib} conformers (10-50 atoms) resident in HBM, vol shape, 10 restarts x "
+                        f"100 steps, per-pair early stop, top-100; {n_queries} queries, Poisson arrivals "
+                        f"at {load:.0%} of single-query capacity",
+            "pairs_per_s": n_queries * n_lib / total, "p50_ms": float(np.percentile(lat, 50)),
+            "p99_ms": float(np.percentile(lat, 99)), "solo_query_ms": solo * 1e3,
+            "library_bytes": lib.nbytes(), "launches": svc.launches}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -108,6 +222,10 @@ def main():
     ap.add_argument("--lpp", type=int, default=0)
     ap.add_argument("--rpw", type=int, default=0)
     ap.add_argument("--seed", type=int, default=1)
+    ap.add_argument("--no-extra", action="store_true", help="skip the C1 / C3 / screening legs")
+    ap.add_argument("--screen-lib", type=int, default=1_000_000)
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="first run a plain 256 MiB device copy (HBM counter calibration for rocprofv3 --pmc)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: every box of this deployment has a single MI355X and the "
@@ -122,6 +240,12 @@ def main():
 
     _lib.load()
     dev = torch.device("cuda:0")
+    if args.copy_calibration:
+        src = torch.ones(64 << 20, dtype=torch.float32, device=dev)
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        log("copy calibration: 268435456 bytes read + 268435456 bytes written")
     B, steps = args.restarts, args.opt_steps
     t_gen = time.perf_counter()
     wl = build_workload(args.workload, args.pairs, args.seed)
@@ -183,6 +307,15 @@ def main():
                      "exp_frac": terms / (kern_ms * 1e-3) / PEAK_EXP_PER_S},
         "score_mean": float(score.mean()),
     }
+    if not args.no_extra:
+        out["c1_paths"] = bench_c1(dev)
+        log("c1 done")
+        out["c3_pharm_shape"] = bench_c3(dev)
+        log("c3 done")
+        sc = bench_screen(dev, n_lib=args.screen_lib)
+        out["screen"] = sc
+        out["screen_p50_ms"], out["screen_p99_ms"] = sc["p50_ms"], sc["p99_ms"]
+        log("screen done")
     if not args.no_cpu:
         cb, cpu_scores = cpu_baseline(wl, q, t, B, steps, args.cpu_sample, spec, opt)
         cb.pop("seconds")

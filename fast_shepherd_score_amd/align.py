@@ -181,9 +181,12 @@ def _prep_seeds(pb: PairBatch, q0: torch.Tensor, t0: torch.Tensor):
 def align(pb: PairBatch, spec: ScoreSpec, q0: torch.Tensor, t0: torch.Tensor,
           opt: OptimSpec = OptimSpec(), *, lanes_per_pose: Optional[int] = None,
           restarts_per_wg: Optional[int] = None, order: str = "cost",
-          self_ov: Optional[torch.Tensor] = None, stream=None, events: Optional[list] = None) -> AlignResult:
+          self_ov: Optional[torch.Tensor] = None, stream=None, events: Optional[list] = None,
+          tune: bool = False) -> AlignResult:
     """Persistent path: every step of every pair in one launch.
 
+    Tiling: explicit ``lanes_per_pose`` / ``restarts_per_wg``, else an autotuned choice
+    (``tune=True``, or one cached for this batch profile), else the cost model's choice.
     ``events``: if a list is given, HIP events bracketing the optimiser kernel and the finishing
     kernels are appended to it as (start, optimiser_end, finish_end).
     """
@@ -192,7 +195,13 @@ def align(pb: PairBatch, spec: ScoreSpec, q0: torch.Tensor, t0: torch.Tensor,
     dev = pb.device
     if self_ov is None:
         self_ov = self_overlaps(pb, spec, stream)
-    lpp0, rpw0 = default_tiling(pb, spec.mode, B, opt.early_stop)
+    if lanes_per_pose is None and restarts_per_wg is None:
+        from . import autotune as _at
+        tuned = _at.autotune(pb, spec, q0, t0, opt.early_stop, self_ov=self_ov) if tune else \
+            _at.cached(pb, spec.mode, B, opt.early_stop)
+        lpp0, rpw0 = tuned if tuned is not None else default_tiling(pb, spec.mode, B, opt.early_stop)
+    else:
+        lpp0, rpw0 = default_tiling(pb, spec.mode, B, opt.early_stop)
     lpp = lanes_per_pose or lpp0
     rpw = restarts_per_wg or (B if opt.early_stop else rpw0)
     check_tiling(B, lpp, rpw, opt.early_stop)

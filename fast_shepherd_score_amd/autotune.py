@@ -1,0 +1,103 @@
+"""Autotuner for the persistent optimiser's tiling (lanes per pose, restarts per workgroup).
+
+The analytic model (:func:`fast_shepherd_score_amd.align.tiling_cost`) ranks every legal tiling;
+the tuner then times the top candidates on the actual batch with a short optimiser run (a few
+steps, early stop off, so each candidate does identical work) and keeps the fastest.  Results are
+cached in memory and, when ``FSS_AUTOTUNE_CACHE`` names a file, on disk as JSON, keyed by mode,
+restart count, early-stop flag and the batch's size profile.
+"""
+from __future__ import annotations
+
+import json
+import os
+import threading
+from typing import Dict, Optional, Tuple
+
+import numpy as np
+import torch
+
+from . import _lib
+from .align import (OptimSpec, ScoreSpec, align, candidate_tilings, self_overlaps, tiling_cost,
+                    tiling_efficiency)
+from .packing import PairBatch
+
+_cache: Dict[str, Tuple[int, int]] = {}
+_lock = threading.Lock()
+
+
+def _counts(pb: PairBatch, mode: int):
+    if mode == _lib.MODE_PHARM:
+        return pb.ref_pcounts, pb.fit_pcounts
+    return pb.ref_counts, pb.fit_counts
+
+
+def shape_key(pb: PairBatch, mode: int, B: int, early_stop: bool) -> str:
+    rc, fc = _counts(pb, mode)
+    q = lambda c: tuple(int(x) for x in np.percentile(c, [10, 50, 90]) // 8 * 8) if c is not None and len(c) else (0,)
+    nb = int(np.log2(max(1, pb.n_pairs)))
+    extra = ""
+    if mode == _lib.MODE_SHAPE_PHARM:
+        extra = str(q(pb.fit_pcounts))
+    return f"m{mode}-B{B}-es{int(early_stop)}-n{nb}-r{q(rc)}-f{q(fc)}{extra}"
+
+
+def _load_disk() -> None:
+    path = os.environ.get("FSS_AUTOTUNE_CACHE")
+    if path and os.path.exists(path):
+        try:
+            with open(path) as f:
+                for k, v in json.load(f).items():
+                    _cache.setdefault(k, (int(v[0]), int(v[1])))
+        except (OSError, ValueError):
+            pass
+
+
+def _save_disk() -> None:
+    path = os.environ.get("FSS_AUTOTUNE_CACHE")
+    if not path:
+        return
+    tmp = path + ".tmp"
+    with open(tmp, "w") as f:
+        json.dump({k: list(v) for k, v in _cache.items()}, f)
+    os.replace(tmp, path)
+
+
+def autotune(pb: PairBatch, spec: ScoreSpec, q0: torch.Tensor, t0: torch.Tensor, early_stop: bool,
+             *, top: int = 5, steps: int = 8, self_ov: Optional[torch.Tensor] = None,
+             verbose: bool = False) -> Tuple[int, int]:
+    """Best (lanes_per_pose, restarts_per_wg) for this batch (timed, cached)."""
+    B = int(q0.shape[1])
+    key = shape_key(pb, spec.mode, B, early_stop)
+    with _lock:
+        if not _cache:
+            _load_disk()
+        if key in _cache:
+            return _cache[key]
+    rc, fc = _counts(pb, spec.mode)
+    cands = sorted(((tiling_cost(rc, fc, B, l, r), -tiling_efficiency(fc, B, l, r), l, r)
+                    for l, r in candidate_tilings(B, early_stop)))[:top]
+    if self_ov is None:
+        self_ov = self_overlaps(pb, spec)
+    opt = OptimSpec(max_steps=steps, early_stop=False)
+    best, best_ms = None, float("inf")
+    for _, _, lpp, rpw in cands:
+        ev = []
+        align(pb, spec, q0, t0, opt, lanes_per_pose=lpp, restarts_per_wg=rpw, self_ov=self_ov)
+        align(pb, spec, q0, t0, opt, lanes_per_pose=lpp, restarts_per_wg=rpw, self_ov=self_ov, events=ev)
+        torch.cuda.synchronize()
+        ms = ev[0][0].elapsed_time(ev[0][1])
+        if verbose:
+            print(f"autotune {key}: lpp={lpp} rpw={rpw} {ms:.3f} ms", flush=True)
+        if ms < best_ms:
+            best, best_ms = (lpp, rpw), ms
+    with _lock:
+        _cache[key] = best
+        _save_disk()
+    return best
+
+
+def cached(pb: PairBatch, mode: int, B: int, early_stop: bool) -> Optional[Tuple[int, int]]:
+    with _lock:
+        if not _cache:
+            _load_disk()
+        return _cache.get(shape_key(pb, mode, B, early_stop))

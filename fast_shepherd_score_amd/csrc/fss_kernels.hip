@@ -468,15 +468,19 @@ __global__ __launch_bounds__(MAXT) void k_align(FssPairBatch pb, KScore ks, LdsP
   const double* so = ak.self_ov + 4 * pair;
   const float invB = 1.0f / (float)ak.B;
 
+  // Adam moments of every restart in LDS (14 floats per group; lane 0 of the group writes, the
+  // group's lanes read: one wavefront, in-order DS operations).  Parameters stay in registers.
+  float* mv = reinterpret_cast<float*>(lossbuf + (ES ? 2 * ak.rpw : 0)) + 16 * grp;
   float p[7];
-  float m[7], v[7];
 #pragma unroll
-  for (int k = 0; k < 7; ++k) { m[k] = 0.f; v[k] = 0.f; p[k] = 0.f; }
+  for (int k = 0; k < 7; ++k) p[k] = 0.f;
   if (active) {
 #pragma unroll
     for (int k = 0; k < 4; ++k) p[k] = ak.q[4 * pose + k];
 #pragma unroll
     for (int k = 0; k < 3; ++k) p[4 + k] = ak.t[3 * pose + k];
+    if (j == 0)
+      for (int k = 0; k < 14; ++k) mv[k] = 0.f;
   }
   __syncthreads();
 
@@ -519,10 +523,11 @@ __global__ __launch_bounds__(MAXT) void k_align(FssPairBatch pb, KScore ks, LdsP
       const float bc2s = (float)sqrt(1.0 - b2p);
 #pragma unroll
       for (int k = 0; k < 7; ++k) {
-        m[k] = fmaf(1.f - ak.b1, g[k] - m[k], m[k]);
-        v[k] = fmaf(1.f - ak.b2, g[k] * g[k], v[k] * ak.b2);
-        const float den = sqrtf(v[k]) / bc2s + ak.eps;
-        p[k] = p[k] - step_size * (m[k] / den);
+        const float mk = fmaf(1.f - ak.b1, g[k] - mv[k], mv[k]);
+        const float vk = fmaf(1.f - ak.b2, g[k] * g[k], mv[7 + k] * ak.b2);
+        const float den = sqrtf(vk) / bc2s + ak.eps;
+        p[k] = p[k] - step_size * (mk / den);
+        if (j == 0) { mv[k] = mk; mv[7 + k] = vk; }
       }
     }
     if (ES) {
@@ -1016,7 +1021,9 @@ int fss_align(const FssPairBatch* batch, const FssScoreParams* sp, const FssAlig
   if (block > 1024) return set_error(FSS_E_INVALID, "restarts_per_wg * lanes_per_pose = %d exceeds 1024 threads", rpw * lpp);
   if (batch->n_pairs == 0) return FSS_OK;
   const LdsPlan lp = plan_lds(batch, ks.mode);
-  const size_t lds = (size_t)lp.end * sizeof(float4) + (ap->early_stop ? 2 * (size_t)rpw * sizeof(double) : 0);
+  const int groups = block / lpp;
+  const size_t lds = (size_t)lp.end * sizeof(float4) + (ap->early_stop ? 2 * (size_t)rpw * sizeof(double) : 0) +
+                     (size_t)groups * 16 * sizeof(float);
   const size_t cap = max_lds_bytes();
   if (lds > cap) return set_error(FSS_E_TOO_LARGE, "clouds need %zu B of LDS, the device allows %zu B", lds, cap);
   AlignK ak;

@@ -54,6 +54,22 @@ class CloudSet:
                    counts.astype(np.int64))
 
     @classmethod
+    def from_flat(cls, xyz: np.ndarray, counts: np.ndarray, w: Optional[np.ndarray] = None,
+                  device="cuda") -> "CloudSet":
+        """From concatenated rows (rows, 3) and per-cloud counts (no per-cloud Python loop)."""
+        counts = np.asarray(counts, dtype=np.int64)
+        rows = int(counts.sum())
+        host = np.zeros((max(rows, 1), 4), dtype=np.float32)
+        host[:rows, :3] = np.asarray(xyz, np.float32).reshape(-1, 3)
+        if w is not None:
+            host[:rows, 3] = np.asarray(w, np.float32).reshape(-1)
+        starts = np.zeros_like(counts)
+        if len(counts) > 1:
+            starts[1:] = np.cumsum(counts)[:-1]
+        rng = np.stack([starts, counts], axis=1).astype(np.int32)
+        return cls(torch.from_numpy(host).to(device), torch.from_numpy(rng).to(device), counts)
+
+    @classmethod
     def from_padded(cls, xyz: np.ndarray, counts: np.ndarray, w: Optional[np.ndarray] = None,
                     device="cuda") -> "CloudSet":
         """From a padded (n, P, 3) array and real counts (vectorised packing)."""

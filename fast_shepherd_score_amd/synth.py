@@ -124,3 +124,22 @@ def pharm_shape_pairs(n_pairs: int, atoms=(5, 60), pharms=(2, 20), seed: int = 2
         out[f"{side}_anchors"] = [p[1] for p in ph[sl]]
         out[f"{side}_vectors"] = [p[2] for p in ph[sl]]
     return out
+
+
+def conformer_library(n: int, atoms=(10, 50), base: int = 20000, seed: int = 5):
+    """n atom clouds: ``base`` random-walk molecules, each posed under n/base uniform random
+    rotations (the screening library; every entry is a distinct rigid conformer)."""
+    rng = np.random.default_rng(seed)
+    base = min(base, n)
+    mols = random_walks(rng, rng.integers(atoms[0], atoms[1] + 1, size=base))
+    q = random_quaternions(rng, n).astype(np.float64)
+    w, x, y, z = q.T
+    R = np.stack([1 - 2 * (y * y + z * z), 2 * (x * y - z * w), 2 * (x * z + y * w),
+                  2 * (x * y + z * w), 1 - 2 * (x * x + z * z), 2 * (y * z - x * w),
+                  2 * (x * z - y * w), 2 * (y * z + x * w), 1 - 2 * (x * x + y * y)], 1).reshape(n, 3, 3)
+    which = np.arange(n) % base
+    counts = np.array([len(mols[k]) for k in which], np.int64)
+    flat = np.concatenate([mols[k] for k in which]).astype(np.float64)
+    owner = np.repeat(np.arange(n), counts)
+    flat = np.einsum("pij,pj->pi", R[owner], flat).astype(np.float32)
+    return flat, counts
