@@ -119,8 +119,8 @@ def bench_c1(dev, reps: int = 3):
     st.capture()
 
     def graph_run():
-        st.q.copy_(qd.reshape(-1, 4))
-        st.t.copy_(td.reshape(-1, 3))
+        st.q.copy_(qd)
+        st.t.copy_(td)
         st.m.zero_()
         st.v.zero_()
         return st.run(use_graph=True)

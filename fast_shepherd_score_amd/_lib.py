@@ -70,6 +70,7 @@ class FssAlignParams(ctypes.Structure):
         ("restarts_per_wg", ctypes.c_int32),
         ("pair_order", ctypes.c_void_p),
         ("skip_finish", ctypes.c_int32),
+        ("variant", ctypes.c_int32),
     ]
 
 

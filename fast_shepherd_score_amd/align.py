@@ -182,7 +182,7 @@ def align(pb: PairBatch, spec: ScoreSpec, q0: torch.Tensor, t0: torch.Tensor,
           opt: OptimSpec = OptimSpec(), *, lanes_per_pose: Optional[int] = None,
           restarts_per_wg: Optional[int] = None, order: str = "cost",
           self_ov: Optional[torch.Tensor] = None, stream=None, events: Optional[list] = None,
-          tune: bool = False) -> AlignResult:
+          tune: bool = False, variant: int = 0) -> AlignResult:
     """Persistent path: every step of every pair in one launch.
 
     Tiling: explicit ``lanes_per_pose`` / ``restarts_per_wg``, else an autotuned choice
@@ -221,6 +221,7 @@ def align(pb: PairBatch, spec: ScoreSpec, q0: torch.Tensor, t0: torch.Tensor,
     ap.early_stop, ap.es_tol, ap.es_patience = int(bool(opt.early_stop)), opt.es_tol, int(opt.es_patience)
     ap.lanes_per_pose, ap.restarts_per_wg = int(lpp), int(rpw)
     ap.pair_order = _lib.ptr(order_t)
+    ap.variant = int(variant)
     out = _lib.FssAlignOut()
     out.q, out.t = _lib.ptr(q), _lib.ptr(t)
     out.pose_score, out.best_score, out.best_idx = _lib.ptr(pose_score), _lib.ptr(best), _lib.ptr(bidx)

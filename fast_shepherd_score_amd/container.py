@@ -115,7 +115,7 @@ def _mode_inputs(pairs: Sequence[MoleculePair], mode: str):
 
 
 def align_pairs(pairs: Sequence[MoleculePair], mode: str, *, alpha=0.81, lam=None,
-                color_weight=0.5, similarity="tanimoto", directionless=True,
+                color_weight=0.5, similarity="tanimoto", directionless=False,
                 num_repeats: int = 50, lr: float = 0.1, max_num_steps: int = 200,
                 early_stop: bool = True, seeds=None, device=None, **tiling):
     """Align every pair in one persistent launch; write results in place; return the result."""

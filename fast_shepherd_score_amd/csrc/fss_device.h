@@ -99,14 +99,15 @@ __device__ __forceinline__ void group_sum(Acc& a, int width, bool grad) {
 //   g_ab = W_a + 2 a~ . b~' + 2 c~_a c~_b,   W_a = -|a~|^2 - c~_a^2,   h_b = |b~'|^2 + c~_b^2
 // LDS rows: ref4[a] = (2 a~, W_a), refc[a] = 2 c~_a (ESP only), fit4[b] = (s b, c~_b).
 // A pair term is then 3-4 FMA + v_exp_f32 + 1 add + 3 FMA; exp2(-h_b) is applied once per fit
-// row.  Since g_ab <= h_b, exp2(g_ab) cannot overflow while h_b <= kHMax; fit rows beyond that
-// radius (about 11.7 A from the ref centroid at alpha 0.81) take the direct form.
+// row.  Since g_ab <= h_b, a sum of up to 2^17 terms exp2(g_ab) stays finite while h_b <= kHMax
+// = 110; fit rows beyond that radius (about 13.7 A from the ref centroid at alpha 0.81) take the
+// direct form.
 // Lane j of a group of `lpp` lanes owns fit rows b = j, j+lpp, ...; two rows per pass over the
 // ref cloud so one LDS broadcast read feeds two pair terms.
 // Output (per lane, scaled units): V = sum E, gt = sum_b f_b, G = sum_b f_b (s b)^T with
 // f_b = sum_a E_ab (a~ - b~'_b).
 // ---------------------------------------------------------------------------------------------
-constexpr float kHMax = 80.f;
+constexpr float kHMax = 110.f;
 
 template <bool ESP>
 __device__ __forceinline__ void direct_pair(const float4* __restrict__ ref, const float* __restrict__ refc,
@@ -123,6 +124,65 @@ __device__ __forceinline__ void direct_pair(const float4* __restrict__ ref, cons
     const float e = fast_exp2(-s2);
     v += e;
     fx = fmaf(e, dx, fx); fy = fmaf(e, dy, fy); fz = fmaf(e, dz, fz);
+  }
+}
+
+// Direct-form twin of shape_accum (the other inner-loop variant): LDS rows ref4[a] = (a~, c~_a),
+// fit4[b] = (s b, c~_b); a pair term is 3 sub + 3 FMA/mul (+ 1 sub + 1 FMA for ESP) + v_exp_f32
+// (negated operand) + 1 add + 3 FMA.  No radius limit.
+template <bool ESP, bool GRAD>
+__device__ __forceinline__ void shape_accum_direct(const float4* __restrict__ ref, int N,
+                                                   const float4* __restrict__ fit, int M, int j, int lpp,
+                                                   const float R[9], const float ts[3], Acc& acc) {
+  for (int b0 = j; b0 < M; b0 += 2 * lpp) {
+    const int b1r = b0 + lpp;
+    const bool has1 = b1r < M;
+    const int b1 = has1 ? b1r : b0;
+    const float4 p0 = fit[b0];
+    const float4 p1 = fit[b1];
+    const float x0 = fmaf(R[0], p0.x, fmaf(R[1], p0.y, fmaf(R[2], p0.z, ts[0])));
+    const float y0 = fmaf(R[3], p0.x, fmaf(R[4], p0.y, fmaf(R[5], p0.z, ts[1])));
+    const float z0 = fmaf(R[6], p0.x, fmaf(R[7], p0.y, fmaf(R[8], p0.z, ts[2])));
+    const float x1 = fmaf(R[0], p1.x, fmaf(R[1], p1.y, fmaf(R[2], p1.z, ts[0])));
+    const float y1 = fmaf(R[3], p1.x, fmaf(R[4], p1.y, fmaf(R[5], p1.z, ts[1])));
+    const float z1 = fmaf(R[6], p1.x, fmaf(R[7], p1.y, fmaf(R[8], p1.z, ts[2])));
+    float v0 = 0.f, v1 = 0.f;
+    float fx0 = 0.f, fy0 = 0.f, fz0 = 0.f, fx1 = 0.f, fy1 = 0.f, fz1 = 0.f;
+#pragma unroll 4
+    for (int a = 0; a < N; ++a) {
+      const float4 A = ref[a];
+      const float dx0 = A.x - x0, dy0 = A.y - y0, dz0 = A.z - z0;
+      const float dx1 = A.x - x1, dy1 = A.y - y1, dz1 = A.z - z1;
+      float s0 = fmaf(dx0, dx0, fmaf(dy0, dy0, dz0 * dz0));
+      float s1 = fmaf(dx1, dx1, fmaf(dy1, dy1, dz1 * dz1));
+      if (ESP) {
+        const float c0 = A.w - p0.w, c1 = A.w - p1.w;
+        s0 = fmaf(c0, c0, s0);
+        s1 = fmaf(c1, c1, s1);
+      }
+      const float e0 = fast_exp2(-s0);
+      const float e1 = fast_exp2(-s1);
+      v0 += e0;
+      v1 += e1;
+      if (GRAD) {
+        fx0 = fmaf(e0, dx0, fx0); fy0 = fmaf(e0, dy0, fy0); fz0 = fmaf(e0, dz0, fz0);
+        fx1 = fmaf(e1, dx1, fx1); fy1 = fmaf(e1, dy1, fy1); fz1 = fmaf(e1, dz1, fz1);
+      }
+    }
+    if (!has1) { v1 = 0.f; fx1 = fy1 = fz1 = 0.f; }
+    acc.V += v0 + v1;
+    if (GRAD) {
+      acc.gt[0] += fx0 + fx1; acc.gt[1] += fy0 + fy1; acc.gt[2] += fz0 + fz1;
+      acc.G[0] = fmaf(fx0, p0.x, fmaf(fx1, p1.x, acc.G[0]));
+      acc.G[1] = fmaf(fx0, p0.y, fmaf(fx1, p1.y, acc.G[1]));
+      acc.G[2] = fmaf(fx0, p0.z, fmaf(fx1, p1.z, acc.G[2]));
+      acc.G[3] = fmaf(fy0, p0.x, fmaf(fy1, p1.x, acc.G[3]));
+      acc.G[4] = fmaf(fy0, p0.y, fmaf(fy1, p1.y, acc.G[4]));
+      acc.G[5] = fmaf(fy0, p0.z, fmaf(fy1, p1.z, acc.G[5]));
+      acc.G[6] = fmaf(fz0, p0.x, fmaf(fz1, p1.x, acc.G[6]));
+      acc.G[7] = fmaf(fz0, p0.y, fmaf(fz1, p1.y, acc.G[7]));
+      acc.G[8] = fmaf(fz0, p0.z, fmaf(fz1, p1.z, acc.G[8]));
+    }
   }
 }
 
@@ -218,11 +278,13 @@ struct PhTab {
 };
 
 // ---------------------------------------------------------------------------------------------
-// Pharmacophore channel, FP32.  anc[i] = (x, y, z, type bits), vec[i] = (unit x, y, z, 0).
+// Pharmacophore channel, FP32.  anc[i] = (x, y, z, type bits), vec[i] = (unit x, y, z, 0); the
+// ref rows are sorted by type, so fit row b (type t) visits only ref rows [poff[t], poff[t+1]).
 // ---------------------------------------------------------------------------------------------
 template <bool GRAD>
 __device__ __forceinline__ void pharm_accum(const float4* __restrict__ ranc,
-                                            const float4* __restrict__ rvec, int N,
+                                            const float4* __restrict__ rvec,
+                                            const int* __restrict__ poff,
                                             const float4* __restrict__ fanc,
                                             const float4* __restrict__ fvec, int M, int j, int lpp,
                                             const float R[9], const float t[3], const PhTab& tab,
@@ -242,9 +304,9 @@ __device__ __forceinline__ void pharm_accum(const float4* __restrict__ ranc,
     const float uy = R[3] * u.x + R[4] * u.y + R[5] * u.z;
     const float uz = R[6] * u.x + R[7] * u.y + R[8] * u.z;
     float fx = 0.f, fy = 0.f, fz = 0.f, hx = 0.f, hy = 0.f, hz = 0.f, v = 0.f;
-    for (int a = 0; a < N; ++a) {
+    const int a_end = poff[tb + 1];
+    for (int a = poff[tb]; a < a_end; ++a) {
       const float4 r = ranc[a];
-      if (__float_as_int(r.w) != tb) continue;
       const float4 rv = rvec[a];
       const float dx = r.x - bx, dy = r.y - by, dz = r.z - bz;
       const float e = fast_exp2(-kap * fmaf(dx, dx, fmaf(dy, dy, dz * dz)));

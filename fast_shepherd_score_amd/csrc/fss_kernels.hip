@@ -113,7 +113,7 @@ int check_batch(const FssPairBatch* pb, int mode) {
 // LDS plan for one workgroup (float4 offsets): shape ref rows, fit rows, ref ESP column,
 // a 16-float4 reduction scratch, then pharm ref anchors / vectors, fit anchors / vectors.
 struct LdsPlan {
-  int ref, fit, refc, red, ranc, rvec, fanc, fvec, end;
+  int ref, fit, refc, red, poff, ranc, rvec, fanc, fvec, end;
 };
 LdsPlan plan_lds(const FssPairBatch* pb, int mode) {
   LdsPlan p;
@@ -123,6 +123,7 @@ LdsPlan plan_lds(const FssPairBatch* pb, int mode) {
   p.fit = o; o += sh ? pb->max_fit : 0;
   p.refc = o; o += mode == FSS_MODE_SHAPE_ESP ? (pb->max_ref + 3) / 4 : 0;
   p.red = o; o += 16;
+  p.poff = o; o += ph ? 4 : 0;
   p.ranc = o; o += ph ? pb->max_ref_ph : 0;
   p.rvec = o; o += ph ? pb->max_ref_ph : 0;
   p.fanc = o; o += ph ? pb->max_fit_ph : 0;
@@ -142,11 +143,12 @@ struct Staged {
   const float4* ref; const float* refc; const float4* fit; int N, M;
   float cs[3];  // scaled ref centroid: the origin of the expanded shape form
   const float4* ranc; const float4* rvec; const float4* fanc; const float4* fvec; int Np, Mp;
+  const int* poff;  // ref pharmacophore rows are sorted by type; type t occupies [poff[t], poff[t+1])
 };
 
 // Cooperative copy of one pair's clouds into LDS (shape rows scaled, pharm vectors normalised,
 // pharm type folded into the anchor's w as integer bits).  Caller syncs afterwards.
-template <bool SH, bool ESP, bool PH>
+template <bool SH, bool ESP, bool PH, bool EXPD = true>
 __device__ __forceinline__ Staged stage_pair(const FssPairBatch& pb, const KScore& ks, LdsPlan lp,
                                              int pair, int ref_side_a, int fit_side_b,
                                              float4* lds) {
@@ -185,8 +187,12 @@ __device__ __forceinline__ Staged stage_pair(const FssPairBatch& pb, const KScor
       const float4 v = ga[i];
       const float ax = s * (v.x - cx), ay = s * (v.y - cy), az = s * (v.z - cz);
       const float cc = ESP ? v.w * sc : 0.f;
-      la[i] = make_float4(2.f * ax, 2.f * ay, 2.f * az, -fmaf(ax, ax, fmaf(ay, ay, fmaf(az, az, cc * cc))));
-      if (ESP) lc[i] = 2.f * cc;
+      if (EXPD) {
+        la[i] = make_float4(2.f * ax, 2.f * ay, 2.f * az, -fmaf(ax, ax, fmaf(ay, ay, fmaf(az, az, cc * cc))));
+        if (ESP) lc[i] = 2.f * cc;
+      } else {
+        la[i] = make_float4(ax, ay, az, cc);
+      }
     }
     for (int i = tid; i < M; i += nt) {
       const float4 v = gb[i];
@@ -206,21 +212,40 @@ __device__ __forceinline__ Staged stage_pair(const FssPairBatch& pb, const KScor
     const int32_t* bt = fit_side_b ? pb.fit_typ : pb.ref_typ;
     const int a0 = ra[2 * pair], N = ra[2 * pair + 1];
     const int b0 = rb[2 * pair], M = rb[2 * pair + 1];
+    // Ref rows go to their type-sorted position (a stable counting sort by direct ranking:
+    // the clouds hold a few dozen features); out-of-range types are filed as Dummy.
+    auto tclamp = [](int ty) { return (ty < 0 || ty >= FSS_N_PTYPES) ? FSS_N_PTYPES - 1 : ty; };
+    int* poff = reinterpret_cast<int*>(lds + lp.poff);
+    if (tid <= FSS_N_PTYPES) {
+      int c = 0;
+      for (int a = 0; a < N; ++a) c += tclamp(at[a0 + a]) < tid;
+      poff[tid] = c;
+    }
     for (int i = tid; i < N + M; i += nt) {
       const bool isref = i < N;
       const int k = isref ? a0 + i : b0 + (i - N);
       const float4 p = isref ? aa[k] : ba[k];
       const float4 v = isref ? av[k] : bv[k];
-      const int ty = isref ? at[k] : bt[k];
+      const int ty = tclamp(isref ? at[k] : bt[k]);
       float n = sqrtf(v.x * v.x + v.y * v.y + v.z * v.z);
       n = n > 1e-12f ? n : 1e-12f;
       const float4 pa4 = make_float4(p.x, p.y, p.z, __int_as_float(ty));
       const float4 va4 = make_float4(v.x / n, v.y / n, v.z / n, 0.f);
-      if (isref) { lds[lp.ranc + i] = pa4; lds[lp.rvec + i] = va4; }
-      else { lds[lp.fanc + (i - N)] = pa4; lds[lp.fvec + (i - N)] = va4; }
+      if (isref) {
+        int pos = 0;
+        for (int a = 0; a < N; ++a) {
+          const int ta = tclamp(at[a0 + a]);
+          pos += (ta < ty) || (ta == ty && a < i);
+        }
+        lds[lp.ranc + pos] = pa4;
+        lds[lp.rvec + pos] = va4;
+      } else {
+        lds[lp.fanc + (i - N)] = pa4;
+        lds[lp.fvec + (i - N)] = va4;
+      }
     }
     st.ranc = lds + lp.ranc; st.rvec = lds + lp.rvec; st.fanc = lds + lp.fanc; st.fvec = lds + lp.fvec;
-    st.Np = N; st.Mp = M;
+    st.Np = N; st.Mp = M; st.poff = poff;
   }
   return st;
 }
@@ -376,7 +401,7 @@ __global__ __launch_bounds__(kPoseBlock) void k_pose_grad(FssPairBatch pb, KScor
     block_sum(sh, red[0], GRAD);
   }
   if (PH) {
-    pharm_accum<GRAD>(st.ranc, st.rvec, st.Np, st.fanc, st.fvec, st.Mp, threadIdx.x, blockDim.x, R, t, ks.tab, ph);
+    pharm_accum<GRAD>(st.ranc, st.rvec, st.poff, st.fanc, st.fvec, st.Mp, threadIdx.x, blockDim.x, R, t, ks.tab, ph);
     block_sum(ph, red[1], GRAD);
   }
   if (threadIdx.x != 0) return;
@@ -448,7 +473,7 @@ struct AlignK {
   int32_t* n_steps;
 };
 
-template <int MODE, bool ES, int MAXT>
+template <int MODE, bool ES, int MAXT, bool EXPD>
 __global__ __launch_bounds__(MAXT) void k_align(FssPairBatch pb, KScore ks, LdsPlan lp, AlignK ak) {
   constexpr bool SH = MODE != FSS_MODE_PHARM;
   constexpr bool ESP = MODE == FSS_MODE_SHAPE_ESP;
@@ -457,7 +482,7 @@ __global__ __launch_bounds__(MAXT) void k_align(FssPairBatch pb, KScore ks, LdsP
   const int slot = blockIdx.x / ak.wg_per_pair;
   const int blk = blockIdx.x - slot * ak.wg_per_pair;
   const int pair = ak.order ? ak.order[slot] : slot;
-  const Staged st = stage_pair<SH, ESP, PH>(pb, ks, lp, pair, 0, 1, lds);
+  const Staged st = stage_pair<SH, ESP, PH, EXPD>(pb, ks, lp, pair, 0, 1, lds);
   double* lossbuf = reinterpret_cast<double*>(lds + lp.end);  // 2 * rpw doubles (ES only)
 
   const int lpp = ak.lpp;
@@ -499,12 +524,13 @@ __global__ __launch_bounds__(MAXT) void k_align(FssPairBatch pb, KScore ks, LdsP
       ph.zero();
       if (SH) {
         const float ts[3] = {fmaf(p[4], ks.s, -st.cs[0]), fmaf(p[5], ks.s, -st.cs[1]), fmaf(p[6], ks.s, -st.cs[2])};
-        shape_accum<ESP, true>(st.ref, st.refc, st.N, st.fit, st.M, j, lpp, R, ts, sh);
+        if (EXPD) shape_accum<ESP, true>(st.ref, st.refc, st.N, st.fit, st.M, j, lpp, R, ts, sh);
+        else shape_accum_direct<ESP, true>(st.ref, st.N, st.fit, st.M, j, lpp, R, ts, sh);
         group_sum(sh, lpp, true);
         shape_finish(sh, ks.K, ks.alpha, ks.s, true);
       }
       if (PH) {
-        pharm_accum<true>(st.ranc, st.rvec, st.Np, st.fanc, st.fvec, st.Mp, j, lpp, R, p + 4, ks.tab, ph);
+        pharm_accum<true>(st.ranc, st.rvec, st.poff, st.fanc, st.fvec, st.Mp, j, lpp, R, p + 4, ks.tab, ph);
         group_sum(ph, lpp, true);
       }
       float ws, wp;
@@ -765,14 +791,19 @@ void launch_pose(const FssPairBatch* pb, const KScore& ks, LdsPlan lp, int n, co
                      *pb, ks, lp, pp, q, t, so, invB, V, gR, gT, gQ, loss, grad);
 }
 
-// Workgroups of up to 512 threads get a 256-VGPR budget, larger ones 128.
+// Workgroups of up to 512 threads get a 256-VGPR budget, larger ones 128.  variant 1 = direct
+// inner loop, otherwise the expanded form.
 template <int MODE, bool ES>
 void launch_align(const FssPairBatch* pb, const KScore& ks, LdsPlan lp, const AlignK& ak, int grid,
-                  int block, size_t lds, hipStream_t s) {
-  if (block <= 512)
-    hipLaunchKernelGGL((k_align<MODE, ES, 512>), dim3(grid), dim3(block), lds, s, *pb, ks, lp, ak);
-  else
-    hipLaunchKernelGGL((k_align<MODE, ES, 1024>), dim3(grid), dim3(block), lds, s, *pb, ks, lp, ak);
+                  int block, size_t lds, int variant, hipStream_t s) {
+  const bool direct = variant == 1 && MODE != FSS_MODE_PHARM;
+  if (block <= 512) {
+    if (direct) hipLaunchKernelGGL((k_align<MODE, ES, 512, false>), dim3(grid), dim3(block), lds, s, *pb, ks, lp, ak);
+    else hipLaunchKernelGGL((k_align<MODE, ES, 512, true>), dim3(grid), dim3(block), lds, s, *pb, ks, lp, ak);
+  } else {
+    if (direct) hipLaunchKernelGGL((k_align<MODE, ES, 1024, false>), dim3(grid), dim3(block), lds, s, *pb, ks, lp, ak);
+    else hipLaunchKernelGGL((k_align<MODE, ES, 1024, true>), dim3(grid), dim3(block), lds, s, *pb, ks, lp, ak);
+  }
 }
 
 int score_check(const FssPairBatch* batch, const FssScoreParams* sp) {
@@ -833,29 +864,145 @@ int pose_score_impl(const FssPairBatch* batch, const KScore& ks, int32_t n_poses
   return check_launch("fss_pose_score");
 }
 
-// Restart seeds on device: q_r = table[r], t_r = com(ref) - R(q_r) com(fit) (one wave per pair).
-// use_pharm: centroids of the pharmacophore anchors instead of the shape clouds.
-__global__ __launch_bounds__(64) void k_seed_poses(FssPairBatch pb, int use_pharm, int B,
+// --- device seeds -------------------------------------------------------------------------------
+// Cyclic Jacobi eigen-decomposition of a symmetric 3x3 matrix (FP64): a -> diag, V columns.
+__device__ void jacobi3(double a[3][3], double V[3][3]) {
+  for (int i = 0; i < 3; ++i)
+    for (int k = 0; k < 3; ++k) V[i][k] = i == k ? 1.0 : 0.0;
+  for (int sweep = 0; sweep < 24; ++sweep) {
+    const double off = a[0][1] * a[0][1] + a[0][2] * a[0][2] + a[1][2] * a[1][2];
+    const double dia = a[0][0] * a[0][0] + a[1][1] * a[1][1] + a[2][2] * a[2][2];
+    if (off <= 1e-30 * dia || off == 0.0) break;
+    for (int pq = 0; pq < 3; ++pq) {
+      const int p = pq == 2 ? 1 : 0, q = pq == 0 ? 1 : 2;
+      if (a[p][q] == 0.0) continue;
+      const double th = (a[q][q] - a[p][p]) / (2.0 * a[p][q]);
+      const double t = (th >= 0 ? 1.0 : -1.0) / (fabs(th) + sqrt(th * th + 1.0));
+      const double c = 1.0 / sqrt(t * t + 1.0), s = t * c;
+      for (int k = 0; k < 3; ++k) {
+        const double akp = a[k][p], akq = a[k][q];
+        a[k][p] = c * akp - s * akq; a[k][q] = s * akp + c * akq;
+      }
+      for (int k = 0; k < 3; ++k) {
+        const double apk = a[p][k], aqk = a[q][k];
+        a[p][k] = c * apk - s * aqk; a[q][k] = s * apk + c * aqk;
+      }
+      for (int k = 0; k < 3; ++k) {
+        const double vkp = V[k][p], vkq = V[k][q];
+        V[k][p] = c * vkp - s * vkq; V[k][q] = s * vkp + c * vkq;
+      }
+    }
+  }
+}
+
+// Principal axes (rows, largest inertia eigenvalue first) from the centred second moments C:
+// inertia = trace(C) I - C (pca_np.py:12-33).
+__device__ void principal_axes(const double C[6], double ax[3][3]) {
+  const double tr = C[0] + C[3] + C[5];
+  double a[3][3] = {{tr - C[0], -C[1], -C[2]}, {-C[1], tr - C[3], -C[4]}, {-C[2], -C[4], tr - C[5]}};
+  double V[3][3];
+  jacobi3(a, V);
+  double w[3] = {a[0][0], a[1][1], a[2][2]};
+  int o[3] = {0, 1, 2};
+  for (int i = 0; i < 2; ++i)
+    for (int k = 0; k < 2 - i; ++k)
+      if (w[o[k]] < w[o[k + 1]]) { const int tmp = o[k]; o[k] = o[k + 1]; o[k + 1] = tmp; }
+  for (int r = 0; r < 3; ++r)
+    for (int k = 0; k < 3; ++k) ax[r][k] = V[k][o[r]];
+}
+
+// Quaternion rotating v1 onto v2 (axis-angle construction of pca_np.py:36-88).
+__device__ void quat_between(const double v1[3], const double v2[3], double q[4]) {
+  const double n1 = sqrt(v1[0] * v1[0] + v1[1] * v1[1] + v1[2] * v1[2]);
+  const double n2 = sqrt(v2[0] * v2[0] + v2[1] * v2[1] + v2[2] * v2[2]);
+  double d = (v1[0] * v2[0] + v1[1] * v2[1] + v1[2] * v2[2]) / (n1 * n2);
+  d = fmin(fmax(d, -1.0), 1.0);
+  const double ang = acos(d);
+  double ax[3] = {v1[1] * v2[2] - v1[2] * v2[1], v1[2] * v2[0] - v1[0] * v2[2], v1[0] * v2[1] - v1[1] * v2[0]};
+  double na = sqrt(ax[0] * ax[0] + ax[1] * ax[1] + ax[2] * ax[2]);
+  if (na < 1e-12) {            // parallel: any axis; antiparallel: an axis perpendicular to v1
+    const bool xmin = fabs(v1[0]) <= fabs(v1[1]) && fabs(v1[0]) <= fabs(v1[2]);
+    const double e[3] = {xmin ? 1.0 : 0.0, xmin ? 0.0 : 1.0, 0.0};
+    ax[0] = v1[1] * e[2] - v1[2] * e[1]; ax[1] = v1[2] * e[0] - v1[0] * e[2]; ax[2] = v1[0] * e[1] - v1[1] * e[0];
+    na = sqrt(ax[0] * ax[0] + ax[1] * ax[1] + ax[2] * ax[2]);
+  }
+  const double sn = sin(0.5 * ang) / na;
+  q[0] = cos(0.5 * ang); q[1] = ax[0] * sn; q[2] = ax[1] * sn; q[3] = ax[2] * sn;
+}
+
+__device__ void qmul(const double p[4], const double q[4], double o[4]) {
+  o[0] = p[0] * q[0] - p[1] * q[1] - p[2] * q[2] - p[3] * q[3];
+  o[1] = p[1] * q[0] + p[0] * q[1] - p[3] * q[2] + p[2] * q[3];
+  o[2] = p[2] * q[0] + p[3] * q[1] + p[0] * q[2] - p[1] * q[3];
+  o[3] = p[3] * q[0] - p[2] * q[1] + p[1] * q[2] + p[0] * q[3];
+}
+
+// Restart seeds, one wave per pair.  flags bit 0: use the pharmacophore anchors instead of the
+// shape clouds; bit 1: the reference's seed rule (alignment/_torch.py:253-326): B == 1 is the
+// identity with zero translation, rows 1-4 are the principal-axis alignments (B >= 5).  Other
+// rows take table[r]; every translation is com(ref) - R(q) com(fit).
+__global__ __launch_bounds__(64) void k_seed_poses(FssPairBatch pb, int flags, int B,
                                                    const float* __restrict__ table, float* q, float* t) {
   const int pair = blockIdx.x, j = threadIdx.x;
+  const bool use_pharm = flags & 1, ref_rule = flags & 2;
   const int* rr = use_pharm ? pb.ref_prng : pb.ref_rng;
   const int* fr = use_pharm ? pb.fit_prng : pb.fit_rng;
   const float4* rp = reinterpret_cast<const float4*>(use_pharm ? pb.ref_anc : pb.ref_pts) + rr[2 * pair];
   const float4* fp = reinterpret_cast<const float4*>(use_pharm ? pb.fit_anc : pb.fit_pts) + fr[2 * pair];
   const int N = rr[2 * pair + 1], M = fr[2 * pair + 1];
-  double c[6] = {0, 0, 0, 0, 0, 0};
-  for (int i = j; i < N; i += 64) { const float4 v = rp[i]; c[0] += v.x; c[1] += v.y; c[2] += v.z; }
-  for (int i = j; i < M; i += 64) { const float4 v = fp[i]; c[3] += v.x; c[4] += v.y; c[5] += v.z; }
-  for (int k = 0; k < 6; ++k) c[k] = group_sum(c[k], 64);
+  // first and second moments of both clouds: c[0..2], s[0..5] ref; c[3..5], s[6..11] fit
+  double m[18];
+  for (int k = 0; k < 18; ++k) m[k] = 0.0;
+  for (int i = j; i < N; i += 64) {
+    const double x = rp[i].x, y = rp[i].y, z = rp[i].z;
+    m[0] += x; m[1] += y; m[2] += z;
+    m[6] += x * x; m[7] += x * y; m[8] += x * z; m[9] += y * y; m[10] += y * z; m[11] += z * z;
+  }
+  for (int i = j; i < M; i += 64) {
+    const double x = fp[i].x, y = fp[i].y, z = fp[i].z;
+    m[3] += x; m[4] += y; m[5] += z;
+    m[12] += x * x; m[13] += x * y; m[14] += x * z; m[15] += y * y; m[16] += y * z; m[17] += z * z;
+  }
+  for (int k = 0; k < 18; ++k) m[k] = group_sum(m[k], 64);
   const double iN = N > 0 ? 1.0 / N : 0.0, iM = M > 0 ? 1.0 / M : 0.0;
+  const double cr[3] = {m[0] * iN, m[1] * iN, m[2] * iN}, cf[3] = {m[3] * iM, m[4] * iM, m[5] * iM};
+  double pca[4][4];
+  const bool do_pca = ref_rule && B >= 5;
+  if (do_pca) {
+    const double Cr[6] = {m[6] * iN - cr[0] * cr[0], m[7] * iN - cr[0] * cr[1], m[8] * iN - cr[0] * cr[2],
+                          m[9] * iN - cr[1] * cr[1], m[10] * iN - cr[1] * cr[2], m[11] * iN - cr[2] * cr[2]};
+    const double Cf[6] = {m[12] * iM - cf[0] * cf[0], m[13] * iM - cf[0] * cf[1], m[14] * iM - cf[0] * cf[2],
+                          m[15] * iM - cf[1] * cf[1], m[16] * iM - cf[1] * cf[2], m[17] * iM - cf[2] * cf[2]};
+    double ar[3][3], af[3][3];
+    principal_axes(Cr, ar);
+    principal_axes(Cf, af);
+    const double fl[4][2] = {{1, 1}, {-1, 1}, {1, -1}, {-1, -1}};
+    for (int k = 0; k < 4; ++k) {
+      const double t0[3] = {fl[k][0] * ar[0][0], fl[k][0] * ar[0][1], fl[k][0] * ar[0][2]};
+      const double t1[3] = {fl[k][1] * ar[1][0], fl[k][1] * ar[1][1], fl[k][1] * ar[1][2]};
+      double q1[4], q2[4], R1[9], qh[4], qn;
+      quat_between(af[0], t0, q1);
+      const float q1f[4] = {(float)q1[0], (float)q1[1], (float)q1[2], (float)q1[3]};
+      rotation_fp64(q1f, qh, qn, R1);
+      const double g1[3] = {R1[0] * af[1][0] + R1[1] * af[1][1] + R1[2] * af[1][2],
+                            R1[3] * af[1][0] + R1[4] * af[1][1] + R1[5] * af[1][2],
+                            R1[6] * af[1][0] + R1[7] * af[1][1] + R1[8] * af[1][2]};
+      quat_between(g1, t1, q2);
+      qmul(q2, q1, pca[k]);
+    }
+  }
   for (int r = j; r < B; r += 64) {
-    const float qq[4] = {table[4 * r], table[4 * r + 1], table[4 * r + 2], table[4 * r + 3]};
+    float qq[4] = {table[4 * r], table[4 * r + 1], table[4 * r + 2], table[4 * r + 3]};
+    if (do_pca && r >= 1 && r <= 4)
+      for (int k = 0; k < 4; ++k) qq[k] = (float)pca[r - 1][k];
+    if (ref_rule && r == 0) { qq[0] = 1.f; qq[1] = qq[2] = qq[3] = 0.f; }
     double qh[4], qn, R[9];
     rotation_fp64(qq, qh, qn, R);
     const long pose = (long)pair * B + r;
     for (int k = 0; k < 4; ++k) q[4 * pose + k] = qq[k];
+    const bool zero_t = ref_rule && B == 1;
     for (int i = 0; i < 3; ++i)
-      t[3 * pose + i] = (float)(c[i] * iN - (R[3 * i] * c[3] + R[3 * i + 1] * c[4] + R[3 * i + 2] * c[5]) * iM);
+      t[3 * pose + i] = zero_t ? 0.f : (float)(cr[i] - (R[3 * i] * cf[0] + R[3 * i + 1] * cf[1] + R[3 * i + 2] * cf[2]));
   }
 }
 
@@ -1038,7 +1185,8 @@ int fss_align(const FssPairBatch* batch, const FssScoreParams* sp, const FssAlig
   hipStream_t s = (hipStream_t)stream;
   const bool es = ap->early_stop != 0;
 #define FSS_ALIGN(M) \
-  (es ? launch_align<M, true>(batch, ks, lp, ak, (int)grid, block, lds, s) : launch_align<M, false>(batch, ks, lp, ak, (int)grid, block, lds, s))
+  (es ? launch_align<M, true>(batch, ks, lp, ak, (int)grid, block, lds, ap->variant, s) \
+      : launch_align<M, false>(batch, ks, lp, ak, (int)grid, block, lds, ap->variant, s))
   switch (ks.mode) {
     case FSS_MODE_SHAPE: FSS_ALIGN(FSS_MODE_SHAPE); break;
     case FSS_MODE_SHAPE_ESP: FSS_ALIGN(FSS_MODE_SHAPE_ESP); break;
@@ -1064,16 +1212,17 @@ int fss_align_finish(const FssPairBatch* batch, const FssScoreParams* sp, int32_
   return align_finish_impl(batch, ks, n_restarts, self_ov, out, (hipStream_t)stream);
 }
 
-int fss_seed_poses(const FssPairBatch* batch, int32_t use_pharm, int32_t n_restarts,
+int fss_seed_poses(const FssPairBatch* batch, int32_t flags, int32_t n_restarts,
                    const float* quat_table, float* q, float* t, void* stream) {
   if (!batch || !quat_table || !q || !t) return set_error(FSS_E_INVALID, "NULL seed arguments");
   if (n_restarts <= 0) return set_error(FSS_E_INVALID, "n_restarts must be > 0");
+  const bool use_pharm = flags & FSS_SEED_PHARM;
   if (use_pharm ? (!batch->ref_prng || !batch->fit_prng || !batch->ref_anc || !batch->fit_anc)
                 : (!batch->ref_rng || !batch->fit_rng || !batch->ref_pts || !batch->fit_pts))
     return set_error(FSS_E_INVALID, "cloud arrays are NULL");
   if (batch->n_pairs <= 0) return FSS_OK;
   hipLaunchKernelGGL(k_seed_poses, dim3(batch->n_pairs), dim3(64), 0, (hipStream_t)stream, *batch,
-                     use_pharm, n_restarts, quat_table, q, t);
+                     (int)flags, n_restarts, quat_table, q, t);
   return check_launch("fss_seed_poses");
 }
 

@@ -94,6 +94,8 @@ typedef struct FssAlignParams {
   const int32_t* pair_order; /* device (n_pairs,) launch order of the pairs (costliest first),
                                 or NULL for 0..n_pairs-1; results are always indexed by pair  */
   int32_t skip_finish;       /* 1: run the optimiser only; call fss_align_finish afterwards    */
+  int32_t variant;           /* shape inner loop: 0 default (expanded form), 1 direct form,
+                                2 expanded form (exp2 of 2a.b' + W_a, factor exp2(-|b'|^2))    */
 } FssAlignParams;
 
 typedef struct FssAlignOut {
@@ -202,12 +204,19 @@ int fss_align_finish(const FssPairBatch* batch, const FssScoreParams* sp, int32_
                      const double* self_ov, FssAlignOut* out, void* stream);
 
 /*
- * Restart seeds on device: pose r of every pair gets quat_table[r] (n_restarts x 4, w x y z) and
- * t = centroid(ref) - R(q) centroid(fit), the centre-of-mass alignment of every seed of
- * alignment/_torch.py:253-326 (centroids of the pharmacophore anchors when use_pharm != 0).
+ * Restart seeds on device (one wave per pair): pose r gets quat_table[r] (n_restarts x 4, w x y z)
+ * and t = centroid(ref) - R(q) centroid(fit).  flags:
+ *   FSS_SEED_PHARM  centroids (and axes) of the pharmacophore anchors instead of the shape clouds;
+ *   FSS_SEED_REF    the reference's seed rule: row 0 is the identity, rows 1-4 the principal-axis
+ *                   alignments of the fit onto the ref (FP64 Jacobi eigensolver; n_restarts >= 5),
+ *                   and n_restarts == 1 means identity with zero translation.
  * q (n_pairs*B, 4), t (n_pairs*B, 3).
+ * Replaces: _initialize_se3_params alignment/_torch.py:253-326 with
+ *   quaternions_for_principal_component_alignment_np alignment/utils/pca_np.py:111-166.
  */
-int fss_seed_poses(const FssPairBatch* batch, int32_t use_pharm, int32_t n_restarts,
+#define FSS_SEED_PHARM 1
+#define FSS_SEED_REF   2
+int fss_seed_poses(const FssPairBatch* batch, int32_t flags, int32_t n_restarts,
                    const float* quat_table, float* q, float* t, void* stream);
 
 /*

@@ -177,3 +177,78 @@ def test_edge_cases(dev):
     so = torch.ones((1, 4), dtype=torch.float64, device=dev)
     with pytest.raises(_lib.FssError, match="LDS"):
         align(pbb, ScoreSpec(), to_dev(q[:1], dev), to_dev(t[:1], dev), OptimSpec(max_steps=1), self_ov=so)
+
+
+def test_graph_replays_match_eager(dev):
+    """The captured step sequence, replayed several times with eager work in between, gives the
+    eager result on every replay (no memset nodes: all state resets are kernels)."""
+    from fast_shepherd_score_amd.align import SteppedAligner
+    n, B = 5, 8
+    pb, spec, _ = make_case(_lib.MODE_SHAPE_ESP, n, dev, seed=21)
+    q, t = random_seeds(n, B, seed=22)
+    qd, td = to_dev(q, dev), to_dev(t, dev)
+    opt = OptimSpec(max_steps=30, early_stop=True)
+    eager = SteppedAligner(pb, spec, qd, td, opt).run(use_graph=False)
+    st = SteppedAligner(pb, spec, qd, td, opt)
+    st.capture()
+    for rep in range(3):
+        st.q.copy_(qd)
+        st.t.copy_(td)
+        st.m.zero_()
+        st.v.zero_()
+        st.last.fill_(1.0)
+        st.counter.zero_()
+        st.active.fill_(1)
+        st.n_steps.zero_()
+        r = st.run(use_graph=True)
+        torch.cuda.synchronize()
+        assert torch.equal(r.pose_score, eager.pose_score), rep
+        assert torch.equal(r.n_steps, eager.n_steps), rep
+        # eager work between replays
+        align(pb, spec, qd, td, OptimSpec(max_steps=5))
+        torch.cuda.synchronize()
+
+
+@pytest.mark.parametrize("mode", [_lib.MODE_SHAPE, _lib.MODE_SHAPE_ESP, _lib.MODE_SHAPE_PHARM])
+def test_inner_loop_variants_agree(dev, mode):
+    """The direct and the expanded-form shape inner loops compute the same optimisation."""
+    pb, spec, pairs = make_case(mode, 4, dev, seed=23)
+    q, t = random_seeds(4, 8, seed=24)
+    opt = OptimSpec(max_steps=60, early_stop=False)
+    a = align(pb, spec, to_dev(q, dev), to_dev(t, dev), opt, variant=1)
+    b = align(pb, spec, to_dev(q, dev), to_dev(t, dev), opt, variant=2)
+    np.testing.assert_allclose(a.pose_score.cpu().numpy(), b.pose_score.cpu().numpy(), atol=2e-4)
+    for i, p in enumerate(pairs):
+        o = orc.optimize(p, q[i], t[i], max_steps=60, early_stop=False)
+        assert abs(float(a.score[i]) - o["score"]) < 1e-3
+
+
+def test_device_seeds_match_reference_rule(dev):
+    """fss_seed_poses with the reference rule == the host restatement of _initialize_se3_params:
+    identity and Fibonacci rows exactly, the four principal-axis rows as a set."""
+    from fast_shepherd_score_amd.packing import CloudSet, PairBatch
+    from fast_shepherd_score_amd.seeds import initial_seeds, pad, quats_from_fibo, rotmat
+    from fast_shepherd_score_amd import synth
+    ref, fit = synth.vol_pairs(6, 15, 40, seed=25)
+    fit = [f + np.array([0.5, -1.0, 2.0], np.float32) for f in fit]
+    pb = PairBatch(CloudSet.from_arrays(ref, device=dev), CloudSet.from_arrays(fit, device=dev))
+    B = 12
+    table = np.zeros((B, 4), np.float32)
+    table[5:] = quats_from_fibo(B - 5)
+    table_d = to_dev(table, dev)
+    q = torch.empty((6, B, 4), dtype=torch.float32, device=dev)
+    t = torch.empty((6, B, 3), dtype=torch.float32, device=dev)
+    bs = pb.struct()
+    _lib.call("fss_seed_poses", _lib.byref(bs), 2, B, _lib.ptr(table_d), _lib.ptr(q), _lib.ptr(t),
+              int(torch.cuda.current_stream().cuda_stream))
+    q, t = q.cpu().numpy(), t.cpu().numpy()
+    (R, rn), (F, fn) = pad(ref), pad(fit)
+    qh, th = initial_seeds(R, rn, F, fn, B)
+    np.testing.assert_allclose(q[:, 0], qh[:, 0], atol=1e-7)
+    np.testing.assert_allclose(q[:, 5:], qh[:, 5:], atol=1e-6)
+    np.testing.assert_allclose(t[:, [0] + list(range(5, B))], th[:, [0] + list(range(5, B))], atol=1e-4)
+    for i in range(6):
+        Rd = rotmat(q[i, 1:5].astype(np.float64))
+        Rh = rotmat(qh[i, 1:5].astype(np.float64))
+        for k in range(4):      # every host PCA rotation appears among the device ones
+            assert min(np.abs(Rd[m] - Rh[k]).max() for m in range(4)) < 1e-4, (i, k)
