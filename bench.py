@@ -115,6 +115,7 @@ def bench_c1(dev, reps: int = 3):
         return (time.perf_counter() - t0) / n, r
 
     tp, rp = timed(lambda: align(pb, spec, qd, td, opt), reps)
+    align(pb, spec, qd, td, opt, tune=True)
     st = SteppedAligner(pb, spec, qd, td, opt)
     st.capture()
 
@@ -155,7 +156,7 @@ def bench_c3(dev, n_pairs: int = 16384):
     spec = ScoreSpec(mode=_lib.MODE_SHAPE_PHARM, color_weight=0.5, directionless=False)
     opt = OptimSpec(max_steps=200, early_stop=False)
     qd, td = torch.from_numpy(q).to(dev), torch.from_numpy(t).to(dev)
-    align(pb, spec, qd, td, opt)
+    align(pb, spec, qd, td, opt, tune=True)
     torch.cuda.synchronize()
     ev = []
     t0 = time.perf_counter()
@@ -168,7 +169,7 @@ def bench_c3(dev, n_pairs: int = 16384):
     return {"workload": f"{n_pairs} uneven pairs: 5-60 atoms, 2-20 pharmacophores (8 types, directional), "
                         "(1-w) shape + w pharm Tanimoto, w=0.5, 50 restarts x 200 steps, one persistent launch",
             "pairs_per_s": n_pairs / dt, "kernel_ms": kms, "lanes_per_pose": res.extra["lanes_per_pose"],
-            "restarts_per_wg": res.extra["restarts_per_wg"],
+            "restarts_per_wg": res.extra["restarts_per_wg"], "inner_loop_variant": res.extra["variant"],
             "achieved_tflops": (shape_terms * 16 + ph_terms * 36) / (kms * 1e-3) / 1e12,
             "score_mean": float(res.score.mean())}
 
@@ -221,6 +222,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--lpp", type=int, default=0)
     ap.add_argument("--rpw", type=int, default=0)
+    ap.add_argument("--variant", type=int, default=0, help="shape inner loop: 1 direct, 2 expanded")
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--no-extra", action="store_true", help="skip the C1 / C3 / screening legs")
     ap.add_argument("--screen-lib", type=int, default=1_000_000)
@@ -264,6 +266,14 @@ def main():
     if args.rpw:
         kw["restarts_per_wg"] = args.rpw
 
+    if not kw:   # autotune tiling + inner-loop variant once, outside the timed region
+        from fast_shepherd_score_amd.autotune import autotune
+        lpp, rpw, var = autotune(pb, spec, qd, td, opt.early_stop, self_ov=self_overlaps(pb, spec))
+        kw = dict(lanes_per_pose=lpp, restarts_per_wg=rpw, variant=var)
+        log(f"autotuned: lanes_per_pose={lpp} restarts_per_wg={rpw} variant={var}")
+    elif args.variant:
+        kw["variant"] = args.variant
+
     def one(events=None):
         so = self_overlaps(pb, spec)
         return align(pb, spec, qd, td, opt, self_ov=so, events=events, **kw)
@@ -298,7 +308,8 @@ def main():
                                f"Adam steps (all executed), lr 0.1, alpha 0.81, persistent launch",
                    "pairs": args.pairs, "restarts": B, "opt_steps": steps,
                    "lanes_per_pose": res.extra["lanes_per_pose"],
-                   "restarts_per_wg": res.extra["restarts_per_wg"]},
+                   "restarts_per_wg": res.extra["restarts_per_wg"],
+                   "inner_loop": {0: "expanded", 1: "direct", 2: "expanded"}[res.extra["variant"]]},
         "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_FP32_TFLOPS,
                      "unit": "TFLOP/s", "frac": achieved / PEAK_FP32_TFLOPS, "traffic": None,
                      "kernel": "k_align", "kernel_ms": kern_ms, "finish_ms": fin_ms,

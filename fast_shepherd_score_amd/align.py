@@ -116,9 +116,11 @@ def tiling_efficiency(counts: np.ndarray, B: int, lpp: int, rpw: int) -> float:
 
 
 def candidate_tilings(B: int, early_stop: bool):
+    divisors = [d for d in range(1, B + 1) if B % d == 0]
     for lpp in (1, 2, 4, 8, 16, 32, 64):
-        rpws = [B] if early_stop else sorted({r for r in (B, (B + 1) // 2, (B + 2) // 3, (B + 3) // 4,
-                                                         8, 16, 32) if r >= 1})
+        rpws = [B] if early_stop else sorted({r for r in divisors + [(B + 1) // 2, (B + 2) // 3,
+                                                                    (B + 3) // 4, 8, 16, 32]
+                                              if 1 <= r <= B})
         for rpw in rpws:
             if rpw * lpp <= 1024 and rpw * lpp >= 64 or (rpw * lpp < 64 and rpw == B):
                 yield lpp, min(rpw, B)
@@ -199,7 +201,11 @@ def align(pb: PairBatch, spec: ScoreSpec, q0: torch.Tensor, t0: torch.Tensor,
         from . import autotune as _at
         tuned = _at.autotune(pb, spec, q0, t0, opt.early_stop, self_ov=self_ov) if tune else \
             _at.cached(pb, spec.mode, B, opt.early_stop)
-        lpp0, rpw0 = tuned if tuned is not None else default_tiling(pb, spec.mode, B, opt.early_stop)
+        if tuned is not None:
+            lpp0, rpw0, var0 = tuned
+            variant = variant or var0
+        else:
+            lpp0, rpw0 = default_tiling(pb, spec.mode, B, opt.early_stop)
     else:
         lpp0, rpw0 = default_tiling(pb, spec.mode, B, opt.early_stop)
     lpp = lanes_per_pose or lpp0
@@ -243,7 +249,8 @@ def align(pb: PairBatch, spec: ScoreSpec, q0: torch.Tensor, t0: torch.Tensor,
         ev[2].record(ts)
         events.append(tuple(ev))
     return AlignResult(best, bidx, tf, q, t, pose_score, nst,
-                       extra={"lanes_per_pose": lpp, "restarts_per_wg": rpw, "order": order_t})
+                       extra={"lanes_per_pose": lpp, "restarts_per_wg": rpw, "variant": int(variant),
+                              "order": order_t})
 
 
 class SteppedAligner:

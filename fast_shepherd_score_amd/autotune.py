@@ -1,8 +1,10 @@
-"""Autotuner for the persistent optimiser's tiling (lanes per pose, restarts per workgroup).
+"""Autotuner for the persistent optimiser: tiling (lanes per pose, restarts per workgroup) and
+shape inner-loop variant (direct / expanded form).
 
 The analytic model (:func:`fast_shepherd_score_amd.align.tiling_cost`) ranks every legal tiling;
-the tuner then times the top candidates on the actual batch with a short optimiser run (a few
-steps, early stop off, so each candidate does identical work) and keeps the fastest.  Results are
+the tuner times the model's top candidates plus the wide-group tilings (64 lanes per pose), each
+with both inner-loop variants, on the actual batch with a short optimiser run (a few steps, early
+stop off, so every candidate does identical work) and keeps the fastest.  Results are
 cached in memory and, when ``FSS_AUTOTUNE_CACHE`` names a file, on disk as JSON, keyed by mode,
 restart count, early-stop flag and the batch's size profile.
 """
@@ -21,7 +23,7 @@ from .align import (OptimSpec, ScoreSpec, align, candidate_tilings, self_overlap
                     tiling_efficiency)
 from .packing import PairBatch
 
-_cache: Dict[str, Tuple[int, int]] = {}
+_cache: Dict[str, Tuple[int, int, int]] = {}
 _lock = threading.Lock()
 
 
@@ -47,7 +49,8 @@ def _load_disk() -> None:
         try:
             with open(path) as f:
                 for k, v in json.load(f).items():
-                    _cache.setdefault(k, (int(v[0]), int(v[1])))
+                    if len(v) == 3:
+                        _cache.setdefault(k, (int(v[0]), int(v[1]), int(v[2])))
         except (OSError, ValueError):
             pass
 
@@ -63,9 +66,9 @@ def _save_disk() -> None:
 
 
 def autotune(pb: PairBatch, spec: ScoreSpec, q0: torch.Tensor, t0: torch.Tensor, early_stop: bool,
-             *, top: int = 5, steps: int = 8, self_ov: Optional[torch.Tensor] = None,
-             verbose: bool = False) -> Tuple[int, int]:
-    """Best (lanes_per_pose, restarts_per_wg) for this batch (timed, cached)."""
+             *, top: int = 4, steps: int = 8, self_ov: Optional[torch.Tensor] = None,
+             verbose: bool = False) -> Tuple[int, int, int]:
+    """Best (lanes_per_pose, restarts_per_wg, variant) for this batch (timed, cached)."""
     B = int(q0.shape[1])
     key = shape_key(pb, spec.mode, B, early_stop)
     with _lock:
@@ -74,29 +77,37 @@ def autotune(pb: PairBatch, spec: ScoreSpec, q0: torch.Tensor, t0: torch.Tensor,
         if key in _cache:
             return _cache[key]
     rc, fc = _counts(pb, spec.mode)
-    cands = sorted(((tiling_cost(rc, fc, B, l, r), -tiling_efficiency(fc, B, l, r), l, r)
-                    for l, r in candidate_tilings(B, early_stop)))[:top]
+    ranked = sorted(((tiling_cost(rc, fc, B, l, r), -tiling_efficiency(fc, B, l, r), l, r)
+                     for l, r in candidate_tilings(B, early_stop)))
+    tilings = [(l, r) for _, _, l, r in ranked[:top]]
+    wide = [(l, r) for _, e, l, r in sorted(ranked, key=lambda c: c[1]) if l == 64][:2]
+    tilings += [t for t in wide if t not in tilings]
+    variants = (1, 2) if spec.mode != _lib.MODE_PHARM else (0,)
     if self_ov is None:
         self_ov = self_overlaps(pb, spec)
     opt = OptimSpec(max_steps=steps, early_stop=False)
     best, best_ms = None, float("inf")
-    for _, _, lpp, rpw in cands:
-        ev = []
-        align(pb, spec, q0, t0, opt, lanes_per_pose=lpp, restarts_per_wg=rpw, self_ov=self_ov)
-        align(pb, spec, q0, t0, opt, lanes_per_pose=lpp, restarts_per_wg=rpw, self_ov=self_ov, events=ev)
-        torch.cuda.synchronize()
-        ms = ev[0][0].elapsed_time(ev[0][1])
-        if verbose:
-            print(f"autotune {key}: lpp={lpp} rpw={rpw} {ms:.3f} ms", flush=True)
-        if ms < best_ms:
-            best, best_ms = (lpp, rpw), ms
+    for lpp, rpw in tilings:
+        if early_stop and rpw < B:
+            continue
+        for var in variants:
+            ev = []
+            kw = dict(lanes_per_pose=lpp, restarts_per_wg=rpw, self_ov=self_ov, variant=var)
+            align(pb, spec, q0, t0, opt, **kw)
+            align(pb, spec, q0, t0, opt, events=ev, **kw)
+            torch.cuda.synchronize()
+            ms = ev[0][0].elapsed_time(ev[0][1])
+            if verbose:
+                print(f"autotune {key}: lpp={lpp} rpw={rpw} variant={var} {ms:.3f} ms", flush=True)
+            if ms < best_ms:
+                best, best_ms = (lpp, rpw, var), ms
     with _lock:
         _cache[key] = best
         _save_disk()
     return best
 
 
-def cached(pb: PairBatch, mode: int, B: int, early_stop: bool) -> Optional[Tuple[int, int]]:
+def cached(pb: PairBatch, mode: int, B: int, early_stop: bool) -> Optional[Tuple[int, int, int]]:
     with _lock:
         if not _cache:
             _load_disk()
