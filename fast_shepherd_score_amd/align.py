@@ -118,8 +118,11 @@ def tiling_efficiency(counts: np.ndarray, B: int, lpp: int, rpw: int) -> float:
 def candidate_tilings(B: int, early_stop: bool):
     divisors = [d for d in range(1, B + 1) if B % d == 0]
     for lpp in (1, 2, 4, 8, 16, 32, 64):
+        one_wave = max(1, 64 // lpp)
         rpws = [B] if early_stop else sorted({r for r in divisors + [(B + 1) // 2, (B + 2) // 3,
-                                                                    (B + 3) // 4, 8, 16, 32]
+                                                                    (B + 3) // 4, one_wave,
+                                                                    2 * one_wave, 4 * one_wave,
+                                                                    8, 16, 32]
                                               if 1 <= r <= B})
         for rpw in rpws:
             if rpw * lpp <= 1024 and rpw * lpp >= 64 or (rpw * lpp < 64 and rpw == B):
@@ -129,20 +132,26 @@ def candidate_tilings(B: int, early_stop: bool):
 N_SIMD = 256 * 4          # MI355X: 256 CUs x 4 SIMDs
 
 
-def tiling_cost(ref_counts, fit_counts, B: int, lpp: int, rpw: int) -> float:
+def tiling_cost(ref_counts, fit_counts, B: int, lpp: int, rpw: int, waves_per_simd: int = 4) -> float:
     """Modelled optimiser-step time of a tiling (arbitrary units).
 
-    A wave's serial work is its lanes' two-row passes over the ref cloud (about 12 issue slots
-    per pair term) plus the butterfly reduction (13 sums x log2(lpp) shuffles) and the per-pose
-    tail; the chip retires the waves of the launch N_SIMD at a time, and a SIMD needs two waves
-    to hide latency, so the step takes per-wave-work x max(waves / N_SIMD, 1/2)."""
+    A wave's serial work is its lanes' two-row passes over the ref cloud (about 10 issue slots
+    per pair term) plus the butterfly reduction and the per-pose tail.  The chip retires the
+    launch's waves N_SIMD at a time; a CU holds as many workgroups as fit in ``4 x
+    waves_per_simd`` wave slots, and a workgroup whose wave count does not divide evenly over
+    the CU's 4 SIMDs leaves SIMDs idle (the measured MI355X behaviour that favours workgroups
+    of 1, 2, 4 or 8 waves)."""
     n = np.maximum(np.asarray(ref_counts, np.float64), 1.0)
     m = np.maximum(np.asarray(fit_counts, np.float64), 1.0)
     passes = np.ceil(m / (2.0 * lpp))
-    per_wave = passes * 2.0 * n * 12.0 + 26.0 * np.log2(max(lpp, 1)) + 150.0
-    waves_per_pair = np.ceil(B / rpw) * np.ceil(rpw * lpp / 64.0)
-    waves = float(waves_per_pair * len(m))
-    return float(np.mean(per_wave)) * max(waves / N_SIMD, 0.5)
+    per_wave = passes * 2.0 * n * 10.0 + 26.0 * np.log2(max(lpp, 1)) + 150.0
+    wpg = int(np.ceil(rpw * lpp / 64.0))
+    slots = 4 * waves_per_simd
+    wgs_per_cu = max(1, slots // wpg)
+    resident = wgs_per_cu * wpg
+    simd_eff = resident / (4.0 * np.ceil(resident / 4.0))
+    waves = float(np.ceil(B / rpw) * wpg * len(m))
+    return float(np.mean(per_wave)) * max(waves / (N_SIMD * simd_eff), 0.5)
 
 
 def default_tiling(pb: PairBatch, mode: int, B: int, early_stop: bool) -> Tuple[int, int]:

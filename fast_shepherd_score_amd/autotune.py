@@ -66,7 +66,7 @@ def _save_disk() -> None:
 
 
 def autotune(pb: PairBatch, spec: ScoreSpec, q0: torch.Tensor, t0: torch.Tensor, early_stop: bool,
-             *, top: int = 4, steps: int = 8, self_ov: Optional[torch.Tensor] = None,
+             *, top: int = 6, steps: int = 8, self_ov: Optional[torch.Tensor] = None,
              verbose: bool = False) -> Tuple[int, int, int]:
     """Best (lanes_per_pose, restarts_per_wg, variant) for this batch (timed, cached)."""
     B = int(q0.shape[1])

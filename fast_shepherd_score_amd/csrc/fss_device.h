@@ -127,27 +127,41 @@ __device__ __forceinline__ void direct_pair(const float4* __restrict__ ref, cons
   }
 }
 
+template <bool GRAD>
+__device__ __forceinline__ void accumulate_rows(Acc& acc, const float4& p0, float v0, float fx0, float fy0,
+                                                float fz0, const float4& p1, float v1, float fx1, float fy1,
+                                                float fz1) {
+  acc.V += v0 + v1;
+  if (GRAD) {
+    acc.gt[0] += fx0 + fx1; acc.gt[1] += fy0 + fy1; acc.gt[2] += fz0 + fz1;
+    acc.G[0] = fmaf(fx0, p0.x, fmaf(fx1, p1.x, acc.G[0]));
+    acc.G[1] = fmaf(fx0, p0.y, fmaf(fx1, p1.y, acc.G[1]));
+    acc.G[2] = fmaf(fx0, p0.z, fmaf(fx1, p1.z, acc.G[2]));
+    acc.G[3] = fmaf(fy0, p0.x, fmaf(fy1, p1.x, acc.G[3]));
+    acc.G[4] = fmaf(fy0, p0.y, fmaf(fy1, p1.y, acc.G[4]));
+    acc.G[5] = fmaf(fy0, p0.z, fmaf(fy1, p1.z, acc.G[5]));
+    acc.G[6] = fmaf(fz0, p0.x, fmaf(fz1, p1.x, acc.G[6]));
+    acc.G[7] = fmaf(fz0, p0.y, fmaf(fz1, p1.y, acc.G[7]));
+    acc.G[8] = fmaf(fz0, p0.z, fmaf(fz1, p1.z, acc.G[8]));
+  }
+}
+
 // Direct-form twin of shape_accum (the other inner-loop variant): LDS rows ref4[a] = (a~, c~_a),
 // fit4[b] = (s b, c~_b); a pair term is 3 sub + 3 FMA/mul (+ 1 sub + 1 FMA for ESP) + v_exp_f32
 // (negated operand) + 1 add + 3 FMA.  No radius limit.
 template <bool ESP, bool GRAD>
-__device__ __forceinline__ void shape_accum_direct(const float4* __restrict__ ref, int N,
-                                                   const float4* __restrict__ fit, int M, int j, int lpp,
-                                                   const float R[9], const float ts[3], Acc& acc) {
-  for (int b0 = j; b0 < M; b0 += 2 * lpp) {
-    const int b1r = b0 + lpp;
-    const bool has1 = b1r < M;
-    const int b1 = has1 ? b1r : b0;
-    const float4 p0 = fit[b0];
-    const float4 p1 = fit[b1];
-    const float x0 = fmaf(R[0], p0.x, fmaf(R[1], p0.y, fmaf(R[2], p0.z, ts[0])));
-    const float y0 = fmaf(R[3], p0.x, fmaf(R[4], p0.y, fmaf(R[5], p0.z, ts[1])));
-    const float z0 = fmaf(R[6], p0.x, fmaf(R[7], p0.y, fmaf(R[8], p0.z, ts[2])));
+__device__ __forceinline__ void direct_rows(const float4* __restrict__ ref, int N, const float4& p0,
+                                            const float4& p1, const float R[9], const float ts[3],
+                                            bool two, Acc& acc) {
+  const float x0 = fmaf(R[0], p0.x, fmaf(R[1], p0.y, fmaf(R[2], p0.z, ts[0])));
+  const float y0 = fmaf(R[3], p0.x, fmaf(R[4], p0.y, fmaf(R[5], p0.z, ts[1])));
+  const float z0 = fmaf(R[6], p0.x, fmaf(R[7], p0.y, fmaf(R[8], p0.z, ts[2])));
+  float v0 = 0.f, v1 = 0.f;
+  float fx0 = 0.f, fy0 = 0.f, fz0 = 0.f, fx1 = 0.f, fy1 = 0.f, fz1 = 0.f;
+  if (two) {
     const float x1 = fmaf(R[0], p1.x, fmaf(R[1], p1.y, fmaf(R[2], p1.z, ts[0])));
     const float y1 = fmaf(R[3], p1.x, fmaf(R[4], p1.y, fmaf(R[5], p1.z, ts[1])));
     const float z1 = fmaf(R[6], p1.x, fmaf(R[7], p1.y, fmaf(R[8], p1.z, ts[2])));
-    float v0 = 0.f, v1 = 0.f;
-    float fx0 = 0.f, fy0 = 0.f, fz0 = 0.f, fx1 = 0.f, fy1 = 0.f, fz1 = 0.f;
 #pragma unroll 4
     for (int a = 0; a < N; ++a) {
       const float4 A = ref[a];
@@ -169,20 +183,60 @@ __device__ __forceinline__ void shape_accum_direct(const float4* __restrict__ re
         fx1 = fmaf(e1, dx1, fx1); fy1 = fmaf(e1, dy1, fy1); fz1 = fmaf(e1, dz1, fz1);
       }
     }
-    if (!has1) { v1 = 0.f; fx1 = fy1 = fz1 = 0.f; }
-    acc.V += v0 + v1;
-    if (GRAD) {
-      acc.gt[0] += fx0 + fx1; acc.gt[1] += fy0 + fy1; acc.gt[2] += fz0 + fz1;
-      acc.G[0] = fmaf(fx0, p0.x, fmaf(fx1, p1.x, acc.G[0]));
-      acc.G[1] = fmaf(fx0, p0.y, fmaf(fx1, p1.y, acc.G[1]));
-      acc.G[2] = fmaf(fx0, p0.z, fmaf(fx1, p1.z, acc.G[2]));
-      acc.G[3] = fmaf(fy0, p0.x, fmaf(fy1, p1.x, acc.G[3]));
-      acc.G[4] = fmaf(fy0, p0.y, fmaf(fy1, p1.y, acc.G[4]));
-      acc.G[5] = fmaf(fy0, p0.z, fmaf(fy1, p1.z, acc.G[5]));
-      acc.G[6] = fmaf(fz0, p0.x, fmaf(fz1, p1.x, acc.G[6]));
-      acc.G[7] = fmaf(fz0, p0.y, fmaf(fz1, p1.y, acc.G[7]));
-      acc.G[8] = fmaf(fz0, p0.z, fmaf(fz1, p1.z, acc.G[8]));
+  } else {
+#pragma unroll 4
+    for (int a = 0; a < N; ++a) {
+      const float4 A = ref[a];
+      const float dx0 = A.x - x0, dy0 = A.y - y0, dz0 = A.z - z0;
+      float s0 = fmaf(dx0, dx0, fmaf(dy0, dy0, dz0 * dz0));
+      if (ESP) {
+        const float c0 = A.w - p0.w;
+        s0 = fmaf(c0, c0, s0);
+      }
+      const float e0 = fast_exp2(-s0);
+      v0 += e0;
+      if (GRAD) { fx0 = fmaf(e0, dx0, fx0); fy0 = fmaf(e0, dy0, fy0); fz0 = fmaf(e0, dz0, fz0); }
     }
+  }
+  accumulate_rows<GRAD>(acc, p0, v0, fx0, fy0, fz0, p1, v1, fx1, fy1, fz1);
+}
+
+template <bool ESP, bool GRAD>
+__device__ __forceinline__ void shape_accum_direct(const float4* __restrict__ ref, int N,
+                                                   const float4* __restrict__ fit, int M, int j, int lpp,
+                                                   const float R[9], const float ts[3], Acc& acc) {
+  int b0 = j;
+  for (; b0 + lpp < M; b0 += 2 * lpp) direct_rows<ESP, GRAD>(ref, N, fit[b0], fit[b0 + lpp], R, ts, true, acc);
+  if (b0 < M) {
+    const float4 p0 = fit[b0];
+    direct_rows<ESP, GRAD>(ref, N, p0, p0, R, ts, false, acc);
+  }
+}
+
+// One fit row against the whole ref cloud, expanded form (the tail pass of shape_accum).
+template <bool ESP, bool GRAD>
+__device__ __forceinline__ void expanded_row(const float4* __restrict__ ref, const float* __restrict__ refc,
+                                             int N, float x, float y, float z, float c, float& v,
+                                             float& fx, float& fy, float& fz) {
+  const float h = fmaf(x, x, fmaf(y, y, fmaf(z, z, c * c)));
+  if (h <= kHMax) {
+    float sx = 0.f, sy = 0.f, sz = 0.f, acc = 0.f;
+#pragma unroll 4
+    for (int a = 0; a < N; ++a) {
+      const float4 A = ref[a];
+      float g = A.w;
+      if (ESP) g = fmaf(refc[a], c, g);
+      g = fmaf(A.x, x, fmaf(A.y, y, fmaf(A.z, z, g)));
+      const float e = fast_exp2(g);
+      acc += e;
+      if (GRAD) { sx = fmaf(e, A.x, sx); sy = fmaf(e, A.y, sy); sz = fmaf(e, A.z, sz); }
+    }
+    const float F = fast_exp2(-h);
+    v = acc * F;
+    if (GRAD) { fx = fmaf(0.5f * F, sx, -x * v); fy = fmaf(0.5f * F, sy, -y * v); fz = fmaf(0.5f * F, sz, -z * v); }
+  } else {
+    v = fx = fy = fz = 0.f;
+    direct_pair<ESP>(ref, refc, N, x, y, z, c, v, fx, fy, fz);
   }
 }
 
@@ -190,10 +244,10 @@ template <bool ESP, bool GRAD>
 __device__ __forceinline__ void shape_accum(const float4* __restrict__ ref, const float* __restrict__ refc,
                                             int N, const float4* __restrict__ fit, int M, int j, int lpp,
                                             const float R[9], const float ts[3], Acc& acc) {
-  for (int b0 = j; b0 < M; b0 += 2 * lpp) {
-    const int b1r = b0 + lpp;
-    const bool has1 = b1r < M;
-    const int b1 = has1 ? b1r : b0;
+  int b0 = j;
+  // two-row passes while this lane has at least two rows left
+  for (; b0 + lpp < M; b0 += 2 * lpp) {
+    const int b1 = b0 + lpp;
     const float4 p0 = fit[b0];
     const float4 p1 = fit[b1];
     const float x0 = fmaf(R[0], p0.x, fmaf(R[1], p0.y, fmaf(R[2], p0.z, ts[0])));
@@ -240,20 +294,17 @@ __device__ __forceinline__ void shape_accum(const float4* __restrict__ ref, cons
       direct_pair<ESP>(ref, refc, N, x0, y0, z0, c0, v0, fx0, fy0, fz0);
       direct_pair<ESP>(ref, refc, N, x1, y1, z1, c1, v1, fx1, fy1, fz1);
     }
-    if (!has1) { v1 = 0.f; fx1 = fy1 = fz1 = 0.f; }
-    acc.V += v0 + v1;
-    if (GRAD) {
-      acc.gt[0] += fx0 + fx1; acc.gt[1] += fy0 + fy1; acc.gt[2] += fz0 + fz1;
-      acc.G[0] = fmaf(fx0, p0.x, fmaf(fx1, p1.x, acc.G[0]));
-      acc.G[1] = fmaf(fx0, p0.y, fmaf(fx1, p1.y, acc.G[1]));
-      acc.G[2] = fmaf(fx0, p0.z, fmaf(fx1, p1.z, acc.G[2]));
-      acc.G[3] = fmaf(fy0, p0.x, fmaf(fy1, p1.x, acc.G[3]));
-      acc.G[4] = fmaf(fy0, p0.y, fmaf(fy1, p1.y, acc.G[4]));
-      acc.G[5] = fmaf(fy0, p0.z, fmaf(fy1, p1.z, acc.G[5]));
-      acc.G[6] = fmaf(fz0, p0.x, fmaf(fz1, p1.x, acc.G[6]));
-      acc.G[7] = fmaf(fz0, p0.y, fmaf(fz1, p1.y, acc.G[7]));
-      acc.G[8] = fmaf(fz0, p0.z, fmaf(fz1, p1.z, acc.G[8]));
-    }
+    accumulate_rows<GRAD>(acc, p0, v0, fx0, fy0, fz0, p1, v1, fx1, fy1, fz1);
+  }
+  // single-row tail pass (lanes with an odd row count)
+  if (b0 < M) {
+    const float4 p0 = fit[b0];
+    const float x0 = fmaf(R[0], p0.x, fmaf(R[1], p0.y, fmaf(R[2], p0.z, ts[0])));
+    const float y0 = fmaf(R[3], p0.x, fmaf(R[4], p0.y, fmaf(R[5], p0.z, ts[1])));
+    const float z0 = fmaf(R[6], p0.x, fmaf(R[7], p0.y, fmaf(R[8], p0.z, ts[2])));
+    float v0, fx0, fy0, fz0;
+    expanded_row<ESP, GRAD>(ref, refc, N, x0, y0, z0, ESP ? p0.w : 0.f, v0, fx0, fy0, fz0);
+    accumulate_rows<GRAD>(acc, p0, v0, fx0, fy0, fz0, p0, 0.f, 0.f, 0.f, 0.f);
   }
 }
 
