@@ -268,7 +268,8 @@ def main():
 
     if not kw:   # autotune tiling + inner-loop variant once, outside the timed region
         from fast_shepherd_score_amd.autotune import autotune
-        lpp, rpw, var = autotune(pb, spec, qd, td, opt.early_stop, self_ov=self_overlaps(pb, spec))
+        lpp, rpw, var = autotune(pb, spec, qd, td, opt.early_stop, max_steps=opt.max_steps,
+                                 self_ov=self_overlaps(pb, spec))
         kw = dict(lanes_per_pose=lpp, restarts_per_wg=rpw, variant=var)
         log(f"autotuned: lanes_per_pose={lpp} restarts_per_wg={rpw} variant={var}")
     elif args.variant:

@@ -105,14 +105,18 @@ def _next_pow2(x: int) -> int:
 
 def tiling_efficiency(counts: np.ndarray, B: int, lpp: int, rpw: int) -> float:
     """Fraction of issued lane-work that is useful for a (lanes_per_pose, restarts_per_wg) tiling:
-    restart slots used x lanes of the rounded-up wavefronts used x fit rows of the two-row passes
-    used (averaged over the pairs' fit counts)."""
-    restart_util = B / (np.ceil(B / rpw) * rpw)
-    threads = rpw * lpp
-    lane_util = threads / (np.ceil(threads / 64.0) * 64.0)
+    active lanes of the wavefronts that run x fit rows of the ceil(M / lpp) row slots each lane
+    group runs (averaged over the pairs' fit counts)."""
     m = np.maximum(np.asarray(counts, np.float64), 1.0)
-    tail_util = float(np.mean(m / (2.0 * lpp * np.ceil(m / (2.0 * lpp)))))
-    return float(restart_util * lane_util * tail_util)
+    slot_util = float(np.mean(m / (lpp * np.ceil(m / lpp))))
+    return float(B * lpp / (64.0 * _waves_per_pair(B, lpp, rpw)) * slot_util)
+
+
+def _waves_per_pair(B: int, lpp: int, rpw: int) -> int:
+    """Wavefronts that run for one pair: a wave whose lane groups are all past restart B exits."""
+    full, last = divmod(B, rpw)
+    per_wg = lambda r: int(np.ceil(r * lpp / 64.0))
+    return full * per_wg(rpw) + (per_wg(last) if last else 0)
 
 
 def candidate_tilings(B: int, early_stop: bool):
@@ -135,22 +139,21 @@ N_SIMD = 256 * 4          # MI355X: 256 CUs x 4 SIMDs
 def tiling_cost(ref_counts, fit_counts, B: int, lpp: int, rpw: int, waves_per_simd: int = 4) -> float:
     """Modelled optimiser-step time of a tiling (arbitrary units).
 
-    A wave's serial work is its lanes' two-row passes over the ref cloud (about 10 issue slots
-    per pair term) plus the butterfly reduction and the per-pose tail.  The chip retires the
-    launch's waves N_SIMD at a time; a CU holds as many workgroups as fit in ``4 x
-    waves_per_simd`` wave slots, and a workgroup whose wave count does not divide evenly over
-    the CU's 4 SIMDs leaves SIMDs idle (the measured MI355X behaviour that favours workgroups
-    of 1, 2, 4 or 8 waves)."""
+    A wave's serial work is its lanes' ceil(M / lpp) row slots over the ref cloud (about 10
+    issue slots per pair term) plus the butterfly reduction and the per-pose tail.  The chip
+    retires the launch's waves N_SIMD at a time; a CU holds as many workgroups as fit in
+    ``4 x waves_per_simd`` wave slots, and a workgroup whose wave count does not divide evenly
+    over the CU's 4 SIMDs leaves SIMDs idle (the measured MI355X behaviour that favours
+    workgroups of 1, 2, 4 or 8 waves)."""
     n = np.maximum(np.asarray(ref_counts, np.float64), 1.0)
     m = np.maximum(np.asarray(fit_counts, np.float64), 1.0)
-    passes = np.ceil(m / (2.0 * lpp))
-    per_wave = passes * 2.0 * n * 10.0 + 26.0 * np.log2(max(lpp, 1)) + 150.0
+    per_wave = np.ceil(m / lpp) * n * 10.0 + 26.0 * np.log2(max(lpp, 1)) + 150.0
     wpg = int(np.ceil(rpw * lpp / 64.0))
     slots = 4 * waves_per_simd
     wgs_per_cu = max(1, slots // wpg)
     resident = wgs_per_cu * wpg
     simd_eff = resident / (4.0 * np.ceil(resident / 4.0))
-    waves = float(np.ceil(B / rpw) * wpg * len(m))
+    waves = float(_waves_per_pair(B, lpp, rpw) * len(m))
     return float(np.mean(per_wave)) * max(waves / (N_SIMD * simd_eff), 0.5)
 
 
@@ -208,8 +211,9 @@ def align(pb: PairBatch, spec: ScoreSpec, q0: torch.Tensor, t0: torch.Tensor,
         self_ov = self_overlaps(pb, spec, stream)
     if lanes_per_pose is None and restarts_per_wg is None:
         from . import autotune as _at
-        tuned = _at.autotune(pb, spec, q0, t0, opt.early_stop, self_ov=self_ov) if tune else \
-            _at.cached(pb, spec.mode, B, opt.early_stop)
+        tuned = _at.autotune(pb, spec, q0, t0, opt.early_stop, max_steps=opt.max_steps,
+                             self_ov=self_ov) if tune else \
+            _at.cached(pb, spec.mode, B, opt.early_stop, opt.max_steps)
         if tuned is not None:
             lpp0, rpw0, var0 = tuned
             variant = variant or var0

@@ -3,10 +3,13 @@ shape inner-loop variant (direct / expanded form).
 
 The analytic model (:func:`fast_shepherd_score_amd.align.tiling_cost`) ranks every legal tiling;
 the tuner times the model's top candidates plus the wide-group tilings (64 lanes per pose), each
-with both inner-loop variants, on the actual batch with a short optimiser run (a few steps, early
-stop off, so every candidate does identical work) and keeps the fastest.  Results are
-cached in memory and, when ``FSS_AUTOTUNE_CACHE`` names a file, on disk as JSON, keyed by mode,
-restart count, early-stop flag and the batch's size profile.
+with both inner-loop variants, on the actual batch.  Each candidate runs the optimiser twice
+with early stop off (so every candidate does identical work), for ``steps`` and ``3 * steps``
+steps; the difference gives the per-step time and the rest the fixed per-launch cost (pair
+staging into LDS, which grows with the workgroup count), and the candidate with the lowest
+predicted time at ``max_steps`` wins.  Results are cached in memory and, when
+``FSS_AUTOTUNE_CACHE`` names a file, on disk as JSON, keyed by mode, restart count, step count,
+early-stop flag and the batch's size profile.
 """
 from __future__ import annotations
 
@@ -33,14 +36,15 @@ def _counts(pb: PairBatch, mode: int):
     return pb.ref_counts, pb.fit_counts
 
 
-def shape_key(pb: PairBatch, mode: int, B: int, early_stop: bool) -> str:
+def shape_key(pb: PairBatch, mode: int, B: int, early_stop: bool, max_steps: int = 200) -> str:
     rc, fc = _counts(pb, mode)
     q = lambda c: tuple(int(x) for x in np.percentile(c, [10, 50, 90]) // 8 * 8) if c is not None and len(c) else (0,)
     nb = int(np.log2(max(1, pb.n_pairs)))
     extra = ""
     if mode == _lib.MODE_SHAPE_PHARM:
         extra = str(q(pb.fit_pcounts))
-    return f"m{mode}-B{B}-es{int(early_stop)}-n{nb}-r{q(rc)}-f{q(fc)}{extra}"
+    ns = int(np.log2(max(1, max_steps)))
+    return f"m{mode}-B{B}-s{ns}-es{int(early_stop)}-n{nb}-r{q(rc)}-f{q(fc)}{extra}"
 
 
 def _load_disk() -> None:
@@ -66,11 +70,12 @@ def _save_disk() -> None:
 
 
 def autotune(pb: PairBatch, spec: ScoreSpec, q0: torch.Tensor, t0: torch.Tensor, early_stop: bool,
-             *, top: int = 6, steps: int = 8, self_ov: Optional[torch.Tensor] = None,
-             verbose: bool = False) -> Tuple[int, int, int]:
-    """Best (lanes_per_pose, restarts_per_wg, variant) for this batch (timed, cached)."""
+             *, max_steps: int = 200, top: int = 6, steps: int = 4,
+             self_ov: Optional[torch.Tensor] = None, verbose: bool = False) -> Tuple[int, int, int]:
+    """Best (lanes_per_pose, restarts_per_wg, variant) for this batch and step budget (timed,
+    cached)."""
     B = int(q0.shape[1])
-    key = shape_key(pb, spec.mode, B, early_stop)
+    key = shape_key(pb, spec.mode, B, early_stop, max_steps)
     with _lock:
         if not _cache:
             _load_disk()
@@ -85,20 +90,25 @@ def autotune(pb: PairBatch, spec: ScoreSpec, q0: torch.Tensor, t0: torch.Tensor,
     variants = (1, 2) if spec.mode != _lib.MODE_PHARM else (0,)
     if self_ov is None:
         self_ov = self_overlaps(pb, spec)
-    opt = OptimSpec(max_steps=steps, early_stop=False)
+    s1, s2 = steps, 3 * steps
     best, best_ms = None, float("inf")
     for lpp, rpw in tilings:
         if early_stop and rpw < B:
             continue
         for var in variants:
-            ev = []
             kw = dict(lanes_per_pose=lpp, restarts_per_wg=rpw, self_ov=self_ov, variant=var)
-            align(pb, spec, q0, t0, opt, **kw)
-            align(pb, spec, q0, t0, opt, events=ev, **kw)
-            torch.cuda.synchronize()
-            ms = ev[0][0].elapsed_time(ev[0][1])
+            align(pb, spec, q0, t0, OptimSpec(max_steps=s1, early_stop=False), **kw)
+            t = []
+            for n in (s1, s2):
+                ev = []
+                align(pb, spec, q0, t0, OptimSpec(max_steps=n, early_stop=False), events=ev, **kw)
+                torch.cuda.synchronize()
+                t.append(ev[0][0].elapsed_time(ev[0][1]))
+            per_step = max(t[1] - t[0], 0.0) / (s2 - s1)
+            ms = t[0] + (max_steps - s1) * per_step
             if verbose:
-                print(f"autotune {key}: lpp={lpp} rpw={rpw} variant={var} {ms:.3f} ms", flush=True)
+                print(f"autotune {key}: lpp={lpp} rpw={rpw} variant={var} {t[0]:.3f}/{t[1]:.3f} ms "
+                      f"-> {ms:.3f} ms at {max_steps} steps", flush=True)
             if ms < best_ms:
                 best, best_ms = (lpp, rpw, var), ms
     with _lock:
@@ -107,8 +117,9 @@ def autotune(pb: PairBatch, spec: ScoreSpec, q0: torch.Tensor, t0: torch.Tensor,
     return best
 
 
-def cached(pb: PairBatch, mode: int, B: int, early_stop: bool) -> Optional[Tuple[int, int, int]]:
+def cached(pb: PairBatch, mode: int, B: int, early_stop: bool,
+           max_steps: int = 200) -> Optional[Tuple[int, int, int]]:
     with _lock:
         if not _cache:
             _load_disk()
-        return _cache.get(shape_key(pb, mode, B, early_stop))
+        return _cache.get(shape_key(pb, mode, B, early_stop, max_steps))

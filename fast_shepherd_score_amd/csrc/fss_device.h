@@ -95,19 +95,24 @@ __device__ __forceinline__ void group_sum(Acc& a, int width, bool grad) {
 //
 // With s = sqrt(alpha/2 log2 e), sc = sqrt(log2 e / lam) and the ref centroid c as origin:
 //   a~ = s (a - c),  b~' = R (s b) + s t - s c,  c~ = sc * esp
-//   E_ab = exp2(-|a~ - b~'|^2 - (c~_a - c~_b)^2) = exp2(g_ab) * exp2(-h_b)
+//   E_ab = exp2(-|a~ - b~'|^2 - (c~_a - c~_b)^2) = exp2(g_ab - D) * exp2(D - h_b)
 //   g_ab = W_a + 2 a~ . b~' + 2 c~_a c~_b,   W_a = -|a~|^2 - c~_a^2,   h_b = |b~'|^2 + c~_b^2
-// LDS rows: ref4[a] = (2 a~, W_a), refc[a] = 2 c~_a (ESP only), fit4[b] = (s b, c~_b).
-// A pair term is then 3-4 FMA + v_exp_f32 + 1 add + 3 FMA; exp2(-h_b) is applied once per fit
-// row.  Since g_ab <= h_b, a sum of up to 2^17 terms exp2(g_ab) stays finite while h_b <= kHMax
-// = 110; fit rows beyond that radius (about 13.7 A from the ref centroid at alpha 0.81) take the
-// direct form.
+// LDS rows: ref4[a] = (2 a~, W_a - D), refc[a] = 2 c~_a (ESP only), fit4[b] = (s b, c~_b).
+// A pair term is then 3-4 FMA + v_exp_f32 + 1 add + 3 FMA; exp2(D - h_b) is applied once per fit
+// row.  The shift D = kExpShift moves the exponent window: exp2(g_ab - D) = E_ab 2^(h_b - D) and
+// |2 a~| E_ab <= 2 (sqrt(h_b) + 1) < 2^5, so a sum over up to 2^14 ref rows (more than LDS holds),
+// gradient sums included, stays finite while h_b <= kHMax = D + 108 (about 18 A from the ref
+// centroid at alpha 0.81), and a term flushes to zero only where E_ab < 2^-40.  Fit rows beyond
+// that radius take the direct form.
 // Lane j of a group of `lpp` lanes owns fit rows b = j, j+lpp, ...; two rows per pass over the
-// ref cloud so one LDS broadcast read feeds two pair terms.
+// ref cloud so one LDS broadcast read feeds two pair terms.  The group runs ceil(M/lpp) row slots:
+// floor of half that in two-row passes (a lane whose second row lies past M runs a copy of its
+// first, zero-weighted), plus one single-row pass when the count is odd.
 // Output (per lane, scaled units): V = sum E, gt = sum_b f_b, G = sum_b f_b (s b)^T with
 // f_b = sum_a E_ab (a~ - b~'_b).
 // ---------------------------------------------------------------------------------------------
-constexpr float kHMax = 110.f;
+constexpr float kExpShift = 86.f;
+constexpr float kHMax = kExpShift + 108.f;
 
 template <bool ESP>
 __device__ __forceinline__ void direct_pair(const float4* __restrict__ ref, const float* __restrict__ refc,
@@ -152,7 +157,7 @@ __device__ __forceinline__ void accumulate_rows(Acc& acc, const float4& p0, floa
 template <bool ESP, bool GRAD>
 __device__ __forceinline__ void direct_rows(const float4* __restrict__ ref, int N, const float4& p0,
                                             const float4& p1, const float R[9], const float ts[3],
-                                            bool two, Acc& acc) {
+                                            bool two, bool keep1, Acc& acc) {
   const float x0 = fmaf(R[0], p0.x, fmaf(R[1], p0.y, fmaf(R[2], p0.z, ts[0])));
   const float y0 = fmaf(R[3], p0.x, fmaf(R[4], p0.y, fmaf(R[5], p0.z, ts[1])));
   const float z0 = fmaf(R[6], p0.x, fmaf(R[7], p0.y, fmaf(R[8], p0.z, ts[2])));
@@ -198,6 +203,7 @@ __device__ __forceinline__ void direct_rows(const float4* __restrict__ ref, int 
       if (GRAD) { fx0 = fmaf(e0, dx0, fx0); fy0 = fmaf(e0, dy0, fy0); fz0 = fmaf(e0, dz0, fz0); }
     }
   }
+  if (!keep1) v1 = fx1 = fy1 = fz1 = 0.f;
   accumulate_rows<GRAD>(acc, p0, v0, fx0, fy0, fz0, p1, v1, fx1, fy1, fz1);
 }
 
@@ -205,11 +211,15 @@ template <bool ESP, bool GRAD>
 __device__ __forceinline__ void shape_accum_direct(const float4* __restrict__ ref, int N,
                                                    const float4* __restrict__ fit, int M, int j, int lpp,
                                                    const float R[9], const float ts[3], Acc& acc) {
+  const int cmax = (M + lpp - 1) / lpp, passes = cmax >> 1;
   int b0 = j;
-  for (; b0 + lpp < M; b0 += 2 * lpp) direct_rows<ESP, GRAD>(ref, N, fit[b0], fit[b0 + lpp], R, ts, true, acc);
-  if (b0 < M) {
+  for (int k = 0; k < passes; ++k, b0 += 2 * lpp) {
+    const bool ok1 = b0 + lpp < M;
+    direct_rows<ESP, GRAD>(ref, N, fit[b0], fit[ok1 ? b0 + lpp : b0], R, ts, true, ok1, acc);
+  }
+  if ((cmax & 1) && b0 < M) {
     const float4 p0 = fit[b0];
-    direct_rows<ESP, GRAD>(ref, N, p0, p0, R, ts, false, acc);
+    direct_rows<ESP, GRAD>(ref, N, p0, p0, R, ts, false, false, acc);
   }
 }
 
@@ -231,7 +241,7 @@ __device__ __forceinline__ void expanded_row(const float4* __restrict__ ref, con
       acc += e;
       if (GRAD) { sx = fmaf(e, A.x, sx); sy = fmaf(e, A.y, sy); sz = fmaf(e, A.z, sz); }
     }
-    const float F = fast_exp2(-h);
+    const float F = fast_exp2(kExpShift - h);
     v = acc * F;
     if (GRAD) { fx = fmaf(0.5f * F, sx, -x * v); fy = fmaf(0.5f * F, sy, -y * v); fz = fmaf(0.5f * F, sz, -z * v); }
   } else {
@@ -244,12 +254,12 @@ template <bool ESP, bool GRAD>
 __device__ __forceinline__ void shape_accum(const float4* __restrict__ ref, const float* __restrict__ refc,
                                             int N, const float4* __restrict__ fit, int M, int j, int lpp,
                                             const float R[9], const float ts[3], Acc& acc) {
+  const int cmax = (M + lpp - 1) / lpp, passes = cmax >> 1;
   int b0 = j;
-  // two-row passes while this lane has at least two rows left
-  for (; b0 + lpp < M; b0 += 2 * lpp) {
-    const int b1 = b0 + lpp;
+  for (int k = 0; k < passes; ++k, b0 += 2 * lpp) {
+    const bool ok1 = b0 + lpp < M;
     const float4 p0 = fit[b0];
-    const float4 p1 = fit[b1];
+    const float4 p1 = fit[ok1 ? b0 + lpp : b0];
     const float x0 = fmaf(R[0], p0.x, fmaf(R[1], p0.y, fmaf(R[2], p0.z, ts[0])));
     const float y0 = fmaf(R[3], p0.x, fmaf(R[4], p0.y, fmaf(R[5], p0.z, ts[1])));
     const float z0 = fmaf(R[6], p0.x, fmaf(R[7], p0.y, fmaf(R[8], p0.z, ts[2])));
@@ -283,7 +293,7 @@ __device__ __forceinline__ void shape_accum(const float4* __restrict__ ref, cons
           sx1 = fmaf(e1, A.x, sx1); sy1 = fmaf(e1, A.y, sy1); sz1 = fmaf(e1, A.z, sz1);
         }
       }
-      const float F0 = fast_exp2(-h0), F1 = fast_exp2(-h1);
+      const float F0 = fast_exp2(kExpShift - h0), F1 = fast_exp2(kExpShift - h1);
       v0 *= F0;
       v1 *= F1;
       if (GRAD) {
@@ -294,10 +304,11 @@ __device__ __forceinline__ void shape_accum(const float4* __restrict__ ref, cons
       direct_pair<ESP>(ref, refc, N, x0, y0, z0, c0, v0, fx0, fy0, fz0);
       direct_pair<ESP>(ref, refc, N, x1, y1, z1, c1, v1, fx1, fy1, fz1);
     }
+    if (!ok1) v1 = fx1 = fy1 = fz1 = 0.f;
     accumulate_rows<GRAD>(acc, p0, v0, fx0, fy0, fz0, p1, v1, fx1, fy1, fz1);
   }
-  // single-row tail pass (lanes with an odd row count)
-  if (b0 < M) {
+  // single-row pass when the row-slot count is odd
+  if ((cmax & 1) && b0 < M) {
     const float4 p0 = fit[b0];
     const float x0 = fmaf(R[0], p0.x, fmaf(R[1], p0.y, fmaf(R[2], p0.z, ts[0])));
     const float y0 = fmaf(R[3], p0.x, fmaf(R[4], p0.y, fmaf(R[5], p0.z, ts[1])));

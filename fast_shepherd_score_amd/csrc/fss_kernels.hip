@@ -188,7 +188,8 @@ __device__ __forceinline__ Staged stage_pair(const FssPairBatch& pb, const KScor
       const float ax = s * (v.x - cx), ay = s * (v.y - cy), az = s * (v.z - cz);
       const float cc = ESP ? v.w * sc : 0.f;
       if (EXPD) {
-        la[i] = make_float4(2.f * ax, 2.f * ay, 2.f * az, -fmaf(ax, ax, fmaf(ay, ay, fmaf(az, az, cc * cc))));
+        la[i] = make_float4(2.f * ax, 2.f * ay, 2.f * az,
+                            -fmaf(ax, ax, fmaf(ay, ay, fmaf(az, az, cc * cc))) - kExpShift);
         if (ESP) lc[i] = 2.f * cc;
       } else {
         la[i] = make_float4(ax, ay, az, cc);

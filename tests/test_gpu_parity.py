@@ -252,3 +252,49 @@ def test_device_seeds_match_reference_rule(dev):
         Rh = rotmat(qh[i, 1:5].astype(np.float64))
         for k in range(4):      # every host PCA rotation appears among the device ones
             assert min(np.abs(Rd[m] - Rh[k]).max() for m in range(4)) < 1e-4, (i, k)
+
+
+@pytest.mark.parametrize("mode", [_lib.MODE_SHAPE, _lib.MODE_SHAPE_ESP])
+def test_far_rows_exponent_window(dev, mode):
+    """Elongated clouds (rows up to ~30 A from the ref centroid) put fit rows in every regime of
+    the expanded inner loop: inside the shifted exponent window, and beyond it (direct form)."""
+    from fast_shepherd_score_amd.align import ScoreSpec
+    from fast_shepherd_score_amd.constants import LAM_SCALING
+    from fast_shepherd_score_amd.packing import CloudSet, PairBatch
+    print(f"case far_rows mode={mode}", flush=True)
+    rng = np.random.default_rng(40)
+    ref, fit, rw, fw = [], [], [], []
+    for n in (90, 130):
+        axis = np.linspace(-28.0, 28.0, n)[:, None] * np.array([[1.0, 0.2, 0.1]])
+        ref.append((axis + rng.normal(scale=1.0, size=(n, 3))).astype(np.float32))
+        fit.append((axis[::-1] + rng.normal(scale=1.0, size=(n, 3))).astype(np.float32))
+        rw.append(rng.normal(scale=0.1, size=n).astype(np.float32))
+        fw.append(rng.normal(scale=0.1, size=n).astype(np.float32))
+    esp = mode == _lib.MODE_SHAPE_ESP
+    lam = 0.3 * LAM_SCALING
+    spec = ScoreSpec(mode=mode, alpha=0.81, lam=lam)
+    pb = PairBatch(CloudSet.from_arrays(ref, rw if esp else None, device=dev),
+                   CloudSet.from_arrays(fit, fw if esp else None, device=dev))
+    B = 4
+    q, _ = random_seeds(2, B, seed=41)
+    q[:, 0] = [1, 0, 0, 0]
+    t = np.zeros((2, B, 3), np.float32)
+    t[:, 1] = [6.0, 0, 0]
+    t[:, 2] = [0, 9.0, -3.0]
+    pp = torch.arange(2, dtype=torch.int32).repeat_interleave(B)
+    V, gR, gT, _ = overlap_grad(pb, spec, pp, to_dev(q.reshape(-1, 4), dev), to_dev(t.reshape(-1, 3), dev))
+    V, gR, gT = V.cpu().numpy(), gR.cpu().numpy(), gT.cpu().numpy()
+    for i in range(2):
+        if esp:
+            p = orc.Pair("esp", ref=ref[i], fit=fit[i], ref_esp=rw[i], fit_esp=fw[i], lam=lam)
+        else:
+            p = orc.Pair("shape", ref=ref[i], fit=fit[i])
+        qn = torch.tensor(q[i], dtype=DT)
+        qn = qn / qn.norm(dim=1, keepdim=True)
+        O, GR, GT = (x.numpy() for x in p.channel_overlaps(orc.rot_from_unit_quat(qn),
+                                                           torch.tensor(t[i], dtype=DT))["shape"])
+        sl = slice(i * B, (i + 1) * B)
+        np.testing.assert_allclose(V[sl, 0], O, rtol=2e-5, atol=1e-6)
+        scale = max(np.abs(GR).max(), np.abs(GT).max(), 1e-6)
+        np.testing.assert_allclose(gR[sl, 0], GR, atol=1e-4 * scale)
+        np.testing.assert_allclose(gT[sl, 0], GT, atol=1e-4 * scale)
