@@ -310,7 +310,8 @@ def main():
                    "pairs": args.pairs, "restarts": B, "opt_steps": steps,
                    "lanes_per_pose": res.extra["lanes_per_pose"],
                    "restarts_per_wg": res.extra["restarts_per_wg"],
-                   "inner_loop": {0: "expanded", 1: "direct", 2: "expanded"}[res.extra["variant"]]},
+                   "inner_loop": {0: "expanded", 1: "direct", 2: "expanded",
+                                  3: "mfma-split-bf16"}[res.extra["variant"]]},
         "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_FP32_TFLOPS,
                      "unit": "TFLOP/s", "frac": achieved / PEAK_FP32_TFLOPS, "traffic": None,
                      "kernel": "k_align", "kernel_ms": kern_ms, "finish_ms": fin_ms,

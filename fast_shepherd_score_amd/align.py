@@ -201,6 +201,8 @@ def align(pb: PairBatch, spec: ScoreSpec, q0: torch.Tensor, t0: torch.Tensor,
 
     Tiling: explicit ``lanes_per_pose`` / ``restarts_per_wg``, else an autotuned choice
     (``tune=True``, or one cached for this batch profile), else the cost model's choice.
+    ``variant`` picks the shape inner loop: 1 direct form, 2 expanded form, 3 split-bf16 MFMA
+    (one wavefront per pose, so 64 lanes per pose), 0 the tuned or default choice.
     ``events``: if a list is given, HIP events bracketing the optimiser kernel and the finishing
     kernels are appended to it as (start, optimiser_end, finish_end).
     """
@@ -221,8 +223,10 @@ def align(pb: PairBatch, spec: ScoreSpec, q0: torch.Tensor, t0: torch.Tensor,
             lpp0, rpw0 = default_tiling(pb, spec.mode, B, opt.early_stop)
     else:
         lpp0, rpw0 = default_tiling(pb, spec.mode, B, opt.early_stop)
+    if variant == 3 and spec.mode != _lib.MODE_PHARM and lanes_per_pose is None:
+        lpp0, rpw0 = 64, (rpw0 if lpp0 == 64 else 4)
     lpp = lanes_per_pose or lpp0
-    rpw = restarts_per_wg or (B if opt.early_stop else rpw0)
+    rpw = restarts_per_wg or (B if opt.early_stop else min(rpw0, B))
     check_tiling(B, lpp, rpw, opt.early_stop)
     order_t = None
     if order == "cost" and P > 1:

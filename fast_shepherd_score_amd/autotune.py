@@ -88,29 +88,33 @@ def autotune(pb: PairBatch, spec: ScoreSpec, q0: torch.Tensor, t0: torch.Tensor,
     wide = [(l, r) for _, e, l, r in sorted(ranked, key=lambda c: c[1]) if l == 64][:2]
     tilings += [t for t in wide if t not in tilings]
     variants = (1, 2) if spec.mode != _lib.MODE_PHARM else (0,)
+    # the MFMA inner loop runs one pose per wavefront: its own tilings (64 lanes per pose)
+    runs = [(l, r, v) for l, r in tilings for v in variants]
+    if spec.mode != _lib.MODE_PHARM:
+        runs += [(64, r, 3) for r in ((B,) if early_stop else sorted({1, 2, 4, 8, min(16, B)}))
+                 if r <= B and 64 * r <= 1024]
     if self_ov is None:
         self_ov = self_overlaps(pb, spec)
     s1, s2 = steps, 3 * steps
     best, best_ms = None, float("inf")
-    for lpp, rpw in tilings:
+    for lpp, rpw, var in runs:
         if early_stop and rpw < B:
             continue
-        for var in variants:
-            kw = dict(lanes_per_pose=lpp, restarts_per_wg=rpw, self_ov=self_ov, variant=var)
-            align(pb, spec, q0, t0, OptimSpec(max_steps=s1, early_stop=False), **kw)
-            t = []
-            for n in (s1, s2):
-                ev = []
-                align(pb, spec, q0, t0, OptimSpec(max_steps=n, early_stop=False), events=ev, **kw)
-                torch.cuda.synchronize()
-                t.append(ev[0][0].elapsed_time(ev[0][1]))
-            per_step = max(t[1] - t[0], 0.0) / (s2 - s1)
-            ms = t[0] + (max_steps - s1) * per_step
-            if verbose:
-                print(f"autotune {key}: lpp={lpp} rpw={rpw} variant={var} {t[0]:.3f}/{t[1]:.3f} ms "
-                      f"-> {ms:.3f} ms at {max_steps} steps", flush=True)
-            if ms < best_ms:
-                best, best_ms = (lpp, rpw, var), ms
+        kw = dict(lanes_per_pose=lpp, restarts_per_wg=rpw, self_ov=self_ov, variant=var)
+        align(pb, spec, q0, t0, OptimSpec(max_steps=s1, early_stop=False), **kw)
+        t = []
+        for n in (s1, s2):
+            ev = []
+            align(pb, spec, q0, t0, OptimSpec(max_steps=n, early_stop=False), events=ev, **kw)
+            torch.cuda.synchronize()
+            t.append(ev[0][0].elapsed_time(ev[0][1]))
+        per_step = max(t[1] - t[0], 0.0) / (s2 - s1)
+        ms = t[0] + (max_steps - s1) * per_step
+        if verbose:
+            print(f"autotune {key}: lpp={lpp} rpw={rpw} variant={var} {t[0]:.3f}/{t[1]:.3f} ms "
+                  f"-> {ms:.3f} ms at {max_steps} steps", flush=True)
+        if ms < best_ms:
+            best, best_ms = (lpp, rpw, var), ms
     with _lock:
         _cache[key] = best
         _save_disk()

@@ -319,6 +319,128 @@ __device__ __forceinline__ void shape_accum(const float4* __restrict__ ref, cons
   }
 }
 
+// ---------------------------------------------------------------------------------------------
+// Shape channel on the matrix cores (inner-loop variant 3): the expanded-form exponent
+//   D_ab = (W_a - D) + 2 a~ . b~' + 2 c~_a c~_b - m_b
+// is a K = 32 product of split-bf16 operands, one v_mfma_f32_16x16x32_bf16 per 16 x 16 tile of
+// (ref row, fit row) pairs.  Each FP32 operand x is split exactly into hi + mid + lo bf16 parts
+// (x - hi - mid - lo = O(2^-24 x)); for x, y and z the 8 K-slots of a lane group hold the
+// products hi.hi, hi.mid, mid.hi, hi.lo, lo.hi, mid.mid, mid.lo, lo.mid (bf16 products are exact
+// in FP32, the dropped lo.lo is O(2^-48)); the fourth group holds the ESP products hi.hi, hi.mid,
+// mid.hi, the ref constant W_a - D (hi, mid, lo) against 1, and 1 against the column shift
+// -m_b (hi, mid).  The result matches an FP32 FMA chain's accuracy.
+// The column shift m_b = max(0, h_b - kHMax) keeps every exponent <= kHMax - D for any fit row, so
+// this variant has no direct-form fallback; exp2(D + m^_b - h_b) with the same rounded m^_b
+// restores the scale once per column.
+//
+// Operand layouts (v_mfma_f32_16x16x32_bf16, lane l, group g = l / 16, r = l % 16):
+//   A: ref row r of the tile, K-slots 8g..8g+7;  B: fit column r, K-slots 8g..8g+7;
+//   D: rows 4g..4g+3 of column r.
+// LDS (per pair, T = ceil(N / 16) tiles, rows past N have zero coordinates and W = -1e30):
+//   amat[64 t + 16 g + r]     8 bf16: A-operand row r of tile t for K-group g
+//   xq[12 t + 3 g + k]        float4: 2 a~_k of ref rows 16 t + 4 g + 0..3 (k = x, y, z)
+// One wavefront owns one pose; a pass covers 32 fit columns (two B tiles) against all T ref
+// tiles, and each lane folds its partial column sums into the pose sums as the expanded form
+// does per fit row.
+// ---------------------------------------------------------------------------------------------
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+
+__device__ __forceinline__ void split3(float x, __bf16& h, __bf16& m, __bf16& l) {
+  h = (__bf16)x;
+  float r = x - (float)h;
+  m = (__bf16)r;
+  r -= (float)m;
+  l = (__bf16)r;
+}
+
+// A-operand K-group row for one ref row: groups 0..2 take 2 a~_k, group 3 the ESP value 2 c~_a and
+// the shifted W_a - D.
+__device__ __forceinline__ bf16x8 mfma_a_coord(float x) {
+  __bf16 h, m, l;
+  split3(x, h, m, l);
+  bf16x8 a;
+  a[0] = h; a[1] = h; a[2] = m; a[3] = h; a[4] = l; a[5] = m; a[6] = m; a[7] = l;
+  return a;
+}
+__device__ __forceinline__ bf16x8 mfma_a_last(float c, float w) {
+  __bf16 ch, cm, cl, wh, wm, wl;
+  split3(c, ch, cm, cl);
+  split3(w, wh, wm, wl);
+  const __bf16 one = (__bf16)1.0f;
+  bf16x8 a;
+  a[0] = ch; a[1] = ch; a[2] = cm; a[3] = wh; a[4] = wm; a[5] = wl; a[6] = one; a[7] = one;
+  return a;
+}
+
+template <bool ESP>
+__device__ __forceinline__ void shape_accum_mfma(const float4* __restrict__ amat, const float4* __restrict__ xq,
+                                                 int T, const float4* __restrict__ fit, int M,
+                                                 const float R[9], const float ts[3], Acc& acc) {
+  const int l = threadIdx.x & 63, g = l >> 4, r = l & 15;
+  const f32x4 zero = {0.f, 0.f, 0.f, 0.f};
+  const __bf16 one = (__bf16)1.0f;
+  const int passes = (M + 31) >> 5;
+  for (int p = 0; p < passes; ++p) {
+    float X[2], Y[2], Z[2], F[2];
+    float4 P[2];
+    bf16x8 Bop[2];
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+      const int b = 32 * p + 16 * k + r;
+      const bool ok = b < M;
+      P[k] = fit[ok ? b : 0];
+      const float4 q = P[k];
+      X[k] = fmaf(R[0], q.x, fmaf(R[1], q.y, fmaf(R[2], q.z, ts[0])));
+      Y[k] = fmaf(R[3], q.x, fmaf(R[4], q.y, fmaf(R[5], q.z, ts[1])));
+      Z[k] = fmaf(R[6], q.x, fmaf(R[7], q.y, fmaf(R[8], q.z, ts[2])));
+      const float c = ESP ? q.w : 0.f;
+      const float h = fmaf(X[k], X[k], fmaf(Y[k], Y[k], fmaf(Z[k], Z[k], c * c)));
+      __bf16 mh, mm, ml;
+      split3(fmaxf(h - kHMax, 0.f), mh, mm, ml);
+      F[k] = ok ? fast_exp2(kExpShift + ((float)mh + (float)mm) - h) : 0.f;
+      __bf16 H, Mi, L;
+      if (g < 3) {
+        split3(g == 0 ? X[k] : (g == 1 ? Y[k] : Z[k]), H, Mi, L);
+        Bop[k][0] = H; Bop[k][1] = Mi; Bop[k][2] = H; Bop[k][3] = L;
+        Bop[k][4] = H; Bop[k][5] = Mi; Bop[k][6] = L; Bop[k][7] = Mi;
+      } else {
+        split3(c, H, Mi, L);
+        Bop[k][0] = H; Bop[k][1] = Mi; Bop[k][2] = H; Bop[k][3] = one;
+        Bop[k][4] = one; Bop[k][5] = one; Bop[k][6] = -mh; Bop[k][7] = -mm;
+      }
+    }
+    float v0 = 0.f, sx0 = 0.f, sy0 = 0.f, sz0 = 0.f, v1 = 0.f, sx1 = 0.f, sy1 = 0.f, sz1 = 0.f;
+    const float4* am = amat + 16 * g + r;
+    const float4* xg = xq + 3 * g;
+#pragma unroll 2
+    for (int t = 0; t < T; ++t) {
+      const float4 araw = am[64 * t];
+      const float4 ax = xg[12 * t], ay = xg[12 * t + 1], az = xg[12 * t + 2];
+      const bf16x8 A = __builtin_bit_cast(bf16x8, araw);
+      const f32x4 d0 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A, Bop[0], zero, 0, 0, 0);
+      const f32x4 d1 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A, Bop[1], zero, 0, 0, 0);
+      const float xa[4] = {ax.x, ax.y, ax.z, ax.w}, ya[4] = {ay.x, ay.y, ay.z, ay.w},
+                  za[4] = {az.x, az.y, az.z, az.w};
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const float e0 = fast_exp2(d0[i]);
+        const float e1 = fast_exp2(d1[i]);
+        v0 += e0;
+        v1 += e1;
+        sx0 = fmaf(e0, xa[i], sx0); sy0 = fmaf(e0, ya[i], sy0); sz0 = fmaf(e0, za[i], sz0);
+        sx1 = fmaf(e1, xa[i], sx1); sy1 = fmaf(e1, ya[i], sy1); sz1 = fmaf(e1, za[i], sz1);
+      }
+    }
+    // The column finish f_b = F_b (s_b / 2 - b~'_b v_b) is linear in the row sums, so every lane
+    // folds its own row-group partials; the pose-level butterfly adds the four groups.
+    const float w0 = v0 * F[0], w1 = v1 * F[1];
+    accumulate_rows<true>(acc, P[0], w0, fmaf(0.5f * F[0], sx0, -X[0] * w0), fmaf(0.5f * F[0], sy0, -Y[0] * w0),
+                          fmaf(0.5f * F[0], sz0, -Z[0] * w0), P[1], w1, fmaf(0.5f * F[1], sx1, -X[1] * w1),
+                          fmaf(0.5f * F[1], sy1, -Y[1] * w1), fmaf(0.5f * F[1], sz1, -Z[1] * w1));
+  }
+}
+
 // Convert scaled shape sums into O, dO/dt, dO/dR (in place).
 __device__ __forceinline__ void shape_finish(Acc& a, float K, float alpha, float s, bool grad) {
   a.V *= K;

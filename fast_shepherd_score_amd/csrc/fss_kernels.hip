@@ -112,16 +112,20 @@ int check_batch(const FssPairBatch* pb, int mode) {
 
 // LDS plan for one workgroup (float4 offsets): shape ref rows, fit rows, ref ESP column,
 // a 16-float4 reduction scratch, then pharm ref anchors / vectors, fit anchors / vectors.
+// With `mfma` (inner-loop variant 3) the ref rows are the split-bf16 A-operand tiles (64 float4
+// per 16 rows) followed by the 2 a~ row quads (12 float4 per 16 rows), and no ESP column.
 struct LdsPlan {
-  int ref, fit, refc, red, poff, ranc, rvec, fanc, fvec, end;
+  int ref, xq, fit, refc, red, poff, ranc, rvec, fanc, fvec, end;
 };
-LdsPlan plan_lds(const FssPairBatch* pb, int mode) {
+LdsPlan plan_lds(const FssPairBatch* pb, int mode, bool mfma = false) {
   LdsPlan p;
   int o = 0;
   const bool sh = uses_shape(mode), ph = uses_pharm(mode);
-  p.ref = o; o += sh ? pb->max_ref : 0;
+  const int tiles = (pb->max_ref + 15) / 16;
+  p.ref = o; o += sh ? (mfma ? 64 * tiles : pb->max_ref) : 0;
+  p.xq = o; o += sh && mfma ? 12 * tiles : 0;
   p.fit = o; o += sh ? pb->max_fit : 0;
-  p.refc = o; o += mode == FSS_MODE_SHAPE_ESP ? (pb->max_ref + 3) / 4 : 0;
+  p.refc = o; o += mode == FSS_MODE_SHAPE_ESP && !mfma ? (pb->max_ref + 3) / 4 : 0;
   p.red = o; o += 16;
   p.poff = o; o += ph ? 4 : 0;
   p.ranc = o; o += ph ? pb->max_ref_ph : 0;
@@ -141,6 +145,7 @@ size_t max_lds_bytes() {
 
 struct Staged {
   const float4* ref; const float* refc; const float4* fit; int N, M;
+  const float4* xq; int T;  // mfma layout: ref = A-operand tiles, xq = 2 a~ row quads, T tiles
   float cs[3];  // scaled ref centroid: the origin of the expanded shape form
   const float4* ranc; const float4* rvec; const float4* fanc; const float4* fvec; int Np, Mp;
   const int* poff;  // ref pharmacophore rows are sorted by type; type t occupies [poff[t], poff[t+1])
@@ -148,7 +153,7 @@ struct Staged {
 
 // Cooperative copy of one pair's clouds into LDS (shape rows scaled, pharm vectors normalised,
 // pharm type folded into the anchor's w as integer bits).  Caller syncs afterwards.
-template <bool SH, bool ESP, bool PH, bool EXPD = true>
+template <bool SH, bool ESP, bool PH, bool EXPD = true, bool MF = false>
 __device__ __forceinline__ Staged stage_pair(const FssPairBatch& pb, const KScore& ks, LdsPlan lp,
                                              int pair, int ref_side_a, int fit_side_b,
                                              float4* lds) {
@@ -183,7 +188,30 @@ __device__ __forceinline__ Staged stage_pair(const FssPairBatch& pb, const KScor
     for (int w = 0; w < (nt >> 6); ++w) { cx += red[3 * w]; cy += red[3 * w + 1]; cz += red[3 * w + 2]; }
     const float invN = N > 0 ? 1.f / (float)N : 0.f;
     cx *= invN; cy *= invN; cz *= invN;
-    for (int i = tid; i < N; i += nt) {
+    if (MF) {
+      const int T = (N + 15) >> 4;
+      float* xqf = reinterpret_cast<float*>(lds + lp.xq);
+      for (int i = tid; i < 16 * T; i += nt) {
+        float ax = 0.f, ay = 0.f, az = 0.f, cc = 0.f, w = -1e30f;
+        if (i < N) {
+          const float4 v = ga[i];
+          ax = s * (v.x - cx); ay = s * (v.y - cy); az = s * (v.z - cz);
+          cc = ESP ? v.w * sc : 0.f;
+          w = -fmaf(ax, ax, fmaf(ay, ay, fmaf(az, az, cc * cc))) - kExpShift;
+        }
+        const int t = i >> 4, r = i & 15;
+        float4* at = la + 64 * t + r;
+        at[0] = __builtin_bit_cast(float4, mfma_a_coord(2.f * ax));
+        at[16] = __builtin_bit_cast(float4, mfma_a_coord(2.f * ay));
+        at[32] = __builtin_bit_cast(float4, mfma_a_coord(2.f * az));
+        at[48] = __builtin_bit_cast(float4, mfma_a_last(2.f * cc, w));
+        float* xr = xqf + 48 * t + 12 * (r >> 2) + (r & 3);
+        xr[0] = 2.f * ax; xr[4] = 2.f * ay; xr[8] = 2.f * az;
+      }
+      st.xq = lds + lp.xq;
+      st.T = T;
+    }
+    for (int i = tid; i < N && !MF; i += nt) {
       const float4 v = ga[i];
       const float ax = s * (v.x - cx), ay = s * (v.y - cy), az = s * (v.z - cz);
       const float cc = ESP ? v.w * sc : 0.f;
@@ -474,8 +502,12 @@ struct AlignK {
   int32_t* n_steps;
 };
 
-template <int MODE, bool ES, int MAXT, bool EXPD>
-__global__ __launch_bounds__(MAXT) void k_align(FssPairBatch pb, KScore ks, LdsPlan lp, AlignK ak) {
+// The MFMA variant is held to 128 VGPRs (4 wavefronts per SIMD): its loop is latency-bound.
+template <int MODE, bool ES, int MAXT, int VAR>
+__global__ __launch_bounds__(MAXT) __attribute__((amdgpu_waves_per_eu(VAR == 3 ? 4 : 1)))
+void k_align(FssPairBatch pb, KScore ks, LdsPlan lp, AlignK ak) {
+  constexpr bool EXPD = VAR != 1;
+  constexpr bool MF = VAR == 3;
   constexpr bool SH = MODE != FSS_MODE_PHARM;
   constexpr bool ESP = MODE == FSS_MODE_SHAPE_ESP;
   constexpr bool PH = MODE == FSS_MODE_PHARM || MODE == FSS_MODE_SHAPE_PHARM;
@@ -483,7 +515,7 @@ __global__ __launch_bounds__(MAXT) void k_align(FssPairBatch pb, KScore ks, LdsP
   const int slot = blockIdx.x / ak.wg_per_pair;
   const int blk = blockIdx.x - slot * ak.wg_per_pair;
   const int pair = ak.order ? ak.order[slot] : slot;
-  const Staged st = stage_pair<SH, ESP, PH, EXPD>(pb, ks, lp, pair, 0, 1, lds);
+  const Staged st = stage_pair<SH, ESP, PH, EXPD, MF>(pb, ks, lp, pair, 0, 1, lds);
   double* lossbuf = reinterpret_cast<double*>(lds + lp.end);  // 2 * rpw doubles (ES only)
 
   const int lpp = ak.lpp;
@@ -525,7 +557,8 @@ __global__ __launch_bounds__(MAXT) void k_align(FssPairBatch pb, KScore ks, LdsP
       ph.zero();
       if (SH) {
         const float ts[3] = {fmaf(p[4], ks.s, -st.cs[0]), fmaf(p[5], ks.s, -st.cs[1]), fmaf(p[6], ks.s, -st.cs[2])};
-        if (EXPD) shape_accum<ESP, true>(st.ref, st.refc, st.N, st.fit, st.M, j, lpp, R, ts, sh);
+        if (MF) shape_accum_mfma<ESP>(st.ref, st.xq, st.T, st.fit, st.M, R, ts, sh);
+        else if (EXPD) shape_accum<ESP, true>(st.ref, st.refc, st.N, st.fit, st.M, j, lpp, R, ts, sh);
         else shape_accum_direct<ESP, true>(st.ref, st.N, st.fit, st.M, j, lpp, R, ts, sh);
         group_sum(sh, lpp, true);
         shape_finish(sh, ks.K, ks.alpha, ks.s, true);
@@ -792,19 +825,28 @@ void launch_pose(const FssPairBatch* pb, const KScore& ks, LdsPlan lp, int n, co
                      *pb, ks, lp, pp, q, t, so, invB, V, gR, gT, gQ, loss, grad);
 }
 
-// Workgroups of up to 512 threads get a 256-VGPR budget, larger ones 128.  variant 1 = direct
-// inner loop, otherwise the expanded form.
+// Workgroups of up to 512 threads get a 256-VGPR budget, larger ones 128.  Shape inner loop:
+// variant 1 = direct form, 3 = split-bf16 MFMA (one wavefront per pose), otherwise the expanded
+// form; the pharmacophore-only mode has no shape loop and always runs as variant 2.
+int effective_variant(int mode, int variant) {
+  if (mode == FSS_MODE_PHARM) return 2;
+  return variant == 1 || variant == 3 ? variant : 2;
+}
 template <int MODE, bool ES>
 void launch_align(const FssPairBatch* pb, const KScore& ks, LdsPlan lp, const AlignK& ak, int grid,
                   int block, size_t lds, int variant, hipStream_t s) {
-  const bool direct = variant == 1 && MODE != FSS_MODE_PHARM;
+  const int v = effective_variant(MODE, variant);
+#define FSS_K(T, V) hipLaunchKernelGGL((k_align<MODE, ES, T, V>), dim3(grid), dim3(block), lds, s, *pb, ks, lp, ak)
   if (block <= 512) {
-    if (direct) hipLaunchKernelGGL((k_align<MODE, ES, 512, false>), dim3(grid), dim3(block), lds, s, *pb, ks, lp, ak);
-    else hipLaunchKernelGGL((k_align<MODE, ES, 512, true>), dim3(grid), dim3(block), lds, s, *pb, ks, lp, ak);
+    if (v == 1) FSS_K(512, 1);
+    else if (v == 3) FSS_K(512, 3);
+    else FSS_K(512, 2);
   } else {
-    if (direct) hipLaunchKernelGGL((k_align<MODE, ES, 1024, false>), dim3(grid), dim3(block), lds, s, *pb, ks, lp, ak);
-    else hipLaunchKernelGGL((k_align<MODE, ES, 1024, true>), dim3(grid), dim3(block), lds, s, *pb, ks, lp, ak);
+    if (v == 1) FSS_K(1024, 1);
+    else if (v == 3) FSS_K(1024, 3);
+    else FSS_K(1024, 2);
   }
+#undef FSS_K
 }
 
 int score_check(const FssPairBatch* batch, const FssScoreParams* sp) {
@@ -1167,8 +1209,11 @@ int fss_align(const FssPairBatch* batch, const FssScoreParams* sp, const FssAlig
     return set_error(FSS_E_INVALID, "early stop couples all %d restarts of a pair: restarts_per_wg (%d) must be >= n_restarts", B, rpw);
   const int block = ((rpw * lpp + 63) / 64) * 64;
   if (block > 1024) return set_error(FSS_E_INVALID, "restarts_per_wg * lanes_per_pose = %d exceeds 1024 threads", rpw * lpp);
+  const bool mfma = effective_variant(ks.mode, ap->variant) == 3;
+  if (mfma && lpp != 64)
+    return set_error(FSS_E_INVALID, "the MFMA inner loop (variant 3) runs one pose per wavefront: lanes_per_pose must be 64");
   if (batch->n_pairs == 0) return FSS_OK;
-  const LdsPlan lp = plan_lds(batch, ks.mode);
+  const LdsPlan lp = plan_lds(batch, ks.mode, mfma);
   const int groups = block / lpp;
   const size_t lds = (size_t)lp.end * sizeof(float4) + (ap->early_stop ? 2 * (size_t)rpw * sizeof(double) : 0) +
                      (size_t)groups * 16 * sizeof(float);

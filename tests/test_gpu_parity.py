@@ -167,10 +167,16 @@ def test_edge_cases(dev):
     q = np.tile(np.array([1, 0, 0, 0], np.float32), (2, 1, 1))
     t = np.zeros((2, 1, 3), np.float32)
     r = align(pb, ScoreSpec(), to_dev(q, dev), to_dev(t, dev), OptimSpec(max_steps=30))
+    r3 = align(pb, ScoreSpec(), to_dev(q, dev), to_dev(t, dev), OptimSpec(max_steps=30), variant=3)
     torch.cuda.synchronize()
     for i in range(2):
         o = orc.optimize(orc.Pair("shape", ref=ref[i], fit=fit[i]), q[i], t[i], max_steps=30)
         assert abs(float(r.score[i]) - o["score"]) < 1e-3
+        assert abs(float(r3.score[i]) - o["score"]) < 1e-3
+    # the MFMA inner loop needs a whole wavefront per pose
+    with pytest.raises(_lib.FssError, match="lanes_per_pose must be 64"):
+        align(pb, ScoreSpec(), to_dev(q, dev), to_dev(t, dev), OptimSpec(max_steps=1), variant=3,
+              lanes_per_pose=16, restarts_per_wg=1)
     # oversized clouds are rejected with a clear error
     big = [np.zeros((20000, 3), np.float32)]
     pbb = PairBatch(CloudSet.from_arrays(big, device=dev), CloudSet.from_arrays(big, device=dev))
@@ -210,17 +216,61 @@ def test_graph_replays_match_eager(dev):
 
 
 @pytest.mark.parametrize("mode", [_lib.MODE_SHAPE, _lib.MODE_SHAPE_ESP, _lib.MODE_SHAPE_PHARM])
-def test_inner_loop_variants_agree(dev, mode):
-    """The direct and the expanded-form shape inner loops compute the same optimisation."""
+@pytest.mark.parametrize("early_stop", [False, True])
+def test_inner_loop_variants_agree(dev, mode, early_stop):
+    """The direct, expanded-form and split-bf16 MFMA shape inner loops compute the same
+    optimisation, each within the best-restart bar of the FP64 oracle."""
+    print(f"case variants mode={mode} es={early_stop}", flush=True)
     pb, spec, pairs = make_case(mode, 4, dev, seed=23)
     q, t = random_seeds(4, 8, seed=24)
-    opt = OptimSpec(max_steps=60, early_stop=False)
-    a = align(pb, spec, to_dev(q, dev), to_dev(t, dev), opt, variant=1)
-    b = align(pb, spec, to_dev(q, dev), to_dev(t, dev), opt, variant=2)
-    np.testing.assert_allclose(a.pose_score.cpu().numpy(), b.pose_score.cpu().numpy(), atol=2e-4)
+    opt = OptimSpec(max_steps=60, early_stop=early_stop)
+    res = {v: align(pb, spec, to_dev(q, dev), to_dev(t, dev), opt, variant=v) for v in (1, 2, 3)}
+    assert res[3].extra["lanes_per_pose"] == 64
+    ref = res[2].pose_score.cpu().numpy()
+    if not early_stop:
+        for v in (1, 3):
+            np.testing.assert_allclose(res[v].pose_score.cpu().numpy(), ref, atol=2e-4)
     for i, p in enumerate(pairs):
-        o = orc.optimize(p, q[i], t[i], max_steps=60, early_stop=False)
-        assert abs(float(a.score[i]) - o["score"]) < 1e-3
+        o = orc.optimize(p, q[i], t[i], max_steps=60, early_stop=early_stop)
+        for v in (1, 2, 3):
+            assert abs(float(res[v].score[i]) - o["score"]) < 1e-3, (v, i)
+
+
+@pytest.mark.parametrize("mode", [_lib.MODE_SHAPE, _lib.MODE_SHAPE_ESP])
+def test_mfma_loop_far_rows_match_oracle(dev, mode):
+    """Elongated clouds (rows up to ~30 A from the ref centroid, where the MFMA loop's column
+    shift is active) through the persistent optimiser on the split-bf16 MFMA inner loop."""
+    from fast_shepherd_score_amd.align import ScoreSpec
+    from fast_shepherd_score_amd.constants import LAM_SCALING
+    from fast_shepherd_score_amd.packing import CloudSet, PairBatch
+    print(f"case mfma_far_rows mode={mode}", flush=True)
+    rng = np.random.default_rng(42)
+    ref, fit, rw, fw = [], [], [], []
+    for n, m in ((70, 45), (37, 90), (130, 17)):
+        axis = np.linspace(-28.0, 28.0, n)[:, None] * np.array([[1.0, 0.2, 0.1]])
+        ref.append((axis + rng.normal(scale=1.0, size=(n, 3))).astype(np.float32))
+        axm = np.linspace(-26.0, 26.0, m)[:, None] * np.array([[1.0, 0.25, -0.1]])
+        fit.append((axm + rng.normal(scale=1.0, size=(m, 3))).astype(np.float32))
+        rw.append(rng.normal(scale=0.1, size=n).astype(np.float32))
+        fw.append(rng.normal(scale=0.1, size=m).astype(np.float32))
+    esp = mode == _lib.MODE_SHAPE_ESP
+    lam = 0.3 * LAM_SCALING
+    spec = ScoreSpec(mode=mode, alpha=0.81, lam=lam)
+    pb = PairBatch(CloudSet.from_arrays(ref, rw if esp else None, device=dev),
+                   CloudSet.from_arrays(fit, fw if esp else None, device=dev))
+    B, steps = 6, 50
+    q, t = random_seeds(3, B, seed=43, t_scale=3.0)
+    opt = OptimSpec(max_steps=steps, early_stop=False)
+    r3 = align(pb, spec, to_dev(q, dev), to_dev(t, dev), opt, variant=3, restarts_per_wg=2)
+    r2 = align(pb, spec, to_dev(q, dev), to_dev(t, dev), opt, variant=2)
+    np.testing.assert_allclose(r3.pose_score.cpu().numpy(), r2.pose_score.cpu().numpy(), atol=2e-4)
+    for i in range(3):
+        if esp:
+            p = orc.Pair("esp", ref=ref[i], fit=fit[i], ref_esp=rw[i], fit_esp=fw[i], lam=lam)
+        else:
+            p = orc.Pair("shape", ref=ref[i], fit=fit[i])
+        o = orc.optimize(p, q[i], t[i], max_steps=steps, early_stop=False)
+        assert abs(float(r3.score[i]) - o["score"]) < 1e-3, (i, float(r3.score[i]), o["score"])
 
 
 def test_device_seeds_match_reference_rule(dev):
