@@ -339,9 +339,9 @@ __device__ __forceinline__ void shape_accum(const float4* __restrict__ ref, cons
 // LDS (per pair, T = ceil(N / 16) tiles, rows past N have zero coordinates and W = -1e30):
 //   amat[64 t + 16 g + r]     8 bf16: A-operand row r of tile t for K-group g
 //   xq[12 t + 3 g + k]        float4: 2 a~_k of ref rows 16 t + 4 g + 0..3 (k = x, y, z)
-// One wavefront owns one pose; a pass covers 32 fit columns (two B tiles) against all T ref
-// tiles, and each lane folds its partial column sums into the pose sums as the expanded form
-// does per fit row.
+// One wavefront owns one pose; a pass covers 32 fit columns (two B tiles; 16 in a last pass
+// with no more than 16 left) against all T ref tiles, and each lane folds its partial column
+// sums into the pose sums as the expanded form does per fit row.
 // ---------------------------------------------------------------------------------------------
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
@@ -373,21 +373,54 @@ __device__ __forceinline__ bf16x8 mfma_a_last(float c, float w) {
   return a;
 }
 
+// The tile loop of one pass: B tile 0 always, B tile 1 when TWO.
+template <bool TWO>
+__device__ __forceinline__ void mfma_tiles(const float4* __restrict__ am, const float4* __restrict__ xg, int T,
+                                           const bf16x8& B0, const bf16x8& B1, float s0[4], float s1[4]) {
+  const f32x4 zero = {0.f, 0.f, 0.f, 0.f};
+  float v0 = 0.f, sx0 = 0.f, sy0 = 0.f, sz0 = 0.f, v1 = 0.f, sx1 = 0.f, sy1 = 0.f, sz1 = 0.f;
+  for (int t = 0; t < T; ++t) {
+    const float4 araw = am[64 * t];
+    const float4 ax = xg[12 * t], ay = xg[12 * t + 1], az = xg[12 * t + 2];
+    const bf16x8 A = __builtin_bit_cast(bf16x8, araw);
+    const f32x4 d0 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A, B0, zero, 0, 0, 0);
+    f32x4 d1 = zero;
+    if (TWO) d1 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A, B1, zero, 0, 0, 0);
+    const float xa[4] = {ax.x, ax.y, ax.z, ax.w}, ya[4] = {ay.x, ay.y, ay.z, ay.w},
+                za[4] = {az.x, az.y, az.z, az.w};
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const float e0 = fast_exp2(d0[i]);
+      v0 += e0;
+      sx0 = fmaf(e0, xa[i], sx0); sy0 = fmaf(e0, ya[i], sy0); sz0 = fmaf(e0, za[i], sz0);
+      if (TWO) {
+        const float e1 = fast_exp2(d1[i]);
+        v1 += e1;
+        sx1 = fmaf(e1, xa[i], sx1); sy1 = fmaf(e1, ya[i], sy1); sz1 = fmaf(e1, za[i], sz1);
+      }
+    }
+  }
+  s0[0] = v0; s0[1] = sx0; s0[2] = sy0; s0[3] = sz0;
+  s1[0] = v1; s1[1] = sx1; s1[2] = sy1; s1[3] = sz1;
+}
+
 template <bool ESP>
 __device__ __forceinline__ void shape_accum_mfma(const float4* __restrict__ amat, const float4* __restrict__ xq,
                                                  int T, const float4* __restrict__ fit, int M,
                                                  const float R[9], const float ts[3], Acc& acc) {
   const int l = threadIdx.x & 63, g = l >> 4, r = l & 15;
-  const f32x4 zero = {0.f, 0.f, 0.f, 0.f};
   const __bf16 one = (__bf16)1.0f;
-  const int passes = (M + 31) >> 5;
-  for (int p = 0; p < passes; ++p) {
+  const float4* am = amat + 16 * g + r;
+  const float4* xg = xq + 3 * g;
+  for (int c0 = 0; c0 < M; c0 += 32) {
+    // a pass takes 32 fit columns (two B tiles), or 16 when no more than 16 remain
+    const bool two = c0 + 16 < M;
     float X[2], Y[2], Z[2], F[2];
     float4 P[2];
     bf16x8 Bop[2];
 #pragma unroll
     for (int k = 0; k < 2; ++k) {
-      const int b = 32 * p + 16 * k + r;
+      const int b = c0 + 16 * k + r;
       const bool ok = b < M;
       P[k] = fit[ok ? b : 0];
       const float4 q = P[k];
@@ -410,34 +443,16 @@ __device__ __forceinline__ void shape_accum_mfma(const float4* __restrict__ amat
         Bop[k][4] = one; Bop[k][5] = one; Bop[k][6] = -mh; Bop[k][7] = -mm;
       }
     }
-    float v0 = 0.f, sx0 = 0.f, sy0 = 0.f, sz0 = 0.f, v1 = 0.f, sx1 = 0.f, sy1 = 0.f, sz1 = 0.f;
-    const float4* am = amat + 16 * g + r;
-    const float4* xg = xq + 3 * g;
-#pragma unroll 2
-    for (int t = 0; t < T; ++t) {
-      const float4 araw = am[64 * t];
-      const float4 ax = xg[12 * t], ay = xg[12 * t + 1], az = xg[12 * t + 2];
-      const bf16x8 A = __builtin_bit_cast(bf16x8, araw);
-      const f32x4 d0 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A, Bop[0], zero, 0, 0, 0);
-      const f32x4 d1 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A, Bop[1], zero, 0, 0, 0);
-      const float xa[4] = {ax.x, ax.y, ax.z, ax.w}, ya[4] = {ay.x, ay.y, ay.z, ay.w},
-                  za[4] = {az.x, az.y, az.z, az.w};
-#pragma unroll
-      for (int i = 0; i < 4; ++i) {
-        const float e0 = fast_exp2(d0[i]);
-        const float e1 = fast_exp2(d1[i]);
-        v0 += e0;
-        v1 += e1;
-        sx0 = fmaf(e0, xa[i], sx0); sy0 = fmaf(e0, ya[i], sy0); sz0 = fmaf(e0, za[i], sz0);
-        sx1 = fmaf(e1, xa[i], sx1); sy1 = fmaf(e1, ya[i], sy1); sz1 = fmaf(e1, za[i], sz1);
-      }
-    }
+    float s0[4], s1[4];
+    if (two) mfma_tiles<true>(am, xg, T, Bop[0], Bop[1], s0, s1);
+    else mfma_tiles<false>(am, xg, T, Bop[0], Bop[1], s0, s1);
     // The column finish f_b = F_b (s_b / 2 - b~'_b v_b) is linear in the row sums, so every lane
-    // folds its own row-group partials; the pose-level butterfly adds the four groups.
-    const float w0 = v0 * F[0], w1 = v1 * F[1];
-    accumulate_rows<true>(acc, P[0], w0, fmaf(0.5f * F[0], sx0, -X[0] * w0), fmaf(0.5f * F[0], sy0, -Y[0] * w0),
-                          fmaf(0.5f * F[0], sz0, -Z[0] * w0), P[1], w1, fmaf(0.5f * F[1], sx1, -X[1] * w1),
-                          fmaf(0.5f * F[1], sy1, -Y[1] * w1), fmaf(0.5f * F[1], sz1, -Z[1] * w1));
+    // folds its own row-group partials; the pose-level butterfly adds the four groups.  Column
+    // 1 of a one-tile pass has F = 0 and zero sums.
+    const float w0 = s0[0] * F[0], w1 = s1[0] * F[1];
+    accumulate_rows<true>(acc, P[0], w0, fmaf(0.5f * F[0], s0[1], -X[0] * w0), fmaf(0.5f * F[0], s0[2], -Y[0] * w0),
+                          fmaf(0.5f * F[0], s0[3], -Z[0] * w0), P[1], w1, fmaf(0.5f * F[1], s1[1], -X[1] * w1),
+                          fmaf(0.5f * F[1], s1[2], -Y[1] * w1), fmaf(0.5f * F[1], s1[3], -Z[1] * w1));
   }
 }
 
