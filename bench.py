@@ -23,10 +23,22 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-# Algorithmic FLOP per pair term (ref point x fit point x pose x step), counting an FMA as 2:
-# shape value+grad 16, ESP value+grad 19 (fused exponent), directional pharm 36.
+# Algorithmic FLOP per pair term (ref point x fit point x pose x step) of the direct form on
+# pre-scaled coordinates: difference 3, squared norm 5, exp 1, value sum 1, gradient sum 6
+# (= 16 for shape), + 3 for the ESP weight (19); directional pharm 36.
 FLOP_PER_TERM = {"shape": 16, "esp": 19, "pharm": 36}
-PEAK_FP32_TFLOPS = 157.3      # MI355X FP32 vector peak (AMD published)
+PEAK_FP32_TFLOPS = 157.3      # MI355X FP32 peak, vector and matrix pipes alike (AMD published)
+TRAFFIC_TABLE = os.path.join(ROOT, "profiles", "k_align_traffic.json")
+
+
+def measured_traffic(key: str):
+    """HBM bytes per k_align launch measured with rocprofv3 for this exact configuration, or None."""
+    try:
+        with open(TRAFFIC_TABLE) as f:
+            row = json.load(f).get(key)
+    except (OSError, ValueError):
+        return None
+    return None if row is None else int(row["bytes"])
 PEAK_EXP_PER_S = 256 * 4 * 16 * 2.4e9 / 4   # v_exp_f32 at quarter rate: 9.8e12 / s
 
 
@@ -298,6 +310,7 @@ def main():
     flop = terms * FLOP_PER_TERM[wl["kind"]]
     achieved = flop / (kern_ms * 1e-3) / 1e12
     value = args.pairs / (dt / args.steps)
+    mfma = res.extra["variant"] == 3
     score = res.score.cpu().numpy()
     out = {
         "metric": "aligned pairs/sec @50 restarts (1 GPU, <=1e-3 vs FP64); screen p50/p99 ms",
@@ -312,8 +325,13 @@ def main():
                    "restarts_per_wg": res.extra["restarts_per_wg"],
                    "inner_loop": {0: "expanded", 1: "direct", 2: "expanded",
                                   3: "mfma-split-bf16"}[res.extra["variant"]]},
-        "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_FP32_TFLOPS,
-                     "unit": "TFLOP/s", "frac": achieved / PEAK_FP32_TFLOPS, "traffic": None,
+        "roofline": {"bound": "mfma" if mfma else "valu", "achieved": achieved, "peak": PEAK_FP32_TFLOPS,
+                     "unit": "TFLOP/s", "frac": achieved / PEAK_FP32_TFLOPS,
+                     "traffic": measured_traffic(f"{args.workload}/{args.pairs}/{B}/{steps}/"
+                                                 f"{res.extra['lanes_per_pose']}/"
+                                                 f"{res.extra['restarts_per_wg']}/{res.extra['variant']}"),
+                     "compute": ("split-bf16 MFMA exponents (FP32 accumulate) + FP32 VALU exp and sums"
+                                 if mfma else "FP32 VALU"),
                      "kernel": "k_align", "kernel_ms": kern_ms, "finish_ms": fin_ms,
                      "pair_terms_per_launch": terms, "flop_per_term": FLOP_PER_TERM[wl["kind"]],
                      "exp_per_s": terms / (kern_ms * 1e-3),

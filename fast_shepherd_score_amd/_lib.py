@@ -20,6 +20,8 @@ MODE_SHAPE_PHARM = 3
 SIM_TANIMOTO = 0
 SIM_TVERSKY = 1
 N_PTYPES = 9
+SEED_PHARM = 1
+SEED_REF = 2
 
 E_INVALID = 9001
 E_TOO_LARGE = 9002

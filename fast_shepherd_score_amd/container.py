@@ -28,7 +28,7 @@ from .align import OptimSpec, ScoreSpec, align
 from .constants import LAM_SCALING
 from .modes import HIP_MODES, MODE_ATTRS, MODE_SEEDS, MODE_STEPS, canonical
 from .packing import CloudSet, PairBatch, PharmSet
-from .seeds import initial_seeds, pad
+from .seeds import device_seeds, initial_seeds, pad
 
 HIP_BACKENDS = ("hip", "rocm", "triton", "cuda", "gpu")
 
@@ -118,7 +118,12 @@ def align_pairs(pairs: Sequence[MoleculePair], mode: str, *, alpha=0.81, lam=Non
                 color_weight=0.5, similarity="tanimoto", directionless=False,
                 num_repeats: int = 50, lr: float = 0.1, max_num_steps: int = 200,
                 early_stop: bool = True, seeds=None, device=None, **tiling):
-    """Align every pair in one persistent launch; write results in place; return the result."""
+    """Align every pair in one persistent launch; write results in place; return the result.
+
+    ``seeds``: None computes the reference's multi-start seeds on the device (fss_seed_poses);
+    "host" computes the same rule on the host in FP64 (seeds.initial_seeds); a (q0, t0) pair of
+    (n_pairs, S, 4) / (n_pairs, S, 3) arrays or tensors is used as given.
+    """
     mode = canonical(mode)
     if mode not in HIP_MODES:
         raise NotImplementedError(f"mode {mode!r} is not implemented by the HIP backend")
@@ -140,16 +145,20 @@ def align_pairs(pairs: Sequence[MoleculePair], mode: str, *, alpha=0.81, lam=Non
         rph = PharmSet.from_arrays(ph["ref_types"], ph["ref_anchors"], ph["ref_vectors"], device=dev)
         fph = PharmSet.from_arrays(ph["fit_types"], ph["fit_anchors"], ph["fit_vectors"], device=dev)
     pb = PairBatch(ref, fit, rph, fph)
+    # Seeds come from the clouds the objective moves: pharm anchors for pharm, else shape.
     if seeds is None:
-        # Seeds come from the clouds the objective moves: pharm anchors for pharm, else shape.
+        q0, t0 = device_seeds(pb, num_repeats, use_pharm=mode == "pharm")
+    elif isinstance(seeds, str) and seeds == "host":
         sref = ph["ref_anchors"] if mode == "pharm" else rc
         sfit = ph["fit_anchors"] if mode == "pharm" else fc
         (R, rn), (Fp, fn) = pad(sref), pad(sfit)
         q0, t0 = initial_seeds(R, rn, Fp, fn, num_repeats)
     else:
         q0, t0 = seeds
-    q0 = torch.as_tensor(np.asarray(q0, np.float32)).to(dev)
-    t0 = torch.as_tensor(np.asarray(t0, np.float32)).to(dev)
+    q0 = torch.as_tensor(np.asarray(q0, np.float32) if not torch.is_tensor(q0) else q0,
+                         dtype=torch.float32).to(dev)
+    t0 = torch.as_tensor(np.asarray(t0, np.float32) if not torch.is_tensor(t0) else t0,
+                         dtype=torch.float32).to(dev)
     opt = OptimSpec(max_steps=int(max_num_steps), lr=float(lr), early_stop=bool(early_stop))
     res = align(pb, spec, q0, t0, opt, **tiling)
     scores = res.score.cpu().numpy()

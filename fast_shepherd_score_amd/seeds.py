@@ -146,6 +146,45 @@ def initial_seeds(ref: np.ndarray, ref_n: np.ndarray, fit: np.ndarray, fit_n: np
     return q.astype(np.float32), t.astype(np.float32)
 
 
+def effective_restarts(num_repeats: int) -> int:
+    """The reference's restart count rule: 1 stays 1, 2-4 become 5 (_torch.py:253-326)."""
+    S = int(num_repeats)
+    if S < 1:
+        raise ValueError("num_repeats must be >= 1")
+    return 5 if 1 < S < 5 else S
+
+
+def reference_table(S: int) -> np.ndarray:
+    """(S, 4) float32 rotation table of the reference rule: identity rows 0-4 (row 0 stays the
+    identity; the device fills rows 1-4 with the principal-axis seeds) and the Fibonacci-sphere
+    rotations (the 45-entry table at S == 50) from row 5 on."""
+    q = np.zeros((S, 4), np.float32)
+    q[:, 0] = 1.0
+    if S > 5:
+        q[5:] = quats_from_fibo(S - 5).astype(np.float32)
+    return q
+
+
+def device_seeds(pb, num_repeats: int, use_pharm: bool = False, stream=None):
+    """Seeds of every pair of a PairBatch on its device (fss_seed_poses with the reference rule):
+    (q (n, S, 4), t (n, S, 3)) float32 tensors, S = effective_restarts(num_repeats).  The
+    centroids and principal axes come from the pharmacophore anchors when use_pharm."""
+    import torch
+    from . import _lib
+    S = effective_restarts(num_repeats)
+    dev = pb.device
+    table = torch.from_numpy(reference_table(S)).to(dev)
+    q = torch.empty((pb.n_pairs, S, 4), dtype=torch.float32, device=dev)
+    t = torch.empty((pb.n_pairs, S, 3), dtype=torch.float32, device=dev)
+    if pb.n_pairs:
+        st = stream if stream is not None else torch.cuda.current_stream(dev)
+        flags = _lib.SEED_REF | (_lib.SEED_PHARM if use_pharm else 0)
+        bs = pb.struct()
+        _lib.call("fss_seed_poses", _lib.byref(bs), flags, S, _lib.ptr(table), _lib.ptr(q),
+                  _lib.ptr(t), int(st.cuda_stream))
+    return q, t
+
+
 def pad(clouds, width: int = 3):
     """List of (k, width) arrays -> padded (n, L, width) float64 array and counts."""
     counts = np.array([len(c) for c in clouds], dtype=np.int64)
