@@ -25,6 +25,11 @@ SEED_REF = 2
 
 E_INVALID = 9001
 E_TOO_LARGE = 9002
+E_CLOSED = 9004
+PENDING = 1
+LIB_SHAPE = 1
+LIB_PHARM = 2
+SCREEN_MAX_TOPK = 1024
 
 _i32p = ctypes.POINTER(ctypes.c_int32)
 _f32p = ctypes.POINTER(ctypes.c_float)
@@ -85,6 +90,46 @@ class FssAlignOut(ctypes.Structure):
     ]
 
 
+class FssLibSpec(ctypes.Structure):
+    _fields_ = [
+        ("device", ctypes.c_int32), ("channels", ctypes.c_int32),
+        ("max_conformers", ctypes.c_int64), ("max_points", ctypes.c_int64),
+        ("max_pharm", ctypes.c_int64),
+    ]
+
+
+class FssScreenParams(ctypes.Structure):
+    _fields_ = [
+        ("n_restarts", ctypes.c_int32), ("max_steps", ctypes.c_int32),
+        ("lr", ctypes.c_float), ("beta1", ctypes.c_float),
+        ("beta2", ctypes.c_float), ("eps", ctypes.c_float),
+        ("early_stop", ctypes.c_int32), ("es_tol", ctypes.c_float),
+        ("es_patience", ctypes.c_int32), ("top_k", ctypes.c_int32),
+        ("launch_pairs", ctypes.c_int32), ("max_active", ctypes.c_int32),
+        ("max_query_points", ctypes.c_int32), ("max_query_pharm", ctypes.c_int32),
+        ("lanes_per_pose", ctypes.c_int32), ("restarts_per_wg", ctypes.c_int32),
+        ("variant", ctypes.c_int32), ("seed_flags", ctypes.c_int32),
+        ("quat_table", ctypes.c_void_p),
+    ]
+
+
+class FssQuery(ctypes.Structure):
+    _fields_ = [
+        ("n_points", ctypes.c_int32), ("pts", ctypes.c_void_p),
+        ("n_pharm", ctypes.c_int32), ("anc", ctypes.c_void_p),
+        ("vec", ctypes.c_void_p), ("typ", ctypes.c_void_p),
+    ]
+
+
+class FssHits(ctypes.Structure):
+    _fields_ = [
+        ("capacity", ctypes.c_int32), ("n", ctypes.c_int32),
+        ("score", ctypes.c_void_p), ("index", ctypes.c_void_p),
+        ("transform", ctypes.c_void_p),
+        ("queue_ms", ctypes.c_double), ("latency_ms", ctypes.c_double),
+    ]
+
+
 # name -> (restype, argtypes); every symbol include/fss.h declares.
 _P = ctypes.c_void_p
 SIGNATURES = {
@@ -110,6 +155,16 @@ SIGNATURES = {
     "fss_score_bwd": (ctypes.c_int, [_P, _P, _P, _P, _P, _P, _P]),
     "fss_score_fwd_f64": (ctypes.c_int, [_P, _P, _P, _P, _P, _P, _P]),
     "fss_score_bwd_f64": (ctypes.c_int, [_P, _P, _P, _P, _P, _P, _P, _P, _P]),
+    "fss_lib_create": (ctypes.c_int, [_P, _P]),
+    "fss_lib_upload": (ctypes.c_int, [_P, ctypes.c_int32, _P, _P, _P, _P, _P, _P, _P]),
+    "fss_lib_size": (ctypes.c_int, [_P, _P]),
+    "fss_lib_destroy": (ctypes.c_int, [_P]),
+    "fss_screen_create": (ctypes.c_int, [_P, _P, _P, _P]),
+    "fss_screen_submit": (ctypes.c_int, [_P, _P, _P]),
+    "fss_screen_poll": (ctypes.c_int, [_P, ctypes.c_int64, ctypes.c_int32, _P]),
+    "fss_screen_stats": (ctypes.c_int, [_P, _P, _P]),
+    "fss_screen_destroy": (ctypes.c_int, [_P]),
+    "fss_plan_launch": (ctypes.c_int, [ctypes.c_int32, _P, ctypes.c_int64, _P]),
 }
 
 _lock = threading.Lock()

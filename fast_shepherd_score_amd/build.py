@@ -7,8 +7,8 @@ import subprocess
 import tempfile
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-SOURCES = [os.path.join(HERE, "csrc", "fss_kernels.hip")]
-DEPS = [os.path.join(HERE, "csrc", "fss_device.h"),
+SOURCES = [os.path.join(HERE, "csrc", "fss_kernels.hip"), os.path.join(HERE, "csrc", "fss_screen.hip")]
+DEPS = [os.path.join(HERE, "csrc", "fss_device.h"), os.path.join(HERE, "csrc", "fss_common.h"),
         os.path.join(os.path.dirname(HERE), "include", "fss.h")]
 OUT = os.path.join(HERE, "libfss_hip.so")
 ARCH = os.environ.get("FSS_OFFLOAD_ARCH", "gfx950")

@@ -20,15 +20,15 @@
 #include <string>
 
 #include "../../include/fss.h"
+#include "fss_common.h"
 #include "fss_device.h"
 
-using namespace fss;
+namespace fss {
 
 namespace {
-
 thread_local std::string g_last_error;
+}
 
-int set_error(int code, const char* fmt, ...) __attribute__((format(printf, 2, 3)));
 int set_error(int code, const char* fmt, ...) {
   char buf[512];
   va_list ap;
@@ -38,6 +38,14 @@ int set_error(int code, const char* fmt, ...) {
   g_last_error = buf;
   return code;
 }
+
+const char* last_error() { return g_last_error.c_str(); }
+
+}  // namespace fss
+
+using namespace fss;
+
+namespace {
 
 int check_launch(const char* what) {
   hipError_t e = hipGetLastError();
@@ -1067,7 +1075,7 @@ extern "C" {
 
 const char* fss_version(void) { return "fss_hip 0.1.0 gfx950"; }
 
-const char* fss_last_error(void) { return g_last_error.c_str(); }
+const char* fss_last_error(void) { return fss::last_error(); }
 
 int fss_device_count(int* count) {
   if (!count) return set_error(FSS_E_INVALID, "count is NULL");

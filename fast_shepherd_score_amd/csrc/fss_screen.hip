@@ -1,0 +1,843 @@
+// Screening service of libfss_hip.so: a conformer library resident in HBM and a worker thread that
+// aligns queued queries against all of it with continuous batching (the C ABI is in fss.h).
+//
+// Structure (restating the reference's screen / _TopK contract, shepherd_score/screen.py:898-1192,
+// over a resident library instead of streamed shards):
+//   * fss_screen_submit copies the query into a host FIFO and wakes the worker;
+//   * the worker admits queries into free slots (query rows copied into the slot's device arena),
+//     then repeatedly fills one launch of up to launch_pairs pairs from every admitted query
+//     (fss_plan_launch), runs fss_seed_poses -> fss_self_overlap -> fss_align on it and merges
+//     every query's new pair scores into its top-k (k_topk_merge, double-buffered per slot);
+//   * a query whose library range is exhausted has its top-k copied to the host, and pollers
+//     waiting on its ticket are woken.
+// Pair results depend only on the pair (fixed tiling, per-pair seeds and early stop), and the
+// merge orders by (score descending, library index ascending), so a query's hits are the same
+// whatever it shares launches with.
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <string.h>
+
+#include <algorithm>
+#include <atomic>
+#include <chrono>
+#include <condition_variable>
+#include <deque>
+#include <mutex>
+#include <string>
+#include <thread>
+#include <unordered_map>
+#include <unordered_set>
+#include <vector>
+
+#include "../../include/fss.h"
+#include "fss_common.h"
+
+using fss::set_error;
+
+struct FssLibrary {
+  int device = 0, channels = 0;
+  int64_t cap_conf = 0, cap_pts = 0, cap_ph = 0;
+  int64_t n = 0, n_pts = 0, n_ph = 0;
+  int32_t max_pts = 0, max_ph = 0;
+  float* pts = nullptr;        // (cap_pts, 4)
+  int32_t* rng = nullptr;      // (cap_conf, 2)
+  float* anc = nullptr;        // (cap_ph, 4)
+  float* vec = nullptr;        // (cap_ph, 4)
+  int32_t* typ = nullptr;      // (cap_ph)
+  int32_t* prng = nullptr;     // (cap_conf, 2)
+  std::vector<int32_t> h_rng, h_prng;   // host copies of the range tables
+  std::mutex mu;
+  int services = 0;
+};
+
+namespace {
+
+using clk = std::chrono::steady_clock;
+
+double ms_since(clk::time_point t0, clk::time_point t1) {
+  return std::chrono::duration<double, std::milli>(t1 - t0).count();
+}
+
+// Switches the calling thread to `dev` for the lifetime of the guard.
+struct DeviceGuard {
+  int prev = -1;
+  hipError_t err = hipSuccess;
+  explicit DeviceGuard(int dev) {
+    if (hipGetDevice(&prev) != hipSuccess) prev = -1;
+    err = hipSetDevice(dev);
+  }
+  ~DeviceGuard() {
+    if (prev >= 0) (void)hipSetDevice(prev);
+  }
+};
+
+int hip_fail(hipError_t e, const char* what) {
+  return set_error((int)e, "%s: %s", what, hipGetErrorString(e));
+}
+
+template <typename T>
+hipError_t dalloc(T** p, int64_t count) {
+  *p = nullptr;
+  if (count <= 0) return hipSuccess;
+  return hipMalloc((void**)p, (size_t)count * sizeof(T));
+}
+
+// ------------------------------------------------------------------------------------------------
+// Top-k merge: one workgroup per query of a launch.  Candidates are the query's current top-k
+// (prev_n entries, sorted) followed by its k new pair scores (library indices base, base+1, ...).
+// The K-th largest key is found by an 8-bit radix select over order-preserving 64-bit keys of the
+// FP64 scores; everything above it and the first ties in candidate order (which is library-index
+// order among equal scores) are kept, then sorted by (score desc, index asc) with a bitonic sort.
+// ------------------------------------------------------------------------------------------------
+struct MergeJob {
+  const double* prev_s;
+  const int64_t* prev_i;
+  const float* prev_tf;
+  double* out_s;
+  int64_t* out_i;
+  float* out_tf;
+  int64_t base;
+  int32_t prev_n, off, k, pad;
+};
+
+constexpr int kMergeThreads = 1024;
+
+__device__ __forceinline__ uint64_t order_key(double v) {
+  if (v != v) return 0ull;  // NaN sorts last
+  const uint64_t u = (uint64_t)__double_as_longlong(v);
+  return (u >> 63) ? ~u : (u | 0x8000000000000000ull);
+}
+
+__global__ __launch_bounds__(kMergeThreads) void k_topk_merge(int K, const MergeJob* __restrict__ jobs,
+                                                              const double* __restrict__ new_s,
+                                                              const float* __restrict__ new_tf) {
+  __shared__ uint32_t hist[256];
+  __shared__ uint64_t skey[FSS_SCREEN_MAX_TOPK];
+  __shared__ int64_t sidx[FSS_SCREEN_MAX_TOPK];
+  __shared__ int32_t scand[FSS_SCREEN_MAX_TOPK];
+  __shared__ uint32_t wave_cnt[kMergeThreads / 64];
+  __shared__ uint64_t s_prefix;
+  __shared__ int32_t s_count, s_ties;
+  const MergeJob jb = jobs[blockIdx.x];
+  const int tid = threadIdx.x, nt = blockDim.x;
+  const int n = jb.prev_n + jb.k;
+  const int need = min(K, n);
+  auto key_of = [&](int c) -> uint64_t {
+    return order_key(c < jb.prev_n ? jb.prev_s[c] : new_s[jb.off + (c - jb.prev_n)]);
+  };
+  auto idx_of = [&](int c) -> int64_t { return c < jb.prev_n ? jb.prev_i[c] : jb.base + (c - jb.prev_n); };
+  if (tid == 0) { s_count = 0; s_ties = 0; }
+  __syncthreads();
+  if (n <= K) {
+    for (int c = tid; c < n; c += nt) { skey[c] = key_of(c); sidx[c] = idx_of(c); scand[c] = c; }
+  } else {
+    // radix select of the K-th largest key
+    uint64_t prefix = 0, mask = 0;
+    int remaining = K;
+    for (int shift = 56; shift >= 0; shift -= 8) {
+      for (int i = tid; i < 256; i += nt) hist[i] = 0u;
+      __syncthreads();
+      for (int c = tid; c < n; c += nt) {
+        const uint64_t key = key_of(c);
+        if ((key & mask) == prefix) atomicAdd(&hist[(key >> shift) & 255u], 1u);
+      }
+      __syncthreads();
+      if (tid == 0) {
+        int cum = 0, d = 255;
+        for (; d > 0; --d) {
+          if (cum + (int)hist[d] >= remaining) break;
+          cum += (int)hist[d];
+        }
+        s_count = remaining - cum;          // ties still needed inside digit d
+        s_prefix = prefix | ((uint64_t)d << shift);
+      }
+      __syncthreads();
+      remaining = s_count;
+      prefix = s_prefix;
+      mask |= 0xFFull << shift;
+      __syncthreads();
+    }
+    const uint64_t T = prefix;              // the K-th largest key; `remaining` copies of it are kept
+    if (tid == 0) s_count = 0;
+    __syncthreads();
+    for (int c = tid; c < n; c += nt) {
+      const uint64_t key = key_of(c);
+      if (key > T) {
+        const int pos = atomicAdd(&s_count, 1);
+        skey[pos] = key; sidx[pos] = idx_of(c); scand[pos] = c;
+      }
+    }
+    __syncthreads();
+    const int above = s_count;              // == K - remaining
+    for (int c0 = 0; c0 < n; c0 += nt) {
+      const int c = c0 + tid;
+      const bool tie = c < n && key_of(c) == T;
+      const uint64_t bal = __ballot(tie);
+      const int lane = tid & 63, w = tid >> 6;
+      const int in_wave = __popcll(bal & ((1ull << lane) - 1ull));
+      if (lane == 0) wave_cnt[w] = (uint32_t)__popcll(bal);
+      __syncthreads();
+      int before = s_ties;
+      for (int i = 0; i < w; ++i) before += (int)wave_cnt[i];
+      const int pos = before + in_wave;
+      if (tie && pos < remaining) { skey[above + pos] = T; sidx[above + pos] = idx_of(c); scand[above + pos] = c; }
+      __syncthreads();
+      if (tid == 0) {
+        int tot = 0;
+        for (int i = 0; i < nt / 64; ++i) tot += (int)wave_cnt[i];
+        s_ties += tot;
+      }
+      __syncthreads();
+      if (s_ties >= remaining) break;
+    }
+  }
+  __syncthreads();
+  // bitonic sort of the `need` selected entries by (key desc, index asc); padding sorts last
+  int L = 1;
+  while (L < need) L <<= 1;
+  for (int i = need + tid; i < L; i += nt) { skey[i] = 0ull; sidx[i] = INT64_MAX; scand[i] = -1; }
+  __syncthreads();
+  for (int size = 2; size <= L; size <<= 1) {
+    for (int stride = size >> 1; stride > 0; stride >>= 1) {
+      for (int i = tid; i < L; i += nt) {
+        const int j = i ^ stride;
+        if (j > i) {
+          // "before" = larger key, then smaller index
+          const bool i_first = skey[i] > skey[j] || (skey[i] == skey[j] && sidx[i] < sidx[j]);
+          const bool up = (i & size) == 0;
+          if (up != i_first) {
+            const uint64_t tk = skey[i]; skey[i] = skey[j]; skey[j] = tk;
+            const int64_t ti = sidx[i]; sidx[i] = sidx[j]; sidx[j] = ti;
+            const int32_t tc = scand[i]; scand[i] = scand[j]; scand[j] = tc;
+          }
+        }
+      }
+      __syncthreads();
+    }
+  }
+  for (int r = tid; r < need; r += nt) {
+    const int c = scand[r];
+    const bool prev = c < jb.prev_n;
+    jb.out_s[r] = prev ? jb.prev_s[c] : new_s[jb.off + (c - jb.prev_n)];
+    jb.out_i[r] = sidx[r];
+  }
+  for (int e = tid; e < need * 16; e += nt) {
+    const int r = e >> 4, m = e & 15;
+    const int c = scand[r];
+    jb.out_tf[e] = c < jb.prev_n ? jb.prev_tf[16 * c + m] : new_tf[16 * (jb.off + (c - jb.prev_n)) + m];
+  }
+}
+
+struct QueryIn {
+  int64_t ticket = 0;
+  std::vector<float> pts, anc, vec;
+  std::vector<int32_t> typ;
+  int32_t np = 0, nph = 0;
+  clk::time_point t_submit;
+};
+
+struct Slot {
+  bool used = false;
+  QueryIn q;
+  int64_t next = 0, lib_n = 0;
+  int32_t filled = 0, buf = 0;
+  clk::time_point t_start;
+};
+
+struct Result {
+  int rc = FSS_OK;
+  std::string err;
+  std::vector<double> score;
+  std::vector<int64_t> index;
+  std::vector<float> tf;
+  double queue_ms = 0, latency_ms = 0;
+};
+
+}  // namespace
+
+struct FssScreen {
+  FssLibrary* lib = nullptr;
+  FssScoreParams sp{};
+  FssScreenParams prm{};
+  int B = 0, K = 0, P = 0, A = 0;
+  int need_shape = 0, need_pharm = 0;
+  hipStream_t stream = nullptr;
+  // device arena
+  float *q = nullptr, *t = nullptr, *table = nullptr, *best_tf = nullptr;
+  double *pose_score = nullptr, *best_score = nullptr, *self_ov = nullptr;
+  int32_t *best_idx = nullptr, *n_steps = nullptr;
+  int32_t *d_rng = nullptr;            // 4 x (P, 2): ref, fit, ref pharm, fit pharm
+  int32_t* h_rng = nullptr;            // pinned host staging of d_rng
+  float *qpts = nullptr, *qanc = nullptr, *qvec = nullptr;
+  int32_t* qtyp = nullptr;
+  double* tk_s = nullptr;              // (A, 2, K)
+  int64_t* tk_i = nullptr;             // (A, 2, K)
+  float* tk_tf = nullptr;              // (A, 2, K, 16)
+  MergeJob* d_jobs = nullptr;
+  MergeJob* h_jobs = nullptr;          // pinned
+  // scheduler state
+  std::vector<Slot> slots;
+  std::deque<QueryIn> pending;
+  std::unordered_set<int64_t> inflight;
+  std::unordered_map<int64_t, Result> results;
+  std::mutex mu;
+  std::condition_variable cv_work, cv_done;
+  bool stop = false;
+  int64_t next_ticket = 1;
+  std::atomic<int64_t> launches{0}, pairs{0};
+  std::thread worker;
+};
+
+namespace {
+
+void free_arena(FssScreen* s) {
+  void* dev_ptrs[] = {s->q, s->t, s->table, s->best_tf, s->pose_score, s->best_score, s->self_ov,
+                      s->best_idx, s->n_steps, s->d_rng, s->qpts, s->qanc, s->qvec, s->qtyp,
+                      s->tk_s, s->tk_i, s->tk_tf, s->d_jobs};
+  for (void* p : dev_ptrs)
+    if (p) (void)hipFree(p);
+  if (s->h_rng) (void)hipHostFree(s->h_rng);
+  if (s->h_jobs) (void)hipHostFree(s->h_jobs);
+  if (s->stream) (void)hipStreamDestroy(s->stream);
+}
+
+void finish(FssScreen* s, int64_t ticket, Result&& r) {
+  std::lock_guard<std::mutex> lk(s->mu);
+  s->inflight.erase(ticket);
+  s->results[ticket] = std::move(r);
+  s->cv_done.notify_all();
+}
+
+// Copies the query rows into the slot's arena (stream-ordered; the slot keeps the host vectors
+// alive until the next synchronisation).
+int admit(FssScreen* s, int slot) {
+  Slot& sl = s->slots[slot];
+  const size_t mq = (size_t)s->prm.max_query_points, mp = (size_t)s->prm.max_query_pharm;
+  hipError_t e = hipSuccess;
+  if (s->need_shape && sl.q.np)
+    e = hipMemcpyAsync(s->qpts + 4 * mq * slot, sl.q.pts.data(), sizeof(float) * 4 * sl.q.np,
+                       hipMemcpyHostToDevice, s->stream);
+  if (e == hipSuccess && s->need_pharm && sl.q.nph) {
+    e = hipMemcpyAsync(s->qanc + 4 * mp * slot, sl.q.anc.data(), sizeof(float) * 4 * sl.q.nph,
+                       hipMemcpyHostToDevice, s->stream);
+    if (e == hipSuccess)
+      e = hipMemcpyAsync(s->qvec + 4 * mp * slot, sl.q.vec.data(), sizeof(float) * 4 * sl.q.nph,
+                         hipMemcpyHostToDevice, s->stream);
+    if (e == hipSuccess)
+      e = hipMemcpyAsync(s->qtyp + mp * slot, sl.q.typ.data(), sizeof(int32_t) * sl.q.nph,
+                         hipMemcpyHostToDevice, s->stream);
+  }
+  if (e != hipSuccess) return hip_fail(e, "query upload");
+  {
+    std::lock_guard<std::mutex> lk(s->lib->mu);
+    sl.lib_n = s->lib->n;
+  }
+  sl.next = 0;
+  sl.filled = 0;
+  sl.buf = 0;
+  sl.t_start = clk::now();
+  return FSS_OK;
+}
+
+// One launch over the admitted slots `act`.
+int launch(FssScreen* s, const std::vector<int>& act) {
+  const int na = (int)act.size();
+  std::vector<int64_t> rem(na), take(na);
+  for (int i = 0; i < na; ++i) rem[i] = s->slots[act[i]].lib_n - s->slots[act[i]].next;
+  fss_plan_launch(na, rem.data(), s->P, take.data());
+  const size_t mq = (size_t)s->prm.max_query_points, mp = (size_t)s->prm.max_query_pharm;
+  int32_t* hr = s->h_rng;
+  int32_t* hf = hr + 2 * (size_t)s->P;
+  int32_t* hrp = hf + 2 * (size_t)s->P;
+  int32_t* hfp = hrp + 2 * (size_t)s->P;
+  int p = 0, nj = 0, max_np = 0, max_nph = 0;
+  int32_t lib_max_pts = 0, lib_max_ph = 0;
+  {
+    std::lock_guard<std::mutex> lk(s->lib->mu);
+    lib_max_pts = s->lib->max_pts;
+    lib_max_ph = s->lib->max_ph;
+    for (int i = 0; i < na; ++i) {
+      if (take[i] == 0) continue;
+      Slot& sl = s->slots[act[i]];
+      MergeJob& jb = s->h_jobs[nj++];
+      const size_t cur = ((size_t)act[i] * 2 + sl.buf) * s->K, nxt = ((size_t)act[i] * 2 + (sl.buf ^ 1)) * s->K;
+      jb.prev_s = s->tk_s + cur; jb.prev_i = s->tk_i + cur; jb.prev_tf = s->tk_tf + 16 * cur;
+      jb.out_s = s->tk_s + nxt; jb.out_i = s->tk_i + nxt; jb.out_tf = s->tk_tf + 16 * nxt;
+      jb.base = sl.next; jb.prev_n = sl.filled; jb.off = p; jb.k = (int32_t)take[i]; jb.pad = 0;
+      max_np = std::max(max_np, sl.q.np);
+      max_nph = std::max(max_nph, sl.q.nph);
+      for (int64_t j = 0; j < take[i]; ++j, ++p) {
+        const int64_t L = sl.next + j;
+        hr[2 * p] = (int32_t)(mq * act[i]); hr[2 * p + 1] = sl.q.np;
+        hrp[2 * p] = (int32_t)(mp * act[i]); hrp[2 * p + 1] = sl.q.nph;
+        if (s->need_shape) { hf[2 * p] = s->lib->h_rng[2 * L]; hf[2 * p + 1] = s->lib->h_rng[2 * L + 1]; }
+        else { hf[2 * p] = 0; hf[2 * p + 1] = 0; }
+        if (s->need_pharm) { hfp[2 * p] = s->lib->h_prng[2 * L]; hfp[2 * p + 1] = s->lib->h_prng[2 * L + 1]; }
+        else { hfp[2 * p] = 0; hfp[2 * p + 1] = 0; }
+      }
+    }
+  }
+  if (p == 0) return FSS_OK;
+  hipError_t e = hipMemcpyAsync(s->d_rng, s->h_rng, sizeof(int32_t) * 8 * (size_t)s->P, hipMemcpyHostToDevice, s->stream);
+  if (e == hipSuccess) e = hipMemcpyAsync(s->d_jobs, s->h_jobs, sizeof(MergeJob) * nj, hipMemcpyHostToDevice, s->stream);
+  if (e != hipSuccess) return hip_fail(e, "launch staging");
+  FssPairBatch pb{};
+  pb.n_pairs = p;
+  pb.ref_pts = s->qpts; pb.fit_pts = s->lib->pts;
+  pb.ref_rng = s->d_rng; pb.fit_rng = s->d_rng + 2 * (size_t)s->P;
+  pb.ref_anc = s->qanc; pb.ref_vec = s->qvec; pb.ref_typ = s->qtyp; pb.ref_prng = s->d_rng + 4 * (size_t)s->P;
+  pb.fit_anc = s->lib->anc; pb.fit_vec = s->lib->vec; pb.fit_typ = s->lib->typ; pb.fit_prng = s->d_rng + 6 * (size_t)s->P;
+  pb.max_ref = s->need_shape ? std::max(max_np, 1) : 0;
+  pb.max_fit = s->need_shape ? std::max(lib_max_pts, 1) : 0;
+  pb.max_ref_ph = s->need_pharm ? std::max(max_nph, 1) : 0;
+  pb.max_fit_ph = s->need_pharm ? std::max(lib_max_ph, 1) : 0;
+  int rc = fss_seed_poses(&pb, s->prm.seed_flags, s->B, s->table, s->q, s->t, s->stream);
+  if (!rc) rc = fss_self_overlap(&pb, &s->sp, s->self_ov, s->stream);
+  if (!rc) {
+    FssAlignParams ap{};
+    ap.n_restarts = s->B; ap.max_steps = s->prm.max_steps;
+    ap.lr = s->prm.lr; ap.beta1 = s->prm.beta1; ap.beta2 = s->prm.beta2; ap.eps = s->prm.eps;
+    ap.early_stop = s->prm.early_stop; ap.es_tol = s->prm.es_tol; ap.es_patience = s->prm.es_patience;
+    ap.lanes_per_pose = s->prm.lanes_per_pose; ap.restarts_per_wg = s->prm.restarts_per_wg;
+    ap.pair_order = nullptr; ap.skip_finish = 0; ap.variant = s->prm.variant;
+    FssAlignOut out{};
+    out.q = s->q; out.t = s->t; out.pose_score = s->pose_score; out.best_score = s->best_score;
+    out.best_idx = s->best_idx; out.best_tf = s->best_tf; out.n_steps = s->n_steps;
+    rc = fss_align(&pb, &s->sp, &ap, s->self_ov, &out, s->stream);
+  }
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_topk_merge, dim3(nj), dim3(kMergeThreads), 0, s->stream, s->K, s->d_jobs,
+                     s->best_score, s->best_tf);
+  e = hipGetLastError();
+  if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
+  if (e != hipSuccess) return hip_fail(e, "screening launch");
+  s->launches.fetch_add(1);
+  s->pairs.fetch_add(p);
+  const auto now = clk::now();
+  for (int i = 0; i < na; ++i) {
+    if (take[i] == 0) continue;
+    Slot& sl = s->slots[act[i]];
+    sl.next += take[i];
+    sl.filled = (int32_t)std::min<int64_t>(s->K, (int64_t)sl.filled + take[i]);
+    sl.buf ^= 1;
+  }
+  for (int i = 0; i < na; ++i) {
+    Slot& sl = s->slots[act[i]];
+    if (sl.next < sl.lib_n) continue;
+    Result r;
+    const size_t cur = ((size_t)act[i] * 2 + sl.buf) * s->K;
+    r.score.resize(sl.filled); r.index.resize(sl.filled); r.tf.resize(16 * (size_t)sl.filled);
+    if (sl.filled) {
+      e = hipMemcpy(r.score.data(), s->tk_s + cur, sizeof(double) * sl.filled, hipMemcpyDeviceToHost);
+      if (e == hipSuccess) e = hipMemcpy(r.index.data(), s->tk_i + cur, sizeof(int64_t) * sl.filled, hipMemcpyDeviceToHost);
+      if (e == hipSuccess) e = hipMemcpy(r.tf.data(), s->tk_tf + 16 * cur, sizeof(float) * 16 * sl.filled, hipMemcpyDeviceToHost);
+      if (e != hipSuccess) return hip_fail(e, "hit download");
+    }
+    r.queue_ms = ms_since(sl.q.t_submit, sl.t_start);
+    r.latency_ms = ms_since(sl.q.t_submit, now);
+    const int64_t ticket = sl.q.ticket;
+    sl.used = false;
+    sl.q = QueryIn();
+    finish(s, ticket, std::move(r));
+  }
+  return FSS_OK;
+}
+
+void fail_active(FssScreen* s, int rc, const std::string& msg) {
+  for (Slot& sl : s->slots) {
+    if (!sl.used) continue;
+    Result r;
+    r.rc = rc;
+    r.err = msg;
+    const int64_t ticket = sl.q.ticket;
+    sl.used = false;
+    sl.q = QueryIn();
+    finish(s, ticket, std::move(r));
+  }
+}
+
+void worker_main(FssScreen* s) {
+  if (hipSetDevice(s->lib->device) != hipSuccess) {
+    std::lock_guard<std::mutex> lk(s->mu);
+    s->stop = true;
+  }
+  for (;;) {
+    std::vector<int> act;
+    std::vector<int> admitted;
+    {
+      std::unique_lock<std::mutex> lk(s->mu);
+      for (int i = 0; i < s->A && !s->pending.empty(); ++i) {
+        if (s->slots[i].used) continue;
+        s->slots[i].q = std::move(s->pending.front());
+        s->pending.pop_front();
+        s->slots[i].used = true;
+        admitted.push_back(i);
+      }
+      for (int i = 0; i < s->A; ++i)
+        if (s->slots[i].used) act.push_back(i);
+      if (act.empty()) {
+        if (s->stop) break;
+        s->cv_work.wait(lk, [&] { return s->stop || !s->pending.empty(); });
+        continue;
+      }
+      if (s->stop) break;
+    }
+    for (int i : admitted) {
+      if (admit(s, i)) {
+        Result r;
+        r.rc = FSS_E_INVALID;
+        r.err = fss::last_error();
+        const int64_t ticket = s->slots[i].q.ticket;
+        s->slots[i].used = false;
+        s->slots[i].q = QueryIn();
+        finish(s, ticket, std::move(r));
+        act.erase(std::remove(act.begin(), act.end(), i), act.end());
+      }
+    }
+    if (act.empty()) continue;
+    const int rc = launch(s, act);
+    if (rc) {
+      (void)hipStreamSynchronize(s->stream);
+      fail_active(s, rc, fss::last_error());
+    }
+  }
+  // closing: fail what is still queued or admitted
+  fail_active(s, FSS_E_CLOSED, "the screening service was closed");
+  std::deque<QueryIn> left;
+  {
+    std::lock_guard<std::mutex> lk(s->mu);
+    left.swap(s->pending);
+  }
+  for (QueryIn& q : left) {
+    Result r;
+    r.rc = FSS_E_CLOSED;
+    r.err = "the screening service was closed";
+    finish(s, q.ticket, std::move(r));
+  }
+}
+
+}  // namespace
+
+extern "C" {
+
+int fss_plan_launch(int32_t n, const int64_t* remaining, int64_t capacity, int64_t* take) {
+  if (n < 0 || (n > 0 && (!remaining || !take)) || capacity < 0)
+    return set_error(FSS_E_INVALID, "bad launch plan arguments");
+  std::vector<int> live;
+  for (int i = 0; i < n; ++i) {
+    take[i] = 0;
+    if (remaining[i] > 0) live.push_back(i);
+  }
+  int64_t left = capacity;
+  while (left > 0 && !live.empty()) {
+    const int64_t share = std::max<int64_t>(1, left / (int64_t)live.size());
+    std::vector<int> nxt;
+    for (int i : live) {
+      const int64_t k = std::min(std::min(share, remaining[i] - take[i]), left);
+      take[i] += k;
+      left -= k;
+      if (take[i] < remaining[i]) nxt.push_back(i);
+      if (left == 0) break;
+    }
+    live.swap(nxt);
+  }
+  return FSS_OK;
+}
+
+int fss_lib_create(const FssLibSpec* spec, FssLibrary** out) {
+  if (!spec || !out) return set_error(FSS_E_INVALID, "NULL library arguments");
+  *out = nullptr;
+  if (!(spec->channels & (FSS_LIB_SHAPE | FSS_LIB_PHARM)) || (spec->channels & ~(FSS_LIB_SHAPE | FSS_LIB_PHARM)))
+    return set_error(FSS_E_INVALID, "channels must be FSS_LIB_SHAPE and/or FSS_LIB_PHARM");
+  if (spec->max_conformers < 1 || spec->max_conformers > INT32_MAX)
+    return set_error(FSS_E_INVALID, "max_conformers must be in [1, 2^31)");
+  if ((spec->channels & FSS_LIB_SHAPE) && (spec->max_points < 1 || spec->max_points > INT32_MAX))
+    return set_error(FSS_E_INVALID, "max_points must be in [1, 2^31)");
+  if ((spec->channels & FSS_LIB_PHARM) && (spec->max_pharm < 1 || spec->max_pharm > INT32_MAX))
+    return set_error(FSS_E_INVALID, "max_pharm must be in [1, 2^31)");
+  DeviceGuard dg(spec->device);
+  if (dg.err != hipSuccess) return hip_fail(dg.err, "fss_lib_create: device");
+  FssLibrary* lib = new FssLibrary();
+  lib->device = spec->device;
+  lib->channels = spec->channels;
+  lib->cap_conf = spec->max_conformers;
+  lib->cap_pts = (spec->channels & FSS_LIB_SHAPE) ? spec->max_points : 0;
+  lib->cap_ph = (spec->channels & FSS_LIB_PHARM) ? spec->max_pharm : 0;
+  hipError_t e = hipSuccess;
+  if (lib->cap_pts) {
+    e = dalloc(&lib->pts, 4 * lib->cap_pts);
+    if (e == hipSuccess) e = dalloc(&lib->rng, 2 * lib->cap_conf);
+  }
+  if (e == hipSuccess && lib->cap_ph) {
+    e = dalloc(&lib->anc, 4 * lib->cap_ph);
+    if (e == hipSuccess) e = dalloc(&lib->vec, 4 * lib->cap_ph);
+    if (e == hipSuccess) e = dalloc(&lib->typ, lib->cap_ph);
+    if (e == hipSuccess) e = dalloc(&lib->prng, 2 * lib->cap_conf);
+  }
+  if (e != hipSuccess) {
+    void* ps[] = {lib->pts, lib->rng, lib->anc, lib->vec, lib->typ, lib->prng};
+    for (void* p : ps)
+      if (p) (void)hipFree(p);
+    delete lib;
+    return hip_fail(e, "fss_lib_create: allocation");
+  }
+  *out = lib;
+  return FSS_OK;
+}
+
+int fss_lib_upload(FssLibrary* lib, int32_t n, const float* pts, const int32_t* counts,
+                   const float* anc, const float* vec, const int32_t* typ, const int32_t* pcounts,
+                   int64_t* first) {
+  if (!lib) return set_error(FSS_E_INVALID, "library is NULL");
+  if (n < 0) return set_error(FSS_E_INVALID, "n < 0");
+  const bool sh = lib->channels & FSS_LIB_SHAPE, ph = lib->channels & FSS_LIB_PHARM;
+  if (n > 0 && sh && (!pts || !counts)) return set_error(FSS_E_INVALID, "shape channel arrays are NULL");
+  if (n > 0 && ph && (!anc || !vec || !typ || !pcounts))
+    return set_error(FSS_E_INVALID, "pharmacophore channel arrays are NULL");
+  std::lock_guard<std::mutex> lk(lib->mu);
+  if (first) *first = lib->n;
+  if (n == 0) return FSS_OK;
+  if (lib->n + n > lib->cap_conf) return set_error(FSS_E_TOO_LARGE, "library capacity %lld conformers exceeded", (long long)lib->cap_conf);
+  int64_t rows = 0, prow = 0;
+  int32_t mx = lib->max_pts, mph = lib->max_ph;
+  for (int32_t i = 0; i < n; ++i) {
+    if (sh) {
+      if (counts[i] < 0) return set_error(FSS_E_INVALID, "negative count");
+      rows += counts[i];
+      mx = std::max(mx, counts[i]);
+    }
+    if (ph) {
+      if (pcounts[i] < 0) return set_error(FSS_E_INVALID, "negative pharmacophore count");
+      prow += pcounts[i];
+      mph = std::max(mph, pcounts[i]);
+    }
+  }
+  if (sh && lib->n_pts + rows > lib->cap_pts) return set_error(FSS_E_TOO_LARGE, "library capacity %lld points exceeded", (long long)lib->cap_pts);
+  if (ph && lib->n_ph + prow > lib->cap_ph) return set_error(FSS_E_TOO_LARGE, "library capacity %lld pharmacophores exceeded", (long long)lib->cap_ph);
+  DeviceGuard dg(lib->device);
+  if (dg.err != hipSuccess) return hip_fail(dg.err, "fss_lib_upload: device");
+  std::vector<int32_t> r(2 * (size_t)n), pr(2 * (size_t)n);
+  int64_t o = lib->n_pts, po = lib->n_ph;
+  for (int32_t i = 0; i < n; ++i) {
+    if (sh) { r[2 * i] = (int32_t)o; r[2 * i + 1] = counts[i]; o += counts[i]; }
+    if (ph) { pr[2 * i] = (int32_t)po; pr[2 * i + 1] = pcounts[i]; po += pcounts[i]; }
+  }
+  hipError_t e = hipSuccess;
+  if (sh) {
+    if (rows) e = hipMemcpy(lib->pts + 4 * lib->n_pts, pts, sizeof(float) * 4 * rows, hipMemcpyDefault);
+    if (e == hipSuccess) e = hipMemcpy(lib->rng + 2 * lib->n, r.data(), sizeof(int32_t) * 2 * n, hipMemcpyHostToDevice);
+  }
+  if (e == hipSuccess && ph) {
+    if (prow) {
+      e = hipMemcpy(lib->anc + 4 * lib->n_ph, anc, sizeof(float) * 4 * prow, hipMemcpyDefault);
+      if (e == hipSuccess) e = hipMemcpy(lib->vec + 4 * lib->n_ph, vec, sizeof(float) * 4 * prow, hipMemcpyDefault);
+      if (e == hipSuccess) e = hipMemcpy(lib->typ + lib->n_ph, typ, sizeof(int32_t) * prow, hipMemcpyDefault);
+    }
+    if (e == hipSuccess) e = hipMemcpy(lib->prng + 2 * lib->n, pr.data(), sizeof(int32_t) * 2 * n, hipMemcpyHostToDevice);
+  }
+  if (e != hipSuccess) return hip_fail(e, "fss_lib_upload");
+  if (sh) lib->h_rng.insert(lib->h_rng.end(), r.begin(), r.end());
+  if (ph) lib->h_prng.insert(lib->h_prng.end(), pr.begin(), pr.end());
+  lib->n += n;
+  lib->n_pts = o;
+  lib->n_ph = po;
+  lib->max_pts = mx;
+  lib->max_ph = mph;
+  return FSS_OK;
+}
+
+int fss_lib_size(const FssLibrary* lib, int64_t* n) {
+  if (!lib || !n) return set_error(FSS_E_INVALID, "NULL library arguments");
+  std::lock_guard<std::mutex> lk(const_cast<FssLibrary*>(lib)->mu);
+  *n = lib->n;
+  return FSS_OK;
+}
+
+int fss_lib_destroy(FssLibrary* lib) {
+  if (!lib) return FSS_OK;
+  {
+    std::lock_guard<std::mutex> lk(lib->mu);
+    if (lib->services > 0) return set_error(FSS_E_INVALID, "the library still has %d screening service(s)", lib->services);
+  }
+  DeviceGuard dg(lib->device);
+  void* ps[] = {lib->pts, lib->rng, lib->anc, lib->vec, lib->typ, lib->prng};
+  for (void* p : ps)
+    if (p) (void)hipFree(p);
+  delete lib;
+  return FSS_OK;
+}
+
+int fss_screen_create(FssLibrary* lib, const FssScoreParams* sp, const FssScreenParams* prm,
+                      FssScreen** out) {
+  if (!lib || !sp || !prm || !out) return set_error(FSS_E_INVALID, "NULL screening arguments");
+  *out = nullptr;
+  if (sp->mode < 0 || sp->mode > 3) return set_error(FSS_E_INVALID, "unknown mode %d", sp->mode);
+  const bool need_shape = sp->mode != FSS_MODE_PHARM;
+  const bool need_pharm = sp->mode == FSS_MODE_PHARM || sp->mode == FSS_MODE_SHAPE_PHARM;
+  if (need_shape && !(lib->channels & FSS_LIB_SHAPE)) return set_error(FSS_E_INVALID, "the mode needs the library's shape channel");
+  if (need_pharm && !(lib->channels & FSS_LIB_PHARM)) return set_error(FSS_E_INVALID, "the mode needs the library's pharmacophore channel");
+  if (prm->n_restarts < 1) return set_error(FSS_E_INVALID, "n_restarts must be >= 1");
+  if (prm->max_steps < 0) return set_error(FSS_E_INVALID, "max_steps must be >= 0");
+  if (prm->top_k < 1 || prm->top_k > FSS_SCREEN_MAX_TOPK) return set_error(FSS_E_INVALID, "top_k must be in [1, %d]", FSS_SCREEN_MAX_TOPK);
+  if (prm->launch_pairs < 1) return set_error(FSS_E_INVALID, "launch_pairs must be >= 1");
+  if (prm->max_active < 1 || prm->max_active > 4096) return set_error(FSS_E_INVALID, "max_active must be in [1, 4096]");
+  if (prm->launch_pairs < prm->max_active) return set_error(FSS_E_INVALID, "launch_pairs must be >= max_active");
+  if (need_shape && prm->max_query_points < 1) return set_error(FSS_E_INVALID, "max_query_points must be >= 1");
+  if (need_pharm && prm->max_query_pharm < 1) return set_error(FSS_E_INVALID, "max_query_pharm must be >= 1");
+  if (!prm->quat_table) return set_error(FSS_E_INVALID, "quat_table is NULL");
+  if ((int64_t)prm->max_active * prm->max_query_points > INT32_MAX || (int64_t)prm->max_active * prm->max_query_pharm > INT32_MAX)
+    return set_error(FSS_E_INVALID, "query arena exceeds 2^31 rows");
+  DeviceGuard dg(lib->device);
+  if (dg.err != hipSuccess) return hip_fail(dg.err, "fss_screen_create: device");
+  FssScreen* s = new FssScreen();
+  s->lib = lib;
+  s->sp = *sp;
+  s->prm = *prm;
+  s->B = prm->n_restarts;
+  s->K = prm->top_k;
+  s->P = prm->launch_pairs;
+  s->A = prm->max_active;
+  s->need_shape = need_shape;
+  s->need_pharm = need_pharm;
+  if (!s->prm.lanes_per_pose) s->prm.lanes_per_pose = 4;
+  if (!s->prm.restarts_per_wg) s->prm.restarts_per_wg = s->B;
+  if (!need_shape) s->prm.max_query_points = 0;
+  if (!need_pharm) s->prm.max_query_pharm = 0;
+  s->slots.resize(s->A);
+  const int64_t PB = (int64_t)s->P * s->B;
+  const int64_t AK = (int64_t)s->A * 2 * s->K;
+  hipError_t e = hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking);
+  if (e == hipSuccess) e = dalloc(&s->q, 4 * PB);
+  if (e == hipSuccess) e = dalloc(&s->t, 3 * PB);
+  if (e == hipSuccess) e = dalloc(&s->pose_score, PB);
+  if (e == hipSuccess) e = dalloc(&s->table, 4 * (int64_t)s->B);
+  if (e == hipSuccess) e = dalloc(&s->best_score, s->P);
+  if (e == hipSuccess) e = dalloc(&s->best_idx, s->P);
+  if (e == hipSuccess) e = dalloc(&s->best_tf, 16 * (int64_t)s->P);
+  if (e == hipSuccess) e = dalloc(&s->n_steps, s->P);
+  if (e == hipSuccess) e = dalloc(&s->self_ov, 4 * (int64_t)s->P);
+  if (e == hipSuccess) e = dalloc(&s->d_rng, 8 * (int64_t)s->P);
+  if (e == hipSuccess) e = hipHostMalloc((void**)&s->h_rng, sizeof(int32_t) * 8 * (size_t)s->P, hipHostMallocDefault);
+  if (e == hipSuccess && need_shape) e = dalloc(&s->qpts, 4 * (int64_t)s->A * s->prm.max_query_points);
+  if (e == hipSuccess && need_pharm) {
+    e = dalloc(&s->qanc, 4 * (int64_t)s->A * s->prm.max_query_pharm);
+    if (e == hipSuccess) e = dalloc(&s->qvec, 4 * (int64_t)s->A * s->prm.max_query_pharm);
+    if (e == hipSuccess) e = dalloc(&s->qtyp, (int64_t)s->A * s->prm.max_query_pharm);
+  }
+  if (e == hipSuccess) e = dalloc(&s->tk_s, AK);
+  if (e == hipSuccess) e = dalloc(&s->tk_i, AK);
+  if (e == hipSuccess) e = dalloc(&s->tk_tf, 16 * AK);
+  if (e == hipSuccess) e = dalloc(&s->d_jobs, s->A);
+  if (e == hipSuccess) e = hipHostMalloc((void**)&s->h_jobs, sizeof(MergeJob) * (size_t)s->A, hipHostMallocDefault);
+  if (e == hipSuccess) e = hipMemcpy(s->table, prm->quat_table, sizeof(float) * 4 * s->B, hipMemcpyHostToDevice);
+  if (e != hipSuccess) {
+    free_arena(s);
+    delete s;
+    return hip_fail(e, "fss_screen_create: allocation");
+  }
+  {
+    std::lock_guard<std::mutex> lk(lib->mu);
+    lib->services += 1;
+  }
+  s->worker = std::thread(worker_main, s);
+  *out = s;
+  return FSS_OK;
+}
+
+int fss_screen_submit(FssScreen* s, const FssQuery* q, int64_t* ticket) {
+  if (!s || !q || !ticket) return set_error(FSS_E_INVALID, "NULL submit arguments");
+  if (s->need_shape) {
+    if (q->n_points < 1 || !q->pts) return set_error(FSS_E_INVALID, "the query needs shape rows");
+    if (q->n_points > s->prm.max_query_points)
+      return set_error(FSS_E_TOO_LARGE, "query has %d rows, the service takes %d", q->n_points, s->prm.max_query_points);
+  }
+  if (s->need_pharm) {
+    if (q->n_pharm < 1 || !q->anc || !q->vec || !q->typ) return set_error(FSS_E_INVALID, "the query needs pharmacophores");
+    if (q->n_pharm > s->prm.max_query_pharm)
+      return set_error(FSS_E_TOO_LARGE, "query has %d pharmacophores, the service takes %d", q->n_pharm, s->prm.max_query_pharm);
+  }
+  QueryIn qi;
+  qi.t_submit = clk::now();
+  if (s->need_shape) {
+    qi.np = q->n_points;
+    qi.pts.assign(q->pts, q->pts + 4 * (size_t)q->n_points);
+  }
+  if (s->need_pharm) {
+    qi.nph = q->n_pharm;
+    qi.anc.assign(q->anc, q->anc + 4 * (size_t)q->n_pharm);
+    qi.vec.assign(q->vec, q->vec + 4 * (size_t)q->n_pharm);
+    qi.typ.assign(q->typ, q->typ + (size_t)q->n_pharm);
+  }
+  {
+    std::lock_guard<std::mutex> lk(s->mu);
+    if (s->stop) return set_error(FSS_E_CLOSED, "the screening service is closing");
+    qi.ticket = s->next_ticket++;
+    *ticket = qi.ticket;
+    s->inflight.insert(qi.ticket);
+    s->pending.push_back(std::move(qi));
+  }
+  s->cv_work.notify_one();
+  return FSS_OK;
+}
+
+int fss_screen_poll(FssScreen* s, int64_t ticket, int32_t wait_ms, FssHits* out) {
+  if (!s) return set_error(FSS_E_INVALID, "service is NULL");
+  Result r;
+  {
+    std::unique_lock<std::mutex> lk(s->mu);
+    auto ready = [&] { return s->results.count(ticket) > 0; };
+    if (!ready()) {
+      if (!s->inflight.count(ticket)) return set_error(FSS_E_INVALID, "unknown ticket %lld", (long long)ticket);
+      if (wait_ms < 0) s->cv_done.wait(lk, ready);
+      else if (wait_ms > 0) s->cv_done.wait_for(lk, std::chrono::milliseconds(wait_ms), ready);
+      if (!ready()) return FSS_PENDING;
+    }
+    auto it = s->results.find(ticket);
+    r = std::move(it->second);
+    s->results.erase(it);
+  }
+  if (r.rc) return set_error(r.rc, "%s", r.err.c_str());
+  if (!out) return FSS_OK;
+  const int n = std::min<int>(out->capacity, (int)r.score.size());
+  if (n > 0 && (!out->score || !out->index || !out->transform))
+    return set_error(FSS_E_INVALID, "NULL hit arrays");
+  for (int i = 0; i < n; ++i) {
+    out->score[i] = r.score[i];
+    out->index[i] = r.index[i];
+  }
+  if (n > 0) memcpy(out->transform, r.tf.data(), sizeof(float) * 16 * n);
+  out->n = n;
+  out->queue_ms = r.queue_ms;
+  out->latency_ms = r.latency_ms;
+  return FSS_OK;
+}
+
+int fss_screen_stats(FssScreen* s, int64_t* launches, int64_t* pairs) {
+  if (!s) return set_error(FSS_E_INVALID, "service is NULL");
+  if (launches) *launches = s->launches.load();
+  if (pairs) *pairs = s->pairs.load();
+  return FSS_OK;
+}
+
+int fss_screen_destroy(FssScreen* s) {
+  if (!s) return FSS_OK;
+  {
+    std::lock_guard<std::mutex> lk(s->mu);
+    s->stop = true;
+  }
+  s->cv_work.notify_all();
+  if (s->worker.joinable()) s->worker.join();
+  {
+    DeviceGuard dg(s->lib->device);
+    (void)hipStreamSynchronize(s->stream);
+    free_arena(s);
+  }
+  {
+    std::lock_guard<std::mutex> lk(s->lib->mu);
+    s->lib->services -= 1;
+  }
+  delete s;
+  return FSS_OK;
+}
+
+}  // extern "C"

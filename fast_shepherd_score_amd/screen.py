@@ -3,35 +3,35 @@
 Front-end restating the reference's screening contract (shepherd_score/screen.py:74, 1038-1192):
 ``screen(query, library, mode, top_k, **align_kwargs) -> [Hit(score, id, transform)]`` sorted by
 score, descending.  The reference streams ``ProfileStore`` shards from disk through per-query
-aligner calls; here the library is packed once into HBM and a :class:`ScreenService` serves a
-stream of queries:
+aligner calls; here the library is packed once into HBM and served by the native screening
+service of libfss_hip.so (``fss_lib_*`` / ``fss_screen_*`` in include/fss.h), which this module
+wraps:
 
-* admission: a request is admitted when an active slot is free and its device workspace fits
-  the HBM budget (:class:`Admission`); otherwise it waits in FIFO order;
-* continuous batching: every launch is filled from *all* active requests (round-robin slices of
-  their remaining library range, :func:`plan_launch`), so a request that arrives while others are
-  on the GPU joins the very next launch and its pairs run side by side with theirs;
-* per-pair results do not depend on the co-scheduled pairs: the tiling is fixed per service, the
-  early stop is per pair, seeds are per pair (device kernel ``fss_seed_poses``), and every
-  reduction has a fixed order -- so a query's hits are bit-identical alone and in the mix.
+* admission: a query takes one of ``max_active`` slots of a device arena reserved when the
+  service is created (sized from the HBM budget); the rest wait in FIFO order;
+* continuous batching: every launch is filled from *all* admitted queries (water-filling over
+  their remaining library ranges, :func:`plan_launch` / ``fss_plan_launch``), so a query that
+  arrives while others are on the GPU joins the very next launch;
+* per-pair results do not depend on the co-scheduled pairs (fixed tiling, per-pair seeds and
+  early stop, fixed-order reductions) and the device-side top-k merge orders by (score
+  descending, library index ascending), so a query's hits are identical alone and in the mix.
 """
 from __future__ import annotations
 
 import collections
-import queue
+import ctypes
 import threading
-import time
 from concurrent.futures import Future
-from dataclasses import dataclass, field
+from dataclasses import dataclass
 from typing import Dict, List, Optional, Sequence
 
 import numpy as np
 import torch
 
 from . import _lib
-from .align import OptimSpec, ScoreSpec, align, self_overlaps
+from .align import OptimSpec, ScoreSpec
 from .constants import LAM_SCALING
-from .packing import CloudSet, PairBatch, PharmSet
+from .packing import CloudSet, PharmSet
 from .seeds import quats_from_fibo
 
 Hit = collections.namedtuple("Hit", ["score", "id", "transform"])
@@ -60,6 +60,51 @@ class ConformerLibrary:
         self.n = n
         self.ids = np.arange(n, dtype=np.int64) if ids is None else np.asarray(ids, np.int64)
 
+    def native(self) -> ctypes.c_void_p:
+        """The library's copy in a native FssLibrary (created and uploaded on first use)."""
+        if getattr(self, "_native", None) is None:
+            lib = _lib.load()
+            spec = _lib.FssLibSpec()
+            dev = (self.shape.pts if self.shape is not None else self.pharm.anc).device
+            spec.device = dev.index if dev.index is not None else torch.cuda.current_device()
+            spec.channels = (_lib.LIB_SHAPE if self.shape is not None else 0) | \
+                (_lib.LIB_PHARM if self.pharm is not None else 0)
+            spec.max_conformers = max(self.n, 1)
+            spec.max_points = max(int(self.shape.counts.sum()), 1) if self.shape is not None else 0
+            spec.max_pharm = max(int(self.pharm.counts.sum()), 1) if self.pharm is not None else 0
+            h = ctypes.c_void_p()
+            _lib.call("fss_lib_create", _lib.byref(spec), ctypes.byref(h))
+            cnt = pcnt = None
+            args = [None] * 6
+            if self.shape is not None:
+                cnt = np.ascontiguousarray(self.shape.counts, np.int32)
+                args[0], args[1] = _lib.ptr(self.shape.pts), cnt.ctypes.data
+            if self.pharm is not None:
+                pcnt = np.ascontiguousarray(self.pharm.counts, np.int32)
+                args[2:] = [_lib.ptr(self.pharm.anc), _lib.ptr(self.pharm.vec),
+                            _lib.ptr(self.pharm.typ), pcnt.ctypes.data]
+            torch.cuda.synchronize(dev)
+            first = ctypes.c_int64()
+            try:
+                _lib.call("fss_lib_upload", h, self.n, *args, ctypes.byref(first))
+            except Exception:
+                lib.fss_lib_destroy(h)
+                raise
+            self._native = h
+        return self._native
+
+    def close(self) -> None:
+        """Release the native copy (no screening service may still use it)."""
+        if getattr(self, "_native", None) is not None:
+            _lib.call("fss_lib_destroy", self._native)
+            self._native = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
     @classmethod
     def from_arrays(cls, points=None, esp=None, pharm_types=None, pharm_anchors=None,
                     pharm_vectors=None, ids=None, device="cuda") -> "ConformerLibrary":
@@ -79,46 +124,10 @@ class ConformerLibrary:
         return tot
 
 
-class Admission:
-    """HBM budget accounting for active requests (pure host logic)."""
-
-    def __init__(self, budget_bytes: int, max_active: int):
-        self.budget = int(budget_bytes)
-        self.max_active = int(max_active)
-        self.used = 0
-        self.active = 0
-
-    def try_admit(self, nbytes: int) -> bool:
-        if self.active >= self.max_active or self.used + nbytes > self.budget:
-            return False
-        self.used += nbytes
-        self.active += 1
-        return True
-
-    def release(self, nbytes: int) -> None:
-        self.used -= nbytes
-        self.active -= 1
-
-
-@dataclass
-class _Request:
-    rid: int
-    query: dict
-    top_k: int
-    t_submit: float
-    future: Future
-    nbytes: int = 0
-    next: int = 0                    # next library index to schedule
-    done: int = 0                    # library entries finished
-    best_s: Optional[torch.Tensor] = None
-    best_i: Optional[torch.Tensor] = None
-    best_tf: Optional[torch.Tensor] = None
-    t_start: float = 0.0
-
-
 def plan_launch(remaining: Sequence[int], capacity: int) -> List[int]:
     """Split ``capacity`` library entries over requests with ``remaining`` entries each:
-    water-filling, so every active request advances in every launch (continuous batching)."""
+    water-filling, so every active request advances in every launch (continuous batching).
+    Host restatement of the native planner ``fss_plan_launch``."""
     take = [0] * len(remaining)
     left = capacity
     live = [i for i, r in enumerate(remaining) if r > 0]
@@ -137,6 +146,15 @@ def plan_launch(remaining: Sequence[int], capacity: int) -> List[int]:
     return take
 
 
+def arena_bytes(launch_pairs: int, restarts: int, max_active: int, top_k: int,
+                max_query_points: int, max_query_pharm: int) -> int:
+    """Device bytes a service reserves at creation (fss_screen_create's arena)."""
+    P, B, A, K = launch_pairs, restarts, max_active, top_k
+    per_pair = B * (4 + 3) * 4 + B * 8 + 8 + 4 + 64 + 4 + 32 + 32
+    per_slot = max_query_points * 16 + max_query_pharm * 36 + 2 * K * (8 + 8 + 64) + 64
+    return P * per_pair + A * per_slot + B * 16
+
+
 @dataclass
 class ScreenResult:
     hits: List[Hit]
@@ -146,56 +164,121 @@ class ScreenResult:
 
 
 class ScreenService:
-    """Serve screening queries against a resident library with continuous batching."""
+    """Serve screening queries against a resident library with continuous batching (a thin
+    host wrapper over the native ``fss_screen_*`` service)."""
 
     def __init__(self, library: ConformerLibrary, mode: str = "vol", *, num_repeats: int = 10,
                  max_num_steps: int = 100, lr: float = 0.1, top_k: int = 100,
                  launch_pairs: int = 65536, hbm_budget_bytes: int = 8 << 30, max_active: int = 16,
                  early_stop: bool = True, alpha: float = 0.81, lam: Optional[float] = None,
                  similarity: str = "tanimoto", directionless: bool = True,
-                 lanes_per_pose: int = 4, device=None):
+                 lanes_per_pose: int = 4, max_query_points: int = 4096,
+                 max_query_pharm: int = 256, device=None):
         if mode not in _KIND:
             raise ValueError(f"unsupported screening mode {mode!r}")
+        if not 1 <= int(top_k) <= _lib.SCREEN_MAX_TOPK:
+            raise ValueError(f"top_k must be in [1, {_lib.SCREEN_MAX_TOPK}]")
         self.lib, self.mode = library, mode
-        self.dev = torch.device(device) if device is not None else (
-            library.shape.pts.device if library.shape is not None else library.pharm.anc.device)
         if lam is None:
             lam = 0.3 * LAM_SCALING if mode == "surf_esp" else 0.1
         self.spec = ScoreSpec(mode=_KIND[mode], alpha=alpha, lam=lam, similarity=similarity,
                               directionless=directionless)
         self.B = int(num_repeats)
         self.opt = OptimSpec(max_steps=int(max_num_steps), lr=float(lr), early_stop=bool(early_stop))
-        self.tiling = dict(lanes_per_pose=int(lanes_per_pose), restarts_per_wg=self.B)
         self.top_k = int(top_k)
         self.launch_pairs = int(launch_pairs)
-        self.admission = Admission(hbm_budget_bytes, max_active)
-        self.table = torch.from_numpy(restart_table(self.B)).to(self.dev)
-        self.stream = torch.cuda.Stream(device=self.dev)
-        self._inbox: "queue.Queue[_Request]" = queue.Queue()
-        self._waiting: collections.deque = collections.deque()
-        self._active: List[_Request] = []
-        self._next_id = 0
-        self._stop = threading.Event()
-        self._lock = threading.Lock()
-        self.launches = 0
-        self._thread = threading.Thread(target=self._loop, name="fss-screen", daemon=True)
-        self._thread.start()
+        shape = self.spec.mode != _lib.MODE_PHARM
+        pharm = self.spec.mode in (_lib.MODE_PHARM, _lib.MODE_SHAPE_PHARM)
+        mqp, mqh = (int(max_query_points) if shape else 0), (int(max_query_pharm) if pharm else 0)
+        # admission: as many query slots as the HBM budget holds, at most max_active
+        fixed = arena_bytes(self.launch_pairs, self.B, 0, self.top_k, mqp, mqh)
+        slot = arena_bytes(0, 0, 1, self.top_k, mqp, mqh)
+        if fixed + slot > hbm_budget_bytes:
+            raise MemoryError("the HBM budget does not hold one query slot and one launch")
+        self.max_active = max(1, min(int(max_active), (hbm_budget_bytes - fixed) // slot,
+                                     self.launch_pairs))
+        self.table = restart_table(self.B)
+        prm = _lib.FssScreenParams()
+        prm.n_restarts, prm.max_steps = self.B, self.opt.max_steps
+        prm.lr, prm.beta1, prm.beta2, prm.eps = self.opt.lr, self.opt.beta1, self.opt.beta2, self.opt.eps
+        prm.early_stop, prm.es_tol = int(self.opt.early_stop), self.opt.es_tol
+        prm.es_patience = int(self.opt.es_patience)
+        prm.top_k, prm.launch_pairs, prm.max_active = self.top_k, self.launch_pairs, self.max_active
+        prm.max_query_points, prm.max_query_pharm = mqp, mqh
+        prm.lanes_per_pose, prm.restarts_per_wg, prm.variant = int(lanes_per_pose), self.B, 0
+        prm.seed_flags = _lib.SEED_PHARM if self.spec.mode == _lib.MODE_PHARM else 0
+        prm.quat_table = self.table.ctypes.data
+        self._sp = self.spec.struct()
+        self._h = ctypes.c_void_p()
+        _lib.call("fss_screen_create", library.native(), _lib.byref(self._sp), _lib.byref(prm),
+                  ctypes.byref(self._h))
+        self._futures: Dict[int, Future] = {}
+        self._submitted: Dict[int, int] = {}
+        self._cv = threading.Condition()
+        self._stop = False
+        self._poller = threading.Thread(target=self._poll_loop, name="fss-screen-poll", daemon=True)
+        self._poller.start()
 
     # -- public ------------------------------------------------------------------------------
     def submit(self, query: dict, top_k: Optional[int] = None) -> Future:
         """Queue a query (dict with the arrays its mode needs: ``points`` [+ ``esp``] and/or
         ``pharm_types``, ``pharm_anchors``, ``pharm_vectors``).  Resolves to a ScreenResult."""
+        k = int(top_k or self.top_k)
+        if not 1 <= k <= self.top_k:
+            raise ValueError(f"top_k must be in [1, {self.top_k}] for this service")
+        q = _lib.FssQuery()
+        keep = []
+        if self.spec.mode != _lib.MODE_PHARM:
+            xyz = np.asarray(query["points"], np.float32).reshape(-1, 3)
+            rows = np.zeros((len(xyz), 4), np.float32)
+            rows[:, :3] = xyz
+            if self.spec.mode == _lib.MODE_SHAPE_ESP:
+                rows[:, 3] = np.asarray(query["esp"], np.float32).reshape(-1)
+            q.n_points, q.pts = len(rows), rows.ctypes.data
+            keep.append(rows)
+        if self.spec.mode in (_lib.MODE_PHARM, _lib.MODE_SHAPE_PHARM):
+            typ = np.ascontiguousarray(query["pharm_types"], np.int32).reshape(-1)
+            anc = np.zeros((len(typ), 4), np.float32)
+            vec = np.zeros((len(typ), 4), np.float32)
+            anc[:, :3] = np.asarray(query["pharm_anchors"], np.float32).reshape(-1, 3)
+            vec[:, :3] = np.asarray(query["pharm_vectors"], np.float32).reshape(-1, 3)
+            q.n_pharm, q.anc, q.vec, q.typ = len(typ), anc.ctypes.data, vec.ctypes.data, typ.ctypes.data
+            keep += [typ, anc, vec]
         fut: Future = Future()
-        with self._lock:
-            rid = self._next_id
-            self._next_id += 1
-        self._inbox.put(_Request(rid, query, int(top_k or self.top_k), time.perf_counter(), fut))
+        ticket = ctypes.c_int64()
+        with self._cv:
+            if self._stop:
+                raise RuntimeError("the screening service is closed")
+            _lib.call("fss_screen_submit", self._h, ctypes.byref(q), ctypes.byref(ticket))
+            self._futures[ticket.value] = fut
+            self._submitted[ticket.value] = k
+            self._cv.notify_all()
         return fut
 
+    @property
+    def launches(self) -> int:
+        """Launches run so far (the final count once closed)."""
+        with self._cv:
+            if self._stop:
+                return self._final_launches
+            n = ctypes.c_int64()
+            _lib.call("fss_screen_stats", self._h, ctypes.byref(n), None)
+            return int(n.value)
+
     def close(self) -> None:
-        self._stop.set()
-        self._inbox.put(None)
-        self._thread.join(timeout=60)
+        with self._cv:
+            if self._stop:
+                return
+            n = ctypes.c_int64()
+            _lib.call("fss_screen_stats", self._h, ctypes.byref(n), None)
+            self._final_launches = int(n.value)
+            self._stop = True
+            self._cv.notify_all()
+        self._poller.join()
+        _lib.call("fss_screen_destroy", self._h)
+        for fut in self._futures.values():
+            if not fut.done():
+                fut.set_exception(RuntimeError("the screening service was closed"))
 
     def __enter__(self):
         return self
@@ -203,117 +286,38 @@ class ScreenService:
     def __exit__(self, *exc):
         self.close()
 
-    # -- scheduler ----------------------------------------------------------------------------
-    def _workspace_bytes(self, req: _Request) -> int:
-        k = min(req.top_k, self.lib.n)
-        npts = sum(len(np.asarray(v)) for key, v in req.query.items() if key in ("points", "pharm_anchors"))
-        return 64 * npts + k * (8 + 8 + 64) + 4096
-
-    def _drain_inbox(self, block: bool) -> None:
-        try:
-            item = self._inbox.get(timeout=0.05) if block else self._inbox.get_nowait()
-            while True:
-                if item is not None:
-                    self._waiting.append(item)
-                item = self._inbox.get_nowait()
-        except queue.Empty:
-            pass
-
-    def _admit(self) -> None:
-        while self._waiting:
-            req = self._waiting[0]
-            req.nbytes = self._workspace_bytes(req)
-            if req.nbytes > self.admission.budget:
-                self._waiting.popleft()
-                req.future.set_exception(MemoryError("query workspace exceeds the HBM budget"))
-                continue
-            if not self.admission.try_admit(req.nbytes):
-                break
-            self._waiting.popleft()
-            req.t_start = time.perf_counter()
-            self._active.append(req)
-
-    def _loop(self) -> None:
-        torch.cuda.set_device(self.dev)
-        while not self._stop.is_set():
-            self._drain_inbox(block=not self._active)
-            self._admit()
-            if not self._active:
-                continue
-            try:
-                self._launch()
-            except Exception as exc:   # fail the requests of this launch, keep serving
-                for r in self._active:
-                    if not r.future.done():
-                        r.future.set_exception(exc)
-                    self.admission.release(r.nbytes)
-                self._active = []
-
-    def _query_sets(self, reqs: Sequence[_Request]):
-        qs = [r.query for r in reqs]
-        shape = pharm = None
-        if self.lib.shape is not None:
-            shape = CloudSet.from_arrays([q["points"] for q in qs],
-                                         [q.get("esp") for q in qs] if self.spec.mode == _lib.MODE_SHAPE_ESP else None,
-                                         device=self.dev)
-        if self.lib.pharm is not None:
-            pharm = PharmSet.from_arrays([q["pharm_types"] for q in qs], [q["pharm_anchors"] for q in qs],
-                                         [q["pharm_vectors"] for q in qs], device=self.dev)
-        return shape, pharm
-
-    def _launch(self) -> None:
-        reqs = list(self._active)
-        take = plan_launch([self.lib.n - r.next for r in reqs], self.launch_pairs)
-        reqs = [r for r, k in zip(reqs, take) if k > 0]
-        take = [k for k in take if k > 0]
-        ref_idx = np.concatenate([np.full(k, i, np.int64) for i, k in enumerate(take)])
-        fit_idx = np.concatenate([np.arange(r.next, r.next + k) for r, k in zip(reqs, take)])
-        with torch.cuda.stream(self.stream):
-            qshape, qpharm = self._query_sets(reqs)
-            pb = PairBatch(qshape, self.lib.shape, qpharm, self.lib.pharm, ref_idx=ref_idx,
-                           fit_idx=fit_idx)
-            P = pb.n_pairs
-            q0 = torch.empty((P, self.B, 4), dtype=torch.float32, device=self.dev)
-            t0 = torch.empty((P, self.B, 3), dtype=torch.float32, device=self.dev)
-            bs = pb.struct()
-            _lib.call("fss_seed_poses", _lib.byref(bs), int(self.spec.mode == _lib.MODE_PHARM), self.B,
-                      _lib.ptr(self.table), _lib.ptr(q0), _lib.ptr(t0), int(self.stream.cuda_stream))
-            so = self_overlaps(pb, self.spec, stream=self.stream)
-            res = align(pb, self.spec, q0, t0, self.opt, order="none", self_ov=so,
-                        stream=self.stream, **self.tiling)
-            off = 0
-            for r, k in zip(reqs, take):
-                s = res.score[off:off + k]
-                idx = torch.arange(r.next, r.next + k, device=self.dev)
-                tf = res.transform[off:off + k]
-                if r.best_s is not None:
-                    s = torch.cat([r.best_s, s])
-                    idx = torch.cat([r.best_i, idx])
-                    tf = torch.cat([r.best_tf, tf])
-                kk = min(r.top_k, s.shape[0])
-                top = torch.topk(s, kk, sorted=True)
-                r.best_s, r.best_i, r.best_tf = top.values, idx[top.indices], tf[top.indices]
-                off += k
-            done_evt = torch.cuda.Event()
-            done_evt.record(self.stream)
-        done_evt.synchronize()
-        self.launches += 1
-        now = time.perf_counter()
-        finished = []
-        for r, k in zip(reqs, take):
-            r.next += k
-            r.done += k
-            if r.done >= self.lib.n:
-                finished.append(r)
-        for r in finished:
-            sc = r.best_s.cpu().numpy()
-            ii = r.best_i.cpu().numpy()
-            tf = r.best_tf.cpu().numpy()
-            hits = [Hit(float(sc[i]), int(self.lib.ids[ii[i]]), tf[i]) for i in range(len(sc))]
-            r.future.set_result(ScreenResult(hits, (now - r.t_submit) * 1e3,
-                                             (r.t_start - r.t_submit) * 1e3, self.lib.n))
-            self._active.remove(r)
-            self.admission.release(r.nbytes)
+    # -- result delivery ----------------------------------------------------------------------
+    def _poll_loop(self) -> None:
+        lib = _lib.load()
+        K = self.top_k
+        score = np.zeros(K, np.float64)
+        index = np.zeros(K, np.int64)
+        tf = np.zeros((K, 16), np.float32)
+        hits = _lib.FssHits()
+        hits.capacity = K
+        hits.score, hits.index, hits.transform = score.ctypes.data, index.ctypes.data, tf.ctypes.data
+        while True:
+            with self._cv:
+                while not self._futures and not self._stop:
+                    self._cv.wait()
+                if self._stop:
+                    return
+                tickets = sorted(self._futures)
+            # wait (in native code, GIL released) up to 5 ms on the oldest ticket, then sweep
+            for i, t in enumerate(tickets):
+                rc = lib.fss_screen_poll(self._h, t, 5 if i == 0 else 0, ctypes.byref(hits))
+                if rc == _lib.PENDING:
+                    continue
+                with self._cv:
+                    fut = self._futures.pop(t)
+                    k = self._submitted.pop(t)
+                if rc != 0:
+                    fut.set_exception(_lib.FssError("fss_screen_poll", rc, lib.fss_last_error().decode()))
+                    continue
+                n = min(int(hits.n), k)
+                res = [Hit(float(score[i]), int(self.lib.ids[index[i]]), tf[i].reshape(4, 4).copy())
+                       for i in range(n)]
+                fut.set_result(ScreenResult(res, float(hits.latency_ms), float(hits.queue_ms), self.lib.n))
 
 
 def screen(query: dict, library: ConformerLibrary, mode: str = "vol", *, top_k: int = 1000,

@@ -46,6 +46,8 @@ extern "C" {
 #define FSS_E_INVALID    9001  /* bad argument (null pointer, size, mode)                   */
 #define FSS_E_TOO_LARGE  9002  /* clouds do not fit the kernel's LDS staging               */
 #define FSS_E_NO_DEVICE  9003
+#define FSS_E_CLOSED     9004  /* the screening service was destroyed before the query finished */
+#define FSS_PENDING      1     /* fss_screen_poll: the query has not finished yet            */
 
 #define FSS_N_PTYPES 9         /* Acceptor Donor Aromatic Hydrophobe Halogen Cation Anion ZnBinder Dummy */
 
@@ -237,6 +239,101 @@ int fss_score_fwd_f64(const FssPairBatch* batch, const FssScoreParams* sp, const
 int fss_score_bwd_f64(const FssPairBatch* batch, const FssScoreParams* sp, const double* ref_pts,
                       const double* fit_pts, const double* ov, const double* gS, double* d_fit,
                       double* d_ref, void* stream);
+
+/* ----------------------------------------------------------------------------------------------
+ * Screening service: a conformer library resident in HBM, and a service that aligns queued
+ * queries against all of it with continuous batching and keeps each query's top-k hits.
+ *
+ * The service owns one worker thread and one HIP stream (default priority) on the library's
+ * device.  Every launch takes up to launch_pairs (query, library entry) pairs from all admitted
+ * queries at once (water-filling: every admitted query advances in every launch), runs
+ * fss_seed_poses, fss_self_overlap and fss_align on them, and merges each query's new pair
+ * scores into its device-resident top-k (ties to the lower library index).  A query's hits do not
+ * depend on the queries it shares launches with.  Device memory is allocated by fss_lib_create /
+ * fss_screen_create only; submit and poll allocate none.  Calls on one service may come from any
+ * thread.
+ * Replaces: screen / screen_many shepherd_score/screen.py:1038-1192 (shard streaming, per-query
+ *   aligner calls, _TopK :898-933, Hit :74); the library replaces the streamed ProfileStore
+ *   shards (screen.py:317-605) with one HBM-resident copy.
+ * -------------------------------------------------------------------------------------------- */
+typedef struct FssLibrary FssLibrary;
+typedef struct FssScreen FssScreen;
+
+#define FSS_LIB_SHAPE 1        /* rows (x, y, z, esp) per conformer */
+#define FSS_LIB_PHARM 2        /* anchors, unit vectors, types per conformer */
+
+typedef struct FssLibSpec {
+  int32_t device;              /* HIP device ordinal */
+  int32_t channels;            /* FSS_LIB_SHAPE | FSS_LIB_PHARM */
+  int64_t max_conformers;
+  int64_t max_points;          /* total shape rows */
+  int64_t max_pharm;           /* total pharmacophore rows */
+} FssLibSpec;
+
+/* Allocates the library's device arrays for the given capacities. */
+int fss_lib_create(const FssLibSpec* spec, FssLibrary** out);
+/* Appends n conformers (host or device pointers; copied before return).  Shape channel: pts
+ * (sum(counts), 4) rows, counts (n).  Pharm channel: anc, vec (sum(pcounts), 4), typ (sum(pcounts)),
+ * pcounts (n).  *first receives the library index of the first appended conformer. */
+int fss_lib_upload(FssLibrary* lib, int32_t n, const float* pts, const int32_t* counts,
+                   const float* anc, const float* vec, const int32_t* typ, const int32_t* pcounts,
+                   int64_t* first);
+int fss_lib_size(const FssLibrary* lib, int64_t* n);
+/* Fails while a service on this library exists. */
+int fss_lib_destroy(FssLibrary* lib);
+
+#define FSS_SCREEN_MAX_TOPK 1024
+
+typedef struct FssScreenParams {
+  int32_t n_restarts;          /* restarts per pair (seeds: identity, then quat_table rows) */
+  int32_t max_steps;
+  float lr, beta1, beta2, eps;
+  int32_t early_stop;          /* per-pair early stop of fss_align */
+  float es_tol;
+  int32_t es_patience;
+  int32_t top_k;               /* 1 .. FSS_SCREEN_MAX_TOPK */
+  int32_t launch_pairs;        /* pairs per launch */
+  int32_t max_active;          /* queries admitted at once (the rest wait in FIFO order) */
+  int32_t max_query_points;    /* shape rows of one query */
+  int32_t max_query_pharm;     /* pharmacophore rows of one query */
+  int32_t lanes_per_pose, restarts_per_wg, variant;   /* fss_align tiling; 0 = defaults */
+  int32_t seed_flags;          /* FSS_SEED_PHARM: seed from the pharmacophore anchors */
+  const float* quat_table;     /* (n_restarts, 4) host rotations of the restarts */
+} FssScreenParams;
+
+typedef struct FssQuery {      /* host arrays, copied by fss_screen_submit */
+  int32_t n_points;
+  const float* pts;            /* (n_points, 4): x, y, z, esp */
+  int32_t n_pharm;
+  const float* anc;            /* (n_pharm, 4) */
+  const float* vec;            /* (n_pharm, 4) */
+  const int32_t* typ;          /* (n_pharm) */
+} FssQuery;
+
+typedef struct FssHits {       /* host arrays of capacity entries, filled by fss_screen_poll */
+  int32_t capacity;
+  int32_t n;                   /* out: hits written, best first */
+  double* score;
+  int64_t* index;              /* library index */
+  float* transform;            /* (capacity, 16): row-major 4x4 moving the library entry */
+  double queue_ms;             /* out: submit -> admission */
+  double latency_ms;           /* out: submit -> result */
+} FssHits;
+
+int fss_screen_create(FssLibrary* lib, const FssScoreParams* sp, const FssScreenParams* prm,
+                      FssScreen** out);
+/* Queues a query against the whole library (its size at admission); *ticket names it. */
+int fss_screen_submit(FssScreen* svc, const FssQuery* q, int64_t* ticket);
+/* Waits up to wait_ms (< 0: no limit) for the ticket's hits.  FSS_OK: hits written and the
+ * ticket released; FSS_PENDING: not finished yet; an error code: the query failed. */
+int fss_screen_poll(FssScreen* svc, int64_t ticket, int32_t wait_ms, FssHits* out);
+int fss_screen_stats(FssScreen* svc, int64_t* launches, int64_t* pairs);
+/* Stops the worker; queries still queued or running fail with FSS_E_CLOSED.  No other call on
+ * the service may be in progress or follow. */
+int fss_screen_destroy(FssScreen* svc);
+/* The service's launch planner (host only): splits capacity over n requests with remaining[i]
+ * entries left each, water-filling so that every request with entries left advances. */
+int fss_plan_launch(int32_t n, const int64_t* remaining, int64_t capacity, int64_t* take);
 
 #ifdef __cplusplus
 }

@@ -39,6 +39,9 @@ int main(void) {
   printf("%zu %zu %zu %zu\n", sizeof(FssPairBatch), sizeof(FssScoreParams), sizeof(FssAlignParams), sizeof(FssAlignOut));
   printf("%zu %zu %zu %zu\n", offsetof(FssPairBatch, max_ref), offsetof(FssScoreParams, ph_cat),
          offsetof(FssAlignParams, pair_order), offsetof(FssAlignOut, n_steps));
+  printf("%zu %zu %zu %zu\n", sizeof(FssLibSpec), sizeof(FssScreenParams), sizeof(FssQuery), sizeof(FssHits));
+  printf("%zu %zu %zu %zu\n", offsetof(FssLibSpec, max_pharm), offsetof(FssScreenParams, quat_table),
+         offsetof(FssQuery, typ), offsetof(FssHits, latency_ms));
   return 0;
 }
 '''
@@ -51,8 +54,12 @@ int main(void) {
     sizes = [int(x) for x in out]
     assert sizes[:4] == [ctypes.sizeof(_lib.FssPairBatch), ctypes.sizeof(_lib.FssScoreParams),
                          ctypes.sizeof(_lib.FssAlignParams), ctypes.sizeof(_lib.FssAlignOut)]
-    assert sizes[4:] == [_lib.FssPairBatch.max_ref.offset, _lib.FssScoreParams.ph_cat.offset,
-                         _lib.FssAlignParams.pair_order.offset, _lib.FssAlignOut.n_steps.offset]
+    assert sizes[4:8] == [_lib.FssPairBatch.max_ref.offset, _lib.FssScoreParams.ph_cat.offset,
+                          _lib.FssAlignParams.pair_order.offset, _lib.FssAlignOut.n_steps.offset]
+    assert sizes[8:12] == [ctypes.sizeof(_lib.FssLibSpec), ctypes.sizeof(_lib.FssScreenParams),
+                           ctypes.sizeof(_lib.FssQuery), ctypes.sizeof(_lib.FssHits)]
+    assert sizes[12:] == [_lib.FssLibSpec.max_pharm.offset, _lib.FssScreenParams.quat_table.offset,
+                          _lib.FssQuery.typ.offset, _lib.FssHits.latency_ms.offset]
 
 
 def test_validation_rejects_bad_arguments_without_launching():
@@ -79,3 +86,32 @@ def test_validation_rejects_bad_arguments_without_launching():
     assert rc == _lib.E_INVALID and b"early stop" in lib.fss_last_error()
     with pytest.raises(_lib.FssError):
         _lib.call("fss_adam_step", 1, 0, 0, 0, 0, 0, 1, 0.1, 0.9, 0.999, 1e-8, 0, 1, 0)
+
+
+@pytest.mark.parametrize("rem,cap", [([10, 10, 10], 12), ([0, 5, 100], 64), ([3], 1000), ([7, 1], 2),
+                                      ([1000] * 7, 65536), ([5, 0, 0, 9], 3), ([], 5), ([4, 4], 0)])
+def test_native_launch_planner_matches_host_restatement(rem, cap):
+    """fss_plan_launch (the screening worker's planner) == screen.plan_launch, host-only."""
+    from fast_shepherd_score_amd.screen import plan_launch
+    lib = _lib.load()
+    n = len(rem)
+    r = (ctypes.c_int64 * max(n, 1))(*rem)
+    take = (ctypes.c_int64 * max(n, 1))()
+    assert lib.fss_plan_launch(n, r, cap, take) == 0
+    assert list(take)[:n] == plan_launch(rem, cap)
+
+
+def test_screening_abi_rejects_bad_arguments_without_device_work():
+    lib = _lib.load()
+    spec = _lib.FssLibSpec()
+    out = ctypes.c_void_p()
+    spec.channels, spec.max_conformers = 4, 10
+    assert lib.fss_lib_create(_lib.byref(spec), ctypes.byref(out)) == _lib.E_INVALID
+    assert b"channels" in lib.fss_last_error()
+    spec.channels, spec.max_conformers, spec.max_points = _lib.LIB_SHAPE, 0, 10
+    assert lib.fss_lib_create(_lib.byref(spec), ctypes.byref(out)) == _lib.E_INVALID
+    assert lib.fss_screen_create(None, None, None, ctypes.byref(out)) == _lib.E_INVALID
+    t = ctypes.c_int64()
+    assert lib.fss_screen_submit(None, None, ctypes.byref(t)) == _lib.E_INVALID
+    assert lib.fss_screen_poll(None, 1, 0, None) == _lib.E_INVALID
+    assert lib.fss_lib_destroy(None) == 0 and lib.fss_screen_destroy(None) == 0

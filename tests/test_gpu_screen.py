@@ -67,3 +67,77 @@ def test_alone_equals_in_mix(dev):
     assert [h.id for h in a] == [h.id for h in b]
     for r in mixed:
         assert r.latency_ms > 0 and r.pairs == 2000
+
+
+def test_native_service_topk_is_exact(dev):
+    """The native service's hits (uploaded from host arrays through the C ABI, several launches,
+    device top-k merges) are exactly the top-k of aligning every pair directly with the same seeds
+    and tiling, ties (duplicated conformers) broken by the lower library index."""
+    import ctypes
+    from fast_shepherd_score_amd import _lib
+    from fast_shepherd_score_amd.align import OptimSpec, ScoreSpec, align, self_overlaps
+    from fast_shepherd_score_amd.packing import CloudSet, PairBatch
+    print("case native screen topk", flush=True)
+    rng = np.random.default_rng(11)
+    base = synth.random_walks(rng, rng.integers(10, 31, size=700))
+    mols = base + base[:300]                   # entries 700..999 duplicate 0..299: exact ties
+    n, B, steps, K = len(mols), 6, 40, 50
+    counts = np.array([len(m) for m in mols], np.int32)
+    rows = np.zeros((int(counts.sum()), 4), np.float32)
+    rows[:, :3] = np.concatenate(mols).astype(np.float32)
+    lib = _lib.load()
+    spec = _lib.FssLibSpec()
+    spec.device, spec.channels, spec.max_conformers, spec.max_points = 0, _lib.LIB_SHAPE, n, len(rows)
+    h = ctypes.c_void_p()
+    _lib.call("fss_lib_create", _lib.byref(spec), ctypes.byref(h))
+    first = ctypes.c_int64()
+    half = int(counts[:500].sum())             # two uploads from host memory
+    _lib.call("fss_lib_upload", h, 500, rows[:half].ctypes.data, counts[:500].ctypes.data,
+              None, None, None, None, ctypes.byref(first))
+    _lib.call("fss_lib_upload", h, n - 500, rows[half:].ctypes.data, counts[500:].ctypes.data,
+              None, None, None, None, ctypes.byref(first))
+    assert first.value == 500
+    table = restart_table(B)
+    sp = ScoreSpec().struct()
+    prm = _lib.FssScreenParams()
+    prm.n_restarts, prm.max_steps, prm.lr, prm.beta1, prm.beta2, prm.eps = B, steps, 0.1, 0.9, 0.999, 1e-8
+    prm.early_stop, prm.es_tol, prm.es_patience = 1, 1e-5, 10
+    prm.top_k, prm.launch_pairs, prm.max_active, prm.max_query_points = K, 333, 4, 64
+    prm.lanes_per_pose, prm.restarts_per_wg, prm.quat_table = 4, B, table.ctypes.data
+    svc = ctypes.c_void_p()
+    _lib.call("fss_screen_create", h, _lib.byref(sp), _lib.byref(prm), ctypes.byref(svc))
+    qxyz = mols[17]
+    qrows = np.zeros((len(qxyz), 4), np.float32)
+    qrows[:, :3] = qxyz
+    q = _lib.FssQuery()
+    q.n_points, q.pts = len(qrows), qrows.ctypes.data
+    ticket = ctypes.c_int64()
+    _lib.call("fss_screen_submit", svc, ctypes.byref(q), ctypes.byref(ticket))
+    score = np.zeros(K, np.float64)
+    index = np.zeros(K, np.int64)
+    tf = np.zeros((K, 16), np.float32)
+    hits = _lib.FssHits()
+    hits.capacity, hits.score, hits.index, hits.transform = K, score.ctypes.data, index.ctypes.data, tf.ctypes.data
+    _lib.call("fss_screen_poll", svc, ticket.value, -1, ctypes.byref(hits))
+    launches = ctypes.c_int64()
+    _lib.call("fss_screen_stats", svc, ctypes.byref(launches), None)
+    _lib.call("fss_screen_destroy", svc)
+    _lib.call("fss_lib_destroy", h)
+    assert hits.n == K and launches.value == 4          # ceil(1000 / 333)
+    # direct: every pair in one align() call with the same seeds and tiling
+    libset = CloudSet.from_arrays(mols, device=dev)
+    pb = PairBatch(CloudSet.from_arrays([qxyz], device=dev), libset,
+                   ref_idx=np.zeros(n, np.int64), fit_idx=np.arange(n))
+    q0 = torch.empty((n, B, 4), dtype=torch.float32, device=dev)
+    t0 = torch.empty((n, B, 3), dtype=torch.float32, device=dev)
+    bs = pb.struct()
+    _lib.call("fss_seed_poses", _lib.byref(bs), 0, B, _lib.ptr(torch.from_numpy(table).to(dev)),
+              _lib.ptr(q0), _lib.ptr(t0), int(torch.cuda.current_stream().cuda_stream))
+    r = align(pb, ScoreSpec(), q0, t0, OptimSpec(max_steps=steps, lr=0.1, early_stop=True),
+              lanes_per_pose=4, restarts_per_wg=B, order="none", self_ov=self_overlaps(pb, ScoreSpec()))
+    s = r.score.cpu().numpy()
+    order = np.lexsort((np.arange(n), -s))[:K]
+    np.testing.assert_array_equal(index, order)
+    np.testing.assert_array_equal(score, s[order])
+    np.testing.assert_array_equal(tf.reshape(K, 4, 4), r.transform.cpu().numpy()[order])
+    assert any(s[i] == s[i + 700] for i in range(300))   # ties were present

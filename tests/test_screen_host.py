@@ -1,8 +1,8 @@
-"""Host logic of the screening service (no GPU): launch planning and HBM admission."""
+"""Host logic of the screening service (no GPU): launch planning and arena sizing."""
 import numpy as np
 import pytest
 
-from fast_shepherd_score_amd.screen import Admission, plan_launch, restart_table
+from fast_shepherd_score_amd.screen import arena_bytes, plan_launch, restart_table
 from fast_shepherd_score_amd.seeds import quats_from_fibo
 
 
@@ -21,17 +21,13 @@ def test_plan_launch_water_fills(rem, cap):
         assert max(unsat) - min(unsat) <= 1 or len(unsat) == 1
 
 
-def test_admission_defers_when_over_budget():
-    a = Admission(budget_bytes=1000, max_active=3)
-    assert a.try_admit(400) and a.try_admit(400)
-    assert not a.try_admit(400)          # over budget -> deferred
-    a.release(400)
-    assert a.try_admit(400)
-    assert a.try_admit(150)              # 950 of 1000 used, third slot
-    assert not a.try_admit(1)            # slots exhausted
-    assert a.used == 950 and a.active == 3
-    b = Admission(budget_bytes=10 ** 9, max_active=2)
-    assert b.try_admit(1) and b.try_admit(1) and not b.try_admit(1)
+def test_arena_bytes_scale_with_the_service_shape():
+    """Device bytes reserved by a screening service: linear in launch pairs and query slots."""
+    a = arena_bytes(1000, 10, 4, 100, 512, 0)
+    assert arena_bytes(2000, 10, 4, 100, 512, 0) - a == arena_bytes(1000, 10, 0, 100, 512, 0) - arena_bytes(0, 10, 0, 100, 512, 0)
+    assert arena_bytes(1000, 10, 8, 100, 512, 0) - a == 4 * arena_bytes(0, 0, 1, 100, 512, 0)
+    # one slot holds the query rows and two top-k buffers of (score, index, 4x4 transform)
+    assert arena_bytes(0, 0, 1, 100, 512, 0) == 512 * 16 + 2 * 100 * 80 + 64
 
 
 def test_restart_table():
