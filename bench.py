@@ -221,6 +221,31 @@ def bench_screen(dev, n_lib: int = 1_000_000, n_queries: int = 24, load: float =
             "library_bytes": lib.nbytes(), "launches": svc.launches}
 
 
+def bench_store(n: int = 200_000, seed: int = 8):
+    """ProfileStore (float16, 100k-conformer shards) streamed into an HBM library."""
+    import tempfile
+    from fast_shepherd_score_amd import synth
+    from fast_shepherd_score_amd.store import MoleculeProfile, ProfileStore, load_library
+    flat, counts = synth.conformer_library(n, seed=seed)
+    starts = np.concatenate([[0], np.cumsum(counts)[:-1]])
+    with tempfile.TemporaryDirectory() as d:
+        path = os.path.join(d, "lib.fss")
+        t0 = time.perf_counter()
+        with ProfileStore.create(path, num_surf_points=64, modes=("vol",)) as st:
+            for i in range(n):
+                st.add(MoleculeProfile(atom_pos=flat[starts[i]:starts[i] + counts[i]]))
+        t_write = time.perf_counter() - t0
+        disk = sum(os.path.getsize(os.path.join(path, f)) for f in os.listdir(path))
+        t1 = time.perf_counter()
+        lib = load_library(path, "vol")
+        t_load = time.perf_counter() - t1
+        nb = lib.nbytes()
+        lib.close()
+    return {"workload": f"{n} conformers (10-50 atoms), vol store, float16, 100k per shard",
+            "write_s": t_write, "load_s": t_load, "conformers_per_s": n / t_load,
+            "disk_mb": disk / 1e6, "disk_mb_per_s": disk / 1e6 / t_load, "hbm_bytes": nb}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -347,6 +372,8 @@ def main():
         out["screen"] = sc
         out["screen_p50_ms"], out["screen_p99_ms"] = sc["p50_ms"], sc["p99_ms"]
         log("screen done")
+        out["store_load"] = bench_store()
+        log("store done")
     if not args.no_cpu:
         cb, cpu_scores = cpu_baseline(wl, q, t, B, steps, args.cpu_sample, spec, opt)
         cb.pop("seconds")

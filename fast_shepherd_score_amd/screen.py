@@ -20,6 +20,7 @@ from __future__ import annotations
 
 import collections
 import ctypes
+import os
 import threading
 from concurrent.futures import Future
 from dataclasses import dataclass
@@ -58,7 +59,26 @@ class ConformerLibrary:
         self.shape, self.pharm = shape, pharm
         n = shape.n if shape is not None else pharm.n
         self.n = n
-        self.ids = np.arange(n, dtype=np.int64) if ids is None else np.asarray(ids, np.int64)
+        self.ids = np.arange(n, dtype=np.int64) if ids is None else np.asarray(ids)
+        self._native = None
+        self._native_bytes = None
+
+    @classmethod
+    def from_native(cls, handle: ctypes.c_void_p, n: int, ids: np.ndarray,
+                    nbytes: int) -> "ConformerLibrary":
+        """A library whose rows live only in a native FssLibrary (see store.load_library)."""
+        lib = cls.__new__(cls)
+        lib.shape = lib.pharm = None
+        lib.n = int(n)
+        lib.ids = np.asarray(ids)
+        lib._native = handle
+        lib._native_bytes = int(nbytes)
+        return lib
+
+    def id_of(self, index: int):
+        """The id of library entry ``index`` as a Python scalar."""
+        v = self.ids[index]
+        return v.item() if isinstance(v, np.generic) else v
 
     def native(self) -> ctypes.c_void_p:
         """The library's copy in a native FssLibrary (created and uploaded on first use)."""
@@ -114,6 +134,8 @@ class ConformerLibrary:
         return cls(shape, pharm, ids)
 
     def nbytes(self) -> int:
+        if self._native_bytes is not None:
+            return self._native_bytes
         tot = 0
         for cs in (self.shape, self.pharm):
             if cs is None:
@@ -315,21 +337,40 @@ class ScreenService:
                     fut.set_exception(_lib.FssError("fss_screen_poll", rc, lib.fss_last_error().decode()))
                     continue
                 n = min(int(hits.n), k)
-                res = [Hit(float(score[i]), int(self.lib.ids[index[i]]), tf[i].reshape(4, 4).copy())
+                res = [Hit(float(score[i]), self.lib.id_of(int(index[i])), tf[i].reshape(4, 4).copy())
                        for i in range(n)]
                 fut.set_result(ScreenResult(res, float(hits.latency_ms), float(hits.queue_ms), self.lib.n))
 
 
-def screen(query: dict, library: ConformerLibrary, mode: str = "vol", *, top_k: int = 1000,
+def _as_library(library, mode: str, device):
+    """(ConformerLibrary, owned): a ProfileStore or store path is loaded into HBM first."""
+    from .store import ProfileStore, load_library
+    if isinstance(library, (ProfileStore, str, os.PathLike)):
+        return load_library(library, mode, device=device if device is not None else 0), True
+    return library, False
+
+
+def screen(query: dict, library, mode: str = "vol", *, top_k: int = 1000, device=None,
            **kwargs) -> List[Hit]:
-    """One query against the whole library; ``Hit(score, id, transform)`` sorted descending."""
-    with ScreenService(library, mode, top_k=top_k, **kwargs) as svc:
-        return svc.submit(query).result().hits
+    """One query against the whole library (a ConformerLibrary, a ProfileStore or a store
+    path); ``Hit(score, id, transform)`` sorted descending."""
+    lib, owned = _as_library(library, mode, device)
+    try:
+        with ScreenService(lib, mode, top_k=top_k, **kwargs) as svc:
+            return svc.submit(query).result().hits
+    finally:
+        if owned:
+            lib.close()
 
 
-def screen_many(queries: Sequence[dict], library: ConformerLibrary, mode: str = "vol", *,
-                top_k: int = 1000, **kwargs) -> List[List[Hit]]:
+def screen_many(queries: Sequence[dict], library, mode: str = "vol", *, top_k: int = 1000,
+                device=None, **kwargs) -> List[List[Hit]]:
     """A panel of queries served together (continuous batching); one hit list per query."""
-    with ScreenService(library, mode, top_k=top_k, **kwargs) as svc:
-        futs = [svc.submit(q) for q in queries]
-        return [f.result().hits for f in futs]
+    lib, owned = _as_library(library, mode, device)
+    try:
+        with ScreenService(lib, mode, top_k=top_k, **kwargs) as svc:
+            futs = [svc.submit(q) for q in queries]
+            return [f.result().hits for f in futs]
+    finally:
+        if owned:
+            lib.close()

@@ -141,3 +141,49 @@ def test_native_service_topk_is_exact(dev):
     np.testing.assert_array_equal(score, s[order])
     np.testing.assert_array_equal(tf.reshape(K, 4, 4), r.transform.cpu().numpy()[order])
     assert any(s[i] == s[i + 700] for i in range(300))   # ties were present
+
+
+@pytest.mark.parametrize("mode", ["vol", "surf_esp", "pharm"])
+def test_store_library_screens_like_in_memory(dev, tmp_path, mode):
+    """A ProfileStore streamed into HBM by store.load_library screens exactly like an in-memory
+    library built from the same (float16-rounded, pre-centred) arrays; screen() also takes the
+    store path directly."""
+    import os
+    from fast_shepherd_score_amd.store import MoleculeProfile, ProfileStore, load_library
+    print(f"case store screen mode={mode}", flush=True)
+    rng = np.random.default_rng(21)
+    S = 48
+    atoms = synth.random_walks(rng, rng.integers(10, 26, size=260))
+    path = os.path.join(tmp_path, "lib.fss")
+    with ProfileStore.create(path, num_surf_points=S, modes=("vol", "surf_esp", "pharm"),
+                             shard_size=100) as st:
+        for i, a in enumerate(atoms):
+            t, anc, vec = synth.pharmacophores(rng, a, int(rng.integers(3, 9)))
+            st.add(MoleculeProfile(atom_pos=a, surf_pos=synth.surface_points(rng, a, S),
+                                   surf_esp=synth.esp_values(rng, S), pharm_types=t.astype(np.int32),
+                                   pharm_ancs=anc, pharm_vecs=vec, id=f"mol{i}"))
+    store = ProfileStore.open(path)
+    profs = [p for shard in store.iter_profiles() for p in shard]
+    if mode == "vol":
+        mem = ConformerLibrary.from_arrays(points=[p.atom_pos for p in profs], ids=None, device=dev)
+        query = {"points": profs[33].atom_pos}
+    elif mode == "surf_esp":
+        mem = ConformerLibrary.from_arrays(points=[p.surf_pos for p in profs],
+                                           esp=[p.surf_esp for p in profs], device=dev)
+        query = {"points": profs[33].surf_pos, "esp": profs[33].surf_esp}
+    else:
+        mem = ConformerLibrary.from_arrays(pharm_types=[p.pharm_types for p in profs],
+                                           pharm_anchors=[p.pharm_ancs for p in profs],
+                                           pharm_vectors=[p.pharm_vecs for p in profs], device=dev)
+        query = {"pharm_types": profs[33].pharm_types, "pharm_anchors": profs[33].pharm_ancs,
+                 "pharm_vectors": profs[33].pharm_vecs}
+    kw = dict(top_k=15, num_repeats=6, max_num_steps=40, launch_pairs=100)
+    lib = load_library(store, mode)
+    assert lib.n == 260 and lib.nbytes() > 0
+    a = screen(query, lib, mode, **kw)
+    b = screen(query, mem, mode, **kw)
+    c = screen(query, path, mode, **kw)
+    lib.close()
+    assert [h.score for h in a] == [h.score for h in b] == [h.score for h in c]
+    assert [h.id for h in a] == [f"mol{h.id}" for h in b] == [h.id for h in c]
+    assert a[0].id == "mol33"
