@@ -17,11 +17,14 @@ MODE_SHAPE = 0
 MODE_SHAPE_ESP = 1
 MODE_PHARM = 2
 MODE_SHAPE_PHARM = 3
+MODE_ESP_COMBO = 4
 SIM_TANIMOTO = 0
 SIM_TVERSKY = 1
 N_PTYPES = 9
 SEED_PHARM = 1
 SEED_REF = 2
+SEED_SURF = 4
+COULOMB_SCALING = 1e4 / (4 * 55.263 * 3.141592653589793)   # score/constants.py:16
 
 E_INVALID = 9001
 E_TOO_LARGE = 9002
@@ -47,6 +50,13 @@ class FssPairBatch(ctypes.Structure):
         ("fit_typ", ctypes.c_void_p), ("fit_prng", ctypes.c_void_p),
         ("max_ref", ctypes.c_int32), ("max_fit", ctypes.c_int32),
         ("max_ref_ph", ctypes.c_int32), ("max_fit_ph", ctypes.c_int32),
+        ("ref_srf", ctypes.c_void_p), ("fit_srf", ctypes.c_void_p),
+        ("ref_srng", ctypes.c_void_p), ("fit_srng", ctypes.c_void_p),
+        ("ref_atm", ctypes.c_void_p), ("fit_atm", ctypes.c_void_p),
+        ("ref_rad", ctypes.c_void_p), ("fit_rad", ctypes.c_void_p),
+        ("ref_arng", ctypes.c_void_p), ("fit_arng", ctypes.c_void_p),
+        ("max_ref_srf", ctypes.c_int32), ("max_fit_srf", ctypes.c_int32),
+        ("max_ref_atm", ctypes.c_int32), ("max_fit_atm", ctypes.c_int32),
     ]
 
 
@@ -61,6 +71,9 @@ class FssScoreParams(ctypes.Structure):
         ("ph_alpha", ctypes.c_double * N_PTYPES),
         ("ph_k", ctypes.c_double * N_PTYPES),
         ("ph_cat", ctypes.c_int32 * N_PTYPES),
+        ("esp_weight", ctypes.c_double),
+        ("probe_radius", ctypes.c_double),
+        ("coulomb", ctypes.c_double),
     ]
 
 

@@ -37,10 +37,13 @@ class ScoreSpec:
     """What is scored: objective mode and its constants."""
     mode: int = _lib.MODE_SHAPE
     alpha: float = 0.81
-    lam: float = 0.3 * LAM_SCALING          # already scaled
+    lam: float = 0.3 * LAM_SCALING          # already scaled (ESP_COMBO: LAM_SCALING * lam)
     color_weight: float = 0.5
     similarity: str = "tanimoto"
     directionless: bool = False
+    esp_weight: float = 0.5                 # ESP_COMBO (vol_and_surf_esp) constants
+    probe_radius: float = 1.0
+    coulomb: float = _lib.COULOMB_SCALING
 
     def struct(self) -> _lib.FssScoreParams:
         s = _lib.FssScoreParams()
@@ -48,6 +51,9 @@ class ScoreSpec:
         s.alpha = float(self.alpha)
         s.lam = float(self.lam)
         s.color_weight = float(self.color_weight)
+        s.esp_weight = float(self.esp_weight)
+        s.probe_radius = float(self.probe_radius)
+        s.coulomb = float(self.coulomb)
         sim = self.similarity.lower()
         if sim == "tanimoto":
             s.similarity, s.sigma = _lib.SIM_TANIMOTO, 0.5
@@ -161,6 +167,8 @@ def default_tiling(pb: PairBatch, mode: int, B: int, early_stop: bool) -> Tuple[
     """(lanes_per_pose, restarts_per_wg) minimising :func:`tiling_cost` over the legal tilings."""
     if mode == _lib.MODE_PHARM:
         rc, fc = pb.ref_pcounts, pb.fit_pcounts
+    elif mode == _lib.MODE_ESP_COMBO:    # lanes walk the surface observers, inner loop the atoms
+        rc, fc = pb.fit_acounts, pb.ref_scounts
     else:
         rc, fc = pb.ref_counts, pb.fit_counts
     rc = rc if rc is not None and len(rc) else np.ones(1)
@@ -223,7 +231,7 @@ def align(pb: PairBatch, spec: ScoreSpec, q0: torch.Tensor, t0: torch.Tensor,
             lpp0, rpw0 = default_tiling(pb, spec.mode, B, opt.early_stop)
     else:
         lpp0, rpw0 = default_tiling(pb, spec.mode, B, opt.early_stop)
-    if variant == 3 and spec.mode != _lib.MODE_PHARM and lanes_per_pose is None:
+    if variant == 3 and spec.mode not in (_lib.MODE_PHARM, _lib.MODE_ESP_COMBO) and lanes_per_pose is None:
         lpp0, rpw0 = 64, (rpw0 if lpp0 == 64 else 4)
     lpp = lanes_per_pose or lpp0
     rpw = restarts_per_wg or (B if opt.early_stop else min(rpw0, B))

@@ -33,6 +33,8 @@ _lock = threading.Lock()
 def _counts(pb: PairBatch, mode: int):
     if mode == _lib.MODE_PHARM:
         return pb.ref_pcounts, pb.fit_pcounts
+    if mode == _lib.MODE_ESP_COMBO:      # lanes walk the surface observers, inner loop the atoms
+        return pb.fit_acounts, pb.ref_scounts
     return pb.ref_counts, pb.fit_counts
 
 
@@ -90,7 +92,7 @@ def autotune(pb: PairBatch, spec: ScoreSpec, q0: torch.Tensor, t0: torch.Tensor,
     variants = (1, 2) if spec.mode != _lib.MODE_PHARM else (0,)
     # the MFMA inner loop runs one pose per wavefront: its own tilings (64 lanes per pose)
     runs = [(l, r, v) for l, r in tilings for v in variants]
-    if spec.mode != _lib.MODE_PHARM:
+    if spec.mode not in (_lib.MODE_PHARM, _lib.MODE_ESP_COMBO):
         runs += [(64, r, 3) for r in ((B,) if early_stop else sorted({1, 2, 4, 8, min(16, B)}))
                  if r <= B and 64 * r <= 1024]
     if self_ov is None:

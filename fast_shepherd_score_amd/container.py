@@ -45,11 +45,14 @@ class Molecule:
 
     Mirrors the attributes the reference's aligners read from a ``Molecule``
     (container/_core.py; screen.py:650-668): ``atom_pos``, ``surf_pos``, ``surf_esp``,
-    ``pharm_types``, ``pharm_ancs``, ``pharm_vecs`` and per-heavy-atom ``atom_charges``.
+    ``pharm_types``, ``pharm_ancs``, ``pharm_vecs``, per-heavy-atom ``atom_charges``, and for
+    vol_and_surf_esp the atoms including hydrogens ``centers_w_H`` with their
+    ``partial_charges`` and vdW ``radii`` (the reference reads them from the RDKit conformer).
     """
 
     def __init__(self, atom_pos, surf_pos=None, surf_esp=None, pharm_types=None,
-                 pharm_ancs=None, pharm_vecs=None, atom_charges=None):
+                 pharm_ancs=None, pharm_vecs=None, atom_charges=None, centers_w_H=None,
+                 partial_charges=None, radii=None):
         f = lambda a: None if a is None else np.ascontiguousarray(np.asarray(a, np.float32))
         self.atom_pos = f(atom_pos)
         self.surf_pos = f(surf_pos)
@@ -58,12 +61,16 @@ class Molecule:
         self.pharm_ancs = f(pharm_ancs)
         self.pharm_vecs = f(pharm_vecs)
         self.atom_charges = f(atom_charges)
+        self.centers_w_H = f(centers_w_H)
+        self.partial_charges = f(partial_charges)
+        self.radii = f(radii)
 
     def centered(self) -> "Molecule":
         c = self.atom_pos.mean(0)
         sub = lambda a: None if a is None else a - c
         return Molecule(self.atom_pos - c, sub(self.surf_pos), self.surf_esp, self.pharm_types,
-                        sub(self.pharm_ancs), self.pharm_vecs, self.atom_charges)
+                        sub(self.pharm_ancs), self.pharm_vecs, self.atom_charges,
+                        sub(self.centers_w_H), self.partial_charges, self.radii)
 
 
 class MoleculePair:
@@ -103,6 +110,8 @@ def _mode_inputs(pairs: Sequence[MoleculePair], mode: str):
         rw, fw = check(need(R, "surf_esp"), "surf_esp"), check(need(F, "surf_esp"), "surf_esp")
     elif mode == "pharm":
         rc, fc = None, None
+    elif mode == "vol_and_surf_esp":     # shape centres are chosen by alpha in align_pairs
+        rc, fc = check(need(R, "atom_pos"), "atom_pos"), check(need(F, "atom_pos"), "atom_pos")
     else:
         raise ValueError(mode)
     if mode in ("pharm", "vol_color"):
@@ -114,10 +123,26 @@ def _mode_inputs(pairs: Sequence[MoleculePair], mode: str):
     return rc, fc, rw, fw, ph
 
 
+def _shaep_sets(pairs: Sequence[MoleculePair], dev):
+    """ShaEP channel of both sides: surfaces with ESP, atoms with hydrogens, charges, radii."""
+    from .packing import ShaepSet
+    out = []
+    for side in ("ref_molec", "fit_molec"):
+        ms = [getattr(p, side) for p in pairs]
+        for a in ("surf_pos", "surf_esp", "centers_w_H", "partial_charges", "radii"):
+            if any(getattr(m, a) is None for m in ms):
+                raise ValueError(f"vol_and_surf_esp needs {a} on every molecule")
+        out.append(ShaepSet.from_arrays([m.surf_pos for m in ms], [m.surf_esp for m in ms],
+                                        [m.centers_w_H for m in ms], [m.partial_charges for m in ms],
+                                        [m.radii for m in ms], device=dev))
+    return out
+
+
 def align_pairs(pairs: Sequence[MoleculePair], mode: str, *, alpha=0.81, lam=None,
                 color_weight=0.5, similarity="tanimoto", directionless=False,
                 num_repeats: int = 50, lr: float = 0.1, max_num_steps: int = 200,
-                early_stop: bool = True, seeds=None, device=None, **tiling):
+                early_stop: bool = True, seeds=None, device=None, probe_radius: float = 1.0,
+                esp_weight: float = 0.5, **tiling):
     """Align every pair in one persistent launch; write results in place; return the result.
 
     ``seeds``: None computes the reference's multi-start seeds on the device (fss_seed_poses);
@@ -133,24 +158,37 @@ def align_pairs(pairs: Sequence[MoleculePair], mode: str, *, alpha=0.81, lam=Non
     rc, fc, rw, fw, ph = _mode_inputs(pairs, mode)
     kind = {"vol": _lib.MODE_SHAPE, "surf": _lib.MODE_SHAPE, "vol_esp": _lib.MODE_SHAPE_ESP,
             "surf_esp": _lib.MODE_SHAPE_ESP, "pharm": _lib.MODE_PHARM,
-            "vol_color": _lib.MODE_SHAPE_PHARM}[mode]
-    if lam is None:
+            "vol_color": _lib.MODE_SHAPE_PHARM, "vol_and_surf_esp": _lib.MODE_ESP_COMBO}[mode]
+    combo = mode == "vol_and_surf_esp"
+    if combo:
+        # volumetric shape term at alpha 0.81, surface shape term otherwise (_core.py:1079-1080);
+        # lam is the comparison's raw lambda (default 0.001), LAM_SCALING applied here
+        if alpha != 0.81:
+            rc = [p.ref_molec.surf_pos for p in pairs]
+            fc = [p.fit_molec.surf_pos for p in pairs]
+        lam = LAM_SCALING * (0.001 if lam is None else lam)
+    elif lam is None:
         lam = 0.3 * LAM_SCALING if mode == "surf_esp" else 0.1
     spec = ScoreSpec(mode=kind, alpha=alpha, lam=lam, color_weight=color_weight,
-                     similarity=similarity, directionless=directionless)
+                     similarity=similarity, directionless=directionless,
+                     esp_weight=esp_weight, probe_radius=probe_radius)
     ref = CloudSet.from_arrays(rc, rw, device=dev) if rc is not None else None
     fit = CloudSet.from_arrays(fc, fw, device=dev) if fc is not None else None
     rph = fph = None
     if ph is not None:
         rph = PharmSet.from_arrays(ph["ref_types"], ph["ref_anchors"], ph["ref_vectors"], device=dev)
         fph = PharmSet.from_arrays(ph["fit_types"], ph["fit_anchors"], ph["fit_vectors"], device=dev)
-    pb = PairBatch(ref, fit, rph, fph)
-    # Seeds come from the clouds the objective moves: pharm anchors for pharm, else shape.
+    rse = fse = None
+    if combo:
+        rse, fse = _shaep_sets(pairs, dev)
+    pb = PairBatch(ref, fit, rph, fph, ref_se=rse, fit_se=fse)
+    # Seeds come from the clouds the objective moves: pharm anchors for pharm, the surface
+    # points for vol_and_surf_esp (alignment/_torch.py:906), else the shape clouds.
     if seeds is None:
-        q0, t0 = device_seeds(pb, num_repeats, use_pharm=mode == "pharm")
+        q0, t0 = device_seeds(pb, num_repeats, use_pharm=mode == "pharm", use_surface=combo)
     elif isinstance(seeds, str) and seeds == "host":
-        sref = ph["ref_anchors"] if mode == "pharm" else rc
-        sfit = ph["fit_anchors"] if mode == "pharm" else fc
+        sref = ph["ref_anchors"] if mode == "pharm" else ([p.ref_molec.surf_pos for p in pairs] if combo else rc)
+        sfit = ph["fit_anchors"] if mode == "pharm" else ([p.fit_molec.surf_pos for p in pairs] if combo else fc)
         (R, rn), (Fp, fn) = pad(sref), pad(sfit)
         q0, t0 = initial_seeds(R, rn, Fp, fn, num_repeats)
     else:
@@ -198,6 +236,21 @@ class MoleculePairBatch:
             X = np.asarray(getattr(p.fit_molec, fit_cloud), np.float64)
             aligned.append((X @ T[:3, :3].T + T[:3, 3]).astype(np.float32))
         return scores, aligned
+
+    def align_with_vol_and_surf_esp(self, alpha: float, lam: float = 0.001, probe_radius: float = 1.0,
+                                    esp_weight: float = 0.5, num_repeats: int = None,
+                                    trans_init: bool = False, lr: float = 0.1,
+                                    max_num_steps: int = None, num_workers: int = 1,
+                                    use_shmap: bool = False, verbose: bool = False,
+                                    backend: Optional[str] = None, return_aligned: bool = False,
+                                    **hip_kw):
+        """ShaEP score: (1 - esp_weight) shape Tanimoto (heavy atoms at alpha 0.81, surface
+        points otherwise) + esp_weight surface ESP comparison (_batch.py:1030-1102)."""
+        return self._run("vol_and_surf_esp", backend, return_aligned, "surf_pos", num_repeats,
+                         max_num_steps, trans_init=trans_init, lr=lr, alpha=alpha, lam=lam,
+                         probe_radius=probe_radius, esp_weight=esp_weight, **self._hip_kw(hip_kw))
+
+    align_with_esp_combo = align_with_vol_and_surf_esp   # legacy alias (_batch.py:1102)
 
     @staticmethod
     def _hip_kw(kw):
@@ -278,9 +331,3 @@ class MoleculePairBatch:
                          max_num_steps, trans_init=trans_init, lr=lr, alpha=alpha,
                          color_weight=color_weight, directionless=directionless,
                          similarity=similarity, **self._hip_kw(hip_kw))
-
-    def align_with_vol_and_surf_esp(self, *a, **k):
-        raise NotImplementedError("vol_and_surf_esp (ShaEP ESP comparison) is not implemented by "
-                                  "the HIP backend")
-
-    align_with_esp_combo = align_with_vol_and_surf_esp

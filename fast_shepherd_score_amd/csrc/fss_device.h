@@ -456,6 +456,164 @@ __device__ __forceinline__ void shape_accum_mfma(const float4* __restrict__ amat
   }
 }
 
+// ---------------------------------------------------------------------------------------------
+// ShaEP surface ESP comparison (FSS_MODE_ESP_COMBO): value and gradient of one pose, FP32.
+//   esp = sum over observer points i not inside the other molecule's radius + probe volume of
+//         exp(-(pe_i - C sum_m q_m / d_im)^2 / lam)
+// for the ref surface against the moved fit atoms (direction 1) and for the moved fit surface
+// against the ref atoms (direction 2) (score/electrostatic_scoring.py:128-227).
+// Coordinates: ref rows relative to the origin o, fit rows in the fit frame; the pose maps
+// x -> R x + tr (tr = t - o).  Direction 1 runs in the fit frame (each observer pulled back by
+// R^T), so the fit atoms are read untransformed and its forces are rotated back once.
+// Gradient: dE_i/dphi_i = 2 E_i u_i / lam with u_i = pe_i - phi_i; d(1/d)/da = (p - a) / d^3;
+// the mask is piecewise constant (zero gradient).  Lanes take observers j, j+lpp, ...; acc gets
+// V = esp, gt = dV/dt, G = dV/dR.  rb_* hold (radius + probe)^2.
+// ---------------------------------------------------------------------------------------------
+struct ShaepK {
+  float coul;   // C
+  float kq;     // log2(e) / lam
+  float c2;     // 2 C / lam
+};
+
+template <bool GRAD>
+__device__ __forceinline__ void shaep_accum(const float4* __restrict__ srf_r, int Ns,
+                                            const float4* __restrict__ atm_f, const float* __restrict__ rb_f,
+                                            int Ma, const float4* __restrict__ srf_f, int Ms,
+                                            const float4* __restrict__ atm_r, const float* __restrict__ rb_r,
+                                            int Na, int j, int lpp, const float R[9], const float tr[3],
+                                            const ShaepK k, Acc& acc) {
+  float f[3] = {0.f, 0.f, 0.f}, Fm[9] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+  for (int i = j; i < Ns; i += lpp) {
+    const float4 P = srf_r[i];
+    const float dx = P.x - tr[0], dy = P.y - tr[1], dz = P.z - tr[2];
+    const float px = fmaf(R[0], dx, fmaf(R[3], dy, R[6] * dz));
+    const float py = fmaf(R[1], dx, fmaf(R[4], dy, R[7] * dz));
+    const float pz = fmaf(R[2], dx, fmaf(R[5], dy, R[8] * dz));
+    float phi = 0.f;
+    bool blocked = false;
+    for (int m = 0; m < Ma; ++m) {
+      const float4 A = atm_f[m];
+      const float ex = px - A.x, ey = py - A.y, ez = pz - A.z;
+      const float d2 = fmaf(ex, ex, fmaf(ey, ey, ez * ez));
+      blocked |= d2 < rb_f[m];
+      phi = fmaf(A.w, __builtin_amdgcn_rsqf(d2), phi);
+    }
+    if (blocked) continue;
+    const float u = fmaf(-k.coul, phi, P.w);
+    const float E = fast_exp2(-(u * u) * k.kq);
+    acc.V += E;
+    if (GRAD) {
+      const float c = E * u * k.c2;
+      for (int m = 0; m < Ma; ++m) {
+        const float4 A = atm_f[m];
+        const float ex = px - A.x, ey = py - A.y, ez = pz - A.z;
+        const float ri = __builtin_amdgcn_rsqf(fmaf(ex, ex, fmaf(ey, ey, ez * ez)));
+        const float w = c * A.w * (ri * ri * ri);
+        const float vx = w * ex, vy = w * ey, vz = w * ez;
+        f[0] += vx; f[1] += vy; f[2] += vz;
+        Fm[0] = fmaf(vx, A.x, Fm[0]); Fm[1] = fmaf(vx, A.y, Fm[1]); Fm[2] = fmaf(vx, A.z, Fm[2]);
+        Fm[3] = fmaf(vy, A.x, Fm[3]); Fm[4] = fmaf(vy, A.y, Fm[4]); Fm[5] = fmaf(vy, A.z, Fm[5]);
+        Fm[6] = fmaf(vz, A.x, Fm[6]); Fm[7] = fmaf(vz, A.y, Fm[7]); Fm[8] = fmaf(vz, A.z, Fm[8]);
+      }
+    }
+  }
+  if (GRAD) {   // fit-frame forces -> world: gt += R f, G += R Fm
+#pragma unroll
+    for (int r = 0; r < 3; ++r) {
+      acc.gt[r] += R[3 * r] * f[0] + R[3 * r + 1] * f[1] + R[3 * r + 2] * f[2];
+#pragma unroll
+      for (int c = 0; c < 3; ++c)
+        acc.G[3 * r + c] += R[3 * r] * Fm[c] + R[3 * r + 1] * Fm[3 + c] + R[3 * r + 2] * Fm[6 + c];
+    }
+  }
+  for (int jj = j; jj < Ms; jj += lpp) {
+    const float4 P0 = srf_f[jj];
+    const float px = fmaf(R[0], P0.x, fmaf(R[1], P0.y, fmaf(R[2], P0.z, tr[0])));
+    const float py = fmaf(R[3], P0.x, fmaf(R[4], P0.y, fmaf(R[5], P0.z, tr[1])));
+    const float pz = fmaf(R[6], P0.x, fmaf(R[7], P0.y, fmaf(R[8], P0.z, tr[2])));
+    float phi = 0.f;
+    bool blocked = false;
+    for (int m = 0; m < Na; ++m) {
+      const float4 A = atm_r[m];
+      const float ex = px - A.x, ey = py - A.y, ez = pz - A.z;
+      const float d2 = fmaf(ex, ex, fmaf(ey, ey, ez * ez));
+      blocked |= d2 < rb_r[m];
+      phi = fmaf(A.w, __builtin_amdgcn_rsqf(d2), phi);
+    }
+    if (blocked) continue;
+    const float u = fmaf(-k.coul, phi, P0.w);
+    const float E = fast_exp2(-(u * u) * k.kq);
+    acc.V += E;
+    if (GRAD) {
+      float sx = 0.f, sy = 0.f, sz = 0.f;
+      for (int m = 0; m < Na; ++m) {
+        const float4 A = atm_r[m];
+        const float ex = px - A.x, ey = py - A.y, ez = pz - A.z;
+        const float ri = __builtin_amdgcn_rsqf(fmaf(ex, ex, fmaf(ey, ey, ez * ez)));
+        const float w = A.w * (ri * ri * ri);
+        sx = fmaf(w, ex, sx); sy = fmaf(w, ey, sy); sz = fmaf(w, ez, sz);
+      }
+      const float c = -E * u * k.c2;
+      const float hx = c * sx, hy = c * sy, hz = c * sz;
+      acc.gt[0] += hx; acc.gt[1] += hy; acc.gt[2] += hz;
+      acc.G[0] = fmaf(hx, P0.x, acc.G[0]); acc.G[1] = fmaf(hx, P0.y, acc.G[1]); acc.G[2] = fmaf(hx, P0.z, acc.G[2]);
+      acc.G[3] = fmaf(hy, P0.x, acc.G[3]); acc.G[4] = fmaf(hy, P0.y, acc.G[4]); acc.G[5] = fmaf(hy, P0.z, acc.G[5]);
+      acc.G[6] = fmaf(hz, P0.x, acc.G[6]); acc.G[7] = fmaf(hz, P0.y, acc.G[7]); acc.G[8] = fmaf(hz, P0.z, acc.G[8]);
+    }
+  }
+}
+
+// FP64 value of the ShaEP comparison of one pose from the packed global arrays (ref rows in the
+// world frame, fit rows in the fit frame, pose x -> R x + t); lanes j of nt.
+__device__ __forceinline__ double shaep_value_f64(const float4* __restrict__ srf_r, int Ns,
+                                                  const float4* __restrict__ atm_f, const float* __restrict__ rad_f,
+                                                  int Ma, const float4* __restrict__ srf_f, int Ms,
+                                                  const float4* __restrict__ atm_r, const float* __restrict__ rad_r,
+                                                  int Na, int j, int nt, const double R[9], const double t[3],
+                                                  double coul, double lam, double probe) {
+  double v = 0.0;
+  for (int i = j; i < Ns; i += nt) {
+    const float4 P = srf_r[i];
+    const double dx = P.x - t[0], dy = P.y - t[1], dz = P.z - t[2];
+    const double px = R[0] * dx + R[3] * dy + R[6] * dz;
+    const double py = R[1] * dx + R[4] * dy + R[7] * dz;
+    const double pz = R[2] * dx + R[5] * dy + R[8] * dz;
+    double phi = 0.0;
+    bool blocked = false;
+    for (int m = 0; m < Ma; ++m) {
+      const float4 A = atm_f[m];
+      const double ex = px - A.x, ey = py - A.y, ez = pz - A.z;
+      const double d = sqrt(ex * ex + ey * ey + ez * ez);
+      blocked |= d < (double)rad_f[m] + probe;
+      phi += (double)A.w / d;
+    }
+    if (!blocked) {
+      const double u = (double)P.w - coul * phi;
+      v += exp(-(u * u) / lam);
+    }
+  }
+  for (int jj = j; jj < Ms; jj += nt) {
+    const float4 P0 = srf_f[jj];
+    const double px = R[0] * P0.x + R[1] * P0.y + R[2] * P0.z + t[0];
+    const double py = R[3] * P0.x + R[4] * P0.y + R[5] * P0.z + t[1];
+    const double pz = R[6] * P0.x + R[7] * P0.y + R[8] * P0.z + t[2];
+    double phi = 0.0;
+    bool blocked = false;
+    for (int m = 0; m < Na; ++m) {
+      const float4 A = atm_r[m];
+      const double ex = px - A.x, ey = py - A.y, ez = pz - A.z;
+      const double d = sqrt(ex * ex + ey * ey + ez * ez);
+      blocked |= d < (double)rad_r[m] + probe;
+      phi += (double)A.w / d;
+    }
+    if (!blocked) {
+      const double u = (double)P0.w - coul * phi;
+      v += exp(-(u * u) / lam);
+    }
+  }
+  return v;
+}
+
 // Convert scaled shape sums into O, dO/dt, dO/dR (in place).
 __device__ __forceinline__ void shape_finish(Acc& a, float K, float alpha, float s, bool grad) {
   a.V *= K;

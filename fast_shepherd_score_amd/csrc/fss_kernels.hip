@@ -55,7 +55,10 @@ int check_launch(const char* what) {
 
 constexpr int kPoseBlock = 256;  // threads of the one-workgroup-per-pose kernels
 
-bool uses_shape(int mode) { return mode == FSS_MODE_SHAPE || mode == FSS_MODE_SHAPE_ESP || mode == FSS_MODE_SHAPE_PHARM; }
+bool uses_shape(int mode) {
+  return mode == FSS_MODE_SHAPE || mode == FSS_MODE_SHAPE_ESP || mode == FSS_MODE_SHAPE_PHARM || mode == FSS_MODE_ESP_COMBO;
+}
+bool uses_shaep(int mode) { return mode == FSS_MODE_ESP_COMBO; }
 bool uses_pharm(int mode) { return mode == FSS_MODE_PHARM || mode == FSS_MODE_SHAPE_PHARM; }
 
 // Device-side scoring constants derived once per call on the host.
@@ -71,12 +74,22 @@ struct KScore {
   double ph_alpha[FSS_N_PTYPES];
   double ph_k[FSS_N_PTYPES];
   int32_t ph_cat[FSS_N_PTYPES];
+  // ShaEP channel (FSS_MODE_ESP_COMBO)
+  ShaepK se;
+  float se_w, se_probe2;         // esp weight, probe radius (staging adds it to each radius)
+  double se_w_d, se_coul_d, se_lam_d, se_probe_d;
 };
 
 int make_kscore(const FssScoreParams* sp, KScore& ks) {
   if (!sp) return set_error(FSS_E_INVALID, "score params are NULL");
-  if (sp->mode < 0 || sp->mode > 3) return set_error(FSS_E_INVALID, "unknown mode %d", sp->mode);
+  if (sp->mode < 0 || sp->mode > 4) return set_error(FSS_E_INVALID, "unknown mode %d", sp->mode);
   if (uses_shape(sp->mode) && !(sp->alpha > 0.0)) return set_error(FSS_E_INVALID, "alpha must be > 0");
+  if (sp->mode == FSS_MODE_ESP_COMBO) {
+    if (!(sp->lam > 0.0)) return set_error(FSS_E_INVALID, "lam must be > 0");
+    if (!(sp->coulomb > 0.0)) return set_error(FSS_E_INVALID, "coulomb must be > 0");
+    if (!(sp->probe_radius >= 0.0)) return set_error(FSS_E_INVALID, "probe_radius must be >= 0");
+    if (!(sp->esp_weight >= 0.0 && sp->esp_weight <= 1.0)) return set_error(FSS_E_INVALID, "esp_weight must be in [0, 1]");
+  }
   if (sp->mode == FSS_MODE_SHAPE_ESP && !(sp->lam > 0.0)) return set_error(FSS_E_INVALID, "lam must be > 0");
   if (sp->similarity != FSS_SIM_TANIMOTO && sp->similarity != FSS_SIM_TVERSKY)
     return set_error(FSS_E_INVALID, "unknown similarity %d", sp->similarity);
@@ -102,6 +115,18 @@ int make_kscore(const FssScoreParams* sp, KScore& ks) {
     ks.tab.k[i] = (float)sp->ph_k[i];
     ks.tab.cat[i] = sp->ph_cat[i];
   }
+  if (sp->mode == FSS_MODE_ESP_COMBO) {
+    ks.lam_d = 1.0;                    // the shape channel has no ESP weighting here
+    ks.se.coul = (float)sp->coulomb;
+    ks.se.kq = (float)(kLog2eD / sp->lam);
+    ks.se.c2 = (float)(2.0 * sp->coulomb / sp->lam);
+    ks.se_w = (float)sp->esp_weight;
+    ks.se_probe2 = (float)sp->probe_radius;
+    ks.se_w_d = sp->esp_weight;
+    ks.se_coul_d = sp->coulomb;
+    ks.se_lam_d = sp->lam;
+    ks.se_probe_d = sp->probe_radius;
+  }
   return FSS_OK;
 }
 
@@ -115,6 +140,13 @@ int check_batch(const FssPairBatch* pb, int mode) {
     return set_error(FSS_E_INVALID, "pharmacophore channel arrays are NULL");
   if (pb->max_ref < 0 || pb->max_fit < 0 || pb->max_ref_ph < 0 || pb->max_fit_ph < 0)
     return set_error(FSS_E_INVALID, "negative cloud maxima");
+  if (uses_shaep(mode)) {
+    if (!pb->ref_srf || !pb->fit_srf || !pb->ref_srng || !pb->fit_srng || !pb->ref_atm || !pb->fit_atm ||
+        !pb->ref_rad || !pb->fit_rad || !pb->ref_arng || !pb->fit_arng)
+      return set_error(FSS_E_INVALID, "ShaEP channel arrays are NULL");
+    if (pb->max_ref_srf < 0 || pb->max_fit_srf < 0 || pb->max_ref_atm < 0 || pb->max_fit_atm < 0)
+      return set_error(FSS_E_INVALID, "negative ShaEP cloud maxima");
+  }
   return FSS_OK;
 }
 
@@ -123,7 +155,9 @@ int check_batch(const FssPairBatch* pb, int mode) {
 // With `mfma` (inner-loop variant 3) the ref rows are the split-bf16 A-operand tiles (64 float4
 // per 16 rows) followed by the 2 a~ row quads (12 float4 per 16 rows), and no ESP column.
 struct LdsPlan {
-  int ref, xq, fit, refc, red, poff, ranc, rvec, fanc, fvec, end;
+  int ref, xq, fit, refc, red, poff, ranc, rvec, fanc, fvec;
+  int srf_r, srf_f, atm_r, atm_f, rb_r, rb_f;   // ShaEP channel
+  int end;
 };
 LdsPlan plan_lds(const FssPairBatch* pb, int mode, bool mfma = false) {
   LdsPlan p;
@@ -140,6 +174,13 @@ LdsPlan plan_lds(const FssPairBatch* pb, int mode, bool mfma = false) {
   p.rvec = o; o += ph ? pb->max_ref_ph : 0;
   p.fanc = o; o += ph ? pb->max_fit_ph : 0;
   p.fvec = o; o += ph ? pb->max_fit_ph : 0;
+  const bool se = uses_shaep(mode);
+  p.srf_r = o; o += se ? pb->max_ref_srf : 0;
+  p.srf_f = o; o += se ? pb->max_fit_srf : 0;
+  p.atm_r = o; o += se ? pb->max_ref_atm : 0;
+  p.atm_f = o; o += se ? pb->max_fit_atm : 0;
+  p.rb_r = o; o += se ? (pb->max_ref_atm + 3) / 4 : 0;
+  p.rb_f = o; o += se ? (pb->max_fit_atm + 3) / 4 : 0;
   p.end = o;
   return p;
 }
@@ -157,11 +198,16 @@ struct Staged {
   float cs[3];  // scaled ref centroid: the origin of the expanded shape form
   const float4* ranc; const float4* rvec; const float4* fanc; const float4* fvec; int Np, Mp;
   const int* poff;  // ref pharmacophore rows are sorted by type; type t occupies [poff[t], poff[t+1])
+  // ShaEP channel: ref rows relative to the origin co (the ref centroid), fit rows raw
+  const float4* srf_r; const float4* srf_f; const float4* atm_r; const float4* atm_f;
+  const float* rb_r; const float* rb_f;   // (radius + probe)^2
+  int Ns, Ms, Na, Ma;
+  float co[3];
 };
 
 // Cooperative copy of one pair's clouds into LDS (shape rows scaled, pharm vectors normalised,
 // pharm type folded into the anchor's w as integer bits).  Caller syncs afterwards.
-template <bool SH, bool ESP, bool PH, bool EXPD = true, bool MF = false>
+template <bool SH, bool ESP, bool PH, bool EXPD = true, bool MF = false, bool SE = false>
 __device__ __forceinline__ Staged stage_pair(const FssPairBatch& pb, const KScore& ks, LdsPlan lp,
                                              int pair, int ref_side_a, int fit_side_b,
                                              float4* lds) {
@@ -237,6 +283,42 @@ __device__ __forceinline__ Staged stage_pair(const FssPairBatch& pb, const KScor
     }
     st.ref = la; st.refc = lc; st.fit = lb; st.N = N; st.M = M;
     st.cs[0] = s * cx; st.cs[1] = s * cy; st.cs[2] = s * cz;
+    st.co[0] = cx; st.co[1] = cy; st.co[2] = cz;
+  }
+  if (SE) {   // sides are (ref, fit) here
+    const int s0 = pb.ref_srng[2 * pair], Ns = pb.ref_srng[2 * pair + 1];
+    const int s1 = pb.fit_srng[2 * pair], Ms = pb.fit_srng[2 * pair + 1];
+    const int a0 = pb.ref_arng[2 * pair], Na = pb.ref_arng[2 * pair + 1];
+    const int a1 = pb.fit_arng[2 * pair], Ma = pb.fit_arng[2 * pair + 1];
+    const float4* gsr = reinterpret_cast<const float4*>(pb.ref_srf) + s0;
+    const float4* gsf = reinterpret_cast<const float4*>(pb.fit_srf) + s1;
+    const float4* gar = reinterpret_cast<const float4*>(pb.ref_atm) + a0;
+    const float4* gaf = reinterpret_cast<const float4*>(pb.fit_atm) + a1;
+    float4* lsr = lds + lp.srf_r;
+    float4* lsf = lds + lp.srf_f;
+    float4* lar = lds + lp.atm_r;
+    float4* laf = lds + lp.atm_f;
+    float* lbr = reinterpret_cast<float*>(lds + lp.rb_r);
+    float* lbf = reinterpret_cast<float*>(lds + lp.rb_f);
+    const float cx = st.co[0], cy = st.co[1], cz = st.co[2], pr = ks.se_probe2;
+    for (int i = tid; i < Ns; i += nt) {
+      const float4 v = gsr[i];
+      lsr[i] = make_float4(v.x - cx, v.y - cy, v.z - cz, v.w);
+    }
+    for (int i = tid; i < Ms; i += nt) lsf[i] = gsf[i];
+    for (int i = tid; i < Na; i += nt) {
+      const float4 v = gar[i];
+      lar[i] = make_float4(v.x - cx, v.y - cy, v.z - cz, v.w);
+      const float r = pb.ref_rad[a0 + i] + pr;
+      lbr[i] = r * r;
+    }
+    for (int i = tid; i < Ma; i += nt) {
+      laf[i] = gaf[i];
+      const float r = pb.fit_rad[a1 + i] + pr;
+      lbf[i] = r * r;
+    }
+    st.srf_r = lsr; st.srf_f = lsf; st.atm_r = lar; st.atm_f = laf; st.rb_r = lbr; st.rb_f = lbf;
+    st.Ns = Ns; st.Ms = Ms; st.Na = Na; st.Ma = Ma;
   }
   if (PH) {
     const int* ra = ref_side_a ? pb.fit_prng : pb.ref_prng;
@@ -291,9 +373,18 @@ __device__ __forceinline__ Staged stage_pair(const FssPairBatch& pb, const KScor
 template <typename T>
 __device__ __forceinline__ T similarity(const KScore& ks, T Os, T Op, const double* so, bool has_sh,
                                         bool has_ph, T& ws, T& wp) {
-  // ws, wp receive dS/dO of the shape and pharm channel.
+  // ws, wp receive dS/dO of the shape and pharm channel (the ShaEP channel for ESP_COMBO).
   T S = 0;
   ws = 0; wp = 0;
+  if (ks.mode == FSS_MODE_ESP_COMBO) {   // so[2] + so[3] = surface point counts
+    const T w = sizeof(T) == 8 ? (T)ks.se_w_d : (T)ks.se_w;
+    const T U = (T)so[0] + (T)so[1];
+    const T D = U - Os;
+    const T nrm = (T)so[2] + (T)so[3];
+    ws = ((T)1 - w) * U / (D * D);
+    wp = w / nrm;
+    return ((T)1 - w) * (Os / D) + w * (Op / nrm);
+  }
   const T wc = sizeof(T) == 8 ? (T)ks.w_d : (T)ks.w;
   const T sg = sizeof(T) == 8 ? (T)ks.sigma_d : (T)ks.sigma;
   const T chs = ks.mode == FSS_MODE_SHAPE_PHARM ? (T)1 - wc : (T)1;
@@ -396,8 +487,9 @@ __global__ __launch_bounds__(kPoseBlock) void k_self_overlap(FssPairBatch pb, KS
     vp = block_sum_d(vp, red);
   }
   if (threadIdx.x == 0) {
+    const int* srng = side ? pb.fit_srng : pb.ref_srng;
     out[4 * pair + side] = SH ? vs : 0.0;
-    out[4 * pair + 2 + side] = PH ? vp : 0.0;
+    out[4 * pair + 2 + side] = PH ? vp : (MODE == FSS_MODE_ESP_COMBO ? (double)srng[2 * pair + 1] : 0.0);
   }
 }
 
@@ -417,10 +509,11 @@ __global__ __launch_bounds__(kPoseBlock) void k_pose_grad(FssPairBatch pb, KScor
   constexpr bool ESP = MODE == FSS_MODE_SHAPE_ESP;
   constexpr bool PH = MODE == FSS_MODE_PHARM || MODE == FSS_MODE_SHAPE_PHARM;
   extern __shared__ float4 lds[];
+  constexpr bool SE = MODE == FSS_MODE_ESP_COMBO;
   __shared__ float red[2][(kPoseBlock / 64) * 13];
   const int pose = blockIdx.x;
   const int pair = pose_pair[pose];
-  const Staged st = stage_pair<SH, ESP, PH>(pb, ks, lp, pair, 0, 1, lds);
+  const Staged st = stage_pair<SH, ESP, PH, true, false, SE>(pb, ks, lp, pair, 0, 1, lds);
   float q[4] = {qin[4 * pose], qin[4 * pose + 1], qin[4 * pose + 2], qin[4 * pose + 3]};
   const float t[3] = {tin[3 * pose], tin[3 * pose + 1], tin[3 * pose + 2]};
   double qh[4], qn, Rd[9];
@@ -439,6 +532,12 @@ __global__ __launch_bounds__(kPoseBlock) void k_pose_grad(FssPairBatch pb, KScor
   }
   if (PH) {
     pharm_accum<GRAD>(st.ranc, st.rvec, st.poff, st.fanc, st.fvec, st.Mp, threadIdx.x, blockDim.x, R, t, ks.tab, ph);
+    block_sum(ph, red[1], GRAD);
+  }
+  if (SE) {   // the ShaEP comparison takes the second channel slot
+    const float tr[3] = {t[0] - st.co[0], t[1] - st.co[1], t[2] - st.co[2]};
+    shaep_accum<GRAD>(st.srf_r, st.Ns, st.atm_f, st.rb_f, st.Ma, st.srf_f, st.Ms, st.atm_r, st.rb_r, st.Na,
+                      threadIdx.x, blockDim.x, R, tr, ks.se, ph);
     block_sum(ph, red[1], GRAD);
   }
   if (threadIdx.x != 0) return;
@@ -516,6 +615,7 @@ __global__ __launch_bounds__(MAXT) __attribute__((amdgpu_waves_per_eu(VAR == 3 ?
 void k_align(FssPairBatch pb, KScore ks, LdsPlan lp, AlignK ak) {
   constexpr bool EXPD = VAR != 1;
   constexpr bool MF = VAR == 3;
+  constexpr bool SE = MODE == FSS_MODE_ESP_COMBO;
   constexpr bool SH = MODE != FSS_MODE_PHARM;
   constexpr bool ESP = MODE == FSS_MODE_SHAPE_ESP;
   constexpr bool PH = MODE == FSS_MODE_PHARM || MODE == FSS_MODE_SHAPE_PHARM;
@@ -523,7 +623,7 @@ void k_align(FssPairBatch pb, KScore ks, LdsPlan lp, AlignK ak) {
   const int slot = blockIdx.x / ak.wg_per_pair;
   const int blk = blockIdx.x - slot * ak.wg_per_pair;
   const int pair = ak.order ? ak.order[slot] : slot;
-  const Staged st = stage_pair<SH, ESP, PH, EXPD, MF>(pb, ks, lp, pair, 0, 1, lds);
+  const Staged st = stage_pair<SH, ESP, PH, EXPD, MF, SE>(pb, ks, lp, pair, 0, 1, lds);
   double* lossbuf = reinterpret_cast<double*>(lds + lp.end);  // 2 * rpw doubles (ES only)
 
   const int lpp = ak.lpp;
@@ -573,6 +673,12 @@ void k_align(FssPairBatch pb, KScore ks, LdsPlan lp, AlignK ak) {
       }
       if (PH) {
         pharm_accum<true>(st.ranc, st.rvec, st.poff, st.fanc, st.fvec, st.Mp, j, lpp, R, p + 4, ks.tab, ph);
+        group_sum(ph, lpp, true);
+      }
+      if (SE) {
+        const float tr[3] = {p[4] - st.co[0], p[5] - st.co[1], p[6] - st.co[2]};
+        shaep_accum<true>(st.srf_r, st.Ns, st.atm_f, st.rb_f, st.Ma, st.srf_f, st.Ms, st.atm_r, st.rb_r, st.Na,
+                          j, lpp, R, tr, ks.se, ph);
         group_sum(ph, lpp, true);
       }
       float ws, wp;
@@ -657,6 +763,16 @@ __global__ __launch_bounds__(256) void k_pose_score(FssPairBatch pb, KScore ks, 
                          reinterpret_cast<const float4*>(pb.fit_anc) + fo,
                          reinterpret_cast<const float4*>(pb.fit_vec) + fo, pb.fit_typ + fo, pb.fit_prng[2 * pair + 1],
                          j, 64, Rd, td, ks.ph_alpha, ks.ph_k, ks.ph_cat);
+    vp = group_sum(vp, 64);
+  }
+  if (MODE == FSS_MODE_ESP_COMBO) {
+    const int s0 = pb.ref_srng[2 * pair], s1 = pb.fit_srng[2 * pair];
+    const int a0 = pb.ref_arng[2 * pair], a1 = pb.fit_arng[2 * pair];
+    vp = shaep_value_f64(reinterpret_cast<const float4*>(pb.ref_srf) + s0, pb.ref_srng[2 * pair + 1],
+                         reinterpret_cast<const float4*>(pb.fit_atm) + a1, pb.fit_rad + a1, pb.fit_arng[2 * pair + 1],
+                         reinterpret_cast<const float4*>(pb.fit_srf) + s1, pb.fit_srng[2 * pair + 1],
+                         reinterpret_cast<const float4*>(pb.ref_atm) + a0, pb.ref_rad + a0, pb.ref_arng[2 * pair + 1],
+                         j, 64, Rd, td, ks.se_coul_d, ks.se_lam_d, ks.se_probe_d);
     vp = group_sum(vp, 64);
   }
   double ws, wp;
@@ -838,6 +954,7 @@ void launch_pose(const FssPairBatch* pb, const KScore& ks, LdsPlan lp, int n, co
 // form; the pharmacophore-only mode has no shape loop and always runs as variant 2.
 int effective_variant(int mode, int variant) {
   if (mode == FSS_MODE_PHARM) return 2;
+  if (mode == FSS_MODE_ESP_COMBO) return variant == 1 ? 1 : 2;
   return variant == 1 || variant == 3 ? variant : 2;
 }
 template <int MODE, bool ES>
@@ -910,6 +1027,7 @@ int pose_score_impl(const FssPairBatch* batch, const KScore& ks, int32_t n_poses
     case FSS_MODE_SHAPE: launch_pose_score<FSS_MODE_SHAPE>(batch, ks, n_poses, B, pose_pair, self_ov, q, t, score, s); break;
     case FSS_MODE_SHAPE_ESP: launch_pose_score<FSS_MODE_SHAPE_ESP>(batch, ks, n_poses, B, pose_pair, self_ov, q, t, score, s); break;
     case FSS_MODE_PHARM: launch_pose_score<FSS_MODE_PHARM>(batch, ks, n_poses, B, pose_pair, self_ov, q, t, score, s); break;
+    case FSS_MODE_ESP_COMBO: launch_pose_score<FSS_MODE_ESP_COMBO>(batch, ks, n_poses, B, pose_pair, self_ov, q, t, score, s); break;
     default: launch_pose_score<FSS_MODE_SHAPE_PHARM>(batch, ks, n_poses, B, pose_pair, self_ov, q, t, score, s); break;
   }
   return check_launch("fss_pose_score");
@@ -1096,6 +1214,7 @@ int fss_self_overlap(const FssPairBatch* batch, const FssScoreParams* sp, double
     case FSS_MODE_SHAPE: launch_self<FSS_MODE_SHAPE>(batch, ks, out, s); break;
     case FSS_MODE_SHAPE_ESP: launch_self<FSS_MODE_SHAPE_ESP>(batch, ks, out, s); break;
     case FSS_MODE_PHARM: launch_self<FSS_MODE_PHARM>(batch, ks, out, s); break;
+    case FSS_MODE_ESP_COMBO: launch_self<FSS_MODE_ESP_COMBO>(batch, ks, out, s); break;
     default: launch_self<FSS_MODE_SHAPE_PHARM>(batch, ks, out, s); break;
   }
   return check_launch("fss_self_overlap");
@@ -1132,6 +1251,7 @@ int fss_overlap_grad(const FssPairBatch* batch, const FssScoreParams* sp, int32_
     case FSS_MODE_SHAPE: FSS_POSE0(FSS_MODE_SHAPE); break;
     case FSS_MODE_SHAPE_ESP: FSS_POSE0(FSS_MODE_SHAPE_ESP); break;
     case FSS_MODE_PHARM: FSS_POSE0(FSS_MODE_PHARM); break;
+    case FSS_MODE_ESP_COMBO: FSS_POSE0(FSS_MODE_ESP_COMBO); break;
     default: FSS_POSE0(FSS_MODE_SHAPE_PHARM); break;
   }
 #undef FSS_POSE0
@@ -1157,6 +1277,7 @@ int fss_pose_loss_grad(const FssPairBatch* batch, const FssScoreParams* sp, int3
     case FSS_MODE_SHAPE: launch_pose<FSS_MODE_SHAPE, true, 1>(batch, ks, lp, n_poses, pose_pair, q, t, self_ov, invB, nullptr, nullptr, nullptr, nullptr, loss, grad, s); break;
     case FSS_MODE_SHAPE_ESP: launch_pose<FSS_MODE_SHAPE_ESP, true, 1>(batch, ks, lp, n_poses, pose_pair, q, t, self_ov, invB, nullptr, nullptr, nullptr, nullptr, loss, grad, s); break;
     case FSS_MODE_PHARM: launch_pose<FSS_MODE_PHARM, true, 1>(batch, ks, lp, n_poses, pose_pair, q, t, self_ov, invB, nullptr, nullptr, nullptr, nullptr, loss, grad, s); break;
+    case FSS_MODE_ESP_COMBO: launch_pose<FSS_MODE_ESP_COMBO, true, 1>(batch, ks, lp, n_poses, pose_pair, q, t, self_ov, invB, nullptr, nullptr, nullptr, nullptr, loss, grad, s); break;
     default: launch_pose<FSS_MODE_SHAPE_PHARM, true, 1>(batch, ks, lp, n_poses, pose_pair, q, t, self_ov, invB, nullptr, nullptr, nullptr, nullptr, loss, grad, s); break;
   }
   return check_launch("fss_pose_loss_grad");
@@ -1245,6 +1366,7 @@ int fss_align(const FssPairBatch* batch, const FssScoreParams* sp, const FssAlig
     case FSS_MODE_SHAPE: FSS_ALIGN(FSS_MODE_SHAPE); break;
     case FSS_MODE_SHAPE_ESP: FSS_ALIGN(FSS_MODE_SHAPE_ESP); break;
     case FSS_MODE_PHARM: FSS_ALIGN(FSS_MODE_PHARM); break;
+    case FSS_MODE_ESP_COMBO: FSS_ALIGN(FSS_MODE_ESP_COMBO); break;
     default: FSS_ALIGN(FSS_MODE_SHAPE_PHARM); break;
   }
 #undef FSS_ALIGN
@@ -1270,13 +1392,21 @@ int fss_seed_poses(const FssPairBatch* batch, int32_t flags, int32_t n_restarts,
                    const float* quat_table, float* q, float* t, void* stream) {
   if (!batch || !quat_table || !q || !t) return set_error(FSS_E_INVALID, "NULL seed arguments");
   if (n_restarts <= 0) return set_error(FSS_E_INVALID, "n_restarts must be > 0");
+  if ((flags & FSS_SEED_PHARM) && (flags & FSS_SEED_SURF))
+    return set_error(FSS_E_INVALID, "FSS_SEED_PHARM and FSS_SEED_SURF are exclusive");
+  // Surface seeds: the kernel reads the ShaEP channel's surface rows through the shape slots.
+  FssPairBatch b = *batch;
+  if (flags & FSS_SEED_SURF) {
+    b.ref_pts = batch->ref_srf; b.ref_rng = batch->ref_srng;
+    b.fit_pts = batch->fit_srf; b.fit_rng = batch->fit_srng;
+  }
   const bool use_pharm = flags & FSS_SEED_PHARM;
-  if (use_pharm ? (!batch->ref_prng || !batch->fit_prng || !batch->ref_anc || !batch->fit_anc)
-                : (!batch->ref_rng || !batch->fit_rng || !batch->ref_pts || !batch->fit_pts))
+  if (use_pharm ? (!b.ref_prng || !b.fit_prng || !b.ref_anc || !b.fit_anc)
+                : (!b.ref_rng || !b.fit_rng || !b.ref_pts || !b.fit_pts))
     return set_error(FSS_E_INVALID, "cloud arrays are NULL");
-  if (batch->n_pairs <= 0) return FSS_OK;
-  hipLaunchKernelGGL(k_seed_poses, dim3(batch->n_pairs), dim3(64), 0, (hipStream_t)stream, *batch,
-                     (int)flags, n_restarts, quat_table, q, t);
+  if (b.n_pairs <= 0) return FSS_OK;
+  hipLaunchKernelGGL(k_seed_poses, dim3(b.n_pairs), dim3(64), 0, (hipStream_t)stream, b,
+                     (int)(flags & (FSS_SEED_PHARM | FSS_SEED_REF)), n_restarts, quat_table, q, t);
   return check_launch("fss_seed_poses");
 }
 

@@ -20,9 +20,8 @@ MODE_SEEDS = {"vol": 10, "surf": 8, "surf_esp": 8, "vol_esp": 16, "vol_and_surf_
               "pharm": 32, "vol_color": 16}
 MODE_STEPS = {"vol": 30, "surf": 40, "surf_esp": 40, "vol_esp": 50, "vol_and_surf_esp": 60,
               "pharm": 50, "vol_color": 40}
-# Modes the HIP backend implements (vol_and_surf_esp needs the ShaEP ESP comparison kernel,
-# which is not part of this hot path).
-HIP_MODES = ("vol", "vol_esp", "surf", "surf_esp", "pharm", "vol_color")
+# Modes the HIP backend implements.
+HIP_MODES = ("vol", "vol_esp", "surf", "surf_esp", "pharm", "vol_color", "vol_and_surf_esp")
 
 
 def canonical(mode: str) -> str:

@@ -124,6 +124,30 @@ class PharmSet:
                    torch.from_numpy(typ).to(dev), torch.from_numpy(rng).to(dev), counts)
 
 
+@dataclass
+class ShaepSet:
+    """Packed ShaEP channel of a set of molecules: surface observer points with their ESP
+    (x, y, z, esp) and atoms including hydrogens with partial charges (x, y, z, q) and vdW radii."""
+    srf: CloudSet              # surface rows, w = ESP at the point
+    atm: CloudSet              # atom rows, w = partial charge
+    rad: torch.Tensor          # (atom rows,) float32 vdW radii
+
+    @property
+    def n(self) -> int:
+        return self.srf.n
+
+    @classmethod
+    def from_arrays(cls, surf: Sequence, surf_esp: Sequence, atoms: Sequence, charges: Sequence,
+                    radii: Sequence, device="cuda") -> "ShaepSet":
+        srf = CloudSet.from_arrays(surf, surf_esp, device=device)
+        atm = CloudSet.from_arrays(atoms, charges, device=device)
+        rows = max(int(atm.counts.sum()), 1)
+        r = np.zeros(rows, np.float32)
+        if int(atm.counts.sum()):
+            r[:int(atm.counts.sum())] = np.concatenate([_as_np(x).reshape(-1) for x in radii])
+        return cls(srf, atm, torch.from_numpy(r).to(device))
+
+
 class PairBatch:
     """Pairs (ref cloud i, fit cloud j) over packed clouds, ready for the C ABI.
 
@@ -134,8 +158,10 @@ class PairBatch:
 
     def __init__(self, ref: Optional[CloudSet] = None, fit: Optional[CloudSet] = None,
                  ref_ph: Optional[PharmSet] = None, fit_ph: Optional[PharmSet] = None,
-                 ref_idx=None, fit_idx=None, n_pairs: Optional[int] = None):
+                 ref_idx=None, fit_idx=None, n_pairs: Optional[int] = None,
+                 ref_se: Optional[ShaepSet] = None, fit_se: Optional[ShaepSet] = None):
         self.ref, self.fit, self.ref_ph, self.fit_ph = ref, fit, ref_ph, fit_ph
+        self.ref_se, self.fit_se = ref_se, fit_se
         some = fit if fit is not None else fit_ph
         if some is None:
             raise ValueError("a PairBatch needs a fit cloud set")
@@ -161,6 +187,11 @@ class PairBatch:
         self.fit_rng, self.fit_counts = pair_rows(fit, self.fit_idx)
         self.ref_prng, self.ref_pcounts = pair_rows(ref_ph, self.ref_idx)
         self.fit_prng, self.fit_pcounts = pair_rows(fit_ph, self.fit_idx)
+        se = lambda x, k: None if x is None else getattr(x, k)
+        self.ref_srng, self.ref_scounts = pair_rows(se(ref_se, "srf"), self.ref_idx)
+        self.fit_srng, self.fit_scounts = pair_rows(se(fit_se, "srf"), self.fit_idx)
+        self.ref_arng, self.ref_acounts = pair_rows(se(ref_se, "atm"), self.ref_idx)
+        self.fit_arng, self.fit_acounts = pair_rows(se(fit_se, "atm"), self.fit_idx)
 
     def _max(self, c):
         return int(c.max()) if c is not None and len(c) else 0
@@ -181,6 +212,14 @@ class PairBatch:
             s.fit_typ, s.fit_prng = p(self.fit_ph.typ), p(self.fit_prng)
         s.max_ref, s.max_fit = self._max(self.ref_counts), self._max(self.fit_counts)
         s.max_ref_ph, s.max_fit_ph = self._max(self.ref_pcounts), self._max(self.fit_pcounts)
+        if self.ref_se is not None:
+            s.ref_srf, s.ref_srng = p(self.ref_se.srf.pts), p(self.ref_srng)
+            s.ref_atm, s.ref_rad, s.ref_arng = p(self.ref_se.atm.pts), p(self.ref_se.rad), p(self.ref_arng)
+        if self.fit_se is not None:
+            s.fit_srf, s.fit_srng = p(self.fit_se.srf.pts), p(self.fit_srng)
+            s.fit_atm, s.fit_rad, s.fit_arng = p(self.fit_se.atm.pts), p(self.fit_se.rad), p(self.fit_arng)
+        s.max_ref_srf, s.max_fit_srf = self._max(self.ref_scounts), self._max(self.fit_scounts)
+        s.max_ref_atm, s.max_fit_atm = self._max(self.ref_acounts), self._max(self.fit_acounts)
         return s
 
     def cost(self, mode: int) -> np.ndarray:
@@ -190,4 +229,7 @@ class PairBatch:
             c += self.ref_counts.astype(np.float64) * self.fit_counts
         if mode in (_lib.MODE_PHARM, _lib.MODE_SHAPE_PHARM) and self.ref_pcounts is not None:
             c += self.ref_pcounts.astype(np.float64) * self.fit_pcounts
+        if mode == _lib.MODE_ESP_COMBO and self.ref_scounts is not None:   # two passes, two directions
+            c += 2.0 * (self.ref_scounts.astype(np.float64) * self.fit_acounts
+                        + self.fit_scounts.astype(np.float64) * self.ref_acounts)
         return c

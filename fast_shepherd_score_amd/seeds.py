@@ -165,10 +165,11 @@ def reference_table(S: int) -> np.ndarray:
     return q
 
 
-def device_seeds(pb, num_repeats: int, use_pharm: bool = False, stream=None):
+def device_seeds(pb, num_repeats: int, use_pharm: bool = False, stream=None, use_surface: bool = False):
     """Seeds of every pair of a PairBatch on its device (fss_seed_poses with the reference rule):
     (q (n, S, 4), t (n, S, 3)) float32 tensors, S = effective_restarts(num_repeats).  The
-    centroids and principal axes come from the pharmacophore anchors when use_pharm."""
+    centroids and principal axes come from the pharmacophore anchors when use_pharm, from the
+    ShaEP channel's surface points when use_surface."""
     import torch
     from . import _lib
     S = effective_restarts(num_repeats)
@@ -178,7 +179,7 @@ def device_seeds(pb, num_repeats: int, use_pharm: bool = False, stream=None):
     t = torch.empty((pb.n_pairs, S, 3), dtype=torch.float32, device=dev)
     if pb.n_pairs:
         st = stream if stream is not None else torch.cuda.current_stream(dev)
-        flags = _lib.SEED_REF | (_lib.SEED_PHARM if use_pharm else 0)
+        flags = _lib.SEED_REF | (_lib.SEED_PHARM if use_pharm else 0) | (_lib.SEED_SURF if use_surface else 0)
         bs = pb.struct()
         _lib.call("fss_seed_poses", _lib.byref(bs), flags, S, _lib.ptr(table), _lib.ptr(q),
                   _lib.ptr(t), int(st.cuda_stream))

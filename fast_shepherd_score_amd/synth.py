@@ -84,6 +84,42 @@ def pharmacophores(rng: np.random.Generator, atoms: np.ndarray, n_ph: int):
     return types.astype(np.int64), anchors.astype(np.float32), vecs.astype(np.float32)
 
 
+H_BOND = 1.09
+VDW_H = 1.1
+SHAEP_PROBE = 1.2
+COULOMB = 1e4 / (4 * 55.263 * np.pi)
+
+
+def shaep_molecule(rng: np.random.Generator, n_heavy: int, n_surf: int) -> dict:
+    """A molecule for the ShaEP (vol_and_surf_esp) score: random-walk heavy atoms (vdW 1.7 A),
+    one or two hydrogens (1.09 A bonds, vdW 1.1 A) on each, partial charges N(0, 0.25) shifted to
+    sum to zero, surface points uniform on the exposed union of (radius + 1.2 A) spheres, and
+    the molecule's own Coulomb ESP at each surface point (COULOMB * sum q / d)."""
+    heavy = random_walks(rng, [n_heavy])[0].astype(np.float64)
+    nh = rng.integers(1, 3, size=n_heavy)
+    hyd = np.concatenate([heavy[i] + H_BOND * _unit(rng, (nh[i],)) for i in range(n_heavy)])
+    atoms = np.concatenate([heavy, hyd])
+    radii = np.concatenate([np.full(n_heavy, VDW), np.full(len(hyd), VDW_H)])
+    q = rng.normal(0.0, 0.25, size=len(atoms))
+    q -= q.mean()
+    rs = radii + SHAEP_PROBE
+    got, need = [], n_surf
+    while need > 0:
+        m = max(2 * need, 64)
+        which = rng.integers(0, len(atoms), size=m)
+        pts = atoms[which] + rs[which, None] * _unit(rng, (m,))
+        d = np.linalg.norm(pts[:, None, :] - atoms[None, :, :], axis=-1)
+        d[np.arange(m), which] = np.inf
+        keep = pts[(d >= rs[None, :] - 1e-9).all(axis=1)]
+        got.append(keep[:need])
+        need -= len(got[-1])
+    surf = np.concatenate(got)
+    esp = COULOMB * (q[None, :] / np.linalg.norm(surf[:, None, :] - atoms[None, :, :], axis=-1)).sum(1)
+    f = lambda a: np.asarray(a, np.float32)
+    return dict(atom_pos=f(heavy), centers_w_H=f(atoms), partial_charges=f(q), radii=f(radii),
+                surf_pos=f(surf), surf_esp=f(esp))
+
+
 def random_quaternions(rng: np.random.Generator, n: int) -> np.ndarray:
     """Uniform random unit quaternions (normalised 4-D Gaussian), (n, 4) float32."""
     q = rng.normal(size=(n, 4))

@@ -36,6 +36,8 @@ extern "C" {
 #define FSS_MODE_SHAPE_ESP    1  /* ESP-weighted shape Tanimoto (surf_esp / vol_esp)        */
 #define FSS_MODE_PHARM        2  /* typed, directional pharmacophore Tanimoto / Tversky     */
 #define FSS_MODE_SHAPE_PHARM  3  /* (1-w)*shape Tanimoto + w*pharm similarity (vol_color)   */
+#define FSS_MODE_ESP_COMBO    4  /* ShaEP: (1-w)*shape Tanimoto + w*surface ESP comparison  *
+                                  * (vol_and_surf_esp); needs the ShaEP channel below         */
 
 /* Pharmacophore similarity. */
 #define FSS_SIM_TANIMOTO 0
@@ -69,6 +71,20 @@ typedef struct FssPairBatch {
   const int32_t* fit_prng;
   /* host-side maxima over the pairs (size the LDS staging); must be >= every count */
   int32_t max_ref, max_fit, max_ref_ph, max_fit_ph;
+  /* ShaEP channel (FSS_MODE_ESP_COMBO only; may be NULL otherwise): observer surface points
+   * (x, y, z, precomputed ESP) and source atoms including hydrogens (x, y, z, partial charge)
+   * with their vdW radii. */
+  const float*   ref_srf;    /* (rows, 4) */
+  const float*   fit_srf;    /* (rows, 4) */
+  const int32_t* ref_srng;   /* (n_pairs, 2) */
+  const int32_t* fit_srng;   /* (n_pairs, 2) */
+  const float*   ref_atm;    /* (rows, 4) */
+  const float*   fit_atm;    /* (rows, 4) */
+  const float*   ref_rad;    /* (rows) */
+  const float*   fit_rad;    /* (rows) */
+  const int32_t* ref_arng;   /* (n_pairs, 2) */
+  const int32_t* fit_arng;   /* (n_pairs, 2) */
+  int32_t max_ref_srf, max_fit_srf, max_ref_atm, max_fit_atm;
 } FssPairBatch;
 
 /* Scoring constants shared by every entry point. */
@@ -76,12 +92,16 @@ typedef struct FssScoreParams {
   int32_t mode;              /* FSS_MODE_*                                                   */
   int32_t similarity;        /* FSS_SIM_* for the pharmacophore channel                     */
   double  alpha;             /* shape Gaussian width (0.81 volumetric)                       */
-  double  lam;               /* ESP lambda, already scaled (surface: 0.3*LAM_SCALING)        */
+  double  lam;               /* ESP lambda, already scaled (surface: 0.3*LAM_SCALING); for
+                                FSS_MODE_ESP_COMBO the comparison's lambda (LAM_SCALING*lam) */
   double  color_weight;      /* w of FSS_MODE_SHAPE_PHARM                                    */
   double  sigma;             /* Tversky sigma                                                */
   double  ph_alpha[FSS_N_PTYPES];   /* per-type pharmacophore alpha                          */
   double  ph_k[FSS_N_PTYPES];       /* per-type (pi/(2 alpha))^1.5                           */
   int32_t ph_cat[FSS_N_PTYPES];     /* 0 isotropic, 1 directional, 2 aromatic, 3 ignored     */
+  double  esp_weight;        /* w of FSS_MODE_ESP_COMBO                                      */
+  double  probe_radius;      /* observer points within radius + probe of an atom are masked  */
+  double  coulomb;           /* ESP = coulomb * sum q / d (1e4 / (4 * 55.263 * pi))          */
 } FssScoreParams;
 
 typedef struct FssAlignParams {
@@ -118,7 +138,8 @@ int fss_device_count(int* count);
 /*
  * Self-overlaps VAA and VBB of every pair at the identity pose (value only).
  * out: (n_pairs, 4) float64 = [VAA_shape, VBB_shape, VAA_pharm, VBB_pharm], evaluated with FP64
- * arithmetic; channels the mode does not use are written as 0.
+ * arithmetic; channels the mode does not use are written as 0.  FSS_MODE_ESP_COMBO writes the
+ * two surface point counts (the ESP comparison's normaliser) in place of the pharm pair.
  * Replaces: accel/kernels/shape_triton.py:436-448 (_batch_self_overlap),
  *           accel/kernels/esp_triton.py:393-425 (_batch_self_overlap_esp),
  *           score/analytical_gradients/_torch.py:364-389, 519-541, 569-599.
@@ -139,6 +160,9 @@ int fss_self_overlap(const FssPairBatch* batch, const FssScoreParams* sp, double
  *   pharm_grad_dq_se3_batch (pharm_triton.py:403-430), pharm_score_grad_se3_batch
  *   (pharm_triton.py:433-467); arithmetic of ag/_torch.py:223-361 and :441-516.
  */
+/* For FSS_MODE_ESP_COMBO the second channel is the ShaEP ESP comparison esp_1 + esp_2
+ *   (score/electrostatic_scoring.py:128-227) and its analytic gradient; the mask of observer
+ *   points inside the other molecule's radius + probe volume has zero gradient. */
 int fss_overlap_grad(const FssPairBatch* batch, const FssScoreParams* sp, int32_t n_poses,
                      const int32_t* pose_pair, const float* q, const float* t, int32_t need_grad,
                      float* V, float* gR, float* gT, float* gQ, void* stream);
@@ -209,6 +233,7 @@ int fss_align_finish(const FssPairBatch* batch, const FssScoreParams* sp, int32_
  * Restart seeds on device (one wave per pair): pose r gets quat_table[r] (n_restarts x 4, w x y z)
  * and t = centroid(ref) - R(q) centroid(fit).  flags:
  *   FSS_SEED_PHARM  centroids (and axes) of the pharmacophore anchors instead of the shape clouds;
+ *   FSS_SEED_SURF   ... of the ShaEP channel's surface points (vol_and_surf_esp seeds);
  *   FSS_SEED_REF    the reference's seed rule: row 0 is the identity, rows 1-4 the principal-axis
  *                   alignments of the fit onto the ref (FP64 Jacobi eigensolver; n_restarts >= 5),
  *                   and n_restarts == 1 means identity with zero translation.
@@ -218,6 +243,7 @@ int fss_align_finish(const FssPairBatch* batch, const FssScoreParams* sp, int32_
  */
 #define FSS_SEED_PHARM 1
 #define FSS_SEED_REF   2
+#define FSS_SEED_SURF  4      /* centroids and axes of the ShaEP channel's surface points */
 int fss_seed_poses(const FssPairBatch* batch, int32_t flags, int32_t n_restarts,
                    const float* quat_table, float* q, float* t, void* stream);
 

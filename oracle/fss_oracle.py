@@ -12,6 +12,8 @@ A restatement, in torch float64 on the CPU, of:
     |dL| <= 1e-5 for > 10 steps early stop on the restart-mean loss, final re-scoring, argmax);
   * alignment/_torch.py:1296-1548 (the shape + pharmacophore combination, objective 1 - mean of
     (1-w) shape Tanimoto + w pharm similarity, restated with the analytic gradients above);
+  * the ShaEP combination score/electrostatic_scoring.py:128-320 with its autograd optimiser
+    alignment/_torch.py:804-1002 (ComboPair);
   * the final re-scoring functions score/gaussian_overlap.py:16-106,
     score/electrostatic_scoring.py:17-125, score/pharmacophore_scoring.py:123-145, 422-912,
     score/gaussian_overlap.py:283-363 and alignment/utils/se3.py:16-189, restated separately
@@ -413,6 +415,98 @@ def pharm_similarity(t1, t2, a1, a2, v1, v2, similarity="tanimoto", directionles
 # ------------------------------------------------------------------------------------------
 # The optimiser (alignment/_torch_analytical.py:23-478)
 # ------------------------------------------------------------------------------------------
+# ------------------------------------------------------------------------------------------
+# ShaEP combination (vol_and_surf_esp): score/electrostatic_scoring.py:128-320 and the
+# autograd optimiser alignment/_torch.py:804-1002
+# ------------------------------------------------------------------------------------------
+COULOMB_SCALING = 1e4 / (4 * 55.263 * np.pi)               # score/constants.py:16
+
+
+def esp_comparison(points, atoms, charges, point_esp, radii, probe: float, lam_eff: float):
+    """sum_i mask_i exp(-(pe_i - C sum_m q_m / d_im)^2 / lam_eff), batched: points (B,N,3),
+    atoms (B,M,3), charges (M,), point_esp (N,), radii (M,) -> (B,).  mask_i = 1 iff every
+    d_im >= radius_m + probe (electrostatic_scoring.py:128-227; lam_eff = LAM_SCALING * lam)."""
+    d = torch.cdist(points, atoms)
+    mask = (d >= radii[None, None, :] + probe).all(dim=2).to(d.dtype)
+    phi = COULOMB_SCALING * (charges[None, None, :] / d).sum(dim=2)
+    return (mask * torch.exp(-(point_esp[None, :] - phi) ** 2 / lam_eff)).sum(dim=1)
+
+
+@dataclass
+class ComboPair:
+    """One pair under the ShaEP score (1-w) shape Tanimoto + w (esp_1 + esp_2) / (N_s + M_s).
+
+    Shape centres (heavy atoms, or surface points when alpha != 0.81), surface points with their
+    ESP, atoms including hydrogens with partial charges and vdW radii, for ref and fit.  ``lam`` is
+    the raw lambda (LAM_SCALING is applied here).  The gradient is torch autograd of the score,
+    as in the reference's optimiser."""
+    ref_centers: np.ndarray
+    fit_centers: np.ndarray
+    ref_surf: np.ndarray
+    fit_surf: np.ndarray
+    ref_surf_esp: np.ndarray
+    fit_surf_esp: np.ndarray
+    ref_atoms: np.ndarray
+    fit_atoms: np.ndarray
+    ref_charges: np.ndarray
+    fit_charges: np.ndarray
+    ref_radii: np.ndarray
+    fit_radii: np.ndarray
+    alpha: float = 0.81
+    lam: float = 0.001
+    probe_radius: float = 1.0
+    esp_weight: float = 0.5
+    mode: str = "esp_combo"
+
+    def __post_init__(self):
+        for k in ("ref_centers", "fit_centers", "ref_surf", "fit_surf", "ref_surf_esp", "fit_surf_esp",
+                  "ref_atoms", "fit_atoms", "ref_charges", "fit_charges", "ref_radii", "fit_radii"):
+            setattr(self, "t" + k, torch.as_tensor(np.asarray(getattr(self, k), np.float64), dtype=DT))
+        self.has_shape, self.has_pharm = True, False
+
+    def self_overlaps(self):
+        return {}
+
+    def score_poses(self, se3):
+        """Per-pose ShaEP score (B,) of poses se3 (B,7) (differentiable in se3)."""
+        T = se3_matrix(se3)
+        B = se3.shape[0]
+        e = lambda x: x.expand(B, *x.shape)
+        shape = tanimoto_shape(e(self.tref_centers), apply_se3(e(self.tfit_centers), T), self.alpha)
+        lam_eff = LAM_SCALING * self.lam
+        esp1 = esp_comparison(e(self.tref_surf), apply_se3(e(self.tfit_atoms), T), self.tfit_charges,
+                              self.tref_surf_esp, self.tfit_radii, self.probe_radius, lam_eff)
+        esp2 = esp_comparison(apply_se3(e(self.tfit_surf), T), e(self.tref_atoms), self.tref_charges,
+                              self.tfit_surf_esp, self.tref_radii, self.probe_radius, lam_eff)
+        nrm = float(len(self.ref_surf) + len(self.fit_surf))
+        return self.esp_weight * (esp1 + esp2) / nrm + (1.0 - self.esp_weight) * shape
+
+    @torch.enable_grad()
+    def esp_value_grad(self, R, t):
+        """esp_1 + esp_2 (B,) and its gradient w.r.t. R (B,3,3) and t (B,3) by autograd."""
+        R = R.detach().clone().requires_grad_(True)
+        t = t.detach().clone().requires_grad_(True)
+        B = R.shape[0]
+        e = lambda x: x.expand(B, *x.shape)
+        lam_eff = LAM_SCALING * self.lam
+        fa = e(self.tfit_atoms) @ R.transpose(1, 2) + t[:, None, :]
+        fs = e(self.tfit_surf) @ R.transpose(1, 2) + t[:, None, :]
+        v = esp_comparison(e(self.tref_surf), fa, self.tfit_charges, self.tref_surf_esp, self.tfit_radii,
+                           self.probe_radius, lam_eff) + \
+            esp_comparison(fs, e(self.tref_atoms), self.tref_charges, self.tfit_surf_esp, self.tref_radii,
+                           self.probe_radius, lam_eff)
+        gR, gt = torch.autograd.grad(v.sum(), (R, t))
+        return v.detach(), gR, gt
+
+    @torch.enable_grad()
+    def loss_grad_se3(self, se3, so):
+        x = se3.detach().clone().requires_grad_(True)
+        per = 1.0 - self.score_poses(x)
+        loss = per.mean()
+        (g,) = torch.autograd.grad(loss, x)
+        return loss.detach(), g, per.detach()
+
+
 def optimize(pair: Pair, q0, t0, lr: float = 0.1, max_steps: int = 200, early_stop: bool = True,
              es_tol: float = 1e-5, es_patience: int = 10):
     """Multi-restart alignment with explicit seeds q0 (B,4), t0 (B,3).

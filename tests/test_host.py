@@ -8,7 +8,7 @@ from fast_shepherd_score_amd.align import (ScoreSpec, candidate_tilings, check_t
                                            tiling_efficiency)
 from fast_shepherd_score_amd.constants import ALPHA, LAM_SCALING, P_TYPES, pharm_tables
 from fast_shepherd_score_amd.container import MoleculePairBatch, resolve_backend
-from fast_shepherd_score_amd.modes import MODE_ATTRS, MODE_SEEDS, MODE_STEPS, canonical
+from fast_shepherd_score_amd.modes import HIP_MODES, MODE_ATTRS, MODE_SEEDS, MODE_STEPS, canonical
 from fast_shepherd_score_amd.packing import CloudSet, PairBatch, PharmSet
 from fast_shepherd_score_amd.seeds import _principal_axes, initial_seeds, pad, quats_from_fibo, rotmat
 
@@ -150,5 +150,8 @@ def test_mode_registry_and_backend_names():
     for b in ("numba", "cpu", "jax", "nope"):
         with pytest.raises(ValueError):
             resolve_backend(b)
+    # every registered mode is served by the HIP backend, through the legacy aliases too
+    assert set(HIP_MODES) == set(MODE_ATTRS)
+    assert MoleculePairBatch.align_with_esp_combo is MoleculePairBatch.align_with_vol_and_surf_esp
     with pytest.raises(NotImplementedError):
-        MoleculePairBatch([]).align_with_vol_and_surf_esp()
+        MoleculePairBatch([]).align_with_vol_and_surf_esp(0.81, trans_init=True)

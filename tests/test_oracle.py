@@ -257,3 +257,129 @@ def test_fibo45_table_is_fibonacci_construction():
     """The reference's 45-seed table equals the Fibonacci-sphere construction (to float32)."""
     from fast_shepherd_score_amd.seeds import quats_from_fibo
     np.testing.assert_allclose(quats_from_fibo(45), np.array(GOLD["fibo45"]), atol=2e-7)
+
+
+# ---- ShaEP combination (vol_and_surf_esp) -------------------------------------------------
+def _combo_pair(seed, n1=8, n2=10, s1=40, s2=50, w=0.5, alpha=0.81):
+    from fast_shepherd_score_amd import synth
+    rng = np.random.default_rng(seed)
+    a, b = synth.shaep_molecule(rng, n1, s1), synth.shaep_molecule(rng, n2, s2)
+    centres = (lambda m: m["atom_pos"]) if alpha == 0.81 else (lambda m: m["surf_pos"])
+    return orc.ComboPair(ref_centers=centres(a), fit_centers=centres(b), ref_surf=a["surf_pos"],
+                         fit_surf=b["surf_pos"], ref_surf_esp=a["surf_esp"], fit_surf_esp=b["surf_esp"],
+                         ref_atoms=a["centers_w_H"], fit_atoms=b["centers_w_H"],
+                         ref_charges=a["partial_charges"], fit_charges=b["partial_charges"],
+                         ref_radii=a["radii"], fit_radii=b["radii"], alpha=alpha, esp_weight=w), a, b
+
+
+def test_esp_comparison_matches_brute_force():
+    import math
+    rng = np.random.default_rng(3)
+    P = rng.normal(size=(7, 3)) * 3
+    A = rng.normal(size=(5, 3)) * 3
+    q, pe, r = rng.normal(size=5) * 0.3, rng.normal(size=7), 0.5 + rng.random(5)
+    probe, lam = 0.4, 0.2
+    want = 0.0
+    for i in range(7):
+        phi, blocked = 0.0, False
+        for m in range(5):
+            d = math.sqrt(sum((P[i, k] - A[m, k]) ** 2 for k in range(3)))
+            blocked = blocked or d < r[m] + probe
+            phi += q[m] / d
+        if not blocked:
+            want += math.exp(-(pe[i] - orc.COULOMB_SCALING * phi) ** 2 / lam)
+    t = lambda x: torch.tensor(x, dtype=torch.float64)
+    got = orc.esp_comparison(t(P)[None], t(A)[None], t(q), t(pe), t(r), probe, lam)
+    assert abs(float(got[0]) - want) < 1e-12 and want > 0
+
+
+def test_esp_comparison_closed_forms_and_mask():
+    t = lambda x: torch.tensor(x, dtype=torch.float64)
+    C = orc.COULOMB_SCALING
+    # one observer 4 A from one atom of charge 0.2, radius 1.5, probe 1.0: unmasked
+    v = orc.esp_comparison(t([[[4.0, 0, 0]]]), t([[[0.0, 0, 0]]]), t([0.2]), t([0.5]), t([1.5]), 1.0, 0.3)
+    assert abs(float(v[0]) - np.exp(-(0.5 - C * 0.2 / 4.0) ** 2 / 0.3)) < 1e-14
+    # the same observer at 2.4 A < 1.5 + 1.0 is inside the volume: masked out
+    v = orc.esp_comparison(t([[[2.4, 0, 0]]]), t([[[0.0, 0, 0]]]), t([0.2]), t([0.5]), t([1.5]), 1.0, 0.3)
+    assert float(v[0]) == 0.0
+
+
+def test_combo_weights_select_the_channels():
+    pair, a, b = _combo_pair(5, w=0.0)
+    se3 = torch.tensor([[1.0, 0, 0, 0, 0.5, -0.3, 0.2], [0.9, 0.1, -0.2, 0.3, 0, 0, 0]], dtype=torch.float64)
+    T = orc.se3_matrix(se3)
+    shape = orc.tanimoto_shape(pair.tref_centers.expand(2, -1, -1),
+                               orc.apply_se3(pair.tfit_centers.expand(2, -1, -1), T), 0.81)
+    assert torch.allclose(pair.score_poses(se3), shape, atol=1e-14)
+    pair1, _, _ = _combo_pair(5, w=1.0)
+    R, tt = T[:, :3, :3], T[:, :3, 3]
+    v, _, _ = pair1.esp_value_grad(R, tt)
+    assert torch.allclose(pair1.score_poses(se3), v / (40 + 50), atol=1e-14)
+
+
+def test_combo_gradients_match_finite_differences():
+    pair, _, _ = _combo_pair(7)
+    rng = np.random.default_rng(8)
+    q = torch.tensor(rng.normal(size=(3, 4)), dtype=torch.float64)
+    se3 = torch.cat([q, torch.tensor(rng.normal(size=(3, 3)) * 0.5, dtype=torch.float64)], 1)
+    loss, g, per = pair.loss_grad_se3(se3, {})
+    h = 1e-6
+    for k in range(7):
+        e = torch.zeros_like(se3)
+        e[:, k] = h
+        fd = (pair.loss_grad_se3(se3 + e, {})[0] - pair.loss_grad_se3(se3 - e, {})[0]) / (2 * h)
+        assert abs(float(fd) - float(g[:, k].sum())) < 1e-6 * max(1.0, abs(float(fd)))
+    # channel gradient w.r.t. R and t
+    T = orc.se3_matrix(se3)
+    R, t = T[:, :3, :3], T[:, :3, 3]
+    v, gR, gt = pair.esp_value_grad(R, t)
+    for k in range(3):
+        e = torch.zeros_like(t)
+        e[:, k] = h
+        fd = (pair.esp_value_grad(R, t + e)[0] - pair.esp_value_grad(R, t - e)[0]) / (2 * h)
+        np.testing.assert_allclose(fd.numpy(), gt[:, k].numpy(), rtol=1e-6, atol=1e-8)
+
+
+def test_combo_score_is_rigid_motion_invariant():
+    pair, a, b = _combo_pair(9)
+    se3 = torch.tensor([[0.8, 0.2, -0.1, 0.4, 0.3, 0.1, -0.5]], dtype=torch.float64)
+    s0 = pair.score_poses(se3)
+    # move the ref by a rigid motion G and compose the fit pose with G: same score
+    G = orc.se3_matrix(torch.tensor([[0.3, -0.7, 0.2, 0.5, 1.0, -2.0, 0.5]], dtype=torch.float64))[0]
+    mv = lambda X: (torch.as_tensor(X, dtype=torch.float64) @ G[:3, :3].T + G[:3, 3]).numpy()
+    moved = orc.ComboPair(ref_centers=mv(a["atom_pos"]), fit_centers=b["atom_pos"], ref_surf=mv(a["surf_pos"]),
+                          fit_surf=b["surf_pos"], ref_surf_esp=a["surf_esp"], fit_surf_esp=b["surf_esp"],
+                          ref_atoms=mv(a["centers_w_H"]), fit_atoms=b["centers_w_H"],
+                          ref_charges=a["partial_charges"], fit_charges=b["partial_charges"],
+                          ref_radii=a["radii"], fit_radii=b["radii"])
+    T = G @ orc.se3_matrix(se3)[0]
+    # back to (q, t): the rotation of T as a quaternion
+    from scipy.spatial.transform import Rotation
+    x, y, z, w = Rotation.from_matrix(T[:3, :3].numpy()).as_quat()
+    se3b = torch.tensor([[w, x, y, z, *T[:3, 3].tolist()]], dtype=torch.float64)
+    assert torch.allclose(moved.score_poses(se3b), s0, atol=1e-10)
+
+
+def test_combo_optimiser_recovers_a_rigid_copy():
+    """orc.optimize on the ShaEP objective (autograd gradient inside the no-grad optimiser loop):
+    a rotated, shifted copy of a molecule is aligned back to score ~1 (shape and ESP alike)."""
+    from fast_shepherd_score_amd import synth
+    m = synth.shaep_molecule(np.random.default_rng(12), 9, 60)
+    Rt = orc.rot_from_unit_quat(torch.tensor([[0.9, 0.2, -0.3, 0.25]], dtype=torch.float64) /
+                                torch.tensor([[0.9, 0.2, -0.3, 0.25]], dtype=torch.float64).norm())[0].numpy()
+    tr = np.array([0.7, -0.4, 0.3])
+    mv = lambda X: X.astype(np.float64) @ Rt.T + tr
+    pair = orc.ComboPair(ref_centers=m["atom_pos"], fit_centers=mv(m["atom_pos"]), ref_surf=m["surf_pos"],
+                         fit_surf=mv(m["surf_pos"]), ref_surf_esp=m["surf_esp"], fit_surf_esp=m["surf_esp"],
+                         ref_atoms=m["centers_w_H"], fit_atoms=mv(m["centers_w_H"]),
+                         ref_charges=m["partial_charges"], fit_charges=m["partial_charges"],
+                         ref_radii=m["radii"], fit_radii=m["radii"])
+    # start from the exact inverse pose, slightly perturbed
+    Rinv = Rt.T
+    from scipy.spatial.transform import Rotation
+    x, y, z, w = Rotation.from_matrix(Rinv).as_quat()
+    q0 = np.array([[w, x, y, z], [w + 0.05, x - 0.04, y, z + 0.03]])
+    t0 = np.stack([-Rinv @ tr, -Rinv @ tr + 0.2])
+    o = orc.optimize(pair, q0, t0, lr=0.02, max_steps=60, early_stop=False)
+    assert o["score"] > 0.999
+    assert o["scores"][1] > 0.99
