@@ -31,14 +31,13 @@ PEAK_FP32_TFLOPS = 157.3      # MI355X FP32 peak, vector and matrix pipes alike 
 TRAFFIC_TABLE = os.path.join(ROOT, "profiles", "k_align_traffic.json")
 
 
-def measured_traffic(key: str):
-    """HBM bytes per k_align launch measured with rocprofv3 for this exact configuration, or None."""
+def measured_row(key: str) -> dict:
+    """rocprofv3 measurements (HBM bytes, VALU issue) of k_align for this exact configuration."""
     try:
         with open(TRAFFIC_TABLE) as f:
-            row = json.load(f).get(key)
+            return json.load(f).get(key) or {}
     except (OSError, ValueError):
-        return None
-    return None if row is None else int(row["bytes"])
+        return {}
 PEAK_EXP_PER_S = 256 * 4 * 16 * 2.4e9 / 4   # v_exp_f32 at quarter rate: 9.8e12 / s
 
 
@@ -376,6 +375,8 @@ def main():
     achieved = flop / (kern_ms * 1e-3) / 1e12
     value = args.pairs / (dt / args.steps)
     mfma = res.extra["variant"] == 3
+    prof = measured_row(f"{args.workload}/{args.pairs}/{B}/{steps}/{res.extra['lanes_per_pose']}/"
+                        f"{res.extra['restarts_per_wg']}/{res.extra['variant']}")
     score = res.score.cpu().numpy()
     out = {
         "metric": "aligned pairs/sec @50 restarts (1 GPU, <=1e-3 vs FP64); screen p50/p99 ms",
@@ -390,13 +391,13 @@ def main():
                    "restarts_per_wg": res.extra["restarts_per_wg"],
                    "inner_loop": {0: "expanded", 1: "direct", 2: "expanded",
                                   3: "mfma-split-bf16"}[res.extra["variant"]]},
-        "roofline": {"bound": "mfma" if mfma else "valu", "achieved": achieved, "peak": PEAK_FP32_TFLOPS,
+        "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_FP32_TFLOPS,
                      "unit": "TFLOP/s", "frac": achieved / PEAK_FP32_TFLOPS,
-                     "traffic": measured_traffic(f"{args.workload}/{args.pairs}/{B}/{steps}/"
-                                                 f"{res.extra['lanes_per_pose']}/"
-                                                 f"{res.extra['restarts_per_wg']}/{res.extra['variant']}"),
-                     "compute": ("split-bf16 MFMA exponents (FP32 accumulate) + FP32 VALU exp and sums"
-                                 if mfma else "FP32 VALU"),
+                     "traffic": prof.get("bytes"),
+                     "valu_issue_frac": prof.get("valu_issue_frac"),
+                     "simd_busy_frac_est": prof.get("simd_busy_frac_est"),
+                     "compute": ("FP32 VALU exp and sums; exponents as exact split-bf16 products "
+                                 "accumulated in FP32 on the matrix cores" if mfma else "FP32 VALU"),
                      "kernel": "k_align", "kernel_ms": kern_ms, "finish_ms": fin_ms,
                      "pair_terms_per_launch": terms, "flop_per_term": FLOP_PER_TERM[wl["kind"]],
                      "exp_per_s": terms / (kern_ms * 1e-3),
