@@ -212,12 +212,63 @@ def bench_screen(dev, n_lib: int = 1_000_000, n_queries: int = 24, load: float =
         res = [f.result() for f in futs]
         total = time.perf_counter() - start
     lat = np.array([r.latency_ms for r in res])
+    # Each (query, conformer) pair stages the conformer's points from HBM once per launch.
+    streamed = n_queries * lib.nbytes()
     return {"workload": f"{n_lib} conformers (10-50 atoms) resident in HBM, vol shape, 10 restarts x "
                         f"100 steps, per-pair early stop, top-100; {n_queries} queries, Poisson arrivals "
                         f"at {load:.0%} of single-query capacity",
             "pairs_per_s": n_queries * n_lib / total, "p50_ms": float(np.percentile(lat, 50)),
             "p99_ms": float(np.percentile(lat, 99)), "solo_query_ms": solo * 1e3,
-            "library_bytes": lib.nbytes(), "launches": svc.launches}
+            "library_bytes": lib.nbytes(), "launches": svc.launches,
+            "hbm_library_gb_per_s": streamed / total / 1e9,
+            "hbm_frac": streamed / total / 8e12}
+
+
+def bench_score_loss(dev, n_pairs: int = 1024, n_points: int = 300, reps: int = 20, seed: int = 4):
+    """Score as a loss: FP32 forward S and backward dS/dfit (incl. dVBB) of 1024 surface+ESP pairs."""
+    import torch
+    from fast_shepherd_score_amd import synth
+    from fast_shepherd_score_amd.autograd import gaussian_similarity
+    from fast_shepherd_score_amd.constants import LAM_SCALING
+    from fast_shepherd_score_amd.seeds import rotmat
+    r, re, f, fe = synth.surf_esp_pairs(n_pairs, points=(n_points, n_points), seed=seed)
+    rng = np.random.default_rng(seed + 1)
+    R = rotmat(synth.random_quaternions(rng, n_pairs).astype(np.float64)).astype(np.float32)
+    fit = np.einsum("pij,pmj->pmi", R, np.stack(f)) + rng.normal(size=(n_pairs, 1, 3)).astype(np.float32)
+    to = lambda x: torch.from_numpy(np.ascontiguousarray(np.stack(x) if isinstance(x, list) else x,
+                                                         dtype=np.float32)).to(dev)
+    ref, rw, fw = to(r), to(re), to(fe)
+    fit = to(fit).requires_grad_(True)
+    lam = 0.3 * LAM_SCALING
+
+    def timed(fn):
+        fn()
+        torch.cuda.synchronize()
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        a.record()
+        for _ in range(reps):
+            fn()
+        b.record()
+        torch.cuda.synchronize()
+        return a.elapsed_time(b) / reps
+
+    def fwd():
+        with torch.no_grad():
+            gaussian_similarity(ref, fit, rw, fw, 0.81, lam)
+
+    def fwd_bwd():
+        fit.grad = None
+        gaussian_similarity(ref, fit, rw, fw, 0.81, lam).sum().backward()
+
+    ms_f, ms_fb = timed(fwd), timed(fwd_bwd)
+    N = M = n_points
+    exp_f, exp_b = n_pairs * (N * M + N * N + M * M), n_pairs * M * (N + M)
+    return {"workload": f"{n_pairs} pairs x {n_points} surface points + ESP (lam 0.3), random poses, "
+                        f"FP32 forward S and backward dS/dfit including dVBB",
+            "fwd_ms": ms_f, "fwd_bwd_ms": ms_fb, "fwd_pairs_per_s": n_pairs / (ms_f * 1e-3),
+            "fwd_bwd_pairs_per_s": n_pairs / (ms_fb * 1e-3),
+            "exp_terms_per_s": (exp_f + exp_b) / (ms_fb * 1e-3),
+            "exp_frac": (exp_f + exp_b) / (ms_fb * 1e-3) / PEAK_EXP_PER_S}
 
 
 def bench_shaep(dev, n_pairs: int = 2048, pool: int = 256, seed: int = 9):
@@ -417,6 +468,8 @@ def main():
         log("store done")
         out["shaep"] = bench_shaep(dev)
         log("shaep done")
+        out["score_loss"] = bench_score_loss(dev)
+        log("score_loss done")
     if not args.no_cpu:
         cb, cpu_scores = cpu_baseline(wl, q, t, B, steps, args.cpu_sample, spec, opt)
         cb.pop("seconds")

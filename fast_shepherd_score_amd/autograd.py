@@ -28,9 +28,19 @@ def _pack(x: torch.Tensor, w: Optional[torch.Tensor]) -> torch.Tensor:
     return torch.cat([x, col.to(x.dtype)], dim=-1).reshape(P * N, 4).contiguous()
 
 
+_RANGES: dict = {}
+
+
 def _ranges(P: int, N: int, dev) -> torch.Tensor:
-    st = torch.arange(P, dtype=torch.int32, device=dev) * N
-    return torch.stack([st, torch.full_like(st, N)], dim=1).contiguous()
+    """(P, 2) int32 [first row, count] of P clouds of N rows each, built once per shape and device."""
+    key = (P, N, str(dev))
+    r = _RANGES.get(key)
+    if r is None:
+        if len(_RANGES) > 64:
+            _RANGES.clear()
+        st = torch.arange(P, dtype=torch.int32, device=dev) * N
+        r = _RANGES[key] = torch.stack([st, torch.full_like(st, N)], dim=1).contiguous()
+    return r
 
 
 class _GaussianSimilarity(torch.autograd.Function):

@@ -826,112 +826,192 @@ __global__ void k_align_finish(int n_pairs, int B, const float* __restrict__ q,
 
 // ------------------------------------------------------------------------------------------------
 // Differentiable similarity of fixed clouds (shape or ESP shape), FP32 or FP64.
+//
+// Both clouds of a pair are staged in LDS as 4-D points p = (s x, s y, s z, s_w w), with
+// s = sqrt(c alpha / 2), s_w = sqrt(c / lam) (0 without ESP), c = log2(e) for FP32 (exp2) and 1
+// for FP64 (exp), after subtracting the pair's first ref point (every sum is translation
+// invariant).  Then w_ab E_ab = base^(-|p_a - p_b|^2): the ESP weight exp(-(w_a - w_b)^2 / lam)
+// is the fourth coordinate of one 4-D Gaussian, and each term costs 4 differences, a squared norm,
+// one exponential and the sums.
 // ------------------------------------------------------------------------------------------------
-template <typename T>
-__device__ __forceinline__ T gauss(T x);
-template <>
-__device__ __forceinline__ float gauss<float>(float x) { return fast_exp2(x * kLog2e); }
-template <>
-__device__ __forceinline__ double gauss<double>(double x) { return exp(x); }
+template <typename T> struct Vec4;
+template <> struct Vec4<float> { using type = float4; };
+template <> struct Vec4<double> { using type = double4; };
+
+template <typename T> __device__ __forceinline__ T neg_exp(T r2);
+template <> __device__ __forceinline__ float neg_exp<float>(float r2) { return fast_exp2(-r2); }
+template <> __device__ __forceinline__ double neg_exp<double>(double r2) { return exp(-r2); }
+template <typename T> constexpr T score_base() { return sizeof(T) == 4 ? (T)kLog2eD : (T)1; }
+
+constexpr int kScoreSlices = 4;                  // backward: lanes sharing one row
+constexpr int kScoreRows = 64 / kScoreSlices;    // backward: rows per wavefront unit
+static_assert(kScoreSlices == 4, "k_score_bwd combines the slices with two shuffles");
 
 template <typename T, bool ESP>
-__device__ __forceinline__ T pair_sum(const T* __restrict__ A, int N, const T* __restrict__ Bp, int M,
-                                      int j, int nt, T ha, T il) {
-  T v = 0;
-  for (int b = j; b < M; b += nt) {
-    const T bx = Bp[4 * b], by = Bp[4 * b + 1], bz = Bp[4 * b + 2], bw = Bp[4 * b + 3];
-    for (int a = 0; a < N; ++a) {
-      const T dx = A[4 * a] - bx, dy = A[4 * a + 1] - by, dz = A[4 * a + 2] - bz;
-      T arg = -ha * (dx * dx + dy * dy + dz * dz);
-      if (ESP) { const T dc = A[4 * a + 3] - bw; arg -= dc * dc * il; }
-      v += gauss<T>(arg);
+__device__ __forceinline__ void stage_cloud(const T* __restrict__ src, int n, const T c[3], T s, T sw,
+                                            typename Vec4<T>::type* dst) {
+  for (int i = threadIdx.x; i < n; i += blockDim.x) {
+    const T* p = src + 4 * (long)i;
+    dst[i] = {(p[0] - c[0]) * s, (p[1] - c[1]) * s, (p[2] - c[2]) * s, ESP ? p[3] * sw : (T)0};
+  }
+}
+
+// Stages the pair's clouds (ref then fit) and returns their counts.
+template <typename T, bool ESP>
+__device__ __forceinline__ void stage_pair_clouds(const T* __restrict__ A, int N, const T* __restrict__ Bp, int M,
+                                                  T alpha, T lam, typename Vec4<T>::type* la,
+                                                  typename Vec4<T>::type* lb, T& s) {
+  const T base = score_base<T>();
+  s = sqrt(alpha / 2 * base);
+  const T sw = ESP ? sqrt(base / lam) : (T)0;
+  const T* o = N > 0 ? A : Bp;
+  const T c[3] = {N + M > 0 ? o[0] : (T)0, N + M > 0 ? o[1] : (T)0, N + M > 0 ? o[2] : (T)0};
+  stage_cloud<T, ESP>(A, N, c, s, sw, la);
+  stage_cloud<T, ESP>(Bp, M, c, s, sw, lb);
+  __syncthreads();
+}
+
+// Sum over the columns [0, nc) of base^(-|q_c - p|^2); every lane reads the same column (LDS broadcast).
+template <typename T, bool ESP>
+__device__ __forceinline__ T tile_sum(const typename Vec4<T>::type* __restrict__ cols, int nc,
+                                      typename Vec4<T>::type p) {
+  T e0 = 0, e1 = 0;
+  int c = 0;
+  for (; c + 1 < nc; c += 2) {
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+      const typename Vec4<T>::type q = cols[c + k];
+      const T dx = q.x - p.x, dy = q.y - p.y, dz = q.z - p.z;
+      T r2 = dx * dx + dy * dy + dz * dz;
+      if (ESP) { const T dw = q.w - p.w; r2 += dw * dw; }
+      (k ? e1 : e0) += neg_exp<T>(r2);
     }
   }
-  return v;
+  if (c < nc) {
+    const typename Vec4<T>::type q = cols[c];
+    const T dx = q.x - p.x, dy = q.y - p.y, dz = q.z - p.z;
+    T r2 = dx * dx + dy * dy + dz * dz;
+    if (ESP) { const T dw = q.w - p.w; r2 += dw * dw; }
+    e0 += neg_exp<T>(r2);
+  }
+  return e0 + e1;
 }
 
-template <typename T>
-__device__ __forceinline__ T block_sum_t(T v, T* red) {
-  v = group_sum(v, 64);
-  const int w = threadIdx.x >> 6, l = threadIdx.x & 63, nw = blockDim.x >> 6;
-  __syncthreads();
-  if (l == 0) red[w] = v;
-  __syncthreads();
-  T s = 0;
-  for (int k = 0; k < nw; ++k) s += red[k];
-  __syncthreads();
-  return s;
-}
-
+// Forward: VAB, VAA and VBB as 64 x 64 tiles (lane = row, columns broadcast from LDS), VAA and VBB
+// over the upper block triangle only (off-diagonal tiles count twice).  The tiles are dealt to the
+// wavefronts round-robin, so a pair's whole product is balanced over its workgroup.
 template <typename T, bool ESP>
 __global__ __launch_bounds__(kPoseBlock) void k_score_fwd(const T* __restrict__ ref, const T* __restrict__ fit,
                                                           const int32_t* __restrict__ rr,
                                                           const int32_t* __restrict__ fr, T alpha, T lam,
                                                           T* S, T* ov) {
-  __shared__ T red[kPoseBlock / 64];
+  using V = typename Vec4<T>::type;
+  extern __shared__ double4 score_smem[];
+  __shared__ T red[3 * (kPoseBlock / 64)];
   const int pair = blockIdx.x;
-  const T* A = ref + 4 * (long)rr[2 * pair];
-  const T* Bp = fit + 4 * (long)fr[2 * pair];
   const int N = rr[2 * pair + 1], M = fr[2 * pair + 1];
-  const T ha = alpha / 2, il = (T)1 / lam;
-  const T K = (T)pow(M_PI / (2.0 * (double)alpha), 1.5);
-  const T vab = block_sum_t(pair_sum<T, ESP>(A, N, Bp, M, threadIdx.x, blockDim.x, ha, il), red) * K;
-  const T vaa = block_sum_t(pair_sum<T, ESP>(A, N, A, N, threadIdx.x, blockDim.x, ha, il), red) * K;
-  const T vbb = block_sum_t(pair_sum<T, ESP>(Bp, M, Bp, M, threadIdx.x, blockDim.x, ha, il), red) * K;
+  V* la = reinterpret_cast<V*>(score_smem);
+  V* lb = la + N;
+  T s;
+  stage_pair_clouds<T, ESP>(ref + 4 * (long)rr[2 * pair], N, fit + 4 * (long)fr[2 * pair], M, alpha, lam, la, lb, s);
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, nw = blockDim.x >> 6;
+  T acc[3] = {0, 0, 0};   // VAB, VAA, VBB sums
+  int u = 0;
+#pragma unroll
+  for (int t = 0; t < 3; ++t) {
+    const V* rows = t == 1 ? la : lb;
+    const V* cols = t == 2 ? lb : la;
+    const int nr = t == 1 ? N : M, nc = t == 2 ? M : N;
+    const int nbr = (nr + 63) / 64, nbc = (nc + 63) / 64;
+    for (int i = 0; i < nbr; ++i) {
+      const int r = i * 64 + lane;
+      const V p = rows[r < nr ? r : nr - 1];
+      for (int j = t == 0 ? 0 : i; j < nbc; ++j, ++u) {
+        if (u % nw != wave) continue;
+        const T e = tile_sum<T, ESP>(cols + 64 * j, min(64, nc - 64 * j), p);
+        acc[t] += r < nr ? (t != 0 && j > i ? 2 * e : e) : (T)0;
+      }
+    }
+  }
+#pragma unroll
+  for (int t = 0; t < 3; ++t) {
+    const T v = group_sum(acc[t], 64);
+    if (lane == 0) red[3 * wave + t] = v;
+  }
+  __syncthreads();
   if (threadIdx.x == 0) {
+    T v[3] = {0, 0, 0};
+    for (int w = 0; w < nw; ++w)
+      for (int t = 0; t < 3; ++t) v[t] += red[3 * w + t];
+    const T K = (T)pow(M_PI / (2.0 * (double)alpha), 1.5);
+    const T vab = v[0] * K, vaa = v[1] * K, vbb = v[2] * K;
     S[pair] = vab / (vaa + vbb - vab);
     ov[4 * pair] = vab; ov[4 * pair + 1] = vaa; ov[4 * pair + 2] = vbb; ov[4 * pair + 3] = 0;
   }
 }
 
-// d/dx_b of sum_{a in A} w E(a, x_b) = K alpha sum_a w E (a - x_b)
+// f += sum over the columns first, first + step, ... < n of base^(-|q - p|^2) (q - p)
 template <typename T, bool ESP>
-__device__ __forceinline__ void point_force(const T* __restrict__ A, int N, T bx, T by, T bz, T bw,
-                                            T ha, T il, T f[3]) {
-  f[0] = f[1] = f[2] = 0;
-  for (int a = 0; a < N; ++a) {
-    const T dx = A[4 * a] - bx, dy = A[4 * a + 1] - by, dz = A[4 * a + 2] - bz;
-    T arg = -ha * (dx * dx + dy * dy + dz * dz);
-    if (ESP) { const T dc = A[4 * a + 3] - bw; arg -= dc * dc * il; }
-    const T e = gauss<T>(arg);
+__device__ __forceinline__ void slice_force(const typename Vec4<T>::type* __restrict__ cols, int n, int first,
+                                            typename Vec4<T>::type p, T f[3]) {
+#pragma unroll 4
+  for (int c = first; c < n; c += kScoreSlices) {
+    const typename Vec4<T>::type q = cols[c];
+    const T dx = q.x - p.x, dy = q.y - p.y, dz = q.z - p.z;
+    T r2 = dx * dx + dy * dy + dz * dz;
+    if (ESP) { const T dw = q.w - p.w; r2 += dw * dw; }
+    const T e = neg_exp<T>(r2);
     f[0] += e * dx; f[1] += e * dy; f[2] += e * dz;
   }
 }
 
+// Backward.  With U = VAA + VBB and D = U - VAB, for a fit point x_b
+//   dS/dx_b = K alpha / D^2 (U sum_a w E (a - x_b) - 2 VAB sum_b' w E (x_b' - x_b))
+// (the second sum is dVBB/dx_b / 2; VBB is invariant only under rigid motions), and the same
+// with the clouds' roles swapped for a ref point.  A wavefront unit is kScoreRows rows; the
+// kScoreSlices lanes of a row split its columns and are combined with a fixed shuffle order, so
+// the result is deterministic.
 template <typename T, bool ESP>
 __global__ __launch_bounds__(kPoseBlock) void k_score_bwd(const T* __restrict__ ref, const T* __restrict__ fit,
                                                           const int32_t* __restrict__ rr,
                                                           const int32_t* __restrict__ fr, T alpha, T lam,
                                                           const T* __restrict__ ov, const T* __restrict__ gS,
                                                           T* d_fit, T* d_ref) {
+  using V = typename Vec4<T>::type;
+  extern __shared__ double4 score_smem[];
   const int pair = blockIdx.x;
-  const T* A = ref + 4 * (long)rr[2 * pair];
-  const T* Bp = fit + 4 * (long)fr[2 * pair];
   const int N = rr[2 * pair + 1], M = fr[2 * pair + 1];
-  const T ha = alpha / 2, il = (T)1 / lam;
+  V* la = reinterpret_cast<V*>(score_smem);
+  V* lb = la + N;
+  T s;
+  stage_pair_clouds<T, ESP>(ref + 4 * (long)rr[2 * pair], N, fit + 4 * (long)fr[2 * pair], M, alpha, lam, la, lb, s);
   const T K = (T)pow(M_PI / (2.0 * (double)alpha), 1.5);
-  const T vab = ov[4 * pair], vaa = ov[4 * pair + 1], vbb = ov[4 * pair + 2];
-  const T U = vaa + vbb, D = U - vab;
-  const T g = gS[pair] * K * alpha / (D * D);
-  // dS/dx_b = (U dVAB/dx_b - VAB dVBB/dx_b) / D^2, dVBB/dx_b = 2 K alpha sum_b' w E (b' - b)
-  for (int b = threadIdx.x; b < M; b += blockDim.x) {
-    const T bx = Bp[4 * b], by = Bp[4 * b + 1], bz = Bp[4 * b + 2], bw = Bp[4 * b + 3];
-    T fa[3], fb[3];
-    point_force<T, ESP>(A, N, bx, by, bz, bw, ha, il, fa);
-    point_force<T, ESP>(Bp, M, bx, by, bz, bw, ha, il, fb);
-    T* o = d_fit + 4 * ((long)fr[2 * pair] + b);
-    for (int i = 0; i < 3; ++i) o[i] = g * (U * fa[i] - 2 * vab * fb[i]);
-    o[3] = 0;
-  }
-  if (d_ref) {
-    for (int a = threadIdx.x; a < N; a += blockDim.x) {
-      const T ax = A[4 * a], ay = A[4 * a + 1], az = A[4 * a + 2], aw = A[4 * a + 3];
-      T fa[3], fb[3];
-      point_force<T, ESP>(Bp, M, ax, ay, az, aw, ha, il, fb);
-      point_force<T, ESP>(A, N, ax, ay, az, aw, ha, il, fa);
-      T* o = d_ref + 4 * ((long)rr[2 * pair] + a);
-      for (int i = 0; i < 3; ++i) o[i] = g * (U * fb[i] - 2 * vab * fa[i]);
-      o[3] = 0;
+  const T vab = ov[4 * pair], U = ov[4 * pair + 1] + ov[4 * pair + 2], D = U - vab;
+  const T g = gS[pair] * K * alpha / (D * D) / s;   // the force sums are in scaled coordinates
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, nw = blockDim.x >> 6;
+  const int sl = lane / kScoreRows, rl = lane % kScoreRows;
+  const int nbf = (M + kScoreRows - 1) / kScoreRows, nba = d_ref ? (N + kScoreRows - 1) / kScoreRows : 0;
+  for (int u = wave; u < nbf + nba; u += nw) {
+    const bool fit_row = u < nbf;
+    const V* own = fit_row ? lb : la;
+    const V* oth = fit_row ? la : lb;
+    const int n_own = fit_row ? M : N, n_oth = fit_row ? N : M;
+    const int row = (fit_row ? u : u - nbf) * kScoreRows + rl;
+    const V p = own[row < n_own ? row : n_own - 1];
+    T fo[3] = {0, 0, 0}, fs[3] = {0, 0, 0};
+    slice_force<T, ESP>(oth, n_oth, sl, p, fo);
+    slice_force<T, ESP>(own, n_own, sl, p, fs);
+    T d[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      T v = U * fo[k] - 2 * vab * fs[k];
+      v += __shfl_xor(v, kScoreRows);
+      v += __shfl_xor(v, 2 * kScoreRows);
+      d[k] = g * v;
+    }
+    if (sl == 0 && row < n_own) {
+      T* o = fit_row ? d_fit + 4 * ((long)fr[2 * pair] + row) : d_ref + 4 * ((long)rr[2 * pair] + row);
+      o[0] = d[0]; o[1] = d[1]; o[2] = d[2]; o[3] = 0;
     }
   }
 }
@@ -984,6 +1064,16 @@ int score_check(const FssPairBatch* batch, const FssScoreParams* sp) {
   return FSS_OK;
 }
 
+// Both clouds of a pair are staged in LDS: 4 values per point.
+int score_lds(const FssPairBatch* batch, size_t elem, size_t& lds) {
+  if (batch->max_ref < 0 || batch->max_fit < 0) return set_error(FSS_E_INVALID, "negative max_ref / max_fit");
+  lds = ((size_t)batch->max_ref + (size_t)batch->max_fit) * 4 * elem;
+  if (lds > max_lds_bytes())
+    return set_error(FSS_E_INVALID, "score fwd/bwd stage both clouds of a pair in LDS: max_ref + max_fit "
+                                    "must be at most %zu", max_lds_bytes() / (4 * elem));
+  return FSS_OK;
+}
+
 template <typename T>
 int score_fwd_t(const FssPairBatch* batch, const FssScoreParams* sp, const T* ref, const T* fit,
                        T* S, T* ov, hipStream_t s) {
@@ -992,10 +1082,12 @@ int score_fwd_t(const FssPairBatch* batch, const FssScoreParams* sp, const T* re
   if (!ref || !fit || !S || !ov) return set_error(FSS_E_INVALID, "NULL score arrays");
   if (batch->n_pairs == 0) return FSS_OK;
   const T lam = sp->mode == FSS_MODE_SHAPE_ESP ? (T)sp->lam : (T)1;
+  size_t lds = 0;
+  if ((rc = score_lds(batch, sizeof(T), lds))) return rc;
   if (sp->mode == FSS_MODE_SHAPE_ESP)
-    hipLaunchKernelGGL((k_score_fwd<T, true>), dim3(batch->n_pairs), dim3(kPoseBlock), 0, s, ref, fit, batch->ref_rng, batch->fit_rng, (T)sp->alpha, lam, S, ov);
+    hipLaunchKernelGGL((k_score_fwd<T, true>), dim3(batch->n_pairs), dim3(kPoseBlock), lds, s, ref, fit, batch->ref_rng, batch->fit_rng, (T)sp->alpha, lam, S, ov);
   else
-    hipLaunchKernelGGL((k_score_fwd<T, false>), dim3(batch->n_pairs), dim3(kPoseBlock), 0, s, ref, fit, batch->ref_rng, batch->fit_rng, (T)sp->alpha, lam, S, ov);
+    hipLaunchKernelGGL((k_score_fwd<T, false>), dim3(batch->n_pairs), dim3(kPoseBlock), lds, s, ref, fit, batch->ref_rng, batch->fit_rng, (T)sp->alpha, lam, S, ov);
   return check_launch("fss_score_fwd");
 }
 
@@ -1007,10 +1099,12 @@ int score_bwd_t(const FssPairBatch* batch, const FssScoreParams* sp, const T* re
   if (!ref || !fit || !ov || !gS || !d_fit) return set_error(FSS_E_INVALID, "NULL score arrays");
   if (batch->n_pairs == 0) return FSS_OK;
   const T lam = sp->mode == FSS_MODE_SHAPE_ESP ? (T)sp->lam : (T)1;
+  size_t lds = 0;
+  if ((rc = score_lds(batch, sizeof(T), lds))) return rc;
   if (sp->mode == FSS_MODE_SHAPE_ESP)
-    hipLaunchKernelGGL((k_score_bwd<T, true>), dim3(batch->n_pairs), dim3(kPoseBlock), 0, s, ref, fit, batch->ref_rng, batch->fit_rng, (T)sp->alpha, lam, ov, gS, d_fit, d_ref);
+    hipLaunchKernelGGL((k_score_bwd<T, true>), dim3(batch->n_pairs), dim3(kPoseBlock), lds, s, ref, fit, batch->ref_rng, batch->fit_rng, (T)sp->alpha, lam, ov, gS, d_fit, d_ref);
   else
-    hipLaunchKernelGGL((k_score_bwd<T, false>), dim3(batch->n_pairs), dim3(kPoseBlock), 0, s, ref, fit, batch->ref_rng, batch->fit_rng, (T)sp->alpha, lam, ov, gS, d_fit, d_ref);
+    hipLaunchKernelGGL((k_score_bwd<T, false>), dim3(batch->n_pairs), dim3(kPoseBlock), lds, s, ref, fit, batch->ref_rng, batch->fit_rng, (T)sp->alpha, lam, ov, gS, d_fit, d_ref);
   return check_launch("fss_score_bwd");
 }
 

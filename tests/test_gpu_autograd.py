@@ -83,3 +83,34 @@ def test_contraction_to_se3_matches_oracle(dev):
     gq = gq - qh[0] * (qh[0] @ gq)                 # tangent part (unit q: |q| = 1)
     np.testing.assert_allclose(gt.numpy(), g_or[0, 4:].numpy(), atol=1e-9)
     np.testing.assert_allclose(gq.numpy(), g_or[0, :4].numpy(), atol=1e-9)
+
+
+@pytest.mark.parametrize("N,M", [(1, 3), (70, 130), (64, 65)])
+def test_ragged_tile_edges_and_determinism(dev, N, M):
+    """Cloud sizes that end mid-tile (64-row forward tiles, 16-row backward units split over 4
+    column slices) against the FP64 oracle; the backward is bitwise repeatable."""
+    ref64, fit64, re64, fe64 = _clouds(3, N, M, 7, torch.float64)
+    x = fit64.clone().requires_grad_(True)
+    r = ref64.clone().requires_grad_(True)
+    orc.tanimoto_esp(r, x, re64, fe64, 0.81, 0.5).sum().backward()
+    grads = []
+    for _ in range(2):
+        ref = ref64.float().to(dev).requires_grad_(True)
+        fit = fit64.float().to(dev).requires_grad_(True)
+        S = gaussian_similarity(ref, fit, re64.float().to(dev), fe64.float().to(dev), 0.81, 0.5)
+        S.sum().backward()
+        grads.append((fit.grad.cpu().numpy(), ref.grad.cpu().numpy()))
+    S_ref = orc.tanimoto_esp(ref64, fit64, re64, fe64, 0.81, 0.5).numpy()
+    np.testing.assert_allclose(S.detach().cpu().numpy(), S_ref, atol=1e-5)
+    for got, want in zip(grads[0], (x.grad.numpy(), r.grad.numpy())):
+        np.testing.assert_allclose(got, want, atol=1e-4 * np.abs(want).max())
+    for a, b in zip(grads[0], grads[1]):
+        assert np.array_equal(a, b)
+
+
+def test_lds_limit_is_reported(dev):
+    """Both clouds of a pair live in LDS: a pair beyond the limit is refused, not run."""
+    from fast_shepherd_score_amd._lib import FssError
+    ref = torch.zeros((1, 5200, 3), device=dev)
+    with pytest.raises(FssError, match="LDS"):
+        gaussian_similarity(ref, ref.clone(), alpha=0.81)
