@@ -32,6 +32,7 @@ E_CLOSED = 9004
 PENDING = 1
 LIB_SHAPE = 1
 LIB_PHARM = 2
+LIB_SHAEP = 4
 SCREEN_MAX_TOPK = 1024
 
 _i32p = ctypes.POINTER(ctypes.c_int32)
@@ -107,8 +108,14 @@ class FssLibSpec(ctypes.Structure):
     _fields_ = [
         ("device", ctypes.c_int32), ("channels", ctypes.c_int32),
         ("max_conformers", ctypes.c_int64), ("max_points", ctypes.c_int64),
-        ("max_pharm", ctypes.c_int64),
+        ("max_pharm", ctypes.c_int64), ("max_surface", ctypes.c_int64),
+        ("max_atoms", ctypes.c_int64),
     ]
+
+
+class FssLibRows(ctypes.Structure):
+    _fields_ = [(k, ctypes.c_void_p) for k in
+                ("pts", "counts", "anc", "vec", "typ", "pcounts", "srf", "scounts", "atm", "rad", "acounts")]
 
 
 class FssScreenParams(ctypes.Structure):
@@ -123,6 +130,8 @@ class FssScreenParams(ctypes.Structure):
         ("lanes_per_pose", ctypes.c_int32), ("restarts_per_wg", ctypes.c_int32),
         ("variant", ctypes.c_int32), ("seed_flags", ctypes.c_int32),
         ("quat_table", ctypes.c_void_p),
+        ("max_query_surface", ctypes.c_int32), ("max_query_atoms", ctypes.c_int32),
+        ("shape_from_surface", ctypes.c_int32),
     ]
 
 
@@ -131,6 +140,8 @@ class FssQuery(ctypes.Structure):
         ("n_points", ctypes.c_int32), ("pts", ctypes.c_void_p),
         ("n_pharm", ctypes.c_int32), ("anc", ctypes.c_void_p),
         ("vec", ctypes.c_void_p), ("typ", ctypes.c_void_p),
+        ("n_surface", ctypes.c_int32), ("srf", ctypes.c_void_p),
+        ("n_atoms", ctypes.c_int32), ("atm", ctypes.c_void_p), ("rad", ctypes.c_void_p),
     ]
 
 
@@ -170,6 +181,7 @@ SIGNATURES = {
     "fss_score_bwd_f64": (ctypes.c_int, [_P, _P, _P, _P, _P, _P, _P, _P, _P]),
     "fss_lib_create": (ctypes.c_int, [_P, _P]),
     "fss_lib_upload": (ctypes.c_int, [_P, ctypes.c_int32, _P, _P, _P, _P, _P, _P, _P]),
+    "fss_lib_append": (ctypes.c_int, [_P, ctypes.c_int32, _P, _P]),
     "fss_lib_size": (ctypes.c_int, [_P, _P]),
     "fss_lib_destroy": (ctypes.c_int, [_P]),
     "fss_screen_create": (ctypes.c_int, [_P, _P, _P, _P]),

@@ -45,7 +45,15 @@ struct FssLibrary {
   float* vec = nullptr;        // (cap_ph, 4)
   int32_t* typ = nullptr;      // (cap_ph)
   int32_t* prng = nullptr;     // (cap_conf, 2)
-  std::vector<int32_t> h_rng, h_prng;   // host copies of the range tables
+  // ShaEP channel
+  int64_t cap_srf = 0, cap_atm = 0, n_srf = 0, n_atm = 0;
+  int32_t max_srf = 0, max_atm = 0;
+  float* srf = nullptr;        // (cap_srf, 4): surface point, ESP
+  int32_t* srng = nullptr;     // (cap_conf, 2)
+  float* atm = nullptr;        // (cap_atm, 4): atom with hydrogens, partial charge
+  float* rad = nullptr;        // (cap_atm)
+  int32_t* arng = nullptr;     // (cap_conf, 2)
+  std::vector<int32_t> h_rng, h_prng, h_srng, h_arng;   // host copies of the range tables
   std::mutex mu;
   int services = 0;
 };
@@ -80,6 +88,14 @@ hipError_t dalloc(T** p, int64_t count) {
   *p = nullptr;
   if (count <= 0) return hipSuccess;
   return hipMalloc((void**)p, (size_t)count * sizeof(T));
+}
+
+void free_library(FssLibrary* lib) {
+  void* ps[] = {lib->pts, lib->rng, lib->anc, lib->vec, lib->typ, lib->prng,
+                lib->srf, lib->srng, lib->atm, lib->rad, lib->arng};
+  for (void* p : ps)
+    if (p) (void)hipFree(p);
+  delete lib;
 }
 
 // ------------------------------------------------------------------------------------------------
@@ -230,9 +246,9 @@ __global__ __launch_bounds__(kMergeThreads) void k_topk_merge(int K, const Merge
 
 struct QueryIn {
   int64_t ticket = 0;
-  std::vector<float> pts, anc, vec;
+  std::vector<float> pts, anc, vec, srf, atm, rad;
   std::vector<int32_t> typ;
-  int32_t np = 0, nph = 0;
+  int32_t np = 0, nph = 0, ns = 0, na = 0;
   clk::time_point t_submit;
 };
 
@@ -253,6 +269,9 @@ struct Result {
   double queue_ms = 0, latency_ms = 0;
 };
 
+// Range tables of one launch, each (P, 2) = [first row, count] per pair.
+enum RngTable { kRef, kFit, kRefPh, kFitPh, kRefSrf, kFitSrf, kRefAtm, kFitAtm, kRngTables };
+
 }  // namespace
 
 struct FssScreen {
@@ -261,15 +280,18 @@ struct FssScreen {
   FssScreenParams prm{};
   int B = 0, K = 0, P = 0, A = 0;
   int need_shape = 0, need_pharm = 0;
+  int need_se = 0;                     // FSS_MODE_ESP_COMBO: the ShaEP channel
+  int shape_rows = 0;                  // the shape term reads the shape channel's rows (else the surface rows)
   hipStream_t stream = nullptr;
   // device arena
   float *q = nullptr, *t = nullptr, *table = nullptr, *best_tf = nullptr;
   double *pose_score = nullptr, *best_score = nullptr, *self_ov = nullptr;
   int32_t *best_idx = nullptr, *n_steps = nullptr;
-  int32_t *d_rng = nullptr;            // 4 x (P, 2): ref, fit, ref pharm, fit pharm
+  int32_t *d_rng = nullptr;            // kRngTables x (P, 2), see RngTable
   int32_t* h_rng = nullptr;            // pinned host staging of d_rng
   float *qpts = nullptr, *qanc = nullptr, *qvec = nullptr;
   int32_t* qtyp = nullptr;
+  float *qsrf = nullptr, *qatm = nullptr, *qrad = nullptr;
   double* tk_s = nullptr;              // (A, 2, K)
   int64_t* tk_i = nullptr;             // (A, 2, K)
   float* tk_tf = nullptr;              // (A, 2, K, 16)
@@ -293,7 +315,7 @@ namespace {
 void free_arena(FssScreen* s) {
   void* dev_ptrs[] = {s->q, s->t, s->table, s->best_tf, s->pose_score, s->best_score, s->self_ov,
                       s->best_idx, s->n_steps, s->d_rng, s->qpts, s->qanc, s->qvec, s->qtyp,
-                      s->tk_s, s->tk_i, s->tk_tf, s->d_jobs};
+                      s->qsrf, s->qatm, s->qrad, s->tk_s, s->tk_i, s->tk_tf, s->d_jobs};
   for (void* p : dev_ptrs)
     if (p) (void)hipFree(p);
   if (s->h_rng) (void)hipHostFree(s->h_rng);
@@ -314,7 +336,7 @@ int admit(FssScreen* s, int slot) {
   Slot& sl = s->slots[slot];
   const size_t mq = (size_t)s->prm.max_query_points, mp = (size_t)s->prm.max_query_pharm;
   hipError_t e = hipSuccess;
-  if (s->need_shape && sl.q.np)
+  if (s->shape_rows && sl.q.np)
     e = hipMemcpyAsync(s->qpts + 4 * mq * slot, sl.q.pts.data(), sizeof(float) * 4 * sl.q.np,
                        hipMemcpyHostToDevice, s->stream);
   if (e == hipSuccess && s->need_pharm && sl.q.nph) {
@@ -325,6 +347,17 @@ int admit(FssScreen* s, int slot) {
                          hipMemcpyHostToDevice, s->stream);
     if (e == hipSuccess)
       e = hipMemcpyAsync(s->qtyp + mp * slot, sl.q.typ.data(), sizeof(int32_t) * sl.q.nph,
+                         hipMemcpyHostToDevice, s->stream);
+  }
+  if (e == hipSuccess && s->need_se) {
+    const size_t ms = (size_t)s->prm.max_query_surface, ma = (size_t)s->prm.max_query_atoms;
+    e = hipMemcpyAsync(s->qsrf + 4 * ms * slot, sl.q.srf.data(), sizeof(float) * 4 * sl.q.ns,
+                       hipMemcpyHostToDevice, s->stream);
+    if (e == hipSuccess)
+      e = hipMemcpyAsync(s->qatm + 4 * ma * slot, sl.q.atm.data(), sizeof(float) * 4 * sl.q.na,
+                         hipMemcpyHostToDevice, s->stream);
+    if (e == hipSuccess)
+      e = hipMemcpyAsync(s->qrad + ma * slot, sl.q.rad.data(), sizeof(float) * sl.q.na,
                          hipMemcpyHostToDevice, s->stream);
   }
   if (e != hipSuccess) return hip_fail(e, "query upload");
@@ -346,16 +379,17 @@ int launch(FssScreen* s, const std::vector<int>& act) {
   for (int i = 0; i < na; ++i) rem[i] = s->slots[act[i]].lib_n - s->slots[act[i]].next;
   fss_plan_launch(na, rem.data(), s->P, take.data());
   const size_t mq = (size_t)s->prm.max_query_points, mp = (size_t)s->prm.max_query_pharm;
-  int32_t* hr = s->h_rng;
-  int32_t* hf = hr + 2 * (size_t)s->P;
-  int32_t* hrp = hf + 2 * (size_t)s->P;
-  int32_t* hfp = hrp + 2 * (size_t)s->P;
-  int p = 0, nj = 0, max_np = 0, max_nph = 0;
-  int32_t lib_max_pts = 0, lib_max_ph = 0;
+  const size_t ms = (size_t)s->prm.max_query_surface, ma = (size_t)s->prm.max_query_atoms;
+  int32_t* tab[kRngTables];
+  for (int k = 0; k < kRngTables; ++k) tab[k] = s->h_rng + 2 * (size_t)s->P * k;
+  // pair p of table k <- (first row, count); a channel the mode does not use gets empty ranges
+  auto set = [&](int k, int p, int32_t first, int32_t count) { tab[k][2 * p] = first; tab[k][2 * p + 1] = count; };
+  int p = 0, nj = 0, max_np = 0, max_nph = 0, max_ns = 0, max_na = 0;
+  int32_t lib_max_pts = 0, lib_max_ph = 0, lib_max_srf = 0, lib_max_atm = 0;
   {
     std::lock_guard<std::mutex> lk(s->lib->mu);
-    lib_max_pts = s->lib->max_pts;
-    lib_max_ph = s->lib->max_ph;
+    const FssLibrary* L = s->lib;
+    lib_max_pts = L->max_pts; lib_max_ph = L->max_ph; lib_max_srf = L->max_srf; lib_max_atm = L->max_atm;
     for (int i = 0; i < na; ++i) {
       if (take[i] == 0) continue;
       Slot& sl = s->slots[act[i]];
@@ -366,31 +400,53 @@ int launch(FssScreen* s, const std::vector<int>& act) {
       jb.base = sl.next; jb.prev_n = sl.filled; jb.off = p; jb.k = (int32_t)take[i]; jb.pad = 0;
       max_np = std::max(max_np, sl.q.np);
       max_nph = std::max(max_nph, sl.q.nph);
+      max_ns = std::max(max_ns, sl.q.ns);
+      max_na = std::max(max_na, sl.q.na);
       for (int64_t j = 0; j < take[i]; ++j, ++p) {
-        const int64_t L = sl.next + j;
-        hr[2 * p] = (int32_t)(mq * act[i]); hr[2 * p + 1] = sl.q.np;
-        hrp[2 * p] = (int32_t)(mp * act[i]); hrp[2 * p + 1] = sl.q.nph;
-        if (s->need_shape) { hf[2 * p] = s->lib->h_rng[2 * L]; hf[2 * p + 1] = s->lib->h_rng[2 * L + 1]; }
-        else { hf[2 * p] = 0; hf[2 * p + 1] = 0; }
-        if (s->need_pharm) { hfp[2 * p] = s->lib->h_prng[2 * L]; hfp[2 * p + 1] = s->lib->h_prng[2 * L + 1]; }
-        else { hfp[2 * p] = 0; hfp[2 * p + 1] = 0; }
+        const size_t e = 2 * (size_t)(sl.next + j);
+        const bool sh = s->shape_rows, ph = s->need_pharm, se = s->need_se;
+        set(kRef, p, (int32_t)(mq * act[i]), sh ? sl.q.np : 0);
+        set(kFit, p, sh ? L->h_rng[e] : 0, sh ? L->h_rng[e + 1] : 0);
+        set(kRefPh, p, (int32_t)(mp * act[i]), ph ? sl.q.nph : 0);
+        set(kFitPh, p, ph ? L->h_prng[e] : 0, ph ? L->h_prng[e + 1] : 0);
+        set(kRefSrf, p, (int32_t)(ms * act[i]), se ? sl.q.ns : 0);
+        set(kFitSrf, p, se ? L->h_srng[e] : 0, se ? L->h_srng[e + 1] : 0);
+        set(kRefAtm, p, (int32_t)(ma * act[i]), se ? sl.q.na : 0);
+        set(kFitAtm, p, se ? L->h_arng[e] : 0, se ? L->h_arng[e + 1] : 0);
       }
     }
   }
   if (p == 0) return FSS_OK;
-  hipError_t e = hipMemcpyAsync(s->d_rng, s->h_rng, sizeof(int32_t) * 8 * (size_t)s->P, hipMemcpyHostToDevice, s->stream);
+  hipError_t e = hipMemcpyAsync(s->d_rng, s->h_rng, sizeof(int32_t) * 2 * kRngTables * (size_t)s->P,
+                                hipMemcpyHostToDevice, s->stream);
   if (e == hipSuccess) e = hipMemcpyAsync(s->d_jobs, s->h_jobs, sizeof(MergeJob) * nj, hipMemcpyHostToDevice, s->stream);
   if (e != hipSuccess) return hip_fail(e, "launch staging");
+  const int32_t* drng[kRngTables];
+  for (int k = 0; k < kRngTables; ++k) drng[k] = s->d_rng + 2 * (size_t)s->P * k;
   FssPairBatch pb{};
   pb.n_pairs = p;
-  pb.ref_pts = s->qpts; pb.fit_pts = s->lib->pts;
-  pb.ref_rng = s->d_rng; pb.fit_rng = s->d_rng + 2 * (size_t)s->P;
-  pb.ref_anc = s->qanc; pb.ref_vec = s->qvec; pb.ref_typ = s->qtyp; pb.ref_prng = s->d_rng + 4 * (size_t)s->P;
-  pb.fit_anc = s->lib->anc; pb.fit_vec = s->lib->vec; pb.fit_typ = s->lib->typ; pb.fit_prng = s->d_rng + 6 * (size_t)s->P;
-  pb.max_ref = s->need_shape ? std::max(max_np, 1) : 0;
-  pb.max_fit = s->need_shape ? std::max(lib_max_pts, 1) : 0;
+  if (s->shape_rows) {
+    pb.ref_pts = s->qpts; pb.fit_pts = s->lib->pts;
+    pb.ref_rng = drng[kRef]; pb.fit_rng = drng[kFit];
+    pb.max_ref = std::max(max_np, 1); pb.max_fit = std::max(lib_max_pts, 1);
+  } else if (s->need_shape) {   // surface shape term of the ShaEP score: the surface rows
+    pb.ref_pts = s->qsrf; pb.fit_pts = s->lib->srf;
+    pb.ref_rng = drng[kRefSrf]; pb.fit_rng = drng[kFitSrf];
+    pb.max_ref = std::max(max_ns, 1); pb.max_fit = std::max(lib_max_srf, 1);
+  } else {
+    pb.ref_rng = drng[kRef]; pb.fit_rng = drng[kFit];
+  }
+  pb.ref_anc = s->qanc; pb.ref_vec = s->qvec; pb.ref_typ = s->qtyp; pb.ref_prng = drng[kRefPh];
+  pb.fit_anc = s->lib->anc; pb.fit_vec = s->lib->vec; pb.fit_typ = s->lib->typ; pb.fit_prng = drng[kFitPh];
   pb.max_ref_ph = s->need_pharm ? std::max(max_nph, 1) : 0;
   pb.max_fit_ph = s->need_pharm ? std::max(lib_max_ph, 1) : 0;
+  if (s->need_se) {
+    pb.ref_srf = s->qsrf; pb.fit_srf = s->lib->srf; pb.ref_srng = drng[kRefSrf]; pb.fit_srng = drng[kFitSrf];
+    pb.ref_atm = s->qatm; pb.fit_atm = s->lib->atm; pb.ref_rad = s->qrad; pb.fit_rad = s->lib->rad;
+    pb.ref_arng = drng[kRefAtm]; pb.fit_arng = drng[kFitAtm];
+    pb.max_ref_srf = std::max(max_ns, 1); pb.max_fit_srf = std::max(lib_max_srf, 1);
+    pb.max_ref_atm = std::max(max_na, 1); pb.max_fit_atm = std::max(lib_max_atm, 1);
+  }
   int rc = fss_seed_poses(&pb, s->prm.seed_flags, s->B, s->table, s->q, s->t, s->stream);
   if (!rc) rc = fss_self_overlap(&pb, &s->sp, s->self_ov, s->stream);
   if (!rc) {
@@ -547,14 +603,18 @@ int fss_plan_launch(int32_t n, const int64_t* remaining, int64_t capacity, int64
 int fss_lib_create(const FssLibSpec* spec, FssLibrary** out) {
   if (!spec || !out) return set_error(FSS_E_INVALID, "NULL library arguments");
   *out = nullptr;
-  if (!(spec->channels & (FSS_LIB_SHAPE | FSS_LIB_PHARM)) || (spec->channels & ~(FSS_LIB_SHAPE | FSS_LIB_PHARM)))
-    return set_error(FSS_E_INVALID, "channels must be FSS_LIB_SHAPE and/or FSS_LIB_PHARM");
+  constexpr int all = FSS_LIB_SHAPE | FSS_LIB_PHARM | FSS_LIB_SHAEP;
+  if (!(spec->channels & all) || (spec->channels & ~all))
+    return set_error(FSS_E_INVALID, "channels must be a combination of FSS_LIB_SHAPE, FSS_LIB_PHARM and FSS_LIB_SHAEP");
   if (spec->max_conformers < 1 || spec->max_conformers > INT32_MAX)
     return set_error(FSS_E_INVALID, "max_conformers must be in [1, 2^31)");
   if ((spec->channels & FSS_LIB_SHAPE) && (spec->max_points < 1 || spec->max_points > INT32_MAX))
     return set_error(FSS_E_INVALID, "max_points must be in [1, 2^31)");
   if ((spec->channels & FSS_LIB_PHARM) && (spec->max_pharm < 1 || spec->max_pharm > INT32_MAX))
     return set_error(FSS_E_INVALID, "max_pharm must be in [1, 2^31)");
+  if ((spec->channels & FSS_LIB_SHAEP) && (spec->max_surface < 1 || spec->max_surface > INT32_MAX ||
+                                           spec->max_atoms < 1 || spec->max_atoms > INT32_MAX))
+    return set_error(FSS_E_INVALID, "max_surface and max_atoms must be in [1, 2^31)");
   DeviceGuard dg(spec->device);
   if (dg.err != hipSuccess) return hip_fail(dg.err, "fss_lib_create: device");
   FssLibrary* lib = new FssLibrary();
@@ -563,6 +623,8 @@ int fss_lib_create(const FssLibSpec* spec, FssLibrary** out) {
   lib->cap_conf = spec->max_conformers;
   lib->cap_pts = (spec->channels & FSS_LIB_SHAPE) ? spec->max_points : 0;
   lib->cap_ph = (spec->channels & FSS_LIB_PHARM) ? spec->max_pharm : 0;
+  lib->cap_srf = (spec->channels & FSS_LIB_SHAEP) ? spec->max_surface : 0;
+  lib->cap_atm = (spec->channels & FSS_LIB_SHAEP) ? spec->max_atoms : 0;
   hipError_t e = hipSuccess;
   if (lib->cap_pts) {
     e = dalloc(&lib->pts, 4 * lib->cap_pts);
@@ -574,11 +636,15 @@ int fss_lib_create(const FssLibSpec* spec, FssLibrary** out) {
     if (e == hipSuccess) e = dalloc(&lib->typ, lib->cap_ph);
     if (e == hipSuccess) e = dalloc(&lib->prng, 2 * lib->cap_conf);
   }
+  if (e == hipSuccess && lib->cap_srf) {
+    e = dalloc(&lib->srf, 4 * lib->cap_srf);
+    if (e == hipSuccess) e = dalloc(&lib->srng, 2 * lib->cap_conf);
+    if (e == hipSuccess) e = dalloc(&lib->atm, 4 * lib->cap_atm);
+    if (e == hipSuccess) e = dalloc(&lib->rad, lib->cap_atm);
+    if (e == hipSuccess) e = dalloc(&lib->arng, 2 * lib->cap_conf);
+  }
   if (e != hipSuccess) {
-    void* ps[] = {lib->pts, lib->rng, lib->anc, lib->vec, lib->typ, lib->prng};
-    for (void* p : ps)
-      if (p) (void)hipFree(p);
-    delete lib;
+    free_library(lib);
     return hip_fail(e, "fss_lib_create: allocation");
   }
   *out = lib;
@@ -588,61 +654,83 @@ int fss_lib_create(const FssLibSpec* spec, FssLibrary** out) {
 int fss_lib_upload(FssLibrary* lib, int32_t n, const float* pts, const int32_t* counts,
                    const float* anc, const float* vec, const int32_t* typ, const int32_t* pcounts,
                    int64_t* first) {
-  if (!lib) return set_error(FSS_E_INVALID, "library is NULL");
+  if (lib && (lib->channels & FSS_LIB_SHAEP))
+    return set_error(FSS_E_INVALID, "the library has a ShaEP channel: append with fss_lib_append");
+  FssLibRows r{};
+  r.pts = pts; r.counts = counts; r.anc = anc; r.vec = vec; r.typ = typ; r.pcounts = pcounts;
+  return fss_lib_append(lib, n, &r, first);
+}
+
+int fss_lib_append(FssLibrary* lib, int32_t n, const FssLibRows* in, int64_t* first) {
+  if (!lib || !in) return set_error(FSS_E_INVALID, "library or rows are NULL");
   if (n < 0) return set_error(FSS_E_INVALID, "n < 0");
   const bool sh = lib->channels & FSS_LIB_SHAPE, ph = lib->channels & FSS_LIB_PHARM;
-  if (n > 0 && sh && (!pts || !counts)) return set_error(FSS_E_INVALID, "shape channel arrays are NULL");
-  if (n > 0 && ph && (!anc || !vec || !typ || !pcounts))
+  const bool se = lib->channels & FSS_LIB_SHAEP;
+  if (n > 0 && sh && (!in->pts || !in->counts)) return set_error(FSS_E_INVALID, "shape channel arrays are NULL");
+  if (n > 0 && ph && (!in->anc || !in->vec || !in->typ || !in->pcounts))
     return set_error(FSS_E_INVALID, "pharmacophore channel arrays are NULL");
+  if (n > 0 && se && (!in->srf || !in->scounts || !in->atm || !in->rad || !in->acounts))
+    return set_error(FSS_E_INVALID, "ShaEP channel arrays are NULL");
   std::lock_guard<std::mutex> lk(lib->mu);
   if (first) *first = lib->n;
   if (n == 0) return FSS_OK;
   if (lib->n + n > lib->cap_conf) return set_error(FSS_E_TOO_LARGE, "library capacity %lld conformers exceeded", (long long)lib->cap_conf);
-  int64_t rows = 0, prow = 0;
-  int32_t mx = lib->max_pts, mph = lib->max_ph;
-  for (int32_t i = 0; i < n; ++i) {
-    if (sh) {
-      if (counts[i] < 0) return set_error(FSS_E_INVALID, "negative count");
-      rows += counts[i];
-      mx = std::max(mx, counts[i]);
+  // per channel: rows appended, running maximum count, range table
+  struct Ch { bool on; const int32_t* cnt; int64_t n_rows, cap; int32_t mx; std::vector<int32_t> r; int64_t rows; };
+  Ch ch[4] = {{sh, in->counts, lib->n_pts, lib->cap_pts, lib->max_pts, {}, 0},
+              {ph, in->pcounts, lib->n_ph, lib->cap_ph, lib->max_ph, {}, 0},
+              {se, in->scounts, lib->n_srf, lib->cap_srf, lib->max_srf, {}, 0},
+              {se, in->acounts, lib->n_atm, lib->cap_atm, lib->max_atm, {}, 0}};
+  static const char* names[4] = {"points", "pharmacophores", "surface points", "atoms"};
+  for (int c = 0; c < 4; ++c) {
+    if (!ch[c].on) continue;
+    ch[c].r.resize(2 * (size_t)n);
+    int64_t o = ch[c].n_rows;
+    for (int32_t i = 0; i < n; ++i) {
+      const int32_t k = ch[c].cnt[i];
+      if (k < 0) return set_error(FSS_E_INVALID, "negative count of %s", names[c]);
+      ch[c].r[2 * i] = (int32_t)std::min<int64_t>(o, INT32_MAX);
+      ch[c].r[2 * i + 1] = k;
+      o += k;
+      ch[c].mx = std::max(ch[c].mx, k);
     }
-    if (ph) {
-      if (pcounts[i] < 0) return set_error(FSS_E_INVALID, "negative pharmacophore count");
-      prow += pcounts[i];
-      mph = std::max(mph, pcounts[i]);
-    }
+    ch[c].rows = o - ch[c].n_rows;
+    if (o > ch[c].cap)
+      return set_error(FSS_E_TOO_LARGE, "library capacity %lld %s exceeded", (long long)ch[c].cap, names[c]);
   }
-  if (sh && lib->n_pts + rows > lib->cap_pts) return set_error(FSS_E_TOO_LARGE, "library capacity %lld points exceeded", (long long)lib->cap_pts);
-  if (ph && lib->n_ph + prow > lib->cap_ph) return set_error(FSS_E_TOO_LARGE, "library capacity %lld pharmacophores exceeded", (long long)lib->cap_ph);
   DeviceGuard dg(lib->device);
-  if (dg.err != hipSuccess) return hip_fail(dg.err, "fss_lib_upload: device");
-  std::vector<int32_t> r(2 * (size_t)n), pr(2 * (size_t)n);
-  int64_t o = lib->n_pts, po = lib->n_ph;
-  for (int32_t i = 0; i < n; ++i) {
-    if (sh) { r[2 * i] = (int32_t)o; r[2 * i + 1] = counts[i]; o += counts[i]; }
-    if (ph) { pr[2 * i] = (int32_t)po; pr[2 * i + 1] = pcounts[i]; po += pcounts[i]; }
-  }
+  if (dg.err != hipSuccess) return hip_fail(dg.err, "fss_lib_append: device");
   hipError_t e = hipSuccess;
+  auto put = [&](void* dst, const void* src, size_t bytes) {
+    if (e == hipSuccess && bytes) e = hipMemcpy(dst, src, bytes, hipMemcpyDefault);
+  };
   if (sh) {
-    if (rows) e = hipMemcpy(lib->pts + 4 * lib->n_pts, pts, sizeof(float) * 4 * rows, hipMemcpyDefault);
-    if (e == hipSuccess) e = hipMemcpy(lib->rng + 2 * lib->n, r.data(), sizeof(int32_t) * 2 * n, hipMemcpyHostToDevice);
+    put(lib->pts + 4 * lib->n_pts, in->pts, sizeof(float) * 4 * ch[0].rows);
+    put(lib->rng + 2 * lib->n, ch[0].r.data(), sizeof(int32_t) * 2 * n);
   }
-  if (e == hipSuccess && ph) {
-    if (prow) {
-      e = hipMemcpy(lib->anc + 4 * lib->n_ph, anc, sizeof(float) * 4 * prow, hipMemcpyDefault);
-      if (e == hipSuccess) e = hipMemcpy(lib->vec + 4 * lib->n_ph, vec, sizeof(float) * 4 * prow, hipMemcpyDefault);
-      if (e == hipSuccess) e = hipMemcpy(lib->typ + lib->n_ph, typ, sizeof(int32_t) * prow, hipMemcpyDefault);
-    }
-    if (e == hipSuccess) e = hipMemcpy(lib->prng + 2 * lib->n, pr.data(), sizeof(int32_t) * 2 * n, hipMemcpyHostToDevice);
+  if (ph) {
+    put(lib->anc + 4 * lib->n_ph, in->anc, sizeof(float) * 4 * ch[1].rows);
+    put(lib->vec + 4 * lib->n_ph, in->vec, sizeof(float) * 4 * ch[1].rows);
+    put(lib->typ + lib->n_ph, in->typ, sizeof(int32_t) * ch[1].rows);
+    put(lib->prng + 2 * lib->n, ch[1].r.data(), sizeof(int32_t) * 2 * n);
   }
-  if (e != hipSuccess) return hip_fail(e, "fss_lib_upload");
-  if (sh) lib->h_rng.insert(lib->h_rng.end(), r.begin(), r.end());
-  if (ph) lib->h_prng.insert(lib->h_prng.end(), pr.begin(), pr.end());
+  if (se) {
+    put(lib->srf + 4 * lib->n_srf, in->srf, sizeof(float) * 4 * ch[2].rows);
+    put(lib->srng + 2 * lib->n, ch[2].r.data(), sizeof(int32_t) * 2 * n);
+    put(lib->atm + 4 * lib->n_atm, in->atm, sizeof(float) * 4 * ch[3].rows);
+    put(lib->rad + lib->n_atm, in->rad, sizeof(float) * ch[3].rows);
+    put(lib->arng + 2 * lib->n, ch[3].r.data(), sizeof(int32_t) * 2 * n);
+  }
+  if (e != hipSuccess) return hip_fail(e, "fss_lib_append");
+  if (sh) { lib->h_rng.insert(lib->h_rng.end(), ch[0].r.begin(), ch[0].r.end()); lib->n_pts += ch[0].rows; lib->max_pts = ch[0].mx; }
+  if (ph) { lib->h_prng.insert(lib->h_prng.end(), ch[1].r.begin(), ch[1].r.end()); lib->n_ph += ch[1].rows; lib->max_ph = ch[1].mx; }
+  if (se) {
+    lib->h_srng.insert(lib->h_srng.end(), ch[2].r.begin(), ch[2].r.end());
+    lib->h_arng.insert(lib->h_arng.end(), ch[3].r.begin(), ch[3].r.end());
+    lib->n_srf += ch[2].rows; lib->max_srf = ch[2].mx;
+    lib->n_atm += ch[3].rows; lib->max_atm = ch[3].mx;
+  }
   lib->n += n;
-  lib->n_pts = o;
-  lib->n_ph = po;
-  lib->max_pts = mx;
-  lib->max_ph = mph;
   return FSS_OK;
 }
 
@@ -660,10 +748,7 @@ int fss_lib_destroy(FssLibrary* lib) {
     if (lib->services > 0) return set_error(FSS_E_INVALID, "the library still has %d screening service(s)", lib->services);
   }
   DeviceGuard dg(lib->device);
-  void* ps[] = {lib->pts, lib->rng, lib->anc, lib->vec, lib->typ, lib->prng};
-  for (void* p : ps)
-    if (p) (void)hipFree(p);
-  delete lib;
+  free_library(lib);
   return FSS_OK;
 }
 
@@ -671,21 +756,28 @@ int fss_screen_create(FssLibrary* lib, const FssScoreParams* sp, const FssScreen
                       FssScreen** out) {
   if (!lib || !sp || !prm || !out) return set_error(FSS_E_INVALID, "NULL screening arguments");
   *out = nullptr;
-  if (sp->mode < 0 || sp->mode > 3) return set_error(FSS_E_INVALID, "unknown mode %d", sp->mode);
+  if (sp->mode < 0 || sp->mode > FSS_MODE_ESP_COMBO) return set_error(FSS_E_INVALID, "unknown mode %d", sp->mode);
   const bool need_shape = sp->mode != FSS_MODE_PHARM;
   const bool need_pharm = sp->mode == FSS_MODE_PHARM || sp->mode == FSS_MODE_SHAPE_PHARM;
-  if (need_shape && !(lib->channels & FSS_LIB_SHAPE)) return set_error(FSS_E_INVALID, "the mode needs the library's shape channel");
+  const bool need_se = sp->mode == FSS_MODE_ESP_COMBO;
+  const bool shape_rows = need_shape && !(need_se && prm->shape_from_surface);
+  if (shape_rows && !(lib->channels & FSS_LIB_SHAPE)) return set_error(FSS_E_INVALID, "the mode needs the library's shape channel");
   if (need_pharm && !(lib->channels & FSS_LIB_PHARM)) return set_error(FSS_E_INVALID, "the mode needs the library's pharmacophore channel");
+  if (need_se && !(lib->channels & FSS_LIB_SHAEP)) return set_error(FSS_E_INVALID, "the mode needs the library's ShaEP channel");
   if (prm->n_restarts < 1) return set_error(FSS_E_INVALID, "n_restarts must be >= 1");
   if (prm->max_steps < 0) return set_error(FSS_E_INVALID, "max_steps must be >= 0");
   if (prm->top_k < 1 || prm->top_k > FSS_SCREEN_MAX_TOPK) return set_error(FSS_E_INVALID, "top_k must be in [1, %d]", FSS_SCREEN_MAX_TOPK);
   if (prm->launch_pairs < 1) return set_error(FSS_E_INVALID, "launch_pairs must be >= 1");
   if (prm->max_active < 1 || prm->max_active > 4096) return set_error(FSS_E_INVALID, "max_active must be in [1, 4096]");
   if (prm->launch_pairs < prm->max_active) return set_error(FSS_E_INVALID, "launch_pairs must be >= max_active");
-  if (need_shape && prm->max_query_points < 1) return set_error(FSS_E_INVALID, "max_query_points must be >= 1");
+  if (shape_rows && prm->max_query_points < 1) return set_error(FSS_E_INVALID, "max_query_points must be >= 1");
+  if (need_se && (prm->max_query_surface < 1 || prm->max_query_atoms < 1))
+    return set_error(FSS_E_INVALID, "max_query_surface and max_query_atoms must be >= 1");
   if (need_pharm && prm->max_query_pharm < 1) return set_error(FSS_E_INVALID, "max_query_pharm must be >= 1");
   if (!prm->quat_table) return set_error(FSS_E_INVALID, "quat_table is NULL");
-  if ((int64_t)prm->max_active * prm->max_query_points > INT32_MAX || (int64_t)prm->max_active * prm->max_query_pharm > INT32_MAX)
+  if ((int64_t)prm->max_active * prm->max_query_points > INT32_MAX || (int64_t)prm->max_active * prm->max_query_pharm > INT32_MAX ||
+      (need_se && ((int64_t)prm->max_active * prm->max_query_surface > INT32_MAX ||
+                   (int64_t)prm->max_active * prm->max_query_atoms > INT32_MAX)))
     return set_error(FSS_E_INVALID, "query arena exceeds 2^31 rows");
   DeviceGuard dg(lib->device);
   if (dg.err != hipSuccess) return hip_fail(dg.err, "fss_screen_create: device");
@@ -699,10 +791,13 @@ int fss_screen_create(FssLibrary* lib, const FssScoreParams* sp, const FssScreen
   s->A = prm->max_active;
   s->need_shape = need_shape;
   s->need_pharm = need_pharm;
+  s->need_se = need_se;
+  s->shape_rows = shape_rows;
   if (!s->prm.lanes_per_pose) s->prm.lanes_per_pose = 4;
   if (!s->prm.restarts_per_wg) s->prm.restarts_per_wg = s->B;
-  if (!need_shape) s->prm.max_query_points = 0;
+  if (!shape_rows) s->prm.max_query_points = 0;
   if (!need_pharm) s->prm.max_query_pharm = 0;
+  if (!need_se) s->prm.max_query_surface = s->prm.max_query_atoms = 0;
   s->slots.resize(s->A);
   const int64_t PB = (int64_t)s->P * s->B;
   const int64_t AK = (int64_t)s->A * 2 * s->K;
@@ -716,9 +811,14 @@ int fss_screen_create(FssLibrary* lib, const FssScoreParams* sp, const FssScreen
   if (e == hipSuccess) e = dalloc(&s->best_tf, 16 * (int64_t)s->P);
   if (e == hipSuccess) e = dalloc(&s->n_steps, s->P);
   if (e == hipSuccess) e = dalloc(&s->self_ov, 4 * (int64_t)s->P);
-  if (e == hipSuccess) e = dalloc(&s->d_rng, 8 * (int64_t)s->P);
-  if (e == hipSuccess) e = hipHostMalloc((void**)&s->h_rng, sizeof(int32_t) * 8 * (size_t)s->P, hipHostMallocDefault);
-  if (e == hipSuccess && need_shape) e = dalloc(&s->qpts, 4 * (int64_t)s->A * s->prm.max_query_points);
+  if (e == hipSuccess) e = dalloc(&s->d_rng, 2 * kRngTables * (int64_t)s->P);
+  if (e == hipSuccess) e = hipHostMalloc((void**)&s->h_rng, sizeof(int32_t) * 2 * kRngTables * (size_t)s->P, hipHostMallocDefault);
+  if (e == hipSuccess && shape_rows) e = dalloc(&s->qpts, 4 * (int64_t)s->A * s->prm.max_query_points);
+  if (e == hipSuccess && need_se) {
+    e = dalloc(&s->qsrf, 4 * (int64_t)s->A * s->prm.max_query_surface);
+    if (e == hipSuccess) e = dalloc(&s->qatm, 4 * (int64_t)s->A * s->prm.max_query_atoms);
+    if (e == hipSuccess) e = dalloc(&s->qrad, (int64_t)s->A * s->prm.max_query_atoms);
+  }
   if (e == hipSuccess && need_pharm) {
     e = dalloc(&s->qanc, 4 * (int64_t)s->A * s->prm.max_query_pharm);
     if (e == hipSuccess) e = dalloc(&s->qvec, 4 * (int64_t)s->A * s->prm.max_query_pharm);
@@ -746,7 +846,7 @@ int fss_screen_create(FssLibrary* lib, const FssScoreParams* sp, const FssScreen
 
 int fss_screen_submit(FssScreen* s, const FssQuery* q, int64_t* ticket) {
   if (!s || !q || !ticket) return set_error(FSS_E_INVALID, "NULL submit arguments");
-  if (s->need_shape) {
+  if (s->shape_rows) {
     if (q->n_points < 1 || !q->pts) return set_error(FSS_E_INVALID, "the query needs shape rows");
     if (q->n_points > s->prm.max_query_points)
       return set_error(FSS_E_TOO_LARGE, "query has %d rows, the service takes %d", q->n_points, s->prm.max_query_points);
@@ -756,9 +856,23 @@ int fss_screen_submit(FssScreen* s, const FssQuery* q, int64_t* ticket) {
     if (q->n_pharm > s->prm.max_query_pharm)
       return set_error(FSS_E_TOO_LARGE, "query has %d pharmacophores, the service takes %d", q->n_pharm, s->prm.max_query_pharm);
   }
+  if (s->need_se) {
+    if (q->n_surface < 1 || !q->srf || q->n_atoms < 1 || !q->atm || !q->rad)
+      return set_error(FSS_E_INVALID, "the query needs ShaEP surface and atom rows");
+    if (q->n_surface > s->prm.max_query_surface || q->n_atoms > s->prm.max_query_atoms)
+      return set_error(FSS_E_TOO_LARGE, "query has %d surface / %d atom rows, the service takes %d / %d",
+                       q->n_surface, q->n_atoms, s->prm.max_query_surface, s->prm.max_query_atoms);
+  }
   QueryIn qi;
   qi.t_submit = clk::now();
-  if (s->need_shape) {
+  if (s->need_se) {
+    qi.ns = q->n_surface;
+    qi.na = q->n_atoms;
+    qi.srf.assign(q->srf, q->srf + 4 * (size_t)q->n_surface);
+    qi.atm.assign(q->atm, q->atm + 4 * (size_t)q->n_atoms);
+    qi.rad.assign(q->rad, q->rad + (size_t)q->n_atoms);
+  }
+  if (s->shape_rows) {
     qi.np = q->n_points;
     qi.pts.assign(q->pts, q->pts + 4 * (size_t)q->n_points);
   }

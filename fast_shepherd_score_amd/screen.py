@@ -32,13 +32,14 @@ import torch
 from . import _lib
 from .align import OptimSpec, ScoreSpec
 from .constants import LAM_SCALING
-from .packing import CloudSet, PharmSet
+from .packing import CloudSet, PharmSet, ShaepSet
 from .seeds import quats_from_fibo
 
 Hit = collections.namedtuple("Hit", ["score", "id", "transform"])
 
 _KIND = {"vol": _lib.MODE_SHAPE, "surf": _lib.MODE_SHAPE, "vol_esp": _lib.MODE_SHAPE_ESP,
-         "surf_esp": _lib.MODE_SHAPE_ESP, "pharm": _lib.MODE_PHARM, "vol_color": _lib.MODE_SHAPE_PHARM}
+         "surf_esp": _lib.MODE_SHAPE_ESP, "pharm": _lib.MODE_PHARM, "vol_color": _lib.MODE_SHAPE_PHARM,
+         "vol_and_surf_esp": _lib.MODE_ESP_COMBO}
 
 
 def restart_table(B: int) -> np.ndarray:
@@ -51,13 +52,19 @@ def restart_table(B: int) -> np.ndarray:
 
 
 class ConformerLibrary:
-    """Packed conformers on one device: shape channel (atoms or surface points [+ ESP]) and/or
-    pharmacophores, with integer ids."""
+    """Packed conformers on one device: shape channel (atoms or surface points [+ ESP]),
+    pharmacophores and/or the ShaEP channel (surface points with ESP, atoms with hydrogens,
+    partial charges and radii), with integer ids."""
 
     def __init__(self, shape: Optional[CloudSet] = None, pharm: Optional[PharmSet] = None,
-                 ids: Optional[np.ndarray] = None):
-        self.shape, self.pharm = shape, pharm
-        n = shape.n if shape is not None else pharm.n
+                 ids: Optional[np.ndarray] = None, shaep: Optional[ShaepSet] = None):
+        self.shape, self.pharm, self.shaep = shape, pharm, shaep
+        some = [c for c in (shape, pharm, shaep) if c is not None]
+        if not some:
+            raise ValueError("a library needs at least one channel")
+        n = some[0].n
+        if any(c.n != n for c in some):
+            raise ValueError("every channel must hold the same conformers")
         self.n = n
         self.ids = np.arange(n, dtype=np.int64) if ids is None else np.asarray(ids)
         self._native = None
@@ -68,7 +75,7 @@ class ConformerLibrary:
                     nbytes: int) -> "ConformerLibrary":
         """A library whose rows live only in a native FssLibrary (see store.load_library)."""
         lib = cls.__new__(cls)
-        lib.shape = lib.pharm = None
+        lib.shape = lib.pharm = lib.shaep = None
         lib.n = int(n)
         lib.ids = np.asarray(ids)
         lib._native = handle
@@ -85,28 +92,42 @@ class ConformerLibrary:
         if getattr(self, "_native", None) is None:
             lib = _lib.load()
             spec = _lib.FssLibSpec()
-            dev = (self.shape.pts if self.shape is not None else self.pharm.anc).device
+            some = next(c for c in (self.shape, self.pharm, self.shaep) if c is not None)
+            dev = (some.pts if isinstance(some, CloudSet) else
+                   some.anc if isinstance(some, PharmSet) else some.srf.pts).device
             spec.device = dev.index if dev.index is not None else torch.cuda.current_device()
             spec.channels = (_lib.LIB_SHAPE if self.shape is not None else 0) | \
-                (_lib.LIB_PHARM if self.pharm is not None else 0)
+                (_lib.LIB_PHARM if self.pharm is not None else 0) | \
+                (_lib.LIB_SHAEP if self.shaep is not None else 0)
             spec.max_conformers = max(self.n, 1)
-            spec.max_points = max(int(self.shape.counts.sum()), 1) if self.shape is not None else 0
-            spec.max_pharm = max(int(self.pharm.counts.sum()), 1) if self.pharm is not None else 0
+            rows = _lib.FssLibRows()
+            keep = []
+
+            def counts(a):
+                c = np.ascontiguousarray(a, np.int32)
+                keep.append(c)
+                return c.ctypes.data
+
+            if self.shape is not None:
+                spec.max_points = max(int(self.shape.counts.sum()), 1)
+                rows.pts, rows.counts = _lib.ptr(self.shape.pts), counts(self.shape.counts)
+            if self.pharm is not None:
+                spec.max_pharm = max(int(self.pharm.counts.sum()), 1)
+                rows.anc, rows.vec, rows.typ = (_lib.ptr(self.pharm.anc), _lib.ptr(self.pharm.vec),
+                                                _lib.ptr(self.pharm.typ))
+                rows.pcounts = counts(self.pharm.counts)
+            if self.shaep is not None:
+                se = self.shaep
+                spec.max_surface = max(int(se.srf.counts.sum()), 1)
+                spec.max_atoms = max(int(se.atm.counts.sum()), 1)
+                rows.srf, rows.scounts = _lib.ptr(se.srf.pts), counts(se.srf.counts)
+                rows.atm, rows.rad, rows.acounts = _lib.ptr(se.atm.pts), _lib.ptr(se.rad), counts(se.atm.counts)
             h = ctypes.c_void_p()
             _lib.call("fss_lib_create", _lib.byref(spec), ctypes.byref(h))
-            cnt = pcnt = None
-            args = [None] * 6
-            if self.shape is not None:
-                cnt = np.ascontiguousarray(self.shape.counts, np.int32)
-                args[0], args[1] = _lib.ptr(self.shape.pts), cnt.ctypes.data
-            if self.pharm is not None:
-                pcnt = np.ascontiguousarray(self.pharm.counts, np.int32)
-                args[2:] = [_lib.ptr(self.pharm.anc), _lib.ptr(self.pharm.vec),
-                            _lib.ptr(self.pharm.typ), pcnt.ctypes.data]
             torch.cuda.synchronize(dev)
             first = ctypes.c_int64()
             try:
-                _lib.call("fss_lib_upload", h, self.n, *args, ctypes.byref(first))
+                _lib.call("fss_lib_append", h, self.n, _lib.byref(rows), ctypes.byref(first))
             except Exception:
                 lib.fss_lib_destroy(h)
                 raise
@@ -127,17 +148,26 @@ class ConformerLibrary:
 
     @classmethod
     def from_arrays(cls, points=None, esp=None, pharm_types=None, pharm_anchors=None,
-                    pharm_vectors=None, ids=None, device="cuda") -> "ConformerLibrary":
+                    pharm_vectors=None, ids=None, device="cuda", surf_points=None, surf_esp=None,
+                    atoms_w_H=None, charges=None, radii=None) -> "ConformerLibrary":
+        """Per-conformer arrays.  ``points`` [+ ``esp``]: the shape channel (heavy atoms for the
+        ShaEP score at alpha 0.81); ``pharm_*``: pharmacophores; ``surf_points``, ``surf_esp``,
+        ``atoms_w_H``, ``charges``, ``radii``: the ShaEP channel (vol_and_surf_esp)."""
         shape = CloudSet.from_arrays(points, esp, device=device) if points is not None else None
         pharm = (PharmSet.from_arrays(pharm_types, pharm_anchors, pharm_vectors, device=device)
                  if pharm_types is not None else None)
-        return cls(shape, pharm, ids)
+        shaep = (ShaepSet.from_arrays(surf_points, surf_esp, atoms_w_H, charges, radii, device=device)
+                 if surf_points is not None else None)
+        return cls(shape, pharm, ids, shaep)
 
     def nbytes(self) -> int:
         if self._native_bytes is not None:
             return self._native_bytes
         tot = 0
-        for cs in (self.shape, self.pharm):
+        parts = [self.shape, self.pharm]
+        if self.shaep is not None:
+            parts += [self.shaep.srf, self.shaep.atm, self.shaep]
+        for cs in parts:
             if cs is None:
                 continue
             for t in vars(cs).values():
@@ -169,11 +199,13 @@ def plan_launch(remaining: Sequence[int], capacity: int) -> List[int]:
 
 
 def arena_bytes(launch_pairs: int, restarts: int, max_active: int, top_k: int,
-                max_query_points: int, max_query_pharm: int) -> int:
+                max_query_points: int, max_query_pharm: int, max_query_surface: int = 0,
+                max_query_atoms: int = 0) -> int:
     """Device bytes a service reserves at creation (fss_screen_create's arena)."""
     P, B, A, K = launch_pairs, restarts, max_active, top_k
-    per_pair = B * (4 + 3) * 4 + B * 8 + 8 + 4 + 64 + 4 + 32 + 32
-    per_slot = max_query_points * 16 + max_query_pharm * 36 + 2 * K * (8 + 8 + 64) + 64
+    per_pair = B * (4 + 3) * 4 + B * 8 + 8 + 4 + 64 + 4 + 32 + 64
+    per_slot = (max_query_points * 16 + max_query_pharm * 36 + max_query_surface * 16 +
+                max_query_atoms * 20 + 2 * K * (8 + 8 + 64) + 64)
     return P * per_pair + A * per_slot + B * 16
 
 
@@ -187,39 +219,60 @@ class ScreenResult:
 
 class ScreenService:
     """Serve screening queries against a resident library with continuous batching (a thin
-    host wrapper over the native ``fss_screen_*`` service)."""
+    host wrapper over the native ``fss_screen_*`` service).
 
-    def __init__(self, library: ConformerLibrary, mode: str = "vol", *, num_repeats: int = 10,
-                 max_num_steps: int = 100, lr: float = 0.1, top_k: int = 100,
+    The alignment keywords follow the reference screen (screen.py:851-882): ``num_repeats`` and
+    ``max_num_steps`` default to the mode's table, ``lam`` is the raw lambda (scaled by
+    LAM_SCALING for surf_esp and vol_and_surf_esp, defaults 0.3 / 0.1 / 0.001), and
+    vol_and_surf_esp scores heavy atoms at alpha 0.81 and surface points otherwise.
+    """
+
+    def __init__(self, library: ConformerLibrary, mode: str = "vol", *, num_repeats: Optional[int] = None,
+                 max_num_steps: Optional[int] = None, lr: float = 0.1, top_k: int = 100,
                  launch_pairs: int = 65536, hbm_budget_bytes: int = 8 << 30, max_active: int = 16,
                  early_stop: bool = True, alpha: float = 0.81, lam: Optional[float] = None,
                  similarity: str = "tanimoto", directionless: bool = True,
-                 lanes_per_pose: int = 4, max_query_points: int = 4096,
-                 max_query_pharm: int = 256, device=None):
+                 lanes_per_pose: Optional[int] = None, max_query_points: int = 4096,
+                 max_query_pharm: int = 256, esp_weight: float = 0.5, probe_radius: float = 1.0,
+                 max_query_surface: int = 4096, max_query_atoms: int = 1024, device=None):
+        from .modes import MODE_SEEDS, MODE_STEPS
         if mode not in _KIND:
             raise ValueError(f"unsupported screening mode {mode!r}")
         if not 1 <= int(top_k) <= _lib.SCREEN_MAX_TOPK:
             raise ValueError(f"top_k must be in [1, {_lib.SCREEN_MAX_TOPK}]")
         self.lib, self.mode = library, mode
-        if lam is None:
-            lam = 0.3 * LAM_SCALING if mode == "surf_esp" else 0.1
+        combo = mode == "vol_and_surf_esp"
+        if mode == "surf_esp":
+            lam = LAM_SCALING * (0.3 if lam is None else lam)
+        elif combo:
+            lam = LAM_SCALING * (0.001 if lam is None else lam)
+        elif lam is None:
+            lam = 0.1
         self.spec = ScoreSpec(mode=_KIND[mode], alpha=alpha, lam=lam, similarity=similarity,
-                              directionless=directionless)
-        self.B = int(num_repeats)
-        self.opt = OptimSpec(max_steps=int(max_num_steps), lr=float(lr), early_stop=bool(early_stop))
+                              directionless=directionless, esp_weight=esp_weight,
+                              probe_radius=probe_radius)
+        self.B = int(MODE_SEEDS[mode] if num_repeats is None else num_repeats)
+        steps = MODE_STEPS[mode] if max_num_steps is None else max_num_steps
+        self.opt = OptimSpec(max_steps=int(steps), lr=float(lr), early_stop=bool(early_stop))
         self.top_k = int(top_k)
         self.launch_pairs = int(launch_pairs)
-        shape = self.spec.mode != _lib.MODE_PHARM
+        # vol_and_surf_esp: the shape term takes the heavy atoms at alpha 0.81, else the surface
+        # rows of the ShaEP channel (_core.py:1079-1080)
+        self.shape_from_surface = combo and alpha != 0.81
+        shape = self.spec.mode != _lib.MODE_PHARM and not self.shape_from_surface
         pharm = self.spec.mode in (_lib.MODE_PHARM, _lib.MODE_SHAPE_PHARM)
         mqp, mqh = (int(max_query_points) if shape else 0), (int(max_query_pharm) if pharm else 0)
+        mqs, mqa = (int(max_query_surface), int(max_query_atoms)) if combo else (0, 0)
         # admission: as many query slots as the HBM budget holds, at most max_active
-        fixed = arena_bytes(self.launch_pairs, self.B, 0, self.top_k, mqp, mqh)
-        slot = arena_bytes(0, 0, 1, self.top_k, mqp, mqh)
+        fixed = arena_bytes(self.launch_pairs, self.B, 0, self.top_k, mqp, mqh, mqs, mqa)
+        slot = arena_bytes(0, 0, 1, self.top_k, mqp, mqh, mqs, mqa)
         if fixed + slot > hbm_budget_bytes:
             raise MemoryError("the HBM budget does not hold one query slot and one launch")
         self.max_active = max(1, min(int(max_active), (hbm_budget_bytes - fixed) // slot,
                                      self.launch_pairs))
         self.table = restart_table(self.B)
+        if lanes_per_pose is None:   # ShaEP poses carry the surface-atom ESP sums: a wider tile
+            lanes_per_pose = 32 if combo else 4
         prm = _lib.FssScreenParams()
         prm.n_restarts, prm.max_steps = self.B, self.opt.max_steps
         prm.lr, prm.beta1, prm.beta2, prm.eps = self.opt.lr, self.opt.beta1, self.opt.beta2, self.opt.eps
@@ -227,8 +280,11 @@ class ScreenService:
         prm.es_patience = int(self.opt.es_patience)
         prm.top_k, prm.launch_pairs, prm.max_active = self.top_k, self.launch_pairs, self.max_active
         prm.max_query_points, prm.max_query_pharm = mqp, mqh
+        prm.max_query_surface, prm.max_query_atoms = mqs, mqa
+        prm.shape_from_surface = int(self.shape_from_surface)
         prm.lanes_per_pose, prm.restarts_per_wg, prm.variant = int(lanes_per_pose), self.B, 0
-        prm.seed_flags = _lib.SEED_PHARM if self.spec.mode == _lib.MODE_PHARM else 0
+        prm.seed_flags = (_lib.SEED_PHARM if self.spec.mode == _lib.MODE_PHARM else
+                          _lib.SEED_SURF if combo else 0)
         prm.quat_table = self.table.ctypes.data
         self._sp = self.spec.struct()
         self._h = ctypes.c_void_p()
@@ -244,13 +300,34 @@ class ScreenService:
     # -- public ------------------------------------------------------------------------------
     def submit(self, query: dict, top_k: Optional[int] = None) -> Future:
         """Queue a query (dict with the arrays its mode needs: ``points`` [+ ``esp``] and/or
-        ``pharm_types``, ``pharm_anchors``, ``pharm_vectors``).  Resolves to a ScreenResult."""
+        ``pharm_types``, ``pharm_anchors``, ``pharm_vectors``; vol_and_surf_esp: ``surf_points``,
+        ``surf_esp``, ``atoms_w_H``, ``charges``, ``radii`` and, at alpha 0.81, ``points`` = the
+        heavy atoms).  Resolves to a ScreenResult."""
         k = int(top_k or self.top_k)
         if not 1 <= k <= self.top_k:
             raise ValueError(f"top_k must be in [1, {self.top_k}] for this service")
         q = _lib.FssQuery()
         keep = []
-        if self.spec.mode != _lib.MODE_PHARM:
+
+        def rows4(xyz, w=None):
+            xyz = np.asarray(xyz, np.float32).reshape(-1, 3)
+            r = np.zeros((len(xyz), 4), np.float32)
+            r[:, :3] = xyz
+            if w is not None:
+                r[:, 3] = np.asarray(w, np.float32).reshape(-1)
+            keep.append(r)
+            return r
+
+        if self.spec.mode == _lib.MODE_ESP_COMBO:
+            srf = rows4(query["surf_points"], query["surf_esp"])
+            atm = rows4(query["atoms_w_H"], query["charges"])
+            rad = np.ascontiguousarray(query["radii"], np.float32).reshape(-1)
+            if len(rad) != len(atm):
+                raise ValueError("radii must have one entry per atom")
+            keep.append(rad)
+            q.n_surface, q.srf = len(srf), srf.ctypes.data
+            q.n_atoms, q.atm, q.rad = len(atm), atm.ctypes.data, rad.ctypes.data
+        if self.spec.mode != _lib.MODE_PHARM and not self.shape_from_surface:
             xyz = np.asarray(query["points"], np.float32).reshape(-1, 3)
             rows = np.zeros((len(xyz), 4), np.float32)
             rows[:, :3] = xyz

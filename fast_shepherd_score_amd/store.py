@@ -348,7 +348,7 @@ def _rows(xyz: np.ndarray, w: Optional[np.ndarray] = None) -> np.ndarray:
 def _shard_channels(a: dict, mode: str, schema: dict):
     """(shape rows, counts) and (anc rows, vec rows, types, counts) of one shard for ``mode``."""
     shape = pharm = None
-    if mode in ("vol", "vol_color"):
+    if mode in ("vol", "vol_color", "vol_and_surf_esp"):   # ShaEP: heavy atoms for alpha 0.81
         ao = a["atom_off"]
         shape = _rows(_f32(a["atom_pos"])), np.diff(ao).astype(np.int32)
     elif mode == "vol_esp":
@@ -372,6 +372,17 @@ def _shard_channels(a: dict, mode: str, schema: dict):
     return shape, pharm
 
 
+def _shard_shaep(a: dict):
+    """ShaEP channel of one with-H shard: (surface rows (x, y, z, ESP), surface counts, atom rows
+    (x, y, z, partial charge) with hydrogens, radii, atom counts)"""
+    sp = _f32(a["surf_pos"])
+    K, S = sp.shape[:2]
+    srf = _rows(sp.reshape(-1, 3), _f32(a["surf_esp"]).reshape(-1))
+    atm = _rows(_f32(a["cwh"]), _f32(a["charges"]))
+    return (srf, np.full(K, S, np.int32), atm, np.ascontiguousarray(_f32(a["radii"])),
+            np.diff(a["all_off"]).astype(np.int32))
+
+
 def _device_index(device) -> int:
     if isinstance(device, int):
         return device
@@ -387,15 +398,14 @@ def load_library(store, mode: str, device=0, shards: Optional[Sequence[int]] = N
     if isinstance(store, (str, os.PathLike)):
         store = ProfileStore.open(str(store))
     m = canonical(mode)
-    if m == "vol_and_surf_esp":
-        raise NotImplementedError("vol_and_surf_esp screening is not implemented by the HIP backend")
     if not store.supports(m):
         raise ValueError(f"store does not support mode {m!r} (modes {store.modes})")
     idx = list(range(store.num_shards)) if shards is None else list(shards)
     want_shape = m != "pharm"
     want_pharm = m in ("pharm", "vol_color")
+    want_se = m == "vol_and_surf_esp"
     # sizing pass: conformers and rows per channel from the offsets alone
-    n = rows = prows = 0
+    n = rows = prows = srows = arows = 0
     for i in idx:
         sh = store.manifest["shards"][i]
         path = os.path.join(store.path, sh["name"])
@@ -410,13 +420,19 @@ def load_library(store, mode: str, device=0, shards: Optional[Sequence[int]] = N
                     rows += int(d["atom_off"][-1])
             if want_pharm:
                 prows += int(d["pharm_off"][-1])
+            if want_se:
+                srows += int(sh["n"]) * store.num_surf_points
+                arows += int(d["all_off"][-1])
     lib = _lib.load()
     spec = _lib.FssLibSpec()
     spec.device = _device_index(device)
-    spec.channels = (_lib.LIB_SHAPE if want_shape else 0) | (_lib.LIB_PHARM if want_pharm else 0)
+    spec.channels = ((_lib.LIB_SHAPE if want_shape else 0) | (_lib.LIB_PHARM if want_pharm else 0) |
+                     (_lib.LIB_SHAEP if want_se else 0))
     spec.max_conformers = max(n, 1)
     spec.max_points = max(rows, 1) if want_shape else 0
     spec.max_pharm = max(prows, 1) if want_pharm else 0
+    spec.max_surface = max(srows, 1) if want_se else 0
+    spec.max_atoms = max(arows, 1) if want_se else 0
     h = ctypes.c_void_p()
     _lib.call("fss_lib_create", _lib.byref(spec), ctypes.byref(h))
     ids = []
@@ -426,15 +442,19 @@ def load_library(store, mode: str, device=0, shards: Optional[Sequence[int]] = N
         for i in idx:
             sh, a = store.read_shard(i)
             shape, pharm = _shard_channels(a, m, store.schema)
-            args = [None] * 6
+            r = _lib.FssLibRows()
             if shape is not None:
-                args[0], args[1] = shape[0].ctypes.data, shape[1].ctypes.data
+                r.pts, r.counts = shape[0].ctypes.data, shape[1].ctypes.data
                 nbytes += shape[0].nbytes
             if pharm is not None:
-                args[2:] = [pharm[0].ctypes.data, pharm[1].ctypes.data, pharm[2].ctypes.data,
-                            pharm[3].ctypes.data]
+                r.anc, r.vec, r.typ, r.pcounts = (pharm[0].ctypes.data, pharm[1].ctypes.data,
+                                                  pharm[2].ctypes.data, pharm[3].ctypes.data)
                 nbytes += pharm[0].nbytes + pharm[1].nbytes + pharm[2].nbytes
-            _lib.call("fss_lib_upload", h, int(sh["n"]), *args, ctypes.byref(first))
+            if want_se:
+                se = _shard_shaep(a)
+                r.srf, r.scounts, r.atm, r.rad, r.acounts = (x.ctypes.data for x in se)
+                nbytes += se[0].nbytes + se[2].nbytes + se[3].nbytes
+            _lib.call("fss_lib_append", h, int(sh["n"]), _lib.byref(r), ctypes.byref(first))
             ids.append(np.asarray(a["ids"]))
     except Exception:
         lib.fss_lib_destroy(h)

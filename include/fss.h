@@ -290,14 +290,34 @@ typedef struct FssScreen FssScreen;
 
 #define FSS_LIB_SHAPE 1        /* rows (x, y, z, esp) per conformer */
 #define FSS_LIB_PHARM 2        /* anchors, unit vectors, types per conformer */
+#define FSS_LIB_SHAEP 4        /* ShaEP: surface rows (x, y, z, ESP), atoms with hydrogens
+                                  (x, y, z, partial charge) and their radii per conformer */
 
 typedef struct FssLibSpec {
   int32_t device;              /* HIP device ordinal */
-  int32_t channels;            /* FSS_LIB_SHAPE | FSS_LIB_PHARM */
+  int32_t channels;            /* FSS_LIB_SHAPE | FSS_LIB_PHARM | FSS_LIB_SHAEP */
   int64_t max_conformers;
   int64_t max_points;          /* total shape rows */
   int64_t max_pharm;           /* total pharmacophore rows */
+  int64_t max_surface;         /* total ShaEP surface rows */
+  int64_t max_atoms;           /* total ShaEP atom rows */
 } FssLibSpec;
+
+/* Rows of n conformers for fss_lib_append (host or device pointers; the channels the library
+ * was created with must be present, the others are ignored). */
+typedef struct FssLibRows {
+  const float* pts;            /* shape: (sum(counts), 4) */
+  const int32_t* counts;       /* (n) */
+  const float* anc;            /* pharm: (sum(pcounts), 4) */
+  const float* vec;            /* (sum(pcounts), 4) */
+  const int32_t* typ;          /* (sum(pcounts)) */
+  const int32_t* pcounts;      /* (n) */
+  const float* srf;            /* ShaEP: (sum(scounts), 4) surface points and ESP */
+  const int32_t* scounts;      /* (n) */
+  const float* atm;            /* (sum(acounts), 4) atoms with hydrogens and partial charges */
+  const float* rad;            /* (sum(acounts)) vdW radii */
+  const int32_t* acounts;      /* (n) */
+} FssLibRows;
 
 /* Allocates the library's device arrays for the given capacities. */
 int fss_lib_create(const FssLibSpec* spec, FssLibrary** out);
@@ -307,6 +327,9 @@ int fss_lib_create(const FssLibSpec* spec, FssLibrary** out);
 int fss_lib_upload(FssLibrary* lib, int32_t n, const float* pts, const int32_t* counts,
                    const float* anc, const float* vec, const int32_t* typ, const int32_t* pcounts,
                    int64_t* first);
+/* fss_lib_upload for every channel, the ShaEP one included.  Replaces: the with-H store
+ * reader's per-molecule views (screen.py:830-846). */
+int fss_lib_append(FssLibrary* lib, int32_t n, const FssLibRows* rows, int64_t* first);
 int fss_lib_size(const FssLibrary* lib, int64_t* n);
 /* Fails while a service on this library exists. */
 int fss_lib_destroy(FssLibrary* lib);
@@ -326,8 +349,13 @@ typedef struct FssScreenParams {
   int32_t max_query_points;    /* shape rows of one query */
   int32_t max_query_pharm;     /* pharmacophore rows of one query */
   int32_t lanes_per_pose, restarts_per_wg, variant;   /* fss_align tiling; 0 = defaults */
-  int32_t seed_flags;          /* FSS_SEED_PHARM: seed from the pharmacophore anchors */
+  int32_t seed_flags;          /* FSS_SEED_PHARM / FSS_SEED_SURF: seed from those rows */
   const float* quat_table;     /* (n_restarts, 4) host rotations of the restarts */
+  /* FSS_MODE_ESP_COMBO only */
+  int32_t max_query_surface;   /* ShaEP surface rows of one query */
+  int32_t max_query_atoms;     /* ShaEP atom rows of one query */
+  int32_t shape_from_surface;  /* 1: the shape term uses the surface rows (alpha != 0.81),
+                                  0: the shape channel's rows (heavy atoms) */
 } FssScreenParams;
 
 typedef struct FssQuery {      /* host arrays, copied by fss_screen_submit */
@@ -337,6 +365,11 @@ typedef struct FssQuery {      /* host arrays, copied by fss_screen_submit */
   const float* anc;            /* (n_pharm, 4) */
   const float* vec;            /* (n_pharm, 4) */
   const int32_t* typ;          /* (n_pharm) */
+  int32_t n_surface;           /* FSS_MODE_ESP_COMBO: */
+  const float* srf;            /* (n_surface, 4): x, y, z, ESP */
+  int32_t n_atoms;
+  const float* atm;            /* (n_atoms, 4): x, y, z, partial charge (with hydrogens) */
+  const float* rad;            /* (n_atoms) */
 } FssQuery;
 
 typedef struct FssHits {       /* host arrays of capacity entries, filled by fss_screen_poll */

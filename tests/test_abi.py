@@ -42,6 +42,8 @@ int main(void) {
   printf("%zu %zu %zu %zu\n", sizeof(FssLibSpec), sizeof(FssScreenParams), sizeof(FssQuery), sizeof(FssHits));
   printf("%zu %zu %zu %zu\n", offsetof(FssLibSpec, max_pharm), offsetof(FssScreenParams, quat_table),
          offsetof(FssQuery, typ), offsetof(FssHits, latency_ms));
+  printf("%zu %zu %zu %zu %zu\n", offsetof(FssLibSpec, max_atoms), offsetof(FssScreenParams, shape_from_surface),
+         offsetof(FssQuery, rad), sizeof(FssLibRows), offsetof(FssLibRows, acounts));
   return 0;
 }
 '''
@@ -58,8 +60,11 @@ int main(void) {
                           _lib.FssAlignParams.pair_order.offset, _lib.FssAlignOut.n_steps.offset]
     assert sizes[8:12] == [ctypes.sizeof(_lib.FssLibSpec), ctypes.sizeof(_lib.FssScreenParams),
                            ctypes.sizeof(_lib.FssQuery), ctypes.sizeof(_lib.FssHits)]
-    assert sizes[12:] == [_lib.FssLibSpec.max_pharm.offset, _lib.FssScreenParams.quat_table.offset,
-                          _lib.FssQuery.typ.offset, _lib.FssHits.latency_ms.offset]
+    assert sizes[12:16] == [_lib.FssLibSpec.max_pharm.offset, _lib.FssScreenParams.quat_table.offset,
+                            _lib.FssQuery.typ.offset, _lib.FssHits.latency_ms.offset]
+    assert sizes[16:] == [_lib.FssLibSpec.max_atoms.offset, _lib.FssScreenParams.shape_from_surface.offset,
+                          _lib.FssQuery.rad.offset, ctypes.sizeof(_lib.FssLibRows),
+                          _lib.FssLibRows.acounts.offset]
 
 
 def test_validation_rejects_bad_arguments_without_launching():
@@ -105,9 +110,13 @@ def test_screening_abi_rejects_bad_arguments_without_device_work():
     lib = _lib.load()
     spec = _lib.FssLibSpec()
     out = ctypes.c_void_p()
-    spec.channels, spec.max_conformers = 4, 10
+    spec.channels, spec.max_conformers = 8, 10
     assert lib.fss_lib_create(_lib.byref(spec), ctypes.byref(out)) == _lib.E_INVALID
     assert b"channels" in lib.fss_last_error()
+    spec.channels, spec.max_surface, spec.max_atoms = _lib.LIB_SHAEP, 0, 10
+    assert lib.fss_lib_create(_lib.byref(spec), ctypes.byref(out)) == _lib.E_INVALID
+    assert b"max_surface" in lib.fss_last_error()
+    assert lib.fss_lib_append(None, 1, None, None) == _lib.E_INVALID
     spec.channels, spec.max_conformers, spec.max_points = _lib.LIB_SHAPE, 0, 10
     assert lib.fss_lib_create(_lib.byref(spec), ctypes.byref(out)) == _lib.E_INVALID
     assert lib.fss_screen_create(None, None, None, ctypes.byref(out)) == _lib.E_INVALID

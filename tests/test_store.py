@@ -196,3 +196,29 @@ def test_schema_rules():
     assert not store_supports(schema_from_modes(["vol"]), "vol_esp")
     with pytest.raises(ValueError):
         schema_from_modes(["bogus"])
+
+
+def test_loader_shaep_rows_match_profiles(tmp_path):
+    """vol_and_surf_esp loads heavy atoms as the shape rows and the with-H ShaEP channel:
+    surface rows (x, y, z, ESP), atom rows (x, y, z, partial charge), radii and counts."""
+    from fast_shepherd_score_amd.store import _shard_shaep
+    path = os.path.join(tmp_path, "lib.fss")
+    profs = [_profile(i, with_h=True, total=7 + i) for i in range(5)]
+    with ProfileStore.create(path, num_surf_points=32, modes=("vol_and_surf_esp",), dtype="float32",
+                             pre_centered=False) as st:
+        for p in profs:
+            st.add(p)
+    st = ProfileStore.open(path)
+    _, a = st.read_shard(0)
+    (rows, counts), pharm = _shard_channels(a, "vol_and_surf_esp", st.schema)
+    assert pharm is None
+    np.testing.assert_array_equal(rows[:, :3], np.concatenate([p.atom_pos for p in profs]))
+    assert not rows[:, 3].any() and counts.tolist() == [len(p.atom_pos) for p in profs]
+    srf, sc, atm, rad, ac = _shard_shaep(a)
+    np.testing.assert_array_equal(srf[:, :3], np.concatenate([p.surf_pos for p in profs]))
+    np.testing.assert_array_equal(srf[:, 3], np.concatenate([p.surf_esp for p in profs]))
+    assert sc.tolist() == [32] * 5
+    np.testing.assert_array_equal(atm[:, :3], np.concatenate([p.centers_w_H for p in profs]))
+    np.testing.assert_array_equal(atm[:, 3], np.concatenate([p.partial_charges for p in profs]))
+    np.testing.assert_array_equal(rad, np.concatenate([p.radii for p in profs]))
+    assert ac.tolist() == [7 + i for i in range(5)]
