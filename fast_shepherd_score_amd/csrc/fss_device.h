@@ -724,6 +724,41 @@ __device__ __forceinline__ double shape_value_f64(const float4* __restrict__ ref
   return v;
 }
 
+// FP32 counterpart of shape_value_f64 (direct form, exp2 with the scale folded in: c_xyz =
+// -alpha/2 log2 e, c_w = -log2 e / lam) for the re-scoring pre-pass that picks the restarts
+// worth an FP64 score.
+template <bool ESP>
+__device__ __forceinline__ float shape_value_f32(const float4* __restrict__ ref, int N,
+                                                 const float4* __restrict__ fit, int M, int j, int lpp,
+                                                 const float R[9], const float t[3], float c_xyz, float c_w) {
+  float v0 = 0.f, v1 = 0.f;
+  for (int b = j; b < M; b += lpp) {
+    const float4 p = fit[b];
+    const float bx = fmaf(R[0], p.x, fmaf(R[1], p.y, fmaf(R[2], p.z, t[0])));
+    const float by = fmaf(R[3], p.x, fmaf(R[4], p.y, fmaf(R[5], p.z, t[1])));
+    const float bz = fmaf(R[6], p.x, fmaf(R[7], p.y, fmaf(R[8], p.z, t[2])));
+    int a = 0;
+    for (; a + 1 < N; a += 2) {
+#pragma unroll
+      for (int k = 0; k < 2; ++k) {
+        const float4 r = ref[a + k];
+        const float dx = r.x - bx, dy = r.y - by, dz = r.z - bz;
+        float arg = c_xyz * fmaf(dx, dx, fmaf(dy, dy, dz * dz));
+        if (ESP) { const float dc = r.w - p.w; arg = fmaf(c_w * dc, dc, arg); }
+        (k ? v1 : v0) += fast_exp2(arg);
+      }
+    }
+    if (a < N) {
+      const float4 r = ref[a];
+      const float dx = r.x - bx, dy = r.y - by, dz = r.z - bz;
+      float arg = c_xyz * fmaf(dx, dx, fmaf(dy, dy, dz * dz));
+      if (ESP) { const float dc = r.w - p.w; arg = fmaf(c_w * dc, dc, arg); }
+      v0 += fast_exp2(arg);
+    }
+  }
+  return v0 + v1;
+}
+
 __device__ __forceinline__ double pharm_value_f64(const float4* __restrict__ ranc,
                                                   const float4* __restrict__ rvec,
                                                   const int32_t* __restrict__ rtyp, int N,

@@ -731,12 +731,14 @@ void k_align(FssPairBatch pb, KScore ks, LdsPlan lp, AlignK ak) {
 // FP64 similarity of every pose, one wave per pose (4 poses per workgroup), straight from the
 // packed unscaled arrays.  pose_pair == NULL: pair = pose / B.
 // ------------------------------------------------------------------------------------------------
+// thr != NULL: only poses whose current score (the FP32 pre-pass) is >= thr[pair] are re-scored.
 template <int MODE>
 __global__ __launch_bounds__(256) void k_pose_score(FssPairBatch pb, KScore ks, int n_poses, int B,
                                                     const int32_t* __restrict__ pose_pair,
                                                     const double* __restrict__ self_ov,
                                                     const float* __restrict__ qin,
-                                                    const float* __restrict__ tin, double* score) {
+                                                    const float* __restrict__ tin, double* score,
+                                                    const double* __restrict__ thr) {
   constexpr bool SH = MODE != FSS_MODE_PHARM;
   constexpr bool ESP = MODE == FSS_MODE_SHAPE_ESP;
   constexpr bool PH = MODE == FSS_MODE_PHARM || MODE == FSS_MODE_SHAPE_PHARM;
@@ -744,6 +746,7 @@ __global__ __launch_bounds__(256) void k_pose_score(FssPairBatch pb, KScore ks, 
   const int j = threadIdx.x & 63;
   if (pose >= n_poses) return;
   const int pair = pose_pair ? pose_pair[pose] : pose / B;
+  if (thr && !(score[pose] >= thr[pair])) return;
   const float q[4] = {qin[4 * (long)pose], qin[4 * (long)pose + 1], qin[4 * (long)pose + 2], qin[4 * (long)pose + 3]};
   double qh[4], qn, Rd[9];
   rotation_fp64(q, qh, qn, Rd);
@@ -778,6 +781,53 @@ __global__ __launch_bounds__(256) void k_pose_score(FssPairBatch pb, KScore ks, 
   double ws, wp;
   const double S = similarity<double>(ks, vs, vp, self_ov + 4 * pair, SH, PH, ws, wp);
   if (j == 0) score[pose] = S;
+}
+
+// Re-scoring pre-pass of the shape modes: the FP32 similarity of every final pose (one wave per
+// pose, direct form), then per pair the threshold max - kRescoreMargin over its restarts.  Only
+// restarts at or above it get the FP64 score; the others keep the FP32 one, whose error (about
+// 1e-6) is far below the margin, so the FP64 argmax over the restarts is unchanged.
+constexpr double kRescoreMargin = 1e-3;
+
+template <int MODE>
+__global__ __launch_bounds__(256) void k_pose_score_f32(FssPairBatch pb, KScore ks, int n_poses, int B,
+                                                        const double* __restrict__ self_ov,
+                                                        const float* __restrict__ qin,
+                                                        const float* __restrict__ tin, double* score) {
+  static_assert(MODE == FSS_MODE_SHAPE || MODE == FSS_MODE_SHAPE_ESP, "shape modes only");
+  constexpr bool ESP = MODE == FSS_MODE_SHAPE_ESP;
+  const int pose = blockIdx.x * 4 + (threadIdx.x >> 6);
+  const int j = threadIdx.x & 63;
+  if (pose >= n_poses) return;
+  const int pair = pose / B;
+  const float q[4] = {qin[4 * (long)pose], qin[4 * (long)pose + 1], qin[4 * (long)pose + 2], qin[4 * (long)pose + 3]};
+  double qh[4], qn, Rd[9];
+  rotation_fp64(q, qh, qn, Rd);
+  float R[9];
+#pragma unroll
+  for (int i = 0; i < 9; ++i) R[i] = (float)Rd[i];
+  const float t[3] = {tin[3 * (long)pose], tin[3 * (long)pose + 1], tin[3 * (long)pose + 2]};
+  const float4* gref = reinterpret_cast<const float4*>(pb.ref_pts) + pb.ref_rng[2 * pair];
+  const float4* gfit = reinterpret_cast<const float4*>(pb.fit_pts) + pb.fit_rng[2 * pair];
+  float v = shape_value_f32<ESP>(gref, pb.ref_rng[2 * pair + 1], gfit, pb.fit_rng[2 * pair + 1], j, 64, R, t,
+                                 -ks.s * ks.s, ESP ? -ks.sc * ks.sc : 0.f);
+  v = group_sum(v, 64) * ks.K;
+  float ws, wp;
+  const float S = similarity<float>(ks, v, 0.f, self_ov + 4 * pair, true, false, ws, wp);
+  if (j == 0) score[pose] = (double)S;
+}
+
+// thr[pair] = (max over the pair's restarts of score) - kRescoreMargin, ignoring NaN scores.  A
+// NaN FP32 score (NaN inputs: the FP64 score is NaN too) is never a candidate.
+__global__ void k_rescore_threshold(int n_pairs, int B, const double* __restrict__ score, double* thr) {
+  const int pair = blockIdx.x * blockDim.x + threadIdx.x;
+  if (pair >= n_pairs) return;
+  double m = -INFINITY;
+  for (int r = 0; r < B; ++r) {
+    const double v = score[(long)pair * B + r];
+    if (v > m) m = v;
+  }
+  thr[pair] = m - kRescoreMargin;
 }
 
 // Per-pair early-stop bookkeeping of the stepped optimiser: mean loss over the pair's B poses,
@@ -1110,8 +1160,9 @@ int score_bwd_t(const FssPairBatch* batch, const FssScoreParams* sp, const T* re
 
 template <int MODE>
 void launch_pose_score(const FssPairBatch* pb, const KScore& ks, int n, int B, const int32_t* pp,
-                              const double* so, const float* q, const float* t, double* out, hipStream_t s) {
-  hipLaunchKernelGGL(k_pose_score<MODE>, dim3((n + 3) / 4), dim3(256), 0, s, *pb, ks, n, B, pp, so, q, t, out);
+                              const double* so, const float* q, const float* t, double* out, hipStream_t s,
+                              const double* thr = nullptr) {
+  hipLaunchKernelGGL(k_pose_score<MODE>, dim3((n + 3) / 4), dim3(256), 0, s, *pb, ks, n, B, pp, so, q, t, out, thr);
 }
 
 int pose_score_impl(const FssPairBatch* batch, const KScore& ks, int32_t n_poses,
@@ -1269,9 +1320,31 @@ __global__ __launch_bounds__(64) void k_seed_poses(FssPairBatch pb, int flags, i
   }
 }
 
+// Final scores and the best restart of every pair.  Shape modes: FP32 pre-pass over every final
+// pose, per-pair threshold (kept in best_score until k_align_finish overwrites it), FP64 for the
+// restarts at or above it.  The other modes score every final pose in FP64.
 int align_finish_impl(const FssPairBatch* batch, const KScore& ks, int B, const double* self_ov,
                       FssAlignOut* out, hipStream_t s) {
-  int rc = pose_score_impl(batch, ks, batch->n_pairs * B, nullptr, B, self_ov, out->q, out->t, out->pose_score, s);
+  const int n = batch->n_pairs * B;
+  int rc;
+  if (ks.mode == FSS_MODE_SHAPE || ks.mode == FSS_MODE_SHAPE_ESP) {
+    if (ks.mode == FSS_MODE_SHAPE)
+      hipLaunchKernelGGL(k_pose_score_f32<FSS_MODE_SHAPE>, dim3((n + 3) / 4), dim3(256), 0, s, *batch, ks, n, B,
+                         self_ov, out->q, out->t, out->pose_score);
+    else
+      hipLaunchKernelGGL(k_pose_score_f32<FSS_MODE_SHAPE_ESP>, dim3((n + 3) / 4), dim3(256), 0, s, *batch, ks, n, B,
+                         self_ov, out->q, out->t, out->pose_score);
+    hipLaunchKernelGGL(k_rescore_threshold, dim3((batch->n_pairs + 255) / 256), dim3(256), 0, s, batch->n_pairs, B,
+                       out->pose_score, out->best_score);
+    if ((rc = check_launch("fss_align (re-scoring pre-pass)"))) return rc;
+    if (ks.mode == FSS_MODE_SHAPE)
+      launch_pose_score<FSS_MODE_SHAPE>(batch, ks, n, B, nullptr, self_ov, out->q, out->t, out->pose_score, s, out->best_score);
+    else
+      launch_pose_score<FSS_MODE_SHAPE_ESP>(batch, ks, n, B, nullptr, self_ov, out->q, out->t, out->pose_score, s, out->best_score);
+    rc = check_launch("fss_align (FP64 re-scoring)");
+  } else {
+    rc = pose_score_impl(batch, ks, n, nullptr, B, self_ov, out->q, out->t, out->pose_score, s);
+  }
   if (rc) return rc;
   hipLaunchKernelGGL(k_align_finish, dim3((batch->n_pairs + 255) / 256), dim3(256), 0, s, batch->n_pairs, B,
                      out->q, out->t, out->pose_score, out->best_score, out->best_idx, out->best_tf);

@@ -105,6 +105,29 @@ def test_persistent_align_matches_oracle(dev, mode, early_stop):
             assert nst[i] == steps
 
 
+@pytest.mark.parametrize("mode", [_lib.MODE_SHAPE, _lib.MODE_SHAPE_ESP])
+def test_fp32_prepass_keeps_the_fp64_argmax(dev, mode):
+    """Shape modes re-score in FP64 only the restarts within 1e-3 of the pair's FP32 best: the best
+    restart and its score equal a full FP64 re-scoring of every restart (fss_pose_score), the
+    candidates' scores are bitwise the FP64 ones and the rest are within 1e-5 of FP64."""
+    print(f"case fp32 prepass mode={mode}", flush=True)
+    n, B, steps = 8, 16, 60
+    pb, spec, pairs = make_case(mode, n, dev, seed=21 + mode)
+    q, t = random_seeds(n, B, seed=22)
+    res = align(pb, spec, to_dev(q, dev), to_dev(t, dev), OptimSpec(max_steps=steps, early_stop=False))
+    so = self_overlaps(pb, spec)
+    pp = torch.arange(n, dtype=torch.int32, device=dev).repeat_interleave(B)
+    full = pose_score(pb, spec, pp, res.q.reshape(-1, 4).contiguous(), res.t.reshape(-1, 3).contiguous(),
+                      so).cpu().numpy().reshape(n, B)
+    got = res.pose_score.cpu().numpy().reshape(n, B)
+    np.testing.assert_allclose(got, full, atol=1e-5)
+    assert (got.argmax(1) == full.argmax(1)).all()
+    np.testing.assert_array_equal(res.score.cpu().numpy(), full.max(1))
+    cand = full >= full.max(1, keepdims=True) - 1e-3 + 2e-5      # safely inside the margin
+    np.testing.assert_array_equal(got[cand], full[cand])
+    assert (~cand).any()                                          # the pre-pass skipped some
+
+
 def test_paths_agree(dev):
     """Persistent, graph-captured stepped and plain per-pair paths: same optimiser."""
     n, B = 4, 8
