@@ -742,10 +742,11 @@ __global__ __launch_bounds__(256) void k_pose_score(FssPairBatch pb, KScore ks, 
   constexpr bool SH = MODE != FSS_MODE_PHARM;
   constexpr bool ESP = MODE == FSS_MODE_SHAPE_ESP;
   constexpr bool PH = MODE == FSS_MODE_PHARM || MODE == FSS_MODE_SHAPE_PHARM;
-  const int pose = blockIdx.x * 4 + (threadIdx.x >> 6);
+  // one pose per wavefront: wave-uniform, so the clouds are read with scalar loads
+  const int pose = __builtin_amdgcn_readfirstlane(blockIdx.x * 4 + (threadIdx.x >> 6));
   const int j = threadIdx.x & 63;
   if (pose >= n_poses) return;
-  const int pair = pose_pair ? pose_pair[pose] : pose / B;
+  const int pair = __builtin_amdgcn_readfirstlane(pose_pair ? pose_pair[pose] : pose / B);
   if (thr && !(score[pose] >= thr[pair])) return;
   const float q[4] = {qin[4 * (long)pose], qin[4 * (long)pose + 1], qin[4 * (long)pose + 2], qin[4 * (long)pose + 3]};
   double qh[4], qn, Rd[9];
@@ -796,7 +797,7 @@ __global__ __launch_bounds__(256) void k_pose_score_f32(FssPairBatch pb, KScore 
                                                         const float* __restrict__ tin, double* score) {
   static_assert(MODE == FSS_MODE_SHAPE || MODE == FSS_MODE_SHAPE_ESP, "shape modes only");
   constexpr bool ESP = MODE == FSS_MODE_SHAPE_ESP;
-  const int pose = blockIdx.x * 4 + (threadIdx.x >> 6);
+  const int pose = __builtin_amdgcn_readfirstlane(blockIdx.x * 4 + (threadIdx.x >> 6));
   const int j = threadIdx.x & 63;
   if (pose >= n_poses) return;
   const int pair = pose / B;
