@@ -372,6 +372,11 @@ def bench_store(n: int = 200_000, seed: int = 8):
             "disk_mb": disk / 1e6, "disk_mb_per_s": disk / 1e6 / t_load, "hbm_bytes": nb}
 
 
+# Secondary legs: name -> function(dev, **kw) returning its result dict.
+LEGS = {"c1": bench_c1, "c3": bench_c3, "screen": bench_screen, "store": lambda dev: bench_store(),
+        "shaep": bench_shaep, "score_loss": bench_score_loss, "screen_shaep": bench_screen_shaep}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -391,6 +396,8 @@ def main():
     ap.add_argument("--screen-lib", type=int, default=1_000_000)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="first run a plain 256 MiB device copy (HBM counter calibration for rocprofv3 --pmc)")
+    ap.add_argument("--only", choices=sorted(LEGS),
+                    help="run one secondary leg alone and print its JSON line (profiling)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: every box of this deployment has a single MI355X and the "
@@ -411,6 +418,10 @@ def main():
         dst.copy_(src)
         torch.cuda.synchronize()
         log("copy calibration: 268435456 bytes read + 268435456 bytes written")
+    if args.only:
+        kw = {"n_lib": args.screen_lib} if args.only == "screen" else {}
+        print(json.dumps({"leg": args.only, **LEGS[args.only](dev, **kw)}), flush=True)
+        return
     B, steps = args.restarts, args.opt_steps
     t_gen = time.perf_counter()
     wl = build_workload(args.workload, args.pairs, args.seed)
