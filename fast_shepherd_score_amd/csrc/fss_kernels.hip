@@ -229,17 +229,19 @@ __device__ __forceinline__ Staged stage_pair(const FssPairBatch& pb, const KScor
     float* lc = reinterpret_cast<float*>(lds + lp.refc);
     float* red = reinterpret_cast<float*>(lds + lp.red);
     const float s = ks.s, sc = ks.sc;
-    // ref centroid (wave butterflies, then one value set per wave through LDS)
+    // ref centroid: lane-strided sums in wavefront 0 and one butterfly, whatever the workgroup
+    // size, so that every tiling and every kernel staging this pair gets the same bits
     float cx = 0.f, cy = 0.f, cz = 0.f;
-    for (int i = tid; i < N; i += nt) {
-      const float4 v = ga[i];
-      cx += v.x; cy += v.y; cz += v.z;
+    if (tid < 64) {
+      for (int i = tid; i < N; i += 64) {
+        const float4 v = ga[i];
+        cx += v.x; cy += v.y; cz += v.z;
+      }
+      cx = group_sum(cx, 64); cy = group_sum(cy, 64); cz = group_sum(cz, 64);
+      if (tid == 0) { red[0] = cx; red[1] = cy; red[2] = cz; }
     }
-    cx = group_sum(cx, 64); cy = group_sum(cy, 64); cz = group_sum(cz, 64);
-    if ((tid & 63) == 0) { red[3 * (tid >> 6)] = cx; red[3 * (tid >> 6) + 1] = cy; red[3 * (tid >> 6) + 2] = cz; }
     __syncthreads();
-    cx = cy = cz = 0.f;
-    for (int w = 0; w < (nt >> 6); ++w) { cx += red[3 * w]; cy += red[3 * w + 1]; cz += red[3 * w + 2]; }
+    cx = red[0]; cy = red[1]; cz = red[2];
     const float invN = N > 0 ? 1.f / (float)N : 0.f;
     cx *= invN; cy *= invN; cz *= invN;
     if (MF) {

@@ -128,6 +128,21 @@ def test_fp32_prepass_keeps_the_fp64_argmax(dev, mode):
     assert (~cand).any()                                          # the pre-pass skipped some
 
 
+def test_workgroup_size_does_not_change_results(dev):
+    """Surface clouds (> 64 points): restarts_per_wg = B and 4 B (one wavefront vs four per
+    workgroup; the extra restart groups idle) give bitwise the same alignment, because the
+    staging centroid is summed the same way for any workgroup size."""
+    n, B = 5, 8
+    pb, spec, pairs = make_case(_lib.MODE_SHAPE_ESP, n, dev, seed=41, points=(70, 130))
+    assert pb.ref_counts.min() > 64
+    q, t = random_seeds(n, B, seed=42)
+    opt = OptimSpec(max_steps=40, lr=0.1, early_stop=True)
+    a, b = (align(pb, spec, to_dev(q, dev), to_dev(t, dev), opt, lanes_per_pose=8, restarts_per_wg=r)
+            for r in (B, 4 * B))
+    for name in ("score", "q", "t", "pose_score", "n_steps"):
+        assert torch.equal(getattr(a, name), getattr(b, name)), name
+
+
 def test_paths_agree(dev):
     """Persistent, graph-captured stepped and plain per-pair paths: same optimiser."""
     n, B = 4, 8
