@@ -138,22 +138,64 @@ def _shaep_sets(pairs: Sequence[MoleculePair], dev):
     return out
 
 
+def _trans_init_align(pairs, mode, kw):
+    """trans_init=True: the seeds of _initialize_se3_params_with_translations (5 + 10 per
+    centre; centres = the ref's atoms, or its pharmacophore anchors in pharm mode; _core.py:671,
+    1094, 1228, 1325) built per pair on the host, then one persistent launch per group of pairs
+    with the same restart count (the optimiser's mean loss and gradient scale are per pair)."""
+    from .seeds import translation_seeds
+    groups = {}
+    for p in pairs:
+        r, f = p.ref_molec, p.fit_molec
+        if mode == "pharm":
+            sref, sfit, cen = r.pharm_ancs, f.pharm_ancs, r.pharm_ancs
+        else:
+            if mode == "vol_and_surf_esp":        # seeded from the surface points (_torch.py:906)
+                sref, sfit = r.surf_pos, f.surf_pos
+            else:
+                rc, fc, _, _, _ = _mode_inputs([p], mode)
+                sref, sfit = rc[0], fc[0]
+            cen = r.atom_pos
+        if sref is None or sfit is None or cen is None:
+            raise ValueError(f"trans_init needs the {mode} clouds and the ref's translation centres")
+        q, t = translation_seeds(sref, sfit, cen)
+        if kw.get("early_stop", True) and len(q) > 1024:
+            raise ValueError(f"trans_init gives {len(q)} restarts for a pair; per-pair early stop keeps a "
+                             "pair's restarts in one workgroup (at most 1024, i.e. 101 centres)")
+        groups.setdefault(len(q), []).append((p, q, t))
+    res = None
+    for items in groups.values():
+        seeds = (np.stack([it[1] for it in items]), np.stack([it[2] for it in items]))
+        res = align_pairs([it[0] for it in items], mode, seeds=seeds, **kw)
+    return res
+
+
 def align_pairs(pairs: Sequence[MoleculePair], mode: str, *, alpha=0.81, lam=None,
                 color_weight=0.5, similarity="tanimoto", directionless=False,
                 num_repeats: int = 50, lr: float = 0.1, max_num_steps: int = 200,
                 early_stop: bool = True, seeds=None, device=None, probe_radius: float = 1.0,
-                esp_weight: float = 0.5, **tiling):
+                esp_weight: float = 0.5, trans_init: bool = False, **tiling):
     """Align every pair in one persistent launch; write results in place; return the result.
 
     ``seeds``: None computes the reference's multi-start seeds on the device (fss_seed_poses);
     "host" computes the same rule on the host in FP64 (seeds.initial_seeds); a (q0, t0) pair of
-    (n_pairs, S, 4) / (n_pairs, S, 3) arrays or tensors is used as given.
+    (n_pairs, S, 4) / (n_pairs, S, 3) arrays or tensors is used as given.  ``trans_init``: the
+    reference's translation-centre seeds instead (num_repeats is then unused); pairs with
+    different restart counts run as separate launches.
     """
     mode = canonical(mode)
     if mode not in HIP_MODES:
         raise NotImplementedError(f"mode {mode!r} is not implemented by the HIP backend")
     if not pairs:
         return None
+    if trans_init:
+        if seeds is not None:
+            raise ValueError("trans_init builds its own seeds: do not pass seeds")
+        kw = dict(alpha=alpha, lam=lam, color_weight=color_weight, similarity=similarity,
+                  directionless=directionless, lr=lr, max_num_steps=max_num_steps,
+                  early_stop=early_stop, device=device, probe_radius=probe_radius,
+                  esp_weight=esp_weight, **tiling)
+        return _trans_init_align(pairs, canonical(mode), kw)
     dev = torch.device(device) if device is not None else pairs[0].device
     rc, fc, rw, fw, ph = _mode_inputs(pairs, mode)
     kind = {"vol": _lib.MODE_SHAPE, "surf": _lib.MODE_SHAPE, "vol_esp": _lib.MODE_SHAPE_ESP,
@@ -220,8 +262,6 @@ class MoleculePairBatch:
     # -- shared plumbing -------------------------------------------------------------------
     def _run(self, mode, backend, return_aligned, fit_cloud, num_repeats, max_num_steps, **kw):
         resolve_backend(backend)
-        if kw.pop("trans_init", False):
-            raise NotImplementedError("trans_init=True seeds are not implemented by the HIP backend")
         if num_repeats is None:
             num_repeats = MODE_SEEDS[mode]
         if max_num_steps is None:

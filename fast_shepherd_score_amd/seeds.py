@@ -146,6 +146,36 @@ def initial_seeds(ref: np.ndarray, ref_n: np.ndarray, fit: np.ndarray, fit_n: np
     return q.astype(np.float32), t.astype(np.float32)
 
 
+def translation_seeds(ref: np.ndarray, fit: np.ndarray, centers: np.ndarray, per_trans: int = 10):
+    """Seeds of one pair with translation centres (trans_init=True), restating
+    ``_initialize_se3_params_with_translations`` (alignment/_torch.py:329-398):
+      * seed 0: identity rotation, translation ref_com - fit_com;
+      * seeds 1-4: the four principal-axis alignments, translation ref_com - R fit_com;
+      * then per_trans Fibonacci-sphere rotations (the 45-point table when per_trans == 45),
+        repeated for every centre c (centre-major), translation c - R fit_com: the rotated fit's
+        centroid lands on the centre.
+    Returns (q (5 + per_trans * C, 4), t (same, 3)) float32; the restart count ignores
+    num_repeats, as in the reference."""
+    ref = np.asarray(ref, np.float64).reshape(-1, 3)
+    fit = np.asarray(fit, np.float64).reshape(-1, 3)
+    centers = np.asarray(centers, np.float64).reshape(-1, 3)
+    C = len(centers)
+    S = 5 + int(per_trans) * C
+    q = np.zeros((S, 4))
+    q[:, 0] = 1.0
+    q[1:5] = pca_quaternions(ref[None], np.array([len(ref)]), fit[None], np.array([len(fit)]))[0]
+    if C:
+        q[5:] = np.tile(quats_from_fibo(int(per_trans)), (C, 1))
+    q = q.astype(np.float32).astype(np.float64)      # stored in float32, R built from those
+    R = rotmat(q)
+    rc, fc = ref.mean(0), fit.mean(0)
+    t = np.empty((S, 3))
+    t[0] = rc - fc
+    t[1:5] = rc[None] - R[1:5] @ fc
+    t[5:] = np.repeat(centers, int(per_trans), axis=0) - R[5:] @ fc
+    return q.astype(np.float32), t.astype(np.float32)
+
+
 def effective_restarts(num_repeats: int) -> int:
     """The reference's restart count rule: 1 stays 1, 2-4 become 5 (_torch.py:253-326)."""
     S = int(num_repeats)

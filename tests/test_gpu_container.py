@@ -109,3 +109,36 @@ def test_defaults_and_errors(dev):
         batch.align_with_vol(backend="numba")
     with pytest.raises(NotImplementedError):
         batch.align_with_pharm(extended_points=True)
+
+
+@pytest.mark.parametrize("mode", ["vol", "surf_esp", "pharm"])
+def test_trans_init_matches_oracle(dev, mode):
+    """trans_init=True: the translation-centre seeds (5 + 10 per ref atom, or per ref anchor in
+    pharm mode), one launch per restart count, against the FP64 oracle from the same seeds."""
+    from fast_shepherd_score_amd.seeds import translation_seeds
+    print(f"case container trans_init mode={mode}", flush=True)
+    refs, fits = _mols(3, 34)
+    pairs = [MoleculePair(r, f, device=dev) for r, f in zip(refs, fits)]
+    batch = MoleculePairBatch(pairs)
+    S = 25
+    if mode == "vol":
+        scores, _ = batch.align_with_vol(trans_init=True, max_num_steps=S, backend="hip")
+    elif mode == "surf_esp":
+        scores, _ = batch.align_with_surf_esp(alpha=0.81, lam=0.3, trans_init=True, max_num_steps=S, backend="hip")
+    else:
+        scores, _, _ = batch.align_with_pharm(trans_init=True, max_num_steps=S, backend="hip")
+    assert len({len(r.atom_pos) for r in refs}) > 1 or mode == "pharm"     # several launch groups
+    for i, (r, f) in enumerate(zip(refs, fits)):
+        if mode == "vol":
+            q0, t0 = translation_seeds(r.atom_pos, f.atom_pos, r.atom_pos)
+            pair = orc.Pair("shape", ref=r.atom_pos, fit=f.atom_pos)
+        elif mode == "surf_esp":
+            q0, t0 = translation_seeds(r.surf_pos, f.surf_pos, r.atom_pos)
+            pair = orc.Pair("esp", ref=r.surf_pos, fit=f.surf_pos, ref_esp=r.surf_esp, fit_esp=f.surf_esp,
+                            lam=0.3 * LAM_SCALING)
+        else:
+            q0, t0 = translation_seeds(r.pharm_ancs, f.pharm_ancs, r.pharm_ancs)
+            pair = orc.Pair("pharm", ref_types=r.pharm_types, fit_types=f.pharm_types, ref_anc=r.pharm_ancs,
+                            fit_anc=f.pharm_ancs, ref_vec=r.pharm_vecs, fit_vec=f.pharm_vecs)
+        o = orc.optimize(pair, q0, t0, lr=0.1, max_steps=S)
+        assert abs(scores[i] - o["score"]) < 1e-3, (mode, i, scores[i], o["score"])

@@ -153,5 +153,27 @@ def test_mode_registry_and_backend_names():
     # every registered mode is served by the HIP backend, through the legacy aliases too
     assert set(HIP_MODES) == set(MODE_ATTRS)
     assert MoleculePairBatch.align_with_esp_combo is MoleculePairBatch.align_with_vol_and_surf_esp
-    with pytest.raises(NotImplementedError):
-        MoleculePairBatch([]).align_with_vol_and_surf_esp(0.81, trans_init=True)
+    scores, aligned = MoleculePairBatch([]).align_with_vol_and_surf_esp(0.81, trans_init=True)
+    assert len(scores) == 0 and aligned == []
+
+
+def test_translation_seeds_rule():
+    """trans_init seeds (alignment/_torch.py:329-398): 5 + 10 per centre; the base five are the
+    ordinary rule's (identity + principal axes, centroid onto centroid); then the Fibonacci
+    rotations repeat centre by centre and put the fit centroid on each centre."""
+    from fast_shepherd_score_amd.seeds import (initial_seeds, quats_from_fibo, reference_table, rotmat,
+                                               translation_seeds)
+    rng = np.random.default_rng(3)
+    ref, fit = rng.normal(size=(14, 3)) * 2, rng.normal(size=(11, 3)) + 1.5
+    cen = rng.normal(size=(4, 3)) * 3
+    q, t = translation_seeds(ref, fit, cen)
+    assert q.shape == (45, 4) and t.shape == (45, 3) and q.dtype == np.float32
+    q5, t5 = initial_seeds(ref[None], np.array([14]), fit[None], np.array([11]), 5)
+    np.testing.assert_array_equal(q[:5], q5[0])
+    np.testing.assert_allclose(t[:5], t5[0], atol=1e-6)
+    np.testing.assert_array_equal(q[5:], np.tile(quats_from_fibo(10).astype(np.float32), (4, 1)))
+    moved = np.einsum("sij,j->si", rotmat(q.astype(np.float64)), fit.mean(0)) + t
+    np.testing.assert_allclose(moved[0], ref.mean(0), atol=1e-5)
+    np.testing.assert_allclose(moved[5:], np.repeat(cen, 10, axis=0), atol=1e-5)
+    q45, _ = translation_seeds(ref, fit, cen[:1], per_trans=45)   # the 50-restart table's rotations
+    np.testing.assert_array_equal(q45[5:], reference_table(50)[5:])
