@@ -5,8 +5,9 @@
 // over a resident library instead of streamed shards):
 //   * fss_screen_submit copies the query into a host FIFO and wakes the worker;
 //   * the worker admits queries into free slots (query rows copied into the slot's device arena),
-//     then repeatedly fills one launch of up to launch_pairs pairs from every admitted query
-//     (fss_plan_launch), runs fss_seed_poses -> fss_self_overlap -> fss_align on it and merges
+//     then repeatedly fills one launch of up to launch_pairs pairs from the admitted queries,
+//     oldest first (fss_plan_launch: the tail of one query and the head of the next share a
+//     launch), runs fss_seed_poses -> fss_self_overlap -> fss_align on it and merges
 //     every query's new pair scores into its top-k (k_topk_merge, double-buffered per slot);
 //   * a query whose library range is exhausted has its top-k copied to the host, and pollers
 //     waiting on its ticket are woken.
@@ -531,6 +532,8 @@ void worker_main(FssScreen* s) {
       }
       for (int i = 0; i < s->A; ++i)
         if (s->slots[i].used) act.push_back(i);
+      // oldest query first: launches are filled in submission (ticket) order
+      std::sort(act.begin(), act.end(), [&](int x, int y) { return s->slots[x].q.ticket < s->slots[y].q.ticket; });
       if (act.empty()) {
         if (s->stop) break;
         s->cv_work.wait(lk, [&] { return s->stop || !s->pending.empty(); });
@@ -579,23 +582,10 @@ extern "C" {
 int fss_plan_launch(int32_t n, const int64_t* remaining, int64_t capacity, int64_t* take) {
   if (n < 0 || (n > 0 && (!remaining || !take)) || capacity < 0)
     return set_error(FSS_E_INVALID, "bad launch plan arguments");
-  std::vector<int> live;
-  for (int i = 0; i < n; ++i) {
-    take[i] = 0;
-    if (remaining[i] > 0) live.push_back(i);
-  }
   int64_t left = capacity;
-  while (left > 0 && !live.empty()) {
-    const int64_t share = std::max<int64_t>(1, left / (int64_t)live.size());
-    std::vector<int> nxt;
-    for (int i : live) {
-      const int64_t k = std::min(std::min(share, remaining[i] - take[i]), left);
-      take[i] += k;
-      left -= k;
-      if (take[i] < remaining[i]) nxt.push_back(i);
-      if (left == 0) break;
-    }
-    live.swap(nxt);
+  for (int i = 0; i < n; ++i) {
+    take[i] = std::min(std::max<int64_t>(remaining[i], 0), left);
+    left -= take[i];
   }
   return FSS_OK;
 }

@@ -9,9 +9,10 @@ wraps:
 
 * admission: a query takes one of ``max_active`` slots of a device arena reserved when the
   service is created (sized from the HBM budget); the rest wait in FIFO order;
-* continuous batching: every launch is filled from *all* admitted queries (water-filling over
-  their remaining library ranges, :func:`plan_launch` / ``fss_plan_launch``), so a query that
-  arrives while others are on the GPU joins the very next launch;
+* continuous batching: every launch is filled from the admitted queries' remaining library
+  ranges, oldest first (:func:`plan_launch` / ``fss_plan_launch``): the tail of one query and
+  the head of the next share a launch, and a query that arrives while others are on the GPU
+  joins as soon as capacity frees;
 * per-pair results do not depend on the co-scheduled pairs (fixed tiling, per-pair seeds and
   early stop, fixed-order reductions) and the device-side top-k merge orders by (score
   descending, library index ascending), so a query's hits are identical alone and in the mix.
@@ -177,24 +178,16 @@ class ConformerLibrary:
 
 
 def plan_launch(remaining: Sequence[int], capacity: int) -> List[int]:
-    """Split ``capacity`` library entries over requests with ``remaining`` entries each:
-    water-filling, so every active request advances in every launch (continuous batching).
-    Host restatement of the native planner ``fss_plan_launch``."""
-    take = [0] * len(remaining)
-    left = capacity
-    live = [i for i, r in enumerate(remaining) if r > 0]
-    while left > 0 and live:
-        share = max(1, left // len(live))
-        nxt = []
-        for i in live:
-            k = min(share, remaining[i] - take[i], left)
-            take[i] += k
-            left -= k
-            if take[i] < remaining[i]:
-                nxt.append(i)
-            if left == 0:
-                break
-        live = nxt
+    """Fill ``capacity`` library entries from requests given oldest first, with ``remaining``
+    entries each: every request takes what it needs of what the earlier ones left (FIFO
+    filling; the tail of one query and the head of the next share a launch).  Host restatement
+    of the native planner ``fss_plan_launch``."""
+    take = []
+    left = max(int(capacity), 0)
+    for r in remaining:
+        k = min(max(int(r), 0), left)
+        take.append(k)
+        left -= k
     return take
 
 

@@ -277,9 +277,10 @@ int fss_score_bwd_f64(const FssPairBatch* batch, const FssScoreParams* sp, const
  * queries against all of it with continuous batching and keeps each query's top-k hits.
  *
  * The service owns one worker thread and one HIP stream (default priority) on the library's
- * device.  Every launch takes up to launch_pairs (query, library entry) pairs from all admitted
- * queries at once (water-filling: every admitted query advances in every launch), runs
- * fss_seed_poses, fss_self_overlap and fss_align on them, and merges each query's new pair
+ * device.  Every launch takes up to launch_pairs (query, library entry) pairs from the admitted
+ * queries, oldest first (the tail of one query and the head of the next share a launch; a query
+ * that arrives while others run joins as soon as capacity frees), runs fss_seed_poses,
+ * fss_self_overlap and fss_align on them, and merges each query's new pair
  * scores into its device-resident top-k (ties to the lower library index).  A query's hits do not
  * depend on the queries it shares launches with.  Device memory is allocated by fss_lib_create /
  * fss_screen_create only; submit and poll allocate none.  Calls on one service may come from any
@@ -396,8 +397,10 @@ int fss_screen_stats(FssScreen* svc, int64_t* launches, int64_t* pairs);
 /* Stops the worker; queries still queued or running fail with FSS_E_CLOSED.  No other call on
  * the service may be in progress or follow. */
 int fss_screen_destroy(FssScreen* svc);
-/* The service's launch planner (host only): splits capacity over n requests with remaining[i]
- * entries left each, water-filling so that every request with entries left advances. */
+/* The service's launch planner (host only): fills capacity from n requests, given oldest first,
+ * with remaining[i] entries left each: take[i] = min(remaining[i], what the earlier ones left).
+ * For whole-library queries this FIFO filling gives lower median and tail latency than equal
+ * shares (processor sharing) at the same throughput. */
 int fss_plan_launch(int32_t n, const int64_t* remaining, int64_t capacity, int64_t* take);
 
 #ifdef __cplusplus

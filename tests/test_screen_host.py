@@ -8,17 +8,14 @@ from fast_shepherd_score_amd.seeds import quats_from_fibo
 
 @pytest.mark.parametrize("rem,cap", [([10, 10, 10], 12), ([0, 5, 100], 64), ([3], 1000), ([7, 1], 2),
                                       ([1000] * 7, 65536), ([5, 0, 0, 9], 3)])
-def test_plan_launch_water_fills(rem, cap):
+def test_plan_launch_fills_oldest_first(rem, cap):
     take = plan_launch(rem, cap)
     assert sum(take) == min(cap, sum(rem))
     assert all(0 <= t <= r for t, r in zip(take, rem))
-    live = [i for i, r in enumerate(rem) if r > 0]
-    if cap >= len(live):          # every active request advances in every launch
-        assert all(take[i] > 0 for i in live)
-    # shares differ by at most one unless a request runs out
-    unsat = [take[i] for i in live if take[i] < rem[i]]
-    if unsat:
-        assert max(unsat) - min(unsat) <= 1 or len(unsat) == 1
+    # a request gets capacity only once every older one is fully served in this launch
+    for i, t in enumerate(take):
+        if t > 0:
+            assert all(take[j] == rem[j] for j in range(i))
 
 
 def test_arena_bytes_scale_with_the_service_shape():
