@@ -245,7 +245,7 @@ int fss_cpu_align(const FssPairBatch* pb, const FssScoreParams* sp, const FssAli
     for (int step = 1; step <= ap->max_steps; ++step) {
       b1p *= ap->beta1; b2p *= ap->beta2;
       const float step_size = (float)(ap->lr / (1.0 - b1p));
-      const float bc2s = (float)sqrt(1.0 - b2p);
+      const float ibc2s = (float)(1.0 / sqrt(1.0 - b2p));
       for (int r = 0; r < B; ++r) {
         double qh[4], qn, Rd[9];
         rotation(qp + 4 * r, qh, qn, Rd);
@@ -271,7 +271,7 @@ int fss_cpu_align(const FssPairBatch* pb, const FssScoreParams* sp, const FssAli
           float* par = k < 4 ? qp + 4 * r + k : tp + 3 * r + (k - 4);
           mm[k] = mm[k] + (1.f - ap->beta1) * (g[k] - mm[k]);
           vv[k] = vv[k] * ap->beta2 + (1.f - ap->beta2) * g[k] * g[k];
-          *par = *par - step_size * (mm[k] / (sqrtf(vv[k]) / bc2s + ap->eps));
+          *par = *par - step_size * (mm[k] * (1.f / std::fma(sqrtf(vv[k]), ibc2s, ap->eps)));
         }
       }
       if (ap->early_stop) {
