@@ -41,6 +41,8 @@ class ScoreSpec:
     color_weight: float = 0.5
     similarity: str = "tanimoto"
     directionless: bool = False
+    extended_points: bool = False           # pharmacophore extended points (categories 4, 5)
+    only_extended: bool = False
     esp_weight: float = 0.5                 # ESP_COMBO (vol_and_surf_esp) constants
     probe_radius: float = 1.0
     coulomb: float = _lib.COULOMB_SCALING
@@ -61,7 +63,7 @@ class ScoreSpec:
             s.similarity, s.sigma = _lib.SIM_TVERSKY, SIGMA_MAP[sim]
         else:
             raise ValueError("similarity must be one of (tanimoto, tversky, tversky_ref, tversky_fit)")
-        a, k, c = pharm_tables(self.directionless)
+        a, k, c = pharm_tables(self.directionless, self.extended_points, self.only_extended)
         for i in range(_lib.N_PTYPES):
             s.ph_alpha[i], s.ph_k[i], s.ph_cat[i] = a[i], k[i], c[i]
         return s

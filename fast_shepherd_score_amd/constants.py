@@ -45,12 +45,17 @@ def gaussian_k(alpha: float) -> float:
     return (math.pi / (2.0 * alpha)) ** 1.5
 
 
-def pharm_tables(directionless: bool = False):
+def pharm_tables(directionless: bool = False, extended_points: bool = False,
+                 only_extended: bool = False):
     """Per-type (alpha, K, category) lists in P_TYPES order.
 
     Category 0 = isotropic point Gaussian (w = 1), 1 = directional (cosine weight clamped to
     [0, 1]), 2 = aromatic (|cosine| weight), 3 = ignored (Dummy).  ``directionless`` maps every
-    real type to category 0 (ROCS-style colour), keeping the per-type alpha.
+    real type to category 0 (ROCS-style colour), keeping the per-type alpha, and turns extended
+    points off (pharmacophore_scoring.py:517-523).  ``extended_points`` scores the directional
+    types by the overlap of their extended points anchor + unit vector, plus the plain anchor
+    overlap (category 4) or without it when ``only_extended`` (category 5)
+    (pharmacophore_scoring.py:284-370, analytical_gradients/_torch.py:288-357).
     """
     alphas, ks, cats = [], [], []
     for p in P_TYPES_LWRCASE:
@@ -64,7 +69,7 @@ def pharm_tables(directionless: bool = False):
         elif p in _NONDIRECTIONAL:
             cats.append(0)
         elif p in _DIRECTIONAL:
-            cats.append(1)
+            cats.append((5 if only_extended else 4) if extended_points else 1)
         elif p in _AROMATIC:
             cats.append(2)
         else:

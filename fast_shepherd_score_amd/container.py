@@ -172,7 +172,7 @@ def _trans_init_align(pairs, mode, kw):
 
 def align_pairs(pairs: Sequence[MoleculePair], mode: str, *, alpha=0.81, lam=None,
                 color_weight=0.5, similarity="tanimoto", directionless=False,
-                num_repeats: int = 50, lr: float = 0.1, max_num_steps: int = 200,
+                extended_points=False, only_extended=False, num_repeats: int = 50, lr: float = 0.1, max_num_steps: int = 200,
                 early_stop: bool = True, seeds=None, device=None, probe_radius: float = 1.0,
                 esp_weight: float = 0.5, trans_init: bool = False, **tiling):
     """Align every pair in one persistent launch; write results in place; return the result.
@@ -181,7 +181,9 @@ def align_pairs(pairs: Sequence[MoleculePair], mode: str, *, alpha=0.81, lam=Non
     "host" computes the same rule on the host in FP64 (seeds.initial_seeds); a (q0, t0) pair of
     (n_pairs, S, 4) / (n_pairs, S, 3) arrays or tensors is used as given.  ``trans_init``: the
     reference's translation-centre seeds instead (num_repeats is then unused); pairs with
-    different restart counts run as separate launches.
+    different restart counts run as separate launches.  ``extended_points`` /
+    ``only_extended``: the pharmacophore channel scores acceptors, donors and halogens by
+    extended-point overlaps (pharmacophore_scoring.py:284-370); off when ``directionless``.
     """
     mode = canonical(mode)
     if mode not in HIP_MODES:
@@ -192,7 +194,8 @@ def align_pairs(pairs: Sequence[MoleculePair], mode: str, *, alpha=0.81, lam=Non
         if seeds is not None:
             raise ValueError("trans_init builds its own seeds: do not pass seeds")
         kw = dict(alpha=alpha, lam=lam, color_weight=color_weight, similarity=similarity,
-                  directionless=directionless, lr=lr, max_num_steps=max_num_steps,
+                  directionless=directionless, extended_points=extended_points,
+                  only_extended=only_extended, lr=lr, max_num_steps=max_num_steps,
                   early_stop=early_stop, device=device, probe_radius=probe_radius,
                   esp_weight=esp_weight, **tiling)
         return _trans_init_align(pairs, canonical(mode), kw)
@@ -213,6 +216,8 @@ def align_pairs(pairs: Sequence[MoleculePair], mode: str, *, alpha=0.81, lam=Non
         lam = 0.3 * LAM_SCALING if mode == "surf_esp" else 0.1
     spec = ScoreSpec(mode=kind, alpha=alpha, lam=lam, color_weight=color_weight,
                      similarity=similarity, directionless=directionless,
+                     extended_points=bool(extended_points) and not directionless,
+                     only_extended=bool(only_extended) and not directionless,
                      esp_weight=esp_weight, probe_radius=probe_radius)
     ref = CloudSet.from_arrays(rc, rw, device=dev) if rc is not None else None
     fit = CloudSet.from_arrays(fc, fw, device=dev) if fc is not None else None
@@ -345,10 +350,9 @@ class MoleculePairBatch:
                          num_workers: int = 1, use_shmap: bool = True, num_buckets: int = 1,
                          verbose: bool = False, backend: Optional[str] = None,
                          return_aligned: bool = False, **hip_kw):
-        if extended_points:
-            raise NotImplementedError("extended_points scoring is not implemented by the HIP backend")
         scores, _ = self._run("pharm", backend, False, None, num_repeats, max_num_steps,
                               trans_init=trans_init, lr=lr, similarity=similarity,
+                              extended_points=extended_points, only_extended=only_extended,
                               **self._hip_kw(hip_kw))
         n = len(self.pairs)
         if not return_aligned:
@@ -364,10 +368,14 @@ class MoleculePairBatch:
                              num_repeats: int = None, trans_init: bool = False, lr: float = 0.1,
                              max_num_steps: int = None, verbose: bool = False,
                              backend: Optional[str] = None, return_aligned: bool = False,
-                             directionless: bool = True, similarity: str = "tanimoto", **hip_kw):
-        """Shape + pharmacophore colour.  ``directionless=False`` keeps the cosine-weighted
-        directional pharmacophore channel (alignment/_torch.py:1376-1548)."""
+                             directionless: bool = True, similarity: str = "tanimoto",
+                             extended_points: bool = False, only_extended: bool = False,
+                             **hip_kw):
+        """Shape + pharmacophore colour.  ``directionless=False`` keeps the directional
+        pharmacophore channel, cosine-weighted or with ``extended_points``
+        (alignment/_torch.py:1376-1548, container/_core.py:1142-1236)."""
         return self._run("vol_color", backend, return_aligned, "atom_pos", num_repeats,
                          max_num_steps, trans_init=trans_init, lr=lr, alpha=alpha,
                          color_weight=color_weight, directionless=directionless,
-                         similarity=similarity, **self._hip_kw(hip_kw))
+                         similarity=similarity, extended_points=extended_points,
+                         only_extended=only_extended, **self._hip_kw(hip_kw))

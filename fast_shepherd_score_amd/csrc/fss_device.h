@@ -8,6 +8,8 @@
 //   pharm    same-type pairs only, alpha/K/category from the type; directional weight
 //            w = (clamp(D,0,1)+2)/3 (cat 1) or (|D|+2)/3 (cat 2), D = (R v_fit).v_ref, with the
 //            weight gradient (1/3) K E c v_ref v_fit^T          (:223-361)
+//            extended points (cat 4, 5): anchor weight 1 or 0, plus K exp(-alpha/2 |e_a - e_b'|^2)
+//            over the extended points e = anchor + unit vector  (:288-357)
 //   rotation R(q_hat) in (w,x,y,z) order and its four Jacobians (:28-139)
 //
 // Shape channel arithmetic runs on pre-scaled coordinates: with s = sqrt(alpha/2 * log2(e)) every
@@ -637,6 +639,12 @@ struct PhTab {
 // ---------------------------------------------------------------------------------------------
 // Pharmacophore channel, FP32.  anc[i] = (x, y, z, type bits), vec[i] = (unit x, y, z, 0); the
 // ref rows are sorted by type, so fit row b (type t) visits only ref rows [poff[t], poff[t+1]).
+// Categories 4 and 5 are the extended-point form of a directional type: the anchor pair term
+// has weight 1 (4) or 0 (5) and no vector factor, and a second isotropic term couples the
+// extended points anchor + unit vector of the two features.  The fit extended point moves with
+// the pose as b + u, so its force f_e enters dO/dt as f_e and dO/dR as f_e (x) (p + u0): it is
+// accumulated into both the anchor force f (outer product with p) and the vector force h
+// (outer product with u0), which category 4/5 leaves free (no cosine weight).
 // ---------------------------------------------------------------------------------------------
 template <bool GRAD>
 __device__ __forceinline__ void pharm_accum(const float4* __restrict__ ranc,
@@ -675,6 +683,8 @@ __device__ __forceinline__ void pharm_accum(const float4* __restrict__ ranc,
       } else if (cat == 2) {
         w = (fabsf(D) + 2.f) * (1.f / 3.f);
         c = D > 0.f ? 1.f : (D < 0.f ? -1.f : 0.f);
+      } else if (cat == 5) {
+        w = 0.f;
       }
       const float we = w * e;
       v = fmaf(kk, we, v);
@@ -683,6 +693,17 @@ __device__ __forceinline__ void pharm_accum(const float4* __restrict__ ranc,
         fx = fmaf(fw, dx, fx); fy = fmaf(fw, dy, fy); fz = fmaf(fw, dz, fz);
         const float hc = kk * (1.f / 3.f) * e * c;
         hx = fmaf(hc, rv.x, hx); hy = fmaf(hc, rv.y, hy); hz = fmaf(hc, rv.z, hz);
+      }
+      if (cat >= 4) {
+        const float ex = (r.x + rv.x) - (bx + ux), ey = (r.y + rv.y) - (by + uy),
+                    ez = (r.z + rv.z) - (bz + uz);
+        const float e2 = fast_exp2(-kap * fmaf(ex, ex, fmaf(ey, ey, ez * ez)));
+        v = fmaf(kk, e2, v);
+        if (GRAD) {
+          const float fe = ak * e2;
+          fx = fmaf(fe, ex, fx); fy = fmaf(fe, ey, fy); fz = fmaf(fe, ez, fz);
+          hx = fmaf(fe, ex, hx); hy = fmaf(fe, ey, hy); hz = fmaf(fe, ez, hz);
+        }
       }
     }
     acc.V += v;
@@ -798,7 +819,13 @@ __device__ __forceinline__ double pharm_value_f64(const float4* __restrict__ ran
       double w = 1.0;
       if (cat == 1) w = (fmin(fmax(D, 0.0), 1.0) + 2.0) / 3.0;
       else if (cat == 2) w = (fabs(D) + 2.0) / 3.0;
+      else if (cat == 5) w = 0.0;
       v += K * w * e;
+      if (cat >= 4) {   // extended points anchor + unit vector, isotropic
+        const double ex = (r.x + rv0.x / rn) - (bx + ux), ey = (r.y + rv0.y / rn) - (by + uy),
+                     ez = (r.z + rv0.z / rn) - (bz + uz);
+        v += K * exp(-ha * (ex * ex + ey * ey + ez * ez));
+      }
     }
   }
   return v;

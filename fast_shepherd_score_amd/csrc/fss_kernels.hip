@@ -93,6 +93,10 @@ int make_kscore(const FssScoreParams* sp, KScore& ks) {
   if (sp->mode == FSS_MODE_SHAPE_ESP && !(sp->lam > 0.0)) return set_error(FSS_E_INVALID, "lam must be > 0");
   if (sp->similarity != FSS_SIM_TANIMOTO && sp->similarity != FSS_SIM_TVERSKY)
     return set_error(FSS_E_INVALID, "unknown similarity %d", sp->similarity);
+  if (uses_pharm(sp->mode))
+    for (int i = 0; i < FSS_N_PTYPES; ++i)
+      if (sp->ph_cat[i] < 0 || sp->ph_cat[i] > 5)
+        return set_error(FSS_E_INVALID, "ph_cat[%d] = %d is not a category (0-5)", i, sp->ph_cat[i]);
   memset(&ks, 0, sizeof(ks));
   ks.mode = sp->mode;
   ks.alpha = (float)sp->alpha;

@@ -218,6 +218,8 @@ class ScreenService:
     ``max_num_steps`` default to the mode's table, ``lam`` is the raw lambda (scaled by
     LAM_SCALING for surf_esp and vol_and_surf_esp, defaults 0.3 / 0.1 / 0.001), and
     vol_and_surf_esp scores heavy atoms at alpha 0.81 and surface points otherwise.
+    ``extended_points`` / ``only_extended`` apply to the directional pharmacophore channel
+    (pharm, or vol_color with ``directionless=False``), as in screen.py:866-867.
     """
 
     def __init__(self, library: ConformerLibrary, mode: str = "vol", *, num_repeats: Optional[int] = None,
@@ -225,6 +227,7 @@ class ScreenService:
                  launch_pairs: int = 65536, hbm_budget_bytes: int = 8 << 30, max_active: int = 16,
                  early_stop: bool = True, alpha: float = 0.81, lam: Optional[float] = None,
                  similarity: str = "tanimoto", directionless: bool = True,
+                 extended_points: bool = False, only_extended: bool = False,
                  lanes_per_pose: Optional[int] = None, max_query_points: int = 4096,
                  max_query_pharm: int = 256, esp_weight: float = 0.5, probe_radius: float = 1.0,
                  max_query_surface: int = 4096, max_query_atoms: int = 1024, device=None):
@@ -242,7 +245,10 @@ class ScreenService:
         elif lam is None:
             lam = 0.1
         self.spec = ScoreSpec(mode=_KIND[mode], alpha=alpha, lam=lam, similarity=similarity,
-                              directionless=directionless, esp_weight=esp_weight,
+                              directionless=directionless,
+                              extended_points=bool(extended_points) and not directionless,
+                              only_extended=bool(only_extended) and not directionless,
+                              esp_weight=esp_weight,
                               probe_radius=probe_radius)
         self.B = int(MODE_SEEDS[mode] if num_repeats is None else num_repeats)
         steps = MODE_STEPS[mode] if max_num_steps is None else max_num_steps

@@ -98,7 +98,11 @@ typedef struct FssScoreParams {
   double  sigma;             /* Tversky sigma                                                */
   double  ph_alpha[FSS_N_PTYPES];   /* per-type pharmacophore alpha                          */
   double  ph_k[FSS_N_PTYPES];       /* per-type (pi/(2 alpha))^1.5                           */
-  int32_t ph_cat[FSS_N_PTYPES];     /* 0 isotropic, 1 directional, 2 aromatic, 3 ignored     */
+  int32_t ph_cat[FSS_N_PTYPES];     /* 0 isotropic, 1 directional, 2 aromatic, 3 ignored;
+                                       extended points (pharmacophore_scoring.py:284-370):
+                                       4 anchor (weight 1) + extended-point overlap,
+                                       5 extended-point overlap only; the extended point is
+                                       anchor + unit vector                                  */
   double  esp_weight;        /* w of FSS_MODE_ESP_COMBO                                      */
   double  probe_radius;      /* observer points within radius + probe of an atom are masked  */
   double  coulomb;           /* ESP = coulomb * sum q / d (1e4 / (4 * 55.263 * pi))          */

@@ -120,11 +120,22 @@ void pharm_grad(const PhCloud& A, const PhCloud& Bc, const float R[9], const flo
       double w = 1, c = 0;
       if (cat == 1) { w = (fmin(fmax(D, 0.0), 1.0) + 2) / 3; c = (D > 0 && D < 1) ? 1 : 0; }
       else if (cat == 2) { w = (fabs(D) + 2) / 3; c = D > 0 ? 1 : (D < 0 ? -1 : 0); }
+      else if (cat == 5) w = 0;
       O += K * w * e;
       const double fw = al * K * w * e;
       fx += fw * dx; fy += fw * dy; fz += fw * dz;
       const double hc = K / 3.0 * e * c;
       hx += hc * vx; hy += hc * vy; hz += hc * vz;
+      if (cat >= 4) {
+        // extended points anchor + unit vector (ag/_torch.py:333-357): the fit point moves as
+        // R (p + u0) + t, so its force adds to both the anchor and the vector outer products
+        const double ex = (r[0] + vx) - (bx + ux), ey = (r[1] + vy) - (by + uy), ez = (r[2] + vz) - (bz + uz);
+        const double e2 = exp(-0.5 * al * (ex * ex + ey * ey + ez * ez));
+        O += K * e2;
+        const double fe = al * K * e2;
+        fx += fe * ex; fy += fe * ey; fz += fe * ez;
+        hx += fe * ex; hy += fe * ey; hz += fe * ez;
+      }
     }
     gt[0] += fx; gt[1] += fy; gt[2] += fz;
     G[0] += fx * p[0] + hx * ux0; G[1] += fx * p[1] + hx * uy0; G[2] += fx * p[2] + hx * uz0;

@@ -156,9 +156,12 @@ def esp_pair_weights(c_fit, c_ref, lam):
     return torch.exp(-((c_fit.unsqueeze(-1) - c_ref.unsqueeze(-2)) ** 2) / lam)
 
 
-def overlap_grad_pharm(R, t, rtyp, ftyp, ranc, fanc, rvec, fvec, directionless=False):
-    """Typed directional pharmacophore overlap and gradients.  ag/_torch.py:223-361
-    (extended_points=False).  All inputs batched (B, ...)."""
+def overlap_grad_pharm(R, t, rtyp, ftyp, ranc, fanc, rvec, fvec, directionless=False,
+                       extended_points=False, only_extended=False):
+    """Typed directional pharmacophore overlap and gradients.  ag/_torch.py:223-361.
+    All inputs batched (B, ...).  ``extended_points``: directional (category 1) pairs lose
+    their cosine weight (anchor weight 1, or 0 with ``only_extended``) and add an isotropic
+    overlap of the extended points anchor + unit vector (ag/_torch.py:288-297, 319-357)."""
     rtyp = rtyp.long()
     ftyp = ftyp.long()
     fanc_t = torch.bmm(R, fanc.permute(0, 2, 1)).permute(0, 2, 1) + t.unsqueeze(1)
@@ -173,7 +176,10 @@ def overlap_grad_pharm(R, t, rtyp, ftyp, ranc, fanc, rvec, fvec, directionless=F
     d2 = ((fanc_t.unsqueeze(2) - ranc.unsqueeze(1)) ** 2).sum(-1)
     E = torch.exp(-a_ab / 2.0 * d2) * mask.to(DT)
     D = torch.bmm(fvn, rvn.permute(0, 2, 1))
-    w = torch.where(c_ab == 1, (D.clamp(0.0, 1.0) + 2.0) / 3.0,
+    w_dir = (D.clamp(0.0, 1.0) + 2.0) / 3.0
+    if extended_points:
+        w_dir = torch.full_like(D, 0.0 if only_extended else 1.0)
+    w = torch.where(c_ab == 1, w_dir,
                     torch.where(c_ab == 2, (D.abs() + 2.0) / 3.0, torch.ones_like(D)))
     wE = w * E
     O = (K_ab * wE).sum(dim=(1, 2))
@@ -184,10 +190,25 @@ def overlap_grad_pharm(R, t, rtyp, ftyp, ranc, fanc, rvec, fvec, directionless=F
     delta = s_ref.unsqueeze(-1) * fanc_t - torch.bmm(aKwE, ranc)
     grad_R = torch.bmm(delta.transpose(1, 2), fanc)
     cdir = ((D > 0.0) & (D < 1.0)).to(DT)
+    if extended_points:
+        cdir = torch.zeros_like(D)             # constant anchor weight: no weight gradient
     carom = torch.sign(D)
     c = torch.where(c_ab == 1, cdir, torch.where(c_ab == 2, carom, torch.zeros_like(D)))
     coeff = (1.0 / 3.0) * K_ab * E * c
     grad_R = grad_R + torch.bmm(torch.bmm(rvn.transpose(1, 2), coeff.transpose(1, 2)), fvn0)
+    if extended_points:
+        # the fit extended point moves as R (p + u0) + t
+        fext = fanc_t + fvn
+        rext = ranc + rvn
+        Ee = torch.exp(-a_ab / 2.0 * ((fext.unsqueeze(2) - rext.unsqueeze(1)) ** 2).sum(-1))
+        Ee = Ee * (mask & (c_ab == 1)).to(DT)
+        O = O + (K_ab * Ee).sum(dim=(1, 2))
+        aKE = -a_ab * K_ab * Ee
+        se_ref = aKE.sum(2)
+        se_fit = aKE.sum(1)
+        grad_t = grad_t + (se_ref.unsqueeze(-1) * fext).sum(1) - (se_fit.unsqueeze(-1) * rext).sum(1)
+        de = se_ref.unsqueeze(-1) * fext - torch.bmm(aKE, rext)
+        grad_R = grad_R + torch.bmm(de.transpose(1, 2), fanc + fvn0)
     return O, grad_R, grad_t
 
 
@@ -231,6 +252,8 @@ class Pair:
     color_weight: float = 0.5
     similarity: str = "tanimoto"
     directionless: bool = False
+    extended_points: bool = False
+    only_extended: bool = False
 
     def __post_init__(self):
         f = lambda a: None if a is None else torch.as_tensor(np.asarray(a, np.float64), dtype=DT)
@@ -242,6 +265,10 @@ class Pair:
         self.tref_typ, self.tfit_typ = g(self.ref_types), g(self.fit_types)
         self.has_shape = self.mode in ("shape", "esp", "shape_pharm")
         self.has_pharm = self.mode in ("pharm", "shape_pharm")
+        if self.directionless:     # pharmacophore_scoring.py:517-523
+            self.extended_points = self.only_extended = False
+        self.ph_kw = dict(directionless=self.directionless, extended_points=self.extended_points,
+                          only_extended=self.only_extended)
 
     # -- self-overlaps (ag/_torch.py:364-389, 519-541, 569-599)
     def self_overlaps(self):
@@ -258,10 +285,10 @@ class Pair:
         if self.has_pharm:
             out["VAA_p"] = overlap_grad_pharm(I, z, self.tref_typ[None], self.tref_typ[None], self.tref_anc[None],
                                               self.tref_anc[None], self.tref_vec[None], self.tref_vec[None],
-                                              self.directionless)[0][0]
+                                              **self.ph_kw)[0][0]
             out["VBB_p"] = overlap_grad_pharm(I, z, self.tfit_typ[None], self.tfit_typ[None], self.tfit_anc[None],
                                               self.tfit_anc[None], self.tfit_vec[None], self.tfit_vec[None],
-                                              self.directionless)[0][0]
+                                              **self.ph_kw)[0][0]
         return out
 
     def channel_overlaps(self, R, t):
@@ -278,7 +305,7 @@ class Pair:
             e = lambda a: a.expand(B, *a.shape)
             res["pharm"] = overlap_grad_pharm(R, t, e(self.tref_typ), e(self.tfit_typ), e(self.tref_anc),
                                               e(self.tfit_anc), e(self.tref_vec), e(self.tfit_vec),
-                                              self.directionless)
+                                              **self.ph_kw)
         return res
 
     def loss_grad_Rt(self, R, t, so):
@@ -336,7 +363,7 @@ class Pair:
             anc = apply_se3(self.tfit_anc.expand(B, -1, -1), T)
             vec = apply_so3(self.tfit_vec.expand(B, -1, -1), T)
             total = total + wp * pharm_similarity(self.tref_typ, self.tfit_typ, self.tref_anc, anc,
-                                                  self.tref_vec, vec, self.similarity, self.directionless)
+                                                  self.tref_vec, vec, self.similarity, **self.ph_kw)
         return total
 
 
@@ -378,10 +405,16 @@ def _vab_cos(c1, c2, v1, v2, alpha, antiparallel):
     return e.sum(dim=(1, 2))
 
 
-def pharm_similarity(t1, t2, a1, a2, v1, v2, similarity="tanimoto", directionless=False):
+def pharm_similarity(t1, t2, a1, a2, v1, v2, similarity="tanimoto", directionless=False,
+                     extended_points=False, only_extended=False):
     """Per-type sum of (cosine-weighted) overlaps, then Tanimoto / Tversky.
-    pharmacophore_scoring.py:123-145, 422-912 (extended_points=False).
+    pharmacophore_scoring.py:123-145, 422-912; with ``extended_points`` the directional types
+    take plain anchor overlaps (skipped with ``only_extended``) plus overlaps of the extended
+    points anchor + unit vector (pharmacophore_scoring.py:284-370, vectors normalised as in
+    the batched call, :562-563); directionless scoring turns extended points off (:517-523).
     t1 (N,), t2 (M,) shared by every pose; a2, v2 (B,M,3) posed; a1, v1 (N,3)."""
+    if directionless:
+        extended_points = only_extended = False
     B = a2.shape[0]
     a1b, v1b = a1.expand(B, -1, -1), v1.expand(B, -1, -1)
     VAB = torch.zeros(B, dtype=DT)
@@ -398,6 +431,10 @@ def pharm_similarity(t1, t2, a1, a2, v1, v2, similarity="tanimoto", directionles
         c1, c2 = a1b[:, m1], a2[:, m2]
         if directionless or name in ("hydrophobe", "znbinder", "anion", "cation"):
             f = lambda x, y, vx, vy: vab(x, y, al)
+        elif extended_points and name in ("acceptor", "donor", "halogen"):
+            def f(x, y, vx, vy, al=al):
+                ext = vab(x + F.normalize(vx, p=2, dim=-1), y + F.normalize(vy, p=2, dim=-1), al)
+                return ext if only_extended else vab(x, y, al) + ext
         else:
             anti = name == "aromatic"
             f = lambda x, y, vx, vy: _vab_cos(x, y, vx, vy, al, anti)

@@ -13,10 +13,13 @@ MODE_NAME = {_lib.MODE_SHAPE: "shape", _lib.MODE_SHAPE_ESP: "esp", _lib.MODE_PHA
 
 
 def make_case(mode, n_pairs, dev, seed=0, atoms=(20, 40), points=(40, 80), pharms=(4, 12),
-              similarity="tanimoto", directionless=False, alpha=0.81):
-    """Returns (PairBatch, ScoreSpec, [oracle.Pair])."""
+              similarity="tanimoto", directionless=False, alpha=0.81, extended_points=False,
+              only_extended=False, vec_scale=1.0):
+    """Returns (PairBatch, ScoreSpec, [oracle.Pair]).  ``vec_scale`` multiplies the
+    pharmacophore vectors (both sides see the same non-unit vectors)."""
     lam = 0.3 * LAM_SCALING
-    spec = ScoreSpec(mode=mode, alpha=alpha, lam=lam, similarity=similarity, directionless=directionless)
+    spec = ScoreSpec(mode=mode, alpha=alpha, lam=lam, similarity=similarity, directionless=directionless,
+                     extended_points=extended_points, only_extended=only_extended)
     if mode == _lib.MODE_SHAPE:
         ref, fit = synth.vol_pairs(n_pairs, atoms[0], atoms[1], seed=seed)
         pb = PairBatch(CloudSet.from_arrays(ref, device=dev), CloudSet.from_arrays(fit, device=dev))
@@ -28,6 +31,9 @@ def make_case(mode, n_pairs, dev, seed=0, atoms=(20, 40), points=(40, 80), pharm
                  for a, c, b, d in zip(r, re, f, fe)]
     else:
         d = synth.pharm_shape_pairs(n_pairs, atoms=atoms, pharms=pharms, seed=seed)
+        for side in ("ref", "fit"):
+            d[f"{side}_vectors"] = [np.asarray(v, np.float32) * np.float32(vec_scale)
+                                    for v in d[f"{side}_vectors"]]
         ref_ph = PharmSet.from_arrays(d["ref_types"], d["ref_anchors"], d["ref_vectors"], device=dev)
         fit_ph = PharmSet.from_arrays(d["fit_types"], d["fit_anchors"], d["fit_vectors"], device=dev)
         ref = CloudSet.from_arrays(d["ref_atoms"], device=dev)
@@ -37,7 +43,8 @@ def make_case(mode, n_pairs, dev, seed=0, atoms=(20, 40), points=(40, 80), pharm
         pairs = [orc.Pair(name, ref=d["ref_atoms"][i], fit=d["fit_atoms"][i], ref_types=d["ref_types"][i],
                           fit_types=d["fit_types"][i], ref_anc=d["ref_anchors"][i], fit_anc=d["fit_anchors"][i],
                           ref_vec=d["ref_vectors"][i], fit_vec=d["fit_vectors"][i], alpha=alpha,
-                          similarity=similarity, directionless=directionless)
+                          similarity=similarity, directionless=directionless,
+                          extended_points=extended_points, only_extended=only_extended)
                  for i in range(n_pairs)]
     return pb, spec, pairs
 

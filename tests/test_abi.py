@@ -75,6 +75,11 @@ def test_validation_rejects_bad_arguments_without_launching():
     pb = _lib.FssPairBatch()
     rc = lib.fss_self_overlap(_lib.byref(pb), _lib.byref(sp), None, None)
     assert rc == _lib.E_INVALID and b"unknown mode" in lib.fss_last_error()
+    sp.mode, sp.alpha = _lib.MODE_PHARM, 0.81
+    sp.ph_cat[2] = 6          # categories are 0-5 (5: extended point only)
+    rc = lib.fss_self_overlap(_lib.byref(pb), _lib.byref(sp), None, None)
+    assert rc == _lib.E_INVALID and b"ph_cat[2] = 6" in lib.fss_last_error()
+    sp.ph_cat[2] = 0
     sp.mode, sp.alpha = _lib.MODE_SHAPE, 0.81
     rc = lib.fss_self_overlap(_lib.byref(pb), _lib.byref(sp), None, None)
     assert rc == _lib.E_INVALID and b"NULL" in lib.fss_last_error()

@@ -31,9 +31,12 @@ def _mols(n, seed):
     return out
 
 
-@pytest.mark.parametrize("mode", ["vol", "surf_esp", "pharm", "vol_color", "vol_esp"])
+@pytest.mark.parametrize("mode", ["vol", "surf_esp", "pharm", "vol_color", "vol_esp", "pharm_ext",
+                                  "pharm_only_ext", "vol_color_ext"])
 def test_align_with_matches_oracle(dev, mode):
     print(f"case container mode={mode}", flush=True)
+    ext = dict(extended_points=mode.endswith("ext"), only_extended=mode == "pharm_only_ext")
+    mode = mode.replace("_only_ext", "").replace("_ext", "")
     refs, fits = _mols(4, 30)
     pairs = [MoleculePair(r, f, device=dev) for r, f in zip(refs, fits)]
     batch = MoleculePairBatch(pairs)
@@ -49,11 +52,12 @@ def test_align_with_matches_oracle(dev, mode):
                                                     backend="triton", return_aligned=True)
         attr, cloud = "transform_surf_esp", "surf_pos"
     elif mode == "pharm":
-        scores, aligned, vecs = batch.align_with_pharm(num_repeats=R, max_num_steps=S, backend="hip", return_aligned=True)
+        scores, aligned, vecs = batch.align_with_pharm(num_repeats=R, max_num_steps=S, backend="hip",
+                                                       return_aligned=True, **ext)
         attr, cloud = "transform_pharm", "pharm_ancs"
     else:
         scores, aligned = batch.align_with_vol_color(num_repeats=R, max_num_steps=S, backend="hip",
-                                                     return_aligned=True, directionless=False)
+                                                     return_aligned=True, directionless=False, **ext)
         attr, cloud = "transform_vol_color", "atom_pos"
     assert scores.dtype == np.float64 and scores.shape == (4,)
     for i, p in enumerate(pairs):
@@ -77,9 +81,9 @@ def test_align_with_matches_oracle(dev, mode):
             pair = orc.Pair("esp", ref=r.surf_pos, fit=f.surf_pos, ref_esp=r.surf_esp, fit_esp=f.surf_esp,
                             lam=0.3 * LAM_SCALING)
         elif mode == "pharm":
-            pair = orc.Pair("pharm", **kw)
+            pair = orc.Pair("pharm", **kw, **ext)
         else:
-            pair = orc.Pair("shape_pharm", ref=r.atom_pos, fit=f.atom_pos, **kw)
+            pair = orc.Pair("shape_pharm", ref=r.atom_pos, fit=f.atom_pos, **kw, **ext)
         o = orc.optimize(pair, q0[0], t0[0], lr=0.1, max_steps=S)
         assert abs(scores[i] - o["score"]) < 1e-3, (mode, i, scores[i], o["score"])
 
@@ -107,8 +111,6 @@ def test_defaults_and_errors(dev):
     assert batch.pairs[0].sim_aligned_vol_noH == scores[0]
     with pytest.raises(ValueError):
         batch.align_with_vol(backend="numba")
-    with pytest.raises(NotImplementedError):
-        batch.align_with_pharm(extended_points=True)
 
 
 @pytest.mark.parametrize("mode", ["vol", "surf_esp", "pharm"])

@@ -187,3 +187,32 @@ def test_store_library_screens_like_in_memory(dev, tmp_path, mode):
     assert [h.score for h in a] == [h.score for h in b] == [h.score for h in c]
     assert [h.id for h in a] == [f"mol{h.id}" for h in b] == [h.id for h in c]
     assert a[0].id == "mol33"
+
+
+@pytest.mark.parametrize("only", [False, True])
+def test_pharm_screen_extended_points_match_oracle(dev, only):
+    """pharm screening with extended points (screen.py:866-867): per-pair best scores equal the
+    FP64 oracle run from the screen's seeds (restart table, anchor centroids)."""
+    print(f"case screen pharm extended only={only}", flush=True)
+    rng = np.random.default_rng(8)
+    atoms = synth.random_walks(rng, rng.integers(10, 25, size=30))
+    f32 = lambda x: np.asarray(x, np.float32).astype(np.float64)   # the values the device holds
+    ph = [(t, f32(a), f32(v)) for t, a, v in (synth.pharmacophores(rng, m, int(rng.integers(4, 10)))
+                                               for m in atoms)]
+    lib = ConformerLibrary.from_arrays(pharm_types=[p[0].astype(np.int32) for p in ph],
+                                       pharm_anchors=[p[1] for p in ph], pharm_vectors=[p[2] for p in ph],
+                                       device=dev)
+    qi = 4
+    query = {"pharm_types": ph[qi][0].astype(np.int32), "pharm_anchors": ph[qi][1], "pharm_vectors": ph[qi][2]}
+    B, steps = 8, 60
+    hits = screen(query, lib, "pharm", top_k=30, num_repeats=B, max_num_steps=steps,
+                  directionless=False, extended_points=True, only_extended=only)
+    table = restart_table(B).astype(np.float64)
+    got = {h.id: h.score for h in hits}
+    for i in (0, 4, 13, 29):
+        ra, fa = ph[qi][1], ph[i][1]
+        t0 = ra.mean(0)[None] - np.einsum("bij,j->bi", rotmat(table), fa.mean(0))
+        pair = orc.Pair("pharm", ref_types=ph[qi][0], fit_types=ph[i][0], ref_anc=ra, fit_anc=fa,
+                        ref_vec=ph[qi][2], fit_vec=ph[i][2], extended_points=True, only_extended=only)
+        o = orc.optimize(pair, table, t0, max_steps=steps)
+        assert abs(got[i] - o["score"]) < 1e-3, (i, got[i], o["score"])

@@ -134,6 +134,12 @@ def test_constants_and_tables():
     a, k, c = pharm_tables()
     assert c == [1, 1, 2, 0, 1, 0, 0, 0, 3]
     assert pharm_tables(directionless=True)[2] == [0, 0, 0, 0, 0, 0, 0, 0, 3]
+    # extended points: acceptor, donor, halogen -> 4 (anchor + extended) or 5 (extended only);
+    # directionless scoring turns them off (pharmacophore_scoring.py:517-523)
+    assert pharm_tables(extended_points=True)[2] == [4, 4, 2, 0, 4, 0, 0, 0, 3]
+    assert pharm_tables(extended_points=True, only_extended=True)[2] == [5, 5, 2, 0, 5, 0, 0, 0, 3]
+    assert pharm_tables(True, True, True)[2] == pharm_tables(directionless=True)[2]
+    assert list(ScoreSpec(mode=_lib.MODE_PHARM, extended_points=True).struct().ph_cat) == [4, 4, 2, 0, 4, 0, 0, 0, 3]
     assert a[2] == 0.7 and a[3] == 0.7 and a[0] == 1.0
     assert ALPHA(200) == pytest.approx(1.118, abs=1e-9)
     s = ScoreSpec(similarity="tversky_fit").struct()

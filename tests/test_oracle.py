@@ -134,6 +134,37 @@ def test_pharm_overlap_special_cases():
     assert float(O) == pytest.approx(K1)
 
 
+def test_pharm_extended_points_special_cases():
+    """Extended points (pharmacophore_scoring.py:284-370): anchors at the origin, so the
+    extended points sit at the unit vectors and |e_a - e_b|^2 = |v_a - v_b|^2."""
+    I = torch.eye(3, dtype=DT)[None]
+    z = torch.zeros((1, 3), dtype=DT)
+    a = torch.zeros((1, 1, 3), dtype=DT)
+    up = torch.tensor([[[0.0, 0.0, 1.0]]], dtype=DT)
+    side = torch.tensor([[[1.0, 0.0, 0.0]]], dtype=DT)
+    def ov(tr, tf, vr, vf, only=False, directionless=False):
+        return float(orc.overlap_grad_pharm(I, z, torch.tensor([[tr]]), torch.tensor([[tf]]), a, a, vr, vf,
+                                            directionless=directionless, extended_points=True,
+                                            only_extended=only)[0])
+    K1, K07 = orc.K_of(1.0), orc.K_of(0.7)
+    assert ov(0, 0, up, up) == pytest.approx(2 * K1)                           # anchor + ext
+    assert ov(1, 1, up, -up) == pytest.approx(K1 * (1 + math.exp(-0.5 * 4)))   # donor, |2 z|^2 = 4
+    assert ov(4, 4, up, side, only=True) == pytest.approx(K1 * math.exp(-0.5 * 2))  # halogen, ext only
+    assert ov(0, 0, up, 3.0 * side, only=True) == pytest.approx(K1 * math.exp(-0.5 * 2))  # unit vectors
+    assert ov(0, 0, 0 * up, 0 * up, only=True) == pytest.approx(K1)           # zero vector: e = anchor
+    assert ov(2, 2, up, -up) == pytest.approx(K07)                             # aromatic unchanged
+    assert ov(3, 3, up, -up, only=True) == pytest.approx(K07)                  # hydrophobe unchanged
+    assert ov(0, 0, up, -up, directionless=True) == pytest.approx(K1)          # directionless wins
+    # scoring restatement: one acceptor pair up/side -> VAB = K(1 + e^-1), VAA = VBB = 2K
+    S = orc.pharm_similarity(torch.tensor([0]), torch.tensor([0]), a[0], a, up[0], side,
+                             extended_points=True)
+    vab = K1 * (1 + math.exp(-1.0))
+    assert float(S[0]) == pytest.approx(vab / (4 * K1 - vab), rel=1e-12)
+    S = orc.pharm_similarity(torch.tensor([0]), torch.tensor([0]), a[0], a, up[0], side,
+                             extended_points=True, only_extended=True)
+    assert float(S[0]) == pytest.approx(math.exp(-1.0) / (2 - math.exp(-1.0)), rel=1e-12)
+
+
 def _finite_difference(pair, se3, h=1e-6):
     so = pair.self_overlaps()
     g = torch.zeros_like(se3)
@@ -163,10 +194,16 @@ def _pairs():
         orc.Pair("pharm", similarity="tversky", **ph),
         orc.Pair("shape_pharm", **ph),
         orc.Pair("shape_pharm", directionless=True, **ph),
+        orc.Pair("pharm", extended_points=True, **ph),
+        orc.Pair("pharm", extended_points=True, only_extended=True, similarity="tversky_ref", **ph),
+        orc.Pair("shape_pharm", extended_points=True, **ph),
     ]
 
 
-@pytest.mark.parametrize("k", range(6))
+N_PAIRS = 9
+
+
+@pytest.mark.parametrize("k", range(N_PAIRS))
 def test_se3_gradient_finite_difference(k):
     pair = _pairs()[k]
     rng = np.random.default_rng(20 + k)
@@ -176,7 +213,7 @@ def test_se3_gradient_finite_difference(k):
     np.testing.assert_allclose(g.numpy(), fd.numpy(), atol=1e-8, rtol=1e-5)
 
 
-@pytest.mark.parametrize("k", range(6))
+@pytest.mark.parametrize("k", range(N_PAIRS))
 def test_se3_gradient_vs_autograd_of_scoring(k):
     """Analytic gradient == autograd of the independently restated scoring functions."""
     pair = _pairs()[k]
