@@ -24,7 +24,7 @@ import numpy as np
 import torch
 
 from . import _lib
-from .align import OptimSpec, ScoreSpec, align
+from .align import OptimSpec, ScoreSpec, align, pose_score, self_overlaps
 from .constants import LAM_SCALING
 from .modes import HIP_MODES, MODE_ATTRS, MODE_SEEDS, MODE_STEPS, canonical
 from .packing import CloudSet, PairBatch, PharmSet
@@ -82,13 +82,134 @@ class MoleculePair:
         self.fit_molec = fit_mol.centered() if do_center else fit_mol
         self.num_surf_points = num_surf_points
         self.device = torch.device(device) if device is not None else torch.device("cuda")
-        for _, (tf, sc) in MODE_ATTRS.items():
+        for tf, sc in list(MODE_ATTRS.values()) + list(_ATTRS_WITH_H.values()):
             setattr(self, tf, None)
             setattr(self, sc, None)
 
+    # -- single-pair API (container/_core.py:612-1357) ---------------------------------------
+    # The same persistent launch as the batch, over one pair.  Like the reference's per-pair
+    # methods it stores the transform as a numpy (4, 4) array and the score as a numpy scalar,
+    # and returns the aligned fit cloud.  ``use_jax`` / ``use_analytical`` / ``verbose`` are
+    # accepted and have no effect: the HIP kernels are the implementation.
+    def _align_one(self, mode, cloud, num_repeats, max_num_steps, no_H=True, **kw):
+        if num_repeats is None:
+            num_repeats = MODE_SEEDS[mode]
+        if max_num_steps is None:
+            max_num_steps = MODE_STEPS[mode]
+        align_pairs([self], mode, num_repeats=num_repeats, max_num_steps=max_num_steps, no_H=no_H, **kw)
+        tf_attr, sc_attr = result_attrs(mode, no_H)
+        T = getattr(self, tf_attr).numpy()
+        setattr(self, tf_attr, T)
+        setattr(self, sc_attr, np.asarray(getattr(self, sc_attr)))
+        return self._moved(cloud, T)
 
-def _mode_inputs(pairs: Sequence[MoleculePair], mode: str):
-    """Host arrays (ref clouds, fit clouds, ref w, fit w, pharm dict) a mode consumes."""
+    def _moved(self, cloud, T):
+        X = np.asarray(getattr(self.fit_molec, cloud), np.float64)
+        Td = np.asarray(T, np.float64)
+        return (X @ Td[:3, :3].T + Td[:3, 3]).astype(np.float32)
+
+    def align_with_vol(self, no_H: bool = True, num_repeats: int = None, trans_init: bool = False,
+                       lr: float = 0.1, max_num_steps: int = None, use_jax: bool = False,
+                       use_analytical: bool = True, verbose: bool = False) -> np.ndarray:
+        return self._align_one("vol", "atom_pos" if no_H else "centers_w_H", num_repeats,
+                               max_num_steps, no_H=no_H, trans_init=trans_init, lr=lr)
+
+    def align_with_vol_esp(self, lam: float = 0.1, no_H: bool = True, num_repeats: int = None,
+                           trans_init: bool = False, lr: float = 0.1, max_num_steps: int = None,
+                           use_jax: bool = False, use_analytical: bool = True,
+                           verbose: bool = False) -> np.ndarray:
+        """``lam`` is raw here, as in the reference (no LAM_SCALING)."""
+        return self._align_one("vol_esp", "atom_pos" if no_H else "centers_w_H", num_repeats,
+                               max_num_steps, no_H=no_H, trans_init=trans_init, lr=lr, lam=lam)
+
+    def align_with_surf(self, alpha: float, num_repeats: int = None, trans_init: bool = False,
+                        lr: float = 0.1, max_num_steps: int = None, use_jax: bool = False,
+                        use_analytical: bool = True, verbose: bool = False) -> np.ndarray:
+        return self._align_one("surf", "surf_pos", num_repeats, max_num_steps,
+                               trans_init=trans_init, lr=lr, alpha=alpha)
+
+    def align_with_surf_esp(self, alpha: float, lam: float = 0.3, num_repeats: int = None,
+                            trans_init: bool = False, lr: float = 0.1, max_num_steps: int = None,
+                            use_jax: bool = False, use_analytical: bool = True,
+                            verbose: bool = False) -> np.ndarray:
+        """``lam`` is scaled by LAM_SCALING, as in the reference."""
+        return self._align_one("surf_esp", "surf_pos", num_repeats, max_num_steps,
+                               trans_init=trans_init, lr=lr, alpha=alpha, lam=LAM_SCALING * lam)
+
+    def align_with_vol_and_surf_esp(self, alpha: float, lam: float = 0.001,
+                                    probe_radius: float = 1.0, esp_weight: float = 0.5,
+                                    num_repeats: int = None, trans_init: bool = False,
+                                    lr: float = 0.1, max_num_steps: int = None,
+                                    use_jax: bool = False, verbose: bool = False) -> np.ndarray:
+        return self._align_one("vol_and_surf_esp", "surf_pos", num_repeats, max_num_steps,
+                               trans_init=trans_init, lr=lr, alpha=alpha, lam=lam,
+                               probe_radius=probe_radius, esp_weight=esp_weight)
+
+    align_with_esp = align_with_surf_esp                  # legacy aliases (_core.py:1138-1139)
+    align_with_esp_combo = align_with_vol_and_surf_esp
+
+    def align_with_vol_color(self, color_weight: float = 0.5, alpha: float = 0.81,
+                             similarity: str = "tanimoto", directionless: bool = True,
+                             extended_points: bool = False, only_extended: bool = False,
+                             num_repeats: int = None, trans_init: bool = False, lr: float = 0.1,
+                             max_num_steps: int = None, verbose: bool = False) -> np.ndarray:
+        return self._align_one("vol_color", "atom_pos", num_repeats, max_num_steps,
+                               trans_init=trans_init, lr=lr, alpha=alpha, color_weight=color_weight,
+                               similarity=similarity, directionless=directionless,
+                               extended_points=extended_points, only_extended=only_extended)
+
+    def align_with_pharm(self, similarity: str = "tanimoto", extended_points: bool = False,
+                         only_extended: bool = False, num_repeats: int = None,
+                         trans_init: bool = False, lr: float = 0.1, max_num_steps: int = None,
+                         use_jax: bool = False, verbose: bool = False, use_vectorized: bool = True,
+                         use_analytical: bool = True):
+        """Returns (aligned anchors, aligned vectors), as the reference does."""
+        anchors = self._align_one("pharm", "pharm_ancs", num_repeats, max_num_steps,
+                                  trans_init=trans_init, lr=lr, similarity=similarity,
+                                  extended_points=extended_points, only_extended=only_extended)
+        R = np.asarray(self.transform_pharm, np.float64)[:3, :3]
+        vectors = (np.asarray(self.fit_molec.pharm_vecs, np.float64) @ R.T).astype(np.float32)
+        return anchors, vectors
+
+    # -- scores of the current alignment (_core.py:1359-1560), FP64 on the device ----------
+    # ``use`` is validated as the reference does; every value runs the HIP re-scoring kernel.
+    # Pharmacophore vectors are normalised, as in the reference's batched scoring
+    # (pharmacophore_scoring.py:562-563); unit vectors score the same either way.
+    def _score_now(self, mode, use, **kw):
+        use = use.lower()
+        accepted = ("jax", "jnp", "torch", "pytorch", "np", "numpy")
+        if use not in accepted:
+            raise ValueError(f"`use` must be in {accepted}. Instead {use} was passed.")
+        return score_pairs([self], mode, **kw)
+
+    def score_with_surf(self, alpha: float, use: str = "np") -> np.ndarray:
+        return self._score_now("surf", use, alpha=alpha)
+
+    def score_with_esp(self, alpha: float, lam: float = 0.3, use: str = "np") -> np.ndarray:
+        return self._score_now("surf_esp", use, alpha=alpha, lam=LAM_SCALING * lam)
+
+    def score_with_pharm(self, similarity: str = "tanimoto", extended_points: bool = False,
+                         only_extended: bool = False, use: str = "np") -> np.ndarray:
+        return self._score_now("pharm", use, similarity=similarity, extended_points=extended_points,
+                               only_extended=only_extended)
+
+
+# vol / vol_esp with hydrogens (no_H=False) write the attributes without the _noH suffix
+_ATTRS_WITH_H = {"vol": ("transform_vol", "sim_aligned_vol"),
+                 "vol_esp": ("transform_vol_esp", "sim_aligned_vol_esp")}
+
+
+def result_attrs(mode: str, no_H: bool = True):
+    """(transform attribute, score attribute) a mode writes (_modes.py:18-26, _core.py:705-710)."""
+    if not no_H and mode in _ATTRS_WITH_H:
+        return _ATTRS_WITH_H[mode]
+    return MODE_ATTRS[mode]
+
+
+def _mode_inputs(pairs: Sequence[MoleculePair], mode: str, no_H: bool = True):
+    """Host arrays (ref clouds, fit clouds, ref w, fit w, pharm dict) a mode consumes.
+    ``no_H=False`` (vol, vol_esp) takes the atoms with hydrogens ``centers_w_H`` and their
+    ``partial_charges``, as ``Molecule.get_positions(no_H=False)`` does (_core.py:332-350)."""
     R = [p.ref_molec for p in pairs]
     F = [p.fit_molec for p in pairs]
     need = lambda ms, a: [getattr(m, a) for m in ms]
@@ -98,6 +219,12 @@ def _mode_inputs(pairs: Sequence[MoleculePair], mode: str):
         return vals
     ph = None
     rw = fw = None
+    if not no_H and mode in ("vol", "vol_esp"):
+        rc, fc = check(need(R, "centers_w_H"), "centers_w_H"), check(need(F, "centers_w_H"), "centers_w_H")
+        if mode == "vol_esp":
+            rw = check(need(R, "partial_charges"), "partial_charges")
+            fw = check(need(F, "partial_charges"), "partial_charges")
+        return rc, fc, rw, fw, None
     if mode in ("vol", "vol_color"):
         rc, fc = check(need(R, "atom_pos"), "atom_pos"), check(need(F, "atom_pos"), "atom_pos")
     elif mode == "vol_esp":
@@ -153,7 +280,7 @@ def _trans_init_align(pairs, mode, kw):
             if mode == "vol_and_surf_esp":        # seeded from the surface points (_torch.py:906)
                 sref, sfit = r.surf_pos, f.surf_pos
             else:
-                rc, fc, _, _, _ = _mode_inputs([p], mode)
+                rc, fc, _, _, _ = _mode_inputs([p], mode, kw.get("no_H", True))
                 sref, sfit = rc[0], fc[0]
             cen = r.atom_pos
         if sref is None or sfit is None or cen is None:
@@ -174,14 +301,15 @@ def align_pairs(pairs: Sequence[MoleculePair], mode: str, *, alpha=0.81, lam=Non
                 color_weight=0.5, similarity="tanimoto", directionless=False,
                 extended_points=False, only_extended=False, num_repeats: int = 50, lr: float = 0.1, max_num_steps: int = 200,
                 early_stop: bool = True, seeds=None, device=None, probe_radius: float = 1.0,
-                esp_weight: float = 0.5, trans_init: bool = False, **tiling):
+                esp_weight: float = 0.5, trans_init: bool = False, no_H: bool = True, **tiling):
     """Align every pair in one persistent launch; write results in place; return the result.
 
     ``seeds``: None computes the reference's multi-start seeds on the device (fss_seed_poses);
     "host" computes the same rule on the host in FP64 (seeds.initial_seeds); a (q0, t0) pair of
     (n_pairs, S, 4) / (n_pairs, S, 3) arrays or tensors is used as given.  ``trans_init``: the
     reference's translation-centre seeds instead (num_repeats is then unused); pairs with
-    different restart counts run as separate launches.  ``extended_points`` /
+    different restart counts run as separate launches.  ``no_H=False`` (vol, vol_esp) aligns
+    the atoms with hydrogens and writes ``transform_vol[_esp]``.  ``extended_points`` /
     ``only_extended``: the pharmacophore channel scores acceptors, donors and halogens by
     extended-point overlaps (pharmacophore_scoring.py:284-370); off when ``directionless``.
     """
@@ -197,10 +325,43 @@ def align_pairs(pairs: Sequence[MoleculePair], mode: str, *, alpha=0.81, lam=Non
                   directionless=directionless, extended_points=extended_points,
                   only_extended=only_extended, lr=lr, max_num_steps=max_num_steps,
                   early_stop=early_stop, device=device, probe_radius=probe_radius,
-                  esp_weight=esp_weight, **tiling)
+                  esp_weight=esp_weight, no_H=no_H, **tiling)
         return _trans_init_align(pairs, canonical(mode), kw)
+    pb, spec, (rc, fc, ph), dev = _pair_batch(pairs, mode, alpha, lam, color_weight, similarity,
+                                              directionless, extended_points, only_extended,
+                                              probe_radius, esp_weight, no_H, device)
+    combo = mode == "vol_and_surf_esp"
+    # Seeds come from the clouds the objective moves: pharm anchors for pharm, the surface
+    # points for vol_and_surf_esp (alignment/_torch.py:906), else the shape clouds.
+    if seeds is None:
+        q0, t0 = device_seeds(pb, num_repeats, use_pharm=mode == "pharm", use_surface=combo)
+    elif isinstance(seeds, str) and seeds == "host":
+        sref = ph["ref_anchors"] if mode == "pharm" else ([p.ref_molec.surf_pos for p in pairs] if combo else rc)
+        sfit = ph["fit_anchors"] if mode == "pharm" else ([p.fit_molec.surf_pos for p in pairs] if combo else fc)
+        (R, rn), (Fp, fn) = pad(sref), pad(sfit)
+        q0, t0 = initial_seeds(R, rn, Fp, fn, num_repeats)
+    else:
+        q0, t0 = seeds
+    q0 = torch.as_tensor(np.asarray(q0, np.float32) if not torch.is_tensor(q0) else q0,
+                         dtype=torch.float32).to(dev)
+    t0 = torch.as_tensor(np.asarray(t0, np.float32) if not torch.is_tensor(t0) else t0,
+                         dtype=torch.float32).to(dev)
+    opt = OptimSpec(max_steps=int(max_num_steps), lr=float(lr), early_stop=bool(early_stop))
+    res = align(pb, spec, q0, t0, opt, **tiling)
+    scores = res.score.cpu().numpy()
+    tfs = res.transform.cpu()
+    tf_attr, sc_attr = result_attrs(mode, no_H)
+    for i, p in enumerate(pairs):
+        setattr(p, tf_attr, tfs[i].clone())
+        setattr(p, sc_attr, float(scores[i]))
+    return res
+
+
+def _pair_batch(pairs, mode, alpha, lam, color_weight, similarity, directionless, extended_points,
+                only_extended, probe_radius, esp_weight, no_H, device):
+    """The device PairBatch and ScoreSpec of a mode (with its default lam), and the host clouds."""
     dev = torch.device(device) if device is not None else pairs[0].device
-    rc, fc, rw, fw, ph = _mode_inputs(pairs, mode)
+    rc, fc, rw, fw, ph = _mode_inputs(pairs, mode, no_H)
     kind = {"vol": _lib.MODE_SHAPE, "surf": _lib.MODE_SHAPE, "vol_esp": _lib.MODE_SHAPE_ESP,
             "surf_esp": _lib.MODE_SHAPE_ESP, "pharm": _lib.MODE_PHARM,
             "vol_color": _lib.MODE_SHAPE_PHARM, "vol_and_surf_esp": _lib.MODE_ESP_COMBO}[mode]
@@ -229,30 +390,29 @@ def align_pairs(pairs: Sequence[MoleculePair], mode: str, *, alpha=0.81, lam=Non
     if combo:
         rse, fse = _shaep_sets(pairs, dev)
     pb = PairBatch(ref, fit, rph, fph, ref_se=rse, fit_se=fse)
-    # Seeds come from the clouds the objective moves: pharm anchors for pharm, the surface
-    # points for vol_and_surf_esp (alignment/_torch.py:906), else the shape clouds.
-    if seeds is None:
-        q0, t0 = device_seeds(pb, num_repeats, use_pharm=mode == "pharm", use_surface=combo)
-    elif isinstance(seeds, str) and seeds == "host":
-        sref = ph["ref_anchors"] if mode == "pharm" else ([p.ref_molec.surf_pos for p in pairs] if combo else rc)
-        sfit = ph["fit_anchors"] if mode == "pharm" else ([p.fit_molec.surf_pos for p in pairs] if combo else fc)
-        (R, rn), (Fp, fn) = pad(sref), pad(sfit)
-        q0, t0 = initial_seeds(R, rn, Fp, fn, num_repeats)
-    else:
-        q0, t0 = seeds
-    q0 = torch.as_tensor(np.asarray(q0, np.float32) if not torch.is_tensor(q0) else q0,
-                         dtype=torch.float32).to(dev)
-    t0 = torch.as_tensor(np.asarray(t0, np.float32) if not torch.is_tensor(t0) else t0,
-                         dtype=torch.float32).to(dev)
-    opt = OptimSpec(max_steps=int(max_num_steps), lr=float(lr), early_stop=bool(early_stop))
-    res = align(pb, spec, q0, t0, opt, **tiling)
-    scores = res.score.cpu().numpy()
-    tfs = res.transform.cpu()
-    tf_attr, sc_attr = MODE_ATTRS[mode]
-    for i, p in enumerate(pairs):
-        setattr(p, tf_attr, tfs[i].clone())
-        setattr(p, sc_attr, float(scores[i]))
-    return res
+    return pb, spec, (rc, fc, ph), dev
+
+
+def score_pairs(pairs: Sequence[MoleculePair], mode: str, *, alpha=0.81, lam=None,
+                color_weight=0.5, similarity="tanimoto", directionless=False,
+                extended_points=False, only_extended=False, probe_radius: float = 1.0,
+                esp_weight: float = 0.5, no_H: bool = True, device=None) -> np.ndarray:
+    """FP64 similarity of every pair at its current pose (no alignment), (n_pairs,) float64:
+    the re-scoring kernel at the identity pose (gaussian_overlap.py:73-106,
+    electrostatic_scoring.py:69-125, pharmacophore_scoring.py:422-912)."""
+    mode = canonical(mode)
+    if mode not in HIP_MODES:
+        raise NotImplementedError(f"mode {mode!r} is not implemented by the HIP backend")
+    if not pairs:
+        return np.zeros(0)
+    pb, spec, _, dev = _pair_batch(pairs, mode, alpha, lam, color_weight, similarity, directionless,
+                                   extended_points, only_extended, probe_radius, esp_weight, no_H,
+                                   device)
+    n = len(pairs)
+    q = torch.zeros((n, 4), dtype=torch.float32)
+    q[:, 0] = 1.0
+    so = self_overlaps(pb, spec)
+    return pose_score(pb, spec, torch.arange(n, dtype=torch.int32), q, torch.zeros((n, 3)), so).cpu().numpy()
 
 
 class MoleculePairBatch:
@@ -272,12 +432,13 @@ class MoleculePairBatch:
         if max_num_steps is None:
             max_num_steps = MODE_STEPS[mode]
         align_pairs(self.pairs, mode, num_repeats=num_repeats, max_num_steps=max_num_steps, **kw)
-        scores = np.array([float(getattr(p, MODE_ATTRS[mode][1])) for p in self.pairs])
+        tf_attr, sc_attr = result_attrs(mode, kw.get("no_H", True))
+        scores = np.array([float(getattr(p, sc_attr)) for p in self.pairs])
         if not return_aligned:
             return scores, [None] * len(self.pairs)
         aligned = []
         for p in self.pairs:
-            T = getattr(p, MODE_ATTRS[mode][0]).numpy().astype(np.float64)
+            T = getattr(p, tf_attr).numpy().astype(np.float64)
             X = np.asarray(getattr(p.fit_molec, fit_cloud), np.float64)
             aligned.append((X @ T[:3, :3].T + T[:3, 3]).astype(np.float32))
         return scores, aligned
@@ -308,21 +469,18 @@ class MoleculePairBatch:
                        use_shmap: bool = True, num_buckets: int = 1, verbose: bool = False,
                        backend: Optional[str] = None, alpha: float = 0.81,
                        return_aligned: bool = False, **hip_kw):
-        if not no_H:
-            raise ValueError("the HIP vol backend aligns heavy atoms only (no_H=True)")
-        return self._run("vol", backend, return_aligned, "atom_pos", num_repeats, max_num_steps,
-                         trans_init=trans_init, lr=lr, alpha=alpha, **self._hip_kw(hip_kw))
+        return self._run("vol", backend, return_aligned, "atom_pos" if no_H else "centers_w_H",
+                         num_repeats, max_num_steps, trans_init=trans_init, lr=lr, alpha=alpha,
+                         no_H=no_H, **self._hip_kw(hip_kw))
 
     def align_with_vol_esp(self, lam: float = 0.1, no_H: bool = True, num_repeats: int = None,
                            trans_init: bool = False, lr: float = 0.1, max_num_steps: int = None,
                            num_workers: int = 1, use_shmap: bool = True, num_buckets: int = 1,
                            verbose: bool = False, backend: Optional[str] = None,
                            alpha: float = 0.81, return_aligned: bool = False, **hip_kw):
-        if not no_H:
-            raise ValueError("the HIP vol_esp backend aligns heavy atoms only (no_H=True)")
-        return self._run("vol_esp", backend, return_aligned, "atom_pos", num_repeats,
-                         max_num_steps, trans_init=trans_init, lr=lr, alpha=alpha, lam=lam,
-                         **self._hip_kw(hip_kw))
+        return self._run("vol_esp", backend, return_aligned, "atom_pos" if no_H else "centers_w_H",
+                         num_repeats, max_num_steps, trans_init=trans_init, lr=lr, alpha=alpha,
+                         lam=lam, no_H=no_H, **self._hip_kw(hip_kw))
 
     def align_with_surf(self, alpha: float, num_repeats: int = None, trans_init: bool = False,
                         lr: float = 0.1, max_num_steps: int = None, use_jax: bool = True,

@@ -183,3 +183,52 @@ def test_translation_seeds_rule():
     np.testing.assert_allclose(moved[5:], np.repeat(cen, 10, axis=0), atol=1e-5)
     q45, _ = translation_seeds(ref, fit, cen[:1], per_trans=45)   # the 50-restart table's rotations
     np.testing.assert_array_equal(q45[5:], reference_table(50)[5:])
+
+
+# Parameter names of the reference's single-pair methods (container/_core.py:612-1560), in order.
+PAIR_API = {
+    "align_with_vol": ["no_H", "num_repeats", "trans_init", "lr", "max_num_steps", "use_jax",
+                       "use_analytical", "verbose"],
+    "align_with_vol_esp": ["lam", "no_H", "num_repeats", "trans_init", "lr", "max_num_steps",
+                           "use_jax", "use_analytical", "verbose"],
+    "align_with_surf": ["alpha", "num_repeats", "trans_init", "lr", "max_num_steps", "use_jax",
+                        "use_analytical", "verbose"],
+    "align_with_surf_esp": ["alpha", "lam", "num_repeats", "trans_init", "lr", "max_num_steps",
+                            "use_jax", "use_analytical", "verbose"],
+    "align_with_vol_and_surf_esp": ["alpha", "lam", "probe_radius", "esp_weight", "num_repeats",
+                                    "trans_init", "lr", "max_num_steps", "use_jax", "verbose"],
+    "align_with_vol_color": ["color_weight", "alpha", "similarity", "directionless",
+                             "extended_points", "only_extended", "num_repeats", "trans_init", "lr",
+                             "max_num_steps", "verbose"],
+    "align_with_pharm": ["similarity", "extended_points", "only_extended", "num_repeats",
+                         "trans_init", "lr", "max_num_steps", "use_jax", "verbose",
+                         "use_vectorized", "use_analytical"],
+    "score_with_surf": ["alpha", "use"],
+    "score_with_esp": ["alpha", "lam", "use"],
+    "score_with_pharm": ["similarity", "extended_points", "only_extended", "use"],
+}
+
+
+def test_single_pair_api_matches_reference_signatures():
+    import inspect
+    from fast_shepherd_score_amd.container import MoleculePair
+    for name, params in PAIR_API.items():
+        got = list(inspect.signature(getattr(MoleculePair, name)).parameters)[1:]
+        assert got == params, name
+    assert MoleculePair.align_with_esp is MoleculePair.align_with_surf_esp
+    assert MoleculePair.align_with_esp_combo is MoleculePair.align_with_vol_and_surf_esp
+
+
+def test_result_attributes_with_and_without_hydrogens():
+    from fast_shepherd_score_amd.container import result_attrs
+    assert result_attrs("vol") == ("transform_vol_noH", "sim_aligned_vol_noH")
+    assert result_attrs("vol", no_H=False) == ("transform_vol", "sim_aligned_vol")
+    assert result_attrs("vol_esp", no_H=False) == ("transform_vol_esp", "sim_aligned_vol_esp")
+    assert result_attrs("surf", no_H=False) == ("transform_surf", "sim_aligned_surf")
+
+
+def test_score_with_rejects_unknown_implementation():
+    from fast_shepherd_score_amd.container import Molecule, MoleculePair
+    m = Molecule(np.zeros((2, 3)), surf_pos=np.zeros((4, 3)))
+    with pytest.raises(ValueError, match="use"):
+        MoleculePair(m, m, device="cpu").score_with_surf(0.81, use="numba")
