@@ -43,34 +43,49 @@ def _ranges(P: int, N: int, dev) -> torch.Tensor:
     return r
 
 
+def _forward(ref, fit, ref_w, fit_w, alpha, lam):
+    """fss_score_fwd[_f64] on (P, N, 3) / (P, M, 3) clouds: S (P,), ov (P, 4) = [VAB, VAA, VBB, 0]
+    and what the backward needs."""
+    if not (ref.is_cuda and fit.is_cuda):
+        raise RuntimeError("gaussian_similarity runs on the GPU only (HIP kernels)")
+    if ref.dtype != fit.dtype or ref.dtype not in (torch.float32, torch.float64):
+        raise TypeError("ref and fit must both be float32 or both float64")
+    P, N, M = ref.shape[0], ref.shape[1], fit.shape[1]
+    esp = ref_w is not None
+    mode = _lib.MODE_SHAPE_ESP if esp else _lib.MODE_SHAPE
+    rp, fp = _pack(ref.detach(), ref_w), _pack(fit.detach(), fit_w)
+    rr, fr = _ranges(P, N, ref.device), _ranges(P, M, ref.device)
+    pb = _lib.FssPairBatch()
+    pb.n_pairs, pb.ref_rng, pb.fit_rng = P, _lib.ptr(rr), _lib.ptr(fr)
+    pb.max_ref, pb.max_fit = N, M
+    sp = _lib.FssScoreParams()
+    sp.mode, sp.alpha, sp.lam = mode, float(alpha), float(lam if esp else 1.0)
+    S = torch.empty(P, dtype=ref.dtype, device=ref.device)
+    ov = torch.empty((P, 4), dtype=ref.dtype, device=ref.device)
+    stream = int(torch.cuda.current_stream().cuda_stream)
+    if ref.dtype == torch.float32:
+        pb.ref_pts, pb.fit_pts = _lib.ptr(rp), _lib.ptr(fp)
+        _lib.call("fss_score_fwd", _lib.byref(pb), _lib.byref(sp), _lib.ptr(S), _lib.ptr(ov), stream)
+    else:
+        _lib.call("fss_score_fwd_f64", _lib.byref(pb), _lib.byref(sp), _lib.ptr(rp), _lib.ptr(fp),
+                  _lib.ptr(S), _lib.ptr(ov), stream)
+    return S, ov, (rp, fp, rr, fr), (P, N, M, mode, float(alpha), float(lam if esp else 1.0))
+
+
+def overlaps(ref: torch.Tensor, fit: torch.Tensor, ref_esp: Optional[torch.Tensor] = None,
+             fit_esp: Optional[torch.Tensor] = None, alpha: float = 0.81,
+             lam: float = 0.3 * LAM_SCALING) -> torch.Tensor:
+    """(P, 3) [VAB, VAA, VBB] of each cloud pair, no gradient (the forward kernel's by-product)."""
+    with torch.no_grad():
+        return _forward(ref, fit, ref_esp, fit_esp, alpha, lam)[1][:, :3]
+
+
 class _GaussianSimilarity(torch.autograd.Function):
     @staticmethod
     def forward(ctx, ref, fit, ref_w, fit_w, alpha, lam):
-        if not (ref.is_cuda and fit.is_cuda):
-            raise RuntimeError("gaussian_similarity runs on the GPU only (HIP kernels)")
-        if ref.dtype != fit.dtype or ref.dtype not in (torch.float32, torch.float64):
-            raise TypeError("ref and fit must both be float32 or both float64")
-        P, N, M = ref.shape[0], ref.shape[1], fit.shape[1]
-        esp = ref_w is not None
-        mode = _lib.MODE_SHAPE_ESP if esp else _lib.MODE_SHAPE
-        rp, fp = _pack(ref.detach(), ref_w), _pack(fit.detach(), fit_w)
-        rr, fr = _ranges(P, N, ref.device), _ranges(P, M, ref.device)
-        pb = _lib.FssPairBatch()
-        pb.n_pairs, pb.ref_rng, pb.fit_rng = P, _lib.ptr(rr), _lib.ptr(fr)
-        pb.max_ref, pb.max_fit = N, M
-        sp = _lib.FssScoreParams()
-        sp.mode, sp.alpha, sp.lam = mode, float(alpha), float(lam if esp else 1.0)
-        S = torch.empty(P, dtype=ref.dtype, device=ref.device)
-        ov = torch.empty((P, 4), dtype=ref.dtype, device=ref.device)
-        stream = int(torch.cuda.current_stream().cuda_stream)
-        if ref.dtype == torch.float32:
-            pb.ref_pts, pb.fit_pts = _lib.ptr(rp), _lib.ptr(fp)
-            _lib.call("fss_score_fwd", _lib.byref(pb), _lib.byref(sp), _lib.ptr(S), _lib.ptr(ov), stream)
-        else:
-            _lib.call("fss_score_fwd_f64", _lib.byref(pb), _lib.byref(sp), _lib.ptr(rp), _lib.ptr(fp),
-                      _lib.ptr(S), _lib.ptr(ov), stream)
+        S, ov, (rp, fp, rr, fr), meta = _forward(ref, fit, ref_w, fit_w, alpha, lam)
         ctx.save_for_backward(rp, fp, rr, fr, ov)
-        ctx.meta = (P, N, M, mode, float(alpha), float(lam if esp else 1.0))
+        ctx.meta = meta
         return S
 
     @staticmethod

@@ -232,3 +232,51 @@ def test_score_with_rejects_unknown_implementation():
     m = Molecule(np.zeros((2, 3)), surf_pos=np.zeros((4, 3)))
     with pytest.raises(ValueError, match="use"):
         MoleculePair(m, m, device="cpu").score_with_surf(0.81, use="numba")
+
+
+# Parameter names of the reference's scoring functions (score/gaussian_overlap.py,
+# electrostatic_scoring.py, pharmacophore_scoring.py), in order.
+SCORE_API = {
+    "VAB_2nd_order": ["centers_1", "centers_2", "alpha"],
+    "shape_tanimoto": ["centers_1", "centers_2", "alpha"],
+    "get_overlap": ["centers_1", "centers_2", "alpha"],
+    "VAB_2nd_order_esp": ["centers_1", "centers_2", "charges_1", "charges_2", "alpha", "lam"],
+    "shape_tanimoto_esp": ["centers_1", "centers_2", "charges_1", "charges_2", "alpha", "lam"],
+    "get_overlap_esp": ["centers_1", "centers_2", "charges_1", "charges_2", "alpha", "lam"],
+    "esp_combo_score": ["centers_w_H_1", "centers_w_H_2", "centers_1", "centers_2", "points_1",
+                        "points_2", "partial_charges_1", "partial_charges_2", "point_charges_1",
+                        "point_charges_2", "radii_1", "radii_2", "alpha", "lam", "probe_radius",
+                        "esp_weight"],
+    "tanimoto_func": ["VAB", "VAA", "VBB"],
+    "tversky_func": ["VAB", "VAA", "VBB", "sigma"],
+    "get_overlap_pharm": ["ptype_1", "ptype_2", "anchors_1", "anchors_2", "vectors_1", "vectors_2",
+                          "similarity", "extended_points", "only_extended",
+                          "precomputed_self_overlaps", "directionless"],
+    "get_pharm_combo_score": ["centers_1", "centers_2", "ptype_1", "ptype_2", "anchors_1",
+                              "anchors_2", "vectors_1", "vectors_2", "alpha", "similarity",
+                              "extended_points", "only_extended", "color_weight", "directionless"],
+}
+
+
+def test_score_module_matches_reference_signatures():
+    import inspect
+    from fast_shepherd_score_amd import score
+    for name, params in SCORE_API.items():
+        assert list(inspect.signature(getattr(score, name)).parameters) == params, name
+    assert sorted(score.__all__) == sorted(SCORE_API)
+
+
+def test_score_module_host_checks():
+    import torch
+    from fast_shepherd_score_amd import score
+    v = torch.tensor([2.0, 1.0])
+    assert torch.allclose(score.tanimoto_func(v, torch.tensor([3.0, 3.0]), torch.tensor([3.0, 3.0])),
+                          torch.tensor([0.5, 0.2]))
+    assert torch.allclose(score.tversky_func(v, torch.tensor([1.0, 2.0]), torch.tensor([1.0, 2.0]), 0.95),
+                          torch.tensor([1.0, 0.5]))           # clamped at 1
+    with pytest.raises(ValueError, match="similarity"):
+        score.get_overlap_pharm(np.zeros(2), np.zeros(2), np.zeros((2, 3)), np.zeros((2, 3)),
+                                np.zeros((2, 3)), np.zeros((2, 3)), similarity="dice")
+    a = torch.zeros((2, 3), requires_grad=True)
+    with pytest.raises(NotImplementedError, match="gradient"):
+        score.get_overlap_pharm(np.zeros(2), np.zeros(2), a, a, a, a)
