@@ -142,6 +142,7 @@ class FssQuery(ctypes.Structure):
         ("vec", ctypes.c_void_p), ("typ", ctypes.c_void_p),
         ("n_surface", ctypes.c_int32), ("srf", ctypes.c_void_p),
         ("n_atoms", ctypes.c_int32), ("atm", ctypes.c_void_p), ("rad", ctypes.c_void_p),
+        ("all_scores", ctypes.c_int32),
     ]
 
 
@@ -151,6 +152,7 @@ class FssHits(ctypes.Structure):
         ("score", ctypes.c_void_p), ("index", ctypes.c_void_p),
         ("transform", ctypes.c_void_p),
         ("queue_ms", ctypes.c_double), ("latency_ms", ctypes.c_double),
+        ("all_scores", ctypes.c_void_p), ("n_all", ctypes.c_int64),
     ]
 
 

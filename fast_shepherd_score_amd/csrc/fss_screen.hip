@@ -19,6 +19,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <limits>
 #include <atomic>
 #include <chrono>
 #include <condition_variable>
@@ -250,6 +251,7 @@ struct QueryIn {
   std::vector<float> pts, anc, vec, srf, atm, rad;
   std::vector<int32_t> typ;
   int32_t np = 0, nph = 0, ns = 0, na = 0;
+  bool want_all = false;
   clk::time_point t_submit;
 };
 
@@ -258,6 +260,7 @@ struct Slot {
   QueryIn q;
   int64_t next = 0, lib_n = 0;
   int32_t filled = 0, buf = 0;
+  std::vector<double> all;             // every entry's best score, when the query asked
   clk::time_point t_start;
 };
 
@@ -267,6 +270,7 @@ struct Result {
   std::vector<double> score;
   std::vector<int64_t> index;
   std::vector<float> tf;
+  std::vector<double> all;
   double queue_ms = 0, latency_ms = 0;
 };
 
@@ -369,6 +373,7 @@ int admit(FssScreen* s, int slot) {
   sl.next = 0;
   sl.filled = 0;
   sl.buf = 0;
+  if (sl.q.want_all) sl.all.assign((size_t)sl.lib_n, std::numeric_limits<double>::quiet_NaN());
   sl.t_start = clk::now();
   return FSS_OK;
 }
@@ -471,9 +476,15 @@ int launch(FssScreen* s, const std::vector<int>& act) {
   s->launches.fetch_add(1);
   s->pairs.fetch_add(p);
   const auto now = clk::now();
-  for (int i = 0; i < na; ++i) {
+  for (int i = 0, off = 0; i < na; ++i) {
     if (take[i] == 0) continue;
     Slot& sl = s->slots[act[i]];
+    if (sl.q.want_all) {   // the launch's pairs [off, off + take) are entries [next, next + take)
+      e = hipMemcpy(sl.all.data() + sl.next, s->best_score + off, sizeof(double) * take[i],
+                    hipMemcpyDeviceToHost);
+      if (e != hipSuccess) return hip_fail(e, "score download");
+    }
+    off += (int)take[i];
     sl.next += take[i];
     sl.filled = (int32_t)std::min<int64_t>(s->K, (int64_t)sl.filled + take[i]);
     sl.buf ^= 1;
@@ -492,6 +503,7 @@ int launch(FssScreen* s, const std::vector<int>& act) {
     }
     r.queue_ms = ms_since(sl.q.t_submit, sl.t_start);
     r.latency_ms = ms_since(sl.q.t_submit, now);
+    r.all.swap(sl.all);
     const int64_t ticket = sl.q.ticket;
     sl.used = false;
     sl.q = QueryIn();
@@ -855,6 +867,7 @@ int fss_screen_submit(FssScreen* s, const FssQuery* q, int64_t* ticket) {
   }
   QueryIn qi;
   qi.t_submit = clk::now();
+  qi.want_all = q->all_scores != 0;
   if (s->need_se) {
     qi.ns = q->n_surface;
     qi.na = q->n_atoms;
@@ -897,11 +910,17 @@ int fss_screen_poll(FssScreen* s, int64_t ticket, int32_t wait_ms, FssHits* out)
       if (!ready()) return FSS_PENDING;
     }
     auto it = s->results.find(ticket);
+    const size_t n_all = it->second.all.size();
+    if (!it->second.rc && n_all && (!out || !out->all_scores || out->n_all < (int64_t)n_all))
+      return set_error(FSS_E_INVALID, "the query asked for all %zu scores: pass all_scores with n_all >= %zu",
+                       n_all, n_all);
     r = std::move(it->second);
     s->results.erase(it);
   }
   if (r.rc) return set_error(r.rc, "%s", r.err.c_str());
   if (!out) return FSS_OK;
+  if (!r.all.empty()) memcpy(out->all_scores, r.all.data(), sizeof(double) * r.all.size());
+  out->n_all = (int64_t)r.all.size();
   const int n = std::min<int>(out->capacity, (int)r.score.size());
   if (n > 0 && (!out->score || !out->index || !out->transform))
     return set_error(FSS_E_INVALID, "NULL hit arrays");

@@ -70,6 +70,11 @@ class ConformerLibrary:
         self.ids = np.arange(n, dtype=np.int64) if ids is None else np.asarray(ids)
         self._native = None
         self._native_bytes = None
+        # surface points per conformer when every surface cloud has the same count (the
+        # reference's store property that picks ALPHA for surf / surf_esp), and whether the
+        # conformers were centred on their heavy-atom mean when stored
+        self.num_surf_points = None
+        self.pre_centered = False
 
     @classmethod
     def from_native(cls, handle: ctypes.c_void_p, n: int, ids: np.ndarray,
@@ -81,6 +86,8 @@ class ConformerLibrary:
         lib.ids = np.asarray(ids)
         lib._native = handle
         lib._native_bytes = int(nbytes)
+        lib.num_surf_points = None
+        lib.pre_centered = False
         return lib
 
     def id_of(self, index: int):
@@ -159,7 +166,11 @@ class ConformerLibrary:
                  if pharm_types is not None else None)
         shaep = (ShaepSet.from_arrays(surf_points, surf_esp, atoms_w_H, charges, radii, device=device)
                  if surf_points is not None else None)
-        return cls(shape, pharm, ids, shaep)
+        lib = cls(shape, pharm, ids, shaep)
+        surf = shaep.srf.counts if shaep is not None else None
+        if surf is not None and len(surf) and (surf == surf[0]).all():
+            lib.num_surf_points = int(surf[0])
+        return lib
 
     def nbytes(self) -> int:
         if self._native_bytes is not None:
@@ -175,6 +186,36 @@ class ConformerLibrary:
                 if isinstance(t, torch.Tensor):
                     tot += t.numel() * t.element_size()
         return tot
+
+
+def _surf_count(library) -> Optional[int]:
+    """Surface points per conformer: the library's record, else the common size of its shape
+    clouds (surface clouds in the surf modes)."""
+    if getattr(library, "num_surf_points", None):
+        return int(library.num_surf_points)
+    sh = getattr(library, "shape", None)
+    if sh is not None and len(sh.counts) and (sh.counts == sh.counts[0]).all():
+        return int(sh.counts[0])
+    return None
+
+
+def resolve_alpha(library, mode: str, alpha: Optional[float]) -> float:
+    """The shape width a screen uses (screen.py:936-959): ``ALPHA(num_surf_points)`` for surf /
+    surf_esp, 0.81 for the atom modes, and vol_and_surf_esp needs it given (0.81 selects the
+    heavy-atom shape term, anything else the surface one)."""
+    from .constants import ALPHA
+    if alpha is not None:
+        return float(alpha)
+    if mode in ("surf", "surf_esp"):
+        n = _surf_count(library)
+        if n is None:
+            raise ValueError("surf / surf_esp screening takes alpha = ALPHA(num_surf_points), but the "
+                             "library's surface clouds differ in size: pass alpha=...")
+        return float(ALPHA(n))
+    if mode == "vol_and_surf_esp":
+        raise ValueError("vol_and_surf_esp requires an explicit alpha (it selects volumetric shape at "
+                         "alpha=0.81, otherwise surface shape); pass alpha=...")
+    return 0.81
 
 
 def plan_launch(remaining: Sequence[int], capacity: int) -> List[int]:
@@ -225,15 +266,17 @@ class ScreenService:
     def __init__(self, library: ConformerLibrary, mode: str = "vol", *, num_repeats: Optional[int] = None,
                  max_num_steps: Optional[int] = None, lr: float = 0.1, top_k: int = 100,
                  launch_pairs: int = 65536, hbm_budget_bytes: int = 8 << 30, max_active: int = 16,
-                 early_stop: bool = True, alpha: float = 0.81, lam: Optional[float] = None,
+                 early_stop: bool = True, alpha: Optional[float] = None, lam: Optional[float] = None,
                  similarity: str = "tanimoto", directionless: bool = True,
                  extended_points: bool = False, only_extended: bool = False,
                  lanes_per_pose: Optional[int] = None, max_query_points: int = 4096,
                  max_query_pharm: int = 256, esp_weight: float = 0.5, probe_radius: float = 1.0,
                  max_query_surface: int = 4096, max_query_atoms: int = 1024, device=None):
-        from .modes import MODE_SEEDS, MODE_STEPS
+        from .modes import MODE_SEEDS, MODE_STEPS, canonical
+        mode = canonical(mode)
         if mode not in _KIND:
             raise ValueError(f"unsupported screening mode {mode!r}")
+        alpha = resolve_alpha(library, mode, alpha)
         if not 1 <= int(top_k) <= _lib.SCREEN_MAX_TOPK:
             raise ValueError(f"top_k must be in [1, {_lib.SCREEN_MAX_TOPK}]")
         self.lib, self.mode = library, mode
@@ -291,21 +334,31 @@ class ScreenService:
                   ctypes.byref(self._h))
         self._futures: Dict[int, Future] = {}
         self._submitted: Dict[int, int] = {}
+        self._outs: Dict[int, object] = {}
         self._cv = threading.Condition()
         self._stop = False
         self._poller = threading.Thread(target=self._poll_loop, name="fss-screen-poll", daemon=True)
         self._poller.start()
 
     # -- public ------------------------------------------------------------------------------
-    def submit(self, query: dict, top_k: Optional[int] = None) -> Future:
-        """Queue a query (dict with the arrays its mode needs: ``points`` [+ ``esp``] and/or
+    def submit(self, query, top_k: Optional[int] = None, scores_out=None) -> Future:
+        """Queue a query and return a Future resolving to a ScreenResult.
+
+        ``query`` is a dict with the arrays its mode needs: ``points`` [+ ``esp``] and/or
         ``pharm_types``, ``pharm_anchors``, ``pharm_vectors``; vol_and_surf_esp: ``surf_points``,
         ``surf_esp``, ``atoms_w_H``, ``charges``, ``radii`` and, at alpha 0.81, ``points`` = the
-        heavy atoms).  Resolves to a ScreenResult."""
+        heavy atoms.  An object with the container's ``Molecule`` / ``MoleculeProfile``
+        attributes is taken too (see :func:`query_arrays`).  ``scores_out``: an array of at
+        least the library's size, written with every entry's best score in library order
+        before the future resolves (screen.py:1174-1176)."""
+        query = query_arrays(query, self.mode)
         k = int(top_k or self.top_k)
         if not 1 <= k <= self.top_k:
             raise ValueError(f"top_k must be in [1, {self.top_k}] for this service")
+        if scores_out is not None and len(scores_out) < self.lib.n:
+            raise ValueError(f"scores_out holds {len(scores_out)} entries, the library {self.lib.n}")
         q = _lib.FssQuery()
+        q.all_scores = int(scores_out is not None)
         keep = []
 
         def rows4(xyz, w=None):
@@ -350,6 +403,8 @@ class ScreenService:
             _lib.call("fss_screen_submit", self._h, ctypes.byref(q), ctypes.byref(ticket))
             self._futures[ticket.value] = fut
             self._submitted[ticket.value] = k
+            if scores_out is not None:
+                self._outs[ticket.value] = scores_out
             self._cv.notify_all()
         return fut
 
@@ -394,6 +449,7 @@ class ScreenService:
         hits = _lib.FssHits()
         hits.capacity = K
         hits.score, hits.index, hits.transform = score.ctypes.data, index.ctypes.data, tf.ctypes.data
+        all_buf = None
         while True:
             with self._cv:
                 while not self._futures and not self._stop:
@@ -401,51 +457,153 @@ class ScreenService:
                 if self._stop:
                     return
                 tickets = sorted(self._futures)
+                outs = {t: self._outs.get(t) for t in tickets}
             # wait (in native code, GIL released) up to 5 ms on the oldest ticket, then sweep
             for i, t in enumerate(tickets):
+                out = outs[t]
+                if out is not None:
+                    if all_buf is None or len(all_buf) < self.lib.n:
+                        all_buf = np.empty(self.lib.n, np.float64)
+                    hits.all_scores, hits.n_all = all_buf.ctypes.data, len(all_buf)
+                else:
+                    hits.all_scores, hits.n_all = None, 0
                 rc = lib.fss_screen_poll(self._h, t, 5 if i == 0 else 0, ctypes.byref(hits))
                 if rc == _lib.PENDING:
                     continue
                 with self._cv:
                     fut = self._futures.pop(t)
                     k = self._submitted.pop(t)
+                    self._outs.pop(t, None)
                 if rc != 0:
                     fut.set_exception(_lib.FssError("fss_screen_poll", rc, lib.fss_last_error().decode()))
                     continue
+                if out is not None:
+                    out[:int(hits.n_all)] = all_buf[:int(hits.n_all)]
                 n = min(int(hits.n), k)
                 res = [Hit(float(score[i]), self.lib.id_of(int(index[i])), tf[i].reshape(4, 4).copy())
                        for i in range(n)]
                 fut.set_result(ScreenResult(res, float(hits.latency_ms), float(hits.queue_ms), self.lib.n))
 
 
-def _as_library(library, mode: str, device):
-    """(ConformerLibrary, owned): a ProfileStore or store path is loaded into HBM first."""
+def _as_library(library, mode: str, device, center: bool = False):
+    """(ConformerLibrary, owned): a ProfileStore or store path is loaded into HBM first, its
+    conformers centred on their heavy-atom means when ``center`` and the store is not
+    pre-centred."""
     from .store import ProfileStore, load_library
     if isinstance(library, (ProfileStore, str, os.PathLike)):
-        return load_library(library, mode, device=device if device is not None else 0), True
+        return load_library(library, mode, device=device if device is not None else 0, center=center), True
     return library, False
 
 
-def screen(query: dict, library, mode: str = "vol", *, top_k: int = 1000, device=None,
-           **kwargs) -> List[Hit]:
-    """One query against the whole library (a ConformerLibrary, a ProfileStore or a store
-    path); ``Hit(score, id, transform)`` sorted descending."""
-    lib, owned = _as_library(library, mode, device)
-    try:
-        with ScreenService(lib, mode, top_k=top_k, **kwargs) as svc:
-            return svc.submit(query).result().hits
-    finally:
-        if owned:
-            lib.close()
+def query_arrays(query, mode: str) -> dict:
+    """The query dict of :meth:`ScreenService.submit` from a dict (returned as is) or from an
+    object with the container's ``Molecule`` / ``MoleculePair`` attributes (``atom_pos``,
+    ``surf_pos``, ``surf_esp``, ``pharm_types``, ``pharm_ancs``, ``pharm_vecs``, heavy-atom
+    ``atom_charges`` or ``partial_charges`` [+ ``nonH_atoms_idx``], ``centers_w_H``, ``radii``)."""
+    if isinstance(query, dict):
+        return query
+    g = lambda k: getattr(query, k, None)
+    d = {"atom_pos": g("atom_pos")}
+    if mode in ("vol", "vol_color", "vol_and_surf_esp"):
+        d["points"] = g("atom_pos")
+    if mode == "vol_esp":
+        ch = g("atom_charges")
+        if ch is None and g("partial_charges") is not None:
+            idx = g("nonH_atoms_idx")
+            ch = np.asarray(g("partial_charges"))[idx] if idx is not None else g("partial_charges")
+        d["points"], d["esp"] = g("atom_pos"), ch
+    if mode in ("surf", "surf_esp"):
+        d["points"] = g("surf_pos")
+        if mode == "surf_esp":
+            d["esp"] = g("surf_esp")
+    if mode in ("pharm", "vol_color"):
+        d.update(pharm_types=g("pharm_types"), pharm_anchors=g("pharm_ancs"), pharm_vectors=g("pharm_vecs"))
+    if mode == "vol_and_surf_esp":
+        d.update(surf_points=g("surf_pos"), surf_esp=g("surf_esp"), atoms_w_H=g("centers_w_H"),
+                 charges=g("partial_charges"), radii=g("radii"))
+    missing = [k for k, v in d.items() if v is None and k != "atom_pos"]
+    if missing:
+        raise ValueError(f"{mode} query is missing {missing}")
+    return d
 
 
-def screen_many(queries: Sequence[dict], library, mode: str = "vol", *, top_k: int = 1000,
-                device=None, **kwargs) -> List[List[Hit]]:
-    """A panel of queries served together (continuous batching); one hit list per query."""
-    lib, owned = _as_library(library, mode, device)
+_COORDS = ("points", "surf_points", "atoms_w_H", "pharm_anchors", "atom_pos")
+
+
+def centered_query(query: dict, mode: str) -> dict:
+    """A copy of the query dict moved so that its heavy atoms' mean is the origin (``atom_pos``
+    when present, else the mean of the rows the restarts are seeded from); vectors and ESP
+    values are unchanged.  The caller's arrays are not modified."""
+    if query.get("atom_pos") is not None:
+        c = np.asarray(query["atom_pos"], np.float64).reshape(-1, 3).mean(0)
+    elif mode == "pharm":
+        c = np.asarray(query["pharm_anchors"], np.float64).reshape(-1, 3).mean(0)
+    elif mode == "vol_and_surf_esp":
+        c = np.asarray(query["surf_points"], np.float64).reshape(-1, 3).mean(0)
+    else:
+        c = np.asarray(query["points"], np.float64).reshape(-1, 3).mean(0)
+    out = dict(query)
+    for k in _COORDS:
+        if out.get(k) is not None:
+            out[k] = (np.asarray(out[k], np.float64).reshape(-1, 3) - c).astype(np.float32)
+    return out
+
+
+def screen(query, library, mode: str = "surf_esp", *, backend: Optional[str] = None,
+           do_center: Optional[bool] = None, top_k: int = 1000, ndev: Optional[int] = None,
+           scores_out=None, alpha: Optional[float] = None, progress: bool = False, device=None,
+           **align_kwargs) -> List[Hit]:
+    """One query against the whole library (a ConformerLibrary, a ProfileStore or a store path);
+    ``Hit(score, id, transform)`` sorted descending.  The reference's signature
+    (screen.py:1141-1192); see :func:`screen_many`."""
+    outs = None if scores_out is None else [scores_out]
+    return screen_many([query], library, mode, backend=backend, do_center=do_center, top_k=top_k,
+                       ndev=ndev, scores_out=outs, alpha=alpha, progress=progress, device=device,
+                       **align_kwargs)[0]
+
+
+def screen_many(queries: Sequence, library, mode: str = "surf_esp", *, backend: Optional[str] = None,
+                do_center: Optional[bool] = None, top_k: int = 1000, ndev: Optional[int] = None,
+                scores_out=None, alpha: Optional[float] = None, progress: bool = False, device=None,
+                **align_kwargs) -> List[List[Hit]]:
+    """A panel of queries served together (continuous batching); one hit list per query
+    (screen.py:1038-1138).
+
+    * ``backend``: None or a HIP alias; ``ndev``: one GPU (None or 1).
+    * ``alpha``: ``ALPHA(num_surf_points)`` for surf / surf_esp, 0.81 for the atom modes,
+      required for vol_and_surf_esp; vol_esp requires ``lam`` (screen.py:936-959).
+    * ``do_center`` (default on): queries are centred on their heavy-atom mean; a store that
+      is not pre-centred is centred on load.  A ConformerLibrary passed in is used as it is.
+      The transforms move library entries onto the (centred) query.
+    * ``scores_out``: one array per query of at least the library's size, written with every
+      entry's best score in library order.
+    * ``progress`` is accepted; the service reports no progress.
+    """
+    from .container import resolve_backend
+    from .modes import canonical
+    resolve_backend(backend)
+    if ndev not in (None, 1):
+        raise ValueError("screening runs on one GPU here (ndev=1)")
+    mode = canonical(mode)
+    if align_kwargs.pop("no_H", True) is False:
+        raise ValueError("screen() aligns heavy atoms only; no_H=False is not supported")
+    if mode == "vol_esp" and "lam" not in align_kwargs:
+        raise ValueError("vol_esp requires an explicit lam=... (the ESP/partial-charge weight); pass lam=...")
+    queries = list(queries)
+    if scores_out is not None and len(scores_out) != len(queries):
+        raise ValueError("scores_out needs one array per query")
+    if mode == "vol_and_surf_esp":
+        resolve_alpha(None, mode, alpha)          # needs no library: fail before loading one
+    center = True if do_center is None else bool(do_center)
+    lib, owned = _as_library(library, mode, device, center=center)
     try:
-        with ScreenService(lib, mode, top_k=top_k, **kwargs) as svc:
-            futs = [svc.submit(q) for q in queries]
+        alpha = resolve_alpha(lib, mode, alpha)
+        qs = [query_arrays(q, mode) for q in queries]
+        if center:
+            qs = [centered_query(q, mode) for q in qs]
+        outs = scores_out if scores_out is not None else [None] * len(qs)
+        with ScreenService(lib, mode, top_k=top_k, alpha=alpha, **align_kwargs) as svc:
+            futs = [svc.submit(q, scores_out=o) for q, o in zip(qs, outs)]
             return [f.result().hits for f in futs]
     finally:
         if owned:

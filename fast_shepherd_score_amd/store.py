@@ -391,9 +391,32 @@ def _device_index(device) -> int:
     return d.index if d.index is not None else torch.cuda.current_device()
 
 
-def load_library(store, mode: str, device=0, shards: Optional[Sequence[int]] = None):
+def _center_shard(a: dict) -> dict:
+    """A shard's arrays with every conformer moved so that its ``atom_pos`` mean is the origin
+    (what a pre-centred store does at write time)."""
+    ao = a["atom_off"]
+    cnt = np.diff(ao)
+    mu = np.add.reduceat(_f32(a["atom_pos"]).astype(np.float64), ao[:-1], axis=0) / np.maximum(cnt, 1)[:, None]
+    mu[cnt == 0] = 0.0
+    out = dict(a)
+    per = lambda off: np.repeat(mu, np.diff(off), axis=0)
+    out["atom_pos"] = (_f32(a["atom_pos"]) - per(ao)).astype(np.float32)
+    if "xyz_noH" in a:
+        out["xyz_noH"] = (_f32(a["xyz_noH"]) - per(a["heavy_off"])).astype(np.float32)
+    if "surf_pos" in a:
+        out["surf_pos"] = (_f32(a["surf_pos"]) - mu[:, None, :]).astype(np.float32)
+    if "pharm_ancs" in a:
+        out["pharm_ancs"] = (_f32(a["pharm_ancs"]) - per(a["pharm_off"])).astype(np.float32)
+    if "cwh" in a:
+        out["cwh"] = (_f32(a["cwh"]) - per(a["all_off"])).astype(np.float32)
+    return out
+
+
+def load_library(store, mode: str, device=0, shards: Optional[Sequence[int]] = None,
+                 center: bool = False):
     """Stream ``store`` (a ProfileStore or its path) into a native HBM library for ``mode``;
-    returns a ConformerLibrary whose rows live only in the native library."""
+    returns a ConformerLibrary whose rows live only in the native library.  ``center``: a store
+    that is not pre-centred has each conformer centred on its ``atom_pos`` mean on the way in."""
     from .screen import ConformerLibrary
     if isinstance(store, (str, os.PathLike)):
         store = ProfileStore.open(str(store))
@@ -439,8 +462,11 @@ def load_library(store, mode: str, device=0, shards: Optional[Sequence[int]] = N
     nbytes = 0
     first = ctypes.c_int64()
     try:
+        recenter = center and not store.pre_centered
         for i in idx:
             sh, a = store.read_shard(i)
+            if recenter:
+                a = _center_shard(a)
             shape, pharm = _shard_channels(a, m, store.schema)
             r = _lib.FssLibRows()
             if shape is not None:
@@ -460,4 +486,7 @@ def load_library(store, mode: str, device=0, shards: Optional[Sequence[int]] = N
         lib.fss_lib_destroy(h)
         raise
     all_ids = np.concatenate(ids) if ids else np.zeros(0, np.int64)
-    return ConformerLibrary.from_native(h, n, all_ids, nbytes)
+    out = ConformerLibrary.from_native(h, n, all_ids, nbytes)
+    out.num_surf_points = store.num_surf_points if store.schema.get("surf") else None
+    out.pre_centered = store.pre_centered or recenter
+    return out

@@ -378,6 +378,9 @@ typedef struct FssQuery {      /* host arrays, copied by fss_screen_submit */
   int32_t n_atoms;
   const float* atm;            /* (n_atoms, 4): x, y, z, partial charge (with hydrogens) */
   const float* rad;            /* (n_atoms) */
+  int32_t all_scores;          /* 1: also keep every library entry's best score, returned by
+                                  fss_screen_poll in FssHits.all_scores (screen.py:1174-1176,
+                                  scores_out) */
 } FssQuery;
 
 typedef struct FssHits {       /* host arrays of capacity entries, filled by fss_screen_poll */
@@ -388,6 +391,10 @@ typedef struct FssHits {       /* host arrays of capacity entries, filled by fss
   float* transform;            /* (capacity, 16): row-major 4x4 moving the library entry */
   double queue_ms;             /* out: submit -> admission */
   double latency_ms;           /* out: submit -> result */
+  double* all_scores;          /* for a query submitted with all_scores: every library entry's
+                                  best score in library order                                 */
+  int64_t n_all;               /* in: capacity of all_scores; out: entries written (the library
+                                  size at admission; 0 when the query did not ask)            */
 } FssHits;
 
 int fss_screen_create(FssLibrary* lib, const FssScoreParams* sp, const FssScreenParams* prm,
@@ -395,7 +402,9 @@ int fss_screen_create(FssLibrary* lib, const FssScoreParams* sp, const FssScreen
 /* Queues a query against the whole library (its size at admission); *ticket names it. */
 int fss_screen_submit(FssScreen* svc, const FssQuery* q, int64_t* ticket);
 /* Waits up to wait_ms (< 0: no limit) for the ticket's hits.  FSS_OK: hits written and the
- * ticket released; FSS_PENDING: not finished yet; an error code: the query failed. */
+ * ticket released; FSS_PENDING: not finished yet; an error code: the query failed.  A query
+ * submitted with all_scores needs out->all_scores with n_all >= the library size: otherwise
+ * FSS_E_INVALID, and the result stays held for a retry with a large enough buffer. */
 int fss_screen_poll(FssScreen* svc, int64_t ticket, int32_t wait_ms, FssHits* out);
 int fss_screen_stats(FssScreen* svc, int64_t* launches, int64_t* pairs);
 /* Stops the worker; queries still queued or running fail with FSS_E_CLOSED.  No other call on

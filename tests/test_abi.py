@@ -44,6 +44,7 @@ int main(void) {
          offsetof(FssQuery, typ), offsetof(FssHits, latency_ms));
   printf("%zu %zu %zu %zu %zu\n", offsetof(FssLibSpec, max_atoms), offsetof(FssScreenParams, shape_from_surface),
          offsetof(FssQuery, rad), sizeof(FssLibRows), offsetof(FssLibRows, acounts));
+  printf("%zu %zu %zu\n", offsetof(FssQuery, all_scores), offsetof(FssHits, all_scores), offsetof(FssHits, n_all));
   return 0;
 }
 '''
@@ -62,9 +63,11 @@ int main(void) {
                            ctypes.sizeof(_lib.FssQuery), ctypes.sizeof(_lib.FssHits)]
     assert sizes[12:16] == [_lib.FssLibSpec.max_pharm.offset, _lib.FssScreenParams.quat_table.offset,
                             _lib.FssQuery.typ.offset, _lib.FssHits.latency_ms.offset]
-    assert sizes[16:] == [_lib.FssLibSpec.max_atoms.offset, _lib.FssScreenParams.shape_from_surface.offset,
-                          _lib.FssQuery.rad.offset, ctypes.sizeof(_lib.FssLibRows),
-                          _lib.FssLibRows.acounts.offset]
+    assert sizes[16:21] == [_lib.FssLibSpec.max_atoms.offset, _lib.FssScreenParams.shape_from_surface.offset,
+                            _lib.FssQuery.rad.offset, ctypes.sizeof(_lib.FssLibRows),
+                            _lib.FssLibRows.acounts.offset]
+    assert sizes[21:] == [_lib.FssQuery.all_scores.offset, _lib.FssHits.all_scores.offset,
+                          _lib.FssHits.n_all.offset]
 
 
 def test_validation_rejects_bad_arguments_without_launching():

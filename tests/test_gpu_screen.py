@@ -9,7 +9,7 @@ import pytest
 import torch
 
 from fast_shepherd_score_amd import synth
-from fast_shepherd_score_amd.screen import ConformerLibrary, ScreenService, restart_table, screen
+from fast_shepherd_score_amd.screen import ConformerLibrary, ScreenService, restart_table, screen, screen_many
 from fast_shepherd_score_amd.seeds import rotmat
 from oracle import fss_oracle as orc
 
@@ -38,7 +38,7 @@ def test_screen_scores_match_oracle(dev):
     mols, lib = _library(40, 2, dev)
     query = {"points": mols[5]}
     B, steps = 8, 60
-    hits = screen(query, lib, "vol", top_k=40, num_repeats=B, max_num_steps=steps)
+    hits = screen(query, lib, "vol", top_k=40, num_repeats=B, max_num_steps=steps, do_center=False)
     table = restart_table(B).astype(np.float64)
     got = {h.id: h.score for h in hits}
     for i in (0, 7, 21, 39):
@@ -152,7 +152,7 @@ def test_store_library_screens_like_in_memory(dev, tmp_path, mode):
     from fast_shepherd_score_amd.store import MoleculeProfile, ProfileStore, load_library
     print(f"case store screen mode={mode}", flush=True)
     rng = np.random.default_rng(21)
-    S = 48
+    S = 50                      # ALPHA(num_surf_points) is tabulated for 50-400 points
     atoms = synth.random_walks(rng, rng.integers(10, 26, size=260))
     path = os.path.join(tmp_path, "lib.fss")
     with ProfileStore.create(path, num_surf_points=S, modes=("vol", "surf_esp", "pharm"),
@@ -205,7 +205,7 @@ def test_pharm_screen_extended_points_match_oracle(dev, only):
     qi = 4
     query = {"pharm_types": ph[qi][0].astype(np.int32), "pharm_anchors": ph[qi][1], "pharm_vectors": ph[qi][2]}
     B, steps = 8, 60
-    hits = screen(query, lib, "pharm", top_k=30, num_repeats=B, max_num_steps=steps,
+    hits = screen(query, lib, "pharm", top_k=30, num_repeats=B, max_num_steps=steps, do_center=False,
                   directionless=False, extended_points=True, only_extended=only)
     table = restart_table(B).astype(np.float64)
     got = {h.id: h.score for h in hits}
@@ -216,3 +216,38 @@ def test_pharm_screen_extended_points_match_oracle(dev, only):
                         ref_vec=ph[qi][2], fit_vec=ph[i][2], extended_points=True, only_extended=only)
         o = orc.optimize(pair, table, t0, max_steps=steps)
         assert abs(got[i] - o["score"]) < 1e-3, (i, got[i], o["score"])
+
+
+def test_scores_out_object_queries_and_default_alpha(dev):
+    """scores_out receives every entry's best score in library order (screen.py:1174-1176) and
+    agrees with the hits; a container Molecule query equals its dict; surf_esp takes
+    ALPHA(num_surf_points) when alpha is not given."""
+    from fast_shepherd_score_amd.constants import ALPHA
+    from fast_shepherd_score_amd.container import Molecule
+    print("case screen scores_out / objects / alpha", flush=True)
+    rng = np.random.default_rng(12)
+    S = 50
+    atoms = synth.random_walks(rng, rng.integers(10, 25, size=150))
+    surf = [synth.surface_points(rng, a, S) for a in atoms]
+    esp = [synth.esp_values(rng, S) for _ in atoms]
+    lib = ConformerLibrary.from_arrays(points=surf, esp=esp, device=dev)
+    qm = Molecule(atoms[9], surf_pos=surf[9], surf_esp=esp[9])
+    kw = dict(top_k=20, num_repeats=6, max_num_steps=30, launch_pairs=64)
+    out = np.full(150, -1.0)
+    hits = screen(qm, lib, "surf_esp", scores_out=out, **kw)
+    assert np.isfinite(out).all() and (out >= 0).all() and (out <= 1 + 1e-9).all()
+    for h in hits:
+        assert out[h.id] == h.score
+    np.testing.assert_array_equal(np.sort(out)[::-1][:20], [h.score for h in hits])
+    as_dict = screen({"points": surf[9], "esp": esp[9], "atom_pos": atoms[9]}, lib, "surf_esp", **kw)
+    explicit = screen(qm, lib, "surf_esp", alpha=ALPHA(S), **kw)
+    assert [h.score for h in as_dict] == [h.score for h in hits] == [h.score for h in explicit]
+    assert [h.id for h in as_dict] == [h.id for h in hits]
+    with pytest.raises(ValueError, match="explicit alpha"):
+        screen(qm, lib, "vol_and_surf_esp", **kw)
+    # a panel with per-query score arrays
+    outs = [np.zeros(150), np.zeros(150)]
+    res = screen_many([qm, Molecule(atoms[3], surf_pos=surf[3], surf_esp=esp[3])], lib, "surf_esp",
+                      scores_out=outs, **kw)
+    np.testing.assert_array_equal(outs[0], out)
+    assert res[1][0].id == 3 and outs[1][3] == res[1][0].score

@@ -85,7 +85,9 @@ def test_shaep_screen_scores_match_oracle(dev):
     B, steps = 4, 30
     lib = _library(mols, dev)
     qm = mols[2]
-    hits = screen(_query(qm), lib, MODE, top_k=6, num_repeats=B, max_num_steps=steps, early_stop=False)
+    # do_center=False: the oracle runs in the query's own frame, from the same seeds
+    hits = screen(_query(qm), lib, MODE, top_k=6, num_repeats=B, max_num_steps=steps, early_stop=False,
+                  alpha=0.81, do_center=False)
     got = {h.id: h.score for h in hits}
     table = restart_table(B).astype(np.float64)
     for i in (0, 2, 5):
@@ -118,7 +120,7 @@ def test_shaep_store_screens_like_in_memory(dev, tmp_path):
             st.add(MoleculeProfile(atom_pos=m["atom_pos"], surf_pos=m["surf_pos"], surf_esp=m["surf_esp"],
                                    partial_charges=m["partial_charges"], radii=m["radii"],
                                    centers_w_H=m["centers_w_H"], nonH_atoms_idx=np.arange(nh), id=i))
-    kw = dict(top_k=12, num_repeats=4, max_num_steps=20, launch_pairs=25)
+    kw = dict(top_k=12, num_repeats=4, max_num_steps=20, launch_pairs=25, alpha=0.81, do_center=False)
     lib = load_library(path, MODE)
     assert lib.n == 70 and lib.nbytes() > 0
     a = screen(_query(mols[11]), lib, MODE, **kw)
@@ -128,3 +130,8 @@ def test_shaep_store_screens_like_in_memory(dev, tmp_path):
     assert [h.score for h in a] == [h.score for h in b] == [h.score for h in c]
     assert [h.id for h in a] == [h.id for h in b] == [h.id for h in c]
     assert a[0].id == 11
+    # default centring (query, and this store on load) moves the frames; the scores agree to the
+    # best-restart bar (FP32 trajectories in another frame; the ESP mask is discontinuous)
+    kw.pop("do_center")
+    d = screen(_query(mols[11]), path, MODE, **kw)
+    np.testing.assert_allclose([h.score for h in d], [h.score for h in a], atol=1e-3)

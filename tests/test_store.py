@@ -222,3 +222,22 @@ def test_loader_shaep_rows_match_profiles(tmp_path):
     np.testing.assert_array_equal(atm[:, 3], np.concatenate([p.partial_charges for p in profs]))
     np.testing.assert_array_equal(rad, np.concatenate([p.radii for p in profs]))
     assert ac.tolist() == [7 + i for i in range(5)]
+
+
+def test_centring_on_load_matches_a_pre_centred_store(tmp_path):
+    """load_library(center=True) of a store written without centring moves every channel of each
+    conformer by its atom mean, as a pre-centred store does at write time."""
+    from fast_shepherd_score_amd.store import _center_shard
+    profs = [_profile(s, with_h=True) for s in range(4)]
+    arrays = {}
+    for pre in (False, True):
+        path = os.path.join(tmp_path, f"lib{int(pre)}.fss")
+        with ProfileStore.create(path, num_surf_points=32, modes=ALL_MODES, dtype="float32",
+                                 pre_centered=pre) as st:
+            for p in profs:
+                st.add(p)
+        arrays[pre] = ProfileStore.open(path).read_shard(0)[1]
+    c = _center_shard(arrays[False])
+    for k in ("atom_pos", "surf_pos", "pharm_ancs", "cwh"):
+        np.testing.assert_allclose(c[k], arrays[True][k], atol=1e-5, err_msg=k)
+    np.testing.assert_array_equal(c["pharm_vecs"], arrays[True]["pharm_vecs"])
