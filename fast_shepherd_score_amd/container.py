@@ -446,10 +446,9 @@ class MoleculePairBatch:
     def align_with_vol_and_surf_esp(self, alpha: float, lam: float = 0.001, probe_radius: float = 1.0,
                                     esp_weight: float = 0.5, num_repeats: int = None,
                                     trans_init: bool = False, lr: float = 0.1,
-                                    max_num_steps: int = None, num_workers: int = 1,
-                                    use_shmap: bool = False, verbose: bool = False,
+                                    max_num_steps: int = None, verbose: bool = False,
                                     backend: Optional[str] = None, return_aligned: bool = False,
-                                    **hip_kw):
+                                    num_workers: int = 1, use_shmap: bool = False, **hip_kw):
         """ShaEP score: (1 - esp_weight) shape Tanimoto (heavy atoms at alpha 0.81, surface
         points otherwise) + esp_weight surface ESP comparison (_batch.py:1030-1102)."""
         return self._run("vol_and_surf_esp", backend, return_aligned, "surf_pos", num_repeats,

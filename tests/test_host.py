@@ -280,3 +280,36 @@ def test_score_module_host_checks():
     a = torch.zeros((2, 3), requires_grad=True)
     with pytest.raises(NotImplementedError, match="gradient"):
         score.get_overlap_pharm(np.zeros(2), np.zeros(2), a, a, a, a)
+
+
+# Parameter names of the reference's MoleculePairBatch methods (container/_batch.py), in order;
+# the HIP container may take further keywords after them.
+BATCH_API = {
+    "align_with_vol": ["no_H", "num_repeats", "trans_init", "lr", "max_num_steps", "num_workers",
+                       "use_shmap", "num_buckets", "verbose", "backend", "alpha", "return_aligned"],
+    "align_with_vol_esp": ["lam", "no_H", "num_repeats", "trans_init", "lr", "max_num_steps",
+                           "num_workers", "use_shmap", "num_buckets", "verbose", "backend", "alpha",
+                           "return_aligned"],
+    "align_with_surf": ["alpha", "num_repeats", "trans_init", "lr", "max_num_steps", "use_jax",
+                        "use_analytical", "num_workers", "use_shmap", "verbose", "backend",
+                        "return_aligned"],
+    "align_with_surf_esp": ["alpha", "lam", "num_repeats", "trans_init", "lr", "max_num_steps",
+                            "use_jax", "use_analytical", "num_workers", "use_shmap", "verbose",
+                            "backend", "return_aligned"],
+    "align_with_vol_and_surf_esp": ["alpha", "lam", "probe_radius", "esp_weight", "num_repeats",
+                                    "trans_init", "lr", "max_num_steps", "verbose", "backend",
+                                    "return_aligned"],
+    "align_with_vol_color": ["color_weight", "alpha", "num_repeats", "trans_init", "lr",
+                             "max_num_steps", "verbose", "backend", "return_aligned"],
+    "align_with_pharm": ["similarity", "extended_points", "only_extended", "num_repeats",
+                         "trans_init", "lr", "max_num_steps", "num_workers", "use_shmap",
+                         "num_buckets", "verbose", "backend", "return_aligned"],
+}
+
+
+def test_batch_api_starts_with_the_reference_signatures():
+    import inspect
+    from fast_shepherd_score_amd.container import MoleculePairBatch
+    for name, params in BATCH_API.items():
+        got = [p for p in inspect.signature(getattr(MoleculePairBatch, name)).parameters][1:]
+        assert got[:len(params)] == params, name
