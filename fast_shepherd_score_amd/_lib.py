@@ -170,6 +170,8 @@ SIGNATURES = {
     "fss_adam_step": (ctypes.c_int, [ctypes.c_int32, _P, _P, _P, _P, _P, ctypes.c_int32,
                                      ctypes.c_float, ctypes.c_float, ctypes.c_float,
                                      ctypes.c_float, _P, ctypes.c_int32, _P]),
+    "fss_adam_qt": (ctypes.c_int, [ctypes.c_int32, _P, _P, _P, _P, _P, _P, _P, _P, ctypes.c_double,
+                                   ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_int32, _P]),
     "fss_early_stop_update": (ctypes.c_int, [ctypes.c_int32, ctypes.c_int32, _P, _P, _P, _P, _P,
                                              ctypes.c_int32, ctypes.c_float, ctypes.c_int32, _P]),
     "fss_pose_score": (ctypes.c_int, [_P, _P, ctypes.c_int32, _P, ctypes.c_int32, _P, _P, _P,

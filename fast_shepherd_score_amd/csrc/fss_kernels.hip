@@ -5,6 +5,7 @@
 //   k_pose_grad      one workgroup per pose: O_AB + dO/dR, dO/dt, dO/dq_hat (per channel), or the
 //                    full per-pose loss and SE(3) gradient of one optimiser step
 //   k_adam           torch.optim.Adam step over (n, 7) parameter rows
+//   k_adam_qt        the accelerated drivers' Adam on unit quaternions (no bias correction)
 //   k_align          persistent optimiser: one workgroup per (pair, restart-block); the pair's
 //                    clouds are staged once in LDS, every lane group of `lpp` lanes owns one
 //                    restart for all steps (parameters and Adam moments live in registers), the
@@ -605,6 +606,46 @@ __global__ void k_adam(int n, float* q, float* t, const float* __restrict__ g, f
   m[i] = mi;
   v[i] = vi;
   *p = adam_update(*p, mi, vi, step_size, ibc2s, eps);
+}
+
+// The reference's accelerated-driver Adam on unit quaternions: one thread per pose row, moments
+// without bias correction, eps inside the square root, optional tangent projection of the
+// quaternion gradient (dq - q (dq . q)), then renormalisation of q.
+__global__ void k_adam_qt(int n, float* q, float* t, const float* __restrict__ dQ,
+                          const float* __restrict__ dT, float* mq, float* vq, float* mt, float* vt,
+                          float lr, float b1, float c1, float b2, float c2, float eps, int proj) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  float qq[4], g[4];
+#pragma unroll
+  for (int k = 0; k < 4; ++k) { qq[k] = q[4 * i + k]; g[k] = dQ[4 * i + k]; }
+  if (proj) {
+    const float radial = fmaf(g[0], qq[0], fmaf(g[1], qq[1], fmaf(g[2], qq[2], g[3] * qq[3])));
+#pragma unroll
+    for (int k = 0; k < 4; ++k) g[k] = fmaf(-qq[k], radial, g[k]);
+  }
+  float nrm2 = 0.f;
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const float m = fmaf(b1, mq[4 * i + k], c1 * g[k]);
+    const float v = fmaf(b2, vq[4 * i + k], c2 * g[k] * g[k]);
+    mq[4 * i + k] = m;
+    vq[4 * i + k] = v;
+    qq[k] -= lr * m / sqrtf(v + eps);
+    nrm2 = fmaf(qq[k], qq[k], nrm2);
+  }
+  const float nrm = sqrtf(nrm2);
+#pragma unroll
+  for (int k = 0; k < 4; ++k) q[4 * i + k] = qq[k] / nrm;
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    const float gk = dT[3 * i + k];
+    const float m = fmaf(b1, mt[3 * i + k], c1 * gk);
+    const float v = fmaf(b2, vt[3 * i + k], c2 * gk * gk);
+    mt[3 * i + k] = m;
+    vt[3 * i + k] = v;
+    t[3 * i + k] -= lr * m / sqrtf(v + eps);
+  }
 }
 
 // ------------------------------------------------------------------------------------------------
@@ -1476,6 +1517,19 @@ int fss_adam_step(int32_t n, float* q, float* t, const float* grad, float* m, fl
   hipLaunchKernelGGL(k_adam, dim3((tot + 255) / 256), dim3(256), 0, (hipStream_t)stream, n, q, t, grad, m, v,
                      step, lr, beta1, beta2, eps, pair_active, pair_active ? n_restarts : 1);
   return check_launch("fss_adam_step");
+}
+
+int fss_adam_qt(int32_t n, float* q, float* t, const float* dQ, const float* dT, float* m_q, float* v_q,
+                float* m_t, float* v_t, double lr, double beta1, double beta2, double eps,
+                int32_t tangent_proj, void* stream) {
+  if (n < 0 || (n > 0 && (!q || !t || !dQ || !dT || !m_q || !v_q || !m_t || !v_t)))
+    return set_error(FSS_E_INVALID, "NULL Adam arrays");
+  if (n == 0) return FSS_OK;
+  // 1 - beta in double, as the reference forms the moment weights from Python floats
+  hipLaunchKernelGGL(k_adam_qt, dim3((n + 255) / 256), dim3(256), 0, (hipStream_t)stream, n, q, t, dQ, dT,
+                     m_q, v_q, m_t, v_t, (float)lr, (float)beta1, (float)(1.0 - beta1), (float)beta2,
+                     (float)(1.0 - beta2), (float)eps, (int)(tangent_proj != 0));
+  return check_launch("fss_adam_qt");
 }
 
 int fss_early_stop_update(int32_t n_pairs, int32_t n_restarts, const float* loss, double* last,

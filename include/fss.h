@@ -199,6 +199,18 @@ int fss_adam_step(int32_t n, float* q, float* t, const float* grad, float* m, fl
                   const int32_t* pair_active, int32_t n_restarts, void* stream);
 
 /*
+ * The accelerated drivers' Adam, in place over n pose rows: q (n,4) unit quaternions, t (n,3),
+ * their gradients dQ, dT and first / second moments.  Moments without bias correction, eps
+ * inside the square root, q renormalised after the step; tangent_proj != 0 first replaces dQ
+ * by dQ - q (dQ . q).
+ * Replaces: fused_adam_qt / fused_adam_qt_with_tangent_proj (accel/kernels/dispatch.py:96-112;
+ *   shape_triton.py:253-406, the CPU twin cpu.py:112-138).
+ */
+int fss_adam_qt(int32_t n, float* q, float* t, const float* dQ, const float* dT, float* m_q, float* v_q,
+                float* m_t, float* v_t, double lr, double beta1, double beta2, double eps,
+                int32_t tangent_proj, void* stream);
+
+/*
  * Early-stop bookkeeping of a stepped optimiser, on device (no host synchronisation):
  * L = mean of the pair's n_restarts losses; counter = |L - last| > tol ? 0 : counter + 1;
  * last = L; n_steps = step; once counter > patience the pair's active flag drops to 0.

@@ -313,3 +313,46 @@ def test_batch_api_starts_with_the_reference_signatures():
     for name, params in BATCH_API.items():
         got = [p for p in inspect.signature(getattr(MoleculePairBatch, name)).parameters][1:]
         assert got[:len(params)] == params, name
+
+
+# Positional and keyword-only parameters of the reference's kernel table (accel/kernels/cpu.py,
+# the same signatures as the Triton kernels).
+KERNEL_API = {
+    "overlap_score_grad_se3_batch": (['A', 'B', 'q', 't'], ['alpha', 'N_real', 'M_real', 'NEED_GRAD', 'BLOCK', 'num_warps', 'num_stages']),
+    "fused_adam_qt_with_tangent_proj": (['q', 't', 'dQ', 'dT', 'm_q', 'v_q', 'm_t', 'v_t', 'lr', 'beta1', 'beta2', 'eps'], []),
+    "fused_adam_qt": (['q', 't', 'dQ', 'dT', 'm_q', 'v_q', 'm_t', 'v_t', 'lr', 'beta1', 'beta2', 'eps'], []),
+    "_batch_self_overlap": (['P_pad', 'N_real', 'alpha'], []),
+    "overlap_score_grad_esp_se3_batch": (['A', 'B', 'charges_A', 'charges_B', 'q', 't'], ['alpha', 'lam', 'N_real', 'M_real', 'NEED_GRAD', 'BLOCK', 'num_warps', 'num_stages']),
+    "esp_comparison_batch": (['points', 'atoms', 'charges', 'point_esp', 'radii'], ['N_real', 'M_real', 'probe_radius', 'lam', 'BLOCK', 'num_warps', 'num_stages']),
+    "pharm_score_grad_se3_batch": (['R', 't', 'ref_types', 'fit_types', 'ref_anchors', 'fit_anchors', 'ref_vectors', 'fit_vectors', 'alphas', 'Ks', 'cats'], ['N_real', 'M_real', 'NEED_GRAD', 'BLOCK', 'num_warps']),
+    "pharm_color_score_grad_se3_batch": (['A', 'B', 'q', 't', 'ref_types', 'fit_types', 'alphas', 'Ks', 'cats'], ['N_real', 'M_real', 'NEED_GRAD', 'BLOCK', 'num_warps', 'num_stages']),
+    "pharm_grad_dq_se3_batch": (['q', 't', 'ref_types', 'fit_types', 'ref_anchors', 'fit_anchors', 'ref_vectors', 'fit_vectors', 'alphas', 'Ks', 'cats'], ['N_real', 'M_real', 'NEED_GRAD', 'BLOCK', 'num_warps', 'num_stages']),
+    "_batch_self_overlap_esp": (['P_pad', 'charges', 'N_real', 'alpha', 'lam'], []),
+}
+
+
+def test_kernel_table_matches_reference_signatures():
+    import inspect
+    from fast_shepherd_score_amd import kernels
+    for name, (pos, kwo) in KERNEL_API.items():
+        ps = inspect.signature(getattr(kernels, name)).parameters.values()
+        assert [p.name for p in ps if p.kind == p.POSITIONAL_OR_KEYWORD] == pos, name
+        assert [p.name for p in ps if p.kind == p.KEYWORD_ONLY] == kwo, name
+    assert sorted(kernels.__all__) == sorted(KERNEL_API)
+
+
+def test_kernel_table_host_helpers():
+    import torch
+    from fast_shepherd_score_amd import kernels
+    from oracle import fss_oracle as orc
+    rng = np.random.default_rng(4)
+    q = torch.tensor(rng.normal(size=(50, 4)))
+    q = q / q.norm(dim=1, keepdim=True)
+    R = orc.rot_from_unit_quat(q)
+    q2 = kernels._quat_from_rot(R)                      # the same rotation, sign aside
+    np.testing.assert_allclose(orc.rot_from_unit_quat(q2).numpy(), R.numpy(), atol=1e-12)
+    rr = kernels._ranges(3, 7, torch.tensor([2, 9, 0]), "cpu")
+    assert rr.tolist() == [[0, 2], [7, 7], [14, 0]]      # counts clamped to the padding
+    with pytest.raises(RuntimeError, match="device tensors"):
+        kernels.overlap_score_grad_se3_batch(torch.zeros(1, 2, 3), torch.zeros(1, 2, 3),
+                                             torch.zeros(1, 4), torch.zeros(1, 3))
