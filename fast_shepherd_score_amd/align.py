@@ -147,12 +147,14 @@ N_SIMD = 256 * 4          # MI355X: 256 CUs x 4 SIMDs
 def tiling_cost(ref_counts, fit_counts, B: int, lpp: int, rpw: int, waves_per_simd: int = 4) -> float:
     """Modelled optimiser-step time of a tiling (arbitrary units).
 
-    A wave's serial work is its lanes' ceil(M / lpp) row slots over the ref cloud (about 10
-    issue slots per pair term) plus the butterfly reduction and the per-pose tail.  The chip
-    retires the launch's waves N_SIMD at a time; a CU holds as many workgroups as fit in
-    ``4 x waves_per_simd`` wave slots, and a workgroup whose wave count does not divide evenly
-    over the CU's 4 SIMDs leaves SIMDs idle (the measured MI355X behaviour that favours
-    workgroups of 1, 2, 4 or 8 waves)."""
+    A wave's serial work U is its lanes' ceil(M / lpp) row slots over the ref cloud (about 10
+    issue slots per pair term) plus the lane-group reduction and the per-pose tail.  A SIMD
+    running w of the launch's waves takes max(LATENCY, w) x U: alone, a wave is latency-bound
+    (about 2.2 times its issue time, measured on small MI355X launches); from about two waves per
+    SIMD on, the SIMD is issue-bound.  The waves spread over the CUs the launch's workgroups
+    reach (at most 256); a CU holds as many workgroups as fit in ``4 x waves_per_simd`` wave
+    slots, and a workgroup whose wave count does not divide evenly over the CU's 4 SIMDs leaves
+    SIMDs idle (the measured MI355X behaviour that favours workgroups of 1, 2, 4 or 8 waves)."""
     n = np.maximum(np.asarray(ref_counts, np.float64), 1.0)
     m = np.maximum(np.asarray(fit_counts, np.float64), 1.0)
     per_wave = np.ceil(m / lpp) * n * 10.0 + 26.0 * np.log2(max(lpp, 1)) + 150.0
@@ -162,7 +164,12 @@ def tiling_cost(ref_counts, fit_counts, B: int, lpp: int, rpw: int, waves_per_si
     resident = wgs_per_cu * wpg
     simd_eff = resident / (4.0 * np.ceil(resident / 4.0))
     waves = float(_waves_per_pair(B, lpp, rpw) * len(m))
-    return float(np.mean(per_wave)) * max(waves / (N_SIMD * simd_eff), 0.5)
+    busy_cu = min(N_SIMD // 4, int(np.ceil(B / rpw)) * len(m))
+    w = max(waves / (4.0 * busy_cu * simd_eff), float(np.ceil(wpg / 4.0)))
+    return float(np.mean(per_wave)) * max(LATENCY_WAVES, w)
+
+
+LATENCY_WAVES = 2.2       # waves per SIMD below which a wave's latency, not issue, sets the step
 
 
 def default_tiling(pb: PairBatch, mode: int, B: int, early_stop: bool) -> Tuple[int, int]:

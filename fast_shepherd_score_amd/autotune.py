@@ -7,7 +7,8 @@ with both inner-loop variants, on the actual batch.  Each candidate runs the opt
 with early stop off (so every candidate does identical work), for ``steps`` and ``3 * steps``
 steps; the difference gives the per-step time and the rest the fixed per-launch cost (pair
 staging into LDS, which grows with the workgroup count), and the candidate with the lowest
-predicted time at ``max_steps`` wins.  Results are cached in memory and, when
+predicted time at ``max_steps`` wins; a runner-up within 5 % is timed again, with the leader,
+over a longer run, and the faster of the two wins.  Results are cached in memory and, when
 ``FSS_AUTOTUNE_CACHE`` names a file, on disk as JSON, keyed by mode, restart count, step count,
 early-stop flag and the batch's size profile.
 """
@@ -98,25 +99,39 @@ def autotune(pb: PairBatch, spec: ScoreSpec, q0: torch.Tensor, t0: torch.Tensor,
     if self_ov is None:
         self_ov = self_overlaps(pb, spec)
     s1, s2 = steps, 3 * steps
-    best, best_ms = None, float("inf")
+
+    def timed(n, kw):
+        ev = []
+        align(pb, spec, q0, t0, OptimSpec(max_steps=n, early_stop=False), events=ev, **kw)
+        torch.cuda.synchronize()
+        return ev[0][0].elapsed_time(ev[0][1])
+
+    scored = []
     for lpp, rpw, var in runs:
         if early_stop and rpw < B:
             continue
         kw = dict(lanes_per_pose=lpp, restarts_per_wg=rpw, self_ov=self_ov, variant=var)
         align(pb, spec, q0, t0, OptimSpec(max_steps=s1, early_stop=False), **kw)
-        t = []
-        for n in (s1, s2):
-            ev = []
-            align(pb, spec, q0, t0, OptimSpec(max_steps=n, early_stop=False), events=ev, **kw)
-            torch.cuda.synchronize()
-            t.append(ev[0][0].elapsed_time(ev[0][1]))
+        t = [timed(n, kw) for n in (s1, s2)]
         per_step = max(t[1] - t[0], 0.0) / (s2 - s1)
         ms = t[0] + (max_steps - s1) * per_step
         if verbose:
             print(f"autotune {key}: lpp={lpp} rpw={rpw} variant={var} {t[0]:.3f}/{t[1]:.3f} ms "
                   f"-> {ms:.3f} ms at {max_steps} steps", flush=True)
-        if ms < best_ms:
-            best, best_ms = (lpp, rpw, var), ms
+        scored.append((ms, (lpp, rpw, var)))
+    scored.sort()
+    best = scored[0][1]
+    # Short trials carry a few per cent of noise: a close runner-up is timed again, with both
+    # candidates over a longer run, and the faster one wins.
+    if len(scored) > 1 and scored[1][0] < 1.05 * scored[0][0] and max_steps > s2:
+        n = min(max_steps, 4 * s2)
+        long_ms = {}
+        for _, c in scored[:2]:
+            kw = dict(lanes_per_pose=c[0], restarts_per_wg=c[1], self_ov=self_ov, variant=c[2])
+            long_ms[c] = min(timed(n, kw) for _ in range(2))
+            if verbose:
+                print(f"autotune {key}: {c} over {n} steps {long_ms[c]:.3f} ms", flush=True)
+        best = min(long_ms, key=long_ms.get)
     with _lock:
         _cache[key] = best
         _save_disk()

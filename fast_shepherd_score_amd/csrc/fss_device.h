@@ -73,23 +73,55 @@ struct Acc {
 };
 
 // Butterfly sum over `width` consecutive lanes (width a power of two <= 64); every lane of the
-// group ends with the bit-identical total (fp add is commutative).
+// group ends with the bit-identical total (fp add is commutative).  Level L adds the value of a
+// lane in the other half of the aligned 2^(L+1)-lane block; once the halves hold uniform partial
+// sums any such partner gives the xor-butterfly's result, so the levels inside a 16-lane row use
+// DPP moves (quad_perm [1,0,3,2], [2,3,0,1], row_half_mirror, row_mirror), level 4 a ds_swizzle
+// xor 16 and level 5 a permute across the two 32-lane halves.
+template <int CTRL>
+__device__ __forceinline__ float dpp_move(float v) {
+  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, 0xF, 0xF, true));
+}
+template <int L>
+__device__ __forceinline__ float sum_level(float v) {
+  if constexpr (L == 0) return v + dpp_move<0xB1>(v);
+  else if constexpr (L == 1) return v + dpp_move<0x4E>(v);
+  else if constexpr (L == 2) return v + dpp_move<0x141>(v);
+  else if constexpr (L == 3) return v + dpp_move<0x140>(v);
+  else if constexpr (L == 4) return v + __int_as_float(__builtin_amdgcn_ds_swizzle(__float_as_int(v), 0x401F));
+  else return v + __shfl_xor(v, 32, 64);
+}
 __device__ __forceinline__ float group_sum(float v, int width) {
-  for (int m = 1; m < width; m <<= 1) v += __shfl_xor(v, m, 64);
+  if (width > 1) v = sum_level<0>(v);
+  if (width > 2) v = sum_level<1>(v);
+  if (width > 4) v = sum_level<2>(v);
+  if (width > 8) v = sum_level<3>(v);
+  if (width > 16) v = sum_level<4>(v);
+  if (width > 32) v = sum_level<5>(v);
   return v;
 }
 __device__ __forceinline__ double group_sum(double v, int width) {
   for (int m = 1; m < width; m <<= 1) v += __shfl_xor(v, m, 64);
   return v;
 }
-__device__ __forceinline__ void group_sum(Acc& a, int width, bool grad) {
-  a.V = group_sum(a.V, width);
+// The 13 sums of a pose level by level, so each level's exchanges are independent.
+template <int L>
+__device__ __forceinline__ void sum_level(Acc& a, bool grad) {
+  a.V = sum_level<L>(a.V);
   if (grad) {
 #pragma unroll
-    for (int i = 0; i < 3; ++i) a.gt[i] = group_sum(a.gt[i], width);
+    for (int i = 0; i < 3; ++i) a.gt[i] = sum_level<L>(a.gt[i]);
 #pragma unroll
-    for (int i = 0; i < 9; ++i) a.G[i] = group_sum(a.G[i], width);
+    for (int i = 0; i < 9; ++i) a.G[i] = sum_level<L>(a.G[i]);
   }
+}
+__device__ __forceinline__ void group_sum(Acc& a, int width, bool grad) {
+  if (width > 1) sum_level<0>(a, grad);
+  if (width > 2) sum_level<1>(a, grad);
+  if (width > 4) sum_level<2>(a, grad);
+  if (width > 8) sum_level<3>(a, grad);
+  if (width > 16) sum_level<4>(a, grad);
+  if (width > 32) sum_level<5>(a, grad);
 }
 
 // ---------------------------------------------------------------------------------------------
