@@ -666,6 +666,7 @@ struct PhTab {
   float ak[FSS_N_PTYPES];    // alpha * K
   float k[FSS_N_PTYPES];     // K
   int cat[FSS_N_PTYPES];
+  int ext;                   // some type has category 4 or 5 (extended points)
 };
 
 // ---------------------------------------------------------------------------------------------
@@ -673,10 +674,10 @@ struct PhTab {
 // ref rows are sorted by type, so fit row b (type t) visits only ref rows [poff[t], poff[t+1]).
 // Categories 4 and 5 are the extended-point form of a directional type: the anchor pair term
 // has weight 1 (4) or 0 (5) and no vector factor, and a second isotropic term couples the
-// extended points anchor + unit vector of the two features.  The fit extended point moves with
-// the pose as b + u, so its force f_e enters dO/dt as f_e and dO/dR as f_e (x) (p + u0): it is
-// accumulated into both the anchor force f (outer product with p) and the vector force h
-// (outer product with u0), which category 4/5 leaves free (no cosine weight).
+// extended points anchor + unit vector of the two features.  That term runs as a second pass,
+// only when the table has such a category, so the common loop keeps its register budget.  The
+// fit extended point moves with the pose as b + u, so its force f_e enters dO/dt as f_e and
+// dO/dR as f_e (x) (p + u0).
 // ---------------------------------------------------------------------------------------------
 template <bool GRAD>
 __device__ __forceinline__ void pharm_accum(const float4* __restrict__ ranc,
@@ -726,17 +727,6 @@ __device__ __forceinline__ void pharm_accum(const float4* __restrict__ ranc,
         const float hc = kk * (1.f / 3.f) * e * c;
         hx = fmaf(hc, rv.x, hx); hy = fmaf(hc, rv.y, hy); hz = fmaf(hc, rv.z, hz);
       }
-      if (cat >= 4) {
-        const float ex = (r.x + rv.x) - (bx + ux), ey = (r.y + rv.y) - (by + uy),
-                    ez = (r.z + rv.z) - (bz + uz);
-        const float e2 = fast_exp2(-kap * fmaf(ex, ex, fmaf(ey, ey, ez * ez)));
-        v = fmaf(kk, e2, v);
-        if (GRAD) {
-          const float fe = ak * e2;
-          fx = fmaf(fe, ex, fx); fy = fmaf(fe, ey, fy); fz = fmaf(fe, ez, fz);
-          hx = fmaf(fe, ex, hx); hy = fmaf(fe, ey, hy); hz = fmaf(fe, ez, hz);
-        }
-      }
     }
     acc.V += v;
     if (GRAD) {
@@ -744,6 +734,39 @@ __device__ __forceinline__ void pharm_accum(const float4* __restrict__ ranc,
       acc.G[0] += fx * p.x + hx * u.x; acc.G[1] += fx * p.y + hx * u.y; acc.G[2] += fx * p.z + hx * u.z;
       acc.G[3] += fy * p.x + hy * u.x; acc.G[4] += fy * p.y + hy * u.y; acc.G[5] += fy * p.z + hy * u.z;
       acc.G[6] += fz * p.x + hz * u.x; acc.G[7] += fz * p.y + hz * u.y; acc.G[8] += fz * p.z + hz * u.z;
+    }
+  }
+  if (!tab.ext) return;
+  for (int b = j; b < M; b += lpp) {   // extended points of categories 4 and 5
+    const float4 p = fanc[b];
+    const int tb = __float_as_int(p.w);
+    if (tb < 0 || tb >= FSS_N_PTYPES || tab.cat[tb] < 4) continue;
+    const float4 u = fvec[b];
+    const float kap = tab.kap[tb], ak = tab.ak[tb], kk = tab.k[tb];
+    // the moved extended point R (p + u) + t
+    const float px = p.x + u.x, py = p.y + u.y, pz = p.z + u.z;
+    const float ex0 = fmaf(R[0], px, fmaf(R[1], py, fmaf(R[2], pz, t[0])));
+    const float ey0 = fmaf(R[3], px, fmaf(R[4], py, fmaf(R[5], pz, t[1])));
+    const float ez0 = fmaf(R[6], px, fmaf(R[7], py, fmaf(R[8], pz, t[2])));
+    float fx = 0.f, fy = 0.f, fz = 0.f, v = 0.f;
+    const int a_end = poff[tb + 1];
+    for (int a = poff[tb]; a < a_end; ++a) {
+      const float4 r = ranc[a];
+      const float4 rv = rvec[a];
+      const float ex = (r.x + rv.x) - ex0, ey = (r.y + rv.y) - ey0, ez = (r.z + rv.z) - ez0;
+      const float e2 = fast_exp2(-kap * fmaf(ex, ex, fmaf(ey, ey, ez * ez)));
+      v = fmaf(kk, e2, v);
+      if (GRAD) {
+        const float fe = ak * e2;
+        fx = fmaf(fe, ex, fx); fy = fmaf(fe, ey, fy); fz = fmaf(fe, ez, fz);
+      }
+    }
+    acc.V += v;
+    if (GRAD) {
+      acc.gt[0] += fx; acc.gt[1] += fy; acc.gt[2] += fz;
+      acc.G[0] += fx * px; acc.G[1] += fx * py; acc.G[2] += fx * pz;
+      acc.G[3] += fy * px; acc.G[4] += fy * py; acc.G[5] += fy * pz;
+      acc.G[6] += fz * px; acc.G[7] += fz * py; acc.G[8] += fz * pz;
     }
   }
 }

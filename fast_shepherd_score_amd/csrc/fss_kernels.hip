@@ -119,6 +119,7 @@ int make_kscore(const FssScoreParams* sp, KScore& ks) {
     ks.tab.ak[i] = (float)(sp->ph_alpha[i] * sp->ph_k[i]);
     ks.tab.k[i] = (float)sp->ph_k[i];
     ks.tab.cat[i] = sp->ph_cat[i];
+    ks.tab.ext |= sp->ph_cat[i] >= 4;
   }
   if (sp->mode == FSS_MODE_ESP_COMBO) {
     ks.lam_d = 1.0;                    // the shape channel has no ESP weighting here
