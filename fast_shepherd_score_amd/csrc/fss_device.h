@@ -31,10 +31,11 @@ __device__ __forceinline__ float fast_exp2(float x) { return __builtin_amdgcn_ex
 
 // Rotation from a raw quaternion in FP64: qn = |q|, qh = q / qn, R(qh) row-major.
 __device__ __forceinline__ void rotation_fp64(const float q[4], double qh[4], double& qn,
-                                              double Rd[9]) {
+                                              double Rd[9], double* inv_qn = nullptr) {
   const double w = q[0], x = q[1], y = q[2], z = q[3];
   qn = sqrt(w * w + x * x + y * y + z * z);
   const double inv = 1.0 / qn;
+  if (inv_qn) *inv_qn = inv;
   qh[0] = w * inv; qh[1] = x * inv; qh[2] = y * inv; qh[3] = z * inv;
   const double a = qh[0], b = qh[1], c = qh[2], d = qh[3];
   Rd[0] = 1.0 - 2.0 * (c * c + d * d); Rd[1] = 2.0 * (b * c - d * a);       Rd[2] = 2.0 * (b * d + c * a);

@@ -420,12 +420,12 @@ __device__ __forceinline__ T similarity(const KScore& ks, T Os, T Op, const doub
 }
 
 // Per-pose loss gradient tail: dL/dR, dL/dt -> [dL/dq_raw, dL/dt] / B (projection in FP64).
+// iq = 1 / |q_raw| as rotation_fp64 computed it.
 __device__ __forceinline__ void se3_grad(const float gR[9], const float gT[3], const double qh[4],
-                                         double qn, float invB, float g[7]) {
+                                         double iq, float invB, float g[7]) {
   double gq[4];
   project_to_quat(gR, qh, gq);
   const double dot = qh[0] * gq[0] + qh[1] * gq[1] + qh[2] * gq[2] + qh[3] * gq[3];
-  const double iq = 1.0 / qn;
 #pragma unroll
   for (int k = 0; k < 4; ++k) g[k] = (float)((gq[k] - qh[k] * dot) * iq) * invB;
 #pragma unroll
@@ -525,8 +525,8 @@ __global__ __launch_bounds__(kPoseBlock) void k_pose_grad(FssPairBatch pb, KScor
   const Staged st = stage_pair<SH, ESP, PH, true, false, SE>(pb, ks, lp, pair, 0, 1, lds);
   float q[4] = {qin[4 * pose], qin[4 * pose + 1], qin[4 * pose + 2], qin[4 * pose + 3]};
   const float t[3] = {tin[3 * pose], tin[3 * pose + 1], tin[3 * pose + 2]};
-  double qh[4], qn, Rd[9];
-  rotation_fp64(q, qh, qn, Rd);
+  double qh[4], qn, iq, Rd[9];
+  rotation_fp64(q, qh, qn, Rd, &iq);
   float R[9];
 #pragma unroll
   for (int i = 0; i < 9; ++i) R[i] = (float)Rd[i];
@@ -570,7 +570,7 @@ __global__ __launch_bounds__(kPoseBlock) void k_pose_grad(FssPairBatch pb, KScor
     for (int i = 0; i < 9; ++i) lR[i] = -(ws * sh.G[i] + wp * ph.G[i]);
     for (int i = 0; i < 3; ++i) lT[i] = -(ws * sh.gt[i] + wp * ph.gt[i]);
     float g[7];
-    se3_grad(lR, lT, qh, qn, invB, g);
+    se3_grad(lR, lT, qh, iq, invB, g);
     loss[pose] = 1.f - S;
     for (int i = 0; i < 7; ++i) grad[7 * pose + i] = g[i];
   }
@@ -712,8 +712,8 @@ void k_align(FssPairBatch pb, KScore ks, LdsPlan lp, AlignK ak) {
   for (int step = 1; step <= ak.max_steps; ++step) {
     float loss = 0.f;
     if (active) {
-      double qh[4], qn, Rd[9];
-      rotation_fp64(p, qh, qn, Rd);
+      double qh[4], qn, iq, Rd[9];
+      rotation_fp64(p, qh, qn, Rd, &iq);
       float R[9];
 #pragma unroll
       for (int i = 0; i < 9; ++i) R[i] = (float)Rd[i];
@@ -747,7 +747,7 @@ void k_align(FssPairBatch pb, KScore ks, LdsPlan lp, AlignK ak) {
 #pragma unroll
       for (int i = 0; i < 3; ++i) lT[i] = -(ws * sh.gt[i] + wp * ph.gt[i]);
       float g[7];
-      se3_grad(lR, lT, qh, qn, invB, g);
+      se3_grad(lR, lT, qh, iq, invB, g);
       b1p *= (double)ak.b1;
       b2p *= (double)ak.b2;
       const float step_size = (float)((double)ak.lr / (1.0 - b1p));
