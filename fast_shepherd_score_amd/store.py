@@ -85,16 +85,42 @@ class MoleculeProfile:
     atom_pos_noH: Optional[np.ndarray] = None
     id: object = None
 
+    def center_to(self, xyz_means) -> None:
+        """Move every position array by -xyz_means; vectors, charges and radii are unchanged
+        (screen.py:178-190)."""
+        mu = np.asarray(xyz_means, dtype=np.float32)
+        for k in ("atom_pos", "atom_pos_noH", "surf_pos", "pharm_ancs", "centers_w_H"):
+            v = getattr(self, k)
+            if v is not None:
+                setattr(self, k, np.asarray(v, np.float32) - mu)
+
     @classmethod
-    def from_molecule(cls, m, id=None) -> "MoleculeProfile":
-        """From anything with the container's Molecule attributes (missing ones stay None)."""
+    def from_molecule(cls, m, *, modes=None, id=None) -> "MoleculeProfile":
+        """From anything with the container's Molecule attributes (missing ones stay None).
+        ``modes``: keep only the arrays those modes need (screen.py:193-197)."""
         g = lambda k: getattr(m, k, None)
-        return cls(atom_pos=np.asarray(m.atom_pos, np.float32), surf_pos=g("surf_pos"),
-                   surf_esp=g("surf_esp"), partial_charges=g("atom_charges") if g("partial_charges") is None
-                   else g("partial_charges"), radii=g("radii"), nonH_atoms_idx=g("nonH_atoms_idx"),
-                   pharm_types=g("pharm_types"), pharm_ancs=g("pharm_ancs"), pharm_vecs=g("pharm_vecs"),
-                   centers_w_H=g("centers_w_H"), atom_pos_noH=g("atom_pos_noH"),
-                   id=id if id is not None else g("id"))
+        p = cls(atom_pos=np.asarray(m.atom_pos, np.float32), surf_pos=g("surf_pos"),
+                surf_esp=g("surf_esp"), partial_charges=g("atom_charges") if g("partial_charges") is None
+                else g("partial_charges"), radii=g("radii"), nonH_atoms_idx=g("nonH_atoms_idx"),
+                pharm_types=g("pharm_types"), pharm_ancs=g("pharm_ancs"), pharm_vecs=g("pharm_vecs"),
+                centers_w_H=g("centers_w_H"), atom_pos_noH=g("atom_pos_noH"),
+                id=id if id is not None else g("id"))
+        if modes is not None:
+            sch = schema_from_modes(modes)
+            drop = []
+            if not sch["surf"]:
+                drop += ["surf_pos"]
+            if not sch["surf_esp"]:
+                drop += ["surf_esp"]
+            if not sch["charges"]:
+                drop += ["partial_charges", "nonH_atoms_idx"]
+            if not sch["with_H"]:
+                drop += ["radii", "centers_w_H"]
+            if not sch["pharm"]:
+                drop += ["pharm_types", "pharm_ancs", "pharm_vecs"]
+            for k in drop:
+                setattr(p, k, None)
+        return p
 
 
 def _f32(a) -> np.ndarray:
@@ -151,7 +177,11 @@ class ProfileStore:
         if len(self._buf) >= self.manifest["shard_size"]:
             self._flush()
 
-    add_profile = add
+    def add_profile(self, profile: "MoleculeProfile", id=None) -> None:
+        """Add a MoleculeProfile as it is (screen.py:420-424); ``id`` defaults to its own."""
+        if not isinstance(profile, MoleculeProfile):
+            raise TypeError("add_profile expects a MoleculeProfile")
+        self.add(profile, id=id if id is not None else profile.id)
 
     def _checked(self, p: MoleculeProfile, id) -> MoleculeProfile:
         sch = self.manifest["schema"]
@@ -307,34 +337,40 @@ class ProfileStore:
 
     def iter_profiles(self) -> Iterator[List[MoleculeProfile]]:
         """One shard at a time as MoleculeProfile lists (float32 in memory)."""
-        sch = self.schema
         for i in range(self.num_shards):
-            sh, a = self.read_shard(i)
-            ao = a["atom_off"]
-            po = a.get("pharm_off")
-            out = []
-            for j in range(sh["n"]):
-                p = MoleculeProfile(atom_pos=_f32(a["atom_pos"][ao[j]:ao[j + 1]]), id=a["ids"][j].item())
-                if sch["surf"]:
-                    p.surf_pos = _f32(a["surf_pos"][j])
-                if sch["surf_esp"]:
-                    p.surf_esp = _f32(a["surf_esp"][j])
-                if sch["charges"]:
-                    if sch["with_H"]:
-                        c0, c1 = a["all_off"][j], a["all_off"][j + 1]
-                        p.partial_charges = _f32(a["charges"][c0:c1])
-                        p.nonH_atoms_idx = a["nonH"][ao[j]:ao[j + 1]]
-                        p.radii = _f32(a["radii"][c0:c1])
-                        p.centers_w_H = _f32(a["cwh"][c0:c1])
-                    else:
-                        ho = a["heavy_off"] if "heavy_off" in a else ao
-                        p.partial_charges = _f32(a["charges"][ho[j]:ho[j + 1]])
-                if sch["pharm"]:
-                    p.pharm_types = a["pharm_types"][po[j]:po[j + 1]]
-                    p.pharm_ancs = _f32(a["pharm_ancs"][po[j]:po[j + 1]])
-                    p.pharm_vecs = _f32(a["pharm_vecs"][po[j]:po[j + 1]])
-                out.append(p)
-            yield out
+            yield self.read_profiles(i)
+
+    iter_shards = iter_profiles            # the reference's name (screen.py:556-560)
+
+    def read_profiles(self, idx: int) -> List[MoleculeProfile]:
+        """Shard ``idx`` as a MoleculeProfile list (random access; screen.py:562-565)."""
+        sch = self.schema
+        sh, a = self.read_shard(idx)
+        ao = a["atom_off"]
+        po = a.get("pharm_off")
+        out = []
+        for j in range(sh["n"]):
+            p = MoleculeProfile(atom_pos=_f32(a["atom_pos"][ao[j]:ao[j + 1]]), id=a["ids"][j].item())
+            if sch["surf"]:
+                p.surf_pos = _f32(a["surf_pos"][j])
+            if sch["surf_esp"]:
+                p.surf_esp = _f32(a["surf_esp"][j])
+            if sch["charges"]:
+                if sch["with_H"]:
+                    c0, c1 = a["all_off"][j], a["all_off"][j + 1]
+                    p.partial_charges = _f32(a["charges"][c0:c1])
+                    p.nonH_atoms_idx = a["nonH"][ao[j]:ao[j + 1]]
+                    p.radii = _f32(a["radii"][c0:c1])
+                    p.centers_w_H = _f32(a["cwh"][c0:c1])
+                else:
+                    ho = a["heavy_off"] if "heavy_off" in a else ao
+                    p.partial_charges = _f32(a["charges"][ho[j]:ho[j + 1]])
+            if sch["pharm"]:
+                p.pharm_types = a["pharm_types"][po[j]:po[j + 1]]
+                p.pharm_ancs = _f32(a["pharm_ancs"][po[j]:po[j + 1]])
+                p.pharm_vecs = _f32(a["pharm_vecs"][po[j]:po[j + 1]])
+            out.append(p)
+        return out
 
 
 def _rows(xyz: np.ndarray, w: Optional[np.ndarray] = None) -> np.ndarray:

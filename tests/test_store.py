@@ -241,3 +241,30 @@ def test_centring_on_load_matches_a_pre_centred_store(tmp_path):
     for k in ("atom_pos", "surf_pos", "pharm_ancs", "cwh"):
         np.testing.assert_allclose(c[k], arrays[True][k], atol=1e-5, err_msg=k)
     np.testing.assert_array_equal(c["pharm_vecs"], arrays[True]["pharm_vecs"])
+
+
+def test_reference_store_helpers(tmp_path):
+    """iter_shards / read_profiles / add_profile / center_to / from_molecule(modes=...) as in
+    screen.py:178-197, 420-424, 556-565."""
+    path = os.path.join(tmp_path, "lib.fss")
+    profs = [_profile(s) for s in range(5)]
+    with ProfileStore.create(path, num_surf_points=32, modes=("vol", "pharm"), dtype="float32",
+                             shard_size=2, pre_centered=False) as st:
+        for p in profs:
+            st.add_profile(p)
+        with pytest.raises(TypeError):
+            st.add_profile(object())
+    store = ProfileStore.open(path)
+    shards = list(store.iter_shards())
+    assert [len(s) for s in shards] == [2, 2, 1]
+    again = store.read_profiles(1)
+    np.testing.assert_array_equal(again[0].atom_pos, shards[1][0].atom_pos)
+    assert [p.id for s in shards for p in s] == [100 + s for s in range(5)]
+    q = MoleculeProfile.from_molecule(profs[0], modes=("pharm",))
+    assert q.surf_pos is None and q.partial_charges is None and q.pharm_types is not None
+    mu = q.atom_pos.mean(0)
+    anc = q.pharm_ancs.copy()
+    q.center_to(mu)
+    np.testing.assert_allclose(q.atom_pos.mean(0), 0.0, atol=1e-5)
+    np.testing.assert_allclose(q.pharm_ancs, anc - mu, atol=1e-6)
+    np.testing.assert_array_equal(q.pharm_vecs, profs[0].pharm_vecs)
