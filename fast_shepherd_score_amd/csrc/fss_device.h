@@ -374,9 +374,10 @@ __device__ __forceinline__ void shape_accum(const float4* __restrict__ ref, cons
 // LDS (per pair, T = ceil(N / 16) tiles, rows past N have zero coordinates and W = -1e30):
 //   amat[64 t + 16 g + r]     8 bf16: A-operand row r of tile t for K-group g
 //   xq[12 t + 3 g + k]        float4: 2 a~_k of ref rows 16 t + 4 g + 0..3 (k = x, y, z)
-// One wavefront owns one pose; a pass covers 32 fit columns (two B tiles; 16 in a last pass
-// with no more than 16 left) against all T ref tiles, and each lane folds its partial column
-// sums into the pose sums as the expanded form does per fit row.
+// One wavefront owns one pose; a pass covers 32 fit columns (two B tiles; the second tile's
+// MFMA is skipped in a last pass with no more than 16 left) against all T ref tiles, and each
+// lane folds the partial sums of one column into the pose sums as the expanded form does per
+// fit row.
 // ---------------------------------------------------------------------------------------------
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
@@ -439,55 +440,96 @@ __device__ __forceinline__ void mfma_tiles(const float4* __restrict__ am, const 
   s1[0] = v1; s1[1] = sx1; s1[2] = sy1; s1[3] = sz1;
 }
 
+// v_permlane16_swap on a register pair (a, b), 16-lane rows: a <- (a.row0, b.row0, a.row2, b.row2),
+// b <- (a.row1, b.row1, a.row3, b.row3).
+__device__ __forceinline__ void row_swap(unsigned& a, unsigned& b) {
+  const auto s = __builtin_amdgcn_permlane16_swap(a, b, false, false);
+  a = s[0];
+  b = s[1];
+}
+__device__ __forceinline__ void row_swap(float& a, float& b) {
+  unsigned ua = __float_as_uint(a), ub = __float_as_uint(b);
+  row_swap(ua, ub);
+  a = __uint_as_float(ua);
+  b = __uint_as_float(ub);
+}
+
+typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
+
+// B-operand K-group of one fit column: a coordinate (groups 0..2) or the ESP value with the
+// constant slots (group 3: 1 against W_a - D, the column shift -m_b against 1).
+__device__ __forceinline__ u32x4 mfma_b_coord(float x) {
+  __bf16 h, m, l;
+  split3(x, h, m, l);
+  bf16x8 b;
+  b[0] = h; b[1] = m; b[2] = h; b[3] = l; b[4] = h; b[5] = m; b[6] = l; b[7] = m;
+  return __builtin_bit_cast(u32x4, b);
+}
+__device__ __forceinline__ u32x4 mfma_b_last(float c, __bf16 mh, __bf16 mm) {
+  __bf16 h, m, l;
+  split3(c, h, m, l);
+  const __bf16 one = (__bf16)1.0f;
+  bf16x8 b;
+  b[0] = h; b[1] = m; b[2] = h; b[3] = one; b[4] = one; b[5] = one; b[6] = -mh; b[7] = -mm;
+  return __builtin_bit_cast(u32x4, b);
+}
+
+// One pass covers the 32 fit columns c0 .. c0 + 31 (B tiles 0 and 1).  Lane (g, r) transforms one
+// column, c0 + 16 (g & 1) + r, and builds its B operands for the K-groups of its row pair
+// {g & 2, (g & 2) + 1}: x and y in rows 0-1, z and the ESP row in rows 2-3.  One permlane16 swap
+// per operand dword then hands every lane its own K-group of both tiles (row 2i holds tile 0's
+// column, row 2i+1 tile 1's).  After the tile loop the same swap on the (tile 0, tile 1) partial
+// sums gives each lane its column's sums over its row pair, so each lane finishes its one column;
+// the pose-level butterfly adds the two row pairs.
 template <bool ESP>
 __device__ __forceinline__ void shape_accum_mfma(const float4* __restrict__ amat, const float4* __restrict__ xq,
                                                  int T, const float4* __restrict__ fit, int M,
                                                  const float R[9], const float ts[3], Acc& acc) {
   const int l = threadIdx.x & 63, g = l >> 4, r = l & 15;
-  const __bf16 one = (__bf16)1.0f;
+  const bool lo = g < 2;
   const float4* am = amat + 16 * g + r;
   const float4* xg = xq + 3 * g;
   for (int c0 = 0; c0 < M; c0 += 32) {
-    // a pass takes 32 fit columns (two B tiles), or 16 when no more than 16 remain
+    // the second tile's MFMA is skipped when no more than 16 columns remain
     const bool two = c0 + 16 < M;
-    float X[2], Y[2], Z[2], F[2];
-    float4 P[2];
-    bf16x8 Bop[2];
+    const int b = c0 + 16 * (g & 1) + r;
+    const bool ok = b < M;
+    const float4 q = fit[ok ? b : 0];
+    const float X = fmaf(R[0], q.x, fmaf(R[1], q.y, fmaf(R[2], q.z, ts[0])));
+    const float Y = fmaf(R[3], q.x, fmaf(R[4], q.y, fmaf(R[5], q.z, ts[1])));
+    const float Z = fmaf(R[6], q.x, fmaf(R[7], q.y, fmaf(R[8], q.z, ts[2])));
+    const float c = ESP ? q.w : 0.f;
+    const float h = fmaf(X, X, fmaf(Y, Y, fmaf(Z, Z, c * c)));
+    __bf16 mh, mm, ml;
+    split3(fmaxf(h - kHMax, 0.f), mh, mm, ml);
+    const float F = ok ? fast_exp2(kExpShift + ((float)mh + (float)mm) - h) : 0.f;
+    u32x4 B0 = mfma_b_coord(lo ? X : Z);
+    u32x4 B1 = lo ? mfma_b_coord(Y) : mfma_b_last(c, mh, mm);
 #pragma unroll
-    for (int k = 0; k < 2; ++k) {
-      const int b = c0 + 16 * k + r;
-      const bool ok = b < M;
-      P[k] = fit[ok ? b : 0];
-      const float4 q = P[k];
-      X[k] = fmaf(R[0], q.x, fmaf(R[1], q.y, fmaf(R[2], q.z, ts[0])));
-      Y[k] = fmaf(R[3], q.x, fmaf(R[4], q.y, fmaf(R[5], q.z, ts[1])));
-      Z[k] = fmaf(R[6], q.x, fmaf(R[7], q.y, fmaf(R[8], q.z, ts[2])));
-      const float c = ESP ? q.w : 0.f;
-      const float h = fmaf(X[k], X[k], fmaf(Y[k], Y[k], fmaf(Z[k], Z[k], c * c)));
-      __bf16 mh, mm, ml;
-      split3(fmaxf(h - kHMax, 0.f), mh, mm, ml);
-      F[k] = ok ? fast_exp2(kExpShift + ((float)mh + (float)mm) - h) : 0.f;
-      __bf16 H, Mi, L;
-      if (g < 3) {
-        split3(g == 0 ? X[k] : (g == 1 ? Y[k] : Z[k]), H, Mi, L);
-        Bop[k][0] = H; Bop[k][1] = Mi; Bop[k][2] = H; Bop[k][3] = L;
-        Bop[k][4] = H; Bop[k][5] = Mi; Bop[k][6] = L; Bop[k][7] = Mi;
-      } else {
-        split3(c, H, Mi, L);
-        Bop[k][0] = H; Bop[k][1] = Mi; Bop[k][2] = H; Bop[k][3] = one;
-        Bop[k][4] = one; Bop[k][5] = one; Bop[k][6] = -mh; Bop[k][7] = -mm;
-      }
+    for (int i = 0; i < 4; ++i) {
+      unsigned a = B0[i], e = B1[i];
+      row_swap(a, e);
+      B0[i] = a;
+      B1[i] = e;
     }
     float s0[4], s1[4];
-    if (two) mfma_tiles<true>(am, xg, T, Bop[0], Bop[1], s0, s1);
-    else mfma_tiles<false>(am, xg, T, Bop[0], Bop[1], s0, s1);
-    // The column finish f_b = F_b (s_b / 2 - b~'_b v_b) is linear in the row sums, so every lane
-    // folds its own row-group partials; the pose-level butterfly adds the four groups.  Column
-    // 1 of a one-tile pass has F = 0 and zero sums.
-    const float w0 = s0[0] * F[0], w1 = s1[0] * F[1];
-    accumulate_rows<true>(acc, P[0], w0, fmaf(0.5f * F[0], s0[1], -X[0] * w0), fmaf(0.5f * F[0], s0[2], -Y[0] * w0),
-                          fmaf(0.5f * F[0], s0[3], -Z[0] * w0), P[1], w1, fmaf(0.5f * F[1], s1[1], -X[1] * w1),
-                          fmaf(0.5f * F[1], s1[2], -Y[1] * w1), fmaf(0.5f * F[1], s1[3], -Z[1] * w1));
+    const bf16x8 Bt0 = __builtin_bit_cast(bf16x8, B0), Bt1 = __builtin_bit_cast(bf16x8, B1);
+    if (two) mfma_tiles<true>(am, xg, T, Bt0, Bt1, s0, s1);
+    else mfma_tiles<false>(am, xg, T, Bt0, Bt1, s0, s1);
+    // the column finish f_b = F_b (s_b / 2 - b~'_b v_b) is linear in the row sums; a column past
+    // M has F = 0, and a one-tile pass leaves zero tile-1 sums
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      row_swap(s0[i], s1[i]);
+      s0[i] += s1[i];
+    }
+    const float w = s0[0] * F, hf = 0.5f * F;
+    const float fx = fmaf(hf, s0[1], -X * w), fy = fmaf(hf, s0[2], -Y * w), fz = fmaf(hf, s0[3], -Z * w);
+    acc.V += w;
+    acc.gt[0] += fx; acc.gt[1] += fy; acc.gt[2] += fz;
+    acc.G[0] = fmaf(fx, q.x, acc.G[0]); acc.G[1] = fmaf(fx, q.y, acc.G[1]); acc.G[2] = fmaf(fx, q.z, acc.G[2]);
+    acc.G[3] = fmaf(fy, q.x, acc.G[3]); acc.G[4] = fmaf(fy, q.y, acc.G[4]); acc.G[5] = fmaf(fy, q.z, acc.G[5]);
+    acc.G[6] = fmaf(fz, q.x, acc.G[6]); acc.G[7] = fmaf(fz, q.y, acc.G[7]); acc.G[8] = fmaf(fz, q.z, acc.G[8]);
   }
 }
 
