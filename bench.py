@@ -401,7 +401,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: every box of this deployment has a single MI355X and the "
-              "pairs shard across GPUs without exchange (see DESIGN.md); run with --gpus 1.",
+              "pairs shard across GPUs without exchange; run with --gpus 1.",
               file=sys.stderr)
         sys.exit(2)
 

@@ -1,6 +1,6 @@
 """HIP kernels vs the FP64 oracle on identical seeded inputs (needs an MI355X).
 
-Tolerances (stated by the north star): per-pose scores within 1e-4 absolute; best-restart scores
+Tolerances: per-pose scores within 1e-4 absolute; best-restart scores
 within 1e-3 under identical restart seeds.  Gradients of the FP32 kernels: 1e-4 relative to the
 largest component.
 """
