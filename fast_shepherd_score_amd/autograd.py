@@ -43,9 +43,27 @@ def _ranges(P: int, N: int, dev) -> torch.Tensor:
     return r
 
 
+def _check_shapes(ref, fit, ref_w, fit_w):
+    """Raise ValueError unless ref is (P, N, 3), fit (P, M, 3) and the ESP values (P, N), (P, M).
+    The kernels index both clouds by the pair count and the row counts taken from these shapes,
+    so a mismatch would read past the packed buffers on the device."""
+    if ref.dim() != 3 or fit.dim() != 3 or ref.shape[-1] != 3 or fit.shape[-1] != 3:
+        raise ValueError(f"ref and fit must be (P, N, 3) and (P, M, 3), got {tuple(ref.shape)} and "
+                         f"{tuple(fit.shape)}")
+    if ref.shape[0] != fit.shape[0]:
+        raise ValueError(f"ref and fit hold {ref.shape[0]} and {fit.shape[0]} clouds: one fit cloud "
+                         "per ref cloud")
+    P, N, M = ref.shape[0], ref.shape[1], fit.shape[1]
+    if ref_w is not None and tuple(ref_w.shape) != (P, N):
+        raise ValueError(f"ref_esp must be ({P}, {N}), got {tuple(ref_w.shape)}")
+    if fit_w is not None and tuple(fit_w.shape) != (P, M):
+        raise ValueError(f"fit_esp must be ({P}, {M}), got {tuple(fit_w.shape)}")
+
+
 def _forward(ref, fit, ref_w, fit_w, alpha, lam):
     """fss_score_fwd[_f64] on (P, N, 3) / (P, M, 3) clouds: S (P,), ov (P, 4) = [VAB, VAA, VBB, 0]
     and what the backward needs."""
+    _check_shapes(ref, fit, ref_w, fit_w)
     if not (ref.is_cuda and fit.is_cuda):
         raise RuntimeError("gaussian_similarity runs on the GPU only (HIP kernels)")
     if ref.dtype != fit.dtype or ref.dtype not in (torch.float32, torch.float64):
@@ -76,6 +94,8 @@ def overlaps(ref: torch.Tensor, fit: torch.Tensor, ref_esp: Optional[torch.Tenso
              fit_esp: Optional[torch.Tensor] = None, alpha: float = 0.81,
              lam: float = 0.3 * LAM_SCALING) -> torch.Tensor:
     """(P, 3) [VAB, VAA, VBB] of each cloud pair, no gradient (the forward kernel's by-product)."""
+    if (ref_esp is None) != (fit_esp is None):
+        raise ValueError("pass ESP values for both clouds or for neither")
     with torch.no_grad():
         return _forward(ref, fit, ref_esp, fit_esp, alpha, lam)[1][:, :3]
 
@@ -124,6 +144,7 @@ def gaussian_similarity(ref: torch.Tensor, fit: torch.Tensor, ref_esp: Optional[
     """
     if (ref_esp is None) != (fit_esp is None):
         raise ValueError("pass ESP values for both clouds or for neither")
+    _check_shapes(ref, fit, ref_esp, fit_esp)
     return _GaussianSimilarity.apply(ref, fit, ref_esp, fit_esp, alpha, lam)
 
 

@@ -378,6 +378,33 @@ int admit(FssScreen* s, int slot) {
   return FSS_OK;
 }
 
+// Hands every slot of `act` whose library range is exhausted its result (top-k copied to the
+// host) and frees the slot.  A query admitted against an empty library finishes here with no
+// launch and no hits.
+int complete_finished(FssScreen* s, const std::vector<int>& act, clk::time_point now) {
+  for (int i : act) {
+    Slot& sl = s->slots[i];
+    if (!sl.used || sl.next < sl.lib_n) continue;
+    Result r;
+    const size_t cur = ((size_t)i * 2 + sl.buf) * s->K;
+    r.score.resize(sl.filled); r.index.resize(sl.filled); r.tf.resize(16 * (size_t)sl.filled);
+    if (sl.filled) {
+      hipError_t e = hipMemcpy(r.score.data(), s->tk_s + cur, sizeof(double) * sl.filled, hipMemcpyDeviceToHost);
+      if (e == hipSuccess) e = hipMemcpy(r.index.data(), s->tk_i + cur, sizeof(int64_t) * sl.filled, hipMemcpyDeviceToHost);
+      if (e == hipSuccess) e = hipMemcpy(r.tf.data(), s->tk_tf + 16 * cur, sizeof(float) * 16 * sl.filled, hipMemcpyDeviceToHost);
+      if (e != hipSuccess) return hip_fail(e, "hit download");
+    }
+    r.queue_ms = ms_since(sl.q.t_submit, sl.t_start);
+    r.latency_ms = ms_since(sl.q.t_submit, now);
+    r.all.swap(sl.all);
+    const int64_t ticket = sl.q.ticket;
+    sl.used = false;
+    sl.q = QueryIn();
+    finish(s, ticket, std::move(r));
+  }
+  return FSS_OK;
+}
+
 // One launch over the admitted slots `act`.
 int launch(FssScreen* s, const std::vector<int>& act) {
   const int na = (int)act.size();
@@ -422,7 +449,7 @@ int launch(FssScreen* s, const std::vector<int>& act) {
       }
     }
   }
-  if (p == 0) return FSS_OK;
+  if (p == 0) return complete_finished(s, act, clk::now());
   hipError_t e = hipMemcpyAsync(s->d_rng, s->h_rng, sizeof(int32_t) * 2 * kRngTables * (size_t)s->P,
                                 hipMemcpyHostToDevice, s->stream);
   if (e == hipSuccess) e = hipMemcpyAsync(s->d_jobs, s->h_jobs, sizeof(MergeJob) * nj, hipMemcpyHostToDevice, s->stream);
@@ -489,27 +516,7 @@ int launch(FssScreen* s, const std::vector<int>& act) {
     sl.filled = (int32_t)std::min<int64_t>(s->K, (int64_t)sl.filled + take[i]);
     sl.buf ^= 1;
   }
-  for (int i = 0; i < na; ++i) {
-    Slot& sl = s->slots[act[i]];
-    if (sl.next < sl.lib_n) continue;
-    Result r;
-    const size_t cur = ((size_t)act[i] * 2 + sl.buf) * s->K;
-    r.score.resize(sl.filled); r.index.resize(sl.filled); r.tf.resize(16 * (size_t)sl.filled);
-    if (sl.filled) {
-      e = hipMemcpy(r.score.data(), s->tk_s + cur, sizeof(double) * sl.filled, hipMemcpyDeviceToHost);
-      if (e == hipSuccess) e = hipMemcpy(r.index.data(), s->tk_i + cur, sizeof(int64_t) * sl.filled, hipMemcpyDeviceToHost);
-      if (e == hipSuccess) e = hipMemcpy(r.tf.data(), s->tk_tf + 16 * cur, sizeof(float) * 16 * sl.filled, hipMemcpyDeviceToHost);
-      if (e != hipSuccess) return hip_fail(e, "hit download");
-    }
-    r.queue_ms = ms_since(sl.q.t_submit, sl.t_start);
-    r.latency_ms = ms_since(sl.q.t_submit, now);
-    r.all.swap(sl.all);
-    const int64_t ticket = sl.q.ticket;
-    sl.used = false;
-    sl.q = QueryIn();
-    finish(s, ticket, std::move(r));
-  }
-  return FSS_OK;
+  return complete_finished(s, act, now);
 }
 
 void fail_active(FssScreen* s, int rc, const std::string& msg) {

@@ -48,6 +48,21 @@ def test_screen_scores_match_oracle(dev):
         assert abs(got[i] - o["score"]) < 1e-3, (i, got[i], o["score"])
 
 
+def test_empty_library_returns_no_hits(dev):
+    """A query against a library with no conformers finishes (no launch) with zero hits, alone
+    and beside a query on the same service."""
+    print("case screen on an empty library", flush=True)
+    empty = ConformerLibrary.from_arrays(points=[], device=dev)
+    assert empty.n == 0
+    query = {"points": synth.random_walks(np.random.default_rng(5), [12])[0]}
+    assert screen(query, empty, "vol", top_k=5, num_repeats=4, max_num_steps=10) == []
+    with ScreenService(empty, "vol", num_repeats=4, max_num_steps=10, top_k=5) as svc:
+        res = [svc.submit(query).result(timeout=60) for _ in range(2)]
+        assert svc.launches == 0
+    for r in res:
+        assert r.hits == [] and r.pairs == 0
+
+
 def test_alone_equals_in_mix(dev):
     mols, lib = _library(2000, 3, dev)
     queries = [{"points": mols[i]} for i in (1, 50, 999)]

@@ -282,6 +282,29 @@ def test_score_module_host_checks():
         score.get_overlap_pharm(np.zeros(2), np.zeros(2), a, a, a, a)
 
 
+def test_score_as_loss_rejects_mismatched_shapes_before_any_launch():
+    """gaussian_similarity / overlaps check the cloud and ESP shapes before packing: CPU tensors
+    reach the shape check first, so each mismatch raises ValueError with no device involved."""
+    from fast_shepherd_score_amd import autograd as ag
+    ref, fit = torch.zeros((3, 5, 3)), torch.zeros((3, 7, 3))
+    bad = [
+        (dict(ref=ref, fit=torch.zeros((2, 7, 3))), "one fit cloud per ref cloud"),
+        (dict(ref=ref, fit=torch.zeros((4, 7, 3))), "one fit cloud per ref cloud"),
+        (dict(ref=torch.zeros((5, 3)), fit=fit), r"\(P, N, 3\)"),
+        (dict(ref=ref, fit=torch.zeros((3, 7, 4))), r"\(P, N, 3\)"),
+        (dict(ref=ref, fit=fit, ref_esp=torch.zeros((3, 4)), fit_esp=torch.zeros((3, 7))), "ref_esp"),
+        (dict(ref=ref, fit=fit, ref_esp=torch.zeros((3, 5)), fit_esp=torch.zeros((2, 7))), "fit_esp"),
+        (dict(ref=ref, fit=fit, ref_esp=torch.zeros((3, 5))), "both clouds"),
+    ]
+    for kw, msg in bad:
+        for fn in (ag.gaussian_similarity, ag.overlaps):
+            with pytest.raises(ValueError, match=msg):
+                fn(**kw)
+    # well-shaped CPU inputs pass the shape check and stop at the device check
+    with pytest.raises(RuntimeError, match="GPU only"):
+        ag.gaussian_similarity(ref, fit, torch.zeros((3, 5)), torch.zeros((3, 7)))
+
+
 # Parameter names of the reference's MoleculePairBatch methods (container/_batch.py), in order;
 # the HIP container may take further keywords after them.
 BATCH_API = {
