@@ -94,6 +94,13 @@ class AlignResult:
     extra: dict = field(default_factory=dict)
 
 
+def es_start(mode: int) -> float:
+    """The early-stop reference loss before the first step: 1.0 for the analytic optimisers
+    (alignment/_torch_analytical.py:110, 283, 420), +inf for the ShaEP optimiser
+    (alignment/_torch.py:922), so its first step never counts towards the patience."""
+    return float("inf") if mode == _lib.MODE_ESP_COMBO else 1.0
+
+
 def _stream(stream=None) -> int:
     s = stream if stream is not None else torch.cuda.current_stream()
     return int(s.cuda_stream)
@@ -306,7 +313,7 @@ class SteppedAligner:
         self.grad = torch.zeros((P * B, 7), dtype=torch.float32, device=dev)
         self.m = torch.zeros((P * B, 7), dtype=torch.float32, device=dev)
         self.v = torch.zeros((P * B, 7), dtype=torch.float32, device=dev)
-        self.last = torch.ones(P, dtype=torch.float64, device=dev)
+        self.last = torch.full((P,), es_start(spec.mode), dtype=torch.float64, device=dev)
         self.counter = torch.zeros(P, dtype=torch.int32, device=dev)
         self.active = torch.ones(P, dtype=torch.int32, device=dev)
         self.n_steps = torch.zeros(P, dtype=torch.int32, device=dev)
@@ -395,7 +402,7 @@ def align_plain(pb: PairBatch, spec: ScoreSpec, q0: torch.Tensor, t0: torch.Tens
         qp, tp = q[p], t[p]
         m.zero_()
         v.zero_()
-        last, counter, done = 1.0, 0, opt.max_steps
+        last, counter, done = es_start(spec.mode), 0, opt.max_steps
         for k in range(1, opt.max_steps + 1):
             _lib.call("fss_pose_loss_grad", _lib.byref(bs), _lib.byref(sp), B, _lib.ptr(pose_pair),
                       B, _lib.ptr(self_ov), _lib.ptr(qp), _lib.ptr(tp), _lib.ptr(loss),

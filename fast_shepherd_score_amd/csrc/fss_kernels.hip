@@ -709,7 +709,9 @@ void k_align(FssPairBatch pb, KScore ks, LdsPlan lp, AlignK ak) {
   }
   __syncthreads();
 
-  double b1p = 1.0, b2p = 1.0, last = 1.0;
+  // early-stop reference loss before the first step: 1.0 for the analytic optimisers
+  // (_torch_analytical.py:110, 283, 420), +inf for the ShaEP one (alignment/_torch.py:922)
+  double b1p = 1.0, b2p = 1.0, last = SE ? (double)INFINITY : 1.0;
   int counter = 0, done = ak.max_steps;
   for (int step = 1; step <= ak.max_steps; ++step) {
     float loss = 0.f;

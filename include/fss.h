@@ -214,7 +214,9 @@ int fss_adam_qt(int32_t n, float* q, float* t, const float* dQ, const float* dT,
  * Early-stop bookkeeping of a stepped optimiser, on device (no host synchronisation):
  * L = mean of the pair's n_restarts losses; counter = |L - last| > tol ? 0 : counter + 1;
  * last = L; n_steps = step; once counter > patience the pair's active flag drops to 0.
- * last (n_pairs) float64 starts at 1.0; counter, n_steps start at 0; active starts at 1.
+ * last (n_pairs) float64 starts at 1.0 (+inf for FSS_MODE_ESP_COMBO, whose reference optimiser
+ * starts there: alignment/_torch.py:922); counter, n_steps start at 0; active starts at 1.
+ * fss_align applies the same two starting values itself.
  * Replaces: the per-step loss.item() test of alignment/_torch_analytical.py:132-139.
  */
 int fss_early_stop_update(int32_t n_pairs, int32_t n_restarts, const float* loss, double* last,

@@ -555,7 +555,10 @@ def optimize(pair: Pair, q0, t0, lr: float = 0.1, max_steps: int = 200, early_st
     se3.requires_grad_(True)
     so = pair.self_overlaps()
     opt = torch.optim.Adam([se3], lr=lr)
-    last, counter, n_steps = 1.0, 0, max_steps
+    # the analytic optimisers start the early-stop comparison at 1.0
+    # (_torch_analytical.py:110, 283, 420), the ShaEP optimiser at +inf (alignment/_torch.py:922)
+    last = math.inf if pair.mode == "esp_combo" else 1.0
+    counter, n_steps = 0, max_steps
     for step in range(max_steps):
         with torch.no_grad():
             loss, grad, _ = pair.loss_grad_se3(se3, so)

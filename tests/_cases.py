@@ -1,4 +1,6 @@
 """Shared synthetic cases: the same seeded inputs as a device PairBatch and as oracle Pairs."""
+import os
+
 import numpy as np
 import torch
 
@@ -58,3 +60,72 @@ def random_seeds(n_pairs, B, seed=0, t_scale=0.0):
 
 def to_dev(a, dev):
     return torch.from_numpy(np.ascontiguousarray(a)).to(dev)
+
+
+# ---------------------------------------------------------------------------------------------
+# Early-stop pins.  In each case every restart keeps one exact loss at every step and an exactly
+# zero gradient (Adam then leaves the pose unchanged), so the rule alone fixes the step count:
+# stop after the first step at which |L_k - L_(k-1)| <= 1e-5 has held more than 10 times running,
+# with L_(-1) = 1.0 (analytic optimisers, _torch_analytical.py:110, 132-139) or +inf (ShaEP,
+# alignment/_torch.py:922, 967-974).
+#   * "optimum": a one-atom cloud aligned with its copy at the identity: L = 0 every step; the
+#     first step resets the counter, steps 2..12 count 1..11, so 12 steps run.
+#   * "apart": clouds 150 A apart: L = 1 every step; from L_(-1) = 1 step 1 already counts, so 11
+#     steps run.  ">=" in place of ">" would give 11 and 10.
+#   * "shaep_apart": the ShaEP score of molecules 150 A apart with surface ESP 10 and zero charges
+#     (esp terms exp(-100 / lam) vanish, and zero charges make the ESP gradient exactly 0): L = 1,
+#     and from L_(-1) = +inf the first step resets, so 12 steps run (11 from 1.0).
+ES_EXPECTED = {"optimum": 12, "apart": 11, "shaep_apart": 12}
+ES_SHIFT = np.array([150.0, 0.0, 0.0])
+
+
+def es_shape_clouds(name):
+    """(ref, fit) float64 clouds of the shape cases."""
+    if name == "optimum":
+        return np.zeros((1, 3)), np.zeros((1, 3))
+    ref = synth.random_walks(np.random.default_rng(3), [6])[0].astype(np.float64)
+    fit = synth.random_walks(np.random.default_rng(4), [5])[0].astype(np.float64) + ES_SHIFT
+    return ref, fit
+
+
+def es_shaep_molecules():
+    """(ref, fit) ShaEP molecule dicts of the "shaep_apart" case."""
+    rng = np.random.default_rng(6)
+    a, b = synth.shaep_molecule(rng, 6, 30), synth.shaep_molecule(rng, 5, 24)
+    for m in (a, b):
+        m["surf_esp"] = np.full(len(m["surf_pos"]), 10.0, np.float32)
+        m["partial_charges"] = np.zeros(len(m["centers_w_H"]), np.float32)
+    for k in ("atom_pos", "surf_pos", "centers_w_H"):
+        b[k] = (b[k].astype(np.float64) + ES_SHIFT).astype(np.float32)
+    return a, b
+
+
+def es_seeds(name, B):
+    """Identity seeds for "optimum" (the pose must stay the exact optimum), random rotations
+    about the origin with zero translation otherwise."""
+    if name == "optimum":
+        q = np.tile(np.array([1.0, 0.0, 0.0, 0.0], np.float32), (1, B, 1))
+    else:
+        q = synth.random_quaternions(np.random.default_rng(7), B).reshape(1, B, 4).astype(np.float32)
+    return q, np.zeros((1, B, 3), np.float32)
+
+
+# ---------------------------------------------------------------------------------------------
+# Config-sized oracle fixtures (tests/golden/make_config_vectors.py)
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def load_config(name):
+    """A config_<name>.npz fixture as a dict, ragged arrays split per pair."""
+    z = np.load(os.path.join(GOLD, f"config_{name}.npz"))
+    split = lambda a, n: np.split(z[a], np.cumsum(z[n])[:-1])
+    d = {k: z[k] for k in z.files}
+    d["kind"] = str(z["kind"])
+    d["ref"], d["fit"] = split("ref_xyz", "ref_n"), split("fit_xyz", "fit_n")
+    if d["kind"] == "esp":
+        d["ref_w"], d["fit_w"] = split("ref_w", "ref_n"), split("fit_w", "fit_n")
+    if d["kind"] == "shape_pharm":
+        for side in ("ref", "fit"):
+            for k in ("types", "anc", "vec"):
+                d[f"{side}_{k}"] = split(f"{side}_{k}", f"{side}_pn")
+    return d

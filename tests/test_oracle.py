@@ -420,3 +420,34 @@ def test_combo_optimiser_recovers_a_rigid_copy():
     o = orc.optimize(pair, q0, t0, lr=0.02, max_steps=60, early_stop=False)
     assert o["score"] > 0.999
     assert o["scores"][1] > 0.99
+
+
+# ---- early-stop rule: closed-form step counts ---------------------------------------------
+@pytest.mark.parametrize("name", ["optimum", "apart", "shaep_apart"])
+def test_early_stop_step_counts_closed_form(name):
+    """The oracle's early stop on losses held exactly constant with exactly zero gradients: the
+    step count is fixed by the rule alone (12 from the optimum, 11 from L = 1 with the analytic
+    optimisers' start value 1.0, 12 for the ShaEP optimiser's +inf start).  A '>=' patience test
+    or the wrong start value gives a different count."""
+    from tests._cases import ES_EXPECTED, es_seeds, es_shape_clouds, es_shaep_molecules
+    B = 3
+    q0, t0 = es_seeds(name, B)
+    if name == "shaep_apart":
+        a, b = es_shaep_molecules()
+        pair = orc.ComboPair(ref_centers=a["atom_pos"], fit_centers=b["atom_pos"], ref_surf=a["surf_pos"],
+                             fit_surf=b["surf_pos"], ref_surf_esp=a["surf_esp"], fit_surf_esp=b["surf_esp"],
+                             ref_atoms=a["centers_w_H"], fit_atoms=b["centers_w_H"],
+                             ref_charges=a["partial_charges"], fit_charges=b["partial_charges"],
+                             ref_radii=a["radii"], fit_radii=b["radii"])
+    else:
+        ref, fit = es_shape_clouds(name)
+        pair = orc.Pair("shape", ref=ref, fit=fit)
+    with torch.no_grad():
+        _, g, per = pair.loss_grad_se3(torch.as_tensor(np.concatenate([q0[0], t0[0]], 1), dtype=DT),
+                                       pair.self_overlaps())
+    assert float(g.abs().max()) == 0.0                   # no pose moves
+    assert torch.all(per == (0.0 if name == "optimum" else 1.0))
+    out = orc.optimize(pair, q0[0], t0[0], max_steps=50)
+    assert out["n_steps"] == ES_EXPECTED[name]
+    np.testing.assert_array_equal(out["se3"], np.concatenate([q0[0], t0[0]], 1))
+    assert orc.optimize(pair, q0[0], t0[0], max_steps=50, es_patience=9)["n_steps"] == ES_EXPECTED[name] - 1
