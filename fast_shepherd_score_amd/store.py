@@ -131,109 +131,215 @@ def _offsets(lengths) -> np.ndarray:
     return np.concatenate([[0], np.cumsum(np.asarray(lengths, np.int64))]).astype(np.int64)
 
 
-class ProfileStore:
-    """Sharded on-disk store of conformer arrays; write once (create, add, close), then open."""
+# Columns of one conformer as a version-1 shard stores them.  A shard concatenates each ragged
+# column behind a CSR offsets array and stacks the fixed-size surface columns:
+#   atom_off   atom_pos, and nonH (the heavy-atom indices) in with-H stores
+#   all_off    charges, radii, cwh (every atom)          with-H stores
+#   heavy_off  xyz_noH                                   when a conformer keeps separate heavy centres
+#   pharm_off  pharm_types (int32), pharm_ancs, pharm_vecs
+#   (stacked)  surf_pos (n, S, 3), surf_esp (n, S)
+# Without hydrogens, `charges` holds the heavy-atom charges and follows heavy_off when it exists,
+# atom_off otherwise.  Coordinates and charges take the store's dtype; offsets and nonH are int64.
+_POSITIONS = ("atom_pos", "surf_pos", "pharm_ancs", "cwh", "xyz_noH")
 
-    def __init__(self, path: str, manifest: dict, mode: str):
-        self.path, self.manifest, self._mode = path, manifest, mode
-        self._buf: List[MoleculeProfile] = []
-        self._n = int(manifest.get("n_total", 0))
-        self._shard_id = len(manifest.get("shards", []))
+
+def _shard_file(k: int) -> str:
+    return f"shard_{k:05d}.npz"
+
+
+_SHARD_FILE = re.compile(r"shard_\d{5}\.npz")
+
+
+def _encode_shard(ids: list, cols: List[dict], schema: dict, dtype) -> dict:
+    """The npz arrays of one shard from its conformers' column dicts."""
+    def cat(key, dt=dtype):
+        return np.concatenate([c[key] for c in cols]).astype(dt)
+
+    def off(key):
+        return _offsets([len(c[key]) for c in cols])
+
+    arrays = {"ids": np.asarray(ids), "atom_off": off("atom_pos"), "atom_pos": cat("atom_pos")}
+    for flag, key in (("surf", "surf_pos"), ("surf_esp", "surf_esp")):
+        if schema[flag]:
+            arrays[key] = np.stack([c[key] for c in cols]).astype(dtype)
+    if schema["charges"]:
+        arrays["charges"] = cat("charges")
+        if schema["with_H"]:
+            arrays["all_off"] = off("charges")
+            arrays["nonH"] = cat("nonH", np.int64)
+            arrays["radii"], arrays["cwh"] = cat("radii"), cat("cwh")
+        if any("xyz_noH" in c for c in cols):
+            heavy = [c.get("xyz_noH", c["atom_pos"]) for c in cols]
+            arrays["heavy_off"] = _offsets([len(c["nonH"] if schema["with_H"] else c["charges"])
+                                            for c in cols])
+            arrays["xyz_noH"] = np.concatenate(heavy).astype(dtype)
+    if schema["pharm"]:
+        arrays["pharm_off"] = off("pharm_types")
+        arrays["pharm_types"] = cat("pharm_types", np.int32)
+        arrays["pharm_ancs"], arrays["pharm_vecs"] = cat("pharm_ancs"), cat("pharm_vecs")
+    return arrays
+
+
+def _decode_shard(a: dict, n: int, schema: dict) -> List[MoleculeProfile]:
+    """The MoleculeProfile list of one shard's arrays (float32 in memory)."""
+    def spans(key):
+        o = a[key]
+        return [slice(int(o[j]), int(o[j + 1])) for j in range(n)]
+
+    atoms = spans("atom_off")
+    every = spans("all_off") if schema["charges"] and schema["with_H"] else None
+    heavy = spans("heavy_off") if "heavy_off" in a else atoms
+    feats = spans("pharm_off") if schema["pharm"] else None
+    out = []
+    for j in range(n):
+        p = MoleculeProfile(atom_pos=_f32(a["atom_pos"][atoms[j]]), id=a["ids"][j].item())
+        if schema["surf"]:
+            p.surf_pos = _f32(a["surf_pos"][j])
+        if schema["surf_esp"]:
+            p.surf_esp = _f32(a["surf_esp"][j])
+        if every is not None:
+            e = every[j]
+            p.partial_charges, p.radii, p.centers_w_H = (_f32(a[k][e]) for k in ("charges", "radii", "cwh"))
+            p.nonH_atoms_idx = a["nonH"][atoms[j]]
+        elif schema["charges"]:
+            p.partial_charges = _f32(a["charges"][heavy[j]])
+        if feats is not None:
+            f = feats[j]
+            p.pharm_types = a["pharm_types"][f]
+            p.pharm_ancs, p.pharm_vecs = _f32(a["pharm_ancs"][f]), _f32(a["pharm_vecs"][f])
+        out.append(p)
+    return out
+
+
+class ProfileStore:
+    """A sharded on-disk conformer store in the version-1 interchange format.
+
+    Written once -- ``create``, ``add`` / ``add_profile``, ``close`` (or a ``with`` block) -- and
+    read through ``open``.  Conformers wait in column form until a shard's worth
+    (``shard_size``) is buffered; each shard write is followed by an atomic manifest update, so
+    an interrupted writer leaves a readable store of the shards already written.
+    """
+
+    def __init__(self, path: str, manifest: dict, writable: bool):
+        self.path = path
+        self.manifest = manifest
+        self._writable = writable
+        self._pending: List[dict] = []        # column dicts not yet in a shard
+        self._pending_ids: list = []
 
     # -- writing ----------------------------------------------------------------------------
     @classmethod
     def create(cls, path: str, *, num_surf_points: int, modes: Sequence[str],
                dtype: str = "float16", shard_size: int = 100_000, pre_centered: bool = True,
                overwrite: bool = False) -> "ProfileStore":
+        """A new empty store at ``path`` for ``modes``; ``overwrite`` replaces an existing
+        store there (its manifest and shard files only; other files in the folder stay)."""
         if dtype not in ("float16", "float32"):
-            raise ValueError("dtype must be 'float16' or 'float32'")
-        if shard_size < 1:
-            raise ValueError("shard_size must be >= 1")
+            raise ValueError(f"unsupported store dtype {dtype!r}: use 'float16' or 'float32'")
+        if int(shard_size) < 1:
+            raise ValueError(f"shard_size must be positive, got {shard_size}")
         schema = schema_from_modes(modes)
         os.makedirs(path, exist_ok=True)
-        mpath = os.path.join(path, MANIFEST)
-        if os.path.exists(mpath) and not overwrite:
-            raise FileExistsError(f"{mpath} already exists; pass overwrite=True to replace it")
-        if overwrite:      # only this store's own files
+        if os.path.exists(os.path.join(path, MANIFEST)):
+            if not overwrite:
+                raise FileExistsError(f"a store already exists at {path!r} (overwrite=True replaces it)")
             for f in os.listdir(path):
-                if f == MANIFEST or re.fullmatch(r"shard_\d{5}\.npz", f):
+                if f == MANIFEST or _SHARD_FILE.fullmatch(f):
                     os.remove(os.path.join(path, f))
-        manifest = dict(version=FORMAT_VERSION, num_surf_points=int(num_surf_points),
-                        modes=list(modes), schema=schema, dtype=dtype, shard_size=int(shard_size),
-                        pre_centered=bool(pre_centered), n_total=0, shards=[])
-        store = cls(path, manifest, "w")
-        store._write_manifest()
+        store = cls(path, {"version": FORMAT_VERSION, "num_surf_points": int(num_surf_points),
+                           "modes": list(modes), "schema": schema, "dtype": dtype,
+                           "shard_size": int(shard_size), "pre_centered": bool(pre_centered),
+                           "n_total": 0, "shards": []}, writable=True)
+        store._save_manifest()
         return store
 
     def add(self, molecule, id=None) -> None:
-        """Buffer one conformer (a MoleculeProfile or a container Molecule); a shard is written
-        every ``shard_size`` conformers.  ``id`` defaults to the position in the store."""
-        if self._mode != "w":
-            raise RuntimeError("store is open for reading")
+        """Buffer one conformer (a MoleculeProfile, or anything with the container Molecule's
+        attributes); ``id`` defaults to the profile's own, else the conformer's position."""
+        if not self._writable:
+            raise RuntimeError(f"the store at {self.path!r} was opened for reading; create() a new one to write")
         prof = molecule if isinstance(molecule, MoleculeProfile) else MoleculeProfile.from_molecule(molecule)
         if id is None:
-            id = prof.id if prof.id is not None else self._n + len(self._buf)
-        self._buf.append(self._checked(prof, id))
-        if len(self._buf) >= self.manifest["shard_size"]:
-            self._flush()
+            id = prof.id if prof.id is not None else self.manifest["n_total"] + len(self._pending)
+        self._pending.append(self._columns(prof, id))
+        self._pending_ids.append(id)
+        if len(self._pending) == self.manifest["shard_size"]:
+            self._write_shard()
 
     def add_profile(self, profile: "MoleculeProfile", id=None) -> None:
-        """Add a MoleculeProfile as it is (screen.py:420-424); ``id`` defaults to its own."""
+        """Add a MoleculeProfile unchanged (screen.py:420-424); ``id`` defaults to its own."""
         if not isinstance(profile, MoleculeProfile):
-            raise TypeError("add_profile expects a MoleculeProfile")
-        self.add(profile, id=id if id is not None else profile.id)
+            raise TypeError(f"add_profile takes a MoleculeProfile, not {type(profile).__name__}")
+        self.add(profile, id=id)
 
-    def _checked(self, p: MoleculeProfile, id) -> MoleculeProfile:
-        sch = self.manifest["schema"]
-        atom_pos = _f32(p.atom_pos).reshape(-1, 3)
-        q = MoleculeProfile(atom_pos=atom_pos, id=id)
-        if sch["surf"]:
-            if p.surf_pos is None:
-                raise ValueError("the store keeps surfaces but the conformer has none")
-            q.surf_pos = _f32(p.surf_pos).reshape(-1, 3)
-            if len(q.surf_pos) != self.manifest["num_surf_points"]:
-                raise ValueError(f"surface has {len(q.surf_pos)} points, the store takes "
-                                 f"{self.manifest['num_surf_points']}")
-        if sch["surf_esp"]:
-            if p.surf_esp is None:
-                raise ValueError("the store keeps surface ESP but the conformer has none")
-            q.surf_esp = _f32(p.surf_esp).reshape(-1)
-        if sch["charges"]:
-            if p.partial_charges is None:
-                raise ValueError("the store keeps partial charges but the conformer has none")
-            ch = _f32(p.partial_charges).reshape(-1)
-            if sch["with_H"]:
-                if p.nonH_atoms_idx is None or p.radii is None or p.centers_w_H is None:
-                    raise ValueError("a with-H store needs nonH_atoms_idx, radii and centers_w_H")
-                q.partial_charges = ch
-                q.nonH_atoms_idx = np.asarray(p.nonH_atoms_idx, np.int64)
-                q.radii = _f32(p.radii).reshape(-1)
-                q.centers_w_H = _f32(p.centers_w_H).reshape(-1, 3)
+    def _columns(self, p: MoleculeProfile, ident) -> dict:
+        """Validate a profile against the schema and return its shard columns (float32 here;
+        the store's dtype is applied when the shard is written)."""
+        schema, S = self.schema, self.num_surf_points
+
+        def required(attr, what):
+            v = getattr(p, attr)
+            if v is None:
+                raise ValueError(f"conformer {ident!r} has no {attr}, which this store keeps ({what})")
+            return v
+
+        xyz = _f32(p.atom_pos).reshape(-1, 3)
+        col = {"atom_pos": xyz}
+        if schema["surf"]:
+            col["surf_pos"] = _f32(required("surf_pos", "surfaces")).reshape(-1, 3)
+            if len(col["surf_pos"]) != S:
+                raise ValueError(f"conformer {ident!r} has {len(col['surf_pos'])} surface points; "
+                                 f"the store holds {S} points per surface")
+        if schema["surf_esp"]:
+            col["surf_esp"] = _f32(required("surf_esp", "surface ESP")).reshape(-1)
+        if schema["charges"]:
+            q = _f32(required("partial_charges", "partial charges")).reshape(-1)
+            if schema["with_H"]:
+                col["charges"] = q
+                col["nonH"] = np.asarray(required("nonH_atoms_idx", "with-H rows"), np.int64).reshape(-1)
+                col["radii"] = _f32(required("radii", "with-H rows")).reshape(-1)
+                col["cwh"] = _f32(required("centers_w_H", "with-H rows")).reshape(-1, 3)
             else:
-                idx = np.asarray(p.nonH_atoms_idx, np.int64) if p.nonH_atoms_idx is not None \
-                    else np.arange(len(ch))
-                q.partial_charges = ch[idx]
+                pick = p.nonH_atoms_idx
+                col["charges"] = q if pick is None else q[np.asarray(pick, np.int64)]
             if p.atom_pos_noH is not None:
-                q.atom_pos_noH = _f32(p.atom_pos_noH).reshape(-1, 3)
-        if sch["pharm"]:
-            if p.pharm_types is None:
-                raise ValueError("the store keeps pharmacophores but the conformer has none")
-            q.pharm_types = np.asarray(p.pharm_types, np.int32).reshape(-1)
-            q.pharm_ancs = _f32(p.pharm_ancs).reshape(-1, 3)
-            q.pharm_vecs = _f32(p.pharm_vecs).reshape(-1, 3)
-        if self.manifest["pre_centered"]:
-            mu = atom_pos.mean(0)
-            q.atom_pos = q.atom_pos - mu
-            for k in ("surf_pos", "pharm_ancs", "centers_w_H", "atom_pos_noH"):
-                v = getattr(q, k)
-                if v is not None:
-                    setattr(q, k, v - mu)
-        return q
+                col["xyz_noH"] = _f32(p.atom_pos_noH).reshape(-1, 3)
+        if schema["pharm"]:
+            col["pharm_types"] = np.asarray(required("pharm_types", "pharmacophores"), np.int32).reshape(-1)
+            col["pharm_ancs"] = _f32(p.pharm_ancs).reshape(-1, 3)
+            col["pharm_vecs"] = _f32(p.pharm_vecs).reshape(-1, 3)
+        if self.pre_centered:          # shift every position column by the heavy-atom mean
+            mu = xyz.mean(0)
+            for key in _POSITIONS:
+                if key in col:
+                    col[key] = col[key] - mu
+        return col
+
+    def _write_shard(self) -> None:
+        if not self._pending:
+            return
+        shards = self.manifest["shards"]
+        name = _shard_file(len(shards))
+        dt = np.float16 if self.manifest["dtype"] == "float16" else np.float32
+        np.savez(os.path.join(self.path, name),
+                 **_encode_shard(self._pending_ids, self._pending, self.schema, dt))
+        n = len(self._pending)
+        shards.append({"name": name, "n": n, "start": self.manifest["n_total"]})
+        self.manifest["n_total"] += n
+        self._pending, self._pending_ids = [], []
+        self._save_manifest()
+
+    def _save_manifest(self) -> None:
+        part = os.path.join(self.path, MANIFEST + ".tmp")
+        with open(part, "w") as fh:
+            json.dump(self.manifest, fh, indent=2)
+        os.replace(part, os.path.join(self.path, MANIFEST))
 
     def close(self) -> None:
-        if self._mode == "w":
-            self._flush()
-            self._write_manifest()
+        """Write the buffered conformers as a last (short) shard; reading stores ignore it."""
+        if self._writable:
+            self._write_shard()
+            self._save_manifest()
 
     def __enter__(self):
         return self
@@ -241,63 +347,16 @@ class ProfileStore:
     def __exit__(self, *exc):
         self.close()
 
-    def _write_manifest(self) -> None:
-        tmp = os.path.join(self.path, MANIFEST + ".tmp")
-        with open(tmp, "w") as fh:
-            json.dump(self.manifest, fh, indent=2)
-        os.replace(tmp, os.path.join(self.path, MANIFEST))
-
-    def _flush(self) -> None:
-        if not self._buf:
-            return
-        sch = self.manifest["schema"]
-        dt = np.float16 if self.manifest["dtype"] == "float16" else np.float32
-        recs = self._buf
-        out = {"ids": np.array([r.id for r in recs])}
-        out["atom_off"] = _offsets([len(r.atom_pos) for r in recs])
-        out["atom_pos"] = np.concatenate([r.atom_pos for r in recs]).astype(dt)
-        if sch["surf"]:
-            out["surf_pos"] = np.stack([r.surf_pos for r in recs]).astype(dt)
-        if sch["surf_esp"]:
-            out["surf_esp"] = np.stack([r.surf_esp for r in recs]).astype(dt)
-        if sch["charges"]:
-            out["charges"] = np.concatenate([r.partial_charges for r in recs]).astype(dt)
-            if sch["with_H"]:
-                out["all_off"] = _offsets([len(r.partial_charges) for r in recs])
-                out["nonH"] = np.concatenate([r.nonH_atoms_idx for r in recs]).astype(np.int64)
-                out["radii"] = np.concatenate([r.radii for r in recs]).astype(dt)
-                out["cwh"] = np.concatenate([r.centers_w_H for r in recs]).astype(dt)
-                heavy = [len(r.nonH_atoms_idx) for r in recs]
-            else:
-                heavy = [len(r.partial_charges) for r in recs]
-            if any(r.atom_pos_noH is not None for r in recs):
-                out["heavy_off"] = _offsets(heavy)
-                out["xyz_noH"] = np.concatenate([r.atom_pos_noH if r.atom_pos_noH is not None
-                                                 else r.atom_pos for r in recs]).astype(dt)
-        if sch["pharm"]:
-            out["pharm_off"] = _offsets([len(r.pharm_types) for r in recs])
-            out["pharm_types"] = np.concatenate([r.pharm_types for r in recs]).astype(np.int32)
-            out["pharm_ancs"] = np.concatenate([r.pharm_ancs for r in recs]).astype(dt)
-            out["pharm_vecs"] = np.concatenate([r.pharm_vecs for r in recs]).astype(dt)
-        name = f"shard_{self._shard_id:05d}.npz"
-        np.savez(os.path.join(self.path, name), **out)
-        self.manifest["shards"].append(dict(name=name, n=len(recs), start=self._n))
-        self._n += len(recs)
-        self.manifest["n_total"] = self._n
-        self._buf = []
-        self._shard_id += 1
-        self._write_manifest()
-
     # -- reading ----------------------------------------------------------------------------
     @classmethod
     def open(cls, path: str) -> "ProfileStore":
         with open(os.path.join(path, MANIFEST)) as fh:
             manifest = json.load(fh)
-        ver = manifest.get("version")
-        if ver != FORMAT_VERSION:
-            raise ValueError(f"store at {path!r} has on-disk format version {ver!r}; this reader "
-                             f"takes version {FORMAT_VERSION}. Rebuild the store.")
-        return cls(path, manifest, "r")
+        found = manifest.get("version")
+        if found != FORMAT_VERSION:
+            raise ValueError(f"{path!r}: manifest format version {found!r}, this package reads version "
+                             f"{FORMAT_VERSION}; write the store again with this version")
+        return cls(path, manifest, writable=False)
 
     def __len__(self) -> int:
         return int(self.manifest["n_total"])
@@ -326,51 +385,25 @@ class ProfileStore:
         return store_supports(self.schema, mode)
 
     def read_shard(self, idx: int):
-        """(shard metadata, {array name: ndarray}) of shard ``idx``; no pickled objects."""
-        sh = self.manifest["shards"][idx]
+        """(shard record, {array name: ndarray}) of shard ``idx``; object arrays are refused."""
+        rec = self.manifest["shards"][idx]
         try:
-            with np.load(os.path.join(self.path, sh["name"]), allow_pickle=False) as data:
-                return sh, {k: data[k] for k in data.files}
+            with np.load(os.path.join(self.path, rec["name"]), allow_pickle=False) as z:
+                return rec, {k: z[k] for k in z.files}
         except ValueError as exc:
-            raise ValueError(f"{sh['name']}: arrays that need unpickling (object ids?) are not "
-                             f"read: {exc}") from exc
-
-    def iter_profiles(self) -> Iterator[List[MoleculeProfile]]:
-        """One shard at a time as MoleculeProfile lists (float32 in memory)."""
-        for i in range(self.num_shards):
-            yield self.read_profiles(i)
-
-    iter_shards = iter_profiles            # the reference's name (screen.py:556-560)
+            raise ValueError(f"{rec['name']} holds arrays that only unpickling could read "
+                             f"(object ids?); they are not loaded: {exc}") from exc
 
     def read_profiles(self, idx: int) -> List[MoleculeProfile]:
-        """Shard ``idx`` as a MoleculeProfile list (random access; screen.py:562-565)."""
-        sch = self.schema
-        sh, a = self.read_shard(idx)
-        ao = a["atom_off"]
-        po = a.get("pharm_off")
-        out = []
-        for j in range(sh["n"]):
-            p = MoleculeProfile(atom_pos=_f32(a["atom_pos"][ao[j]:ao[j + 1]]), id=a["ids"][j].item())
-            if sch["surf"]:
-                p.surf_pos = _f32(a["surf_pos"][j])
-            if sch["surf_esp"]:
-                p.surf_esp = _f32(a["surf_esp"][j])
-            if sch["charges"]:
-                if sch["with_H"]:
-                    c0, c1 = a["all_off"][j], a["all_off"][j + 1]
-                    p.partial_charges = _f32(a["charges"][c0:c1])
-                    p.nonH_atoms_idx = a["nonH"][ao[j]:ao[j + 1]]
-                    p.radii = _f32(a["radii"][c0:c1])
-                    p.centers_w_H = _f32(a["cwh"][c0:c1])
-                else:
-                    ho = a["heavy_off"] if "heavy_off" in a else ao
-                    p.partial_charges = _f32(a["charges"][ho[j]:ho[j + 1]])
-            if sch["pharm"]:
-                p.pharm_types = a["pharm_types"][po[j]:po[j + 1]]
-                p.pharm_ancs = _f32(a["pharm_ancs"][po[j]:po[j + 1]])
-                p.pharm_vecs = _f32(a["pharm_vecs"][po[j]:po[j + 1]])
-            out.append(p)
-        return out
+        """Shard ``idx`` as a MoleculeProfile list, random access (screen.py:562-565)."""
+        rec, arrays = self.read_shard(idx)
+        return _decode_shard(arrays, int(rec["n"]), self.schema)
+
+    def iter_profiles(self) -> Iterator[List[MoleculeProfile]]:
+        """The shards in order, each as a MoleculeProfile list (float32 in memory)."""
+        return (self.read_profiles(i) for i in range(self.num_shards))
+
+    iter_shards = iter_profiles            # the reference's name (screen.py:556-560)
 
 
 def _rows(xyz: np.ndarray, w: Optional[np.ndarray] = None) -> np.ndarray:
