@@ -34,6 +34,7 @@ LIB_SHAPE = 1
 LIB_PHARM = 2
 LIB_SHAEP = 4
 SCREEN_MAX_TOPK = 1024
+NOALLOC_SCRATCH_BYTES = 8 << 20
 
 _i32p = ctypes.POINTER(ctypes.c_int32)
 _f32p = ctypes.POINTER(ctypes.c_float)
@@ -177,6 +178,7 @@ SIGNATURES = {
     "fss_pose_score": (ctypes.c_int, [_P, _P, ctypes.c_int32, _P, ctypes.c_int32, _P, _P, _P,
                                       _P, _P]),
     "fss_align": (ctypes.c_int, [_P, _P, _P, _P, _P, _P]),
+    "fss_align_noalloc": (ctypes.c_int, [_P, _P, _P, _P, _P]),
     "fss_align_finish": (ctypes.c_int, [_P, _P, ctypes.c_int32, _P, _P, _P]),
     "fss_seed_poses": (ctypes.c_int, [_P, ctypes.c_int32, ctypes.c_int32, _P, _P, _P, _P]),
     "fss_score_fwd": (ctypes.c_int, [_P, _P, _P, _P, _P]),

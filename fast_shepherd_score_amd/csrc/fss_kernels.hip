@@ -18,6 +18,7 @@
 #include <stdarg.h>
 #include <stdio.h>
 #include <string.h>
+#include <mutex>
 #include <string>
 
 #include "../../include/fss.h"
@@ -1412,6 +1413,118 @@ int align_finish_impl(const FssPairBatch* batch, const KScore& ks, int B, const 
   return check_launch("fss_align (finish)");
 }
 
+// Tiling checks and launch geometry of the persistent optimiser (shared by fss_align and
+// fss_align_noalloc).  The outputs other than pose_score are checked here.
+struct AlignPlan {
+  int block;
+  LdsPlan lp;
+  size_t lds;
+  AlignK ak;
+  long grid;
+};
+
+int plan_align(const FssPairBatch* batch, const KScore& ks, const FssAlignParams* ap, const FssAlignOut* out,
+               AlignPlan& pl) {
+  if (!out->q || !out->t || !out->best_score || !out->best_idx || !out->best_tf || !out->n_steps)
+    return set_error(FSS_E_INVALID, "NULL align outputs");
+  const int B = ap->n_restarts, lpp = ap->lanes_per_pose, rpw = ap->restarts_per_wg;
+  if (B <= 0) return set_error(FSS_E_INVALID, "n_restarts must be > 0");
+  if (ap->max_steps < 0) return set_error(FSS_E_INVALID, "max_steps must be >= 0");
+  if (lpp < 1 || lpp > 64 || (lpp & (lpp - 1))) return set_error(FSS_E_INVALID, "lanes_per_pose must be a power of two in [1, 64]");
+  if (rpw < 1) return set_error(FSS_E_INVALID, "restarts_per_wg must be >= 1");
+  if (ap->early_stop && rpw < B)
+    return set_error(FSS_E_INVALID, "early stop couples all %d restarts of a pair: restarts_per_wg (%d) must be >= n_restarts", B, rpw);
+  pl.block = ((rpw * lpp + 63) / 64) * 64;
+  if (pl.block > 1024) return set_error(FSS_E_INVALID, "restarts_per_wg * lanes_per_pose = %d exceeds 1024 threads", rpw * lpp);
+  const bool mfma = effective_variant(ks.mode, ap->variant) == 3;
+  if (mfma && lpp != 64)
+    return set_error(FSS_E_INVALID, "the MFMA inner loop (variant 3) runs one pose per wavefront: lanes_per_pose must be 64");
+  pl.lp = plan_lds(batch, ks.mode, mfma);
+  const int groups = pl.block / lpp;
+  pl.lds = (size_t)pl.lp.end * sizeof(float4) + (ap->early_stop ? 2 * (size_t)rpw * sizeof(double) : 0) +
+           (size_t)groups * 16 * sizeof(float);
+  const size_t cap = max_lds_bytes();
+  if (pl.lds > cap) return set_error(FSS_E_TOO_LARGE, "clouds need %zu B of LDS, the device allows %zu B", pl.lds, cap);
+  AlignK& ak = pl.ak;
+  ak.B = B; ak.max_steps = ap->max_steps;
+  ak.lr = ap->lr; ak.b1 = ap->beta1; ak.b2 = ap->beta2; ak.eps = ap->eps;
+  ak.es_tol = ap->es_tol; ak.es_pat = ap->es_patience;
+  ak.lpp = lpp; ak.rpw = rpw; ak.wg_per_pair = (B + rpw - 1) / rpw;
+  ak.order = ap->pair_order;
+  pl.grid = (long)batch->n_pairs * ak.wg_per_pair;
+  if (pl.grid > 0x7fffffffL) return set_error(FSS_E_INVALID, "grid too large");
+  return FSS_OK;
+}
+
+// The optimiser launch and (unless skip_finish) the re-scoring and best-restart finish, with the
+// self-overlaps and the per-restart score buffer given explicitly.
+int run_align(const FssPairBatch* batch, const KScore& ks, const FssAlignParams* ap, AlignPlan pl,
+              const double* self_ov, double* pose_score, const FssAlignOut* out, hipStream_t s) {
+  pl.ak.self_ov = self_ov;
+  pl.ak.q = out->q; pl.ak.t = out->t; pl.ak.pose_score = pose_score; pl.ak.n_steps = out->n_steps;
+  const bool es = ap->early_stop != 0;
+#define FSS_ALIGN(M) \
+  (es ? launch_align<M, true>(batch, ks, pl.lp, pl.ak, (int)pl.grid, pl.block, pl.lds, ap->variant, s) \
+      : launch_align<M, false>(batch, ks, pl.lp, pl.ak, (int)pl.grid, pl.block, pl.lds, ap->variant, s))
+  switch (ks.mode) {
+    case FSS_MODE_SHAPE: FSS_ALIGN(FSS_MODE_SHAPE); break;
+    case FSS_MODE_SHAPE_ESP: FSS_ALIGN(FSS_MODE_SHAPE_ESP); break;
+    case FSS_MODE_PHARM: FSS_ALIGN(FSS_MODE_PHARM); break;
+    case FSS_MODE_ESP_COMBO: FSS_ALIGN(FSS_MODE_ESP_COMBO); break;
+    default: FSS_ALIGN(FSS_MODE_SHAPE_PHARM); break;
+  }
+#undef FSS_ALIGN
+  int rc;
+  if ((rc = check_launch("fss_align"))) return rc;
+  if (ap->skip_finish) return FSS_OK;
+  FssAlignOut fo = *out;
+  fo.pose_score = pose_score;
+  return align_finish_impl(batch, ks, ap->n_restarts, self_ov, &fo, s);
+}
+
+}  // namespace
+
+// ------------------------------------------------------------------------------------------------
+// No-allocation alignment (fss_align_noalloc).  The self-overlaps (4 doubles per pair) and the
+// per-restart re-scoring buffer (1 double per pose) live in this file-scope device array, one copy
+// per GPU, instead of in buffers the caller allocates.  Host code never passes `scratch` itself to
+// a kernel (on the host the name is only a stand-in): each device's copy is looked up once with
+// hipGetSymbolAddress and that device address is used.  Every call on a GPU shares the array and
+// overwrites what it uses (the self-overlap and re-scoring kernels write every entry they later
+// read), so nothing is cleared between calls; calls that overlap in time on one GPU race.
+// ------------------------------------------------------------------------------------------------
+static __device__ __align__(256) unsigned char scratch[FSS_NOALLOC_SCRATCH_BYTES];
+
+namespace {
+
+constexpr int kScratchDevices = 64;
+std::mutex g_scratch_mu;
+unsigned char* g_scratch[kScratchDevices] = {};
+
+// This GPU's copy of `scratch`: resolved on the first call on the current device, with that
+// device current, and kept for the life of the process.
+int scratch_address(unsigned char** out) {
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return set_error((int)e, "fss_align_noalloc: hipGetDevice: %s", hipGetErrorString(e));
+  if (dev < 0 || dev >= kScratchDevices)
+    return set_error(FSS_E_INVALID, "fss_align_noalloc: device %d is beyond the %d devices it tracks", dev,
+                     kScratchDevices);
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch[dev]) {
+    void* p = nullptr;
+    e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+      return set_error(e != hipSuccess ? (int)e : FSS_E_NO_DEVICE, "fss_align_noalloc: hipGetSymbolAddress: %s",
+                       hipGetErrorString(e));
+    g_scratch[dev] = static_cast<unsigned char*>(p);
+  }
+  *out = g_scratch[dev];
+  return FSS_OK;
+}
+
+constexpr size_t round256(size_t b) { return (b + 255) & ~(size_t)255; }
+
 }  // namespace
 
 // ================================================================================================
@@ -1568,52 +1681,48 @@ int fss_align(const FssPairBatch* batch, const FssScoreParams* sp, const FssAlig
   if (rc) return rc;
   if ((rc = check_batch(batch, ks.mode))) return rc;
   if (!ap || !out || !self_ov) return set_error(FSS_E_INVALID, "NULL align arguments");
-  if (!out->q || !out->t || !out->pose_score || !out->best_score || !out->best_idx || !out->best_tf || !out->n_steps)
-    return set_error(FSS_E_INVALID, "NULL align outputs");
-  const int B = ap->n_restarts, lpp = ap->lanes_per_pose, rpw = ap->restarts_per_wg;
-  if (B <= 0) return set_error(FSS_E_INVALID, "n_restarts must be > 0");
-  if (ap->max_steps < 0) return set_error(FSS_E_INVALID, "max_steps must be >= 0");
-  if (lpp < 1 || lpp > 64 || (lpp & (lpp - 1))) return set_error(FSS_E_INVALID, "lanes_per_pose must be a power of two in [1, 64]");
-  if (rpw < 1) return set_error(FSS_E_INVALID, "restarts_per_wg must be >= 1");
-  if (ap->early_stop && rpw < B)
-    return set_error(FSS_E_INVALID, "early stop couples all %d restarts of a pair: restarts_per_wg (%d) must be >= n_restarts", B, rpw);
-  const int block = ((rpw * lpp + 63) / 64) * 64;
-  if (block > 1024) return set_error(FSS_E_INVALID, "restarts_per_wg * lanes_per_pose = %d exceeds 1024 threads", rpw * lpp);
-  const bool mfma = effective_variant(ks.mode, ap->variant) == 3;
-  if (mfma && lpp != 64)
-    return set_error(FSS_E_INVALID, "the MFMA inner loop (variant 3) runs one pose per wavefront: lanes_per_pose must be 64");
+  if (!out->pose_score) return set_error(FSS_E_INVALID, "NULL align outputs");
+  AlignPlan pl;
+  if ((rc = plan_align(batch, ks, ap, out, pl))) return rc;
   if (batch->n_pairs == 0) return FSS_OK;
-  const LdsPlan lp = plan_lds(batch, ks.mode, mfma);
-  const int groups = block / lpp;
-  const size_t lds = (size_t)lp.end * sizeof(float4) + (ap->early_stop ? 2 * (size_t)rpw * sizeof(double) : 0) +
-                     (size_t)groups * 16 * sizeof(float);
-  const size_t cap = max_lds_bytes();
-  if (lds > cap) return set_error(FSS_E_TOO_LARGE, "clouds need %zu B of LDS, the device allows %zu B", lds, cap);
-  AlignK ak;
-  ak.B = B; ak.max_steps = ap->max_steps;
-  ak.lr = ap->lr; ak.b1 = ap->beta1; ak.b2 = ap->beta2; ak.eps = ap->eps;
-  ak.es_tol = ap->es_tol; ak.es_pat = ap->es_patience;
-  ak.lpp = lpp; ak.rpw = rpw; ak.wg_per_pair = (B + rpw - 1) / rpw;
-  ak.order = ap->pair_order; ak.self_ov = self_ov;
-  ak.q = out->q; ak.t = out->t; ak.pose_score = out->pose_score; ak.n_steps = out->n_steps;
-  const long grid = (long)batch->n_pairs * ak.wg_per_pair;
-  if (grid > 0x7fffffffL) return set_error(FSS_E_INVALID, "grid too large");
+  return run_align(batch, ks, ap, pl, self_ov, out->pose_score, out, (hipStream_t)stream);
+}
+
+int fss_align_noalloc(const FssPairBatch* batch, const FssScoreParams* sp, const FssAlignParams* ap,
+                      FssAlignOut* out, void* stream) {
+  KScore ks;
+  int rc = make_kscore(sp, ks);
+  if (rc) return rc;
+  if ((rc = check_batch(batch, ks.mode))) return rc;
+  if (!ap || !out) return set_error(FSS_E_INVALID, "NULL align arguments");
+  if (ap->skip_finish)
+    return set_error(FSS_E_INVALID, "fss_align_noalloc always finishes (its self-overlaps stay in the scratch "
+                                    "buffer): skip_finish must be 0");
+  AlignPlan pl;
+  if ((rc = plan_align(batch, ks, ap, out, pl))) return rc;
+  const size_t so_bytes = round256((size_t)batch->n_pairs * 4 * sizeof(double));
+  const size_t need = so_bytes + (size_t)batch->n_pairs * (size_t)ap->n_restarts * sizeof(double);
+  if (need > (size_t)FSS_NOALLOC_SCRATCH_BYTES)
+    return set_error(FSS_E_TOO_LARGE,
+                     "fss_align_noalloc: %d pairs x %d restarts need %zu B of scratch; the library holds %zu B "
+                     "per GPU (at most about %zu poses): split the batch or use fss_align",
+                     batch->n_pairs, ap->n_restarts, need, (size_t)FSS_NOALLOC_SCRATCH_BYTES,
+                     (size_t)FSS_NOALLOC_SCRATCH_BYTES / sizeof(double));
+  if (batch->n_pairs == 0) return FSS_OK;
+  unsigned char* base = nullptr;
+  if ((rc = scratch_address(&base))) return rc;
+  double* so = reinterpret_cast<double*>(base);
+  double* ps = reinterpret_cast<double*>(base + so_bytes);
   hipStream_t s = (hipStream_t)stream;
-  const bool es = ap->early_stop != 0;
-#define FSS_ALIGN(M) \
-  (es ? launch_align<M, true>(batch, ks, lp, ak, (int)grid, block, lds, ap->variant, s) \
-      : launch_align<M, false>(batch, ks, lp, ak, (int)grid, block, lds, ap->variant, s))
   switch (ks.mode) {
-    case FSS_MODE_SHAPE: FSS_ALIGN(FSS_MODE_SHAPE); break;
-    case FSS_MODE_SHAPE_ESP: FSS_ALIGN(FSS_MODE_SHAPE_ESP); break;
-    case FSS_MODE_PHARM: FSS_ALIGN(FSS_MODE_PHARM); break;
-    case FSS_MODE_ESP_COMBO: FSS_ALIGN(FSS_MODE_ESP_COMBO); break;
-    default: FSS_ALIGN(FSS_MODE_SHAPE_PHARM); break;
+    case FSS_MODE_SHAPE: launch_self<FSS_MODE_SHAPE>(batch, ks, so, s); break;
+    case FSS_MODE_SHAPE_ESP: launch_self<FSS_MODE_SHAPE_ESP>(batch, ks, so, s); break;
+    case FSS_MODE_PHARM: launch_self<FSS_MODE_PHARM>(batch, ks, so, s); break;
+    case FSS_MODE_ESP_COMBO: launch_self<FSS_MODE_ESP_COMBO>(batch, ks, so, s); break;
+    default: launch_self<FSS_MODE_SHAPE_PHARM>(batch, ks, so, s); break;
   }
-#undef FSS_ALIGN
-  if ((rc = check_launch("fss_align"))) return rc;
-  if (ap->skip_finish) return FSS_OK;
-  return align_finish_impl(batch, ks, B, self_ov, out, s);
+  if ((rc = check_launch("fss_align_noalloc (self-overlaps)"))) return rc;
+  return run_align(batch, ks, ap, pl, so, ps, out, s);
 }
 
 int fss_align_finish(const FssPairBatch* batch, const FssScoreParams* sp, int32_t n_restarts,

@@ -244,6 +244,22 @@ int fss_align(const FssPairBatch* batch, const FssScoreParams* sp, const FssAlig
               const double* self_ov, FssAlignOut* out, void* stream);
 
 /*
+ * fss_align without any device allocation by the caller or the library: the self-overlaps are
+ * computed inside (as fss_self_overlap does) and, with the per-restart re-scoring scores, kept
+ * in a static device array of FSS_NOALLOC_SCRATCH_BYTES per GPU, held until the process ends.
+ * The caller passes the batch, the parameters, the seeds in out->q / out->t and the outputs
+ * best_score, best_idx, best_tf and n_steps; out->pose_score is not used; skip_finish must be 0.
+ * Batches needing more than the scratch (256-byte-rounded 32 B per pair + 8 B per pose) are
+ * rejected with FSS_E_TOO_LARGE.  Results equal fss_align's.  Every call on one GPU shares that
+ * GPU's scratch, so calls that overlap in time on one GPU race; nothing in it needs clearing.
+ * No counterpart in the reference: for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ */
+#define FSS_NOALLOC_SCRATCH_BYTES (8u << 20)
+int fss_align_noalloc(const FssPairBatch* batch, const FssScoreParams* sp, const FssAlignParams* ap,
+                      FssAlignOut* out, void* stream);
+
+/*
  * The finishing half of fss_align (FP64 re-scoring of every final pose, best restart, SE(3)
  * matrix) for a call made with skip_finish = 1; reads out->q, out->t.
  */
