@@ -77,11 +77,8 @@ def pair_terms(wl, B: int, steps: int) -> float:
     return float((nr * nf).sum()) * B * steps
 
 
-def cpu_baseline(wl, q, t, B, steps, sample: int, spec, opt):
-    """The oracle package's C++ restatement, timed on this host's cores in a child process."""
-    import subprocess
-    import tempfile
-    idx = np.arange(min(sample, len(wl["fit"])))
+def _cpu_sample(path, wl, q, t, idx, spec, opt):
+    """Write pairs ``idx`` of a workload with their seeds as a sample file for oracle.cpu_bench."""
     cat = lambda xs: np.concatenate([np.asarray(xs[i], np.float32) for i in idx])
     arrs = dict(n=len(idx), ref=cat(wl["ref"]), fit=cat(wl["fit"]),
                 ref_n=np.array([len(wl["ref"][i]) for i in idx]),
@@ -90,19 +87,39 @@ def cpu_baseline(wl, q, t, B, steps, sample: int, spec, opt):
                 steps=opt.max_steps, lr=opt.lr, early_stop=opt.early_stop)
     if wl.get("ref_w") is not None:
         arrs["ref_w"], arrs["fit_w"] = cat(wl["ref_w"]), cat(wl["fit_w"])
+    np.savez(path, **arrs)
+
+
+def cpu_baseline(wl, q, t, B, steps, sample: int, spec, opt, runs: int = 3):
+    """The oracle package's C++ restatement of the numba backend, timed on this host's cores in a
+    child process: the first ``sample`` pairs of the measured workload and, at full size, the
+    64-pair volumetric workload of the c1 leg (same seeds); median of ``runs`` runs each."""
+    import subprocess
+    import tempfile
+    from fast_shepherd_score_amd.align import OptimSpec, ScoreSpec
+    idx = np.arange(min(sample, len(wl["fit"])))
+    c1 = build_workload("vol", 64, 3)
+    q1, t1 = seeds_for(c1, 50, 3)
     with tempfile.TemporaryDirectory() as d:
-        path = os.path.join(d, "sample.npz")
-        np.savez(path, **arrs)
-        res = subprocess.run([sys.executable, "-m", "oracle.cpu_bench", path], cwd=ROOT,
-                             capture_output=True, text=True, timeout=600)
+        paths = [os.path.join(d, "headline.npz"), os.path.join(d, "c1.npz")]
+        _cpu_sample(paths[0], wl, q, t, idx, spec, opt)
+        _cpu_sample(paths[1], c1, q1, t1, np.arange(64), ScoreSpec(), OptimSpec(max_steps=200, early_stop=False))
+        res = subprocess.run([sys.executable, "-m", "oracle.cpu_bench", "--runs", str(runs), *paths],
+                             cwd=ROOT, capture_output=True, text=True, timeout=900)
     if res.returncode != 0:
         raise RuntimeError(f"cpu baseline failed: {res.stderr[-2000:]}")
     r = json.loads(res.stdout.strip().splitlines()[-1])
-    dt = r["seconds"]
+    head, full = r["samples"]
+    dt = head["seconds"]
+    host = f"{r['cpu_model']}, {r['vector_bits']}-bit vectors"
     return {"value": len(idx) / dt, "unit": "aligned pairs/s", "cores": r["cores"], "kind": "port",
-            "sample": f"first {len(idx)} pairs of the same workload, same seeds, {B} restarts x "
-                      f"{steps} steps, {dt:.1f} s on {r['cores']} threads ({r['cpu_model']})",
-            "seconds": dt}, np.array(r["score"])
+            "sample": f"first {len(idx)} pairs of the same workload, same seeds, {B} restarts x {steps} "
+                      f"steps; median of {len(head['runs'])} runs ({', '.join(f'{x:.1f}' for x in head['runs'])} s) "
+                      f"on {r['cores']} threads ({host})",
+            "cpu_model": r["cpu_model"],
+            "c1_full": {"workload": "64 atom pairs (20-40 heavy atoms), 50 restarts x 200 steps (all executed)",
+                        "pairs_per_s": 64 / full["seconds"], "runs_s": full["runs"]},
+            "seconds": dt}, np.array(head["score"])
 
 
 def bench_c1(dev, reps: int = 3):
@@ -386,7 +403,7 @@ def main():
     ap.add_argument("--pairs", type=int, default=4096)
     ap.add_argument("--restarts", type=int, default=50)
     ap.add_argument("--opt-steps", type=int, default=200)
-    ap.add_argument("--cpu-sample", type=int, default=48)
+    ap.add_argument("--cpu-sample", type=int, default=128)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--lpp", type=int, default=0)
     ap.add_argument("--rpw", type=int, default=0)

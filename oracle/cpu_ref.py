@@ -1,4 +1,5 @@
-"""ctypes wrapper of oracle/libfss_cpu.so, the C++/OpenMP CPU baseline (test infrastructure)."""
+"""ctypes wrapper of oracle/libfss_cpu.so, the C++ (std::thread pool, vectorised kernel) CPU baseline
+(test infrastructure)."""
 from __future__ import annotations
 
 import ctypes
