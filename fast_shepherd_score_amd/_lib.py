@@ -93,6 +93,7 @@ class FssAlignParams(ctypes.Structure):
         ("pair_order", ctypes.c_void_p),
         ("skip_finish", ctypes.c_int32),
         ("variant", ctypes.c_int32),
+        ("groups_per_wg", ctypes.c_int32),
     ]
 
 
@@ -193,7 +194,7 @@ SIGNATURES = {
     "fss_screen_create": (ctypes.c_int, [_P, _P, _P, _P]),
     "fss_screen_submit": (ctypes.c_int, [_P, _P, _P]),
     "fss_screen_poll": (ctypes.c_int, [_P, ctypes.c_int64, ctypes.c_int32, _P]),
-    "fss_screen_stats": (ctypes.c_int, [_P, _P, _P]),
+    "fss_screen_stats": (ctypes.c_int, [_P, _P, _P, _P]),
     "fss_screen_destroy": (ctypes.c_int, [_P]),
     "fss_plan_launch": (ctypes.c_int, [ctypes.c_int32, _P, ctypes.c_int64, _P]),
 }

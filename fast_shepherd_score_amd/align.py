@@ -135,6 +135,10 @@ def _waves_per_pair(B: int, lpp: int, rpw: int) -> int:
 
 
 def candidate_tilings(B: int, early_stop: bool):
+    # one wavefront holding every restart with the widest lane group that fits: 64 // B lanes,
+    # a non-power-of-two width when B does not divide 64 (10 restarts -> 6 lanes each)
+    if 2 <= B <= 32 and (64 // B) & (64 // B - 1):
+        yield 64 // B, B
     divisors = [d for d in range(1, B + 1) if B % d == 0]
     for lpp in (1, 2, 4, 8, 16, 32, 64):
         one_wave = max(1, 64 // lpp)
@@ -164,7 +168,10 @@ def tiling_cost(ref_counts, fit_counts, B: int, lpp: int, rpw: int, waves_per_si
     SIMDs idle (the measured MI355X behaviour that favours workgroups of 1, 2, 4 or 8 waves)."""
     n = np.maximum(np.asarray(ref_counts, np.float64), 1.0)
     m = np.maximum(np.asarray(fit_counts, np.float64), 1.0)
-    per_wave = np.ceil(m / lpp) * n * 10.0 + 26.0 * np.log2(max(lpp, 1)) + 150.0
+    # lane-group reduction: DPP levels for power-of-two widths, a shuffle tree plus a broadcast
+    # (about 1.5 times the cost per level) otherwise
+    levels = np.log2(max(lpp, 1)) if lpp & (lpp - 1) == 0 else 1.5 * (np.ceil(np.log2(lpp)) + 1)
+    per_wave = np.ceil(m / lpp) * n * 10.0 + 26.0 * levels + 150.0
     wpg = int(np.ceil(rpw * lpp / 64.0))
     slots = 4 * waves_per_simd
     wgs_per_cu = max(1, slots // wpg)
@@ -195,13 +202,17 @@ def default_tiling(pb: PairBatch, mode: int, B: int, early_stop: bool) -> Tuple[
     return cands[0][2], cands[0][3]
 
 
-def check_tiling(B: int, lpp: int, rpw: int, early_stop: bool) -> None:
-    if lpp < 1 or lpp > 64 or lpp & (lpp - 1):
-        raise ValueError("lanes_per_pose must be a power of two in [1, 64]")
+def check_tiling(B: int, lpp: int, rpw: int, early_stop: bool, groups: int = 0) -> None:
+    g = groups or rpw
+    if g > rpw:
+        raise ValueError("groups_per_wg exceeds restarts_per_wg")
+    if lpp < 1 or lpp > 64 or (lpp & (lpp - 1) and g * lpp > 64):
+        raise ValueError("lanes_per_pose must be in [1, 64], and a power of two unless the workgroup's "
+                         "lane groups x lanes_per_pose fit one wavefront (64)")
     if early_stop and rpw < B:
         raise ValueError("per-pair early stop needs every restart of a pair in one workgroup")
-    if rpw * lpp > 1024:
-        raise ValueError("restarts_per_wg * lanes_per_pose exceeds 1024 threads")
+    if g * lpp > 1024:
+        raise ValueError("lane groups x lanes_per_pose exceed 1024 threads")
 
 
 def _prep_seeds(pb: PairBatch, q0: torch.Tensor, t0: torch.Tensor):
@@ -220,13 +231,15 @@ def align(pb: PairBatch, spec: ScoreSpec, q0: torch.Tensor, t0: torch.Tensor,
           opt: OptimSpec = OptimSpec(), *, lanes_per_pose: Optional[int] = None,
           restarts_per_wg: Optional[int] = None, order: str = "cost",
           self_ov: Optional[torch.Tensor] = None, stream=None, events: Optional[list] = None,
-          tune: bool = False, variant: int = 0) -> AlignResult:
+          tune: bool = False, variant: int = 0, groups_per_wg: Optional[int] = None) -> AlignResult:
     """Persistent path: every step of every pair in one launch.
 
     Tiling: explicit ``lanes_per_pose`` / ``restarts_per_wg``, else an autotuned choice
     (``tune=True``, or one cached for this batch profile), else the cost model's choice.
     ``variant`` picks the shape inner loop: 1 direct form, 2 expanded form, 3 split-bf16 MFMA
     (one wavefront per pose, so 64 lanes per pose), 0 the tuned or default choice.
+    ``groups_per_wg``: lane groups per workgroup, each running restarts_per_wg / groups_per_wg
+    restarts in turn every step (None: the tuned choice, else one restart per group).
     ``events``: if a list is given, HIP events bracketing the optimiser kernel and the finishing
     kernels are appended to it as (start, optimiser_end, finish_end).
     """
@@ -241,8 +254,10 @@ def align(pb: PairBatch, spec: ScoreSpec, q0: torch.Tensor, t0: torch.Tensor,
                              self_ov=self_ov) if tune else \
             _at.cached(pb, spec.mode, B, opt.early_stop, opt.max_steps)
         if tuned is not None:
-            lpp0, rpw0, var0 = tuned
+            lpp0, rpw0, var0, grp0 = tuned
             variant = variant or var0
+            if groups_per_wg is None and var0 == variant:
+                groups_per_wg = grp0
         else:
             lpp0, rpw0 = default_tiling(pb, spec.mode, B, opt.early_stop)
     else:
@@ -251,7 +266,12 @@ def align(pb: PairBatch, spec: ScoreSpec, q0: torch.Tensor, t0: torch.Tensor,
         lpp0, rpw0 = 64, (rpw0 if lpp0 == 64 else 4)
     lpp = lanes_per_pose or lpp0
     rpw = restarts_per_wg or (B if opt.early_stop else min(rpw0, B))
-    check_tiling(B, lpp, rpw, opt.early_stop)
+    grp = int(groups_per_wg or 0)
+    if variant == 3 and lanes_per_pose is None and opt.early_stop and B > 16 and not grp:
+        # the MFMA loop needs a wavefront per pose: with every restart of a pair in one
+        # workgroup, 50 restarts run as 5 wavefronts of 10 restarts each
+        grp = max(d for d in range(1, 17) if B % d == 0)
+    check_tiling(B, lpp, rpw, opt.early_stop, grp)
     order_t = None
     if order == "cost" and P > 1:
         c = pb.cost(spec.mode)
@@ -269,6 +289,7 @@ def align(pb: PairBatch, spec: ScoreSpec, q0: torch.Tensor, t0: torch.Tensor,
     ap.lanes_per_pose, ap.restarts_per_wg = int(lpp), int(rpw)
     ap.pair_order = _lib.ptr(order_t)
     ap.variant = int(variant)
+    ap.groups_per_wg = grp
     out = _lib.FssAlignOut()
     out.q, out.t = _lib.ptr(q), _lib.ptr(t)
     out.pose_score, out.best_score, out.best_idx = _lib.ptr(pose_score), _lib.ptr(best), _lib.ptr(bidx)
@@ -291,7 +312,7 @@ def align(pb: PairBatch, spec: ScoreSpec, q0: torch.Tensor, t0: torch.Tensor,
         events.append(tuple(ev))
     return AlignResult(best, bidx, tf, q, t, pose_score, nst,
                        extra={"lanes_per_pose": lpp, "restarts_per_wg": rpw, "variant": int(variant),
-                              "order": order_t})
+                              "groups_per_wg": grp or rpw, "order": order_t})
 
 
 class SteppedAligner:

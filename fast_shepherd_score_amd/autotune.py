@@ -1,5 +1,5 @@
-"""Autotuner for the persistent optimiser: tiling (lanes per pose, restarts per workgroup) and
-shape inner-loop variant (direct / expanded form).
+"""Autotuner for the persistent optimiser: tiling (lanes per pose, restarts per workgroup, lane
+groups per workgroup) and shape inner-loop variant (direct / expanded form / split-bf16 MFMA).
 
 The analytic model (:func:`fast_shepherd_score_amd.align.tiling_cost`) ranks every legal tiling;
 the tuner times the model's top candidates plus the wide-group tilings (64 lanes per pose), each
@@ -27,7 +27,7 @@ from .align import (OptimSpec, ScoreSpec, align, candidate_tilings, self_overlap
                     tiling_efficiency)
 from .packing import PairBatch
 
-_cache: Dict[str, Tuple[int, int, int]] = {}
+_cache: Dict[str, Tuple[int, int, int, int]] = {}
 _lock = threading.Lock()
 
 
@@ -56,8 +56,8 @@ def _load_disk() -> None:
         try:
             with open(path) as f:
                 for k, v in json.load(f).items():
-                    if len(v) == 3:
-                        _cache.setdefault(k, (int(v[0]), int(v[1]), int(v[2])))
+                    if len(v) in (3, 4):
+                        _cache.setdefault(k, (int(v[0]), int(v[1]), int(v[2]), int(v[3]) if len(v) == 4 else 0))
         except (OSError, ValueError):
             pass
 
@@ -74,9 +74,9 @@ def _save_disk() -> None:
 
 def autotune(pb: PairBatch, spec: ScoreSpec, q0: torch.Tensor, t0: torch.Tensor, early_stop: bool,
              *, max_steps: int = 200, top: int = 6, steps: int = 4,
-             self_ov: Optional[torch.Tensor] = None, verbose: bool = False) -> Tuple[int, int, int]:
-    """Best (lanes_per_pose, restarts_per_wg, variant) for this batch and step budget (timed,
-    cached)."""
+             self_ov: Optional[torch.Tensor] = None, verbose: bool = False) -> Tuple[int, int, int, int]:
+    """Best (lanes_per_pose, restarts_per_wg, variant, groups_per_wg) for this batch and step
+    budget (timed, cached); groups_per_wg 0 means one restart per lane group."""
     B = int(q0.shape[1])
     key = shape_key(pb, spec.mode, B, early_stop, max_steps)
     with _lock:
@@ -92,10 +92,13 @@ def autotune(pb: PairBatch, spec: ScoreSpec, q0: torch.Tensor, t0: torch.Tensor,
     tilings += [t for t in wide if t not in tilings]
     variants = (1, 2) if spec.mode != _lib.MODE_PHARM else (0,)
     # the MFMA inner loop runs one pose per wavefront: its own tilings (64 lanes per pose)
-    runs = [(l, r, v) for l, r in tilings for v in variants]
+    runs = [(l, r, v, 0) for l, r in tilings for v in variants]
     if spec.mode not in (_lib.MODE_PHARM, _lib.MODE_ESP_COMBO):
-        runs += [(64, r, 3) for r in ((B,) if early_stop else sorted({1, 2, 4, 8, min(16, B)}))
+        runs += [(64, r, 3, 0) for r in ((B,) if early_stop else sorted({1, 2, 4, 8, min(16, B)}))
                  if r <= B and 64 * r <= 1024]
+        if early_stop and B > 16:
+            # every restart of a pair in one workgroup of g wavefronts, each running B / g in turn
+            runs += [(64, B, 3, g) for g in (2, 4, 5, 8, 10, 16) if g < B and B % g == 0][:3]
     if self_ov is None:
         self_ov = self_overlaps(pb, spec)
     s1, s2 = steps, 3 * steps
@@ -107,18 +110,18 @@ def autotune(pb: PairBatch, spec: ScoreSpec, q0: torch.Tensor, t0: torch.Tensor,
         return ev[0][0].elapsed_time(ev[0][1])
 
     scored = []
-    for lpp, rpw, var in runs:
+    for lpp, rpw, var, grp in runs:
         if early_stop and rpw < B:
             continue
-        kw = dict(lanes_per_pose=lpp, restarts_per_wg=rpw, self_ov=self_ov, variant=var)
+        kw = dict(lanes_per_pose=lpp, restarts_per_wg=rpw, self_ov=self_ov, variant=var, groups_per_wg=grp)
         align(pb, spec, q0, t0, OptimSpec(max_steps=s1, early_stop=False), **kw)
         t = [timed(n, kw) for n in (s1, s2)]
         per_step = max(t[1] - t[0], 0.0) / (s2 - s1)
         ms = t[0] + (max_steps - s1) * per_step
         if verbose:
-            print(f"autotune {key}: lpp={lpp} rpw={rpw} variant={var} {t[0]:.3f}/{t[1]:.3f} ms "
+            print(f"autotune {key}: lpp={lpp} rpw={rpw} variant={var} groups={grp} {t[0]:.3f}/{t[1]:.3f} ms "
                   f"-> {ms:.3f} ms at {max_steps} steps", flush=True)
-        scored.append((ms, (lpp, rpw, var)))
+        scored.append((ms, (lpp, rpw, var, grp)))
     scored.sort()
     best = scored[0][1]
     # Short trials are noisy where a step takes well under a millisecond (the per-step estimate
@@ -133,7 +136,8 @@ def autotune(pb: PairBatch, spec: ScoreSpec, q0: torch.Tensor, t0: torch.Tensor,
         n = min(max_steps, 4 * s2)
         long_ms = {}
         for c in close:
-            kw = dict(lanes_per_pose=c[0], restarts_per_wg=c[1], self_ov=self_ov, variant=c[2])
+            kw = dict(lanes_per_pose=c[0], restarts_per_wg=c[1], self_ov=self_ov, variant=c[2],
+                      groups_per_wg=c[3])
             long_ms[c] = min(timed(n, kw) for _ in range(2))
             if verbose:
                 print(f"autotune {key}: {c} over {n} steps {long_ms[c]:.3f} ms", flush=True)
@@ -145,7 +149,7 @@ def autotune(pb: PairBatch, spec: ScoreSpec, q0: torch.Tensor, t0: torch.Tensor,
 
 
 def cached(pb: PairBatch, mode: int, B: int, early_stop: bool,
-           max_steps: int = 200) -> Optional[Tuple[int, int, int]]:
+           max_steps: int = 200) -> Optional[Tuple[int, int, int, int]]:
     with _lock:
         if not _cache:
             _load_disk()

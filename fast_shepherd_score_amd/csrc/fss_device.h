@@ -125,6 +125,34 @@ __device__ __forceinline__ void group_sum(Acc& a, int width, bool grad) {
   if (width > 32) sum_level<5>(a, grad);
 }
 
+// Sum over a group of `width` consecutive lanes for any width (groups need not be aligned to a
+// power of two; a group may not straddle the wavefront).  j = the lane's index in its group.
+// Fixed tree: at stride s, lanes with j % 2s == 0 add the partial of lane j + s (if in the
+// group); the group's first lane then holds the total, which every lane takes back, so all lanes
+// of a group end with the same bits whatever the other groups hold.
+__device__ __forceinline__ float group_sum_any(float v, int width, int j) {
+  const int lane = threadIdx.x & 63;
+  for (int s = 1; s < width; s <<= 1) {
+    const float o = __shfl(v, lane + s < 64 ? lane + s : lane, 64);
+    if ((j & (2 * s - 1)) == 0 && j + s < width) v += o;
+  }
+  return __shfl(v, lane - j, 64);
+}
+__device__ __forceinline__ void group_sum_any(Acc& a, int width, int j, bool grad) {
+  a.V = group_sum_any(a.V, width, j);
+  if (grad) {
+#pragma unroll
+    for (int i = 0; i < 3; ++i) a.gt[i] = group_sum_any(a.gt[i], width, j);
+#pragma unroll
+    for (int i = 0; i < 9; ++i) a.G[i] = group_sum_any(a.G[i], width, j);
+  }
+}
+// A pose's sums over its lane group: DPP levels for power-of-two widths, the shuffle tree else.
+__device__ __forceinline__ void pose_sum(Acc& a, int width, int j, bool grad) {
+  if ((width & (width - 1)) == 0) group_sum(a, width, grad);
+  else group_sum_any(a, width, j, grad);
+}
+
 // ---------------------------------------------------------------------------------------------
 // Shape channel, FP32, expanded form on scaled coordinates staged in LDS.
 //

@@ -659,6 +659,7 @@ struct AlignK {
   float es_tol;
   int es_pat;
   int lpp, rpw, wg_per_pair;
+  int groups;                 // lane groups per workgroup; each runs ceil(rpw / groups) restarts
   const int32_t* order;
   const double* self_ov;
   float* q;
@@ -686,27 +687,41 @@ void k_align(FssPairBatch pb, KScore ks, LdsPlan lp, AlignK ak) {
   const Staged st = stage_pair<SH, ESP, PH, EXPD, MF, SE>(pb, ks, lp, pair, 0, 1, lds);
   double* lossbuf = reinterpret_cast<double*>(lds + lp.end);  // 2 * rpw doubles (ES only)
 
-  const int lpp = ak.lpp;
+  const int lpp = ak.lpp, groups = ak.groups;
   const int grp = threadIdx.x / lpp, j = threadIdx.x - grp * lpp;
-  const int r = blk * ak.rpw + grp;
-  const bool active = grp < ak.rpw && r < ak.B;
-  const long pose = (long)pair * ak.B + (active ? r : 0);
+  // Each lane group runs the restarts grp, grp + groups, ... of the workgroup's block, one after
+  // the other in every step.  A restart's Adam moments (and, when its group runs more than one,
+  // its parameters) live in a 24-float LDS slot: m at 0..6, v at 7..13, q and t at 14..20; lane 0
+  // of the group writes, the group's lanes read (one wavefront, in-order DS operations).  With
+  // one restart per group the parameters stay in registers across steps.
+  const int rpg = (ak.rpw + groups - 1) / groups;
   const double* so = ak.self_ov + 4 * pair;
   const float invB = 1.0f / (float)ak.B;
-
-  // Adam moments of every restart in LDS (14 floats per group; lane 0 of the group writes, the
-  // group's lanes read: one wavefront, in-order DS operations).  Parameters stay in registers.
-  float* mv = reinterpret_cast<float*>(lossbuf + (ES ? 2 * ak.rpw : 0)) + 16 * grp;
+  float* slots = reinterpret_cast<float*>(lossbuf + (ES ? 2 * ak.rpw : 0));
+  auto local = [&](int i) { return grp + i * groups; };
+  auto runs = [&](int rl) { return grp < groups && rl < ak.rpw && blk * ak.rpw + rl < ak.B; };
+  auto pose_of = [&](int rl) { return (long)pair * ak.B + blk * ak.rpw + rl; };
   float p[7];
 #pragma unroll
   for (int k = 0; k < 7; ++k) p[k] = 0.f;
-  if (active) {
+  for (int i = 0; i < rpg; ++i) {
+    const int rl = local(i);
+    if (!runs(rl)) continue;
+    const long pose = pose_of(rl);
+    float* sl = slots + 24 * rl;
+    if (rpg == 1) {
 #pragma unroll
-    for (int k = 0; k < 4; ++k) p[k] = ak.q[4 * pose + k];
+      for (int k = 0; k < 4; ++k) p[k] = ak.q[4 * pose + k];
 #pragma unroll
-    for (int k = 0; k < 3; ++k) p[4 + k] = ak.t[3 * pose + k];
-    if (j == 0)
-      for (int k = 0; k < 14; ++k) mv[k] = 0.f;
+      for (int k = 0; k < 3; ++k) p[4 + k] = ak.t[3 * pose + k];
+    }
+    if (j == 0) {
+      for (int k = 0; k < 14; ++k) sl[k] = 0.f;
+      if (rpg > 1) {
+        for (int k = 0; k < 4; ++k) sl[14 + k] = ak.q[4 * pose + k];
+        for (int k = 0; k < 3; ++k) sl[18 + k] = ak.t[3 * pose + k];
+      }
+    }
   }
   __syncthreads();
 
@@ -715,13 +730,24 @@ void k_align(FssPairBatch pb, KScore ks, LdsPlan lp, AlignK ak) {
   double b1p = 1.0, b2p = 1.0, last = SE ? (double)INFINITY : 1.0;
   int counter = 0, done = ak.max_steps;
   for (int step = 1; step <= ak.max_steps; ++step) {
-    float loss = 0.f;
-    if (active) {
+    b1p *= (double)ak.b1;
+    b2p *= (double)ak.b2;
+    const float step_size = (float)((double)ak.lr / (1.0 - b1p));
+    const float ibc2s = (float)(1.0 / sqrt(1.0 - b2p));
+    double* buf = lossbuf + (step & 1) * ak.rpw;
+    for (int i = 0; i < rpg; ++i) {
+      const int rl = local(i);
+      if (!runs(rl)) continue;
+      float* sl = slots + 24 * rl;
+      if (rpg > 1) {
+#pragma unroll
+        for (int k = 0; k < 7; ++k) p[k] = sl[14 + k];
+      }
       double qh[4], qn, iq, Rd[9];
       rotation_fp64(p, qh, qn, Rd, &iq);
       float R[9];
 #pragma unroll
-      for (int i = 0; i < 9; ++i) R[i] = (float)Rd[i];
+      for (int k = 0; k < 9; ++k) R[k] = (float)Rd[k];
       Acc sh, ph;
       sh.zero();
       ph.zero();
@@ -730,44 +756,43 @@ void k_align(FssPairBatch pb, KScore ks, LdsPlan lp, AlignK ak) {
         if (MF) shape_accum_mfma<ESP>(st.ref, st.xq, st.T, st.fit, st.M, R, ts, sh);
         else if (EXPD) shape_accum<ESP, true>(st.ref, st.refc, st.N, st.fit, st.M, j, lpp, R, ts, sh);
         else shape_accum_direct<ESP, true>(st.ref, st.N, st.fit, st.M, j, lpp, R, ts, sh);
-        group_sum(sh, lpp, true);
+        pose_sum(sh, lpp, j, true);
         shape_finish(sh, ks.K, ks.alpha, ks.s, true);
       }
       if (PH) {
         pharm_accum<true>(st.ranc, st.rvec, st.poff, st.fanc, st.fvec, st.Mp, j, lpp, R, p + 4, ks.tab, ph);
-        group_sum(ph, lpp, true);
+        pose_sum(ph, lpp, j, true);
       }
       if (SE) {
         const float tr[3] = {p[4] - st.co[0], p[5] - st.co[1], p[6] - st.co[2]};
         shaep_accum<true>(st.srf_r, st.Ns, st.atm_f, st.rb_f, st.Ma, st.srf_f, st.Ms, st.atm_r, st.rb_r, st.Na,
                           j, lpp, R, tr, ks.se, ph);
-        group_sum(ph, lpp, true);
+        pose_sum(ph, lpp, j, true);
       }
       float ws, wp;
       const float S = similarity<float>(ks, sh.V, ph.V, so, SH, PH, ws, wp);
-      loss = 1.f - S;
+      const float loss = 1.f - S;
       float lR[9], lT[3];
 #pragma unroll
-      for (int i = 0; i < 9; ++i) lR[i] = -(ws * sh.G[i] + wp * ph.G[i]);
+      for (int k = 0; k < 9; ++k) lR[k] = -(ws * sh.G[k] + wp * ph.G[k]);
 #pragma unroll
-      for (int i = 0; i < 3; ++i) lT[i] = -(ws * sh.gt[i] + wp * ph.gt[i]);
+      for (int k = 0; k < 3; ++k) lT[k] = -(ws * sh.gt[k] + wp * ph.gt[k]);
       float g[7];
       se3_grad(lR, lT, qh, iq, invB, g);
-      b1p *= (double)ak.b1;
-      b2p *= (double)ak.b2;
-      const float step_size = (float)((double)ak.lr / (1.0 - b1p));
-      const float ibc2s = (float)(1.0 / sqrt(1.0 - b2p));
 #pragma unroll
       for (int k = 0; k < 7; ++k) {
-        const float mk = fmaf(1.f - ak.b1, g[k] - mv[k], mv[k]);
-        const float vk = fmaf(1.f - ak.b2, g[k] * g[k], mv[7 + k] * ak.b2);
+        const float mk = fmaf(1.f - ak.b1, g[k] - sl[k], sl[k]);
+        const float vk = fmaf(1.f - ak.b2, g[k] * g[k], sl[7 + k] * ak.b2);
         p[k] = adam_update(p[k], mk, vk, step_size, ibc2s, ak.eps);
-        if (j == 0) { mv[k] = mk; mv[7 + k] = vk; }
+        if (j == 0) {
+          sl[k] = mk;
+          sl[7 + k] = vk;
+          if (rpg > 1) sl[14 + k] = p[k];
+        }
       }
+      if (ES && j == 0) buf[rl] = (double)loss;
     }
     if (ES) {
-      double* buf = lossbuf + (step & 1) * ak.rpw;
-      if (active && j == 0) buf[grp] = (double)loss;
       __syncthreads();
       double L = 0.0;
       for (int i = 0; i < ak.B; ++i) L += buf[i];
@@ -779,11 +804,15 @@ void k_align(FssPairBatch pb, KScore ks, LdsPlan lp, AlignK ak) {
     }
   }
 
-  if (active && j == 0) {
+  for (int i = 0; i < rpg; ++i) {
+    const int rl = local(i);
+    if (!runs(rl) || j != 0) continue;
+    const long pose = pose_of(rl);
+    const float* src = rpg > 1 ? slots + 24 * rl + 14 : p;
 #pragma unroll
-    for (int k = 0; k < 4; ++k) ak.q[4 * pose + k] = p[k];
+    for (int k = 0; k < 4; ++k) ak.q[4 * pose + k] = src[k];
 #pragma unroll
-    for (int k = 0; k < 3; ++k) ak.t[3 * pose + k] = p[4 + k];
+    for (int k = 0; k < 3; ++k) ak.t[3 * pose + k] = src[4 + k];
   }
   if (threadIdx.x == 0 && blk == 0) ak.n_steps[pair] = done;
 }
@@ -1430,19 +1459,24 @@ int plan_align(const FssPairBatch* batch, const KScore& ks, const FssAlignParams
   const int B = ap->n_restarts, lpp = ap->lanes_per_pose, rpw = ap->restarts_per_wg;
   if (B <= 0) return set_error(FSS_E_INVALID, "n_restarts must be > 0");
   if (ap->max_steps < 0) return set_error(FSS_E_INVALID, "max_steps must be >= 0");
-  if (lpp < 1 || lpp > 64 || (lpp & (lpp - 1))) return set_error(FSS_E_INVALID, "lanes_per_pose must be a power of two in [1, 64]");
   if (rpw < 1) return set_error(FSS_E_INVALID, "restarts_per_wg must be >= 1");
+  if (lpp < 1 || lpp > 64) return set_error(FSS_E_INVALID, "lanes_per_pose must be in [1, 64]");
   if (ap->early_stop && rpw < B)
     return set_error(FSS_E_INVALID, "early stop couples all %d restarts of a pair: restarts_per_wg (%d) must be >= n_restarts", B, rpw);
-  pl.block = ((rpw * lpp + 63) / 64) * 64;
-  if (pl.block > 1024) return set_error(FSS_E_INVALID, "restarts_per_wg * lanes_per_pose = %d exceeds 1024 threads", rpw * lpp);
+  const int groups = ap->groups_per_wg > 0 ? ap->groups_per_wg : rpw;
+  if (groups > rpw) return set_error(FSS_E_INVALID, "groups_per_wg (%d) exceeds restarts_per_wg (%d)", groups, rpw);
+  if ((lpp & (lpp - 1)) && (long)groups * lpp > 64)
+    return set_error(FSS_E_INVALID, "lanes_per_pose must be a power of two unless the workgroup's "
+                                    "groups_per_wg * lanes_per_pose lanes fit one wavefront (64)");
+  pl.block = ((groups * lpp + 63) / 64) * 64;
+  if (pl.block > 1024)
+    return set_error(FSS_E_INVALID, "%d lane groups of %d lanes exceed 1024 threads: lower groups_per_wg", groups, lpp);
   const bool mfma = effective_variant(ks.mode, ap->variant) == 3;
   if (mfma && lpp != 64)
     return set_error(FSS_E_INVALID, "the MFMA inner loop (variant 3) runs one pose per wavefront: lanes_per_pose must be 64");
   pl.lp = plan_lds(batch, ks.mode, mfma);
-  const int groups = pl.block / lpp;
   pl.lds = (size_t)pl.lp.end * sizeof(float4) + (ap->early_stop ? 2 * (size_t)rpw * sizeof(double) : 0) +
-           (size_t)groups * 16 * sizeof(float);
+           (size_t)rpw * 24 * sizeof(float);
   const size_t cap = max_lds_bytes();
   if (pl.lds > cap) return set_error(FSS_E_TOO_LARGE, "clouds need %zu B of LDS, the device allows %zu B", pl.lds, cap);
   AlignK& ak = pl.ak;
@@ -1450,6 +1484,7 @@ int plan_align(const FssPairBatch* batch, const KScore& ks, const FssAlignParams
   ak.lr = ap->lr; ak.b1 = ap->beta1; ak.b2 = ap->beta2; ak.eps = ap->eps;
   ak.es_tol = ap->es_tol; ak.es_pat = ap->es_patience;
   ak.lpp = lpp; ak.rpw = rpw; ak.wg_per_pair = (B + rpw - 1) / rpw;
+  ak.groups = groups;
   ak.order = ap->pair_order;
   pl.grid = (long)batch->n_pairs * ak.wg_per_pair;
   if (pl.grid > 0x7fffffffL) return set_error(FSS_E_INVALID, "grid too large");

@@ -302,6 +302,7 @@ struct FssScreen {
   float* tk_tf = nullptr;              // (A, 2, K, 16)
   MergeJob* d_jobs = nullptr;
   MergeJob* h_jobs = nullptr;          // pinned
+  int32_t* h_steps = nullptr;          // pinned: the last launch's per-pair step counts
   // scheduler state
   std::vector<Slot> slots;
   std::deque<QueryIn> pending;
@@ -311,7 +312,7 @@ struct FssScreen {
   std::condition_variable cv_work, cv_done;
   bool stop = false;
   int64_t next_ticket = 1;
-  std::atomic<int64_t> launches{0}, pairs{0};
+  std::atomic<int64_t> launches{0}, pairs{0}, pair_steps{0};
   std::thread worker;
 };
 
@@ -325,6 +326,7 @@ void free_arena(FssScreen* s) {
     if (p) (void)hipFree(p);
   if (s->h_rng) (void)hipHostFree(s->h_rng);
   if (s->h_jobs) (void)hipHostFree(s->h_jobs);
+  if (s->h_steps) (void)hipHostFree(s->h_steps);
   if (s->stream) (void)hipStreamDestroy(s->stream);
 }
 
@@ -498,10 +500,15 @@ int launch(FssScreen* s, const std::vector<int>& act) {
   hipLaunchKernelGGL(k_topk_merge, dim3(nj), dim3(kMergeThreads), 0, s->stream, s->K, s->d_jobs,
                      s->best_score, s->best_tf);
   e = hipGetLastError();
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(s->h_steps, s->n_steps, sizeof(int32_t) * (size_t)p, hipMemcpyDeviceToHost, s->stream);
   if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
   if (e != hipSuccess) return hip_fail(e, "screening launch");
   s->launches.fetch_add(1);
   s->pairs.fetch_add(p);
+  int64_t steps = 0;
+  for (int i = 0; i < p; ++i) steps += s->h_steps[i];
+  s->pair_steps.fetch_add(steps);
   const auto now = clk::now();
   for (int i = 0, off = 0; i < na; ++i) {
     if (take[i] == 0) continue;
@@ -802,7 +809,7 @@ int fss_screen_create(FssLibrary* lib, const FssScoreParams* sp, const FssScreen
   s->need_pharm = need_pharm;
   s->need_se = need_se;
   s->shape_rows = shape_rows;
-  if (!s->prm.lanes_per_pose) s->prm.lanes_per_pose = 4;
+  if (!s->prm.lanes_per_pose) s->prm.lanes_per_pose = s->B <= 64 ? 64 / s->B : 1;   // one wavefront per pair
   if (!s->prm.restarts_per_wg) s->prm.restarts_per_wg = s->B;
   if (!shape_rows) s->prm.max_query_points = 0;
   if (!need_pharm) s->prm.max_query_pharm = 0;
@@ -838,6 +845,7 @@ int fss_screen_create(FssLibrary* lib, const FssScoreParams* sp, const FssScreen
   if (e == hipSuccess) e = dalloc(&s->tk_tf, 16 * AK);
   if (e == hipSuccess) e = dalloc(&s->d_jobs, s->A);
   if (e == hipSuccess) e = hipHostMalloc((void**)&s->h_jobs, sizeof(MergeJob) * (size_t)s->A, hipHostMallocDefault);
+  if (e == hipSuccess) e = hipHostMalloc((void**)&s->h_steps, sizeof(int32_t) * (size_t)s->P, hipHostMallocDefault);
   if (e == hipSuccess) e = hipMemcpy(s->table, prm->quat_table, sizeof(float) * 4 * s->B, hipMemcpyHostToDevice);
   if (e != hipSuccess) {
     free_arena(s);
@@ -942,10 +950,11 @@ int fss_screen_poll(FssScreen* s, int64_t ticket, int32_t wait_ms, FssHits* out)
   return FSS_OK;
 }
 
-int fss_screen_stats(FssScreen* s, int64_t* launches, int64_t* pairs) {
+int fss_screen_stats(FssScreen* s, int64_t* launches, int64_t* pairs, int64_t* pair_steps) {
   if (!s) return set_error(FSS_E_INVALID, "service is NULL");
   if (launches) *launches = s->launches.load();
   if (pairs) *pairs = s->pairs.load();
+  if (pair_steps) *pair_steps = s->pair_steps.load();
   return FSS_OK;
 }
 

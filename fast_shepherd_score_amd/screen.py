@@ -313,8 +313,10 @@ class ScreenService:
         self.max_active = max(1, min(int(max_active), (hbm_budget_bytes - fixed) // slot,
                                      self.launch_pairs))
         self.table = restart_table(self.B)
-        if lanes_per_pose is None:   # ShaEP poses carry the surface-atom ESP sums: a wider tile
-            lanes_per_pose = 32 if combo else 4
+        if lanes_per_pose is None:
+            # ShaEP poses carry the surface-atom ESP sums: a wider tile; otherwise every restart of
+            # a pair in one wavefront with as many lanes each as fit (10 restarts: 6 lanes each)
+            lanes_per_pose = 32 if combo else max(1, 64 // self.B)
         prm = _lib.FssScreenParams()
         prm.n_restarts, prm.max_steps = self.B, self.opt.max_steps
         prm.lr, prm.beta1, prm.beta2, prm.eps = self.opt.lr, self.opt.beta1, self.opt.beta2, self.opt.eps
@@ -415,16 +417,28 @@ class ScreenService:
             if self._stop:
                 return self._final_launches
             n = ctypes.c_int64()
-            _lib.call("fss_screen_stats", self._h, ctypes.byref(n), None)
+            _lib.call("fss_screen_stats", self._h, ctypes.byref(n), None, None)
             return int(n.value)
 
+    def stats(self) -> dict:
+        """Launches run, (query, library entry) pairs aligned and optimiser steps those pairs ran
+        (summed over pairs), so far (the final counts once closed)."""
+        with self._cv:
+            if self._stop:
+                return dict(self._final_stats)
+            v = [ctypes.c_int64() for _ in range(3)]
+            _lib.call("fss_screen_stats", self._h, *(ctypes.byref(x) for x in v))
+            return {"launches": int(v[0].value), "pairs": int(v[1].value), "pair_steps": int(v[2].value)}
+
     def close(self) -> None:
+        final = self.stats()
         with self._cv:
             if self._stop:
                 return
             n = ctypes.c_int64()
-            _lib.call("fss_screen_stats", self._h, ctypes.byref(n), None)
+            _lib.call("fss_screen_stats", self._h, ctypes.byref(n), None, None)
             self._final_launches = int(n.value)
+            self._final_stats = final
             self._stop = True
             self._cv.notify_all()
         self._poller.join()

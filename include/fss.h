@@ -115,13 +115,19 @@ typedef struct FssAlignParams {
   int32_t early_stop;        /* 1: stop a pair once |dL| <= es_tol held > es_patience steps  */
   float   es_tol;
   int32_t es_patience;
-  int32_t lanes_per_pose;    /* 1..64, power of two: lanes sharing one pose's N x M sum      */
+  int32_t lanes_per_pose;    /* 1..64: lanes sharing one pose's N x M sum; a power of two     *
+                              * unless restarts_per_wg * lanes_per_pose <= 64 (one wavefront) */
   int32_t restarts_per_wg;   /* poses per workgroup; >= B when early_stop is set             */
   const int32_t* pair_order; /* device (n_pairs,) launch order of the pairs (costliest first),
                                 or NULL for 0..n_pairs-1; results are always indexed by pair  */
   int32_t skip_finish;       /* 1: run the optimiser only; call fss_align_finish afterwards    */
   int32_t variant;           /* shape inner loop: 0 default (expanded form), 1 direct form,
-                                2 expanded form (exp2 of 2a.b' + W_a, factor exp2(-|b'|^2))    */
+                                2 expanded form (exp2 of 2a.b' + W_a, factor exp2(-|b'|^2)),
+                                3 split-bf16 MFMA (lanes_per_pose 64)                          */
+  int32_t groups_per_wg;     /* lane groups per workgroup (0: restarts_per_wg); each group runs
+                                ceil(restarts_per_wg / groups_per_wg) restarts in turn every
+                                step, so per-pair early stop can keep all of a pair's restarts in
+                                one workgroup with fewer threads (the MFMA loop at 50 restarts) */
 } FssAlignParams;
 
 typedef struct FssAlignOut {
@@ -386,7 +392,9 @@ typedef struct FssScreenParams {
   int32_t max_active;          /* queries admitted at once (the rest wait in FIFO order) */
   int32_t max_query_points;    /* shape rows of one query */
   int32_t max_query_pharm;     /* pharmacophore rows of one query */
-  int32_t lanes_per_pose, restarts_per_wg, variant;   /* fss_align tiling; 0 = defaults */
+  int32_t lanes_per_pose, restarts_per_wg, variant;   /* fss_align tiling; 0 = defaults (every
+                                                         restart in one wavefront, 64 / n_restarts
+                                                         lanes each, for up to 64 restarts) */
   int32_t seed_flags;          /* FSS_SEED_PHARM / FSS_SEED_SURF: seed from those rows */
   const float* quat_table;     /* (n_restarts, 4) host rotations of the restarts */
   /* FSS_MODE_ESP_COMBO only */
@@ -436,7 +444,9 @@ int fss_screen_submit(FssScreen* svc, const FssQuery* q, int64_t* ticket);
  * submitted with all_scores needs out->all_scores with n_all >= the library size: otherwise
  * FSS_E_INVALID, and the result stays held for a retry with a large enough buffer. */
 int fss_screen_poll(FssScreen* svc, int64_t ticket, int32_t wait_ms, FssHits* out);
-int fss_screen_stats(FssScreen* svc, int64_t* launches, int64_t* pairs);
+/* Launches run, (query, library entry) pairs aligned, and the optimiser steps those pairs ran
+ * (summed over pairs; each step of a pair covers all its restarts).  Any pointer may be NULL. */
+int fss_screen_stats(FssScreen* svc, int64_t* launches, int64_t* pairs, int64_t* pair_steps);
 /* Stops the worker; queries still queued or running fail with FSS_E_CLOSED.  No other call on
  * the service may be in progress or follow. */
 int fss_screen_destroy(FssScreen* svc);

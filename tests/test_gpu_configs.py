@@ -121,10 +121,20 @@ def test_c2_drop_in_default_early_stop_matches_fp64(dev):
     d = load("c2_es")
     assert bool(d["early_stop"])
     pb, spec, pairs = build(d, dev)
-    res = align(pb, spec, to_dev(d["q0"], dev), to_dev(d["t0"], dev), opt_of(d))
+    q0, t0 = to_dev(d["q0"], dev), to_dev(d["t0"], dev)
+    res = align(pb, spec, q0, t0, opt_of(d))
     print(f"steps HIP {res.n_steps.cpu().numpy()} oracle {d['n_steps']}", flush=True)
     check_best("C2 early stop", res, d)
     check_final_poses(res, pairs)
+    # the MFMA loop with every restart of the pair in one workgroup: 50 restarts as wavefronts
+    # that each run several in turn
+    mf = align(pb, spec, q0, t0, opt_of(d), variant=3)
+    assert mf.extra["variant"] == 3 and mf.extra["groups_per_wg"] < 50
+    check_best(f"C2 early stop, MFMA loop ({mf.extra['groups_per_wg']} wavefronts x "
+               f"{50 // mf.extra['groups_per_wg']} restarts)", mf, d)
+    tuned = align(pb, spec, q0, t0, opt_of(d), tune=True)
+    print(f"autotuned with early stop: {tuned.extra}", flush=True)
+    check_best("C2 early stop, autotuned", tuned, d)
 
 
 def test_c1_size_all_paths_match_fp64(dev):

@@ -182,9 +182,10 @@ def test_deterministic_repeat(dev):
     assert torch.equal(a.pose_score, b.pose_score) and torch.equal(a.q, b.q)
 
 
-@pytest.mark.parametrize("lpp,rpw", [(1, 10), (4, 10), (16, 10), (64, 10), (8, 3), (32, 1)])
+@pytest.mark.parametrize("lpp,rpw", [(1, 10), (4, 10), (16, 10), (64, 10), (8, 3), (32, 1), (6, 10), (3, 10), (5, 10)])
 def test_tilings_agree(dev, lpp, rpw):
-    """Every legal (lanes_per_pose, restarts_per_wg) computes the same optimisation."""
+    """Every legal (lanes_per_pose, restarts_per_wg) computes the same optimisation, including
+    lane groups of non-power-of-two width inside one wavefront (6, 3 or 5 lanes x 10 restarts)."""
     n, B = 3, 10
     pb, spec, _ = make_case(_lib.MODE_SHAPE, n, dev, seed=15)
     q, t = random_seeds(n, B, seed=16)
@@ -386,3 +387,23 @@ def test_far_rows_exponent_window(dev, mode):
         scale = max(np.abs(GR).max(), np.abs(GT).max(), 1e-6)
         np.testing.assert_allclose(gR[sl, 0], GR, atol=1e-4 * scale)
         np.testing.assert_allclose(gT[sl, 0], GT, atol=1e-4 * scale)
+
+
+@pytest.mark.parametrize("mode,lpp,groups,variant", [
+    (_lib.MODE_SHAPE_ESP, 16, 5, 2), (_lib.MODE_SHAPE_ESP, 6, 5, 2), (_lib.MODE_SHAPE_ESP, 64, 5, 3),
+    (_lib.MODE_SHAPE, 64, 2, 3), (_lib.MODE_SHAPE_PHARM, 8, 3, 2), (_lib.MODE_PHARM, 4, 4, 2)])
+def test_groups_running_restarts_in_turn_are_bitwise_equal(dev, mode, lpp, groups, variant):
+    """groups_per_wg < restarts_per_wg: each lane group runs several restarts in turn per step
+    (parameters and moments through LDS).  Every restart's arithmetic is unchanged, so the
+    results equal one restart per group bit for bit, early stop included."""
+    print(f"case groups mode={mode} lpp={lpp} groups={groups} variant={variant}", flush=True)
+    n, B = 3, 10
+    pb, spec, _ = make_case(mode, n, dev, seed=81 + mode)
+    q, t = random_seeds(n, B, seed=82)
+    opt = OptimSpec(max_steps=70, early_stop=True)
+    kw = dict(lanes_per_pose=lpp, restarts_per_wg=B, variant=variant)
+    one = align(pb, spec, to_dev(q, dev), to_dev(t, dev), opt, **kw)
+    turn = align(pb, spec, to_dev(q, dev), to_dev(t, dev), opt, groups_per_wg=groups, **kw)
+    assert turn.extra["groups_per_wg"] == groups
+    for name in ("score", "q", "t", "pose_score", "n_steps"):
+        assert torch.equal(getattr(one, name), getattr(turn, name)), name

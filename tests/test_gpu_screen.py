@@ -134,8 +134,8 @@ def test_native_service_topk_is_exact(dev):
     hits = _lib.FssHits()
     hits.capacity, hits.score, hits.index, hits.transform = K, score.ctypes.data, index.ctypes.data, tf.ctypes.data
     _lib.call("fss_screen_poll", svc, ticket.value, -1, ctypes.byref(hits))
-    launches = ctypes.c_int64()
-    _lib.call("fss_screen_stats", svc, ctypes.byref(launches), None)
+    launches, pairs, pair_steps = ctypes.c_int64(), ctypes.c_int64(), ctypes.c_int64()
+    _lib.call("fss_screen_stats", svc, ctypes.byref(launches), ctypes.byref(pairs), ctypes.byref(pair_steps))
     _lib.call("fss_screen_destroy", svc)
     _lib.call("fss_lib_destroy", h)
     assert hits.n == K and launches.value == 4          # ceil(1000 / 333)
@@ -151,6 +151,7 @@ def test_native_service_topk_is_exact(dev):
     r = align(pb, ScoreSpec(), q0, t0, OptimSpec(max_steps=steps, lr=0.1, early_stop=True),
               lanes_per_pose=4, restarts_per_wg=B, order="none", self_ov=self_overlaps(pb, ScoreSpec()))
     s = r.score.cpu().numpy()
+    assert pairs.value == n and pair_steps.value == int(r.n_steps.sum())     # the executed steps
     order = np.lexsort((np.arange(n), -s))[:K]
     np.testing.assert_array_equal(index, order)
     np.testing.assert_array_equal(score, s[order])
