@@ -35,6 +35,8 @@ LIB_PHARM = 2
 LIB_SHAEP = 4
 SCREEN_MAX_TOPK = 1024
 NOALLOC_SCRATCH_BYTES = 8 << 20
+QPOOLS = 4                    # query-row pools: shape rows, pharmacophores, ShaEP surface, atoms
+QPOOL_ROW_BYTES = (16, 36, 16, 20)
 
 _i32p = ctypes.POINTER(ctypes.c_int32)
 _f32p = ctypes.POINTER(ctypes.c_float)
@@ -134,6 +136,7 @@ class FssScreenParams(ctypes.Structure):
         ("quat_table", ctypes.c_void_p),
         ("max_query_surface", ctypes.c_int32), ("max_query_atoms", ctypes.c_int32),
         ("shape_from_surface", ctypes.c_int32),
+        ("query_budget_bytes", ctypes.c_int64),
     ]
 
 
@@ -197,6 +200,14 @@ SIGNATURES = {
     "fss_screen_stats": (ctypes.c_int, [_P, _P, _P, _P]),
     "fss_screen_destroy": (ctypes.c_int, [_P]),
     "fss_plan_launch": (ctypes.c_int, [ctypes.c_int32, _P, ctypes.c_int64, _P]),
+    "fss_plan_query_pools": (ctypes.c_int, [_P, _P, _P]),
+    "fss_rowpool_create": (ctypes.c_int, [ctypes.c_int64, _P]),
+    "fss_rowpool_alloc": (ctypes.c_int, [_P, ctypes.c_int64, _P]),
+    "fss_rowpool_free": (ctypes.c_int, [_P, ctypes.c_int64, ctypes.c_int64]),
+    "fss_rowpool_stats": (ctypes.c_int, [_P, _P, _P]),
+    "fss_rowpool_destroy": (ctypes.c_int, [_P]),
+    "fss_admit_fifo": (ctypes.c_int, [_P, ctypes.c_int32, ctypes.c_int32, _P, ctypes.c_int32, _P, _P]),
+    "fss_screen_admission": (ctypes.c_int, [_P, _P, _P, _P]),
 }
 
 _lock = threading.Lock()

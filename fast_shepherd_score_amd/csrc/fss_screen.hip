@@ -24,6 +24,7 @@
 #include <chrono>
 #include <condition_variable>
 #include <deque>
+#include <map>
 #include <mutex>
 #include <string>
 #include <thread>
@@ -246,18 +247,123 @@ __global__ __launch_bounds__(kMergeThreads) void k_topk_merge(int K, const Merge
   }
 }
 
+// ------------------------------------------------------------------------------------------------
+// Query memory.  An admitted query's rows live in four device pools (shape rows; pharmacophore
+// anchors, vectors and types; ShaEP surface rows; ShaEP atoms and radii), each a run of rows that
+// a first-fit allocator hands out in contiguous ranges and merges back on release.  A query is
+// admitted only when every pool has a range for its rows, so admission is by the memory it needs.
+// ------------------------------------------------------------------------------------------------
+class RowPool {
+ public:
+  void reset(int64_t rows) {
+    free_.clear();
+    cap_ = rows;
+    if (rows > 0) free_[0] = rows;
+  }
+  int64_t capacity() const { return cap_; }
+  // First free range of n rows (n == 0 always fits at row 0).
+  bool alloc(int64_t n, int64_t* first) {
+    if (n <= 0) { *first = 0; return true; }
+    for (auto it = free_.begin(); it != free_.end(); ++it) {
+      if (it->second < n) continue;
+      *first = it->first;
+      const int64_t rest = it->second - n, at = it->first + n;
+      free_.erase(it);
+      if (rest) free_[at] = rest;
+      return true;
+    }
+    return false;
+  }
+  void release(int64_t first, int64_t n) {
+    if (n <= 0) return;
+    auto it = free_.emplace(first, n).first;
+    auto nx = std::next(it);
+    if (nx != free_.end() && it->first + it->second == nx->first) {
+      it->second += nx->second;
+      free_.erase(nx);
+    }
+    if (it != free_.begin()) {
+      auto pv = std::prev(it);
+      if (pv->first + pv->second == it->first) {
+        pv->second += it->second;
+        free_.erase(it);
+      }
+    }
+  }
+  // True when [first, first + n) lies in the pool and overlaps no free range.
+  bool allocated(int64_t first, int64_t n) const {
+    if (n <= 0) return n == 0;
+    if (first < 0 || first + n > cap_) return false;
+    auto it = free_.upper_bound(first);
+    if (it != free_.end() && it->first < first + n) return false;
+    if (it != free_.begin()) {
+      auto pv = std::prev(it);
+      if (pv->first + pv->second > first) return false;
+    }
+    return true;
+  }
+  void stats(int64_t* free_rows, int64_t* largest) const {
+    int64_t f = 0, l = 0;
+    for (const auto& kv : free_) { f += kv.second; l = std::max(l, kv.second); }
+    if (free_rows) *free_rows = f;
+    if (largest) *largest = l;
+  }
+
+ private:
+  std::map<int64_t, int64_t> free_;   // first row -> length of each free range
+  int64_t cap_ = 0;
+};
+
+// Admission, oldest first: query k of n needs rows[k * np + c] rows of pool c; it is admitted
+// while a slot is free and every pool has a range for it.  The first query that does not fit
+// ends the round (no later query overtakes it).  first[k * np + c] receives the ranges; returns
+// the number admitted.
+int admit_fifo(RowPool* const* pools, int np, int n, const int64_t* rows, int free_slots, int64_t* first) {
+  int k = 0;
+  for (; k < n && k < free_slots; ++k) {
+    int c = 0;
+    for (; c < np; ++c)
+      if (!pools[c]->alloc(rows[(size_t)k * np + c], &first[(size_t)k * np + c])) break;
+    if (c < np) {
+      for (int d = 0; d < c; ++d) pools[d]->release(first[(size_t)k * np + d], rows[(size_t)k * np + d]);
+      break;
+    }
+  }
+  return k;
+}
+
+enum QueryPool { kQPts, kQPharm, kQSurf, kQAtoms, kQPools };
+constexpr int64_t kQRowBytes[kQPools] = {16, 36, 16, 20};   // bytes of one row in each pool
+
 struct QueryIn {
   int64_t ticket = 0;
   std::vector<float> pts, anc, vec, srf, atm, rad;
   std::vector<int32_t> typ;
   int32_t np = 0, nph = 0, ns = 0, na = 0;
   bool want_all = false;
+  bool mem_wait = false;               // has waited for pool room with a slot free
   clk::time_point t_submit;
 };
+
+// The channels a mode reads, as fss_screen_create sizes them.
+struct Channels {
+  bool shape, pharm, se, shape_rows;
+};
+
+Channels channels_of(int mode, int shape_from_surface) {
+  Channels c;
+  c.shape = mode != FSS_MODE_PHARM;
+  c.pharm = mode == FSS_MODE_PHARM || mode == FSS_MODE_SHAPE_PHARM;
+  c.se = mode == FSS_MODE_ESP_COMBO;
+  c.shape_rows = c.shape && !(c.se && shape_from_surface);
+  return c;
+}
 
 struct Slot {
   bool used = false;
   QueryIn q;
+  int64_t first[kQPools] = {0, 0, 0, 0};   // the query's row ranges in the pools
+  int64_t rows[kQPools] = {0, 0, 0, 0};
   int64_t next = 0, lib_n = 0;
   int32_t filled = 0, buf = 0;
   std::vector<double> all;             // every entry's best score, when the query asked
@@ -277,7 +383,15 @@ struct Result {
 // Range tables of one launch, each (P, 2) = [first row, count] per pair.
 enum RngTable { kRef, kFit, kRefPh, kFitPh, kRefSrf, kFitSrf, kRefAtm, kFitAtm, kRngTables };
 
+static_assert(kQPools == FSS_QPOOLS && kQPts == FSS_QPOOL_POINTS && kQPharm == FSS_QPOOL_PHARM &&
+                  kQSurf == FSS_QPOOL_SURFACE && kQAtoms == FSS_QPOOL_ATOMS,
+              "pool numbering of fss.h");
+
 }  // namespace
+
+struct FssRowPool {
+  RowPool pool;
+};
 
 struct FssScreen {
   FssLibrary* lib = nullptr;
@@ -294,9 +408,12 @@ struct FssScreen {
   int32_t *best_idx = nullptr, *n_steps = nullptr;
   int32_t *d_rng = nullptr;            // kRngTables x (P, 2), see RngTable
   int32_t* h_rng = nullptr;            // pinned host staging of d_rng
-  float *qpts = nullptr, *qanc = nullptr, *qvec = nullptr;
+  float *qpts = nullptr, *qanc = nullptr, *qvec = nullptr;   // query pools (rows of QueryPool)
   int32_t* qtyp = nullptr;
   float *qsrf = nullptr, *qatm = nullptr, *qrad = nullptr;
+  RowPool pools[kQPools];              // guarded by mu
+  int64_t pool_rows[kQPools] = {0, 0, 0, 0};
+  std::atomic<int64_t> admitted_max{0}, memory_waits{0};
   double* tk_s = nullptr;              // (A, 2, K)
   int64_t* tk_i = nullptr;             // (A, 2, K)
   float* tk_tf = nullptr;              // (A, 2, K, 16)
@@ -337,34 +454,50 @@ void finish(FssScreen* s, int64_t ticket, Result&& r) {
   s->cv_done.notify_all();
 }
 
-// Copies the query rows into the slot's arena (stream-ordered; the slot keeps the host vectors
-// alive until the next synchronisation).
+// Frees slot i: its pool ranges go back to the pools and its query is dropped; returns the
+// query's ticket.
+int64_t release_slot(FssScreen* s, int i) {
+  Slot& sl = s->slots[i];
+  const int64_t ticket = sl.q.ticket;
+  {
+    std::lock_guard<std::mutex> lk(s->mu);
+    for (int c = 0; c < kQPools; ++c) {
+      s->pools[c].release(sl.first[c], sl.rows[c]);
+      sl.first[c] = sl.rows[c] = 0;
+    }
+  }
+  sl.used = false;
+  sl.q = QueryIn();
+  return ticket;
+}
+
+// Copies the query rows into the slot's pool ranges (stream-ordered; the slot keeps the host
+// vectors alive until the next synchronisation).
 int admit(FssScreen* s, int slot) {
   Slot& sl = s->slots[slot];
-  const size_t mq = (size_t)s->prm.max_query_points, mp = (size_t)s->prm.max_query_pharm;
+  const int64_t* f = sl.first;
   hipError_t e = hipSuccess;
   if (s->shape_rows && sl.q.np)
-    e = hipMemcpyAsync(s->qpts + 4 * mq * slot, sl.q.pts.data(), sizeof(float) * 4 * sl.q.np,
+    e = hipMemcpyAsync(s->qpts + 4 * f[kQPts], sl.q.pts.data(), sizeof(float) * 4 * sl.q.np,
                        hipMemcpyHostToDevice, s->stream);
   if (e == hipSuccess && s->need_pharm && sl.q.nph) {
-    e = hipMemcpyAsync(s->qanc + 4 * mp * slot, sl.q.anc.data(), sizeof(float) * 4 * sl.q.nph,
+    e = hipMemcpyAsync(s->qanc + 4 * f[kQPharm], sl.q.anc.data(), sizeof(float) * 4 * sl.q.nph,
                        hipMemcpyHostToDevice, s->stream);
     if (e == hipSuccess)
-      e = hipMemcpyAsync(s->qvec + 4 * mp * slot, sl.q.vec.data(), sizeof(float) * 4 * sl.q.nph,
+      e = hipMemcpyAsync(s->qvec + 4 * f[kQPharm], sl.q.vec.data(), sizeof(float) * 4 * sl.q.nph,
                          hipMemcpyHostToDevice, s->stream);
     if (e == hipSuccess)
-      e = hipMemcpyAsync(s->qtyp + mp * slot, sl.q.typ.data(), sizeof(int32_t) * sl.q.nph,
+      e = hipMemcpyAsync(s->qtyp + f[kQPharm], sl.q.typ.data(), sizeof(int32_t) * sl.q.nph,
                          hipMemcpyHostToDevice, s->stream);
   }
   if (e == hipSuccess && s->need_se) {
-    const size_t ms = (size_t)s->prm.max_query_surface, ma = (size_t)s->prm.max_query_atoms;
-    e = hipMemcpyAsync(s->qsrf + 4 * ms * slot, sl.q.srf.data(), sizeof(float) * 4 * sl.q.ns,
+    e = hipMemcpyAsync(s->qsrf + 4 * f[kQSurf], sl.q.srf.data(), sizeof(float) * 4 * sl.q.ns,
                        hipMemcpyHostToDevice, s->stream);
     if (e == hipSuccess)
-      e = hipMemcpyAsync(s->qatm + 4 * ma * slot, sl.q.atm.data(), sizeof(float) * 4 * sl.q.na,
+      e = hipMemcpyAsync(s->qatm + 4 * f[kQAtoms], sl.q.atm.data(), sizeof(float) * 4 * sl.q.na,
                          hipMemcpyHostToDevice, s->stream);
     if (e == hipSuccess)
-      e = hipMemcpyAsync(s->qrad + ma * slot, sl.q.rad.data(), sizeof(float) * sl.q.na,
+      e = hipMemcpyAsync(s->qrad + f[kQAtoms], sl.q.rad.data(), sizeof(float) * sl.q.na,
                          hipMemcpyHostToDevice, s->stream);
   }
   if (e != hipSuccess) return hip_fail(e, "query upload");
@@ -399,10 +532,7 @@ int complete_finished(FssScreen* s, const std::vector<int>& act, clk::time_point
     r.queue_ms = ms_since(sl.q.t_submit, sl.t_start);
     r.latency_ms = ms_since(sl.q.t_submit, now);
     r.all.swap(sl.all);
-    const int64_t ticket = sl.q.ticket;
-    sl.used = false;
-    sl.q = QueryIn();
-    finish(s, ticket, std::move(r));
+    finish(s, release_slot(s, i), std::move(r));
   }
   return FSS_OK;
 }
@@ -413,8 +543,6 @@ int launch(FssScreen* s, const std::vector<int>& act) {
   std::vector<int64_t> rem(na), take(na);
   for (int i = 0; i < na; ++i) rem[i] = s->slots[act[i]].lib_n - s->slots[act[i]].next;
   fss_plan_launch(na, rem.data(), s->P, take.data());
-  const size_t mq = (size_t)s->prm.max_query_points, mp = (size_t)s->prm.max_query_pharm;
-  const size_t ms = (size_t)s->prm.max_query_surface, ma = (size_t)s->prm.max_query_atoms;
   int32_t* tab[kRngTables];
   for (int k = 0; k < kRngTables; ++k) tab[k] = s->h_rng + 2 * (size_t)s->P * k;
   // pair p of table k <- (first row, count); a channel the mode does not use gets empty ranges
@@ -440,13 +568,13 @@ int launch(FssScreen* s, const std::vector<int>& act) {
       for (int64_t j = 0; j < take[i]; ++j, ++p) {
         const size_t e = 2 * (size_t)(sl.next + j);
         const bool sh = s->shape_rows, ph = s->need_pharm, se = s->need_se;
-        set(kRef, p, (int32_t)(mq * act[i]), sh ? sl.q.np : 0);
+        set(kRef, p, (int32_t)sl.first[kQPts], sh ? sl.q.np : 0);
         set(kFit, p, sh ? L->h_rng[e] : 0, sh ? L->h_rng[e + 1] : 0);
-        set(kRefPh, p, (int32_t)(mp * act[i]), ph ? sl.q.nph : 0);
+        set(kRefPh, p, (int32_t)sl.first[kQPharm], ph ? sl.q.nph : 0);
         set(kFitPh, p, ph ? L->h_prng[e] : 0, ph ? L->h_prng[e + 1] : 0);
-        set(kRefSrf, p, (int32_t)(ms * act[i]), se ? sl.q.ns : 0);
+        set(kRefSrf, p, (int32_t)sl.first[kQSurf], se ? sl.q.ns : 0);
         set(kFitSrf, p, se ? L->h_srng[e] : 0, se ? L->h_srng[e + 1] : 0);
-        set(kRefAtm, p, (int32_t)(ma * act[i]), se ? sl.q.na : 0);
+        set(kRefAtm, p, (int32_t)sl.first[kQAtoms], se ? sl.q.na : 0);
         set(kFitAtm, p, se ? L->h_arng[e] : 0, se ? L->h_arng[e + 1] : 0);
       }
     }
@@ -527,16 +655,53 @@ int launch(FssScreen* s, const std::vector<int>& act) {
 }
 
 void fail_active(FssScreen* s, int rc, const std::string& msg) {
-  for (Slot& sl : s->slots) {
-    if (!sl.used) continue;
+  for (int i = 0; i < s->A; ++i) {
+    if (!s->slots[i].used) continue;
     Result r;
     r.rc = rc;
     r.err = msg;
-    const int64_t ticket = sl.q.ticket;
-    sl.used = false;
-    sl.q = QueryIn();
-    finish(s, ticket, std::move(r));
+    finish(s, release_slot(s, i), std::move(r));
   }
+}
+
+// Admits queued queries, oldest first, into free slots while the pools have room for their rows
+// (called with s->mu held).  Returns the slots admitted.
+std::vector<int> admit_pending(FssScreen* s) {
+  std::vector<int> free_idx, admitted;
+  for (int i = 0; i < s->A; ++i)
+    if (!s->slots[i].used) free_idx.push_back(i);
+  const int n = (int)std::min(free_idx.size(), s->pending.size());
+  if (n == 0) return admitted;
+  std::vector<int64_t> rows((size_t)n * kQPools), first((size_t)n * kQPools);
+  for (int k = 0; k < n; ++k) {
+    const QueryIn& q = s->pending[k];
+    int64_t* r = &rows[(size_t)k * kQPools];
+    r[kQPts] = s->shape_rows ? q.np : 0;
+    r[kQPharm] = s->need_pharm ? q.nph : 0;
+    r[kQSurf] = s->need_se ? q.ns : 0;
+    r[kQAtoms] = s->need_se ? q.na : 0;
+  }
+  RowPool* pp[kQPools];
+  for (int c = 0; c < kQPools; ++c) pp[c] = &s->pools[c];
+  const int got = admit_fifo(pp, kQPools, n, rows.data(), n, first.data());
+  if (got < n && !s->pending[got].mem_wait) {   // a slot is free, the pools are not
+    s->pending[got].mem_wait = true;
+    s->memory_waits.fetch_add(1);
+  }
+  for (int k = 0; k < got; ++k) {
+    Slot& sl = s->slots[free_idx[k]];
+    sl.q = std::move(s->pending.front());
+    s->pending.pop_front();
+    for (int c = 0; c < kQPools; ++c) {
+      sl.first[c] = first[(size_t)k * kQPools + c];
+      sl.rows[c] = rows[(size_t)k * kQPools + c];
+    }
+    sl.used = true;
+    admitted.push_back(free_idx[k]);
+  }
+  const int64_t held = s->A - (int64_t)free_idx.size() + got;
+  if (held > s->admitted_max.load()) s->admitted_max.store(held);
+  return admitted;
 }
 
 void worker_main(FssScreen* s) {
@@ -549,19 +714,15 @@ void worker_main(FssScreen* s) {
     std::vector<int> admitted;
     {
       std::unique_lock<std::mutex> lk(s->mu);
-      for (int i = 0; i < s->A && !s->pending.empty(); ++i) {
-        if (s->slots[i].used) continue;
-        s->slots[i].q = std::move(s->pending.front());
-        s->pending.pop_front();
-        s->slots[i].used = true;
-        admitted.push_back(i);
-      }
+      if (!s->stop) admitted = admit_pending(s);
       for (int i = 0; i < s->A; ++i)
         if (s->slots[i].used) act.push_back(i);
       // oldest query first: launches are filled in submission (ticket) order
       std::sort(act.begin(), act.end(), [&](int x, int y) { return s->slots[x].q.ticket < s->slots[y].q.ticket; });
       if (act.empty()) {
         if (s->stop) break;
+        // with every slot free the pools are whole and hold any accepted query, so a query is
+        // left queued here only while the service stops
         s->cv_work.wait(lk, [&] { return s->stop || !s->pending.empty(); });
         continue;
       }
@@ -572,10 +733,7 @@ void worker_main(FssScreen* s) {
         Result r;
         r.rc = FSS_E_INVALID;
         r.err = fss::last_error();
-        const int64_t ticket = s->slots[i].q.ticket;
-        s->slots[i].used = false;
-        s->slots[i].q = QueryIn();
-        finish(s, ticket, std::move(r));
+        finish(s, release_slot(s, i), std::move(r));
         act.erase(std::remove(act.begin(), act.end(), i), act.end());
       }
     }
@@ -613,6 +771,80 @@ int fss_plan_launch(int32_t n, const int64_t* remaining, int64_t capacity, int64
     take[i] = std::min(std::max<int64_t>(remaining[i], 0), left);
     left -= take[i];
   }
+  return FSS_OK;
+}
+
+int fss_plan_query_pools(const FssScoreParams* sp, const FssScreenParams* prm, int64_t rows[FSS_QPOOLS]) {
+  if (!sp || !prm || !rows) return set_error(FSS_E_INVALID, "NULL pool plan arguments");
+  if (prm->max_active < 1) return set_error(FSS_E_INVALID, "max_active must be >= 1");
+  if (prm->query_budget_bytes < 0) return set_error(FSS_E_INVALID, "query_budget_bytes must be >= 0");
+  const Channels ch = channels_of(sp->mode, prm->shape_from_surface);
+  const int64_t need[kQPools] = {ch.shape_rows ? std::max(prm->max_query_points, 0) : 0,
+                                 ch.pharm ? std::max(prm->max_query_pharm, 0) : 0,
+                                 ch.se ? std::max(prm->max_query_surface, 0) : 0,
+                                 ch.se ? std::max(prm->max_query_atoms, 0) : 0};
+  const int64_t A = prm->max_active;
+  int64_t one = 0;
+  for (int c = 0; c < kQPools; ++c) one += need[c] * kQRowBytes[c];
+  const int64_t budget = prm->query_budget_bytes;
+  if (budget == 0 || budget >= A * one) {
+    for (int c = 0; c < kQPools; ++c) rows[c] = A * need[c];
+  } else {
+    if (budget < one)
+      return set_error(FSS_E_INVALID, "query_budget_bytes %lld does not hold one query of the maximum size (%lld bytes)",
+                       (long long)budget, (long long)one);
+    // m maximum queries' worth of every pool, m = budget / one in [1, A)
+    for (int c = 0; c < kQPools; ++c)
+      rows[c] = std::min(A * need[c], std::max(need[c], (int64_t)((__int128)budget * need[c] / one)));
+  }
+  for (int c = 0; c < kQPools; ++c)
+    if (rows[c] > INT32_MAX) return set_error(FSS_E_INVALID, "query pool %d exceeds 2^31 rows", c);
+  return FSS_OK;
+}
+
+int fss_rowpool_create(int64_t rows, FssRowPool** out) {
+  if (!out || rows < 0) return set_error(FSS_E_INVALID, "bad row pool arguments");
+  *out = new FssRowPool();
+  (*out)->pool.reset(rows);
+  return FSS_OK;
+}
+
+int fss_rowpool_alloc(FssRowPool* p, int64_t n, int64_t* first) {
+  if (!p || !first || n < 0) return set_error(FSS_E_INVALID, "bad row pool arguments");
+  return p->pool.alloc(n, first) ? FSS_OK : FSS_PENDING;
+}
+
+int fss_rowpool_free(FssRowPool* p, int64_t first, int64_t n) {
+  if (!p || !p->pool.allocated(first, n))
+    return set_error(FSS_E_INVALID, "rows [%lld, %lld) are not an allocated range", (long long)first,
+                     (long long)(first + n));
+  p->pool.release(first, n);
+  return FSS_OK;
+}
+
+int fss_rowpool_stats(const FssRowPool* p, int64_t* free_rows, int64_t* largest_free) {
+  if (!p) return set_error(FSS_E_INVALID, "row pool is NULL");
+  p->pool.stats(free_rows, largest_free);
+  return FSS_OK;
+}
+
+int fss_rowpool_destroy(FssRowPool* p) {
+  delete p;
+  return FSS_OK;
+}
+
+int fss_admit_fifo(FssRowPool* const* pools, int32_t n_pools, int32_t n, const int64_t* rows,
+                   int32_t free_slots, int64_t* first, int32_t* admitted) {
+  if (!admitted || n < 0 || n_pools < 0 || (n > 0 && n_pools > 0 && (!pools || !rows || !first)))
+    return set_error(FSS_E_INVALID, "bad admission arguments");
+  std::vector<RowPool*> pp((size_t)n_pools);
+  for (int c = 0; c < n_pools; ++c) {
+    if (!pools[c]) return set_error(FSS_E_INVALID, "row pool %d is NULL", c);
+    pp[c] = &pools[c]->pool;
+  }
+  for (int64_t k = 0; k < (int64_t)n * n_pools; ++k)
+    if (rows[k] < 0) return set_error(FSS_E_INVALID, "negative row count");
+  *admitted = admit_fifo(pp.data(), n_pools, n, rows, std::max(free_slots, 0), first);
   return FSS_OK;
 }
 
@@ -773,10 +1005,8 @@ int fss_screen_create(FssLibrary* lib, const FssScoreParams* sp, const FssScreen
   if (!lib || !sp || !prm || !out) return set_error(FSS_E_INVALID, "NULL screening arguments");
   *out = nullptr;
   if (sp->mode < 0 || sp->mode > FSS_MODE_ESP_COMBO) return set_error(FSS_E_INVALID, "unknown mode %d", sp->mode);
-  const bool need_shape = sp->mode != FSS_MODE_PHARM;
-  const bool need_pharm = sp->mode == FSS_MODE_PHARM || sp->mode == FSS_MODE_SHAPE_PHARM;
-  const bool need_se = sp->mode == FSS_MODE_ESP_COMBO;
-  const bool shape_rows = need_shape && !(need_se && prm->shape_from_surface);
+  const Channels ch = channels_of(sp->mode, prm->shape_from_surface);
+  const bool need_shape = ch.shape, need_pharm = ch.pharm, need_se = ch.se, shape_rows = ch.shape_rows;
   if (shape_rows && !(lib->channels & FSS_LIB_SHAPE)) return set_error(FSS_E_INVALID, "the mode needs the library's shape channel");
   if (need_pharm && !(lib->channels & FSS_LIB_PHARM)) return set_error(FSS_E_INVALID, "the mode needs the library's pharmacophore channel");
   if (need_se && !(lib->channels & FSS_LIB_SHAEP)) return set_error(FSS_E_INVALID, "the mode needs the library's ShaEP channel");
@@ -791,10 +1021,8 @@ int fss_screen_create(FssLibrary* lib, const FssScoreParams* sp, const FssScreen
     return set_error(FSS_E_INVALID, "max_query_surface and max_query_atoms must be >= 1");
   if (need_pharm && prm->max_query_pharm < 1) return set_error(FSS_E_INVALID, "max_query_pharm must be >= 1");
   if (!prm->quat_table) return set_error(FSS_E_INVALID, "quat_table is NULL");
-  if ((int64_t)prm->max_active * prm->max_query_points > INT32_MAX || (int64_t)prm->max_active * prm->max_query_pharm > INT32_MAX ||
-      (need_se && ((int64_t)prm->max_active * prm->max_query_surface > INT32_MAX ||
-                   (int64_t)prm->max_active * prm->max_query_atoms > INT32_MAX)))
-    return set_error(FSS_E_INVALID, "query arena exceeds 2^31 rows");
+  int64_t pool_rows[kQPools];
+  if (int rc = fss_plan_query_pools(sp, prm, pool_rows)) return rc;
   DeviceGuard dg(lib->device);
   if (dg.err != hipSuccess) return hip_fail(dg.err, "fss_screen_create: device");
   FssScreen* s = new FssScreen();
@@ -829,17 +1057,17 @@ int fss_screen_create(FssLibrary* lib, const FssScoreParams* sp, const FssScreen
   if (e == hipSuccess) e = dalloc(&s->self_ov, 4 * (int64_t)s->P);
   if (e == hipSuccess) e = dalloc(&s->d_rng, 2 * kRngTables * (int64_t)s->P);
   if (e == hipSuccess) e = hipHostMalloc((void**)&s->h_rng, sizeof(int32_t) * 2 * kRngTables * (size_t)s->P, hipHostMallocDefault);
-  if (e == hipSuccess && shape_rows) e = dalloc(&s->qpts, 4 * (int64_t)s->A * s->prm.max_query_points);
-  if (e == hipSuccess && need_se) {
-    e = dalloc(&s->qsrf, 4 * (int64_t)s->A * s->prm.max_query_surface);
-    if (e == hipSuccess) e = dalloc(&s->qatm, 4 * (int64_t)s->A * s->prm.max_query_atoms);
-    if (e == hipSuccess) e = dalloc(&s->qrad, (int64_t)s->A * s->prm.max_query_atoms);
+  for (int c = 0; c < kQPools; ++c) {
+    s->pool_rows[c] = pool_rows[c];
+    s->pools[c].reset(pool_rows[c]);
   }
-  if (e == hipSuccess && need_pharm) {
-    e = dalloc(&s->qanc, 4 * (int64_t)s->A * s->prm.max_query_pharm);
-    if (e == hipSuccess) e = dalloc(&s->qvec, 4 * (int64_t)s->A * s->prm.max_query_pharm);
-    if (e == hipSuccess) e = dalloc(&s->qtyp, (int64_t)s->A * s->prm.max_query_pharm);
-  }
+  if (e == hipSuccess) e = dalloc(&s->qpts, 4 * pool_rows[kQPts]);
+  if (e == hipSuccess) e = dalloc(&s->qsrf, 4 * pool_rows[kQSurf]);
+  if (e == hipSuccess) e = dalloc(&s->qatm, 4 * pool_rows[kQAtoms]);
+  if (e == hipSuccess) e = dalloc(&s->qrad, pool_rows[kQAtoms]);
+  if (e == hipSuccess) e = dalloc(&s->qanc, 4 * pool_rows[kQPharm]);
+  if (e == hipSuccess) e = dalloc(&s->qvec, 4 * pool_rows[kQPharm]);
+  if (e == hipSuccess) e = dalloc(&s->qtyp, pool_rows[kQPharm]);
   if (e == hipSuccess) e = dalloc(&s->tk_s, AK);
   if (e == hipSuccess) e = dalloc(&s->tk_i, AK);
   if (e == hipSuccess) e = dalloc(&s->tk_tf, 16 * AK);
@@ -955,6 +1183,15 @@ int fss_screen_stats(FssScreen* s, int64_t* launches, int64_t* pairs, int64_t* p
   if (launches) *launches = s->launches.load();
   if (pairs) *pairs = s->pairs.load();
   if (pair_steps) *pair_steps = s->pair_steps.load();
+  return FSS_OK;
+}
+
+int fss_screen_admission(FssScreen* s, int64_t* pool_rows, int64_t* max_admitted, int64_t* memory_waits) {
+  if (!s) return set_error(FSS_E_INVALID, "service is NULL");
+  if (pool_rows)
+    for (int c = 0; c < kQPools; ++c) pool_rows[c] = s->pool_rows[c];
+  if (max_admitted) *max_admitted = s->admitted_max.load();
+  if (memory_waits) *memory_waits = s->memory_waits.load();
   return FSS_OK;
 }
 

@@ -7,8 +7,11 @@ aligner calls; here the library is packed once into HBM and served by the native
 service of libfss_hip.so (``fss_lib_*`` / ``fss_screen_*`` in include/fss.h), which this module
 wraps:
 
-* admission: a query takes one of ``max_active`` slots of a device arena reserved when the
-  service is created (sized from the HBM budget); the rest wait in FIFO order;
+* admission by memory: a query takes one of ``max_active`` top-k slots and, for its own rows,
+  ranges of four device row pools (shape rows, pharmacophores, ShaEP surface, ShaEP atoms)
+  reserved when the service is created and sized from the HBM budget
+  (:func:`plan_query_pools`); queries wait in FIFO order until a slot and room for their rows
+  are free, and no later query overtakes a waiting one (:func:`admit_fifo`);
 * continuous batching: every launch is filled from the admitted queries' remaining library
   ranges, oldest first (:func:`plan_launch` / ``fss_plan_launch``): the tail of one query and
   the head of the next share a launch, and a query that arrives while others are on the GPU
@@ -232,15 +235,97 @@ def plan_launch(remaining: Sequence[int], capacity: int) -> List[int]:
     return take
 
 
+def query_row_bytes(max_query_points: int, max_query_pharm: int, max_query_surface: int = 0,
+                    max_query_atoms: int = 0) -> int:
+    """Device bytes of one query of the maximum size in the service's row pools."""
+    counts = (max_query_points, max_query_pharm, max_query_surface, max_query_atoms)
+    return sum(int(n) * b for n, b in zip(counts, _lib.QPOOL_ROW_BYTES))
+
+
 def arena_bytes(launch_pairs: int, restarts: int, max_active: int, top_k: int,
                 max_query_points: int, max_query_pharm: int, max_query_surface: int = 0,
-                max_query_atoms: int = 0) -> int:
-    """Device bytes a service reserves at creation (fss_screen_create's arena)."""
+                max_query_atoms: int = 0, query_budget_bytes: int = 0) -> int:
+    """Device bytes a service reserves at creation (fss_screen_create's arena), at most: the
+    launch arena, ``max_active`` top-k slots and the query row pools (``query_budget_bytes``, or
+    ``max_active`` queries of the maximum size when that is 0 or larger)."""
     P, B, A, K = launch_pairs, restarts, max_active, top_k
     per_pair = B * (4 + 3) * 4 + B * 8 + 8 + 4 + 64 + 4 + 32 + 64
-    per_slot = (max_query_points * 16 + max_query_pharm * 36 + max_query_surface * 16 +
-                max_query_atoms * 20 + 2 * K * (8 + 8 + 64) + 64)
-    return P * per_pair + A * per_slot + B * 16
+    per_slot = 2 * K * (8 + 8 + 64) + 64
+    rows = A * query_row_bytes(max_query_points, max_query_pharm, max_query_surface, max_query_atoms)
+    if query_budget_bytes:
+        rows = min(rows, int(query_budget_bytes))
+    return P * per_pair + A * per_slot + rows + B * 16
+
+
+def plan_query_pools(mode: int, max_active: int, max_query_points: int = 0, max_query_pharm: int = 0,
+                     max_query_surface: int = 0, max_query_atoms: int = 0,
+                     query_budget_bytes: int = 0, shape_from_surface: bool = False) -> List[int]:
+    """Rows of the four query pools a service with these parameters reserves (native
+    ``fss_plan_query_pools``; raises when the budget does not hold one query of the maximum
+    size)."""
+    sp = _lib.FssScoreParams()
+    sp.mode = int(mode)
+    prm = _lib.FssScreenParams()
+    prm.max_active = int(max_active)
+    prm.max_query_points, prm.max_query_pharm = int(max_query_points), int(max_query_pharm)
+    prm.max_query_surface, prm.max_query_atoms = int(max_query_surface), int(max_query_atoms)
+    prm.query_budget_bytes, prm.shape_from_surface = int(query_budget_bytes), int(shape_from_surface)
+    rows = (ctypes.c_int64 * _lib.QPOOLS)()
+    _lib.call("fss_plan_query_pools", _lib.byref(sp), _lib.byref(prm), rows)
+    return [int(x) for x in rows]
+
+
+class RowPool:
+    """The service's first-fit pool of device rows, host side only (native ``fss_rowpool_*``):
+    contiguous ranges handed out first-fit and merged with free neighbours on release."""
+
+    def __init__(self, rows: int):
+        self._h = ctypes.c_void_p()
+        _lib.call("fss_rowpool_create", ctypes.c_int64(int(rows)), ctypes.byref(self._h))
+
+    def alloc(self, n: int) -> Optional[int]:
+        """First row of a free range of ``n`` rows, or None when none is long enough."""
+        first = ctypes.c_int64()
+        rc = _lib.load().fss_rowpool_alloc(self._h, int(n), ctypes.byref(first))
+        if rc == _lib.PENDING:
+            return None
+        if rc:
+            raise _lib.FssError("fss_rowpool_alloc", rc, _lib.load().fss_last_error().decode())
+        return int(first.value)
+
+    def free(self, first: int, n: int) -> None:
+        _lib.call("fss_rowpool_free", self._h, int(first), int(n))
+
+    def stats(self) -> tuple:
+        """(free rows, longest free range)."""
+        f, l = ctypes.c_int64(), ctypes.c_int64()
+        _lib.call("fss_rowpool_stats", self._h, ctypes.byref(f), ctypes.byref(l))
+        return int(f.value), int(l.value)
+
+    def close(self) -> None:
+        if self._h:
+            _lib.call("fss_rowpool_destroy", self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def admit_fifo(pools: Sequence[RowPool], rows: Sequence[Sequence[int]], free_slots: int) -> List[List[int]]:
+    """The service's admission step (native ``fss_admit_fifo``): queries given oldest first,
+    needing ``rows[k][c]`` rows of ``pools[c]``, are admitted in order while slots are free and
+    every pool has room; the first that does not fit stops the step.  Returns the admitted
+    queries' first rows, one list per query."""
+    n, npools = len(rows), len(pools)
+    flat = (ctypes.c_int64 * max(1, n * npools))(*[int(x) for r in rows for x in r])
+    first = (ctypes.c_int64 * max(1, n * npools))()
+    handles = (ctypes.c_void_p * max(1, npools))(*[p._h.value for p in pools])
+    got = ctypes.c_int32()
+    _lib.call("fss_admit_fifo", handles, npools, n, flat, int(free_slots), first, ctypes.byref(got))
+    return [[int(first[k * npools + c]) for c in range(npools)] for k in range(got.value)]
 
 
 @dataclass
@@ -266,6 +351,7 @@ class ScreenService:
     def __init__(self, library: ConformerLibrary, mode: str = "vol", *, num_repeats: Optional[int] = None,
                  max_num_steps: Optional[int] = None, lr: float = 0.1, top_k: int = 100,
                  launch_pairs: int = 65536, hbm_budget_bytes: int = 8 << 30, max_active: int = 16,
+                 query_budget_bytes: Optional[int] = None,
                  early_stop: bool = True, alpha: Optional[float] = None, lam: Optional[float] = None,
                  similarity: str = "tanimoto", directionless: bool = True,
                  extended_points: bool = False, only_extended: bool = False,
@@ -305,13 +391,21 @@ class ScreenService:
         pharm = self.spec.mode in (_lib.MODE_PHARM, _lib.MODE_SHAPE_PHARM)
         mqp, mqh = (int(max_query_points) if shape else 0), (int(max_query_pharm) if pharm else 0)
         mqs, mqa = (int(max_query_surface), int(max_query_atoms)) if combo else (0, 0)
-        # admission: as many query slots as the HBM budget holds, at most max_active
-        fixed = arena_bytes(self.launch_pairs, self.B, 0, self.top_k, mqp, mqh, mqs, mqa)
-        slot = arena_bytes(0, 0, 1, self.top_k, mqp, mqh, mqs, mqa)
-        if fixed + slot > hbm_budget_bytes:
-            raise MemoryError("the HBM budget does not hold one query slot and one launch")
-        self.max_active = max(1, min(int(max_active), (hbm_budget_bytes - fixed) // slot,
+        # admission by memory: the launch arena, then up to max_active top-k slots, then the query
+        # row pools from what is left of the budget (query_budget_bytes overrides that share)
+        one = query_row_bytes(mqp, mqh, mqs, mqa)
+        fixed = arena_bytes(self.launch_pairs, self.B, 0, self.top_k, 0, 0)
+        slot = arena_bytes(0, 0, 1, self.top_k, 0, 0)
+        if fixed + slot + one > hbm_budget_bytes:
+            raise MemoryError("the HBM budget does not hold one launch, one query slot and one query")
+        self.max_active = max(1, min(int(max_active), (hbm_budget_bytes - fixed - one) // slot,
                                      self.launch_pairs))
+        if query_budget_bytes is None:
+            query_budget_bytes = hbm_budget_bytes - fixed - self.max_active * slot
+        if query_budget_bytes < one:
+            raise MemoryError(f"query_budget_bytes {query_budget_bytes} does not hold one query of the "
+                              f"maximum size ({one} bytes)")
+        self.query_budget_bytes = int(min(query_budget_bytes, self.max_active * one))
         self.table = restart_table(self.B)
         if lanes_per_pose is None:
             # ShaEP poses carry the surface-atom ESP sums: a wider tile; otherwise every restart of
@@ -326,6 +420,7 @@ class ScreenService:
         prm.max_query_points, prm.max_query_pharm = mqp, mqh
         prm.max_query_surface, prm.max_query_atoms = mqs, mqa
         prm.shape_from_surface = int(self.shape_from_surface)
+        prm.query_budget_bytes = self.query_budget_bytes
         prm.lanes_per_pose, prm.restarts_per_wg, prm.variant = int(lanes_per_pose), self.B, 0
         prm.seed_flags = (_lib.SEED_PHARM if self.spec.mode == _lib.MODE_PHARM else
                           _lib.SEED_SURF if combo else 0)
@@ -430,8 +525,21 @@ class ScreenService:
             _lib.call("fss_screen_stats", self._h, *(ctypes.byref(x) for x in v))
             return {"launches": int(v[0].value), "pairs": int(v[1].value), "pair_steps": int(v[2].value)}
 
+    def admission(self) -> dict:
+        """Rows of the four query pools, the most queries admitted at once, and how many queries
+        waited for pool room while a slot was free."""
+        rows = (ctypes.c_int64 * _lib.QPOOLS)()
+        most, waits = ctypes.c_int64(), ctypes.c_int64()
+        with self._cv:
+            if self._stop:
+                return dict(self._final_admission)
+            _lib.call("fss_screen_admission", self._h, rows, ctypes.byref(most), ctypes.byref(waits))
+        return {"pool_rows": [int(x) for x in rows], "max_admitted": int(most.value),
+                "memory_waits": int(waits.value)}
+
     def close(self) -> None:
         final = self.stats()
+        final_adm = self.admission() if not self._stop else None
         with self._cv:
             if self._stop:
                 return
@@ -439,6 +547,7 @@ class ScreenService:
             _lib.call("fss_screen_stats", self._h, ctypes.byref(n), None, None)
             self._final_launches = int(n.value)
             self._final_stats = final
+            self._final_admission = final_adm
             self._stop = True
             self._cv.notify_all()
         self._poller.join()

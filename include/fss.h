@@ -402,6 +402,10 @@ typedef struct FssScreenParams {
   int32_t max_query_atoms;     /* ShaEP atom rows of one query */
   int32_t shape_from_surface;  /* 1: the shape term uses the surface rows (alpha != 0.81),
                                   0: the shape channel's rows (heavy atoms) */
+  int64_t query_budget_bytes;  /* device bytes for the rows of admitted queries (see
+                                  fss_plan_query_pools); 0: max_active queries of the maximum
+                                  size.  A queued query is admitted, oldest first, when a slot
+                                  and room for its own rows are free. */
 } FssScreenParams;
 
 typedef struct FssQuery {      /* host arrays, copied by fss_screen_submit */
@@ -455,6 +459,46 @@ int fss_screen_destroy(FssScreen* svc);
  * For whole-library queries this FIFO filling gives lower median and tail latency than equal
  * shares (processor sharing) at the same throughput. */
 int fss_plan_launch(int32_t n, const int64_t* remaining, int64_t capacity, int64_t* take);
+
+/* Admission by memory.  An admitted query's rows live in four device pools (FSS_QPOOL_*: shape
+ * rows, 16 bytes a row; pharmacophore anchor + vector + type, 36; ShaEP surface rows, 16; ShaEP
+ * atom + radius, 20); each pool hands out contiguous row ranges first-fit and merges them back
+ * when the query finishes.  This replaces the reference's admission by free device memory
+ * (accel/batch/_bucket.py:76-98, _pad.py:24-97: split a bucket until it fits, retry on OOM) with
+ * a budget fixed when the service is created, so a query never fails for memory: it waits. */
+#define FSS_QPOOL_POINTS 0
+#define FSS_QPOOL_PHARM 1
+#define FSS_QPOOL_SURFACE 2
+#define FSS_QPOOL_ATOMS 3
+#define FSS_QPOOLS 4
+/* Rows of each pool that fss_screen_create reserves for these parameters (host only).  Pools the
+ * mode does not read get 0 rows.  With query_budget_bytes = 0, or at least max_active queries of
+ * the maximum size, every pool holds max_active maximum queries; otherwise the budget is split
+ * over the pools in proportion to one maximum query's bytes in each, and each pool holds at
+ * least one maximum query.  FSS_E_INVALID if the budget does not hold one maximum query. */
+int fss_plan_query_pools(const FssScoreParams* sp, const FssScreenParams* prm, int64_t rows[FSS_QPOOLS]);
+/* The first-fit row pool of the service (host only; no device memory). */
+typedef struct FssRowPool FssRowPool;
+int fss_rowpool_create(int64_t rows, FssRowPool** out);
+/* First free range of n rows: FSS_OK and *first, or FSS_PENDING when no range is long enough.
+ * n = 0 always succeeds at row 0. */
+int fss_rowpool_alloc(FssRowPool* pool, int64_t n, int64_t* first);
+/* Returns a range from fss_rowpool_alloc; FSS_E_INVALID if it is not wholly allocated. */
+int fss_rowpool_free(FssRowPool* pool, int64_t first, int64_t n);
+int fss_rowpool_stats(const FssRowPool* pool, int64_t* free_rows, int64_t* largest_free);
+int fss_rowpool_destroy(FssRowPool* pool);
+/* The service's admission step (host only): queries k = 0 .. n-1, oldest first, needing
+ * rows[k * n_pools + c] rows of pools[c], are admitted in order while fewer than free_slots have
+ * been and every pool has a range for the next one; the first that does not fit stops the step
+ * (no later query overtakes it).  first[k * n_pools + c] receives the ranges of the admitted
+ * ones, *admitted their number. */
+int fss_admit_fifo(FssRowPool* const* pools, int32_t n_pools, int32_t n, const int64_t* rows,
+                   int32_t free_slots, int64_t* first, int32_t* admitted);
+/* The service's pool sizes (rows[FSS_QPOOLS], as fss_plan_query_pools), the most queries it has
+ * held admitted at once, and how many queries waited for pool room while a slot was free (each
+ * counted once).  Any pointer may be NULL. */
+int fss_screen_admission(FssScreen* svc, int64_t* pool_rows, int64_t* max_admitted,
+                         int64_t* memory_waits);
 
 #ifdef __cplusplus
 }

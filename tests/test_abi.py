@@ -44,7 +44,8 @@ int main(void) {
          offsetof(FssQuery, typ), offsetof(FssHits, latency_ms));
   printf("%zu %zu %zu %zu %zu\n", offsetof(FssLibSpec, max_atoms), offsetof(FssScreenParams, shape_from_surface),
          offsetof(FssQuery, rad), sizeof(FssLibRows), offsetof(FssLibRows, acounts));
-  printf("%zu %zu %zu\n", offsetof(FssQuery, all_scores), offsetof(FssHits, all_scores), offsetof(FssHits, n_all));
+  printf("%zu %zu %zu %zu\n", offsetof(FssQuery, all_scores), offsetof(FssHits, all_scores), offsetof(FssHits, n_all),
+         offsetof(FssScreenParams, query_budget_bytes));
   return 0;
 }
 '''
@@ -67,7 +68,7 @@ int main(void) {
                             _lib.FssQuery.rad.offset, ctypes.sizeof(_lib.FssLibRows),
                             _lib.FssLibRows.acounts.offset]
     assert sizes[21:] == [_lib.FssQuery.all_scores.offset, _lib.FssHits.all_scores.offset,
-                          _lib.FssHits.n_all.offset]
+                          _lib.FssHits.n_all.offset, _lib.FssScreenParams.query_budget_bytes.offset]
 
 
 def test_validation_rejects_bad_arguments_without_launching():

@@ -84,6 +84,31 @@ def test_alone_equals_in_mix(dev):
         assert r.latency_ms > 0 and r.pairs == 2000
 
 
+def test_memory_budget_defers_admission_and_keeps_hits(dev):
+    """A query budget of two maximum-size queries: queries of mixed sizes wait for pool room
+    with slots free, are admitted by their own rows, and get the same hits as on a service that
+    admits them all at once."""
+    print("case screen with a two-query row budget", flush=True)
+    mols, lib = _library(600, 11, dev)
+    rng = np.random.default_rng(12)
+    queries = [{"points": synth.random_walks(rng, [int(n)])[0]} for n in rng.integers(8, 33, size=8)]
+    kw = dict(num_repeats=6, max_num_steps=30, top_k=10, launch_pairs=700, max_active=8,
+              max_query_points=32)
+    with ScreenService(lib, "vol", **kw) as svc:
+        free = [f.result(timeout=300) for f in [svc.submit(q) for q in queries]]
+        adm = svc.admission()
+    assert adm["pool_rows"] == [8 * 32, 0, 0, 0] and adm["memory_waits"] == 0
+    with ScreenService(lib, "vol", query_budget_bytes=2 * 32 * 16, **kw) as svc:
+        tight = [f.result(timeout=300) for f in [svc.submit(q) for q in queries]]
+        adm = svc.admission()
+    assert adm["pool_rows"] == [64, 0, 0, 0]
+    assert adm["memory_waits"] > 0 and 2 <= adm["max_admitted"] < 8, adm
+    for a, b in zip(free, tight):
+        assert [h.id for h in a.hits] == [h.id for h in b.hits]
+        assert [h.score for h in a.hits] == [h.score for h in b.hits]
+        assert b.pairs == 600
+
+
 def test_native_service_topk_is_exact(dev):
     """The native service's hits (uploaded from host arrays through the C ABI, several launches,
     device top-k merges) are exactly the top-k of aligning every pair directly with the same seeds
