@@ -23,11 +23,14 @@ Differences from the shipped reference, by design: everything is float64 (the re
 optimisers allocate float32 seeds and matrices, alignment/_torch.py:292,297, se3.py:127,143, so
 they cannot run in FP64 as shipped), and restart seeds are passed in explicitly.
 
-Pinning (see tests/test_oracle.py): closed forms, brute-force double loops, finite differences,
+Pinning (tests/test_oracle.py): closed forms, brute-force double loops, finite differences,
 torch.autograd of the scoring restatement, the reference's known-answer rotation
-(tests/test_alignment_utils.py:50-56), rigid-motion invariances.  The absolute optimiser
-trajectory (torch.optim.Adam) is pinned only through the library routine itself: parity unpinned
-beyond that, as the reference holds no golden optimiser outputs.
+(tests/test_alignment_utils.py:50-56), rigid-motion invariances.  tests/test_golden_reference.py
+checks the overlap / gradient / SE(3) loss functions, the pharmacophore, ESP and ShaEP scores
+against float64 outputs of the reference's own score modules (tests/golden/reference_vectors.npz)
+to 1e-12.  The absolute optimiser trajectory (torch.optim.Adam) is pinned only through the
+library routine itself: parity unpinned beyond that, as the reference holds no golden optimiser
+outputs.
 """
 from __future__ import annotations
 
