@@ -33,6 +33,7 @@ PENDING = 1
 LIB_SHAPE = 1
 LIB_PHARM = 2
 LIB_SHAEP = 4
+LIB_SURFACE = 8
 SCREEN_MAX_TOPK = 1024
 NOALLOC_SCRATCH_BYTES = 8 << 20
 QPOOLS = 4                    # query-row pools: shape rows, pharmacophores, ShaEP surface, atoms
@@ -208,6 +209,7 @@ SIGNATURES = {
     "fss_rowpool_destroy": (ctypes.c_int, [_P]),
     "fss_admit_fifo": (ctypes.c_int, [_P, ctypes.c_int32, ctypes.c_int32, _P, ctypes.c_int32, _P, _P]),
     "fss_screen_admission": (ctypes.c_int, [_P, _P, _P, _P]),
+    "fss_screen_terms": (ctypes.c_int, [_P, _P, _P, _P]),
 }
 
 _lock = threading.Lock()

@@ -346,8 +346,10 @@ struct QueryIn {
 };
 
 // The channels a mode reads, as fss_screen_create sizes them.
+// shape_rows: the shape term reads the shape channel; srf: the query and library surface rows
+// are read (the ShaEP score, or the surface shape modes with shape_from_surface).
 struct Channels {
-  bool shape, pharm, se, shape_rows;
+  bool shape, pharm, se, shape_rows, srf;
 };
 
 Channels channels_of(int mode, int shape_from_surface) {
@@ -355,7 +357,10 @@ Channels channels_of(int mode, int shape_from_surface) {
   c.shape = mode != FSS_MODE_PHARM;
   c.pharm = mode == FSS_MODE_PHARM || mode == FSS_MODE_SHAPE_PHARM;
   c.se = mode == FSS_MODE_ESP_COMBO;
-  c.shape_rows = c.shape && !(c.se && shape_from_surface);
+  const bool surf_shape = shape_from_surface &&
+      (c.se || mode == FSS_MODE_SHAPE || mode == FSS_MODE_SHAPE_ESP);
+  c.shape_rows = c.shape && !surf_shape;
+  c.srf = c.se || surf_shape;
   return c;
 }
 
@@ -400,6 +405,7 @@ struct FssScreen {
   int B = 0, K = 0, P = 0, A = 0;
   int need_shape = 0, need_pharm = 0;
   int need_se = 0;                     // FSS_MODE_ESP_COMBO: the ShaEP channel
+  int need_srf = 0;                    // surface rows (ShaEP, or the surface shape modes)
   int shape_rows = 0;                  // the shape term reads the shape channel's rows (else the surface rows)
   hipStream_t stream = nullptr;
   // device arena
@@ -430,6 +436,8 @@ struct FssScreen {
   bool stop = false;
   int64_t next_ticket = 1;
   std::atomic<int64_t> launches{0}, pairs{0}, pair_steps{0};
+  // sum over aligned pairs of executed steps x restarts x row pairs (worker thread writes, under mu)
+  double shape_terms = 0, pharm_terms = 0, esp_terms = 0;
   std::thread worker;
 };
 
@@ -490,12 +498,12 @@ int admit(FssScreen* s, int slot) {
       e = hipMemcpyAsync(s->qtyp + f[kQPharm], sl.q.typ.data(), sizeof(int32_t) * sl.q.nph,
                          hipMemcpyHostToDevice, s->stream);
   }
-  if (e == hipSuccess && s->need_se) {
+  if (e == hipSuccess && s->need_srf)
     e = hipMemcpyAsync(s->qsrf + 4 * f[kQSurf], sl.q.srf.data(), sizeof(float) * 4 * sl.q.ns,
                        hipMemcpyHostToDevice, s->stream);
-    if (e == hipSuccess)
-      e = hipMemcpyAsync(s->qatm + 4 * f[kQAtoms], sl.q.atm.data(), sizeof(float) * 4 * sl.q.na,
-                         hipMemcpyHostToDevice, s->stream);
+  if (e == hipSuccess && s->need_se) {
+    e = hipMemcpyAsync(s->qatm + 4 * f[kQAtoms], sl.q.atm.data(), sizeof(float) * 4 * sl.q.na,
+                       hipMemcpyHostToDevice, s->stream);
     if (e == hipSuccess)
       e = hipMemcpyAsync(s->qrad + f[kQAtoms], sl.q.rad.data(), sizeof(float) * sl.q.na,
                          hipMemcpyHostToDevice, s->stream);
@@ -567,13 +575,13 @@ int launch(FssScreen* s, const std::vector<int>& act) {
       max_na = std::max(max_na, sl.q.na);
       for (int64_t j = 0; j < take[i]; ++j, ++p) {
         const size_t e = 2 * (size_t)(sl.next + j);
-        const bool sh = s->shape_rows, ph = s->need_pharm, se = s->need_se;
+        const bool sh = s->shape_rows, ph = s->need_pharm, se = s->need_se, sf = s->need_srf;
         set(kRef, p, (int32_t)sl.first[kQPts], sh ? sl.q.np : 0);
         set(kFit, p, sh ? L->h_rng[e] : 0, sh ? L->h_rng[e + 1] : 0);
         set(kRefPh, p, (int32_t)sl.first[kQPharm], ph ? sl.q.nph : 0);
         set(kFitPh, p, ph ? L->h_prng[e] : 0, ph ? L->h_prng[e + 1] : 0);
-        set(kRefSrf, p, (int32_t)sl.first[kQSurf], se ? sl.q.ns : 0);
-        set(kFitSrf, p, se ? L->h_srng[e] : 0, se ? L->h_srng[e + 1] : 0);
+        set(kRefSrf, p, (int32_t)sl.first[kQSurf], sf ? sl.q.ns : 0);
+        set(kFitSrf, p, sf ? L->h_srng[e] : 0, sf ? L->h_srng[e + 1] : 0);
         set(kRefAtm, p, (int32_t)sl.first[kQAtoms], se ? sl.q.na : 0);
         set(kFitAtm, p, se ? L->h_arng[e] : 0, se ? L->h_arng[e + 1] : 0);
       }
@@ -592,7 +600,7 @@ int launch(FssScreen* s, const std::vector<int>& act) {
     pb.ref_pts = s->qpts; pb.fit_pts = s->lib->pts;
     pb.ref_rng = drng[kRef]; pb.fit_rng = drng[kFit];
     pb.max_ref = std::max(max_np, 1); pb.max_fit = std::max(lib_max_pts, 1);
-  } else if (s->need_shape) {   // surface shape term of the ShaEP score: the surface rows
+  } else if (s->need_shape) {   // the shape term on the surface rows (ShaEP, or a surface mode)
     pb.ref_pts = s->qsrf; pb.fit_pts = s->lib->srf;
     pb.ref_rng = drng[kRefSrf]; pb.fit_rng = drng[kFitSrf];
     pb.max_ref = std::max(max_ns, 1); pb.max_fit = std::max(lib_max_srf, 1);
@@ -635,8 +643,24 @@ int launch(FssScreen* s, const std::vector<int>& act) {
   s->launches.fetch_add(1);
   s->pairs.fetch_add(p);
   int64_t steps = 0;
-  for (int i = 0; i < p; ++i) steps += s->h_steps[i];
+  double st_shape = 0, st_pharm = 0, st_esp = 0;
+  const int kShapeRef = s->shape_rows ? kRef : kRefSrf, kShapeFit = s->shape_rows ? kFit : kFitSrf;
+  auto cnt = [&](int k, int i) { return (double)tab[k][2 * i + 1]; };
+  for (int i = 0; i < p; ++i) {
+    const double st = (double)s->h_steps[i] * s->B;
+    steps += s->h_steps[i];
+    if (s->need_shape) st_shape += st * cnt(kShapeRef, i) * cnt(kShapeFit, i);
+    if (s->need_pharm) st_pharm += st * cnt(kRefPh, i) * cnt(kFitPh, i);
+    if (s->need_se)    // each side's surface observers against the other side's atoms
+      st_esp += st * (cnt(kRefSrf, i) * cnt(kFitAtm, i) + cnt(kFitSrf, i) * cnt(kRefAtm, i));
+  }
   s->pair_steps.fetch_add(steps);
+  {
+    std::lock_guard<std::mutex> lk(s->mu);
+    s->shape_terms += st_shape;
+    s->pharm_terms += st_pharm;
+    s->esp_terms += st_esp;
+  }
   const auto now = clk::now();
   for (int i = 0, off = 0; i < na; ++i) {
     if (take[i] == 0) continue;
@@ -678,7 +702,7 @@ std::vector<int> admit_pending(FssScreen* s) {
     int64_t* r = &rows[(size_t)k * kQPools];
     r[kQPts] = s->shape_rows ? q.np : 0;
     r[kQPharm] = s->need_pharm ? q.nph : 0;
-    r[kQSurf] = s->need_se ? q.ns : 0;
+    r[kQSurf] = s->need_srf ? q.ns : 0;
     r[kQAtoms] = s->need_se ? q.na : 0;
   }
   RowPool* pp[kQPools];
@@ -781,7 +805,7 @@ int fss_plan_query_pools(const FssScoreParams* sp, const FssScreenParams* prm, i
   const Channels ch = channels_of(sp->mode, prm->shape_from_surface);
   const int64_t need[kQPools] = {ch.shape_rows ? std::max(prm->max_query_points, 0) : 0,
                                  ch.pharm ? std::max(prm->max_query_pharm, 0) : 0,
-                                 ch.se ? std::max(prm->max_query_surface, 0) : 0,
+                                 ch.srf ? std::max(prm->max_query_surface, 0) : 0,
                                  ch.se ? std::max(prm->max_query_atoms, 0) : 0};
   const int64_t A = prm->max_active;
   int64_t one = 0;
@@ -851,9 +875,12 @@ int fss_admit_fifo(FssRowPool* const* pools, int32_t n_pools, int32_t n, const i
 int fss_lib_create(const FssLibSpec* spec, FssLibrary** out) {
   if (!spec || !out) return set_error(FSS_E_INVALID, "NULL library arguments");
   *out = nullptr;
-  constexpr int all = FSS_LIB_SHAPE | FSS_LIB_PHARM | FSS_LIB_SHAEP;
+  constexpr int all = FSS_LIB_SHAPE | FSS_LIB_PHARM | FSS_LIB_SHAEP | FSS_LIB_SURFACE;
   if (!(spec->channels & all) || (spec->channels & ~all))
-    return set_error(FSS_E_INVALID, "channels must be a combination of FSS_LIB_SHAPE, FSS_LIB_PHARM and FSS_LIB_SHAEP");
+    return set_error(FSS_E_INVALID, "channels must be a combination of FSS_LIB_SHAPE, FSS_LIB_PHARM, "
+                     "FSS_LIB_SHAEP and FSS_LIB_SURFACE");
+  if ((spec->channels & FSS_LIB_SHAEP) && (spec->channels & FSS_LIB_SURFACE))
+    return set_error(FSS_E_INVALID, "FSS_LIB_SHAEP already holds the surface rows: drop FSS_LIB_SURFACE");
   if (spec->max_conformers < 1 || spec->max_conformers > INT32_MAX)
     return set_error(FSS_E_INVALID, "max_conformers must be in [1, 2^31)");
   if ((spec->channels & FSS_LIB_SHAPE) && (spec->max_points < 1 || spec->max_points > INT32_MAX))
@@ -863,6 +890,8 @@ int fss_lib_create(const FssLibSpec* spec, FssLibrary** out) {
   if ((spec->channels & FSS_LIB_SHAEP) && (spec->max_surface < 1 || spec->max_surface > INT32_MAX ||
                                            spec->max_atoms < 1 || spec->max_atoms > INT32_MAX))
     return set_error(FSS_E_INVALID, "max_surface and max_atoms must be in [1, 2^31)");
+  if ((spec->channels & FSS_LIB_SURFACE) && (spec->max_surface < 1 || spec->max_surface > INT32_MAX))
+    return set_error(FSS_E_INVALID, "max_surface must be in [1, 2^31)");
   DeviceGuard dg(spec->device);
   if (dg.err != hipSuccess) return hip_fail(dg.err, "fss_lib_create: device");
   FssLibrary* lib = new FssLibrary();
@@ -871,7 +900,7 @@ int fss_lib_create(const FssLibSpec* spec, FssLibrary** out) {
   lib->cap_conf = spec->max_conformers;
   lib->cap_pts = (spec->channels & FSS_LIB_SHAPE) ? spec->max_points : 0;
   lib->cap_ph = (spec->channels & FSS_LIB_PHARM) ? spec->max_pharm : 0;
-  lib->cap_srf = (spec->channels & FSS_LIB_SHAEP) ? spec->max_surface : 0;
+  lib->cap_srf = (spec->channels & (FSS_LIB_SHAEP | FSS_LIB_SURFACE)) ? spec->max_surface : 0;
   lib->cap_atm = (spec->channels & FSS_LIB_SHAEP) ? spec->max_atoms : 0;
   hipError_t e = hipSuccess;
   if (lib->cap_pts) {
@@ -887,7 +916,9 @@ int fss_lib_create(const FssLibSpec* spec, FssLibrary** out) {
   if (e == hipSuccess && lib->cap_srf) {
     e = dalloc(&lib->srf, 4 * lib->cap_srf);
     if (e == hipSuccess) e = dalloc(&lib->srng, 2 * lib->cap_conf);
-    if (e == hipSuccess) e = dalloc(&lib->atm, 4 * lib->cap_atm);
+  }
+  if (e == hipSuccess && lib->cap_atm) {
+    e = dalloc(&lib->atm, 4 * lib->cap_atm);
     if (e == hipSuccess) e = dalloc(&lib->rad, lib->cap_atm);
     if (e == hipSuccess) e = dalloc(&lib->arng, 2 * lib->cap_conf);
   }
@@ -902,8 +933,8 @@ int fss_lib_create(const FssLibSpec* spec, FssLibrary** out) {
 int fss_lib_upload(FssLibrary* lib, int32_t n, const float* pts, const int32_t* counts,
                    const float* anc, const float* vec, const int32_t* typ, const int32_t* pcounts,
                    int64_t* first) {
-  if (lib && (lib->channels & FSS_LIB_SHAEP))
-    return set_error(FSS_E_INVALID, "the library has a ShaEP channel: append with fss_lib_append");
+  if (lib && (lib->channels & (FSS_LIB_SHAEP | FSS_LIB_SURFACE)))
+    return set_error(FSS_E_INVALID, "the library has surface rows: append with fss_lib_append");
   FssLibRows r{};
   r.pts = pts; r.counts = counts; r.anc = anc; r.vec = vec; r.typ = typ; r.pcounts = pcounts;
   return fss_lib_append(lib, n, &r, first);
@@ -914,11 +945,13 @@ int fss_lib_append(FssLibrary* lib, int32_t n, const FssLibRows* in, int64_t* fi
   if (n < 0) return set_error(FSS_E_INVALID, "n < 0");
   const bool sh = lib->channels & FSS_LIB_SHAPE, ph = lib->channels & FSS_LIB_PHARM;
   const bool se = lib->channels & FSS_LIB_SHAEP;
+  const bool sf = lib->channels & (FSS_LIB_SHAEP | FSS_LIB_SURFACE);   // surface rows
   if (n > 0 && sh && (!in->pts || !in->counts)) return set_error(FSS_E_INVALID, "shape channel arrays are NULL");
   if (n > 0 && ph && (!in->anc || !in->vec || !in->typ || !in->pcounts))
     return set_error(FSS_E_INVALID, "pharmacophore channel arrays are NULL");
-  if (n > 0 && se && (!in->srf || !in->scounts || !in->atm || !in->rad || !in->acounts))
-    return set_error(FSS_E_INVALID, "ShaEP channel arrays are NULL");
+  if (n > 0 && sf && (!in->srf || !in->scounts)) return set_error(FSS_E_INVALID, "surface arrays are NULL");
+  if (n > 0 && se && (!in->atm || !in->rad || !in->acounts))
+    return set_error(FSS_E_INVALID, "ShaEP atom arrays are NULL");
   std::lock_guard<std::mutex> lk(lib->mu);
   if (first) *first = lib->n;
   if (n == 0) return FSS_OK;
@@ -927,7 +960,7 @@ int fss_lib_append(FssLibrary* lib, int32_t n, const FssLibRows* in, int64_t* fi
   struct Ch { bool on; const int32_t* cnt; int64_t n_rows, cap; int32_t mx; std::vector<int32_t> r; int64_t rows; };
   Ch ch[4] = {{sh, in->counts, lib->n_pts, lib->cap_pts, lib->max_pts, {}, 0},
               {ph, in->pcounts, lib->n_ph, lib->cap_ph, lib->max_ph, {}, 0},
-              {se, in->scounts, lib->n_srf, lib->cap_srf, lib->max_srf, {}, 0},
+              {sf, in->scounts, lib->n_srf, lib->cap_srf, lib->max_srf, {}, 0},
               {se, in->acounts, lib->n_atm, lib->cap_atm, lib->max_atm, {}, 0}};
   static const char* names[4] = {"points", "pharmacophores", "surface points", "atoms"};
   for (int c = 0; c < 4; ++c) {
@@ -962,9 +995,11 @@ int fss_lib_append(FssLibrary* lib, int32_t n, const FssLibRows* in, int64_t* fi
     put(lib->typ + lib->n_ph, in->typ, sizeof(int32_t) * ch[1].rows);
     put(lib->prng + 2 * lib->n, ch[1].r.data(), sizeof(int32_t) * 2 * n);
   }
-  if (se) {
+  if (sf) {
     put(lib->srf + 4 * lib->n_srf, in->srf, sizeof(float) * 4 * ch[2].rows);
     put(lib->srng + 2 * lib->n, ch[2].r.data(), sizeof(int32_t) * 2 * n);
+  }
+  if (se) {
     put(lib->atm + 4 * lib->n_atm, in->atm, sizeof(float) * 4 * ch[3].rows);
     put(lib->rad + lib->n_atm, in->rad, sizeof(float) * ch[3].rows);
     put(lib->arng + 2 * lib->n, ch[3].r.data(), sizeof(int32_t) * 2 * n);
@@ -972,10 +1007,12 @@ int fss_lib_append(FssLibrary* lib, int32_t n, const FssLibRows* in, int64_t* fi
   if (e != hipSuccess) return hip_fail(e, "fss_lib_append");
   if (sh) { lib->h_rng.insert(lib->h_rng.end(), ch[0].r.begin(), ch[0].r.end()); lib->n_pts += ch[0].rows; lib->max_pts = ch[0].mx; }
   if (ph) { lib->h_prng.insert(lib->h_prng.end(), ch[1].r.begin(), ch[1].r.end()); lib->n_ph += ch[1].rows; lib->max_ph = ch[1].mx; }
-  if (se) {
+  if (sf) {
     lib->h_srng.insert(lib->h_srng.end(), ch[2].r.begin(), ch[2].r.end());
-    lib->h_arng.insert(lib->h_arng.end(), ch[3].r.begin(), ch[3].r.end());
     lib->n_srf += ch[2].rows; lib->max_srf = ch[2].mx;
+  }
+  if (se) {
+    lib->h_arng.insert(lib->h_arng.end(), ch[3].r.begin(), ch[3].r.end());
     lib->n_atm += ch[3].rows; lib->max_atm = ch[3].mx;
   }
   lib->n += n;
@@ -1010,6 +1047,8 @@ int fss_screen_create(FssLibrary* lib, const FssScoreParams* sp, const FssScreen
   if (shape_rows && !(lib->channels & FSS_LIB_SHAPE)) return set_error(FSS_E_INVALID, "the mode needs the library's shape channel");
   if (need_pharm && !(lib->channels & FSS_LIB_PHARM)) return set_error(FSS_E_INVALID, "the mode needs the library's pharmacophore channel");
   if (need_se && !(lib->channels & FSS_LIB_SHAEP)) return set_error(FSS_E_INVALID, "the mode needs the library's ShaEP channel");
+  if (ch.srf && !(lib->channels & (FSS_LIB_SHAEP | FSS_LIB_SURFACE)))
+    return set_error(FSS_E_INVALID, "shape_from_surface needs the library's surface rows");
   if (prm->n_restarts < 1) return set_error(FSS_E_INVALID, "n_restarts must be >= 1");
   if (prm->max_steps < 0) return set_error(FSS_E_INVALID, "max_steps must be >= 0");
   if (prm->top_k < 1 || prm->top_k > FSS_SCREEN_MAX_TOPK) return set_error(FSS_E_INVALID, "top_k must be in [1, %d]", FSS_SCREEN_MAX_TOPK);
@@ -1017,8 +1056,8 @@ int fss_screen_create(FssLibrary* lib, const FssScoreParams* sp, const FssScreen
   if (prm->max_active < 1 || prm->max_active > 4096) return set_error(FSS_E_INVALID, "max_active must be in [1, 4096]");
   if (prm->launch_pairs < prm->max_active) return set_error(FSS_E_INVALID, "launch_pairs must be >= max_active");
   if (shape_rows && prm->max_query_points < 1) return set_error(FSS_E_INVALID, "max_query_points must be >= 1");
-  if (need_se && (prm->max_query_surface < 1 || prm->max_query_atoms < 1))
-    return set_error(FSS_E_INVALID, "max_query_surface and max_query_atoms must be >= 1");
+  if (ch.srf && prm->max_query_surface < 1) return set_error(FSS_E_INVALID, "max_query_surface must be >= 1");
+  if (need_se && prm->max_query_atoms < 1) return set_error(FSS_E_INVALID, "max_query_atoms must be >= 1");
   if (need_pharm && prm->max_query_pharm < 1) return set_error(FSS_E_INVALID, "max_query_pharm must be >= 1");
   if (!prm->quat_table) return set_error(FSS_E_INVALID, "quat_table is NULL");
   int64_t pool_rows[kQPools];
@@ -1036,12 +1075,14 @@ int fss_screen_create(FssLibrary* lib, const FssScoreParams* sp, const FssScreen
   s->need_shape = need_shape;
   s->need_pharm = need_pharm;
   s->need_se = need_se;
+  s->need_srf = ch.srf;
   s->shape_rows = shape_rows;
   if (!s->prm.lanes_per_pose) s->prm.lanes_per_pose = s->B <= 64 ? 64 / s->B : 1;   // one wavefront per pair
   if (!s->prm.restarts_per_wg) s->prm.restarts_per_wg = s->B;
   if (!shape_rows) s->prm.max_query_points = 0;
   if (!need_pharm) s->prm.max_query_pharm = 0;
-  if (!need_se) s->prm.max_query_surface = s->prm.max_query_atoms = 0;
+  if (!ch.srf) s->prm.max_query_surface = 0;
+  if (!need_se) s->prm.max_query_atoms = 0;
   s->slots.resize(s->A);
   const int64_t PB = (int64_t)s->P * s->B;
   const int64_t AK = (int64_t)s->A * 2 * s->K;
@@ -1101,20 +1142,27 @@ int fss_screen_submit(FssScreen* s, const FssQuery* q, int64_t* ticket) {
     if (q->n_pharm > s->prm.max_query_pharm)
       return set_error(FSS_E_TOO_LARGE, "query has %d pharmacophores, the service takes %d", q->n_pharm, s->prm.max_query_pharm);
   }
+  if (s->need_srf) {
+    if (q->n_surface < 1 || !q->srf) return set_error(FSS_E_INVALID, "the query needs surface rows");
+    if (q->n_surface > s->prm.max_query_surface)
+      return set_error(FSS_E_TOO_LARGE, "query has %d surface rows, the service takes %d", q->n_surface,
+                       s->prm.max_query_surface);
+  }
   if (s->need_se) {
-    if (q->n_surface < 1 || !q->srf || q->n_atoms < 1 || !q->atm || !q->rad)
-      return set_error(FSS_E_INVALID, "the query needs ShaEP surface and atom rows");
-    if (q->n_surface > s->prm.max_query_surface || q->n_atoms > s->prm.max_query_atoms)
-      return set_error(FSS_E_TOO_LARGE, "query has %d surface / %d atom rows, the service takes %d / %d",
-                       q->n_surface, q->n_atoms, s->prm.max_query_surface, s->prm.max_query_atoms);
+    if (q->n_atoms < 1 || !q->atm || !q->rad) return set_error(FSS_E_INVALID, "the query needs ShaEP atom rows");
+    if (q->n_atoms > s->prm.max_query_atoms)
+      return set_error(FSS_E_TOO_LARGE, "query has %d atom rows, the service takes %d", q->n_atoms,
+                       s->prm.max_query_atoms);
   }
   QueryIn qi;
   qi.t_submit = clk::now();
   qi.want_all = q->all_scores != 0;
-  if (s->need_se) {
+  if (s->need_srf) {
     qi.ns = q->n_surface;
-    qi.na = q->n_atoms;
     qi.srf.assign(q->srf, q->srf + 4 * (size_t)q->n_surface);
+  }
+  if (s->need_se) {
+    qi.na = q->n_atoms;
     qi.atm.assign(q->atm, q->atm + 4 * (size_t)q->n_atoms);
     qi.rad.assign(q->rad, q->rad + (size_t)q->n_atoms);
   }
@@ -1183,6 +1231,15 @@ int fss_screen_stats(FssScreen* s, int64_t* launches, int64_t* pairs, int64_t* p
   if (launches) *launches = s->launches.load();
   if (pairs) *pairs = s->pairs.load();
   if (pair_steps) *pair_steps = s->pair_steps.load();
+  return FSS_OK;
+}
+
+int fss_screen_terms(FssScreen* s, double* shape_terms, double* pharm_terms, double* esp_terms) {
+  if (!s) return set_error(FSS_E_INVALID, "service is NULL");
+  std::lock_guard<std::mutex> lk(s->mu);
+  if (shape_terms) *shape_terms = s->shape_terms;
+  if (pharm_terms) *pharm_terms = s->pharm_terms;
+  if (esp_terms) *esp_terms = s->esp_terms;
   return FSS_OK;
 }
 

@@ -57,13 +57,18 @@ def restart_table(B: int) -> np.ndarray:
 
 class ConformerLibrary:
     """Packed conformers on one device: shape channel (atoms or surface points [+ ESP]),
-    pharmacophores and/or the ShaEP channel (surface points with ESP, atoms with hydrogens,
-    partial charges and radii), with integer ids."""
+    pharmacophores, and either the ShaEP channel (surface points with ESP, atoms with hydrogens,
+    partial charges and radii) or surface rows alone (points with ESP), with integer ids.  The
+    surf / surf_esp screens read a library's surface rows when it has them, so one library with
+    heavy atoms, pharmacophores and surfaces serves the atom and the surface modes."""
 
     def __init__(self, shape: Optional[CloudSet] = None, pharm: Optional[PharmSet] = None,
-                 ids: Optional[np.ndarray] = None, shaep: Optional[ShaepSet] = None):
-        self.shape, self.pharm, self.shaep = shape, pharm, shaep
-        some = [c for c in (shape, pharm, shaep) if c is not None]
+                 ids: Optional[np.ndarray] = None, shaep: Optional[ShaepSet] = None,
+                 surface: Optional[CloudSet] = None):
+        if shaep is not None and surface is not None:
+            raise ValueError("the ShaEP channel holds the surface rows: pass shaep or surface, not both")
+        self.shape, self.pharm, self.shaep, self.surface = shape, pharm, shaep, surface
+        some = [c for c in (shape, pharm, shaep, surface) if c is not None]
         if not some:
             raise ValueError("a library needs at least one channel")
         n = some[0].n
@@ -84,14 +89,21 @@ class ConformerLibrary:
                     nbytes: int) -> "ConformerLibrary":
         """A library whose rows live only in a native FssLibrary (see store.load_library)."""
         lib = cls.__new__(cls)
-        lib.shape = lib.pharm = lib.shaep = None
+        lib.shape = lib.pharm = lib.shaep = lib.surface = None
         lib.n = int(n)
         lib.ids = np.asarray(ids)
         lib._native = handle
         lib._native_bytes = int(nbytes)
+        lib._native_surface = False        # whether the native library holds surface rows
         lib.num_surf_points = None
         lib.pre_centered = False
         return lib
+
+    @property
+    def has_surface_rows(self) -> bool:
+        """Whether the library holds surface rows (the ShaEP channel or surface rows alone)."""
+        return (getattr(self, "shaep", None) is not None or getattr(self, "surface", None) is not None
+                or bool(getattr(self, "_native_surface", False)))
 
     def id_of(self, index: int):
         """The id of library entry ``index`` as a Python scalar."""
@@ -103,13 +115,14 @@ class ConformerLibrary:
         if getattr(self, "_native", None) is None:
             lib = _lib.load()
             spec = _lib.FssLibSpec()
-            some = next(c for c in (self.shape, self.pharm, self.shaep) if c is not None)
+            some = next(c for c in (self.shape, self.pharm, self.shaep, self.surface) if c is not None)
             dev = (some.pts if isinstance(some, CloudSet) else
                    some.anc if isinstance(some, PharmSet) else some.srf.pts).device
             spec.device = dev.index if dev.index is not None else torch.cuda.current_device()
             spec.channels = (_lib.LIB_SHAPE if self.shape is not None else 0) | \
                 (_lib.LIB_PHARM if self.pharm is not None else 0) | \
-                (_lib.LIB_SHAEP if self.shaep is not None else 0)
+                (_lib.LIB_SHAEP if self.shaep is not None else 0) | \
+                (_lib.LIB_SURFACE if self.surface is not None else 0)
             spec.max_conformers = max(self.n, 1)
             rows = _lib.FssLibRows()
             keep = []
@@ -133,6 +146,9 @@ class ConformerLibrary:
                 spec.max_atoms = max(int(se.atm.counts.sum()), 1)
                 rows.srf, rows.scounts = _lib.ptr(se.srf.pts), counts(se.srf.counts)
                 rows.atm, rows.rad, rows.acounts = _lib.ptr(se.atm.pts), _lib.ptr(se.rad), counts(se.atm.counts)
+            if self.surface is not None:
+                spec.max_surface = max(int(self.surface.counts.sum()), 1)
+                rows.srf, rows.scounts = _lib.ptr(self.surface.pts), counts(self.surface.counts)
             h = ctypes.c_void_p()
             _lib.call("fss_lib_create", _lib.byref(spec), ctypes.byref(h))
             torch.cuda.synchronize(dev)
@@ -163,14 +179,18 @@ class ConformerLibrary:
                     atoms_w_H=None, charges=None, radii=None) -> "ConformerLibrary":
         """Per-conformer arrays.  ``points`` [+ ``esp``]: the shape channel (heavy atoms for the
         ShaEP score at alpha 0.81); ``pharm_*``: pharmacophores; ``surf_points``, ``surf_esp``,
-        ``atoms_w_H``, ``charges``, ``radii``: the ShaEP channel (vol_and_surf_esp)."""
+        ``atoms_w_H``, ``charges``, ``radii``: the ShaEP channel (vol_and_surf_esp), or with
+        ``surf_points`` [+ ``surf_esp``] alone, surface rows for the surf / surf_esp modes."""
         shape = CloudSet.from_arrays(points, esp, device=device) if points is not None else None
         pharm = (PharmSet.from_arrays(pharm_types, pharm_anchors, pharm_vectors, device=device)
                  if pharm_types is not None else None)
-        shaep = (ShaepSet.from_arrays(surf_points, surf_esp, atoms_w_H, charges, radii, device=device)
-                 if surf_points is not None else None)
-        lib = cls(shape, pharm, ids, shaep)
-        surf = shaep.srf.counts if shaep is not None else None
+        shaep = surface = None
+        if surf_points is not None and atoms_w_H is not None:
+            shaep = ShaepSet.from_arrays(surf_points, surf_esp, atoms_w_H, charges, radii, device=device)
+        elif surf_points is not None:
+            surface = CloudSet.from_arrays(surf_points, surf_esp, device=device)
+        lib = cls(shape, pharm, ids, shaep, surface)
+        surf = shaep.srf.counts if shaep is not None else surface.counts if surface is not None else None
         if surf is not None and len(surf) and (surf == surf[0]).all():
             lib.num_surf_points = int(surf[0])
         return lib
@@ -179,7 +199,7 @@ class ConformerLibrary:
         if self._native_bytes is not None:
             return self._native_bytes
         tot = 0
-        parts = [self.shape, self.pharm]
+        parts = [self.shape, self.pharm, self.surface]
         if self.shaep is not None:
             parts += [self.shaep.srf, self.shaep.atm, self.shaep]
         for cs in parts:
@@ -386,11 +406,14 @@ class ScreenService:
         self.launch_pairs = int(launch_pairs)
         # vol_and_surf_esp: the shape term takes the heavy atoms at alpha 0.81, else the surface
         # rows of the ShaEP channel (_core.py:1079-1080)
-        self.shape_from_surface = combo and alpha != 0.81
+        # surf / surf_esp: the library's surface rows when it has them, else its shape channel
+        self.surface_mode = mode in ("surf", "surf_esp") and library.has_surface_rows
+        self.shape_from_surface = (combo and alpha != 0.81) or self.surface_mode
         shape = self.spec.mode != _lib.MODE_PHARM and not self.shape_from_surface
         pharm = self.spec.mode in (_lib.MODE_PHARM, _lib.MODE_SHAPE_PHARM)
         mqp, mqh = (int(max_query_points) if shape else 0), (int(max_query_pharm) if pharm else 0)
-        mqs, mqa = (int(max_query_surface), int(max_query_atoms)) if combo else (0, 0)
+        mqs = int(max_query_surface) if (combo or self.surface_mode) else 0
+        mqa = int(max_query_atoms) if combo else 0
         # admission by memory: the launch arena, then up to max_active top-k slots, then the query
         # row pools from what is left of the budget (query_budget_bytes overrides that share)
         one = query_row_bytes(mqp, mqh, mqs, mqa)
@@ -476,6 +499,9 @@ class ScreenService:
             keep.append(rad)
             q.n_surface, q.srf = len(srf), srf.ctypes.data
             q.n_atoms, q.atm, q.rad = len(atm), atm.ctypes.data, rad.ctypes.data
+        if self.surface_mode:      # the query's surface: points [+ esp] as surface rows
+            srf = rows4(query["points"], query.get("esp") if self.spec.mode == _lib.MODE_SHAPE_ESP else None)
+            q.n_surface, q.srf = len(srf), srf.ctypes.data
         if self.spec.mode != _lib.MODE_PHARM and not self.shape_from_surface:
             xyz = np.asarray(query["points"], np.float32).reshape(-1, 3)
             rows = np.zeros((len(xyz), 4), np.float32)
@@ -516,14 +542,18 @@ class ScreenService:
             return int(n.value)
 
     def stats(self) -> dict:
-        """Launches run, (query, library entry) pairs aligned and optimiser steps those pairs ran
-        (summed over pairs), so far (the final counts once closed)."""
+        """Launches run, (query, library entry) pairs aligned, optimiser steps those pairs ran
+        (summed over pairs) and the row-pair terms those steps evaluated (steps x restarts x
+        query rows x entry rows, per score term), so far (the final counts once closed)."""
         with self._cv:
             if self._stop:
                 return dict(self._final_stats)
             v = [ctypes.c_int64() for _ in range(3)]
             _lib.call("fss_screen_stats", self._h, *(ctypes.byref(x) for x in v))
-            return {"launches": int(v[0].value), "pairs": int(v[1].value), "pair_steps": int(v[2].value)}
+            w = [ctypes.c_double() for _ in range(3)]
+            _lib.call("fss_screen_terms", self._h, *(ctypes.byref(x) for x in w))
+            return {"launches": int(v[0].value), "pairs": int(v[1].value), "pair_steps": int(v[2].value),
+                    "shape_terms": w[0].value, "pharm_terms": w[1].value, "esp_terms": w[2].value}
 
     def admission(self) -> dict:
         """Rows of the four query pools, the most queries admitted at once, and how many queries

@@ -558,4 +558,5 @@ def load_library(store, mode: str, device=0, shards: Optional[Sequence[int]] = N
     out = ConformerLibrary.from_native(h, n, all_ids, nbytes)
     out.num_surf_points = store.num_surf_points if store.schema.get("surf") else None
     out.pre_centered = store.pre_centered or recenter
+    out._native_surface = want_se
     return out

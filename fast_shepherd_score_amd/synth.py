@@ -179,3 +179,84 @@ def conformer_library(n: int, atoms=(10, 50), base: int = 20000, seed: int = 5):
     owner = np.repeat(np.arange(n), counts)
     flat = np.einsum("pij,pj->pi", R[owner], flat).astype(np.float32)
     return flat, counts
+
+
+def screening_molecules(n: int, atoms=(10, 50), n_surf: int = 200, pharms=(1, 16), seed: int = 7):
+    """n molecules of the screening workload: U{atoms} random-walk heavy atoms, n_surf surface
+    points with ESP values, U{pharms} pharmacophores.  Returns a list of dicts with the
+    container's attribute names (atom_pos, surf_pos, surf_esp, pharm_types, pharm_ancs,
+    pharm_vecs)."""
+    rng = np.random.default_rng(seed)
+    mols = random_walks(rng, rng.integers(atoms[0], atoms[1] + 1, size=n))
+    nph = rng.integers(pharms[0], pharms[1] + 1, size=n)
+    out = []
+    for m, k in zip(mols, nph):
+        t, a, v = pharmacophores(rng, m, int(k))
+        out.append(dict(atom_pos=m, surf_pos=surface_points(rng, m, n_surf), surf_esp=esp_values(rng, n_surf),
+                        pharm_types=t.astype(np.int32), pharm_ancs=a, pharm_vecs=v))
+    return out
+
+
+def screening_library(n: int, device, atoms=(10, 50), n_surf: int = 200, pharms=(1, 16),
+                      base: int = 8000, seed: int = 5, chunk: int = 65536):
+    """The screening library: ``base`` molecules of :func:`screening_molecules`, each posed under
+    n/base uniform random rotations (every entry a distinct rigid conformer).  Posing runs on
+    the host in chunks of ``chunk`` entries (numpy); the packed rows are then copied to
+    ``device``.  Returns (shape CloudSet of the heavy atoms, PharmSet, surface CloudSet with ESP
+    in w)."""
+    import torch
+    from .packing import CloudSet, PharmSet
+    mols = screening_molecules(min(base, n), atoms, n_surf, pharms, seed)
+    base = len(mols)
+    rng = np.random.default_rng(seed + 1)
+    q = random_quaternions(rng, n).astype(np.float64)
+    which = np.arange(n) % base
+
+    def rotations(lo, hi):
+        w, x, y, z = q[lo:hi].T
+        return np.stack([1 - 2 * (y * y + z * z), 2 * (x * y - z * w), 2 * (x * z + y * w),
+                         2 * (x * y + z * w), 1 - 2 * (x * x + z * z), 2 * (y * z - x * w),
+                         2 * (x * z - y * w), 2 * (y * z + x * w), 1 - 2 * (x * x + y * y)],
+                        1).reshape(hi - lo, 3, 3).astype(np.float32)
+
+    def posed(key, w_key=None, rotate_only=False):
+        """Packed (rows, 4) float32 of every entry's rows of ``key`` (entry i from base molecule
+        i % base), rotated; w = the rows of ``w_key`` when given; and the per-entry counts."""
+        bc = np.array([len(m[key]) for m in mols], np.int64)
+        bstart = np.cumsum(bc) - bc
+        brows = np.concatenate([np.asarray(m[key], np.float32).reshape(-1, 3) for m in mols])
+        bw = np.concatenate([np.asarray(m[w_key], np.float32).reshape(-1) for m in mols]) if w_key else None
+        counts = bc[which]
+        first = np.cumsum(counts) - counts
+        out = np.zeros((max(int(counts.sum()), 1), 4), np.float32)
+        for lo in range(0, n, chunk):
+            hi = min(n, lo + chunk)
+            cnt = counts[lo:hi]
+            owner = np.repeat(np.arange(hi - lo), cnt)
+            local = np.arange(int(cnt.sum())) - (first[lo:hi] - first[lo])[owner]
+            src = bstart[which[lo:hi]][owner] + local
+            dst = slice(int(first[lo]), int(first[lo]) + int(cnt.sum()))
+            out[dst, :3] = np.einsum("rij,rj->ri", rotations(lo, hi)[owner], brows[src])
+            if bw is not None:
+                out[dst, 3] = bw[src]
+        return out, counts
+
+    def rng_table(counts):
+        return torch.from_numpy(np.stack([np.cumsum(counts) - counts, counts], 1).astype(np.int32)).to(device)
+
+    dev = lambda a: torch.from_numpy(a).to(device)
+    pts, ac = posed("atom_pos")
+    shape = CloudSet(dev(pts), rng_table(ac), ac)
+    del pts
+    srf, sc = posed("surf_pos", "surf_esp")
+    surface = CloudSet(dev(srf), rng_table(sc), sc)
+    del srf
+    anc, pc = posed("pharm_ancs")
+    vec, _ = posed("pharm_vecs")
+    tb = np.concatenate([m["pharm_types"] for m in mols]).astype(np.int32)
+    bstart = np.cumsum([len(m["pharm_types"]) for m in mols]) - np.array([len(m["pharm_types"]) for m in mols])
+    first = np.cumsum(pc) - pc
+    owner = np.repeat(np.arange(n), pc)
+    typ = tb[bstart[which][owner] + (np.arange(int(pc.sum())) - first[owner])]
+    pharm = PharmSet(dev(anc), dev(vec), dev(np.ascontiguousarray(typ, np.int32)), rng_table(pc), pc)
+    return shape, pharm, surface

@@ -336,14 +336,17 @@ typedef struct FssScreen FssScreen;
 #define FSS_LIB_PHARM 2        /* anchors, unit vectors, types per conformer */
 #define FSS_LIB_SHAEP 4        /* ShaEP: surface rows (x, y, z, ESP), atoms with hydrogens
                                   (x, y, z, partial charge) and their radii per conformer */
+#define FSS_LIB_SURFACE 8      /* surface rows (x, y, z, ESP) alone, for the surface shape modes
+                                  (FSS_LIB_SHAEP holds them too); with FSS_LIB_SHAPE holding the
+                                  heavy atoms, one library serves the atom and surface modes */
 
 typedef struct FssLibSpec {
   int32_t device;              /* HIP device ordinal */
-  int32_t channels;            /* FSS_LIB_SHAPE | FSS_LIB_PHARM | FSS_LIB_SHAEP */
+  int32_t channels;            /* FSS_LIB_SHAPE | FSS_LIB_PHARM | FSS_LIB_SHAEP | FSS_LIB_SURFACE */
   int64_t max_conformers;
   int64_t max_points;          /* total shape rows */
   int64_t max_pharm;           /* total pharmacophore rows */
-  int64_t max_surface;         /* total ShaEP surface rows */
+  int64_t max_surface;         /* total surface rows (FSS_LIB_SHAEP or FSS_LIB_SURFACE) */
   int64_t max_atoms;           /* total ShaEP atom rows */
 } FssLibSpec;
 
@@ -356,9 +359,9 @@ typedef struct FssLibRows {
   const float* vec;            /* (sum(pcounts), 4) */
   const int32_t* typ;          /* (sum(pcounts)) */
   const int32_t* pcounts;      /* (n) */
-  const float* srf;            /* ShaEP: (sum(scounts), 4) surface points and ESP */
+  const float* srf;            /* ShaEP / surface: (sum(scounts), 4) surface points and ESP */
   const int32_t* scounts;      /* (n) */
-  const float* atm;            /* (sum(acounts), 4) atoms with hydrogens and partial charges */
+  const float* atm;            /* ShaEP: (sum(acounts), 4) atoms with hydrogens and partial charges */
   const float* rad;            /* (sum(acounts)) vdW radii */
   const int32_t* acounts;      /* (n) */
 } FssLibRows;
@@ -371,8 +374,8 @@ int fss_lib_create(const FssLibSpec* spec, FssLibrary** out);
 int fss_lib_upload(FssLibrary* lib, int32_t n, const float* pts, const int32_t* counts,
                    const float* anc, const float* vec, const int32_t* typ, const int32_t* pcounts,
                    int64_t* first);
-/* fss_lib_upload for every channel, the ShaEP one included.  Replaces: the with-H store
- * reader's per-molecule views (screen.py:830-846). */
+/* fss_lib_upload for every channel, the ShaEP and surface ones included.  Replaces: the with-H
+ * store reader's per-molecule views (screen.py:830-846). */
 int fss_lib_append(FssLibrary* lib, int32_t n, const FssLibRows* rows, int64_t* first);
 int fss_lib_size(const FssLibrary* lib, int64_t* n);
 /* Fails while a service on this library exists. */
@@ -397,11 +400,14 @@ typedef struct FssScreenParams {
                                                          lanes each, for up to 64 restarts) */
   int32_t seed_flags;          /* FSS_SEED_PHARM / FSS_SEED_SURF: seed from those rows */
   const float* quat_table;     /* (n_restarts, 4) host rotations of the restarts */
-  /* FSS_MODE_ESP_COMBO only */
-  int32_t max_query_surface;   /* ShaEP surface rows of one query */
-  int32_t max_query_atoms;     /* ShaEP atom rows of one query */
-  int32_t shape_from_surface;  /* 1: the shape term uses the surface rows (alpha != 0.81),
-                                  0: the shape channel's rows (heavy atoms) */
+  int32_t max_query_surface;   /* surface rows of one query (FSS_MODE_ESP_COMBO, or
+                                  shape_from_surface) */
+  int32_t max_query_atoms;     /* ShaEP atom rows of one query (FSS_MODE_ESP_COMBO) */
+  int32_t shape_from_surface;  /* 1: the shape term uses the library's surface rows and the
+                                  query's srf rows: FSS_MODE_ESP_COMBO at alpha != 0.81, and
+                                  the surface modes (FSS_MODE_SHAPE / FSS_MODE_SHAPE_ESP, ESP in
+                                  the rows' w) on a library with surface rows.  0: the shape
+                                  channel's rows. */
   int64_t query_budget_bytes;  /* device bytes for the rows of admitted queries (see
                                   fss_plan_query_pools); 0: max_active queries of the maximum
                                   size.  A queued query is admitted, oldest first, when a slot
@@ -451,6 +457,12 @@ int fss_screen_poll(FssScreen* svc, int64_t ticket, int32_t wait_ms, FssHits* ou
 /* Launches run, (query, library entry) pairs aligned, and the optimiser steps those pairs ran
  * (summed over pairs; each step of a pair covers all its restarts).  Any pointer may be NULL. */
 int fss_screen_stats(FssScreen* svc, int64_t* launches, int64_t* pairs, int64_t* pair_steps);
+/* The work those steps held: summed over aligned pairs, executed steps x restarts x row pairs of
+ * the shape term (query rows x entry rows), of the pharmacophore term (all row pairs, every
+ * type), and of the ShaEP potential (each side's surface rows x the other side's atoms).  The
+ * algorithmic FLOP of a screen is these counts times the per-term costs.  Any pointer may be
+ * NULL. */
+int fss_screen_terms(FssScreen* svc, double* shape_terms, double* pharm_terms, double* esp_terms);
 /* Stops the worker; queries still queued or running fail with FSS_E_CLOSED.  No other call on
  * the service may be in progress or follow. */
 int fss_screen_destroy(FssScreen* svc);

@@ -119,9 +119,15 @@ def test_screening_abi_rejects_bad_arguments_without_device_work():
     lib = _lib.load()
     spec = _lib.FssLibSpec()
     out = ctypes.c_void_p()
-    spec.channels, spec.max_conformers = 8, 10
+    spec.channels, spec.max_conformers = 16, 10
     assert lib.fss_lib_create(_lib.byref(spec), ctypes.byref(out)) == _lib.E_INVALID
     assert b"channels" in lib.fss_last_error()
+    spec.channels, spec.max_surface = _lib.LIB_SHAEP | _lib.LIB_SURFACE, 10
+    assert lib.fss_lib_create(_lib.byref(spec), ctypes.byref(out)) == _lib.E_INVALID
+    assert b"already holds the surface rows" in lib.fss_last_error()
+    spec.channels, spec.max_surface = _lib.LIB_SURFACE, 0
+    assert lib.fss_lib_create(_lib.byref(spec), ctypes.byref(out)) == _lib.E_INVALID
+    assert b"max_surface" in lib.fss_last_error()
     spec.channels, spec.max_surface, spec.max_atoms = _lib.LIB_SHAEP, 0, 10
     assert lib.fss_lib_create(_lib.byref(spec), ctypes.byref(out)) == _lib.E_INVALID
     assert b"max_surface" in lib.fss_last_error()

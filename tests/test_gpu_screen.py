@@ -230,6 +230,72 @@ def test_store_library_screens_like_in_memory(dev, tmp_path, mode):
     assert a[0].id == "mol33"
 
 
+def test_one_library_serves_atom_and_surface_modes(dev):
+    """A library with heavy atoms, pharmacophores and surface rows (points + ESP, no ShaEP atoms)
+    screens vol, vol_color, pharm, surf and surf_esp with the same hits as single-purpose
+    libraries holding each mode's rows in the shape channel."""
+    print("case one library, five modes", flush=True)
+    rng = np.random.default_rng(31)
+    S = 50
+    atoms = synth.random_walks(rng, rng.integers(10, 26, size=180))
+    surf = [synth.surface_points(rng, a, S) for a in atoms]
+    esp = [synth.esp_values(rng, S) for _ in atoms]
+    ph = [synth.pharmacophores(rng, a, int(rng.integers(1, 17))) for a in atoms]
+    pk = dict(pharm_types=[p[0].astype(np.int32) for p in ph], pharm_anchors=[p[1] for p in ph],
+              pharm_vectors=[p[2] for p in ph])
+    full = ConformerLibrary.from_arrays(points=atoms, surf_points=surf, surf_esp=esp, device=dev, **pk)
+    assert full.has_surface_rows and full.num_surf_points == S and full.shaep is None
+    single = {"vol": ConformerLibrary.from_arrays(points=atoms, device=dev),
+              "vol_color": ConformerLibrary.from_arrays(points=atoms, device=dev, **pk),
+              "pharm": ConformerLibrary.from_arrays(device=dev, **pk),
+              "surf": ConformerLibrary.from_arrays(points=surf, device=dev),
+              "surf_esp": ConformerLibrary.from_arrays(points=surf, esp=esp, device=dev)}
+    k = 17
+    qph = {"pharm_types": ph[k][0].astype(np.int32), "pharm_anchors": ph[k][1], "pharm_vectors": ph[k][2]}
+    queries = {"vol": {"points": atoms[k]}, "vol_color": {"points": atoms[k], **qph}, "pharm": qph,
+               "surf": {"points": surf[k], "atom_pos": atoms[k]},
+               "surf_esp": {"points": surf[k], "esp": esp[k], "atom_pos": atoms[k]}}
+    kw = dict(top_k=12, num_repeats=6, max_num_steps=30, launch_pairs=128)
+    for mode, lib in single.items():
+        a = screen(queries[mode], full, mode, **kw)
+        b = screen(queries[mode], lib, mode, **kw)
+        assert [h.id for h in a] == [h.id for h in b], mode
+        assert [h.score for h in a] == [h.score for h in b], mode
+        assert a[0].id == k, mode
+
+
+def test_screen_terms_count_every_executed_row_pair(dev):
+    """With early stop off every pair runs max_steps: the service's term counts are exactly
+    steps x restarts x query rows x entry rows summed over the library, per score term; the
+    screening library builder's entries are rigid copies of its base molecules."""
+    from fast_shepherd_score_amd.screen import ConformerLibrary, ScreenService
+    print("case screen term counts", flush=True)
+    shape, pharm, surface = synth.screening_library(60, dev, base=25, seed=3)
+    lib = ConformerLibrary(shape=shape, pharm=pharm, surface=surface)
+    lib.num_surf_points = 200
+    base = synth.screening_molecules(25, seed=3)
+    assert list(shape.counts) == [len(base[i % 25]["atom_pos"]) for i in range(60)]
+    s, c = shape.rng[37].tolist()
+    a, b = shape.pts[s:s + c, :3].cpu().numpy(), base[37 % 25]["atom_pos"]
+    d = lambda x: np.linalg.norm(x[:, None] - x[None], axis=-1)
+    np.testing.assert_allclose(d(a), d(b), atol=1e-4)
+    q = synth.screening_molecules(1, seed=99)[0]
+    B, steps = 4, 20
+    kw = dict(num_repeats=B, max_num_steps=steps, top_k=10, early_stop=False, launch_pairs=64)
+    with ScreenService(lib, "vol_color", **kw) as svc:
+        svc.submit({"points": q["atom_pos"], "pharm_types": q["pharm_types"], "pharm_anchors": q["pharm_ancs"],
+                    "pharm_vectors": q["pharm_vecs"]}).result(timeout=120)
+        st = svc.stats()
+    assert st["pairs"] == 60 and st["pair_steps"] == 60 * steps
+    assert st["shape_terms"] == steps * B * len(q["atom_pos"]) * int(shape.counts.sum())
+    assert st["pharm_terms"] == steps * B * len(q["pharm_types"]) * int(pharm.counts.sum())
+    with ScreenService(lib, "surf_esp", **kw) as svc:
+        hits = svc.submit({"points": q["surf_pos"], "esp": q["surf_esp"], "atom_pos": q["atom_pos"]}).result(timeout=120)
+        st = svc.stats()
+    assert st["shape_terms"] == steps * B * 200 * 200 * 60 and st["pharm_terms"] == 0
+    assert len(hits.hits) == 10
+
+
 @pytest.mark.parametrize("only", [False, True])
 def test_pharm_screen_extended_points_match_oracle(dev, only):
     """pharm screening with extended points (screen.py:866-867): per-pair best scores equal the
