@@ -240,6 +240,8 @@ def _poisson_run(svc, queries, rate: float, n: int, rng):
         while time.perf_counter() - start < arrivals[k]:
             time.sleep(0.0005)
         futs.append(svc.submit(queries[pick[k]]))
+        if (k + 1) % 50 == 0:
+            log(f"  {k + 1}/{n} queries submitted, {sum(f.done() for f in futs)} done")
     res = [f.result() for f in futs]
     wall = time.perf_counter() - start
     after = svc.stats()
@@ -293,6 +295,7 @@ def bench_screen(dev, n_lib: int = 1_000_000, n_queries: int = 40, loads=(0.5, 0
                          "hbm_library_gb_per_s": n_queries * atom_bytes / wall / 1e9})
             log(f"screen vol load {load:.0%}: p50 {legs[-1]['p50_ms']:.0f} ms, p99 {legs[-1]['p99_ms']:.0f} ms")
         adm = svc.admission()
+        total = svc.stats()
     per_query = warm["shape_terms"] / (len(sat) + 1)
     out["vol"] = {"capacity_pairs_per_s": cap_qps * n_lib, "capacity_queries_per_s": cap_qps,
                   "mean_pair_steps": warm["pair_steps"] / max(warm["pairs"], 1),
@@ -300,7 +303,9 @@ def bench_screen(dev, n_lib: int = 1_000_000, n_queries: int = 40, loads=(0.5, 0
                   # 16 FLOP per executed shape term, over the saturated burst's wall time (a lower
                   # bound on the kernel's rate: staging, seeding and merges are inside it)
                   "achieved_tflops_wall": per_query * len(sat) * 16 / busy / 1e12,
-                  "poisson": legs, "max_admitted": adm["max_admitted"]}
+                  "poisson": legs, "max_admitted": adm["max_admitted"],
+                  # the whole service's work, for a kernel-time roofline from a trace of this run
+                  "total": {k: total[k] for k in ("launches", "pairs", "pair_steps", "shape_terms")}}
     with ScreenService(lib, "surf_esp", launch_pairs=65536, **kw) as svc:
         t1 = time.perf_counter()
         svc.submit(qs[-1]).result()
@@ -313,15 +318,19 @@ def bench_screen(dev, n_lib: int = 1_000_000, n_queries: int = 40, loads=(0.5, 0
                        "pairs_per_s": surf_queries * n_lib / busy,
                        "p50_ms": float(np.percentile([r.latency_ms for r in res], 50)),
                        "mean_pair_steps": st["pair_steps"] / max(st["pairs"], 1),
-                       "achieved_tflops_wall": st["shape_terms"] * surf_queries / (surf_queries + 1) * 19 / busy / 1e12}
+                       "achieved_tflops_wall": st["shape_terms"] * surf_queries / (surf_queries + 1) * 19 / busy / 1e12,
+                       "total": {k: st[k] for k in ("launches", "pairs", "pair_steps", "shape_terms")}}
     log("screen surf_esp done")
     with ScreenService(lib, "vol_color", launch_pairs=262144, **kw) as svc:
         svc.submit(qc[-1]).result()
         t1 = time.perf_counter()
         res = [f.result() for f in [svc.submit(q) for q in qc[:color_queries]]]
         busy = time.perf_counter() - t1
+        st = svc.stats()
     out["vol_color"] = {"queries": color_queries, "pairs_per_s": color_queries * n_lib / busy,
-                        "p50_ms": float(np.percentile([r.latency_ms for r in res], 50))}
+                        "p50_ms": float(np.percentile([r.latency_ms for r in res], 50)),
+                        "total": {k: st[k] for k in ("launches", "pairs", "pair_steps", "shape_terms",
+                                                     "pharm_terms")}}
     lib.close()
     return out
 

@@ -147,9 +147,58 @@ __device__ __forceinline__ void group_sum_any(Acc& a, int width, int j, bool gra
     for (int i = 0; i < 9; ++i) a.G[i] = group_sum_any(a.G[i], width, j);
   }
 }
-// A pose's sums over its lane group: DPP levels for power-of-two widths, the shuffle tree else.
-__device__ __forceinline__ void pose_sum(Acc& a, int width, int j, bool grad) {
+// The same sums through LDS, for a workgroup that is one wavefront of `nl` grouped lanes (lane
+// groups of any width): every lane stores its 13 partials (red[13 * lane + k], an odd stride, so
+// the stores of a row of lanes hit distinct banks); lane j of a group then adds partial k = j,
+// j + width, ... over the group's lanes in lane order and stores the total at
+// red[13 * (nl + grp) + k], and every lane of the group loads the 13 totals.  One lane forms each total, so a group's lanes end with the
+// same bits.  A wavefront's LDS operations complete in issue order; the wavefront-scope fences
+// keep the compiler from moving the loads of other lanes' values across the stores.
+__device__ __forceinline__ void group_sum_lds(Acc& a, int width, int j, int grp, int nl, float* red,
+                                              bool grad) {
+  const int lane = threadIdx.x, n = grad ? 13 : 1;
+  float v[13];
+  v[0] = a.V;
+#pragma unroll
+  for (int i = 0; i < 3; ++i) v[1 + i] = a.gt[i];
+#pragma unroll
+  for (int i = 0; i < 9; ++i) v[4 + i] = a.G[i];
+  for (int k = 0; k < n; ++k) red[13 * lane + k] = v[k];
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  const float* part = red + 13 * (lane - j);
+  float* tot = red + 13 * (nl + grp);
+  for (int k = j; k < n; k += width) {
+    float s = part[k];
+    for (int i = 1; i < width; ++i) s += part[13 * i + k];
+    tot[k] = s;
+  }
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  a.V = tot[0];
+  if (grad) {
+#pragma unroll
+    for (int i = 0; i < 3; ++i) a.gt[i] = tot[1 + i];
+#pragma unroll
+    for (int i = 0; i < 9; ++i) a.G[i] = tot[4 + i];
+  }
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+// Floats of LDS group_sum_lds takes for `groups` lane groups of `width` lanes: 13 partials per
+// lane, 13 totals per group.
+constexpr int red_floats(int groups, int width) { return 13 * (groups * width + groups); }
+
+// A pose's sums over its lane group: DPP levels for power-of-two widths; else the LDS exchange
+// when the kernel reserved `red` (a one-wavefront workgroup of nl grouped lanes), the shuffle
+// tree otherwise.
+__device__ __forceinline__ void pose_sum(Acc& a, int width, int j, bool grad, int grp = 0, int nl = 0,
+                                         float* red = nullptr) {
   if ((width & (width - 1)) == 0) group_sum(a, width, grad);
+  else if (red) group_sum_lds(a, width, j, grp, nl, red, grad);
   else group_sum_any(a, width, j, grad);
 }
 

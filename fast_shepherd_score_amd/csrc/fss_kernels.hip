@@ -18,6 +18,8 @@
 #include <stdarg.h>
 #include <stdio.h>
 #include <string.h>
+
+#include <cmath>
 #include <mutex>
 #include <string>
 
@@ -191,6 +193,8 @@ LdsPlan plan_lds(const FssPairBatch* pb, int mode, bool mfma = false) {
   p.end = o;
   return p;
 }
+
+constexpr size_t kCuLdsBytes = 160 * 1024;   // LDS of one gfx950 compute unit
 
 size_t max_lds_bytes() {
   int dev = 0, v = 0;
@@ -653,13 +657,21 @@ __global__ void k_adam_qt(int n, float* q, float* t, const float* __restrict__ d
 // ------------------------------------------------------------------------------------------------
 // Persistent optimiser.
 // ------------------------------------------------------------------------------------------------
+// Adam's bias-corrected step factors of the first kSchedSteps steps, formed on the host in FP64
+// exactly as the kernel would (the same correctly rounded products, quotient and square root),
+// so a step reads two floats from the kernel arguments instead of an FP64 divide and square root.
+constexpr int kSchedSteps = 256;
+
 struct AlignK {
   int B, max_steps;
   float lr, b1, b2, eps;
+  float sched[2 * kSchedSteps];   // step s (1-based): lr / (1 - b1^s), 1 / sqrt(1 - b2^s)
+  double b1p_tail, b2p_tail;      // b1^kSchedSteps, b2^kSchedSteps: the products the kernel continues
   float es_tol;
   int es_pat;
   int lpp, rpw, wg_per_pair;
   int groups;                 // lane groups per workgroup; each runs ceil(rpw / groups) restarts
+  int red_lds;                // lane groups of a non-power-of-two width sum through LDS
   const int32_t* order;
   const double* self_ov;
   float* q;
@@ -698,6 +710,9 @@ void k_align(FssPairBatch pb, KScore ks, LdsPlan lp, AlignK ak) {
   const double* so = ak.self_ov + 4 * pair;
   const float invB = 1.0f / (float)ak.B;
   float* slots = reinterpret_cast<float*>(lossbuf + (ES ? 2 * ak.rpw : 0));
+  // lane groups of a width other than a power of two sum through LDS when the plan reserved it
+  float* red = ak.red_lds ? slots + 24 * ak.rpw : nullptr;
+  const int nl = groups * lpp;
   auto local = [&](int i) { return grp + i * groups; };
   auto runs = [&](int rl) { return grp < groups && rl < ak.rpw && blk * ak.rpw + rl < ak.B; };
   auto pose_of = [&](int rl) { return (long)pair * ak.B + blk * ak.rpw + rl; };
@@ -727,13 +742,19 @@ void k_align(FssPairBatch pb, KScore ks, LdsPlan lp, AlignK ak) {
 
   // early-stop reference loss before the first step: 1.0 for the analytic optimisers
   // (_torch_analytical.py:110, 283, 420), +inf for the ShaEP one (alignment/_torch.py:922)
-  double b1p = 1.0, b2p = 1.0, last = SE ? (double)INFINITY : 1.0;
+  double b1p = ak.b1p_tail, b2p = ak.b2p_tail, last = SE ? (double)INFINITY : 1.0;
   int counter = 0, done = ak.max_steps;
   for (int step = 1; step <= ak.max_steps; ++step) {
-    b1p *= (double)ak.b1;
-    b2p *= (double)ak.b2;
-    const float step_size = (float)((double)ak.lr / (1.0 - b1p));
-    const float ibc2s = (float)(1.0 / sqrt(1.0 - b2p));
+    float step_size, ibc2s;
+    if (step <= kSchedSteps) {
+      step_size = ak.sched[2 * step - 2];
+      ibc2s = ak.sched[2 * step - 1];
+    } else {
+      b1p *= (double)ak.b1;
+      b2p *= (double)ak.b2;
+      step_size = (float)((double)ak.lr / (1.0 - b1p));
+      ibc2s = (float)(1.0 / sqrt(1.0 - b2p));
+    }
     double* buf = lossbuf + (step & 1) * ak.rpw;
     for (int i = 0; i < rpg; ++i) {
       const int rl = local(i);
@@ -756,18 +777,18 @@ void k_align(FssPairBatch pb, KScore ks, LdsPlan lp, AlignK ak) {
         if (MF) shape_accum_mfma<ESP>(st.ref, st.xq, st.T, st.fit, st.M, R, ts, sh);
         else if (EXPD) shape_accum<ESP, true>(st.ref, st.refc, st.N, st.fit, st.M, j, lpp, R, ts, sh);
         else shape_accum_direct<ESP, true>(st.ref, st.N, st.fit, st.M, j, lpp, R, ts, sh);
-        pose_sum(sh, lpp, j, true);
+        pose_sum(sh, lpp, j, true, grp, nl, red);
         shape_finish(sh, ks.K, ks.alpha, ks.s, true);
       }
       if (PH) {
         pharm_accum<true>(st.ranc, st.rvec, st.poff, st.fanc, st.fvec, st.Mp, j, lpp, R, p + 4, ks.tab, ph);
-        pose_sum(ph, lpp, j, true);
+        pose_sum(ph, lpp, j, true, grp, nl, red);
       }
       if (SE) {
         const float tr[3] = {p[4] - st.co[0], p[5] - st.co[1], p[6] - st.co[2]};
         shaep_accum<true>(st.srf_r, st.Ns, st.atm_f, st.rb_f, st.Ma, st.srf_f, st.Ms, st.atm_r, st.rb_r, st.Na,
                           j, lpp, R, tr, ks.se, ph);
-        pose_sum(ph, lpp, j, true);
+        pose_sum(ph, lpp, j, true, grp, nl, red);
       }
       float ws, wp;
       const float S = similarity<float>(ks, sh.V, ph.V, so, SH, PH, ws, wp);
@@ -1477,14 +1498,32 @@ int plan_align(const FssPairBatch* batch, const KScore& ks, const FssAlignParams
   pl.lp = plan_lds(batch, ks.mode, mfma);
   pl.lds = (size_t)pl.lp.end * sizeof(float4) + (ap->early_stop ? 2 * (size_t)rpw * sizeof(double) : 0) +
            (size_t)rpw * 24 * sizeof(float);
+  // Lane groups of a non-power-of-two width sum through LDS (about a sixth of the shuffle tree's
+  // instructions per step), unless the scratch would cost occupancy: with it a workgroup must
+  // still fit 16 to a CU (4 one-wavefront workgroups per SIMD, the VGPR limit).
+  const size_t red = (size_t)red_floats(groups, lpp) * sizeof(float);
+  const bool red_lds = (lpp & (lpp - 1)) && pl.lds + red <= kCuLdsBytes / 16;
+  if (red_lds) pl.lds += red;
   const size_t cap = max_lds_bytes();
   if (pl.lds > cap) return set_error(FSS_E_TOO_LARGE, "clouds need %zu B of LDS, the device allows %zu B", pl.lds, cap);
   AlignK& ak = pl.ak;
   ak.B = B; ak.max_steps = ap->max_steps;
   ak.lr = ap->lr; ak.b1 = ap->beta1; ak.b2 = ap->beta2; ak.eps = ap->eps;
+  {
+    double b1p = 1.0, b2p = 1.0;
+    for (int k = 0; k < kSchedSteps; ++k) {
+      b1p *= (double)ak.b1;
+      b2p *= (double)ak.b2;
+      ak.sched[2 * k] = (float)((double)ak.lr / (1.0 - b1p));
+      ak.sched[2 * k + 1] = (float)(1.0 / std::sqrt(1.0 - b2p));
+    }
+    ak.b1p_tail = b1p;
+    ak.b2p_tail = b2p;
+  }
   ak.es_tol = ap->es_tol; ak.es_pat = ap->es_patience;
   ak.lpp = lpp; ak.rpw = rpw; ak.wg_per_pair = (B + rpw - 1) / rpw;
   ak.groups = groups;
+  ak.red_lds = red_lds;
   ak.order = ap->pair_order;
   pl.grid = (long)batch->n_pairs * ak.wg_per_pair;
   if (pl.grid > 0x7fffffffL) return set_error(FSS_E_INVALID, "grid too large");
