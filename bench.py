@@ -304,6 +304,7 @@ def bench_screen(dev, n_lib: int = 1_000_000, n_queries: int = 40, loads=(0.5, 0
                   # bound on the kernel's rate: staging, seeding and merges are inside it)
                   "achieved_tflops_wall": per_query * len(sat) * 16 / busy / 1e12,
                   "poisson": legs, "max_admitted": adm["max_admitted"],
+                  "tiling": {"lanes_per_pose": svc.lanes_per_pose, "pairs_per_wg": svc.pairs_per_wg},
                   # the whole service's work, for a kernel-time roofline from a trace of this run
                   "total": {k: total[k] for k in ("launches", "pairs", "pair_steps", "shape_terms")}}
     with ScreenService(lib, "surf_esp", launch_pairs=65536, **kw) as svc:

@@ -97,6 +97,7 @@ class FssAlignParams(ctypes.Structure):
         ("skip_finish", ctypes.c_int32),
         ("variant", ctypes.c_int32),
         ("groups_per_wg", ctypes.c_int32),
+        ("pairs_per_wg", ctypes.c_int32),
     ]
 
 
@@ -138,6 +139,7 @@ class FssScreenParams(ctypes.Structure):
         ("max_query_surface", ctypes.c_int32), ("max_query_atoms", ctypes.c_int32),
         ("shape_from_surface", ctypes.c_int32),
         ("query_budget_bytes", ctypes.c_int64),
+        ("pairs_per_wg", ctypes.c_int32),
     ]
 
 

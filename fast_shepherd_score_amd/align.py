@@ -202,17 +202,20 @@ def default_tiling(pb: PairBatch, mode: int, B: int, early_stop: bool) -> Tuple[
     return cands[0][2], cands[0][3]
 
 
-def check_tiling(B: int, lpp: int, rpw: int, early_stop: bool, groups: int = 0) -> None:
+def check_tiling(B: int, lpp: int, rpw: int, early_stop: bool, groups: int = 0, pairs: int = 1) -> None:
     g = groups or rpw
+    ppw = max(int(pairs or 1), 1)
     if g > rpw:
         raise ValueError("groups_per_wg exceeds restarts_per_wg")
-    if lpp < 1 or lpp > 64 or (lpp & (lpp - 1) and g * lpp > 64):
+    if ppw > 1 and rpw < B:
+        raise ValueError("pairs_per_wg > 1 needs every restart of each pair in the workgroup")
+    if lpp < 1 or lpp > 64 or (lpp & (lpp - 1) and ppw * g * lpp > 64):
         raise ValueError("lanes_per_pose must be in [1, 64], and a power of two unless the workgroup's "
-                         "lane groups x lanes_per_pose fit one wavefront (64)")
+                         "pairs x lane groups x lanes_per_pose fit one wavefront (64)")
     if early_stop and rpw < B:
         raise ValueError("per-pair early stop needs every restart of a pair in one workgroup")
-    if g * lpp > 1024:
-        raise ValueError("lane groups x lanes_per_pose exceed 1024 threads")
+    if ppw * g * lpp > 1024:
+        raise ValueError("pairs x lane groups x lanes_per_pose exceed 1024 threads")
 
 
 def _prep_seeds(pb: PairBatch, q0: torch.Tensor, t0: torch.Tensor):
@@ -231,7 +234,8 @@ def align(pb: PairBatch, spec: ScoreSpec, q0: torch.Tensor, t0: torch.Tensor,
           opt: OptimSpec = OptimSpec(), *, lanes_per_pose: Optional[int] = None,
           restarts_per_wg: Optional[int] = None, order: str = "cost",
           self_ov: Optional[torch.Tensor] = None, stream=None, events: Optional[list] = None,
-          tune: bool = False, variant: int = 0, groups_per_wg: Optional[int] = None) -> AlignResult:
+          tune: bool = False, variant: int = 0, groups_per_wg: Optional[int] = None,
+          pairs_per_wg: Optional[int] = None) -> AlignResult:
     """Persistent path: every step of every pair in one launch.
 
     Tiling: explicit ``lanes_per_pose`` / ``restarts_per_wg``, else an autotuned choice
@@ -240,6 +244,8 @@ def align(pb: PairBatch, spec: ScoreSpec, q0: torch.Tensor, t0: torch.Tensor,
     (one wavefront per pose, so 64 lanes per pose), 0 the tuned or default choice.
     ``groups_per_wg``: lane groups per workgroup, each running restarts_per_wg / groups_per_wg
     restarts in turn every step (None: the tuned choice, else one restart per group).
+    ``pairs_per_wg``: pairs per workgroup (each with all its restarts): consecutive pairs of the
+    launch order share a workgroup, so small clouds share one wavefront's per-step pose work.
     ``events``: if a list is given, HIP events bracketing the optimiser kernel and the finishing
     kernels are appended to it as (start, optimiser_end, finish_end).
     """
@@ -271,7 +277,8 @@ def align(pb: PairBatch, spec: ScoreSpec, q0: torch.Tensor, t0: torch.Tensor,
         # the MFMA loop needs a wavefront per pose: with every restart of a pair in one
         # workgroup, 50 restarts run as 5 wavefronts of 10 restarts each
         grp = max(d for d in range(1, 17) if B % d == 0)
-    check_tiling(B, lpp, rpw, opt.early_stop, grp)
+    ppw = int(pairs_per_wg or 1)
+    check_tiling(B, lpp, rpw, opt.early_stop, grp, ppw)
     order_t = None
     if order == "cost" and P > 1:
         c = pb.cost(spec.mode)
@@ -290,6 +297,7 @@ def align(pb: PairBatch, spec: ScoreSpec, q0: torch.Tensor, t0: torch.Tensor,
     ap.pair_order = _lib.ptr(order_t)
     ap.variant = int(variant)
     ap.groups_per_wg = grp
+    ap.pairs_per_wg = ppw
     out = _lib.FssAlignOut()
     out.q, out.t = _lib.ptr(q), _lib.ptr(t)
     out.pose_score, out.best_score, out.best_idx = _lib.ptr(pose_score), _lib.ptr(best), _lib.ptr(bidx)
@@ -312,7 +320,7 @@ def align(pb: PairBatch, spec: ScoreSpec, q0: torch.Tensor, t0: torch.Tensor,
         events.append(tuple(ev))
     return AlignResult(best, bidx, tf, q, t, pose_score, nst,
                        extra={"lanes_per_pose": lpp, "restarts_per_wg": rpw, "variant": int(variant),
-                              "groups_per_wg": grp or rpw, "order": order_t})
+                              "groups_per_wg": grp or rpw, "pairs_per_wg": ppw, "order": order_t})
 
 
 class SteppedAligner:

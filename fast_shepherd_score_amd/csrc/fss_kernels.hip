@@ -159,8 +159,8 @@ int check_batch(const FssPairBatch* pb, int mode) {
   return FSS_OK;
 }
 
-// LDS plan for one workgroup (float4 offsets): shape ref rows, fit rows, ref ESP column,
-// a 16-float4 reduction scratch, then pharm ref anchors / vectors, fit anchors / vectors.
+// LDS plan for one pair (float4 offsets): shape ref rows, fit rows, ref ESP column, one float4
+// for the ref centroid, then pharm ref anchors / vectors, fit anchors / vectors.
 // With `mfma` (inner-loop variant 3) the ref rows are the split-bf16 A-operand tiles (64 float4
 // per 16 rows) followed by the 2 a~ row quads (12 float4 per 16 rows), and no ESP column.
 struct LdsPlan {
@@ -177,7 +177,7 @@ LdsPlan plan_lds(const FssPairBatch* pb, int mode, bool mfma = false) {
   p.xq = o; o += sh && mfma ? 12 * tiles : 0;
   p.fit = o; o += sh ? pb->max_fit : 0;
   p.refc = o; o += mode == FSS_MODE_SHAPE_ESP && !mfma ? (pb->max_ref + 3) / 4 : 0;
-  p.red = o; o += 16;
+  p.red = o; o += 1;                 // the ref centroid of stage_pair (3 floats)
   p.poff = o; o += ph ? 4 : 0;
   p.ranc = o; o += ph ? pb->max_ref_ph : 0;
   p.rvec = o; o += ph ? pb->max_ref_ph : 0;
@@ -672,6 +672,8 @@ struct AlignK {
   int lpp, rpw, wg_per_pair;
   int groups;                 // lane groups per workgroup; each runs ceil(rpw / groups) restarts
   int red_lds;                // lane groups of a non-power-of-two width sum through LDS
+  int ppw;                    // pairs per workgroup (> 1: each with all its restarts)
+  int region;                 // float4s between the LDS regions of a workgroup's pairs
   const int32_t* order;
   const double* self_ov;
   float* q;
@@ -683,7 +685,9 @@ struct AlignK {
 // Every instance is held to 128 VGPRs (4 wavefronts per SIMD).  The MFMA loop is latency-bound,
 // and shape + pharm with early stop would otherwise take 133 VGPRs: with 4 spilled dwords it
 // runs 6 % faster at 4 wavefronts than at 3 (16384 pairs, 38.2 -> 35.8 ms).
-template <int MODE, bool ES, int MAXT, int VAR>
+// MP: several pairs per workgroup (AlignK::ppw > 1), a separate instance so the one-pair
+// instances keep their pair's staging offsets and counts uniform (scalar) across the workgroup.
+template <int MODE, bool ES, int MAXT, int VAR, bool MP = false>
 __global__ __launch_bounds__(MAXT) __attribute__((amdgpu_waves_per_eu(4)))
 void k_align(FssPairBatch pb, KScore ks, LdsPlan lp, AlignK ak) {
   constexpr bool EXPD = VAR != 1;
@@ -693,28 +697,51 @@ void k_align(FssPairBatch pb, KScore ks, LdsPlan lp, AlignK ak) {
   constexpr bool ESP = MODE == FSS_MODE_SHAPE_ESP;
   constexpr bool PH = MODE == FSS_MODE_PHARM || MODE == FSS_MODE_SHAPE_PHARM;
   extern __shared__ float4 lds[];
-  const int slot = blockIdx.x / ak.wg_per_pair;
-  const int blk = blockIdx.x - slot * ak.wg_per_pair;
-  const int pair = ak.order ? ak.order[slot] : slot;
-  const Staged st = stage_pair<SH, ESP, PH, EXPD, MF, SE>(pb, ks, lp, pair, 0, 1, lds);
-  double* lossbuf = reinterpret_cast<double*>(lds + lp.end);  // 2 * rpw doubles (ES only)
-
-  const int lpp = ak.lpp, groups = ak.groups;
-  const int grp = threadIdx.x / lpp, j = threadIdx.x - grp * lpp;
+  const int lpp = ak.lpp, groups = ak.groups, ppw = MP ? ak.ppw : 1;
+  // ppw pairs per workgroup (each with every restart in the workgroup): lane group g works on
+  // the workgroup's pair g / groups; with one pair the workgroup may hold a block of restarts.
+  const int grp_all = threadIdx.x / lpp, j = threadIdx.x - grp_all * lpp;
+  const int pl = MP ? grp_all / groups : (grp_all < groups ? 0 : 1);
+  const int grp = MP ? grp_all - pl * groups : grp_all;
+  const int slot0 = ppw > 1 ? blockIdx.x * ppw : blockIdx.x / ak.wg_per_pair;
+  const int blk = ppw > 1 ? 0 : blockIdx.x - slot0 * ak.wg_per_pair;
+  // lanes past the workgroup's lane groups (and groups of a missing last pair) have no pair
+  const bool has_pair = pl < ppw && (ppw == 1 || slot0 + pl < pb.n_pairs);
+  const int pl_buf = ppw == 1 ? 0 : pl;     // one pair: every lane reads that pair's buffers
+  // one pair: the workgroup's pair for every lane (uniform); several: the lane's own
+  const int pair = !MP ? (ak.order ? ak.order[slot0] : slot0)
+                       : has_pair ? (ak.order ? ak.order[slot0 + pl] : slot0 + pl) : 0;
+  Staged st{};
+  if (ppw == 1) {
+    st = stage_pair<SH, ESP, PH, EXPD, MF, SE>(pb, ks, lp, pair, 0, 1, lds);
+  } else {
+    for (int k = 0; k < ppw && slot0 + k < pb.n_pairs; ++k) {   // the same trip count everywhere
+      const int pk = ak.order ? ak.order[slot0 + k] : slot0 + k;
+      const Staged sk = stage_pair<SH, ESP, PH, EXPD, MF, SE>(pb, ks, lp, pk, 0, 1, lds + k * ak.region);
+      if (k == pl) st = sk;
+    }
+  }
+  // per pair of the workgroup: 2 * rpw doubles of restart losses (ES only), then rpw slots
+  float4* const tail = lds + (MP ? ppw * ak.region : lp.end);
+  double* lossbuf = reinterpret_cast<double*>(tail) + (ES ? pl_buf * 2 * ak.rpw : 0);
   // Each lane group runs the restarts grp, grp + groups, ... of the workgroup's block, one after
   // the other in every step.  A restart's Adam moments (and, when its group runs more than one,
-  // its parameters) live in a 24-float LDS slot: m at 0..6, v at 7..13, q and t at 14..20; lane 0
-  // of the group writes, the group's lanes read (one wavefront, in-order DS operations).  With
-  // one restart per group the parameters stay in registers across steps.
+  // its parameters) live in an LDS slot: m at 0..6, v at 7..13, and q and t at 14..20 in the
+  // 24-float slots of groups running several restarts; lane 0 of the group writes, the group's
+  // lanes read (one wavefront, in-order DS operations).  With one restart per group the
+  // parameters stay in registers across steps and a slot is 14 floats.
   const int rpg = (ak.rpw + groups - 1) / groups;
   const double* so = ak.self_ov + 4 * pair;
   const float invB = 1.0f / (float)ak.B;
-  float* slots = reinterpret_cast<float*>(lossbuf + (ES ? 2 * ak.rpw : 0));
+  float* slot_base = reinterpret_cast<float*>(reinterpret_cast<double*>(tail) + (ES ? ppw * 2 * ak.rpw : 0));
+  // a restart's slot: Adam moments, and its parameters when its group runs several restarts
+  const int sls = rpg > 1 ? 24 : 14;
+  float* slots = slot_base + pl_buf * sls * ak.rpw;
   // lane groups of a width other than a power of two sum through LDS when the plan reserved it
-  float* red = ak.red_lds ? slots + 24 * ak.rpw : nullptr;
-  const int nl = groups * lpp;
+  float* red = ak.red_lds ? slot_base + ppw * sls * ak.rpw : nullptr;
+  const int nl = ppw * groups * lpp;
   auto local = [&](int i) { return grp + i * groups; };
-  auto runs = [&](int rl) { return grp < groups && rl < ak.rpw && blk * ak.rpw + rl < ak.B; };
+  auto runs = [&](int rl) { return has_pair && rl < ak.rpw && blk * ak.rpw + rl < ak.B; };
   auto pose_of = [&](int rl) { return (long)pair * ak.B + blk * ak.rpw + rl; };
   float p[7];
 #pragma unroll
@@ -723,7 +750,7 @@ void k_align(FssPairBatch pb, KScore ks, LdsPlan lp, AlignK ak) {
     const int rl = local(i);
     if (!runs(rl)) continue;
     const long pose = pose_of(rl);
-    float* sl = slots + 24 * rl;
+    float* sl = slots + sls * rl;
     if (rpg == 1) {
 #pragma unroll
       for (int k = 0; k < 4; ++k) p[k] = ak.q[4 * pose + k];
@@ -744,6 +771,8 @@ void k_align(FssPairBatch pb, KScore ks, LdsPlan lp, AlignK ak) {
   // (_torch_analytical.py:110, 283, 420), +inf for the ShaEP one (alignment/_torch.py:922)
   double b1p = ak.b1p_tail, b2p = ak.b2p_tail, last = SE ? (double)INFINITY : 1.0;
   int counter = 0, done = ak.max_steps;
+  // this lane's pair still steps: with one pair every lane (grouped or not) decides alike
+  bool active = ppw == 1 || has_pair;
   for (int step = 1; step <= ak.max_steps; ++step) {
     float step_size, ibc2s;
     if (step <= kSchedSteps) {
@@ -758,8 +787,8 @@ void k_align(FssPairBatch pb, KScore ks, LdsPlan lp, AlignK ak) {
     double* buf = lossbuf + (step & 1) * ak.rpw;
     for (int i = 0; i < rpg; ++i) {
       const int rl = local(i);
-      if (!runs(rl)) continue;
-      float* sl = slots + 24 * rl;
+      if (!active || !runs(rl)) continue;
+      float* sl = slots + sls * rl;
       if (rpg > 1) {
 #pragma unroll
         for (int k = 0; k < 7; ++k) p[k] = sl[14 + k];
@@ -777,18 +806,18 @@ void k_align(FssPairBatch pb, KScore ks, LdsPlan lp, AlignK ak) {
         if (MF) shape_accum_mfma<ESP>(st.ref, st.xq, st.T, st.fit, st.M, R, ts, sh);
         else if (EXPD) shape_accum<ESP, true>(st.ref, st.refc, st.N, st.fit, st.M, j, lpp, R, ts, sh);
         else shape_accum_direct<ESP, true>(st.ref, st.N, st.fit, st.M, j, lpp, R, ts, sh);
-        pose_sum(sh, lpp, j, true, grp, nl, red);
+        pose_sum(sh, lpp, j, true, grp_all, nl, red);
         shape_finish(sh, ks.K, ks.alpha, ks.s, true);
       }
       if (PH) {
         pharm_accum<true>(st.ranc, st.rvec, st.poff, st.fanc, st.fvec, st.Mp, j, lpp, R, p + 4, ks.tab, ph);
-        pose_sum(ph, lpp, j, true, grp, nl, red);
+        pose_sum(ph, lpp, j, true, grp_all, nl, red);
       }
       if (SE) {
         const float tr[3] = {p[4] - st.co[0], p[5] - st.co[1], p[6] - st.co[2]};
         shaep_accum<true>(st.srf_r, st.Ns, st.atm_f, st.rb_f, st.Ma, st.srf_f, st.Ms, st.atm_r, st.rb_r, st.Na,
                           j, lpp, R, tr, ks.se, ph);
-        pose_sum(ph, lpp, j, true, grp, nl, red);
+        pose_sum(ph, lpp, j, true, grp_all, nl, red);
       }
       float ws, wp;
       const float S = similarity<float>(ks, sh.V, ph.V, so, SH, PH, ws, wp);
@@ -815,13 +844,17 @@ void k_align(FssPairBatch pb, KScore ks, LdsPlan lp, AlignK ak) {
     }
     if (ES) {
       __syncthreads();
-      double L = 0.0;
-      for (int i = 0; i < ak.B; ++i) L += buf[i];
-      L /= (double)ak.B;
-      if (fabs(L - last) > (double)ak.es_tol) counter = 0;
-      else ++counter;
-      last = L;
-      if (counter > ak.es_pat) { done = step; break; }
+      if (active) {
+        double L = 0.0;
+        for (int i = 0; i < ak.B; ++i) L += buf[i];
+        L /= (double)ak.B;
+        if (fabs(L - last) > (double)ak.es_tol) counter = 0;
+        else ++counter;
+        last = L;
+        if (counter > ak.es_pat) { done = step; active = false; }
+      }
+      // one pair: every lane decided alike; several: step on while any pair of the workgroup runs
+      if (ppw == 1 ? !active : !__syncthreads_or(active)) break;
     }
   }
 
@@ -829,13 +862,13 @@ void k_align(FssPairBatch pb, KScore ks, LdsPlan lp, AlignK ak) {
     const int rl = local(i);
     if (!runs(rl) || j != 0) continue;
     const long pose = pose_of(rl);
-    const float* src = rpg > 1 ? slots + 24 * rl + 14 : p;
+    const float* src = rpg > 1 ? slots + sls * rl + 14 : p;
 #pragma unroll
     for (int k = 0; k < 4; ++k) ak.q[4 * pose + k] = src[k];
 #pragma unroll
     for (int k = 0; k < 3; ++k) ak.t[3 * pose + k] = src[4 + k];
   }
-  if (threadIdx.x == 0 && blk == 0) ak.n_steps[pair] = done;
+  if (has_pair && grp == 0 && j == 0 && blk == 0) ak.n_steps[pair] = done;
 }
 
 // ------------------------------------------------------------------------------------------------
@@ -1204,7 +1237,10 @@ void launch_align(const FssPairBatch* pb, const KScore& ks, LdsPlan lp, const Al
                   int block, size_t lds, int variant, hipStream_t s) {
   const int v = effective_variant(MODE, variant);
 #define FSS_K(T, V) hipLaunchKernelGGL((k_align<MODE, ES, T, V>), dim3(grid), dim3(block), lds, s, *pb, ks, lp, ak)
-  if (block <= 512) {
+  if (ak.ppw > 1) {   // several pairs per workgroup: VALU inner loops, at most 512 lanes
+    if (v == 1) hipLaunchKernelGGL((k_align<MODE, ES, 512, 1, true>), dim3(grid), dim3(block), lds, s, *pb, ks, lp, ak);
+    else hipLaunchKernelGGL((k_align<MODE, ES, 512, 2, true>), dim3(grid), dim3(block), lds, s, *pb, ks, lp, ak);
+  } else if (block <= 512) {
     if (v == 1) FSS_K(512, 1);
     else if (v == 3) FSS_K(512, 3);
     else FSS_K(512, 2);
@@ -1486,22 +1522,36 @@ int plan_align(const FssPairBatch* batch, const KScore& ks, const FssAlignParams
     return set_error(FSS_E_INVALID, "early stop couples all %d restarts of a pair: restarts_per_wg (%d) must be >= n_restarts", B, rpw);
   const int groups = ap->groups_per_wg > 0 ? ap->groups_per_wg : rpw;
   if (groups > rpw) return set_error(FSS_E_INVALID, "groups_per_wg (%d) exceeds restarts_per_wg (%d)", groups, rpw);
-  if ((lpp & (lpp - 1)) && (long)groups * lpp > 64)
+  const int ppw = ap->pairs_per_wg > 0 ? ap->pairs_per_wg : 1;
+  if (ppw > 1 && rpw < B)
+    return set_error(FSS_E_INVALID, "pairs_per_wg > 1 puts every restart of each pair in the workgroup: "
+                                    "restarts_per_wg (%d) must be >= n_restarts (%d)", rpw, B);
+  const long lanes = (long)ppw * groups * lpp;
+  if ((lpp & (lpp - 1)) && lanes > 64)
     return set_error(FSS_E_INVALID, "lanes_per_pose must be a power of two unless the workgroup's "
-                                    "groups_per_wg * lanes_per_pose lanes fit one wavefront (64)");
-  pl.block = ((groups * lpp + 63) / 64) * 64;
-  if (pl.block > 1024)
-    return set_error(FSS_E_INVALID, "%d lane groups of %d lanes exceed 1024 threads: lower groups_per_wg", groups, lpp);
+                                    "pairs_per_wg * groups_per_wg * lanes_per_pose lanes fit one wavefront (64)");
+  if (lanes > 1024)
+    return set_error(FSS_E_INVALID, "%d pairs x %d lane groups of %d lanes exceed 1024 threads: lower "
+                                    "pairs_per_wg or groups_per_wg", ppw, groups, lpp);
+  if (ppw > 1 && lanes > 512)
+    return set_error(FSS_E_INVALID, "a workgroup of %d pairs holds at most 512 lanes (%ld asked)", ppw, lanes);
+  pl.block = (int)((lanes + 63) / 64) * 64;
   const bool mfma = effective_variant(ks.mode, ap->variant) == 3;
+  if (mfma && ppw > 1)
+    return set_error(FSS_E_INVALID, "pairs_per_wg > 1 is for the VALU inner loops (variant 1 or 2)");
   if (mfma && lpp != 64)
     return set_error(FSS_E_INVALID, "the MFMA inner loop (variant 3) runs one pose per wavefront: lanes_per_pose must be 64");
   pl.lp = plan_lds(batch, ks.mode, mfma);
-  pl.lds = (size_t)pl.lp.end * sizeof(float4) + (ap->early_stop ? 2 * (size_t)rpw * sizeof(double) : 0) +
-           (size_t)rpw * 24 * sizeof(float);
+  // Several pairs: their LDS regions start 16 B apart modulo 128 B (a region of 8k + 1 float4s),
+  // so the lanes of different pairs reading the same row index hit different banks.
+  const int region = ppw > 1 ? pl.lp.end + ((1 - pl.lp.end) % 8 + 8) % 8 : pl.lp.end;
+  const int sls = (rpw + groups - 1) / groups > 1 ? 24 : 14;   // floats of a restart's slot
+  pl.lds = (size_t)ppw * region * sizeof(float4) +
+           ppw * ((ap->early_stop ? 2 * (size_t)rpw * sizeof(double) : 0) + (size_t)rpw * sls * sizeof(float));
   // Lane groups of a non-power-of-two width sum through LDS (about a sixth of the shuffle tree's
   // instructions per step), unless the scratch would cost occupancy: with it a workgroup must
   // still fit 16 to a CU (4 one-wavefront workgroups per SIMD, the VGPR limit).
-  const size_t red = (size_t)red_floats(groups, lpp) * sizeof(float);
+  const size_t red = (size_t)red_floats(ppw * groups, lpp) * sizeof(float);
   const bool red_lds = (lpp & (lpp - 1)) && pl.lds + red <= kCuLdsBytes / 16;
   if (red_lds) pl.lds += red;
   const size_t cap = max_lds_bytes();
@@ -1524,8 +1574,10 @@ int plan_align(const FssPairBatch* batch, const KScore& ks, const FssAlignParams
   ak.lpp = lpp; ak.rpw = rpw; ak.wg_per_pair = (B + rpw - 1) / rpw;
   ak.groups = groups;
   ak.red_lds = red_lds;
+  ak.ppw = ppw;
+  ak.region = region;
   ak.order = ap->pair_order;
-  pl.grid = (long)batch->n_pairs * ak.wg_per_pair;
+  pl.grid = ppw > 1 ? ((long)batch->n_pairs + ppw - 1) / ppw : (long)batch->n_pairs * ak.wg_per_pair;
   if (pl.grid > 0x7fffffffL) return set_error(FSS_E_INVALID, "grid too large");
   return FSS_OK;
 }

@@ -426,6 +426,7 @@ struct FssScreen {
   MergeJob* d_jobs = nullptr;
   MergeJob* h_jobs = nullptr;          // pinned
   int32_t* h_steps = nullptr;          // pinned: the last launch's per-pair step counts
+  int32_t *d_order = nullptr, *h_order = nullptr;   // launch order by size (pairs_per_wg > 1)
   // scheduler state
   std::vector<Slot> slots;
   std::deque<QueryIn> pending;
@@ -446,12 +447,13 @@ namespace {
 void free_arena(FssScreen* s) {
   void* dev_ptrs[] = {s->q, s->t, s->table, s->best_tf, s->pose_score, s->best_score, s->self_ov,
                       s->best_idx, s->n_steps, s->d_rng, s->qpts, s->qanc, s->qvec, s->qtyp,
-                      s->qsrf, s->qatm, s->qrad, s->tk_s, s->tk_i, s->tk_tf, s->d_jobs};
+                      s->qsrf, s->qatm, s->qrad, s->tk_s, s->tk_i, s->tk_tf, s->d_jobs, s->d_order};
   for (void* p : dev_ptrs)
     if (p) (void)hipFree(p);
   if (s->h_rng) (void)hipHostFree(s->h_rng);
   if (s->h_jobs) (void)hipHostFree(s->h_jobs);
   if (s->h_steps) (void)hipHostFree(s->h_steps);
+  if (s->h_order) (void)hipHostFree(s->h_order);
   if (s->stream) (void)hipStreamDestroy(s->stream);
 }
 
@@ -588,8 +590,19 @@ int launch(FssScreen* s, const std::vector<int>& act) {
     }
   }
   if (p == 0) return complete_finished(s, act, clk::now());
+  if (s->h_order) {   // pairs by their entry's row count, largest first (stable counting sort)
+    const int kc = s->need_shape ? (s->shape_rows ? kFit : kFitSrf) : kFitPh;
+    int32_t mx = 0;
+    for (int i = 0; i < p; ++i) mx = std::max(mx, tab[kc][2 * i + 1]);
+    std::vector<int32_t> start((size_t)mx + 2, 0);
+    for (int i = 0; i < p; ++i) ++start[(size_t)(mx - tab[kc][2 * i + 1]) + 1];
+    for (int c = 0; c <= mx; ++c) start[c + 1] += start[c];
+    for (int i = 0; i < p; ++i) s->h_order[start[(size_t)(mx - tab[kc][2 * i + 1])]++] = i;
+  }
   hipError_t e = hipMemcpyAsync(s->d_rng, s->h_rng, sizeof(int32_t) * 2 * kRngTables * (size_t)s->P,
                                 hipMemcpyHostToDevice, s->stream);
+  if (e == hipSuccess && s->h_order)
+    e = hipMemcpyAsync(s->d_order, s->h_order, sizeof(int32_t) * p, hipMemcpyHostToDevice, s->stream);
   if (e == hipSuccess) e = hipMemcpyAsync(s->d_jobs, s->h_jobs, sizeof(MergeJob) * nj, hipMemcpyHostToDevice, s->stream);
   if (e != hipSuccess) return hip_fail(e, "launch staging");
   const int32_t* drng[kRngTables];
@@ -626,7 +639,8 @@ int launch(FssScreen* s, const std::vector<int>& act) {
     ap.lr = s->prm.lr; ap.beta1 = s->prm.beta1; ap.beta2 = s->prm.beta2; ap.eps = s->prm.eps;
     ap.early_stop = s->prm.early_stop; ap.es_tol = s->prm.es_tol; ap.es_patience = s->prm.es_patience;
     ap.lanes_per_pose = s->prm.lanes_per_pose; ap.restarts_per_wg = s->prm.restarts_per_wg;
-    ap.pair_order = nullptr; ap.skip_finish = 0; ap.variant = s->prm.variant;
+    ap.pair_order = s->d_order; ap.skip_finish = 0; ap.variant = s->prm.variant;
+    ap.pairs_per_wg = s->prm.pairs_per_wg;
     FssAlignOut out{};
     out.q = s->q; out.t = s->t; out.pose_score = s->pose_score; out.best_score = s->best_score;
     out.best_idx = s->best_idx; out.best_tf = s->best_tf; out.n_steps = s->n_steps;
@@ -1060,6 +1074,7 @@ int fss_screen_create(FssLibrary* lib, const FssScoreParams* sp, const FssScreen
   if (need_se && prm->max_query_atoms < 1) return set_error(FSS_E_INVALID, "max_query_atoms must be >= 1");
   if (need_pharm && prm->max_query_pharm < 1) return set_error(FSS_E_INVALID, "max_query_pharm must be >= 1");
   if (!prm->quat_table) return set_error(FSS_E_INVALID, "quat_table is NULL");
+  if (prm->pairs_per_wg < 0 || prm->pairs_per_wg > 64) return set_error(FSS_E_INVALID, "pairs_per_wg must be in [0, 64]");
   int64_t pool_rows[kQPools];
   if (int rc = fss_plan_query_pools(sp, prm, pool_rows)) return rc;
   DeviceGuard dg(lib->device);
@@ -1115,6 +1130,10 @@ int fss_screen_create(FssLibrary* lib, const FssScoreParams* sp, const FssScreen
   if (e == hipSuccess) e = dalloc(&s->d_jobs, s->A);
   if (e == hipSuccess) e = hipHostMalloc((void**)&s->h_jobs, sizeof(MergeJob) * (size_t)s->A, hipHostMallocDefault);
   if (e == hipSuccess) e = hipHostMalloc((void**)&s->h_steps, sizeof(int32_t) * (size_t)s->P, hipHostMallocDefault);
+  if (e == hipSuccess && s->prm.pairs_per_wg > 1) {
+    e = dalloc(&s->d_order, s->P);
+    if (e == hipSuccess) e = hipHostMalloc((void**)&s->h_order, sizeof(int32_t) * (size_t)s->P, hipHostMallocDefault);
+  }
   if (e == hipSuccess) e = hipMemcpy(s->table, prm->quat_table, sizeof(float) * 4 * s->B, hipMemcpyHostToDevice);
   if (e != hipSuccess) {
     free_arena(s);

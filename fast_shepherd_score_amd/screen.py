@@ -375,7 +375,8 @@ class ScreenService:
                  early_stop: bool = True, alpha: Optional[float] = None, lam: Optional[float] = None,
                  similarity: str = "tanimoto", directionless: bool = True,
                  extended_points: bool = False, only_extended: bool = False,
-                 lanes_per_pose: Optional[int] = None, max_query_points: int = 4096,
+                 lanes_per_pose: Optional[int] = None, pairs_per_wg: Optional[int] = None,
+                 max_query_points: int = 4096,
                  max_query_pharm: int = 256, esp_weight: float = 0.5, probe_radius: float = 1.0,
                  max_query_surface: int = 4096, max_query_atoms: int = 1024, device=None):
         from .modes import MODE_SEEDS, MODE_STEPS, canonical
@@ -430,10 +431,18 @@ class ScreenService:
                               f"maximum size ({one} bytes)")
         self.query_budget_bytes = int(min(query_budget_bytes, self.max_active * one))
         self.table = restart_table(self.B)
-        if lanes_per_pose is None:
-            # ShaEP poses carry the surface-atom ESP sums: a wider tile; otherwise every restart of
-            # a pair in one wavefront with as many lanes each as fit (10 restarts: 6 lanes each)
-            lanes_per_pose = 32 if combo else max(1, 64 // self.B)
+        if lanes_per_pose is None and pairs_per_wg is None:
+            # ShaEP poses carry the surface-atom ESP sums: a wider tile.  Atom-sized clouds: 2 lanes
+            # per restart and 32 / B pairs per workgroup, so the per-step pose work is shared by
+            # several pairs' restarts (10 restarts: 3 pairs x 10 x 2 lanes).  Surfaces: every restart
+            # of a pair in one wavefront with as many lanes each as fit (10 restarts: 6 lanes).
+            if not combo and mode not in ("surf", "surf_esp") and 32 // self.B >= 2:
+                lanes_per_pose, pairs_per_wg = 2, 32 // self.B
+            else:
+                lanes_per_pose = 32 if combo else max(1, 64 // self.B)
+        elif lanes_per_pose is None:
+            lanes_per_pose = 32 if combo else max(1, 64 // (self.B * int(pairs_per_wg)))
+        self.lanes_per_pose, self.pairs_per_wg = int(lanes_per_pose), int(pairs_per_wg or 1)
         prm = _lib.FssScreenParams()
         prm.n_restarts, prm.max_steps = self.B, self.opt.max_steps
         prm.lr, prm.beta1, prm.beta2, prm.eps = self.opt.lr, self.opt.beta1, self.opt.beta2, self.opt.eps
@@ -444,7 +453,8 @@ class ScreenService:
         prm.max_query_surface, prm.max_query_atoms = mqs, mqa
         prm.shape_from_surface = int(self.shape_from_surface)
         prm.query_budget_bytes = self.query_budget_bytes
-        prm.lanes_per_pose, prm.restarts_per_wg, prm.variant = int(lanes_per_pose), self.B, 0
+        prm.lanes_per_pose, prm.restarts_per_wg, prm.variant = self.lanes_per_pose, self.B, 0
+        prm.pairs_per_wg = self.pairs_per_wg
         prm.seed_flags = (_lib.SEED_PHARM if self.spec.mode == _lib.MODE_PHARM else
                           _lib.SEED_SURF if combo else 0)
         prm.quat_table = self.table.ctypes.data

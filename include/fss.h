@@ -116,7 +116,7 @@ typedef struct FssAlignParams {
   float   es_tol;
   int32_t es_patience;
   int32_t lanes_per_pose;    /* 1..64: lanes sharing one pose's N x M sum; a power of two     *
-                              * unless restarts_per_wg * lanes_per_pose <= 64 (one wavefront) */
+                              * unless the workgroup's lanes fit one wavefront (64)            */
   int32_t restarts_per_wg;   /* poses per workgroup; >= B when early_stop is set             */
   const int32_t* pair_order; /* device (n_pairs,) launch order of the pairs (costliest first),
                                 or NULL for 0..n_pairs-1; results are always indexed by pair  */
@@ -128,6 +128,10 @@ typedef struct FssAlignParams {
                                 ceil(restarts_per_wg / groups_per_wg) restarts in turn every
                                 step, so per-pair early stop can keep all of a pair's restarts in
                                 one workgroup with fewer threads (the MFMA loop at 50 restarts) */
+  int32_t pairs_per_wg;      /* pairs per workgroup (0 or 1: one); > 1 needs restarts_per_wg >=
+                                n_restarts: the workgroup's lane groups work on consecutive
+                                pairs of the launch order, so small clouds share one wavefront's
+                                per-step pose work (e.g. 6 pairs x 10 restarts x 1 lane) */
 } FssAlignParams;
 
 typedef struct FssAlignOut {
@@ -412,6 +416,9 @@ typedef struct FssScreenParams {
                                   fss_plan_query_pools); 0: max_active queries of the maximum
                                   size.  A queued query is admitted, oldest first, when a slot
                                   and room for its own rows are free. */
+  int32_t pairs_per_wg;        /* fss_align pairs per workgroup (0 or 1: one); > 1 also orders
+                                  each launch's pairs by their entry's row count, so the pairs
+                                  sharing a workgroup have alike sizes */
 } FssScreenParams;
 
 typedef struct FssQuery {      /* host arrays, copied by fss_screen_submit */

@@ -196,6 +196,32 @@ def test_tilings_agree(dev, lpp, rpw):
     np.testing.assert_allclose(r.pose_score.cpu().numpy(), ref.pose_score.cpu().numpy(), atol=1e-4)
 
 
+@pytest.mark.parametrize("mode", [_lib.MODE_SHAPE, _lib.MODE_SHAPE_ESP, _lib.MODE_SHAPE_PHARM])
+@pytest.mark.parametrize("lpp,ppw", [(1, 6), (2, 3), (3, 2), (1, 4), (6, 1)])
+@pytest.mark.parametrize("es", [True, False])
+def test_pairs_per_workgroup(dev, mode, lpp, ppw, es):
+    """Several pairs per workgroup (each with all 10 restarts) compute the same optimisation as
+    one pair per workgroup, with per-pair early stop, uneven clouds in one workgroup and a last
+    workgroup holding fewer pairs (7 pairs); the best restarts match the FP64 oracle."""
+    print(f"case pairs_per_wg mode={mode} lpp={lpp} ppw={ppw} es={es}", flush=True)
+    n, B = 7, 10
+    pb, spec, pairs = make_case(mode, n, dev, seed=41, atoms=(10, 40), pharms=(2, 12))
+    q, t = random_seeds(n, B, seed=42)
+    opt = OptimSpec(max_steps=60, early_stop=es)
+    ref = align(pb, spec, to_dev(q, dev), to_dev(t, dev), opt, lanes_per_pose=16, restarts_per_wg=B)
+    r = align(pb, spec, to_dev(q, dev), to_dev(t, dev), opt, lanes_per_pose=lpp, restarts_per_wg=B,
+              pairs_per_wg=ppw)
+    assert r.extra["pairs_per_wg"] == ppw
+    np.testing.assert_allclose(r.pose_score.cpu().numpy(), ref.pose_score.cpu().numpy(), atol=1e-4)
+    if not es:
+        assert (r.n_steps.cpu() == 60).all()
+    if mode == _lib.MODE_SHAPE:
+        for i in (0, 3, 6):
+            o = orc.optimize(pairs[i], q[i].astype(np.float64), t[i].astype(np.float64), max_steps=60,
+                             early_stop=es)
+            assert abs(float(r.score[i]) - o["score"]) < 1e-3
+
+
 def test_edge_cases(dev):
     from fast_shepherd_score_amd.align import ScoreSpec
     from fast_shepherd_score_amd.packing import CloudSet, PairBatch

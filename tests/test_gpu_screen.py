@@ -109,15 +109,18 @@ def test_memory_budget_defers_admission_and_keeps_hits(dev):
         assert b.pairs == 600
 
 
-def test_native_service_topk_is_exact(dev):
+@pytest.mark.parametrize("lpp,ppw", [(4, 1), (2, 3)])
+def test_native_service_topk_is_exact(dev, lpp, ppw):
     """The native service's hits (uploaded from host arrays through the C ABI, several launches,
     device top-k merges) are exactly the top-k of aligning every pair directly with the same seeds
-    and tiling, ties (duplicated conformers) broken by the lower library index."""
+    and tiling, ties (duplicated conformers) broken by the lower library index.  With several
+    pairs per workgroup the service orders each launch by entry size; a pair's result does not
+    depend on the pairs it shares a workgroup with, so the hits still match bit for bit."""
     import ctypes
     from fast_shepherd_score_amd import _lib
     from fast_shepherd_score_amd.align import OptimSpec, ScoreSpec, align, self_overlaps
     from fast_shepherd_score_amd.packing import CloudSet, PairBatch
-    print("case native screen topk", flush=True)
+    print(f"case native screen topk lpp={lpp} ppw={ppw}", flush=True)
     rng = np.random.default_rng(11)
     base = synth.random_walks(rng, rng.integers(10, 31, size=700))
     mols = base + base[:300]                   # entries 700..999 duplicate 0..299: exact ties
@@ -143,7 +146,8 @@ def test_native_service_topk_is_exact(dev):
     prm.n_restarts, prm.max_steps, prm.lr, prm.beta1, prm.beta2, prm.eps = B, steps, 0.1, 0.9, 0.999, 1e-8
     prm.early_stop, prm.es_tol, prm.es_patience = 1, 1e-5, 10
     prm.top_k, prm.launch_pairs, prm.max_active, prm.max_query_points = K, 333, 4, 64
-    prm.lanes_per_pose, prm.restarts_per_wg, prm.quat_table = 4, B, table.ctypes.data
+    prm.lanes_per_pose, prm.restarts_per_wg, prm.quat_table = lpp, B, table.ctypes.data
+    prm.pairs_per_wg = ppw
     svc = ctypes.c_void_p()
     _lib.call("fss_screen_create", h, _lib.byref(sp), _lib.byref(prm), ctypes.byref(svc))
     qxyz = mols[17]
@@ -174,7 +178,8 @@ def test_native_service_topk_is_exact(dev):
     _lib.call("fss_seed_poses", _lib.byref(bs), 0, B, _lib.ptr(torch.from_numpy(table).to(dev)),
               _lib.ptr(q0), _lib.ptr(t0), int(torch.cuda.current_stream().cuda_stream))
     r = align(pb, ScoreSpec(), q0, t0, OptimSpec(max_steps=steps, lr=0.1, early_stop=True),
-              lanes_per_pose=4, restarts_per_wg=B, order="none", self_ov=self_overlaps(pb, ScoreSpec()))
+              lanes_per_pose=lpp, restarts_per_wg=B, pairs_per_wg=ppw, order="none",
+              self_ov=self_overlaps(pb, ScoreSpec()))
     s = r.score.cpu().numpy()
     assert pairs.value == n and pair_steps.value == int(r.n_steps.sum())     # the executed steps
     order = np.lexsort((np.arange(n), -s))[:K]
