@@ -684,7 +684,9 @@ struct AlignK {
 
 // Every instance is held to 128 VGPRs (4 wavefronts per SIMD).  The MFMA loop is latency-bound,
 // and shape + pharm with early stop would otherwise take 133 VGPRs: with 4 spilled dwords it
-// runs 6 % faster at 4 wavefronts than at 3 (16384 pairs, 38.2 -> 35.8 ms).
+// runs 6 % faster at 4 wavefronts than at 3 (16384 pairs, 38.2 -> 35.8 ms).  The several-pairs
+// instances spill 68-168 B at 128 VGPRs, outside the pair-term loops; at 3 wavefronts they spill
+// nothing and screen 3 % slower (vol, 10 restarts: 4.17M -> 4.05M pairs/s).
 // MP: several pairs per workgroup (AlignK::ppw > 1), a separate instance so the one-pair
 // instances keep their pair's staging offsets and counts uniform (scalar) across the workgroup.
 template <int MODE, bool ES, int MAXT, int VAR, bool MP = false>
