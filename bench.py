@@ -248,12 +248,14 @@ def _poisson_run(svc, queries, rate: float, n: int, rng):
     return res, wall, {k: after[k] - before[k] for k in after}
 
 
-def bench_screen(dev, n_lib: int = 1_000_000, n_queries: int = 40, loads=(0.5, 0.7, 0.95),
-                 surf_queries: int = 2, color_queries: int = 4, seed: int = 6):
+def bench_screen(dev, n_lib: int = 1_000_000, n_queries=(100, 200, 100), loads=(0.5, 0.7, 0.95),
+                 surf_queries: int = 2, color_queries: int = 4, seed: int = 6,
+                 modes=("vol", "surf_esp", "vol_color")):
     """The screening service on the 1M-conformer library (heavy atoms, 200 surface points with
     ESP, 1-16 pharmacophores per conformer, all resident in HBM): vol under Poisson streams at
     fractions of its measured capacity, surf_esp and vol_color on the same library, every mode
-    at 10 restarts x 100 steps with per-pair early stop."""
+    at 10 restarts x 100 steps with per-pair early stop.  ``n_queries``: queries per load point
+    (one count for every point, or one per entry of ``loads``)."""
     import torch
     from fast_shepherd_score_amd import synth
     from fast_shepherd_score_amd.screen import ConformerLibrary, ScreenService
@@ -277,61 +279,65 @@ def bench_screen(dev, n_lib: int = 1_000_000, n_queries: int = 40, loads=(0.5, 0
                        f"the same generator, 10 restarts x 100 steps, per-pair early stop, top-100, "
                        f"admission by query bytes",
            "library_bytes": lib.nbytes()}
-    with ScreenService(lib, "vol", launch_pairs=262144, **kw) as svc:
-        svc.submit(qv[-1]).result()                                  # warm-up
-        t1 = time.perf_counter()
-        sat = [f.result() for f in [svc.submit(q) for q in qv[:6]]]
-        busy = time.perf_counter() - t1
-        cap_qps = len(sat) / busy
-        warm = svc.stats()                       # the warm-up query and the saturated burst
-        legs = []
-        for load in loads:
-            res, wall, st = _poisson_run(svc, qv, load * cap_qps, n_queries, rng)
-            lat = np.array([r.latency_ms for r in res])
-            legs.append({"load": load, "queries": n_queries, "pairs_per_s": n_queries * n_lib / wall,
-                         "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
-                         "mean_queue_ms": float(np.mean([r.queue_ms for r in res])),
-                         "mean_pair_steps": st["pair_steps"] / max(st["pairs"], 1),
-                         "hbm_library_gb_per_s": n_queries * atom_bytes / wall / 1e9})
-            log(f"screen vol load {load:.0%}: p50 {legs[-1]['p50_ms']:.0f} ms, p99 {legs[-1]['p99_ms']:.0f} ms")
-        adm = svc.admission()
-        total = svc.stats()
-    per_query = warm["shape_terms"] / (len(sat) + 1)
-    out["vol"] = {"capacity_pairs_per_s": cap_qps * n_lib, "capacity_queries_per_s": cap_qps,
-                  "mean_pair_steps": warm["pair_steps"] / max(warm["pairs"], 1),
-                  "shape_terms_per_query": per_query,
-                  # 16 FLOP per executed shape term, over the saturated burst's wall time (a lower
-                  # bound on the kernel's rate: staging, seeding and merges are inside it)
-                  "achieved_tflops_wall": per_query * len(sat) * 16 / busy / 1e12,
-                  "poisson": legs, "max_admitted": adm["max_admitted"],
-                  "tiling": {"lanes_per_pose": svc.lanes_per_pose, "pairs_per_wg": svc.pairs_per_wg},
-                  # the whole service's work, for a kernel-time roofline from a trace of this run
-                  "total": {k: total[k] for k in ("launches", "pairs", "pair_steps", "shape_terms")}}
-    with ScreenService(lib, "surf_esp", launch_pairs=65536, **kw) as svc:
-        t1 = time.perf_counter()
-        svc.submit(qs[-1]).result()
-        solo = time.perf_counter() - t1
-        t1 = time.perf_counter()
-        res = [f.result() for f in [svc.submit(q) for q in qs[:surf_queries]]]
-        busy = time.perf_counter() - t1
-        st = svc.stats()
-    out["surf_esp"] = {"solo_query_ms": solo * 1e3, "queries": surf_queries,
-                       "pairs_per_s": surf_queries * n_lib / busy,
-                       "p50_ms": float(np.percentile([r.latency_ms for r in res], 50)),
-                       "mean_pair_steps": st["pair_steps"] / max(st["pairs"], 1),
-                       "achieved_tflops_wall": st["shape_terms"] * surf_queries / (surf_queries + 1) * 19 / busy / 1e12,
-                       "total": {k: st[k] for k in ("launches", "pairs", "pair_steps", "shape_terms")}}
-    log("screen surf_esp done")
-    with ScreenService(lib, "vol_color", launch_pairs=262144, **kw) as svc:
-        svc.submit(qc[-1]).result()
-        t1 = time.perf_counter()
-        res = [f.result() for f in [svc.submit(q) for q in qc[:color_queries]]]
-        busy = time.perf_counter() - t1
-        st = svc.stats()
-    out["vol_color"] = {"queries": color_queries, "pairs_per_s": color_queries * n_lib / busy,
-                        "p50_ms": float(np.percentile([r.latency_ms for r in res], 50)),
-                        "total": {k: st[k] for k in ("launches", "pairs", "pair_steps", "shape_terms",
-                                                     "pharm_terms")}}
+    if "vol" in modes:
+        with ScreenService(lib, "vol", launch_pairs=262144, **kw) as svc:
+            svc.submit(qv[-1]).result()                                  # warm-up
+            t1 = time.perf_counter()
+            sat = [f.result() for f in [svc.submit(q) for q in qv[:6]]]
+            busy = time.perf_counter() - t1
+            cap_qps = len(sat) / busy
+            warm = svc.stats()                       # the warm-up query and the saturated burst
+            legs = []
+            counts = [n_queries] * len(loads) if np.isscalar(n_queries) else list(n_queries)
+            for load, nq in zip(loads, counts):
+                res, wall, st = _poisson_run(svc, qv, load * cap_qps, nq, rng)
+                lat = np.array([r.latency_ms for r in res])
+                legs.append({"load": load, "queries": nq, "pairs_per_s": nq * n_lib / wall,
+                             "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
+                             "mean_queue_ms": float(np.mean([r.queue_ms for r in res])),
+                             "mean_pair_steps": st["pair_steps"] / max(st["pairs"], 1),
+                             "hbm_library_gb_per_s": nq * atom_bytes / wall / 1e9})
+                log(f"screen vol load {load:.0%}: p50 {legs[-1]['p50_ms']:.0f} ms, p99 {legs[-1]['p99_ms']:.0f} ms")
+            adm = svc.admission()
+            total = svc.stats()
+        per_query = warm["shape_terms"] / (len(sat) + 1)
+        out["vol"] = {"capacity_pairs_per_s": cap_qps * n_lib, "capacity_queries_per_s": cap_qps,
+                      "mean_pair_steps": warm["pair_steps"] / max(warm["pairs"], 1),
+                      "shape_terms_per_query": per_query,
+                      # 16 FLOP per executed shape term, over the saturated burst's wall time (a lower
+                      # bound on the kernel's rate: staging, seeding and merges are inside it)
+                      "achieved_tflops_wall": per_query * len(sat) * 16 / busy / 1e12,
+                      "poisson": legs, "max_admitted": adm["max_admitted"],
+                      "tiling": {"lanes_per_pose": svc.lanes_per_pose, "pairs_per_wg": svc.pairs_per_wg},
+                      # the whole service's work, for a kernel-time roofline from a trace of this run
+                      "total": {k: total[k] for k in ("launches", "pairs", "pair_steps", "shape_terms")}}
+    if "surf_esp" in modes:
+        with ScreenService(lib, "surf_esp", launch_pairs=65536, **kw) as svc:
+            t1 = time.perf_counter()
+            svc.submit(qs[-1]).result()
+            solo = time.perf_counter() - t1
+            t1 = time.perf_counter()
+            res = [f.result() for f in [svc.submit(q) for q in qs[:surf_queries]]]
+            busy = time.perf_counter() - t1
+            st = svc.stats()
+        out["surf_esp"] = {"solo_query_ms": solo * 1e3, "queries": surf_queries,
+                           "pairs_per_s": surf_queries * n_lib / busy,
+                           "p50_ms": float(np.percentile([r.latency_ms for r in res], 50)),
+                           "mean_pair_steps": st["pair_steps"] / max(st["pairs"], 1),
+                           "achieved_tflops_wall": st["shape_terms"] * surf_queries / (surf_queries + 1) * 19 / busy / 1e12,
+                           "total": {k: st[k] for k in ("launches", "pairs", "pair_steps", "shape_terms")}}
+        log("screen surf_esp done")
+    if "vol_color" in modes:
+        with ScreenService(lib, "vol_color", launch_pairs=262144, **kw) as svc:
+            svc.submit(qc[-1]).result()
+            t1 = time.perf_counter()
+            res = [f.result() for f in [svc.submit(q) for q in qc[:color_queries]]]
+            busy = time.perf_counter() - t1
+            st = svc.stats()
+        out["vol_color"] = {"queries": color_queries, "pairs_per_s": color_queries * n_lib / busy,
+                            "p50_ms": float(np.percentile([r.latency_ms for r in res], 50)),
+                            "total": {k: st[k] for k in ("launches", "pairs", "pair_steps", "shape_terms",
+                                                         "pharm_terms")}}
     lib.close()
     return out
 
@@ -489,6 +495,13 @@ LEGS = {"c1": bench_c1, "c3": bench_c3, "screen": bench_screen, "store": lambda 
         "shaep": bench_shaep, "score_loss": bench_score_loss, "screen_shaep": bench_screen_shaep}
 
 
+def screen_kw(args) -> dict:
+    kw = {"n_lib": args.screen_lib, "modes": tuple(args.screen_modes.split(","))}
+    if args.screen_queries:
+        kw["n_queries"] = args.screen_queries
+    return kw
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -506,7 +519,10 @@ def main():
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--no-extra", action="store_true", help="skip the C1 / C3 / screening legs")
     ap.add_argument("--screen-lib", type=int, default=1_000_000)
-    ap.add_argument("--screen-queries", type=int, default=40, help="queries per Poisson load point")
+    ap.add_argument("--screen-queries", type=int, default=0,
+                    help="queries per Poisson load point (default: 100 at 50 %% and 95 %%, 200 at 70 %%)")
+    ap.add_argument("--screen-modes", default="vol,surf_esp,vol_color",
+                    help="comma-separated screening modes of the screen leg")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="first run a plain 256 MiB device copy (HBM counter calibration for rocprofv3 --pmc)")
     ap.add_argument("--only", choices=sorted(LEGS),
@@ -532,7 +548,7 @@ def main():
         torch.cuda.synchronize()
         log("copy calibration: 268435456 bytes read + 268435456 bytes written")
     if args.only:
-        kw = {"n_lib": args.screen_lib, "n_queries": args.screen_queries} if args.only == "screen" else {}
+        kw = screen_kw(args) if args.only == "screen" else {}
         print(json.dumps({"leg": args.only, **LEGS[args.only](dev, **kw)}), flush=True)
         return
     B, steps = args.restarts, args.opt_steps
@@ -622,10 +638,11 @@ def main():
         log("c1 done")
         out["c3_pharm_shape"] = bench_c3(dev)
         log("c3 done")
-        sc = bench_screen(dev, n_lib=args.screen_lib, n_queries=args.screen_queries)
+        sc = bench_screen(dev, **screen_kw(args))
         out["screen"] = sc
-        at70 = min(sc["vol"]["poisson"], key=lambda r: abs(r["load"] - 0.7))
-        out["screen_p50_ms"], out["screen_p99_ms"] = at70["p50_ms"], at70["p99_ms"]
+        if "vol" in sc:
+            at70 = min(sc["vol"]["poisson"], key=lambda r: abs(r["load"] - 0.7))
+            out["screen_p50_ms"], out["screen_p99_ms"] = at70["p50_ms"], at70["p99_ms"]
         log("screen done")
         out["store_load"] = bench_store()
         log("store done")
