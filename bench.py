@@ -617,7 +617,7 @@ def main():
                    "lanes_per_pose": res.extra["lanes_per_pose"],
                    "restarts_per_wg": res.extra["restarts_per_wg"],
                    "inner_loop": {0: "expanded", 1: "direct", 2: "expanded",
-                                  3: "mfma-split-bf16"}[res.extra["variant"]]},
+                                  3: "mfma-split-bf16", 4: "expanded-streamed"}[res.extra["variant"]]},
         "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_FP32_TFLOPS,
                      "unit": "TFLOP/s", "frac": achieved / PEAK_FP32_TFLOPS,
                      "traffic": prof.get("bytes"),

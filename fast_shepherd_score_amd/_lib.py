@@ -98,6 +98,17 @@ class FssAlignParams(ctypes.Structure):
         ("variant", ctypes.c_int32),
         ("groups_per_wg", ctypes.c_int32),
         ("pairs_per_wg", ctypes.c_int32),
+        ("tile_rows", ctypes.c_int32),
+    ]
+
+
+class FssAlignPlanInfo(ctypes.Structure):
+    _fields_ = [
+        ("variant", ctypes.c_int32),
+        ("tile_ref", ctypes.c_int32), ("tile_fit", ctypes.c_int32),
+        ("block", ctypes.c_int32),
+        ("grid", ctypes.c_int64),
+        ("lds_bytes", ctypes.c_int64),
     ]
 
 
@@ -186,6 +197,7 @@ SIGNATURES = {
                                       _P, _P]),
     "fss_align": (ctypes.c_int, [_P, _P, _P, _P, _P, _P]),
     "fss_align_noalloc": (ctypes.c_int, [_P, _P, _P, _P, _P]),
+    "fss_align_plan": (ctypes.c_int, [_P, _P, _P, _P]),
     "fss_align_finish": (ctypes.c_int, [_P, _P, ctypes.c_int32, _P, _P, _P]),
     "fss_seed_poses": (ctypes.c_int, [_P, ctypes.c_int32, ctypes.c_int32, _P, _P, _P, _P]),
     "fss_score_fwd": (ctypes.c_int, [_P, _P, _P, _P, _P]),

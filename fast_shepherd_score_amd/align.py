@@ -235,7 +235,7 @@ def align(pb: PairBatch, spec: ScoreSpec, q0: torch.Tensor, t0: torch.Tensor,
           restarts_per_wg: Optional[int] = None, order: str = "cost",
           self_ov: Optional[torch.Tensor] = None, stream=None, events: Optional[list] = None,
           tune: bool = False, variant: int = 0, groups_per_wg: Optional[int] = None,
-          pairs_per_wg: Optional[int] = None) -> AlignResult:
+          pairs_per_wg: Optional[int] = None, tile_rows: int = 0) -> AlignResult:
     """Persistent path: every step of every pair in one launch.
 
     Tiling: explicit ``lanes_per_pose`` / ``restarts_per_wg``, else an autotuned choice
@@ -246,6 +246,10 @@ def align(pb: PairBatch, spec: ScoreSpec, q0: torch.Tensor, t0: torch.Tensor,
     restarts in turn every step (None: the tuned choice, else one restart per group).
     ``pairs_per_wg``: pairs per workgroup (each with all its restarts): consecutive pairs of the
     launch order share a workgroup, so small clouds share one wavefront's per-step pose work.
+    Clouds that do not fit LDS in the requested layout run the expanded form, streamed through
+    LDS tiles in the shape modes (variant 4; ``tile_rows`` rows of each side per tile, 0 for
+    1024); ``variant=4`` asks for the streamed form at any size.  ``extra`` reports the launch
+    (``variant``, ``tile_ref``, ``tile_fit``).
     ``events``: if a list is given, HIP events bracketing the optimiser kernel and the finishing
     kernels are appended to it as (start, optimiser_end, finish_end).
     """
@@ -298,11 +302,14 @@ def align(pb: PairBatch, spec: ScoreSpec, q0: torch.Tensor, t0: torch.Tensor,
     ap.variant = int(variant)
     ap.groups_per_wg = grp
     ap.pairs_per_wg = ppw
+    ap.tile_rows = int(tile_rows)
+    bs, sp = pb.struct(), spec.struct()
+    plan = _lib.FssAlignPlanInfo()
+    _lib.call("fss_align_plan", _lib.byref(bs), _lib.byref(sp), _lib.byref(ap), _lib.byref(plan))
     out = _lib.FssAlignOut()
     out.q, out.t = _lib.ptr(q), _lib.ptr(t)
     out.pose_score, out.best_score, out.best_idx = _lib.ptr(pose_score), _lib.ptr(best), _lib.ptr(bidx)
     out.best_tf, out.n_steps = _lib.ptr(tf), _lib.ptr(nst)
-    bs, sp = pb.struct(), spec.struct()
     if events is None:
         _lib.call("fss_align", _lib.byref(bs), _lib.byref(sp), _lib.byref(ap), _lib.ptr(self_ov),
                   _lib.byref(out), _stream(stream))
@@ -319,8 +326,9 @@ def align(pb: PairBatch, spec: ScoreSpec, q0: torch.Tensor, t0: torch.Tensor,
         ev[2].record(ts)
         events.append(tuple(ev))
     return AlignResult(best, bidx, tf, q, t, pose_score, nst,
-                       extra={"lanes_per_pose": lpp, "restarts_per_wg": rpw, "variant": int(variant),
-                              "groups_per_wg": grp or rpw, "pairs_per_wg": ppw, "order": order_t})
+                       extra={"lanes_per_pose": lpp, "restarts_per_wg": rpw, "variant": int(plan.variant),
+                              "groups_per_wg": grp or rpw, "pairs_per_wg": ppw, "order": order_t,
+                              "tile_ref": int(plan.tile_ref), "tile_fit": int(plan.tile_fit)})
 
 
 class SteppedAligner:

@@ -19,6 +19,7 @@
 #include <stdio.h>
 #include <string.h>
 
+#include <algorithm>
 #include <cmath>
 #include <mutex>
 #include <string>
@@ -167,6 +168,7 @@ struct LdsPlan {
   int ref, xq, fit, refc, red, poff, ranc, rvec, fanc, fvec;
   int srf_r, srf_f, atm_r, atm_f, rb_r, rb_f;   // ShaEP channel
   int end;
+  int tile_ref, tile_fit;   // > 0: streamed shape clouds, this many rows of each side per tile
 };
 LdsPlan plan_lds(const FssPairBatch* pb, int mode, bool mfma = false) {
   LdsPlan p;
@@ -191,15 +193,38 @@ LdsPlan plan_lds(const FssPairBatch* pb, int mode, bool mfma = false) {
   p.rb_r = o; o += se ? (pb->max_ref_atm + 3) / 4 : 0;
   p.rb_f = o; o += se ? (pb->max_fit_atm + 3) / 4 : 0;
   p.end = o;
+  p.tile_ref = p.tile_fit = 0;
+  return p;
+}
+
+// Shape modes whose clouds may be streamed through LDS tiles (clouds of any size).
+constexpr bool tileable(int mode) { return mode == FSS_MODE_SHAPE || mode == FSS_MODE_SHAPE_ESP; }
+
+// LDS plan of the streamed shape path: one tile of tr ref rows (expanded form, plus the ESP
+// column) and one of tf fit rows, then the ref centroid slot.
+LdsPlan plan_lds_tiled(int mode, int tr, int tf) {
+  LdsPlan p{};
+  int o = 0;
+  p.ref = o; o += tr;
+  p.xq = o;
+  p.fit = o; o += tf;
+  p.refc = o; o += mode == FSS_MODE_SHAPE_ESP ? (tr + 3) / 4 : 0;
+  p.red = o; o += 1;
+  p.poff = p.ranc = p.rvec = p.fanc = p.fvec = o;
+  p.srf_r = p.srf_f = p.atm_r = p.atm_f = p.rb_r = p.rb_f = o;
+  p.end = o;
+  p.tile_ref = tr;
+  p.tile_fit = tf;
   return p;
 }
 
 constexpr size_t kCuLdsBytes = 160 * 1024;   // LDS of one gfx950 compute unit
 
+// LDS per workgroup on the current device; without one (host-only planning) gfx950's 160 KiB.
 size_t max_lds_bytes() {
   int dev = 0, v = 0;
-  if (hipGetDevice(&dev) != hipSuccess) return 65536;
-  if (hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess) return 65536;
+  if (hipGetDevice(&dev) != hipSuccess) return kCuLdsBytes;
+  if (hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess) return kCuLdsBytes;
   return (size_t)v;
 }
 
@@ -214,7 +239,103 @@ struct Staged {
   const float* rb_r; const float* rb_f;   // (radius + probe)^2
   int Ns, Ms, Na, Ma;
   float co[3];
+  const float4* gref; const float4* gfit;   // streamed shape path: the pair's rows in HBM
 };
+
+// Ref centroid of rows ga[0..N): lane-strided sums in wavefront 0 and one butterfly, whatever the
+// workgroup size, so that every tiling and every kernel staging this pair gets the same bits.
+// Every thread of the workgroup calls it (one barrier inside); red: 3 floats of LDS.
+__device__ __forceinline__ void ref_centroid(const float4* __restrict__ ga, int N, float* red, float c[3]) {
+  const int tid = threadIdx.x;
+  float cx = 0.f, cy = 0.f, cz = 0.f;
+  if (tid < 64) {
+    for (int i = tid; i < N; i += 64) {
+      const float4 v = ga[i];
+      cx += v.x; cy += v.y; cz += v.z;
+    }
+    cx = group_sum(cx, 64); cy = group_sum(cy, 64); cz = group_sum(cz, 64);
+    if (tid == 0) { red[0] = cx; red[1] = cy; red[2] = cz; }
+  }
+  __syncthreads();
+  const float invN = N > 0 ? 1.f / (float)N : 0.f;
+  c[0] = red[0] * invN; c[1] = red[1] * invN; c[2] = red[2] * invN;
+}
+
+// Expanded-form LDS rows of ref rows ga[i0 .. i0+n) (see shape_accum): la[i] = (2 a~, W_a - D),
+// lc[i] = 2 c~_a (ESP only), a~ = s (a - c).
+template <bool ESP>
+__device__ __forceinline__ void stage_ref_rows(const float4* __restrict__ ga, int i0, int n, const float c[3],
+                                               float s, float sc, float4* la, float* lc) {
+  for (int i = threadIdx.x; i < n; i += blockDim.x) {
+    const float4 v = ga[i0 + i];
+    const float ax = s * (v.x - c[0]), ay = s * (v.y - c[1]), az = s * (v.z - c[2]);
+    const float cc = ESP ? v.w * sc : 0.f;
+    la[i] = make_float4(2.f * ax, 2.f * ay, 2.f * az, -fmaf(ax, ax, fmaf(ay, ay, fmaf(az, az, cc * cc))) - kExpShift);
+    if (ESP) lc[i] = 2.f * cc;
+  }
+}
+
+// Scaled fit rows gb[i0 .. i0+n): lb[i] = (s b, c~_b).
+template <bool ESP>
+__device__ __forceinline__ void stage_fit_rows(const float4* __restrict__ gb, int i0, int n, float s, float sc,
+                                               float4* lb) {
+  for (int i = threadIdx.x; i < n; i += blockDim.x) {
+    const float4 v = gb[i0 + i];
+    lb[i] = make_float4(v.x * s, v.y * s, v.z * s, ESP ? v.w * sc : 0.f);
+  }
+}
+
+// Streamed shape path: the pair's centroid and row pointers; the tiles are staged per pass.
+template <bool ESP>
+__device__ __forceinline__ Staged stage_tiled(const FssPairBatch& pb, const KScore& ks, LdsPlan lp, int pair,
+                                              float4* lds) {
+  Staged st{};
+  const int a0 = pb.ref_rng[2 * pair], N = pb.ref_rng[2 * pair + 1];
+  const int b0 = pb.fit_rng[2 * pair], M = pb.fit_rng[2 * pair + 1];
+  st.gref = reinterpret_cast<const float4*>(pb.ref_pts) + a0;
+  st.gfit = reinterpret_cast<const float4*>(pb.fit_pts) + b0;
+  float c[3];
+  ref_centroid(st.gref, N, reinterpret_cast<float*>(lds + lp.red), c);
+  st.ref = lds + lp.ref;
+  st.refc = reinterpret_cast<const float*>(lds + lp.refc);
+  st.fit = lds + lp.fit;
+  st.N = N; st.M = M;
+  st.cs[0] = ks.s * c[0]; st.cs[1] = ks.s * c[1]; st.cs[2] = ks.s * c[2];
+  st.co[0] = c[0]; st.co[1] = c[1]; st.co[2] = c[2];
+  return st;
+}
+
+// Tile residency of the streamed shape path (uniform across the workgroup).
+struct TileState {
+  int r0 = -1, f0 = -1;
+};
+
+// One pose's shape sums over every (ref tile, fit tile) of the streamed path.  Every thread of the
+// workgroup calls it: a tile is staged between two barriers when it is not already resident;
+// `work` lanes accumulate (lane j of a group of lpp lanes takes the tile's fit rows j, j+lpp, ...).
+template <bool ESP, bool GRAD>
+__device__ __forceinline__ void shape_accum_tiled(const Staged& st, const KScore& ks, LdsPlan lp, TileState& ts_,
+                                                  bool work, int j, int lpp, const float R[9], const float ts[3],
+                                                  Acc& acc) {
+  float4* la = const_cast<float4*>(st.ref);
+  float* lc = const_cast<float*>(st.refc);
+  float4* lb = const_cast<float4*>(st.fit);
+  for (int f0 = 0; f0 < st.M; f0 += lp.tile_fit) {
+    const int nf = min(lp.tile_fit, st.M - f0);
+    for (int r0 = 0; r0 < st.N; r0 += lp.tile_ref) {
+      const int nr = min(lp.tile_ref, st.N - r0);
+      if (ts_.r0 != r0 || ts_.f0 != f0) {
+        __syncthreads();   // every lane is done with the resident tiles
+        if (ts_.r0 != r0) stage_ref_rows<ESP>(st.gref, r0, nr, st.co, ks.s, ks.sc, la, lc);
+        if (ts_.f0 != f0) stage_fit_rows<ESP>(st.gfit, f0, nf, ks.s, ks.sc, lb);
+        ts_.r0 = r0;
+        ts_.f0 = f0;
+        __syncthreads();
+      }
+      if (work) shape_accum<ESP, GRAD>(la, lc, nr, lb, nf, j, lpp, R, ts, acc);
+    }
+  }
+}
 
 // Cooperative copy of one pair's clouds into LDS (shape rows scaled, pharm vectors normalised,
 // pharm type folded into the anchor's w as integer bits).  Caller syncs afterwards.
@@ -240,21 +361,9 @@ __device__ __forceinline__ Staged stage_pair(const FssPairBatch& pb, const KScor
     float* lc = reinterpret_cast<float*>(lds + lp.refc);
     float* red = reinterpret_cast<float*>(lds + lp.red);
     const float s = ks.s, sc = ks.sc;
-    // ref centroid: lane-strided sums in wavefront 0 and one butterfly, whatever the workgroup
-    // size, so that every tiling and every kernel staging this pair gets the same bits
-    float cx = 0.f, cy = 0.f, cz = 0.f;
-    if (tid < 64) {
-      for (int i = tid; i < N; i += 64) {
-        const float4 v = ga[i];
-        cx += v.x; cy += v.y; cz += v.z;
-      }
-      cx = group_sum(cx, 64); cy = group_sum(cy, 64); cz = group_sum(cz, 64);
-      if (tid == 0) { red[0] = cx; red[1] = cy; red[2] = cz; }
-    }
-    __syncthreads();
-    cx = red[0]; cy = red[1]; cz = red[2];
-    const float invN = N > 0 ? 1.f / (float)N : 0.f;
-    cx *= invN; cy *= invN; cz *= invN;
+    float cen[3];
+    ref_centroid(ga, N, red, cen);
+    const float cx = cen[0], cy = cen[1], cz = cen[2];
     if (MF) {
       const int T = (N + 15) >> 4;
       float* xqf = reinterpret_cast<float*>(lds + lp.xq);
@@ -278,22 +387,16 @@ __device__ __forceinline__ Staged stage_pair(const FssPairBatch& pb, const KScor
       st.xq = lds + lp.xq;
       st.T = T;
     }
-    for (int i = tid; i < N && !MF; i += nt) {
-      const float4 v = ga[i];
-      const float ax = s * (v.x - cx), ay = s * (v.y - cy), az = s * (v.z - cz);
-      const float cc = ESP ? v.w * sc : 0.f;
-      if (EXPD) {
-        la[i] = make_float4(2.f * ax, 2.f * ay, 2.f * az,
-                            -fmaf(ax, ax, fmaf(ay, ay, fmaf(az, az, cc * cc))) - kExpShift);
-        if (ESP) lc[i] = 2.f * cc;
-      } else {
-        la[i] = make_float4(ax, ay, az, cc);
+    if (EXPD && !MF) {
+      stage_ref_rows<ESP>(ga, 0, N, cen, s, sc, la, lc);
+    } else {
+      for (int i = tid; i < N && !MF; i += nt) {
+        const float4 v = ga[i];
+        const float ax = s * (v.x - cx), ay = s * (v.y - cy), az = s * (v.z - cz);
+        la[i] = make_float4(ax, ay, az, ESP ? v.w * sc : 0.f);
       }
     }
-    for (int i = tid; i < M; i += nt) {
-      const float4 v = gb[i];
-      lb[i] = make_float4(v.x * s, v.y * s, v.z * s, ESP ? v.w * sc : 0.f);
-    }
+    stage_fit_rows<ESP>(gb, 0, M, s, sc, lb);
     st.ref = la; st.refc = lc; st.fit = lb; st.N = N; st.M = M;
     st.cs[0] = s * cx; st.cs[1] = s * cy; st.cs[2] = s * cz;
     st.co[0] = cx; st.co[1] = cy; st.co[2] = cz;
@@ -527,7 +630,12 @@ __global__ __launch_bounds__(kPoseBlock) void k_pose_grad(FssPairBatch pb, KScor
   __shared__ float red[2][(kPoseBlock / 64) * 13];
   const int pose = blockIdx.x;
   const int pair = pose_pair[pose];
-  const Staged st = stage_pair<SH, ESP, PH, true, false, SE>(pb, ks, lp, pair, 0, 1, lds);
+  // streamed shape clouds when the plan says so (tileable modes whose clouds exceed LDS)
+  bool tl = false;
+  if constexpr (tileable(MODE)) tl = lp.tile_ref > 0;
+  TileState tiles;
+  const Staged st = tl ? stage_tiled<ESP>(pb, ks, lp, pair, lds)
+                       : stage_pair<SH, ESP, PH, true, false, SE>(pb, ks, lp, pair, 0, 1, lds);
   float q[4] = {qin[4 * pose], qin[4 * pose + 1], qin[4 * pose + 2], qin[4 * pose + 3]};
   const float t[3] = {tin[3 * pose], tin[3 * pose + 1], tin[3 * pose + 2]};
   double qh[4], qn, iq, Rd[9];
@@ -541,7 +649,8 @@ __global__ __launch_bounds__(kPoseBlock) void k_pose_grad(FssPairBatch pb, KScor
   ph.zero();
   if (SH) {
     const float ts[3] = {fmaf(t[0], ks.s, -st.cs[0]), fmaf(t[1], ks.s, -st.cs[1]), fmaf(t[2], ks.s, -st.cs[2])};
-    shape_accum<ESP, GRAD>(st.ref, st.refc, st.N, st.fit, st.M, threadIdx.x, blockDim.x, R, ts, sh);
+    if (tl) shape_accum_tiled<ESP, GRAD>(st, ks, lp, tiles, true, threadIdx.x, blockDim.x, R, ts, sh);
+    else shape_accum<ESP, GRAD>(st.ref, st.refc, st.N, st.fit, st.M, threadIdx.x, blockDim.x, R, ts, sh);
     block_sum(sh, red[0], GRAD);
   }
   if (PH) {
@@ -694,10 +803,12 @@ __global__ __launch_bounds__(MAXT) __attribute__((amdgpu_waves_per_eu(4)))
 void k_align(FssPairBatch pb, KScore ks, LdsPlan lp, AlignK ak) {
   constexpr bool EXPD = VAR != 1;
   constexpr bool MF = VAR == 3;
+  constexpr bool TL = VAR == 4;   // streamed shape clouds (tileable modes, one pair per workgroup)
   constexpr bool SE = MODE == FSS_MODE_ESP_COMBO;
   constexpr bool SH = MODE != FSS_MODE_PHARM;
   constexpr bool ESP = MODE == FSS_MODE_SHAPE_ESP;
   constexpr bool PH = MODE == FSS_MODE_PHARM || MODE == FSS_MODE_SHAPE_PHARM;
+  static_assert(!TL || (tileable(MODE) && !MP), "the streamed path runs the shape modes, one pair per workgroup");
   extern __shared__ float4 lds[];
   const int lpp = ak.lpp, groups = ak.groups, ppw = MP ? ak.ppw : 1;
   // ppw pairs per workgroup (each with every restart in the workgroup): lane group g works on
@@ -714,7 +825,10 @@ void k_align(FssPairBatch pb, KScore ks, LdsPlan lp, AlignK ak) {
   const int pair = !MP ? (ak.order ? ak.order[slot0] : slot0)
                        : has_pair ? (ak.order ? ak.order[slot0 + pl] : slot0 + pl) : 0;
   Staged st{};
-  if (ppw == 1) {
+  TileState tiles;
+  if (TL) {
+    st = stage_tiled<ESP>(pb, ks, lp, pair, lds);
+  } else if (ppw == 1) {
     st = stage_pair<SH, ESP, PH, EXPD, MF, SE>(pb, ks, lp, pair, 0, 1, lds);
   } else {
     for (int k = 0; k < ppw && slot0 + k < pb.n_pairs; ++k) {   // the same trip count everywhere
@@ -789,9 +903,11 @@ void k_align(FssPairBatch pb, KScore ks, LdsPlan lp, AlignK ak) {
     double* buf = lossbuf + (step & 1) * ak.rpw;
     for (int i = 0; i < rpg; ++i) {
       const int rl = local(i);
-      if (!active || !runs(rl)) continue;
-      float* sl = slots + sls * rl;
-      if (rpg > 1) {
+      // the streamed path's lanes without a restart still stage tiles and meet their barriers
+      const bool work = active && runs(rl);
+      if (!TL && !work) continue;
+      float* sl = slots + sls * (work ? rl : 0);
+      if (rpg > 1 && work) {
 #pragma unroll
         for (int k = 0; k < 7; ++k) p[k] = sl[14 + k];
       }
@@ -805,9 +921,11 @@ void k_align(FssPairBatch pb, KScore ks, LdsPlan lp, AlignK ak) {
       ph.zero();
       if (SH) {
         const float ts[3] = {fmaf(p[4], ks.s, -st.cs[0]), fmaf(p[5], ks.s, -st.cs[1]), fmaf(p[6], ks.s, -st.cs[2])};
-        if (MF) shape_accum_mfma<ESP>(st.ref, st.xq, st.T, st.fit, st.M, R, ts, sh);
+        if (TL) shape_accum_tiled<ESP, true>(st, ks, lp, tiles, work, j, lpp, R, ts, sh);
+        else if (MF) shape_accum_mfma<ESP>(st.ref, st.xq, st.T, st.fit, st.M, R, ts, sh);
         else if (EXPD) shape_accum<ESP, true>(st.ref, st.refc, st.N, st.fit, st.M, j, lpp, R, ts, sh);
         else shape_accum_direct<ESP, true>(st.ref, st.N, st.fit, st.M, j, lpp, R, ts, sh);
+        if (TL && !work) continue;
         pose_sum(sh, lpp, j, true, grp_all, nl, red);
         shape_finish(sh, ks.K, ks.alpha, ks.s, true);
       }
@@ -1232,6 +1350,7 @@ void launch_pose(const FssPairBatch* pb, const KScore& ks, LdsPlan lp, int n, co
 int effective_variant(int mode, int variant) {
   if (mode == FSS_MODE_PHARM) return 2;
   if (mode == FSS_MODE_ESP_COMBO) return variant == 1 ? 1 : 2;
+  if (variant == 4) return tileable(mode) ? 4 : 2;
   return variant == 1 || variant == 3 ? variant : 2;
 }
 template <int MODE, bool ES>
@@ -1239,6 +1358,13 @@ void launch_align(const FssPairBatch* pb, const KScore& ks, LdsPlan lp, const Al
                   int block, size_t lds, int variant, hipStream_t s) {
   const int v = effective_variant(MODE, variant);
 #define FSS_K(T, V) hipLaunchKernelGGL((k_align<MODE, ES, T, V>), dim3(grid), dim3(block), lds, s, *pb, ks, lp, ak)
+  if constexpr (tileable(MODE)) {
+    if (v == 4) {   // streamed shape clouds
+      if (block <= 512) FSS_K(512, 4);
+      else FSS_K(1024, 4);
+      return;
+    }
+  }
   if (ak.ppw > 1) {   // several pairs per workgroup: VALU inner loops, at most 512 lanes
     if (v == 1) hipLaunchKernelGGL((k_align<MODE, ES, 512, 1, true>), dim3(grid), dim3(block), lds, s, *pb, ks, lp, ak);
     else hipLaunchKernelGGL((k_align<MODE, ES, 512, 2, true>), dim3(grid), dim3(block), lds, s, *pb, ks, lp, ak);
@@ -1509,12 +1635,17 @@ struct AlignPlan {
   size_t lds;
   AlignK ak;
   long grid;
+  int variant;   // the inner loop launched: ap->variant, or the expanded / streamed form when the
+                 // clouds do not fit the requested layout in LDS
 };
 
-int plan_align(const FssPairBatch* batch, const KScore& ks, const FssAlignParams* ap, const FssAlignOut* out,
-               AlignPlan& pl) {
+int check_align_out(const FssAlignOut* out) {
   if (!out->q || !out->t || !out->best_score || !out->best_idx || !out->best_tf || !out->n_steps)
     return set_error(FSS_E_INVALID, "NULL align outputs");
+  return FSS_OK;
+}
+
+int plan_align(const FssPairBatch* batch, const KScore& ks, const FssAlignParams* ap, AlignPlan& pl) {
   const int B = ap->n_restarts, lpp = ap->lanes_per_pose, rpw = ap->restarts_per_wg;
   if (B <= 0) return set_error(FSS_E_INVALID, "n_restarts must be > 0");
   if (ap->max_steps < 0) return set_error(FSS_E_INVALID, "max_steps must be >= 0");
@@ -1538,26 +1669,59 @@ int plan_align(const FssPairBatch* batch, const KScore& ks, const FssAlignParams
   if (ppw > 1 && lanes > 512)
     return set_error(FSS_E_INVALID, "a workgroup of %d pairs holds at most 512 lanes (%ld asked)", ppw, lanes);
   pl.block = (int)((lanes + 63) / 64) * 64;
-  const bool mfma = effective_variant(ks.mode, ap->variant) == 3;
-  if (mfma && ppw > 1)
+  int var = effective_variant(ks.mode, ap->variant);
+  if (var == 3 && ppw > 1)
     return set_error(FSS_E_INVALID, "pairs_per_wg > 1 is for the VALU inner loops (variant 1 or 2)");
-  if (mfma && lpp != 64)
+  if (var == 3 && lpp != 64)
     return set_error(FSS_E_INVALID, "the MFMA inner loop (variant 3) runs one pose per wavefront: lanes_per_pose must be 64");
-  pl.lp = plan_lds(batch, ks.mode, mfma);
+  if (ap->tile_rows < 0) return set_error(FSS_E_INVALID, "tile_rows must be >= 0");
+  const size_t cap = max_lds_bytes();
+  const int sls = (rpw + groups - 1) / groups > 1 ? 24 : 14;   // floats of a restart's slot
+  const size_t tail = ppw * ((ap->early_stop ? 2 * (size_t)rpw * sizeof(double) : 0) + (size_t)rpw * sls * sizeof(float));
   // Several pairs: their LDS regions start 16 B apart modulo 128 B (a region of 8k + 1 float4s),
   // so the lanes of different pairs reading the same row index hit different banks.
-  const int region = ppw > 1 ? pl.lp.end + ((1 - pl.lp.end) % 8 + 8) % 8 : pl.lp.end;
-  const int sls = (rpw + groups - 1) / groups > 1 ? 24 : 14;   // floats of a restart's slot
-  pl.lds = (size_t)ppw * region * sizeof(float4) +
-           ppw * ((ap->early_stop ? 2 * (size_t)rpw * sizeof(double) : 0) + (size_t)rpw * sls * sizeof(float));
+  auto region_of = [&](const LdsPlan& lp) { return ppw > 1 ? lp.end + ((1 - lp.end) % 8 + 8) % 8 : lp.end; };
+  pl.lp = plan_lds(batch, ks.mode, var == 3);
+  int region = region_of(pl.lp);
+  pl.lds = (size_t)ppw * region * sizeof(float4) + tail;
+  if (var == 3 && pl.lds > cap) {
+    // the MFMA layout takes 76 B per ref row; clouds that fit the expanded form's 20 B run it
+    const LdsPlan e = plan_lds(batch, ks.mode, false);
+    if ((size_t)e.end * sizeof(float4) + tail <= cap) {
+      var = 2;
+      pl.lp = e;
+      region = e.end;
+      pl.lds = (size_t)region * sizeof(float4) + tail;
+    }
+  }
+  if (tileable(ks.mode) && (var == 4 || pl.lds > cap)) {
+    // Streamed clouds: each optimiser step passes over (ref tile, fit tile) pairs staged in LDS,
+    // so a pair of any size runs; the tiles are restaged only when a side has more than one.
+    if (ppw > 1)
+      return set_error(FSS_E_TOO_LARGE, "clouds of up to %d x %d rows do not fit LDS: the streamed path runs one "
+                                        "pair per workgroup (pairs_per_wg 1)", batch->max_ref, batch->max_fit);
+    if (var == 1)
+      return set_error(FSS_E_TOO_LARGE, "clouds of up to %d x %d rows do not fit LDS, and the streamed path has "
+                                        "the expanded form only (variant 0, 2 or 4)", batch->max_ref, batch->max_fit);
+    const int t0 = ap->tile_rows > 0 ? ap->tile_rows : 1024;
+    int tr = std::max(1, std::min(batch->max_ref, t0)), tf = std::max(1, std::min(batch->max_fit, t0));
+    while ((size_t)plan_lds_tiled(ks.mode, tr, tf).end * sizeof(float4) + tail > cap && std::max(tr, tf) > 16) {
+      if (tr >= tf) tr = (tr + 1) / 2;
+      else tf = (tf + 1) / 2;
+    }
+    var = 4;
+    pl.lp = plan_lds_tiled(ks.mode, tr, tf);
+    region = pl.lp.end;
+    pl.lds = (size_t)region * sizeof(float4) + tail;
+  }
   // Lane groups of a non-power-of-two width sum through LDS (about a sixth of the shuffle tree's
   // instructions per step), unless the scratch would cost occupancy: with it a workgroup must
   // still fit 16 to a CU (4 one-wavefront workgroups per SIMD, the VGPR limit).
   const size_t red = (size_t)red_floats(ppw * groups, lpp) * sizeof(float);
-  const bool red_lds = (lpp & (lpp - 1)) && pl.lds + red <= kCuLdsBytes / 16;
+  const bool red_lds = (lpp & (lpp - 1)) && var != 4 && pl.lds + red <= kCuLdsBytes / 16;
   if (red_lds) pl.lds += red;
-  const size_t cap = max_lds_bytes();
   if (pl.lds > cap) return set_error(FSS_E_TOO_LARGE, "clouds need %zu B of LDS, the device allows %zu B", pl.lds, cap);
+  pl.variant = var;
   AlignK& ak = pl.ak;
   ak.B = B; ak.max_steps = ap->max_steps;
   ak.lr = ap->lr; ak.b1 = ap->beta1; ak.b2 = ap->beta2; ak.eps = ap->eps;
@@ -1592,8 +1756,8 @@ int run_align(const FssPairBatch* batch, const KScore& ks, const FssAlignParams*
   pl.ak.q = out->q; pl.ak.t = out->t; pl.ak.pose_score = pose_score; pl.ak.n_steps = out->n_steps;
   const bool es = ap->early_stop != 0;
 #define FSS_ALIGN(M) \
-  (es ? launch_align<M, true>(batch, ks, pl.lp, pl.ak, (int)pl.grid, pl.block, pl.lds, ap->variant, s) \
-      : launch_align<M, false>(batch, ks, pl.lp, pl.ak, (int)pl.grid, pl.block, pl.lds, ap->variant, s))
+  (es ? launch_align<M, true>(batch, ks, pl.lp, pl.ak, (int)pl.grid, pl.block, pl.lds, pl.variant, s) \
+      : launch_align<M, false>(batch, ks, pl.lp, pl.ak, (int)pl.grid, pl.block, pl.lds, pl.variant, s))
   switch (ks.mode) {
     case FSS_MODE_SHAPE: FSS_ALIGN(FSS_MODE_SHAPE); break;
     case FSS_MODE_SHAPE_ESP: FSS_ALIGN(FSS_MODE_SHAPE_ESP); break;
@@ -1689,13 +1853,18 @@ int fss_self_overlap(const FssPairBatch* batch, const FssScoreParams* sp, double
   return check_launch("fss_self_overlap");
 }
 
-static int check_pose_lds(const FssPairBatch* batch, const LdsPlan& lp) {
-  const size_t need = (size_t)lp.end * sizeof(float4);
+// LDS plan of the one-workgroup-per-pose kernels: the whole pair when it fits, else (shape modes)
+// streamed tiles of up to 1024 rows per side.
+static int plan_pose_lds(const FssPairBatch* batch, int mode, LdsPlan& lp) {
+  lp = plan_lds(batch, mode);
   const size_t cap = max_lds_bytes() - 2 * (kPoseBlock / 64) * 13 * sizeof(float);
-  if (need > cap)
-    return set_error(FSS_E_TOO_LARGE, "clouds need %zu B of LDS, the device allows %zu B", need, cap);
-  (void)batch;
-  return FSS_OK;
+  if ((size_t)lp.end * sizeof(float4) <= cap) return FSS_OK;
+  if (tileable(mode)) {
+    lp = plan_lds_tiled(mode, std::max(1, std::min(batch->max_ref, 1024)), std::max(1, std::min(batch->max_fit, 1024)));
+    if ((size_t)lp.end * sizeof(float4) <= cap) return FSS_OK;
+  }
+  return set_error(FSS_E_TOO_LARGE, "clouds need %zu B of LDS, the device allows %zu B",
+                   (size_t)lp.end * sizeof(float4), cap);
 }
 
 int fss_overlap_grad(const FssPairBatch* batch, const FssScoreParams* sp, int32_t n_poses,
@@ -1708,8 +1877,8 @@ int fss_overlap_grad(const FssPairBatch* batch, const FssScoreParams* sp, int32_
   if (n_poses < 0 || !pose_pair || !q || !t || !V) return set_error(FSS_E_INVALID, "NULL pose arrays");
   if (need_grad && (!gR || !gT || !gQ)) return set_error(FSS_E_INVALID, "NULL gradient outputs");
   if (n_poses == 0) return FSS_OK;
-  const LdsPlan lp = plan_lds(batch, ks.mode);
-  if ((rc = check_pose_lds(batch, lp))) return rc;
+  LdsPlan lp;
+  if ((rc = plan_pose_lds(batch, ks.mode, lp))) return rc;
   hipStream_t s = (hipStream_t)stream;
 #define FSS_POSE0(M)                                                                                      \
   do {                                                                                                    \
@@ -1738,8 +1907,8 @@ int fss_pose_loss_grad(const FssPairBatch* batch, const FssScoreParams* sp, int3
     return set_error(FSS_E_INVALID, "NULL pose arrays");
   if (n_restarts <= 0) return set_error(FSS_E_INVALID, "n_restarts must be > 0");
   if (n_poses == 0) return FSS_OK;
-  const LdsPlan lp = plan_lds(batch, ks.mode);
-  if ((rc = check_pose_lds(batch, lp))) return rc;
+  LdsPlan lp;
+  if ((rc = plan_pose_lds(batch, ks.mode, lp))) return rc;
   hipStream_t s = (hipStream_t)stream;
   const float invB = 1.0f / (float)n_restarts;
   switch (ks.mode) {
@@ -1810,10 +1979,29 @@ int fss_align(const FssPairBatch* batch, const FssScoreParams* sp, const FssAlig
   if ((rc = check_batch(batch, ks.mode))) return rc;
   if (!ap || !out || !self_ov) return set_error(FSS_E_INVALID, "NULL align arguments");
   if (!out->pose_score) return set_error(FSS_E_INVALID, "NULL align outputs");
+  if ((rc = check_align_out(out))) return rc;
   AlignPlan pl;
-  if ((rc = plan_align(batch, ks, ap, out, pl))) return rc;
+  if ((rc = plan_align(batch, ks, ap, pl))) return rc;
   if (batch->n_pairs == 0) return FSS_OK;
   return run_align(batch, ks, ap, pl, self_ov, out->pose_score, out, (hipStream_t)stream);
+}
+
+int fss_align_plan(const FssPairBatch* batch, const FssScoreParams* sp, const FssAlignParams* ap,
+                   FssAlignPlanInfo* info) {
+  KScore ks;
+  int rc = make_kscore(sp, ks);
+  if (rc) return rc;
+  if ((rc = check_batch(batch, ks.mode))) return rc;
+  if (!ap || !info) return set_error(FSS_E_INVALID, "NULL align arguments");
+  AlignPlan pl;
+  if ((rc = plan_align(batch, ks, ap, pl))) return rc;
+  info->variant = pl.variant;
+  info->tile_ref = pl.lp.tile_ref;
+  info->tile_fit = pl.lp.tile_fit;
+  info->block = pl.block;
+  info->grid = pl.grid;
+  info->lds_bytes = (int64_t)pl.lds;
+  return FSS_OK;
 }
 
 int fss_align_noalloc(const FssPairBatch* batch, const FssScoreParams* sp, const FssAlignParams* ap,
@@ -1826,8 +2014,9 @@ int fss_align_noalloc(const FssPairBatch* batch, const FssScoreParams* sp, const
   if (ap->skip_finish)
     return set_error(FSS_E_INVALID, "fss_align_noalloc always finishes (its self-overlaps stay in the scratch "
                                     "buffer): skip_finish must be 0");
+  if ((rc = check_align_out(out))) return rc;
   AlignPlan pl;
-  if ((rc = plan_align(batch, ks, ap, out, pl))) return rc;
+  if ((rc = plan_align(batch, ks, ap, pl))) return rc;
   const size_t so_bytes = round256((size_t)batch->n_pairs * 4 * sizeof(double));
   const size_t need = so_bytes + (size_t)batch->n_pairs * (size_t)ap->n_restarts * sizeof(double);
   if (need > (size_t)FSS_NOALLOC_SCRATCH_BYTES)

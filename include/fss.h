@@ -123,7 +123,12 @@ typedef struct FssAlignParams {
   int32_t skip_finish;       /* 1: run the optimiser only; call fss_align_finish afterwards    */
   int32_t variant;           /* shape inner loop: 0 default (expanded form), 1 direct form,
                                 2 expanded form (exp2 of 2a.b' + W_a, factor exp2(-|b'|^2)),
-                                3 split-bf16 MFMA (lanes_per_pose 64)                          */
+                                3 split-bf16 MFMA (lanes_per_pose 64), 4 expanded form over
+                                clouds streamed through LDS tiles.  Clouds that do not fit LDS
+                                in the requested layout run the expanded form (3 -> 2), and in
+                                the shape modes (FSS_MODE_SHAPE, FSS_MODE_SHAPE_ESP) the
+                                streamed form (0, 2, 3 -> 4; pairs_per_wg 1); other modes then
+                                fail with FSS_E_TOO_LARGE                                      */
   int32_t groups_per_wg;     /* lane groups per workgroup (0: restarts_per_wg); each group runs
                                 ceil(restarts_per_wg / groups_per_wg) restarts in turn every
                                 step, so per-pair early stop can keep all of a pair's restarts in
@@ -132,6 +137,8 @@ typedef struct FssAlignParams {
                                 n_restarts: the workgroup's lane groups work on consecutive
                                 pairs of the launch order, so small clouds share one wavefront's
                                 per-step pose work (e.g. 6 pairs x 10 restarts x 1 lane) */
+  int32_t tile_rows;         /* streamed form: rows of each cloud per LDS tile (0: 1024, fewer
+                                when the workgroup's LDS needs it)                             */
 } FssAlignParams;
 
 typedef struct FssAlignOut {
@@ -252,6 +259,24 @@ int fss_pose_score(const FssPairBatch* batch, const FssScoreParams* sp, int32_t 
  */
 int fss_align(const FssPairBatch* batch, const FssScoreParams* sp, const FssAlignParams* ap,
               const double* self_ov, FssAlignOut* out, void* stream);
+
+/*
+ * The launch fss_align would make for these arguments, without launching: the inner loop it
+ * runs (variant 1-4, after the fallbacks for clouds that do not fit LDS), the streamed form's
+ * tile rows per side (0 when the clouds are staged whole), the workgroup size, the grid and the
+ * LDS bytes per workgroup.  Fails as fss_align would (FSS_E_INVALID, FSS_E_TOO_LARGE).
+ * No counterpart in the reference (its Triton kernels tile every pair): lets a caller see when a
+ * batch takes the streamed path.
+ */
+typedef struct FssAlignPlanInfo {
+  int32_t variant;
+  int32_t tile_ref, tile_fit;
+  int32_t block;
+  int64_t grid;
+  int64_t lds_bytes;
+} FssAlignPlanInfo;
+int fss_align_plan(const FssPairBatch* batch, const FssScoreParams* sp, const FssAlignParams* ap,
+                   FssAlignPlanInfo* info);
 
 /*
  * fss_align without any device allocation by the caller or the library: the self-overlaps are

@@ -131,7 +131,39 @@ def vol_color_default():
                 q0=q0, t0=t0, steps=40, early_stop=True, directionless=True)
 
 
-CONFIGS = {"c1": c1, "c2": c2, "c2_es": c2_es, "c3": c3, "vol_color_default": vol_color_default}
+def _large_seeds(ref, fit, B, seed):
+    rng = np.random.default_rng(seed)
+    q = synth.random_quaternions(rng, len(ref) * B).reshape(len(ref), B, 4)
+    return q, com_seeds(ref, fit, q)
+
+
+@lru_cache(maxsize=None)
+def large_esp():
+    """Surface shape + ESP clouds beyond one workgroup's LDS (the streamed path; seed 21): a
+    12000-point surface of a 600-atom random walk against a 300-point ligand surface, and two
+    6000-point surfaces of 300-atom walks.  2 restarts x 30 steps."""
+    rng = np.random.default_rng(21)
+    atoms = synth.random_walks(rng, [600, 40, 300, 300])
+    pts = [synth.surface_points(rng, a, n) for a, n in zip(atoms, [12000, 300, 6000, 6000])]
+    esp = [synth.esp_values(rng, len(x)) for x in pts]
+    ref, fit = [pts[0], pts[2]], [pts[1], pts[3]]
+    q0, t0 = _large_seeds(ref, fit, 2, 22)
+    return dict(kind="esp", idx=[0, 1], ref=ref, fit=fit, ref_w=[esp[0], esp[2]], fit_w=[esp[1], esp[3]],
+                q0=q0, t0=t0, lam=0.3 * LAM_SCALING, steps=30, early_stop=False)
+
+
+@lru_cache(maxsize=None)
+def large_vol():
+    """Volumetric shape beyond one workgroup's LDS (seed 23): a 9000-atom random walk against a
+    2000-atom one (11000 rows of 16 B).  2 restarts x 30 steps, per-pair early stop."""
+    rng = np.random.default_rng(23)
+    ref, fit = synth.random_walks(rng, [9000, 2000])
+    q0, t0 = _large_seeds([ref], [fit], 2, 24)
+    return dict(kind="shape", idx=[0], ref=[ref], fit=[fit], q0=q0, t0=t0, steps=30, early_stop=True)
+
+
+CONFIGS = {"c1": c1, "c2": c2, "c2_es": c2_es, "c3": c3, "vol_color_default": vol_color_default,
+           "large_esp": large_esp, "large_vol": large_vol}
 
 
 # ------------------------------------------------------------------------------- oracle runs
@@ -208,8 +240,9 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--jobs", type=int, default=4)
     ap.add_argument("--only", nargs="*", default=sorted(CONFIGS))
+    ap.add_argument("--threads", type=int, default=0, help="torch threads per job (0: cpus / jobs)")
     a = ap.parse_args()
-    threads = max(1, (os.cpu_count() or 8) // a.jobs)
+    threads = a.threads or max(1, (os.cpu_count() or 8) // a.jobs)
     data = {name: CONFIGS[name]() for name in a.only}
     work = [(name, i, threads, pair_data(data[name], i)) for name in a.only
             for i in range(len(data[name]["idx"]))]

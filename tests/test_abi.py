@@ -46,6 +46,8 @@ int main(void) {
          offsetof(FssQuery, rad), sizeof(FssLibRows), offsetof(FssLibRows, acounts));
   printf("%zu %zu %zu %zu\n", offsetof(FssQuery, all_scores), offsetof(FssHits, all_scores), offsetof(FssHits, n_all),
          offsetof(FssScreenParams, query_budget_bytes));
+  printf("%zu %zu %zu\n", offsetof(FssAlignParams, tile_rows), sizeof(FssAlignPlanInfo),
+         offsetof(FssAlignPlanInfo, lds_bytes));
   return 0;
 }
 '''
@@ -67,8 +69,10 @@ int main(void) {
     assert sizes[16:21] == [_lib.FssLibSpec.max_atoms.offset, _lib.FssScreenParams.shape_from_surface.offset,
                             _lib.FssQuery.rad.offset, ctypes.sizeof(_lib.FssLibRows),
                             _lib.FssLibRows.acounts.offset]
-    assert sizes[21:] == [_lib.FssQuery.all_scores.offset, _lib.FssHits.all_scores.offset,
-                          _lib.FssHits.n_all.offset, _lib.FssScreenParams.query_budget_bytes.offset]
+    assert sizes[21:25] == [_lib.FssQuery.all_scores.offset, _lib.FssHits.all_scores.offset,
+                            _lib.FssHits.n_all.offset, _lib.FssScreenParams.query_budget_bytes.offset]
+    assert sizes[25:] == [_lib.FssAlignParams.tile_rows.offset, ctypes.sizeof(_lib.FssAlignPlanInfo),
+                          _lib.FssAlignPlanInfo.lds_bytes.offset]
 
 
 def test_validation_rejects_bad_arguments_without_launching():

@@ -242,12 +242,16 @@ def test_edge_cases(dev):
     with pytest.raises(_lib.FssError, match="lanes_per_pose must be 64"):
         align(pb, ScoreSpec(), to_dev(q, dev), to_dev(t, dev), OptimSpec(max_steps=1), variant=3,
               lanes_per_pose=16, restarts_per_wg=1)
-    # oversized clouds are rejected with a clear error
+    # clouds beyond LDS: the shape modes stream them through LDS tiles; several pairs per
+    # workgroup cannot, and are refused with a clear error
     big = [np.zeros((20000, 3), np.float32)]
     pbb = PairBatch(CloudSet.from_arrays(big, device=dev), CloudSet.from_arrays(big, device=dev))
     so = torch.ones((1, 4), dtype=torch.float64, device=dev)
+    rb = align(pbb, ScoreSpec(), to_dev(q[:1], dev), to_dev(t[:1], dev), OptimSpec(max_steps=1), self_ov=so)
+    assert rb.extra["variant"] == 4 and rb.extra["tile_ref"] == 1024
     with pytest.raises(_lib.FssError, match="LDS"):
-        align(pbb, ScoreSpec(), to_dev(q[:1], dev), to_dev(t[:1], dev), OptimSpec(max_steps=1), self_ov=so)
+        align(pbb, ScoreSpec(), to_dev(q[:1], dev), to_dev(t[:1], dev), OptimSpec(max_steps=1), self_ov=so,
+              lanes_per_pose=1, restarts_per_wg=1, pairs_per_wg=2)
 
 
 def test_graph_replays_match_eager(dev):
