@@ -7,9 +7,13 @@ import subprocess
 import tempfile
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-SOURCES = [os.path.join(HERE, "csrc", "fss_kernels.hip"), os.path.join(HERE, "csrc", "fss_screen.hip")]
-DEPS = [os.path.join(HERE, "csrc", "fss_device.h"), os.path.join(HERE, "csrc", "fss_common.h"),
-        os.path.join(os.path.dirname(HERE), "include", "fss.h")]
+# Device code: the alignment kernels and the screening top-k merge.  Host-only C++: the error
+# helpers and the screening service (HIP runtime API only, so it also builds with g++ against a
+# host stand-in of that API for the sanitizer runs in tests/sanitize).
+SOURCES = [os.path.join(HERE, "csrc", f) for f in ("fss_kernels.hip", "fss_topk.hip", "fss_common.cpp",
+                                                   "fss_screen.cpp")]
+DEPS = [os.path.join(HERE, "csrc", f) for f in ("fss_device.h", "fss_common.h", "fss_topk.h")] + \
+    [os.path.join(os.path.dirname(HERE), "include", "fss.h")]
 OUT = os.path.join(HERE, "libfss_hip.so")
 ARCH = os.environ.get("FSS_OFFLOAD_ARCH", "gfx950")
 # Packed-FP32 instructions are disabled for every device pass.
