@@ -202,6 +202,36 @@ def default_tiling(pb: PairBatch, mode: int, B: int, early_stop: bool) -> Tuple[
     return cands[0][2], cands[0][3]
 
 
+def is_streamed(pb: PairBatch, spec: ScoreSpec, B: int) -> bool:
+    """True when the batch's clouds do not fit one workgroup's LDS in the expanded form, so
+    fss_align streams them through LDS tiles (shape modes; other modes then fail to plan)."""
+    ap = _lib.FssAlignParams()
+    ap.n_restarts, ap.max_steps = B, 1
+    ap.lanes_per_pose, ap.restarts_per_wg, ap.variant = 1, 1, 2
+    info = _lib.FssAlignPlanInfo()
+    bs, sp = pb.struct(), spec.struct()
+    lib = _lib.load()
+    if lib.fss_align_plan(_lib.byref(bs), _lib.byref(sp), _lib.byref(ap), _lib.byref(info)) != 0:
+        return False          # the launch reports its own error
+    return info.variant == 4
+
+
+def streamed_tiling(P: int, B: int, early_stop: bool, max_fit: int, waves_wanted: int = 8 * N_SIMD):
+    """(lanes_per_pose, restarts_per_wg, groups_per_wg) of the streamed path.  A lane owns fit
+    rows, so a pose's lanes span more wavefronts (up to 1024 lanes) only while the launch holds
+    fewer than about ``waves_wanted`` wavefronts and every lane keeps at least four fit rows.
+    Workgroups of one-wavefront groups hold two restarts (measured fastest on 12000 x 300-row
+    surface pairs); with early stop a workgroup holds every restart of the pair, its lane groups
+    running the rest in turn."""
+    lpp = 64
+    while lpp < 1024 and P * B * lpp // 64 < waves_wanted and 8 * lpp <= max_fit:
+        lpp *= 2
+    if early_stop:
+        return lpp, B, max(1, min(B, 1024 // lpp))
+    groups = max(1, min(B, 128 // lpp))
+    return lpp, groups, groups
+
+
 def check_tiling(B: int, lpp: int, rpw: int, early_stop: bool, groups: int = 0, pairs: int = 1) -> None:
     g = groups or rpw
     ppw = max(int(pairs or 1), 1)
@@ -209,9 +239,11 @@ def check_tiling(B: int, lpp: int, rpw: int, early_stop: bool, groups: int = 0, 
         raise ValueError("groups_per_wg exceeds restarts_per_wg")
     if ppw > 1 and rpw < B:
         raise ValueError("pairs_per_wg > 1 needs every restart of each pair in the workgroup")
-    if lpp < 1 or lpp > 64 or (lpp & (lpp - 1) and ppw * g * lpp > 64):
-        raise ValueError("lanes_per_pose must be in [1, 64], and a power of two unless the workgroup's "
-                         "pairs x lane groups x lanes_per_pose fit one wavefront (64)")
+    if lpp > 64 and lpp in (128, 256, 512, 1024) and ppw == 1 and g * lpp <= 1024:
+        pass                  # lane groups of several wavefronts (the streamed form checks the rest)
+    elif lpp < 1 or lpp > 64 or (lpp & (lpp - 1) and ppw * g * lpp > 64):
+        raise ValueError("lanes_per_pose must be in [1, 64] (or 128 .. 1024 on the streamed form), and a power "
+                         "of two unless the workgroup's pairs x lane groups x lanes_per_pose fit one wavefront (64)")
     if early_stop and rpw < B:
         raise ValueError("per-pair early stop needs every restart of a pair in one workgroup")
     if ppw * g * lpp > 1024:
@@ -258,6 +290,12 @@ def align(pb: PairBatch, spec: ScoreSpec, q0: torch.Tensor, t0: torch.Tensor,
     dev = pb.device
     if self_ov is None:
         self_ov = self_overlaps(pb, spec, stream)
+    streamed = lanes_per_pose is None and restarts_per_wg is None and is_streamed(pb, spec, B)
+    if streamed:
+        # clouds beyond LDS: lane groups of several wavefronts when the launch is short of them
+        max_fit = int(pb.fit_counts.max()) if pb.fit_counts is not None and len(pb.fit_counts) else 1
+        lanes_per_pose, restarts_per_wg, groups_per_wg = streamed_tiling(P, B, opt.early_stop, max_fit)
+        variant = 4
     if lanes_per_pose is None and restarts_per_wg is None:
         from . import autotune as _at
         tuned = _at.autotune(pb, spec, q0, t0, opt.early_stop, max_steps=opt.max_steps,

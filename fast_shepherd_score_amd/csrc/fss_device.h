@@ -192,6 +192,36 @@ __device__ __forceinline__ void group_sum_lds(Acc& a, int width, int j, int grp,
 // lane, 13 totals per group.
 constexpr int red_floats(int groups, int width) { return 13 * (groups * width + groups); }
 
+// A pose's sums over a lane group of `width` > 64 lanes, whole wavefronts (the streamed path):
+// each wavefront's sums by DPP levels, then every lane adds its group's wavefront totals from LDS
+// in wavefront order, so a group's lanes end with the same bits.  Every thread of the workgroup
+// calls it (two barriers); red holds 13 floats per wavefront of the workgroup.
+__device__ __forceinline__ void pose_sum_waves(Acc& a, int width, float* red) {
+  group_sum(a, 64, true);
+  const int w = threadIdx.x >> 6, nw = width >> 6, w0 = (threadIdx.x / width) * nw;
+  __syncthreads();   // the previous sums' readers are done with red
+  if ((threadIdx.x & 63) == 0) {
+    float* r = red + 13 * w;
+    r[0] = a.V;
+#pragma unroll
+    for (int i = 0; i < 3; ++i) r[1 + i] = a.gt[i];
+#pragma unroll
+    for (int i = 0; i < 9; ++i) r[4 + i] = a.G[i];
+  }
+  __syncthreads();
+  Acc s;
+  s.zero();
+  for (int k = 0; k < nw; ++k) {
+    const float* r = red + 13 * (w0 + k);
+    s.V += r[0];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) s.gt[i] += r[1 + i];
+#pragma unroll
+    for (int i = 0; i < 9; ++i) s.G[i] += r[4 + i];
+  }
+  a = s;
+}
+
 // A pose's sums over its lane group: DPP levels for power-of-two widths; else the LDS exchange
 // when the kernel reserved `red` (a one-wavefront workgroup of nl grouped lanes), the shuffle
 // tree otherwise.

@@ -761,6 +761,7 @@ struct AlignK {
   int lpp, rpw, wg_per_pair;
   int groups;                 // lane groups per workgroup; each runs ceil(rpw / groups) restarts
   int red_lds;                // lane groups of a non-power-of-two width sum through LDS
+  int xwave;                  // lane groups of several wavefronts (streamed path) sum through LDS
   int ppw;                    // pairs per workgroup (> 1: each with all its restarts)
   int region;                 // float4s between the LDS regions of a workgroup's pairs
   const int32_t* order;
@@ -824,8 +825,9 @@ void k_align(FssPairBatch pb, KScore ks, LdsPlan lp, AlignK ak) {
   // the other in every step.  A restart's Adam moments (and, when its group runs more than one,
   // its parameters) live in an LDS slot: m at 0..6, v at 7..13, and q and t at 14..20 in the
   // 24-float slots of groups running several restarts; lane 0 of the group writes, the group's
-  // lanes read (one wavefront, in-order DS operations).  With one restart per group the
-  // parameters stay in registers across steps and a slot is 14 floats.
+  // lanes read (within one wavefront the DS operations complete in order; a group of several
+  // wavefronts, streamed path only, reads and writes on either side of a barrier).  With one
+  // restart per group the parameters stay in registers across steps and a slot is 14 floats.
   const int rpg = (ak.rpw + groups - 1) / groups;
   const double* so = ak.self_ov + 4 * pair;
   const float invB = 1.0f / (float)ak.B;
@@ -834,7 +836,7 @@ void k_align(FssPairBatch pb, KScore ks, LdsPlan lp, AlignK ak) {
   const int sls = rpg > 1 ? 24 : 14;
   float* slots = slot_base + pl_buf * sls * ak.rpw;
   // lane groups of a width other than a power of two sum through LDS when the plan reserved it
-  float* red = ak.red_lds ? slot_base + ppw * sls * ak.rpw : nullptr;
+  float* red = ak.red_lds || ak.xwave ? slot_base + ppw * sls * ak.rpw : nullptr;
   const int nl = ppw * groups * lpp;
   auto local = [&](int i) { return grp + i * groups; };
   auto runs = [&](int rl) { return has_pair && rl < ak.rpw && blk * ak.rpw + rl < ak.B; };
@@ -905,8 +907,10 @@ void k_align(FssPairBatch pb, KScore ks, LdsPlan lp, AlignK ak) {
         else if (MF) shape_accum_mfma<ESP>(st.ref, st.xq, st.T, st.fit, st.M, R, ts, sh);
         else if (EXPD) shape_accum<ESP, true>(st.ref, st.refc, st.N, st.fit, st.M, j, lpp, R, ts, sh);
         else shape_accum_direct<ESP, true>(st.ref, st.N, st.fit, st.M, j, lpp, R, ts, sh);
-        if (TL && !work) continue;
-        pose_sum(sh, lpp, j, true, grp_all, nl, red);
+        // lane groups of several wavefronts: every lane goes on to the slot barrier below
+        if (TL && ak.xwave) pose_sum_waves(sh, lpp, red);
+        if (TL && !ak.xwave && !work) continue;
+        if (!TL || !ak.xwave) pose_sum(sh, lpp, j, true, grp_all, nl, red);
         shape_finish(sh, ks.K, ks.alpha, ks.s, true);
       }
       if (PH) {
@@ -929,15 +933,37 @@ void k_align(FssPairBatch pb, KScore ks, LdsPlan lp, AlignK ak) {
       for (int k = 0; k < 3; ++k) lT[k] = -(ws * sh.gt[k] + wp * ph.gt[k]);
       float g[7];
       se3_grad(lR, lT, qh, iq, invB, g);
+      if constexpr (TL) {
+        // A lane group of several wavefronts reads its restart's slot in every wavefront: all of
+        // them read before the group's lane 0 rewrites it.
+        float mk[7], vk[7];
 #pragma unroll
-      for (int k = 0; k < 7; ++k) {
-        const float mk = fmaf(1.f - ak.b1, g[k] - sl[k], sl[k]);
-        const float vk = fmaf(1.f - ak.b2, g[k] * g[k], sl[7 + k] * ak.b2);
-        p[k] = adam_update(p[k], mk, vk, step_size, ibc2s, ak.eps);
+        for (int k = 0; k < 7; ++k) {
+          mk[k] = fmaf(1.f - ak.b1, g[k] - sl[k], sl[k]);
+          vk[k] = fmaf(1.f - ak.b2, g[k] * g[k], sl[7 + k] * ak.b2);
+          p[k] = adam_update(p[k], mk[k], vk[k], step_size, ibc2s, ak.eps);
+        }
+        if (ak.xwave) __syncthreads();
+        if (!work) continue;
         if (j == 0) {
-          sl[k] = mk;
-          sl[7 + k] = vk;
-          if (rpg > 1) sl[14 + k] = p[k];
+#pragma unroll
+          for (int k = 0; k < 7; ++k) {
+            sl[k] = mk[k];
+            sl[7 + k] = vk[k];
+            if (rpg > 1) sl[14 + k] = p[k];
+          }
+        }
+      } else {
+#pragma unroll
+        for (int k = 0; k < 7; ++k) {
+          const float mk = fmaf(1.f - ak.b1, g[k] - sl[k], sl[k]);
+          const float vk = fmaf(1.f - ak.b2, g[k] * g[k], sl[7 + k] * ak.b2);
+          p[k] = adam_update(p[k], mk, vk, step_size, ibc2s, ak.eps);
+          if (j == 0) {
+            sl[k] = mk;
+            sl[7 + k] = vk;
+            if (rpg > 1) sl[14 + k] = p[k];
+          }
         }
       }
       if (ES && j == 0) buf[rl] = (double)loss;
@@ -1630,7 +1656,8 @@ int plan_align(const FssPairBatch* batch, const KScore& ks, const FssAlignParams
   if (B <= 0) return set_error(FSS_E_INVALID, "n_restarts must be > 0");
   if (ap->max_steps < 0) return set_error(FSS_E_INVALID, "max_steps must be >= 0");
   if (rpw < 1) return set_error(FSS_E_INVALID, "restarts_per_wg must be >= 1");
-  if (lpp < 1 || lpp > 64) return set_error(FSS_E_INVALID, "lanes_per_pose must be in [1, 64]");
+  if (lpp < 1 || lpp > 1024 || (lpp > 64 && (lpp & (lpp - 1))))
+    return set_error(FSS_E_INVALID, "lanes_per_pose must be in [1, 64], or 128, 256, 512 or 1024 on the streamed form");
   if (ap->early_stop && rpw < B)
     return set_error(FSS_E_INVALID, "early stop couples all %d restarts of a pair: restarts_per_wg (%d) must be >= n_restarts", B, rpw);
   const int groups = ap->groups_per_wg > 0 ? ap->groups_per_wg : rpw;
@@ -1657,7 +1684,10 @@ int plan_align(const FssPairBatch* batch, const KScore& ks, const FssAlignParams
   if (ap->tile_rows < 0) return set_error(FSS_E_INVALID, "tile_rows must be >= 0");
   const size_t cap = max_lds_bytes();
   const int sls = (rpw + groups - 1) / groups > 1 ? 24 : 14;   // floats of a restart's slot
-  const size_t tail = ppw * ((ap->early_stop ? 2 * (size_t)rpw * sizeof(double) : 0) + (size_t)rpw * sls * sizeof(float));
+  // lane groups of several wavefronts sum through 13 floats of LDS per wavefront
+  const bool xwave = lpp > 64;
+  const size_t tail = ppw * ((ap->early_stop ? 2 * (size_t)rpw * sizeof(double) : 0) + (size_t)rpw * sls * sizeof(float)) +
+                      (xwave ? 13 * sizeof(float) * (size_t)(pl.block / 64) : 0);
   // Several pairs: their LDS regions start 16 B apart modulo 128 B (a region of 8k + 1 float4s),
   // so the lanes of different pairs reading the same row index hit different banks.
   auto region_of = [&](const LdsPlan& lp) { return ppw > 1 ? lp.end + ((1 - lp.end) % 8 + 8) % 8 : lp.end; };
@@ -1700,6 +1730,10 @@ int plan_align(const FssPairBatch* batch, const KScore& ks, const FssAlignParams
   const size_t red = (size_t)red_floats(ppw * groups, lpp) * sizeof(float);
   const bool red_lds = (lpp & (lpp - 1)) && var != 4 && pl.lds + red <= kCuLdsBytes / 16;
   if (red_lds) pl.lds += red;
+  // lane groups of several wavefronts: the streamed form only (every lane meets every barrier)
+  if (xwave && var != 4)
+    return set_error(FSS_E_INVALID, "lanes_per_pose above 64 runs on the streamed form only (variant 4, or "
+                                    "clouds beyond LDS in the shape modes)");
   if (pl.lds > cap) return set_error(FSS_E_TOO_LARGE, "clouds need %zu B of LDS, the device allows %zu B", pl.lds, cap);
   pl.variant = var;
   AlignK& ak = pl.ak;
@@ -1720,6 +1754,7 @@ int plan_align(const FssPairBatch* batch, const KScore& ks, const FssAlignParams
   ak.lpp = lpp; ak.rpw = rpw; ak.wg_per_pair = (B + rpw - 1) / rpw;
   ak.groups = groups;
   ak.red_lds = red_lds;
+  ak.xwave = xwave;
   ak.ppw = ppw;
   ak.region = region;
   ak.order = ap->pair_order;

@@ -5,12 +5,14 @@ FP64 oracle (needs an MI355X).
     ragged last tiles): best restarts within 1e-3 of the oracle's from the same seeds, FP64
     re-scoring of the final poses within 1e-4 of the oracle's scoring, and close to the staged
     expanded form on the same batch.  Tilings with lanes that hold no restart (a partial last
-    restart block) and lane groups that run two restarts each; early stop on and off.
+    restart block), lane groups that run two or three restarts each, and lane groups of two and
+    four wavefronts (pose sums across wavefronts through LDS); early stop on and off.
   * Clouds beyond one workgroup's LDS (config_large_*.npz, the oracle's alignments computed once
     by tests/golden/make_config_vectors.py): a 12000-point surface + ESP against a 300-point one,
     two 6000-point surfaces, a 9000-atom against a 2000-atom volume.  fss_align takes the
-    streamed path by itself; the per-pose loss and SE(3) gradient at the seeds (the one-pose
-    kernel, streamed as well) are checked against the oracle too.
+    streamed path by itself, with lane groups of up to 1024 lanes so that the few poses fill the
+    GPU; the per-pose loss and SE(3) gradient at the seeds (the one-pose kernel, streamed as
+    well) are checked against the oracle too.
 """
 import numpy as np
 import pytest
@@ -56,9 +58,11 @@ def test_forced_streaming_matches_oracle_and_staged(dev, kind, early_stop):
               for i, p in enumerate(pairs)]
     want = np.array([o["score"] for o in oracle])
     tilings = [dict(lanes_per_pose=16, restarts_per_wg=6),
-               dict(lanes_per_pose=64, restarts_per_wg=6, groups_per_wg=3)]
+               dict(lanes_per_pose=64, restarts_per_wg=6, groups_per_wg=3),
+               # lane groups of two wavefronts, each group running three restarts in turn
+               dict(lanes_per_pose=128, restarts_per_wg=6, groups_per_wg=2)]
     if not early_stop:   # 6 restarts in blocks of 4: the second block's last two groups idle
-        tilings.append(dict(lanes_per_pose=32, restarts_per_wg=4))
+        tilings += [dict(lanes_per_pose=32, restarts_per_wg=4), dict(lanes_per_pose=256, restarts_per_wg=4)]
     for kw in tilings:
         res = align(pb, spec, q0, t0, opt, variant=4, tile_rows=48, **kw)
         assert res.extra["variant"] == 4 and res.extra["tile_ref"] == res.extra["tile_fit"] == 48

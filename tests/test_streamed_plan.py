@@ -73,6 +73,20 @@ def test_streamed_form_on_request_and_tile_rows():
     assert rc == 0 and info.variant == 4 and info.block == 640 and info.lds_bytes <= LDS
 
 
+def test_lane_groups_of_several_wavefronts():
+    rc, info, err = plan(_lib.MODE_SHAPE_ESP, 12000, 300, lpp=256, rpw=4)
+    assert rc == 0 and info.variant == 4 and info.block == 1024, err
+    rc, info, err = plan(_lib.MODE_SHAPE, 12000, 300, lpp=1024, rpw=10, early_stop=1, B=10, groups=1)
+    assert rc == 0 and info.variant == 4 and info.block == 1024, err
+    # only the streamed form takes them
+    rc, _, err = plan(_lib.MODE_SHAPE, 300, 300, lpp=256, rpw=2)
+    assert rc == _lib.E_INVALID and b"streamed" in err
+    rc, _, err = plan(_lib.MODE_SHAPE, 12000, 300, lpp=96, rpw=1)
+    assert rc == _lib.E_INVALID
+    rc, _, err = plan(_lib.MODE_SHAPE, 12000, 300, lpp=512, rpw=4)
+    assert rc == _lib.E_INVALID and b"1024 threads" in err
+
+
 def test_tiles_shrink_to_fit_a_large_restart_tail():
     # 1300 restarts in one workgroup, two per lane group, with early stop: 20.8 KiB of losses and
     # 124.8 KiB of Adam slots leave 14.4 KiB for the tiles
