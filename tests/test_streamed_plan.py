@@ -3,7 +3,8 @@
 fss_align_plan reports the launch fss_align would make.  Clouds that do not fit the workgroup's
 LDS (160 KiB on gfx950) in the requested layout fall back to the expanded form (MFMA layout
 3 -> 2) and, in the shape modes, to the expanded form over clouds streamed through LDS tiles
-(variant 4); the other modes are refused with FSS_E_TOO_LARGE.
+(variant 4); the other modes are refused with FSS_E_TOO_LARGE.  The default inner loop also
+streams clouds whose staged LDS would leave a CU fewer than 8 wavefronts.
 """
 import ctypes
 
@@ -61,6 +62,17 @@ def test_clouds_beyond_lds_stream_through_tiles(mode, variant):
     # a side that is smaller than a tile is staged whole
     rc, info, _ = plan(mode, 12000, 300)
     assert rc == 0 and info.variant == 4 and (info.tile_ref, info.tile_fit) == (1024, 300)
+
+
+def test_default_loop_streams_clouds_that_starve_occupancy():
+    # 7000 x 300 surface + ESP rows staged whole take 145 KiB: one two-wavefront workgroup per CU
+    rc, info, _ = plan(_lib.MODE_SHAPE_ESP, 7000, 300, variant=0, lpp=64, rpw=2)
+    assert rc == 0 and info.variant == 4 and info.tile_ref == 1024
+    # the expanded form asked for by name stays staged; small clouds stay staged
+    rc, info, _ = plan(_lib.MODE_SHAPE_ESP, 7000, 300, variant=2, lpp=64, rpw=2)
+    assert rc == 0 and info.variant == 2 and info.tile_ref == 0
+    rc, info, _ = plan(_lib.MODE_SHAPE_ESP, 400, 400, variant=0, lpp=64, rpw=2)
+    assert rc == 0 and info.variant == 2
 
 
 def test_streamed_form_on_request_and_tile_rows():
