@@ -1337,6 +1337,152 @@ __global__ __launch_bounds__(kPoseBlock) void k_score_bwd(const T* __restrict__ 
   }
 }
 
+// ------------------------------------------------------------------------------------------------
+// Clouds beyond LDS: the same forward and backward with each wavefront staging the 64 points of
+// the column block it works on (one per lane) into a buffer of its own, and the row points read
+// from HBM; the column partition per row (blocks of 64; slice c % 4 in the backward) is the staged
+// kernels', so the sums are taken in the same order.
+// ------------------------------------------------------------------------------------------------
+// Scaled 4-D point i of a cloud, as stage_cloud writes it.
+template <typename T, bool ESP>
+__device__ __forceinline__ typename Vec4<T>::type score_point(const T* __restrict__ src, int i, const T c[3],
+                                                              T s, T sw) {
+  const T* p = src + 4 * (long)i;
+  return {(p[0] - c[0]) * s, (p[1] - c[1]) * s, (p[2] - c[2]) * s, ESP ? p[3] * sw : (T)0};
+}
+
+// Points [c0, c0 + n) of a cloud into the calling wavefront's buffer (lane k writes point c0 + k).
+// The wavefront-scope fences order the stores after the previous block's loads and before this
+// block's (a wavefront's LDS operations complete in issue order).
+template <typename T, bool ESP>
+__device__ __forceinline__ void stage_block(const T* __restrict__ src, int c0, int n, const T c[3], T s, T sw,
+                                            typename Vec4<T>::type* wbuf) {
+  const int lane = threadIdx.x & 63;
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  if (lane < n) wbuf[lane] = score_point<T, ESP>(src, c0 + lane, c, s, sw);
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+template <typename T, bool ESP>
+__device__ __forceinline__ void score_frame(const T* __restrict__ A, int N, const T* __restrict__ Bp, int M,
+                                            T alpha, T lam, T c[3], T& s, T& sw) {
+  const T base = score_base<T>();
+  s = sqrt(alpha / 2 * base);
+  sw = ESP ? sqrt(base / lam) : (T)0;
+  const T* o = N > 0 ? A : Bp;
+  for (int k = 0; k < 3; ++k) c[k] = N + M > 0 ? o[k] : (T)0;
+}
+
+template <typename T, bool ESP>
+__global__ __launch_bounds__(kPoseBlock) void k_score_fwd_stream(const T* __restrict__ ref, const T* __restrict__ fit,
+                                                                 const int32_t* __restrict__ rr,
+                                                                 const int32_t* __restrict__ fr, T alpha, T lam,
+                                                                 T* S, T* ov) {
+  using V = typename Vec4<T>::type;
+  __shared__ V wbufs[kPoseBlock];
+  __shared__ T red[3 * (kPoseBlock / 64)];
+  const int pair = blockIdx.x;
+  const int N = rr[2 * pair + 1], M = fr[2 * pair + 1];
+  const T* A = ref + 4 * (long)rr[2 * pair];
+  const T* Bp = fit + 4 * (long)fr[2 * pair];
+  T c[3], s, sw;
+  score_frame<T, ESP>(A, N, Bp, M, alpha, lam, c, s, sw);
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, nw = blockDim.x >> 6;
+  V* wbuf = wbufs + 64 * wave;
+  T acc[3] = {0, 0, 0};
+  int u = 0;
+#pragma unroll
+  for (int t = 0; t < 3; ++t) {
+    const T* rows = t == 1 ? A : Bp;
+    const T* cols = t == 2 ? Bp : A;
+    const int nr = t == 1 ? N : M, nc = t == 2 ? M : N;
+    const int nbr = (nr + 63) / 64, nbc = (nc + 63) / 64;
+    for (int i = 0; i < nbr; ++i) {
+      const int r = i * 64 + lane;
+      const V p = score_point<T, ESP>(rows, r < nr ? r : nr - 1, c, s, sw);
+      for (int j = t == 0 ? 0 : i; j < nbc; ++j, ++u) {
+        if (u % nw != wave) continue;
+        const int n = min(64, nc - 64 * j);
+        stage_block<T, ESP>(cols, 64 * j, n, c, s, sw, wbuf);
+        const T e = tile_sum<T, ESP>(wbuf, n, p);
+        acc[t] += r < nr ? (t != 0 && j > i ? 2 * e : e) : (T)0;
+      }
+    }
+  }
+#pragma unroll
+  for (int t = 0; t < 3; ++t) {
+    const T v = group_sum(acc[t], 64);
+    if (lane == 0) red[3 * wave + t] = v;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    T v[3] = {0, 0, 0};
+    for (int w = 0; w < nw; ++w)
+      for (int t = 0; t < 3; ++t) v[t] += red[3 * w + t];
+    const T K = (T)pow(M_PI / (2.0 * (double)alpha), 1.5);
+    const T vab = v[0] * K, vaa = v[1] * K, vbb = v[2] * K;
+    S[pair] = vab / (vaa + vbb - vab);
+    ov[4 * pair] = vab; ov[4 * pair + 1] = vaa; ov[4 * pair + 2] = vbb; ov[4 * pair + 3] = 0;
+  }
+}
+
+template <typename T, bool ESP>
+__global__ __launch_bounds__(kPoseBlock) void k_score_bwd_stream(const T* __restrict__ ref, const T* __restrict__ fit,
+                                                                 const int32_t* __restrict__ rr,
+                                                                 const int32_t* __restrict__ fr, T alpha, T lam,
+                                                                 const T* __restrict__ ov, const T* __restrict__ gS,
+                                                                 T* d_fit, T* d_ref) {
+  using V = typename Vec4<T>::type;
+  __shared__ V wbufs[kPoseBlock];
+  const int pair = blockIdx.x;
+  const int N = rr[2 * pair + 1], M = fr[2 * pair + 1];
+  const T* A = ref + 4 * (long)rr[2 * pair];
+  const T* Bp = fit + 4 * (long)fr[2 * pair];
+  T c[3], s, sw;
+  score_frame<T, ESP>(A, N, Bp, M, alpha, lam, c, s, sw);
+  const T K = (T)pow(M_PI / (2.0 * (double)alpha), 1.5);
+  const T vab = ov[4 * pair], U = ov[4 * pair + 1] + ov[4 * pair + 2], D = U - vab;
+  const T g = gS[pair] * K * alpha / (D * D) / s;
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, nw = blockDim.x >> 6;
+  V* wbuf = wbufs + 64 * wave;
+  const int sl = lane / kScoreRows, rl = lane % kScoreRows;
+  const int nbf = (M + kScoreRows - 1) / kScoreRows, nba = d_ref ? (N + kScoreRows - 1) / kScoreRows : 0;
+  for (int u = wave; u < nbf + nba; u += nw) {
+    const bool fit_row = u < nbf;
+    const T* own = fit_row ? Bp : A;
+    const T* oth = fit_row ? A : Bp;
+    const int n_own = fit_row ? M : N, n_oth = fit_row ? N : M;
+    const int row = (fit_row ? u : u - nbf) * kScoreRows + rl;
+    const V p = score_point<T, ESP>(own, row < n_own ? row : n_own - 1, c, s, sw);
+    T fo[3] = {0, 0, 0}, fs[3] = {0, 0, 0};
+    for (int c0 = 0; c0 < n_oth; c0 += 64) {
+      const int n = min(64, n_oth - c0);
+      stage_block<T, ESP>(oth, c0, n, c, s, sw, wbuf);
+      slice_force<T, ESP>(wbuf, n, sl, p, fo);
+    }
+    for (int c0 = 0; c0 < n_own; c0 += 64) {
+      const int n = min(64, n_own - c0);
+      stage_block<T, ESP>(own, c0, n, c, s, sw, wbuf);
+      slice_force<T, ESP>(wbuf, n, sl, p, fs);
+    }
+    T d[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      T v = U * fo[k] - 2 * vab * fs[k];
+      v += __shfl_xor(v, kScoreRows);
+      v += __shfl_xor(v, 2 * kScoreRows);
+      d[k] = g * v;
+    }
+    if (sl == 0 && row < n_own) {
+      T* o = fit_row ? d_fit + 4 * ((long)fr[2 * pair] + row) : d_ref + 4 * ((long)rr[2 * pair] + row);
+      o[0] = d[0]; o[1] = d[1]; o[2] = d[2]; o[3] = 0;
+    }
+  }
+}
+
 template <int MODE>
 void launch_self(const FssPairBatch* pb, const KScore& ks, double* out, hipStream_t s) {
   hipLaunchKernelGGL(k_self_overlap<MODE>, dim3(pb->n_pairs, 2), dim3(kPoseBlock), 0, s, *pb, ks, out);
@@ -1396,13 +1542,12 @@ int score_check(const FssPairBatch* batch, const FssScoreParams* sp) {
   return FSS_OK;
 }
 
-// Both clouds of a pair are staged in LDS: 4 values per point.
+// Both clouds of a pair are staged in LDS (4 values per point) when they fit; lds = 0 selects the
+// streamed kernels (a 64-point block per wavefront).
 int score_lds(const FssPairBatch* batch, size_t elem, size_t& lds) {
   if (batch->max_ref < 0 || batch->max_fit < 0) return set_error(FSS_E_INVALID, "negative max_ref / max_fit");
   lds = ((size_t)batch->max_ref + (size_t)batch->max_fit) * 4 * elem;
-  if (lds > max_lds_bytes())
-    return set_error(FSS_E_INVALID, "score fwd/bwd stage both clouds of a pair in LDS: max_ref + max_fit "
-                                    "must be at most %zu", max_lds_bytes() / (4 * elem));
+  if (lds > max_lds_bytes()) lds = 0;
   return FSS_OK;
 }
 
@@ -1416,7 +1561,12 @@ int score_fwd_t(const FssPairBatch* batch, const FssScoreParams* sp, const T* re
   const T lam = sp->mode == FSS_MODE_SHAPE_ESP ? (T)sp->lam : (T)1;
   size_t lds = 0;
   if ((rc = score_lds(batch, sizeof(T), lds))) return rc;
-  if (sp->mode == FSS_MODE_SHAPE_ESP)
+  const bool esp = sp->mode == FSS_MODE_SHAPE_ESP;
+  if (!lds && esp)
+    hipLaunchKernelGGL((k_score_fwd_stream<T, true>), dim3(batch->n_pairs), dim3(kPoseBlock), 0, s, ref, fit, batch->ref_rng, batch->fit_rng, (T)sp->alpha, lam, S, ov);
+  else if (!lds)
+    hipLaunchKernelGGL((k_score_fwd_stream<T, false>), dim3(batch->n_pairs), dim3(kPoseBlock), 0, s, ref, fit, batch->ref_rng, batch->fit_rng, (T)sp->alpha, lam, S, ov);
+  else if (esp)
     hipLaunchKernelGGL((k_score_fwd<T, true>), dim3(batch->n_pairs), dim3(kPoseBlock), lds, s, ref, fit, batch->ref_rng, batch->fit_rng, (T)sp->alpha, lam, S, ov);
   else
     hipLaunchKernelGGL((k_score_fwd<T, false>), dim3(batch->n_pairs), dim3(kPoseBlock), lds, s, ref, fit, batch->ref_rng, batch->fit_rng, (T)sp->alpha, lam, S, ov);
@@ -1433,7 +1583,12 @@ int score_bwd_t(const FssPairBatch* batch, const FssScoreParams* sp, const T* re
   const T lam = sp->mode == FSS_MODE_SHAPE_ESP ? (T)sp->lam : (T)1;
   size_t lds = 0;
   if ((rc = score_lds(batch, sizeof(T), lds))) return rc;
-  if (sp->mode == FSS_MODE_SHAPE_ESP)
+  const bool esp = sp->mode == FSS_MODE_SHAPE_ESP;
+  if (!lds && esp)
+    hipLaunchKernelGGL((k_score_bwd_stream<T, true>), dim3(batch->n_pairs), dim3(kPoseBlock), 0, s, ref, fit, batch->ref_rng, batch->fit_rng, (T)sp->alpha, lam, ov, gS, d_fit, d_ref);
+  else if (!lds)
+    hipLaunchKernelGGL((k_score_bwd_stream<T, false>), dim3(batch->n_pairs), dim3(kPoseBlock), 0, s, ref, fit, batch->ref_rng, batch->fit_rng, (T)sp->alpha, lam, ov, gS, d_fit, d_ref);
+  else if (esp)
     hipLaunchKernelGGL((k_score_bwd<T, true>), dim3(batch->n_pairs), dim3(kPoseBlock), lds, s, ref, fit, batch->ref_rng, batch->fit_rng, (T)sp->alpha, lam, ov, gS, d_fit, d_ref);
   else
     hipLaunchKernelGGL((k_score_bwd<T, false>), dim3(batch->n_pairs), dim3(kPoseBlock), lds, s, ref, fit, batch->ref_rng, batch->fit_rng, (T)sp->alpha, lam, ov, gS, d_fit, d_ref);

@@ -108,9 +108,24 @@ def test_ragged_tile_edges_and_determinism(dev, N, M):
         assert np.array_equal(a, b)
 
 
-def test_lds_limit_is_reported(dev):
-    """Both clouds of a pair live in LDS: a pair beyond the limit is refused, not run."""
-    from fast_shepherd_score_amd._lib import FssError
-    ref = torch.zeros((1, 5200, 3), device=dev)
-    with pytest.raises(FssError, match="LDS"):
-        gaussian_similarity(ref, ref.clone(), alpha=0.81)
+@pytest.mark.parametrize("dtype,N,M", [(torch.float32, 6000, 4500), (torch.float64, 3000, 2500)])
+def test_clouds_beyond_lds_stream(dev, dtype, N, M):
+    """A pair whose clouds exceed LDS (FP32: 10240 points, FP64: 5120) runs the streamed kernels:
+    S and both gradients against autograd of the FP64 oracle (ESP weights on)."""
+    print(f"case score-as-loss streamed {dtype} N={N} M={M}", flush=True)
+    ref64, fit64, re64, fe64 = _clouds(1, N, M, 5, torch.float64)
+    ref64, fit64 = ref64 * 6.0, fit64 * 6.0          # molecule-sized spread for that many points
+    x = fit64.clone().requires_grad_(True)
+    r = ref64.clone().requires_grad_(True)
+    S_ref = orc.tanimoto_esp(r, x, re64, fe64, 0.81, 0.5)
+    S_ref.sum().backward()
+    ref = ref64.to(dtype).to(dev).requires_grad_(True)
+    fit = fit64.to(dtype).to(dev).requires_grad_(True)
+    S = gaussian_similarity(ref, fit, re64.to(dtype).to(dev), fe64.to(dtype).to(dev), 0.81, 0.5)
+    S.sum().backward()
+    tol = 1e-10 if dtype == torch.float64 else 1e-5
+    np.testing.assert_allclose(S.detach().cpu().numpy(), S_ref.detach().numpy(), atol=tol)
+    gtol = 1e-9 if dtype == torch.float64 else 1e-4
+    for got, want in ((fit.grad, x.grad), (ref.grad, r.grad)):
+        g, w = got.cpu().numpy(), want.numpy()
+        np.testing.assert_allclose(g, w, atol=gtol * np.abs(w).max())
