@@ -317,8 +317,10 @@ def align(pb: PairBatch, spec: ScoreSpec, q0: torch.Tensor, t0: torch.Tensor,
     grp = int(groups_per_wg or 0)
     if variant == 3 and lanes_per_pose is None and opt.early_stop and B > 16 and not grp:
         # the MFMA loop needs a wavefront per pose: with every restart of a pair in one
-        # workgroup, 50 restarts run as 5 wavefronts of 10 restarts each
-        grp = max(d for d in range(1, 17) if B % d == 0)
+        # workgroup, 50 restarts run as 8 wavefronts of 6-7 restarts each (two such workgroups
+        # fill a CU's 16 wavefront slots; 425 ms against 626 ms for 5 wavefronts of 10 on the
+        # 4096-pair surface + ESP batch)
+        grp = 8
     ppw = int(pairs_per_wg or 1)
     check_tiling(B, lpp, rpw, opt.early_stop, grp, ppw)
     order_t = None

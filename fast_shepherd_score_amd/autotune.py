@@ -97,15 +97,16 @@ def autotune(pb: PairBatch, spec: ScoreSpec, q0: torch.Tensor, t0: torch.Tensor,
         runs += [(64, r, 3, 0) for r in ((B,) if early_stop else sorted({1, 2, 4, 8, min(16, B)}))
                  if r <= B and 64 * r <= 1024]
         if early_stop and B > 16:
-            # every restart of a pair in one workgroup of g wavefronts, each running B / g in turn
-            runs += [(64, B, 3, g) for g in (2, 4, 5, 8, 10, 16) if g < B and B % g == 0][:3]
+            # every restart of a pair in one workgroup of g wavefronts, each running about B / g
+            # in turn (8 wavefronts: two workgroups fill a CU's 16 wavefront slots)
+            runs += [(64, B, 3, g) for g in (5, 8, 10, 16) if g < B]
     if self_ov is None:
         self_ov = self_overlaps(pb, spec)
     s1, s2 = steps, 3 * steps
 
-    def timed(n, kw):
+    def timed(n, kw, es=False):
         ev = []
-        align(pb, spec, q0, t0, OptimSpec(max_steps=n, early_stop=False), events=ev, **kw)
+        align(pb, spec, q0, t0, OptimSpec(max_steps=n, early_stop=es), events=ev, **kw)
         torch.cuda.synchronize()
         return ev[0][0].elapsed_time(ev[0][1])
 
@@ -132,7 +133,20 @@ def autotune(pb: PairBatch, spec: ScoreSpec, q0: torch.Tensor, t0: torch.Tensor,
     model_first = min(((ms, c) for ms, c in scored if (c[0], c[1]) == tilings[0]), default=None)
     if model_first is not None and model_first[1] not in close:
         close.append(model_first[1])
-    if len(close) > 1 and max_steps > s2:
+    if early_stop and max_steps > s2:
+        # Per-pair early stop ends pairs at different steps, and the launch's tail then depends on
+        # the workgroup size, which trials without it do not show: the best three and every
+        # grouped MFMA tiling run in full, with early stop (measured on the 4096-pair surface +
+        # ESP batch: 8 wavefronts of 6-7 restarts 425 ms, 5 of 10 restarts 626 ms).
+        cands = list(dict.fromkeys([c for _, c in scored[:3]] + [c for _, c in scored if c[2] == 3 and c[3]]))
+        full = {}
+        for c in cands:
+            kw = dict(lanes_per_pose=c[0], restarts_per_wg=c[1], self_ov=self_ov, variant=c[2], groups_per_wg=c[3])
+            full[c] = timed(max_steps, kw, es=True)
+            if verbose:
+                print(f"autotune {key}: {c} with early stop over {max_steps} steps {full[c]:.3f} ms", flush=True)
+        best = min(full, key=full.get)
+    elif len(close) > 1 and max_steps > s2:
         n = min(max_steps, 4 * s2)
         long_ms = {}
         for c in close:
