@@ -202,6 +202,18 @@ def default_tiling(pb: PairBatch, mode: int, B: int, early_stop: bool) -> Tuple[
     return cands[0][2], cands[0][3]
 
 
+def mfma_by_default(pb: PairBatch, mode: int, early_stop: bool) -> bool:
+    """Whether the untuned default is the split-bf16 MFMA loop (shape modes): from clouds of about
+    100 points per side with early stop, 150 without (measured on surface + ESP batches of 2048
+    pairs, 50 x 200: 60-90 points 30 ms VALU / 37 ms MFMA; 100-150 81 / 59 with early stop, 81 /
+    83 without; 150-200 161 / 93 and 153 / 138; 200-400 points 1075 / 466 and 956 / 672 ms on
+    4096 pairs)."""
+    if mode not in (_lib.MODE_SHAPE, _lib.MODE_SHAPE_ESP) or pb.ref_counts is None or not len(pb.ref_counts):
+        return False
+    size = float(np.sqrt(np.mean(pb.ref_counts) * np.mean(pb.fit_counts)))
+    return size >= (100.0 if early_stop else 150.0)
+
+
 def is_streamed(pb: PairBatch, spec: ScoreSpec, B: int) -> bool:
     """True when the batch's clouds do not fit one workgroup's LDS in the expanded form, so
     fss_align streams them through LDS tiles (shape modes; other modes then fail to plan)."""
@@ -273,7 +285,9 @@ def align(pb: PairBatch, spec: ScoreSpec, q0: torch.Tensor, t0: torch.Tensor,
     Tiling: explicit ``lanes_per_pose`` / ``restarts_per_wg``, else an autotuned choice
     (``tune=True``, or one cached for this batch profile), else the cost model's choice.
     ``variant`` picks the shape inner loop: 1 direct form, 2 expanded form, 3 split-bf16 MFMA
-    (one wavefront per pose, so 64 lanes per pose), 0 the tuned or default choice.
+    (one wavefront per pose, so 64 lanes per pose), 0 the tuned or default choice (untuned: the
+    MFMA loop for shape-mode clouds of about 100 points per side with early stop, 150 without,
+    see :func:`mfma_by_default`; the expanded form otherwise).
     ``groups_per_wg``: lane groups per workgroup, each running restarts_per_wg / groups_per_wg
     restarts in turn every step (None: the tuned choice, else one restart per group).
     ``pairs_per_wg``: pairs per workgroup (each with all its restarts): consecutive pairs of the
@@ -308,6 +322,8 @@ def align(pb: PairBatch, spec: ScoreSpec, q0: torch.Tensor, t0: torch.Tensor,
                 groups_per_wg = grp0
         else:
             lpp0, rpw0 = default_tiling(pb, spec.mode, B, opt.early_stop)
+            if variant == 0 and not streamed and mfma_by_default(pb, spec.mode, opt.early_stop):
+                variant = 3
     else:
         lpp0, rpw0 = default_tiling(pb, spec.mode, B, opt.early_stop)
     if variant == 3 and spec.mode not in (_lib.MODE_PHARM, _lib.MODE_ESP_COMBO) and lanes_per_pose is None:
