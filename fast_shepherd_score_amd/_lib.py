@@ -151,6 +151,7 @@ class FssScreenParams(ctypes.Structure):
         ("shape_from_surface", ctypes.c_int32),
         ("query_budget_bytes", ctypes.c_int64),
         ("pairs_per_wg", ctypes.c_int32),
+        ("groups_per_wg", ctypes.c_int32),
     ]
 
 

@@ -497,6 +497,7 @@ int launch(FssScreen* s, const std::vector<int>& act) {
     ap.lanes_per_pose = s->prm.lanes_per_pose; ap.restarts_per_wg = s->prm.restarts_per_wg;
     ap.pair_order = s->d_order; ap.skip_finish = 0; ap.variant = s->prm.variant;
     ap.pairs_per_wg = s->prm.pairs_per_wg;
+    ap.groups_per_wg = s->prm.groups_per_wg;
     FssAlignOut out{};
     out.q = s->q; out.t = s->t; out.pose_score = s->pose_score; out.best_score = s->best_score;
     out.best_idx = s->best_idx; out.best_tf = s->best_tf; out.n_steps = s->n_steps;
@@ -929,6 +930,7 @@ int fss_screen_create(FssLibrary* lib, const FssScoreParams* sp, const FssScreen
   if (need_pharm && prm->max_query_pharm < 1) return set_error(FSS_E_INVALID, "max_query_pharm must be >= 1");
   if (!prm->quat_table) return set_error(FSS_E_INVALID, "quat_table is NULL");
   if (prm->pairs_per_wg < 0 || prm->pairs_per_wg > 64) return set_error(FSS_E_INVALID, "pairs_per_wg must be in [0, 64]");
+  if (prm->groups_per_wg < 0) return set_error(FSS_E_INVALID, "groups_per_wg must be >= 0");
   int64_t pool_rows[kQPools];
   if (int rc = fss_plan_query_pools(sp, prm, pool_rows)) return rc;
   DeviceGuard dg(lib->device);

@@ -366,6 +366,10 @@ class ScreenService:
     vol_and_surf_esp scores heavy atoms at alpha 0.81 and surface points otherwise.
     ``extended_points`` / ``only_extended`` apply to the directional pharmacophore channel
     (pharm, or vol_color with ``directionless=False``), as in screen.py:866-867.
+    Tiling (``lanes_per_pose``, ``pairs_per_wg``, ``variant``, ``groups_per_wg``: fss_align's
+    meanings) defaults by mode: atom clouds 2 lanes per restart and 32 / restarts pairs per
+    workgroup; surf / surf_esp the split-bf16 MFMA loop with two wavefronts per pair; ShaEP 32
+    lanes per restart.
     """
 
     def __init__(self, library: ConformerLibrary, mode: str = "vol", *, num_repeats: Optional[int] = None,
@@ -376,6 +380,7 @@ class ScreenService:
                  similarity: str = "tanimoto", directionless: bool = True,
                  extended_points: bool = False, only_extended: bool = False,
                  lanes_per_pose: Optional[int] = None, pairs_per_wg: Optional[int] = None,
+                 variant: Optional[int] = None, groups_per_wg: Optional[int] = None,
                  max_query_points: int = 4096,
                  max_query_pharm: int = 256, esp_weight: float = 0.5, probe_radius: float = 1.0,
                  max_query_surface: int = 4096, max_query_atoms: int = 1024, device=None):
@@ -438,11 +443,18 @@ class ScreenService:
             # of a pair in one wavefront with as many lanes each as fit (10 restarts: 6 lanes).
             if not combo and mode not in ("surf", "surf_esp") and 32 // self.B >= 2:
                 lanes_per_pose, pairs_per_wg = 2, 32 // self.B
+            elif mode in ("surf", "surf_esp") and variant is None and groups_per_wg is None:
+                # surfaces: the split-bf16 MFMA loop, a wavefront per pose, two wavefronts per
+                # pair taking the restarts in turn (1M-entry-sized sweep, 100k conformers of 200
+                # points + ESP, 10 x 100: 100k pairs/s on 6-lane VALU groups, 85k / 81k / 117k /
+                # 123k on the MFMA loop with 10 / 5 / 4 / 2 wavefronts)
+                lanes_per_pose, variant, groups_per_wg = 64, 3, min(2, self.B)
             else:
                 lanes_per_pose = 32 if combo else max(1, 64 // self.B)
         elif lanes_per_pose is None:
             lanes_per_pose = 32 if combo else max(1, 64 // (self.B * int(pairs_per_wg)))
         self.lanes_per_pose, self.pairs_per_wg = int(lanes_per_pose), int(pairs_per_wg or 1)
+        self.variant, self.groups_per_wg = int(variant or 0), int(groups_per_wg or 0)
         prm = _lib.FssScreenParams()
         prm.n_restarts, prm.max_steps = self.B, self.opt.max_steps
         prm.lr, prm.beta1, prm.beta2, prm.eps = self.opt.lr, self.opt.beta1, self.opt.beta2, self.opt.eps
@@ -453,8 +465,9 @@ class ScreenService:
         prm.max_query_surface, prm.max_query_atoms = mqs, mqa
         prm.shape_from_surface = int(self.shape_from_surface)
         prm.query_budget_bytes = self.query_budget_bytes
-        prm.lanes_per_pose, prm.restarts_per_wg, prm.variant = self.lanes_per_pose, self.B, 0
+        prm.lanes_per_pose, prm.restarts_per_wg, prm.variant = self.lanes_per_pose, self.B, self.variant
         prm.pairs_per_wg = self.pairs_per_wg
+        prm.groups_per_wg = self.groups_per_wg
         prm.seed_flags = (_lib.SEED_PHARM if self.spec.mode == _lib.MODE_PHARM else
                           _lib.SEED_SURF if combo else 0)
         prm.quat_table = self.table.ctypes.data

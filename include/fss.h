@@ -445,6 +445,9 @@ typedef struct FssScreenParams {
   int32_t pairs_per_wg;        /* fss_align pairs per workgroup (0 or 1: one); > 1 also orders
                                   each launch's pairs by their entry's row count, so the pairs
                                   sharing a workgroup have alike sizes */
+  int32_t groups_per_wg;       /* fss_align lane groups per workgroup (0: restarts_per_wg); each
+                                  runs its share of the pair's restarts in turn (the MFMA loop,
+                                  variant 3, for surface modes) */
 } FssScreenParams;
 
 typedef struct FssQuery {      /* host arrays, copied by fss_screen_submit */

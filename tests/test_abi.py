@@ -46,8 +46,8 @@ int main(void) {
          offsetof(FssQuery, rad), sizeof(FssLibRows), offsetof(FssLibRows, acounts));
   printf("%zu %zu %zu %zu\n", offsetof(FssQuery, all_scores), offsetof(FssHits, all_scores), offsetof(FssHits, n_all),
          offsetof(FssScreenParams, query_budget_bytes));
-  printf("%zu %zu %zu\n", offsetof(FssAlignParams, tile_rows), sizeof(FssAlignPlanInfo),
-         offsetof(FssAlignPlanInfo, lds_bytes));
+  printf("%zu %zu %zu %zu\n", offsetof(FssAlignParams, tile_rows), sizeof(FssAlignPlanInfo),
+         offsetof(FssAlignPlanInfo, lds_bytes), offsetof(FssScreenParams, groups_per_wg));
   return 0;
 }
 '''
@@ -72,7 +72,7 @@ int main(void) {
     assert sizes[21:25] == [_lib.FssQuery.all_scores.offset, _lib.FssHits.all_scores.offset,
                             _lib.FssHits.n_all.offset, _lib.FssScreenParams.query_budget_bytes.offset]
     assert sizes[25:] == [_lib.FssAlignParams.tile_rows.offset, ctypes.sizeof(_lib.FssAlignPlanInfo),
-                          _lib.FssAlignPlanInfo.lds_bytes.offset]
+                          _lib.FssAlignPlanInfo.lds_bytes.offset, _lib.FssScreenParams.groups_per_wg.offset]
 
 
 def test_validation_rejects_bad_arguments_without_launching():
