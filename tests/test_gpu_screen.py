@@ -48,6 +48,31 @@ def test_screen_scores_match_oracle(dev):
         assert abs(got[i] - o["score"]) < 1e-3, (i, got[i], o["score"])
 
 
+def test_surface_screen_scores_match_oracle(dev):
+    """surf_esp screening runs the split-bf16 MFMA loop (two wavefronts per pair) by default; its
+    per-entry scores agree with the FP64 oracle's alignment under the same seeds."""
+    from fast_shepherd_score_amd.constants import LAM_SCALING
+    print("case surf_esp screen vs oracle: 12 entries of 60-110 surface points, 6 x 50", flush=True)
+    rng = np.random.default_rng(9)
+    atoms = synth.random_walks(rng, rng.integers(12, 25, size=12))
+    surf = [synth.surface_points(rng, a, int(n)) for a, n in zip(atoms, rng.integers(60, 111, size=12))]
+    esp = [synth.esp_values(rng, len(x)) for x in surf]
+    lib = ConformerLibrary.from_arrays(points=surf, esp=esp, device=dev)
+    B, steps, lam = 6, 50, 0.3
+    with ScreenService(lib, "surf_esp", num_repeats=B, max_num_steps=steps, top_k=12, lam=lam, alpha=0.81) as svc:
+        assert svc.variant == 3 and svc.lanes_per_pose == 64 and svc.groups_per_wg == 2
+        hits = svc.submit({"points": surf[3], "esp": esp[3]}).result(timeout=120).hits
+    table = restart_table(B).astype(np.float64)
+    got = {h.id: h.score for h in hits}
+    assert len(got) == 12
+    for i in (0, 3, 7, 11):
+        ref, fit = surf[3].astype(np.float64), surf[i].astype(np.float64)
+        t0 = ref.mean(0)[None] - np.einsum("bij,j->bi", rotmat(table), fit.mean(0))
+        o = orc.optimize(orc.Pair("esp", ref=ref, fit=fit, ref_esp=esp[3], fit_esp=esp[i], lam=lam * LAM_SCALING),
+                         table, t0, max_steps=steps)
+        assert abs(got[i] - o["score"]) < 1e-3, (i, got[i], o["score"])
+
+
 def test_empty_library_returns_no_hits(dev):
     """A query against a library with no conformers finishes (no launch) with zero hits, alone
     and beside a query on the same service."""
