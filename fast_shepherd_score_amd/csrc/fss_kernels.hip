@@ -177,20 +177,29 @@ LdsPlan plan_lds(const FssPairBatch* pb, int mode, bool mfma = false) {
   return p;
 }
 
-// Shape modes whose clouds may be streamed through LDS tiles (clouds of any size).
-constexpr bool tileable(int mode) { return mode == FSS_MODE_SHAPE || mode == FSS_MODE_SHAPE_ESP; }
+// Modes whose shape clouds may be streamed through LDS tiles (clouds of any size); the
+// pharmacophore rows of the shape + pharmacophore mode stay staged whole.
+constexpr bool tileable(int mode) {
+  return mode == FSS_MODE_SHAPE || mode == FSS_MODE_SHAPE_ESP || mode == FSS_MODE_SHAPE_PHARM;
+}
 
 // LDS plan of the streamed shape path: one tile of tr ref rows (expanded form, plus the ESP
-// column) and one of tf fit rows, then the ref centroid slot.
-LdsPlan plan_lds_tiled(int mode, int tr, int tf) {
+// column) and one of tf fit rows, the ref centroid slot, then the pair's pharmacophore rows
+// (shape + pharmacophore mode) as plan_lds lays them out.
+LdsPlan plan_lds_tiled(const FssPairBatch* pb, int mode, int tr, int tf) {
   LdsPlan p{};
   int o = 0;
+  const bool ph = mode == FSS_MODE_SHAPE_PHARM;
   p.ref = o; o += tr;
   p.xq = o;
   p.fit = o; o += tf;
   p.refc = o; o += mode == FSS_MODE_SHAPE_ESP ? (tr + 3) / 4 : 0;
   p.red = o; o += 1;
-  p.poff = p.ranc = p.rvec = p.fanc = p.fvec = o;
+  p.poff = o; o += ph ? 4 : 0;
+  p.ranc = o; o += ph ? pb->max_ref_ph : 0;
+  p.rvec = o; o += ph ? pb->max_ref_ph : 0;
+  p.fanc = o; o += ph ? pb->max_fit_ph : 0;
+  p.fvec = o; o += ph ? pb->max_fit_ph : 0;
   p.srf_r = p.srf_f = p.atm_r = p.atm_f = p.rb_r = p.rb_f = o;
   p.end = o;
   p.tile_ref = tr;
@@ -221,6 +230,58 @@ struct Staged {
   float co[3];
   const float4* gref; const float4* gfit;   // streamed shape path: the pair's rows in HBM
 };
+
+// Pharmacophore rows of one pair into LDS (anchor with the type in w as integer bits, unit
+// vector): ref rows in type-sorted order with the per-type offsets, fit rows as given.  Caller
+// syncs afterwards.  ref_side_a / fit_side_b as in stage_pair.
+__device__ __forceinline__ void stage_pharm(const FssPairBatch& pb, LdsPlan lp, int pair, int ref_side_a,
+                                            int fit_side_b, float4* lds, Staged& st) {
+  const int tid = threadIdx.x, nt = blockDim.x;
+  const int* ra = ref_side_a ? pb.fit_prng : pb.ref_prng;
+  const int* rb = fit_side_b ? pb.fit_prng : pb.ref_prng;
+  const float4* aa = reinterpret_cast<const float4*>(ref_side_a ? pb.fit_anc : pb.ref_anc);
+  const float4* av = reinterpret_cast<const float4*>(ref_side_a ? pb.fit_vec : pb.ref_vec);
+  const int32_t* at = ref_side_a ? pb.fit_typ : pb.ref_typ;
+  const float4* ba = reinterpret_cast<const float4*>(fit_side_b ? pb.fit_anc : pb.ref_anc);
+  const float4* bv = reinterpret_cast<const float4*>(fit_side_b ? pb.fit_vec : pb.ref_vec);
+  const int32_t* bt = fit_side_b ? pb.fit_typ : pb.ref_typ;
+  const int a0 = ra[2 * pair], N = ra[2 * pair + 1];
+  const int b0 = rb[2 * pair], M = rb[2 * pair + 1];
+  // Ref rows go to their type-sorted position (a stable counting sort by direct ranking:
+  // the clouds hold a few dozen features); out-of-range types are filed as Dummy.
+  auto tclamp = [](int ty) { return (ty < 0 || ty >= FSS_N_PTYPES) ? FSS_N_PTYPES - 1 : ty; };
+  int* poff = reinterpret_cast<int*>(lds + lp.poff);
+  if (tid <= FSS_N_PTYPES) {
+    int c = 0;
+    for (int a = 0; a < N; ++a) c += tclamp(at[a0 + a]) < tid;
+    poff[tid] = c;
+  }
+  for (int i = tid; i < N + M; i += nt) {
+    const bool isref = i < N;
+    const int k = isref ? a0 + i : b0 + (i - N);
+    const float4 p = isref ? aa[k] : ba[k];
+    const float4 v = isref ? av[k] : bv[k];
+    const int ty = tclamp(isref ? at[k] : bt[k]);
+    float n = sqrtf(v.x * v.x + v.y * v.y + v.z * v.z);
+    n = n > 1e-12f ? n : 1e-12f;
+    const float4 pa4 = make_float4(p.x, p.y, p.z, __int_as_float(ty));
+    const float4 va4 = make_float4(v.x / n, v.y / n, v.z / n, 0.f);
+    if (isref) {
+      int pos = 0;
+      for (int a = 0; a < N; ++a) {
+        const int ta = tclamp(at[a0 + a]);
+        pos += (ta < ty) || (ta == ty && a < i);
+      }
+      lds[lp.ranc + pos] = pa4;
+      lds[lp.rvec + pos] = va4;
+    } else {
+      lds[lp.fanc + (i - N)] = pa4;
+      lds[lp.fvec + (i - N)] = va4;
+    }
+  }
+  st.ranc = lds + lp.ranc; st.rvec = lds + lp.rvec; st.fanc = lds + lp.fanc; st.fvec = lds + lp.fvec;
+  st.Np = N; st.Mp = M; st.poff = poff;
+}
 
 // Ref centroid of rows ga[0..N): lane-strided sums in wavefront 0 and one butterfly, whatever the
 // workgroup size, so that every tiling and every kernel staging this pair gets the same bits.
@@ -265,8 +326,9 @@ __device__ __forceinline__ void stage_fit_rows(const float4* __restrict__ gb, in
   }
 }
 
-// Streamed shape path: the pair's centroid and row pointers; the tiles are staged per pass.
-template <bool ESP>
+// Streamed shape path: the pair's centroid and row pointers (the tiles are staged per pass),
+// and its pharmacophore rows staged whole (PH).  Caller syncs afterwards.
+template <bool ESP, bool PH>
 __device__ __forceinline__ Staged stage_tiled(const FssPairBatch& pb, const KScore& ks, LdsPlan lp, int pair,
                                               float4* lds) {
   Staged st{};
@@ -282,6 +344,7 @@ __device__ __forceinline__ Staged stage_tiled(const FssPairBatch& pb, const KSco
   st.N = N; st.M = M;
   st.cs[0] = ks.s * c[0]; st.cs[1] = ks.s * c[1]; st.cs[2] = ks.s * c[2];
   st.co[0] = c[0]; st.co[1] = c[1]; st.co[2] = c[2];
+  if (PH) stage_pharm(pb, lp, pair, 0, 1, lds, st);
   return st;
 }
 
@@ -416,52 +479,7 @@ __device__ __forceinline__ Staged stage_pair(const FssPairBatch& pb, const KScor
     st.srf_r = lsr; st.srf_f = lsf; st.atm_r = lar; st.atm_f = laf; st.rb_r = lbr; st.rb_f = lbf;
     st.Ns = Ns; st.Ms = Ms; st.Na = Na; st.Ma = Ma;
   }
-  if (PH) {
-    const int* ra = ref_side_a ? pb.fit_prng : pb.ref_prng;
-    const int* rb = fit_side_b ? pb.fit_prng : pb.ref_prng;
-    const float4* aa = reinterpret_cast<const float4*>(ref_side_a ? pb.fit_anc : pb.ref_anc);
-    const float4* av = reinterpret_cast<const float4*>(ref_side_a ? pb.fit_vec : pb.ref_vec);
-    const int32_t* at = ref_side_a ? pb.fit_typ : pb.ref_typ;
-    const float4* ba = reinterpret_cast<const float4*>(fit_side_b ? pb.fit_anc : pb.ref_anc);
-    const float4* bv = reinterpret_cast<const float4*>(fit_side_b ? pb.fit_vec : pb.ref_vec);
-    const int32_t* bt = fit_side_b ? pb.fit_typ : pb.ref_typ;
-    const int a0 = ra[2 * pair], N = ra[2 * pair + 1];
-    const int b0 = rb[2 * pair], M = rb[2 * pair + 1];
-    // Ref rows go to their type-sorted position (a stable counting sort by direct ranking:
-    // the clouds hold a few dozen features); out-of-range types are filed as Dummy.
-    auto tclamp = [](int ty) { return (ty < 0 || ty >= FSS_N_PTYPES) ? FSS_N_PTYPES - 1 : ty; };
-    int* poff = reinterpret_cast<int*>(lds + lp.poff);
-    if (tid <= FSS_N_PTYPES) {
-      int c = 0;
-      for (int a = 0; a < N; ++a) c += tclamp(at[a0 + a]) < tid;
-      poff[tid] = c;
-    }
-    for (int i = tid; i < N + M; i += nt) {
-      const bool isref = i < N;
-      const int k = isref ? a0 + i : b0 + (i - N);
-      const float4 p = isref ? aa[k] : ba[k];
-      const float4 v = isref ? av[k] : bv[k];
-      const int ty = tclamp(isref ? at[k] : bt[k]);
-      float n = sqrtf(v.x * v.x + v.y * v.y + v.z * v.z);
-      n = n > 1e-12f ? n : 1e-12f;
-      const float4 pa4 = make_float4(p.x, p.y, p.z, __int_as_float(ty));
-      const float4 va4 = make_float4(v.x / n, v.y / n, v.z / n, 0.f);
-      if (isref) {
-        int pos = 0;
-        for (int a = 0; a < N; ++a) {
-          const int ta = tclamp(at[a0 + a]);
-          pos += (ta < ty) || (ta == ty && a < i);
-        }
-        lds[lp.ranc + pos] = pa4;
-        lds[lp.rvec + pos] = va4;
-      } else {
-        lds[lp.fanc + (i - N)] = pa4;
-        lds[lp.fvec + (i - N)] = va4;
-      }
-    }
-    st.ranc = lds + lp.ranc; st.rvec = lds + lp.rvec; st.fanc = lds + lp.fanc; st.fvec = lds + lp.fvec;
-    st.Np = N; st.Mp = M; st.poff = poff;
-  }
+  if (PH) stage_pharm(pb, lp, pair, ref_side_a, fit_side_b, lds, st);
   return st;
 }
 
@@ -614,7 +632,7 @@ __global__ __launch_bounds__(kPoseBlock) void k_pose_grad(FssPairBatch pb, KScor
   bool tl = false;
   if constexpr (tileable(MODE)) tl = lp.tile_ref > 0;
   TileState tiles;
-  const Staged st = tl ? stage_tiled<ESP>(pb, ks, lp, pair, lds)
+  const Staged st = tl ? stage_tiled<ESP, PH>(pb, ks, lp, pair, lds)
                        : stage_pair<SH, ESP, PH, true, false, SE>(pb, ks, lp, pair, 0, 1, lds);
   float q[4] = {qin[4 * pose], qin[4 * pose + 1], qin[4 * pose + 2], qin[4 * pose + 3]};
   const float t[3] = {tin[3 * pose], tin[3 * pose + 1], tin[3 * pose + 2]};
@@ -789,7 +807,7 @@ void k_align(FssPairBatch pb, KScore ks, LdsPlan lp, AlignK ak) {
   constexpr bool SH = MODE != FSS_MODE_PHARM;
   constexpr bool ESP = MODE == FSS_MODE_SHAPE_ESP;
   constexpr bool PH = MODE == FSS_MODE_PHARM || MODE == FSS_MODE_SHAPE_PHARM;
-  static_assert(!TL || (tileable(MODE) && !MP), "the streamed path runs the shape modes, one pair per workgroup");
+  static_assert(!TL || (tileable(MODE) && !MP), "the streamed path runs the tileable modes, one pair per workgroup");
   extern __shared__ float4 lds[];
   const int lpp = ak.lpp, groups = ak.groups, ppw = MP ? ak.ppw : 1;
   // ppw pairs per workgroup (each with every restart in the workgroup): lane group g works on
@@ -808,7 +826,7 @@ void k_align(FssPairBatch pb, KScore ks, LdsPlan lp, AlignK ak) {
   Staged st{};
   TileState tiles;
   if (TL) {
-    st = stage_tiled<ESP>(pb, ks, lp, pair, lds);
+    st = stage_tiled<ESP, PH>(pb, ks, lp, pair, lds);
   } else if (ppw == 1) {
     st = stage_pair<SH, ESP, PH, EXPD, MF, SE>(pb, ks, lp, pair, 0, 1, lds);
   } else {
@@ -914,8 +932,10 @@ void k_align(FssPairBatch pb, KScore ks, LdsPlan lp, AlignK ak) {
         shape_finish(sh, ks.K, ks.alpha, ks.s, true);
       }
       if (PH) {
-        pharm_accum<true>(st.ranc, st.rvec, st.poff, st.fanc, st.fvec, st.Mp, j, lpp, R, p + 4, ks.tab, ph);
-        pose_sum(ph, lpp, j, true, grp_all, nl, red);
+        // streamed path, lane groups of several wavefronts: every lane meets the sums' barriers
+        if (!TL || work) pharm_accum<true>(st.ranc, st.rvec, st.poff, st.fanc, st.fvec, st.Mp, j, lpp, R, p + 4, ks.tab, ph);
+        if (TL && ak.xwave) pose_sum_waves(ph, lpp, red);
+        else pose_sum(ph, lpp, j, true, grp_all, nl, red);
       }
       if (SE) {
         const float tr[3] = {p[4] - st.co[0], p[5] - st.co[1], p[6] - st.co[2]};
@@ -1878,12 +1898,12 @@ int plan_align(const FssPairBatch* batch, const KScore& ks, const FssAlignParams
                                         "the expanded form only (variant 0, 2 or 4)", batch->max_ref, batch->max_fit);
     const int t0 = ap->tile_rows > 0 ? ap->tile_rows : 1024;
     int tr = std::max(1, std::min(batch->max_ref, t0)), tf = std::max(1, std::min(batch->max_fit, t0));
-    while ((size_t)plan_lds_tiled(ks.mode, tr, tf).end * sizeof(float4) + tail > cap && std::max(tr, tf) > 16) {
+    while ((size_t)plan_lds_tiled(batch, ks.mode, tr, tf).end * sizeof(float4) + tail > cap && std::max(tr, tf) > 16) {
       if (tr >= tf) tr = (tr + 1) / 2;
       else tf = (tf + 1) / 2;
     }
     var = 4;
-    pl.lp = plan_lds_tiled(ks.mode, tr, tf);
+    pl.lp = plan_lds_tiled(batch, ks.mode, tr, tf);
     region = pl.lp.end;
     pl.lds = (size_t)region * sizeof(float4) + tail;
   }
@@ -2041,7 +2061,7 @@ static int plan_pose_lds(const FssPairBatch* batch, int mode, LdsPlan& lp) {
   if (need <= cap && (!tileable(mode) || (kCuLdsBytes / std::max<size_t>(need, 1)) * (kPoseBlock / 64) >= 8))
     return FSS_OK;
   if (tileable(mode)) {
-    lp = plan_lds_tiled(mode, std::max(1, std::min(batch->max_ref, 1024)), std::max(1, std::min(batch->max_fit, 1024)));
+    lp = plan_lds_tiled(batch, mode, std::max(1, std::min(batch->max_ref, 1024)), std::max(1, std::min(batch->max_fit, 1024)));
     if ((size_t)lp.end * sizeof(float4) <= cap) return FSS_OK;
   }
   if (need <= cap) {

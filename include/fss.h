@@ -126,9 +126,10 @@ typedef struct FssAlignParams {
                                 3 split-bf16 MFMA (lanes_per_pose 64), 4 expanded form over
                                 clouds streamed through LDS tiles.  Clouds that do not fit LDS
                                 in the requested layout run the expanded form (3 -> 2), and in
-                                the shape modes (FSS_MODE_SHAPE, FSS_MODE_SHAPE_ESP) the
-                                streamed form (0, 2, 3 -> 4; pairs_per_wg 1); other modes then
-                                fail with FSS_E_TOO_LARGE                                      */
+                                FSS_MODE_SHAPE, FSS_MODE_SHAPE_ESP and FSS_MODE_SHAPE_PHARM
+                                (whose pharmacophore rows stay staged whole) the streamed form
+                                (0, 2, 3 -> 4; pairs_per_wg 1); the other modes then fail with
+                                FSS_E_TOO_LARGE                                                */
   int32_t groups_per_wg;     /* lane groups per workgroup (0: restarts_per_wg); each group runs
                                 ceil(restarts_per_wg / groups_per_wg) restarts in turn every
                                 step, so per-pair early stop can keep all of a pair's restarts in

@@ -2,7 +2,8 @@
 FP64 oracle (needs an MI355X).
 
   * Forced on small batches (variant 4, 48-row tiles, so both sides span several tiles with
-    ragged last tiles): best restarts within 1e-3 of the oracle's from the same seeds, FP64
+    ragged last tiles; surface + ESP, atoms, and atoms + directional pharmacophores, whose
+    pharmacophore rows stay staged whole): best restarts within 1e-3 of the oracle's from the same seeds, FP64
     re-scoring of the final poses within 1e-4 of the oracle's scoring, and close to the staged
     expanded form on the same batch.  Tilings with lanes that hold no restart (a partial last
     restart block), lane groups that run two or three restarts each, and lane groups of two and
@@ -32,6 +33,12 @@ def small_case(kind, n=3, B=6, seed=31):
     if kind == "esp":
         r, re, f, fe = synth.surf_esp_pairs(n, atoms=(15, 40), points=(150, 260), seed=seed)
         d = dict(kind="esp", ref=r, fit=f, ref_w=re, fit_w=fe, lam=0.3 * LAM_SCALING)
+    elif kind == "shape_pharm":
+        ps = synth.pharm_shape_pairs(n, atoms=(40, 120), pharms=(4, 14), seed=seed)
+        r, f = ps["ref_atoms"], ps["fit_atoms"]
+        d = dict(kind="shape_pharm", ref=r, fit=f, ref_types=ps["ref_types"], fit_types=ps["fit_types"],
+                 ref_anc=ps["ref_anchors"], fit_anc=ps["fit_anchors"], ref_vec=ps["ref_vectors"],
+                 fit_vec=ps["fit_vectors"], directionless=False)
     else:
         r, f = synth.vol_pairs(n, 40, 120, seed=seed)
         d = dict(kind="shape", ref=r, fit=f)
@@ -44,7 +51,7 @@ def small_case(kind, n=3, B=6, seed=31):
     return d
 
 
-@pytest.mark.parametrize("kind", ["esp", "shape"])
+@pytest.mark.parametrize("kind", ["esp", "shape", "shape_pharm"])
 @pytest.mark.parametrize("early_stop", [False, True])
 def test_forced_streaming_matches_oracle_and_staged(dev, kind, early_stop):
     d = small_case(kind)

@@ -107,6 +107,14 @@ def test_tiles_shrink_to_fit_a_large_restart_tail():
     assert info.variant == 4 and info.lds_bytes <= LDS and max(info.tile_ref, info.tile_fit) < 1024
 
 
+def test_shape_pharm_streams_its_shape_clouds():
+    rc, info, err = plan(_lib.MODE_SHAPE_PHARM, 20000, 3000)
+    assert rc == 0 and info.variant == 4 and info.tile_ref == 1024, err
+    # the pharmacophore rows (8 per side here) stay staged next to the tiles
+    rc2, info2, _ = plan(_lib.MODE_SHAPE, 20000, 3000)
+    assert info.lds_bytes - info2.lds_bytes == 16 * (4 + 4 * 8)
+
+
 def test_refusals():
     rc, _, err = plan(_lib.MODE_PHARM, 40000, 40000)
     assert rc == _lib.E_TOO_LARGE and b"LDS" in err
