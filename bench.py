@@ -38,7 +38,9 @@ def measured_row(key: str) -> dict:
             return json.load(f).get(key) or {}
     except (OSError, ValueError):
         return {}
-PEAK_EXP_PER_S = 256 * 4 * 16 * 2.4e9 / 4   # v_exp_f32 at quarter rate: 9.8e12 / s
+# v_exp_f32: one wave64 instruction per 8 cycles per SIMD (measured on the box: 8.2-8.4 cycles,
+# not overlapped with FMAs), 1024 SIMDs at 2.4 GHz: 1.97e13 exp / s
+PEAK_EXP_PER_S = 256 * 4 * 64 / 8 * 2.4e9
 
 
 def log(*a):
@@ -382,7 +384,7 @@ def bench_screen_shaep(dev, n_lib: int = 50_000, pool: int = 400, n_queries: int
             "p50_ms": float(np.percentile(lat, 50)), "max_ms": float(lat.max())}
 
 
-def bench_score_loss(dev, n_pairs: int = 1024, n_points: int = 300, reps: int = 20, seed: int = 4):
+def bench_score_loss(dev, n_pairs: int = 1024, n_points: int = 300, reps: int = 31, seed: int = 4):
     """Score as a loss: FP32 forward S and backward dS/dfit (incl. dVBB) of 1024 surface+ESP pairs."""
     import torch
     from fast_shepherd_score_amd import synth
@@ -400,15 +402,19 @@ def bench_score_loss(dev, n_pairs: int = 1024, n_points: int = 300, reps: int = 
     lam = 0.3 * LAM_SCALING
 
     def timed(fn):
-        fn()
-        torch.cuda.synchronize()
-        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-        a.record()
-        for _ in range(reps):
+        # each call between its own events, median over the calls: a rare host stall of tens of
+        # ms (seen in the first backward of a process and now and then after) moves a mean of
+        # 20 calls of ~0.2 ms tenfold, the median not at all
+        for _ in range(3):
             fn()
-        b.record()
         torch.cuda.synchronize()
-        return a.elapsed_time(b) / reps
+        ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(reps)]
+        for a, b in ev:
+            a.record()
+            fn()
+            b.record()
+        torch.cuda.synchronize()
+        return float(np.median([a.elapsed_time(b) for a, b in ev]))
 
     def fwd():
         with torch.no_grad():

@@ -1184,6 +1184,7 @@ template <> __device__ __forceinline__ float neg_exp<float>(float r2) { return f
 template <> __device__ __forceinline__ double neg_exp<double>(double r2) { return exp(-r2); }
 template <typename T> constexpr T score_base() { return sizeof(T) == 4 ? (T)kLog2eD : (T)1; }
 
+constexpr int kScoreBlock = 512;                 // threads of the score-as-loss kernels (8 wavefronts)
 constexpr int kScoreSlices = 4;                  // backward: lanes sharing one row
 constexpr int kScoreRows = 64 / kScoreSlices;    // backward: rows per wavefront unit
 static_assert(kScoreSlices == 4, "k_score_bwd combines the slices with two shuffles");
@@ -1212,43 +1213,48 @@ __device__ __forceinline__ void stage_pair_clouds(const T* __restrict__ A, int N
   __syncthreads();
 }
 
-// Sum over the columns [0, nc) of base^(-|q_c - p|^2); every lane reads the same column (LDS broadcast).
+// base^(-|q - p|^2) of one (column, row) point pair.
+template <typename T, bool ESP>
+__device__ __forceinline__ T pair_exp(const typename Vec4<T>::type& q, const typename Vec4<T>::type& p) {
+  const T dx = q.x - p.x, dy = q.y - p.y, dz = q.z - p.z;
+  T r2 = dx * dx + dy * dy + dz * dz;
+  if (ESP) { const T dw = q.w - p.w; r2 += dw * dw; }
+  return neg_exp<T>(r2);
+}
+
+// Sum over the columns [0, nc) of base^(-|q_c - p|^2); every lane reads the same column (LDS
+// broadcast).  Four running sums take the columns c % 4 (four loads in flight per step), added
+// in a fixed order at the end.
 template <typename T, bool ESP>
 __device__ __forceinline__ T tile_sum(const typename Vec4<T>::type* __restrict__ cols, int nc,
                                       typename Vec4<T>::type p) {
-  T e0 = 0, e1 = 0;
+  using V = typename Vec4<T>::type;
+  T e[4] = {0, 0, 0, 0};
   int c = 0;
-  for (; c + 1 < nc; c += 2) {
+  for (; c + 3 < nc; c += 4) {
+    V q[4];
 #pragma unroll
-    for (int k = 0; k < 2; ++k) {
-      const typename Vec4<T>::type q = cols[c + k];
-      const T dx = q.x - p.x, dy = q.y - p.y, dz = q.z - p.z;
-      T r2 = dx * dx + dy * dy + dz * dz;
-      if (ESP) { const T dw = q.w - p.w; r2 += dw * dw; }
-      (k ? e1 : e0) += neg_exp<T>(r2);
-    }
+    for (int k = 0; k < 4; ++k) q[k] = cols[c + k];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) e[k] += pair_exp<T, ESP>(q[k], p);
   }
-  if (c < nc) {
-    const typename Vec4<T>::type q = cols[c];
-    const T dx = q.x - p.x, dy = q.y - p.y, dz = q.z - p.z;
-    T r2 = dx * dx + dy * dy + dz * dz;
-    if (ESP) { const T dw = q.w - p.w; r2 += dw * dw; }
-    e0 += neg_exp<T>(r2);
-  }
-  return e0 + e1;
+#pragma unroll
+  for (int k = 0; k < 3; ++k)
+    if (c + k < nc) e[k] += pair_exp<T, ESP>(cols[c + k], p);
+  return (e[0] + e[1]) + (e[2] + e[3]);
 }
 
 // Forward: VAB, VAA and VBB as 64 x 64 tiles (lane = row, columns broadcast from LDS), VAA and VBB
 // over the upper block triangle only (off-diagonal tiles count twice).  The tiles are dealt to the
 // wavefronts round-robin, so a pair's whole product is balanced over its workgroup.
 template <typename T, bool ESP>
-__global__ __launch_bounds__(kPoseBlock) void k_score_fwd(const T* __restrict__ ref, const T* __restrict__ fit,
+__global__ __launch_bounds__(kScoreBlock) void k_score_fwd(const T* __restrict__ ref, const T* __restrict__ fit,
                                                           const int32_t* __restrict__ rr,
                                                           const int32_t* __restrict__ fr, T alpha, T lam,
                                                           T* S, T* ov) {
   using V = typename Vec4<T>::type;
   extern __shared__ double4 score_smem[];
-  __shared__ T red[3 * (kPoseBlock / 64)];
+  __shared__ T red[3 * (kScoreBlock / 64)];
   const int pair = blockIdx.x;
   const int N = rr[2 * pair + 1], M = fr[2 * pair + 1];
   V* la = reinterpret_cast<V*>(score_smem);
@@ -1291,20 +1297,40 @@ __global__ __launch_bounds__(kPoseBlock) void k_score_fwd(const T* __restrict__ 
   }
 }
 
-// f += sum over the columns first, first + step, ... < n of base^(-|q - p|^2) (q - p)
+// Sum over the columns first, first + kScoreSlices, ... < n of base^(-|q - p|^2) (q - p) into two
+// running sums, a over the slice's even-numbered columns and b over its odd-numbered ones (two
+// loads in flight); the caller adds a + b after its last call.  A block of 64 columns holds an
+// even number of each slice's columns, so calls over consecutive 64-column blocks add the same
+// terms to the same sums in the same order as one call over the whole range.
 template <typename T, bool ESP>
 __device__ __forceinline__ void slice_force(const typename Vec4<T>::type* __restrict__ cols, int n, int first,
-                                            typename Vec4<T>::type p, T f[3]) {
-#pragma unroll 4
-  for (int c = first; c < n; c += kScoreSlices) {
-    const typename Vec4<T>::type q = cols[c];
+                                            typename Vec4<T>::type p, T a[3], T b[3]) {
+  using V = typename Vec4<T>::type;
+  int c = first;
+  for (; c + kScoreSlices < n; c += 2 * kScoreSlices) {
+    const V q0 = cols[c], q1 = cols[c + kScoreSlices];
+    const T dx0 = q0.x - p.x, dy0 = q0.y - p.y, dz0 = q0.z - p.z;
+    const T dx1 = q1.x - p.x, dy1 = q1.y - p.y, dz1 = q1.z - p.z;
+    T r0 = dx0 * dx0 + dy0 * dy0 + dz0 * dz0, r1 = dx1 * dx1 + dy1 * dy1 + dz1 * dz1;
+    if (ESP) {
+      const T dw0 = q0.w - p.w, dw1 = q1.w - p.w;
+      r0 += dw0 * dw0;
+      r1 += dw1 * dw1;
+    }
+    const T e0 = neg_exp<T>(r0), e1 = neg_exp<T>(r1);
+    a[0] += e0 * dx0; a[1] += e0 * dy0; a[2] += e0 * dz0;
+    b[0] += e1 * dx1; b[1] += e1 * dy1; b[2] += e1 * dz1;
+  }
+  if (c < n) {
+    const V q = cols[c];
     const T dx = q.x - p.x, dy = q.y - p.y, dz = q.z - p.z;
     T r2 = dx * dx + dy * dy + dz * dz;
     if (ESP) { const T dw = q.w - p.w; r2 += dw * dw; }
     const T e = neg_exp<T>(r2);
-    f[0] += e * dx; f[1] += e * dy; f[2] += e * dz;
+    a[0] += e * dx; a[1] += e * dy; a[2] += e * dz;
   }
 }
+static_assert(64 % (2 * kScoreSlices) == 0, "a 64-column block must hold an even count of each slice's columns");
 
 // Backward.  With U = VAA + VBB and D = U - VAB, for a fit point x_b
 //   dS/dx_b = K alpha / D^2 (U sum_a w E (a - x_b) - 2 VAB sum_b' w E (x_b' - x_b))
@@ -1313,7 +1339,7 @@ __device__ __forceinline__ void slice_force(const typename Vec4<T>::type* __rest
 // kScoreSlices lanes of a row split its columns and are combined with a fixed shuffle order, so
 // the result is deterministic.
 template <typename T, bool ESP>
-__global__ __launch_bounds__(kPoseBlock) void k_score_bwd(const T* __restrict__ ref, const T* __restrict__ fit,
+__global__ __launch_bounds__(kScoreBlock) void k_score_bwd(const T* __restrict__ ref, const T* __restrict__ fit,
                                                           const int32_t* __restrict__ rr,
                                                           const int32_t* __restrict__ fr, T alpha, T lam,
                                                           const T* __restrict__ ov, const T* __restrict__ gS,
@@ -1339,9 +1365,15 @@ __global__ __launch_bounds__(kPoseBlock) void k_score_bwd(const T* __restrict__ 
     const int n_own = fit_row ? M : N, n_oth = fit_row ? N : M;
     const int row = (fit_row ? u : u - nbf) * kScoreRows + rl;
     const V p = own[row < n_own ? row : n_own - 1];
-    T fo[3] = {0, 0, 0}, fs[3] = {0, 0, 0};
-    slice_force<T, ESP>(oth, n_oth, sl, p, fo);
-    slice_force<T, ESP>(own, n_own, sl, p, fs);
+    T oa[3] = {0, 0, 0}, ob[3] = {0, 0, 0}, sa[3] = {0, 0, 0}, sb[3] = {0, 0, 0};
+    slice_force<T, ESP>(oth, n_oth, sl, p, oa, ob);
+    slice_force<T, ESP>(own, n_own, sl, p, sa, sb);
+    T fo[3], fs[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      fo[k] = oa[k] + ob[k];
+      fs[k] = sa[k] + sb[k];
+    }
     T d[3];
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
@@ -1397,13 +1429,13 @@ __device__ __forceinline__ void score_frame(const T* __restrict__ A, int N, cons
 }
 
 template <typename T, bool ESP>
-__global__ __launch_bounds__(kPoseBlock) void k_score_fwd_stream(const T* __restrict__ ref, const T* __restrict__ fit,
+__global__ __launch_bounds__(kScoreBlock) void k_score_fwd_stream(const T* __restrict__ ref, const T* __restrict__ fit,
                                                                  const int32_t* __restrict__ rr,
                                                                  const int32_t* __restrict__ fr, T alpha, T lam,
                                                                  T* S, T* ov) {
   using V = typename Vec4<T>::type;
-  __shared__ V wbufs[kPoseBlock];
-  __shared__ T red[3 * (kPoseBlock / 64)];
+  __shared__ V wbufs[kScoreBlock];
+  __shared__ T red[3 * (kScoreBlock / 64)];
   const int pair = blockIdx.x;
   const int N = rr[2 * pair + 1], M = fr[2 * pair + 1];
   const T* A = ref + 4 * (long)rr[2 * pair];
@@ -1450,13 +1482,13 @@ __global__ __launch_bounds__(kPoseBlock) void k_score_fwd_stream(const T* __rest
 }
 
 template <typename T, bool ESP>
-__global__ __launch_bounds__(kPoseBlock) void k_score_bwd_stream(const T* __restrict__ ref, const T* __restrict__ fit,
+__global__ __launch_bounds__(kScoreBlock) void k_score_bwd_stream(const T* __restrict__ ref, const T* __restrict__ fit,
                                                                  const int32_t* __restrict__ rr,
                                                                  const int32_t* __restrict__ fr, T alpha, T lam,
                                                                  const T* __restrict__ ov, const T* __restrict__ gS,
                                                                  T* d_fit, T* d_ref) {
   using V = typename Vec4<T>::type;
-  __shared__ V wbufs[kPoseBlock];
+  __shared__ V wbufs[kScoreBlock];
   const int pair = blockIdx.x;
   const int N = rr[2 * pair + 1], M = fr[2 * pair + 1];
   const T* A = ref + 4 * (long)rr[2 * pair];
@@ -1477,16 +1509,22 @@ __global__ __launch_bounds__(kPoseBlock) void k_score_bwd_stream(const T* __rest
     const int n_own = fit_row ? M : N, n_oth = fit_row ? N : M;
     const int row = (fit_row ? u : u - nbf) * kScoreRows + rl;
     const V p = score_point<T, ESP>(own, row < n_own ? row : n_own - 1, c, s, sw);
-    T fo[3] = {0, 0, 0}, fs[3] = {0, 0, 0};
+    T oa[3] = {0, 0, 0}, ob[3] = {0, 0, 0}, sa[3] = {0, 0, 0}, sb[3] = {0, 0, 0};
     for (int c0 = 0; c0 < n_oth; c0 += 64) {
       const int n = min(64, n_oth - c0);
       stage_block<T, ESP>(oth, c0, n, c, s, sw, wbuf);
-      slice_force<T, ESP>(wbuf, n, sl, p, fo);
+      slice_force<T, ESP>(wbuf, n, sl, p, oa, ob);
     }
     for (int c0 = 0; c0 < n_own; c0 += 64) {
       const int n = min(64, n_own - c0);
       stage_block<T, ESP>(own, c0, n, c, s, sw, wbuf);
-      slice_force<T, ESP>(wbuf, n, sl, p, fs);
+      slice_force<T, ESP>(wbuf, n, sl, p, sa, sb);
+    }
+    T fo[3], fs[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      fo[k] = oa[k] + ob[k];
+      fs[k] = sa[k] + sb[k];
     }
     T d[3];
 #pragma unroll
@@ -1562,12 +1600,13 @@ int score_check(const FssPairBatch* batch, const FssScoreParams* sp) {
   return FSS_OK;
 }
 
-// Both clouds of a pair are staged in LDS (4 values per point) when they fit; lds = 0 selects the
-// streamed kernels (a 64-point block per wavefront).
+// Both clouds of a pair are staged in LDS (4 values per point) when they fit beside the forward's
+// static reduction scratch (3 sums per wavefront); lds = 0 selects the streamed kernels (a
+// 64-point block per wavefront).
 int score_lds(const FssPairBatch* batch, size_t elem, size_t& lds) {
   if (batch->max_ref < 0 || batch->max_fit < 0) return set_error(FSS_E_INVALID, "negative max_ref / max_fit");
   lds = ((size_t)batch->max_ref + (size_t)batch->max_fit) * 4 * elem;
-  if (lds > max_lds_bytes()) lds = 0;
+  if (lds + 3 * (kScoreBlock / 64) * elem > max_lds_bytes()) lds = 0;
   return FSS_OK;
 }
 
@@ -1583,13 +1622,13 @@ int score_fwd_t(const FssPairBatch* batch, const FssScoreParams* sp, const T* re
   if ((rc = score_lds(batch, sizeof(T), lds))) return rc;
   const bool esp = sp->mode == FSS_MODE_SHAPE_ESP;
   if (!lds && esp)
-    hipLaunchKernelGGL((k_score_fwd_stream<T, true>), dim3(batch->n_pairs), dim3(kPoseBlock), 0, s, ref, fit, batch->ref_rng, batch->fit_rng, (T)sp->alpha, lam, S, ov);
+    hipLaunchKernelGGL((k_score_fwd_stream<T, true>), dim3(batch->n_pairs), dim3(kScoreBlock), 0, s, ref, fit, batch->ref_rng, batch->fit_rng, (T)sp->alpha, lam, S, ov);
   else if (!lds)
-    hipLaunchKernelGGL((k_score_fwd_stream<T, false>), dim3(batch->n_pairs), dim3(kPoseBlock), 0, s, ref, fit, batch->ref_rng, batch->fit_rng, (T)sp->alpha, lam, S, ov);
+    hipLaunchKernelGGL((k_score_fwd_stream<T, false>), dim3(batch->n_pairs), dim3(kScoreBlock), 0, s, ref, fit, batch->ref_rng, batch->fit_rng, (T)sp->alpha, lam, S, ov);
   else if (esp)
-    hipLaunchKernelGGL((k_score_fwd<T, true>), dim3(batch->n_pairs), dim3(kPoseBlock), lds, s, ref, fit, batch->ref_rng, batch->fit_rng, (T)sp->alpha, lam, S, ov);
+    hipLaunchKernelGGL((k_score_fwd<T, true>), dim3(batch->n_pairs), dim3(kScoreBlock), lds, s, ref, fit, batch->ref_rng, batch->fit_rng, (T)sp->alpha, lam, S, ov);
   else
-    hipLaunchKernelGGL((k_score_fwd<T, false>), dim3(batch->n_pairs), dim3(kPoseBlock), lds, s, ref, fit, batch->ref_rng, batch->fit_rng, (T)sp->alpha, lam, S, ov);
+    hipLaunchKernelGGL((k_score_fwd<T, false>), dim3(batch->n_pairs), dim3(kScoreBlock), lds, s, ref, fit, batch->ref_rng, batch->fit_rng, (T)sp->alpha, lam, S, ov);
   return check_launch("fss_score_fwd");
 }
 
@@ -1605,13 +1644,13 @@ int score_bwd_t(const FssPairBatch* batch, const FssScoreParams* sp, const T* re
   if ((rc = score_lds(batch, sizeof(T), lds))) return rc;
   const bool esp = sp->mode == FSS_MODE_SHAPE_ESP;
   if (!lds && esp)
-    hipLaunchKernelGGL((k_score_bwd_stream<T, true>), dim3(batch->n_pairs), dim3(kPoseBlock), 0, s, ref, fit, batch->ref_rng, batch->fit_rng, (T)sp->alpha, lam, ov, gS, d_fit, d_ref);
+    hipLaunchKernelGGL((k_score_bwd_stream<T, true>), dim3(batch->n_pairs), dim3(kScoreBlock), 0, s, ref, fit, batch->ref_rng, batch->fit_rng, (T)sp->alpha, lam, ov, gS, d_fit, d_ref);
   else if (!lds)
-    hipLaunchKernelGGL((k_score_bwd_stream<T, false>), dim3(batch->n_pairs), dim3(kPoseBlock), 0, s, ref, fit, batch->ref_rng, batch->fit_rng, (T)sp->alpha, lam, ov, gS, d_fit, d_ref);
+    hipLaunchKernelGGL((k_score_bwd_stream<T, false>), dim3(batch->n_pairs), dim3(kScoreBlock), 0, s, ref, fit, batch->ref_rng, batch->fit_rng, (T)sp->alpha, lam, ov, gS, d_fit, d_ref);
   else if (esp)
-    hipLaunchKernelGGL((k_score_bwd<T, true>), dim3(batch->n_pairs), dim3(kPoseBlock), lds, s, ref, fit, batch->ref_rng, batch->fit_rng, (T)sp->alpha, lam, ov, gS, d_fit, d_ref);
+    hipLaunchKernelGGL((k_score_bwd<T, true>), dim3(batch->n_pairs), dim3(kScoreBlock), lds, s, ref, fit, batch->ref_rng, batch->fit_rng, (T)sp->alpha, lam, ov, gS, d_fit, d_ref);
   else
-    hipLaunchKernelGGL((k_score_bwd<T, false>), dim3(batch->n_pairs), dim3(kPoseBlock), lds, s, ref, fit, batch->ref_rng, batch->fit_rng, (T)sp->alpha, lam, ov, gS, d_fit, d_ref);
+    hipLaunchKernelGGL((k_score_bwd<T, false>), dim3(batch->n_pairs), dim3(kScoreBlock), lds, s, ref, fit, batch->ref_rng, batch->fit_rng, (T)sp->alpha, lam, ov, gS, d_fit, d_ref);
   return check_launch("fss_score_bwd");
 }
 

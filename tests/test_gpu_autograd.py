@@ -110,7 +110,7 @@ def test_ragged_tile_edges_and_determinism(dev, N, M):
 
 @pytest.mark.parametrize("dtype,N,M", [(torch.float32, 6000, 4500), (torch.float64, 3000, 2500)])
 def test_clouds_beyond_lds_stream(dev, dtype, N, M):
-    """A pair whose clouds exceed LDS (FP32: 10240 points, FP64: 5120) runs the streamed kernels:
+    """A pair whose clouds exceed LDS (FP32: 10234 points, FP64: 5114) runs the streamed kernels:
     S and both gradients against autograd of the FP64 oracle (ESP weights on)."""
     print(f"case score-as-loss streamed {dtype} N={N} M={M}", flush=True)
     ref64, fit64, re64, fe64 = _clouds(1, N, M, 5, torch.float64)
@@ -129,3 +129,45 @@ def test_clouds_beyond_lds_stream(dev, dtype, N, M):
     for got, want in ((fit.grad, x.grad), (ref.grad, r.grad)):
         g, w = got.cpu().numpy(), want.numpy()
         np.testing.assert_allclose(g, w, atol=gtol * np.abs(w).max())
+
+
+def _score_ragged(clouds, lam, dev):
+    """fss_score_fwd / fss_score_bwd (FP32, ESP) through the C ABI on a ragged batch of
+    (ref (N, 3), fit (M, 3), ref_esp (N,), fit_esp (M,)) pairs: S, overlaps, dS/dfit, dS/dref."""
+    from fast_shepherd_score_amd import _lib
+    pack = lambda xs, ws: torch.cat([torch.cat([x, w[:, None]], 1) for x, w in zip(xs, ws)]).float().to(dev).contiguous()
+    rng = lambda ns: torch.tensor([[sum(ns[:i]), n] for i, n in enumerate(ns)], dtype=torch.int32, device=dev)
+    rp, fp = pack([c[0] for c in clouds], [c[2] for c in clouds]), pack([c[1] for c in clouds], [c[3] for c in clouds])
+    ns, ms = [len(c[0]) for c in clouds], [len(c[1]) for c in clouds]
+    rr, fr = rng(ns), rng(ms)
+    pb = _lib.FssPairBatch()
+    pb.n_pairs, pb.ref_pts, pb.fit_pts, pb.ref_rng, pb.fit_rng = len(clouds), _lib.ptr(rp), _lib.ptr(fp), _lib.ptr(rr), _lib.ptr(fr)
+    pb.max_ref, pb.max_fit = max(ns), max(ms)
+    sp = _lib.FssScoreParams()
+    sp.mode, sp.alpha, sp.lam = _lib.MODE_SHAPE_ESP, 0.81, lam
+    P = len(clouds)
+    S = torch.empty(P, device=dev)
+    ov = torch.empty((P, 4), device=dev)
+    stream = int(torch.cuda.current_stream().cuda_stream)
+    _lib.call("fss_score_fwd", _lib.byref(pb), _lib.byref(sp), _lib.ptr(S), _lib.ptr(ov), stream)
+    g = torch.ones(P, device=dev)
+    d_fit = torch.empty((sum(ms), 4), device=dev)
+    d_ref = torch.empty((sum(ns), 4), device=dev)
+    _lib.call("fss_score_bwd", _lib.byref(pb), _lib.byref(sp), _lib.ptr(ov), _lib.ptr(g), _lib.ptr(d_fit),
+              _lib.ptr(d_ref), stream)
+    torch.cuda.synchronize()
+    return S.cpu(), ov.cpu(), d_fit.cpu(), d_ref.cpu()
+
+
+def test_streamed_equals_staged_bitwise(dev):
+    """The streamed kernels take the staged kernels' sums in the same order: a small pair gives
+    the same bits when its batch also holds a pair beyond LDS (which streams the whole batch)."""
+    print("case score-as-loss staged vs streamed, 300 x 280 beside 6000 x 4500", flush=True)
+    small = [x[0] for x in _clouds(1, 300, 280, 11, torch.float64)]
+    big = [x[0] for x in _clouds(1, 6000, 4500, 12, torch.float64)]
+    big[0], big[1] = big[0] * 6.0, big[1] * 6.0
+    S1, ov1, df1, dr1 = _score_ragged([small], 0.5, dev)
+    S2, ov2, df2, dr2 = _score_ragged([small, big], 0.5, dev)
+    assert torch.isfinite(S2).all()
+    assert torch.equal(S1[0], S2[0]) and torch.equal(ov1[0], ov2[0])
+    assert torch.equal(df1, df2[:280]) and torch.equal(dr1, dr2[:300])
