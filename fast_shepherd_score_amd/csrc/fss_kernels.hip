@@ -1170,10 +1170,10 @@ __global__ void k_align_finish(int n_pairs, int B, const float* __restrict__ q,
 //
 // Both clouds of a pair are staged in LDS as 4-D points p = (s x, s y, s z, s_w w), with
 // s = sqrt(c alpha / 2), s_w = sqrt(c / lam) (0 without ESP), c = log2(e) for FP32 (exp2) and 1
-// for FP64 (exp), after subtracting the pair's first ref point (every sum is translation
+// for FP64 (exp), after subtracting the ref cloud's centroid (every sum is translation
 // invariant).  Then w_ab E_ab = base^(-|p_a - p_b|^2): the ESP weight exp(-(w_a - w_b)^2 / lam)
-// is the fourth coordinate of one 4-D Gaussian, and each term costs 4 differences, a squared norm,
-// one exponential and the sums.
+// is the fourth coordinate of one 4-D Gaussian, and each term costs one exponent (expanded form
+// in FP32, direct form in FP64), one exponential and the sums.
 // ------------------------------------------------------------------------------------------------
 template <typename T> struct Vec4;
 template <> struct Vec4<float> { using type = float4; };
@@ -1185,35 +1185,169 @@ template <> __device__ __forceinline__ double neg_exp<double>(double r2) { retur
 template <typename T> constexpr T score_base() { return sizeof(T) == 4 ? (T)kLog2eD : (T)1; }
 
 constexpr int kScoreBlock = 512;                 // threads of the score-as-loss kernels (8 wavefronts)
+static_assert((kScoreBlock / 64 & (kScoreBlock / 64 - 1)) == 0, "the forward deals tiles by unit & (wavefronts - 1)");
 constexpr int kScoreSlices = 4;                  // backward: lanes sharing one row
 constexpr int kScoreRows = 64 / kScoreSlices;    // backward: rows per wavefront unit
 static_assert(kScoreSlices == 4, "k_score_bwd combines the slices with two shuffles");
 
-template <typename T, bool ESP>
-__device__ __forceinline__ void stage_cloud(const T* __restrict__ src, int n, const T c[3], T s, T sw,
-                                            typename Vec4<T>::type* dst) {
+// FP32 instances take the exponent in the expanded form -|p - q|^2 = W_p + W_q + 2 p.q, with
+// W = -|p|^2 staged beside each point: 1 add and 3-4 FMA per term in place of 4 differences and
+// a squared norm.  The frame's origin is the ref cloud's centroid, so |p|^2 stays at molecule
+// scale (a few tens) and the rounding of the cancelling terms is a few 1e-6 of a term.  FP64
+// instances (gradcheck) take the direct form.
+template <typename T> constexpr bool score_expanded() { return sizeof(T) == 4; }
+
+// Origin of the scaled frame: the ref cloud's centroid (the fit's when the ref is empty, 0 when
+// both are), summed in a fixed order: per-thread strided sums, the wavefront's sums, then the
+// wavefronts' sums in order.  red holds 3 values per wavefront; every thread calls it (two barriers).
+template <typename T>
+__device__ __forceinline__ void score_origin(const T* __restrict__ A, int N, const T* __restrict__ Bp, int M,
+                                             T* red, T c[3]) {
+  const T* src = N > 0 ? A : Bp;
+  const int n = N > 0 ? N : M;
+  T sx = 0, sy = 0, sz = 0;
   for (int i = threadIdx.x; i < n; i += blockDim.x) {
-    const T* p = src + 4 * (long)i;
-    dst[i] = {(p[0] - c[0]) * s, (p[1] - c[1]) * s, (p[2] - c[2]) * s, ESP ? p[3] * sw : (T)0};
+    sx += src[4 * (long)i];
+    sy += src[4 * (long)i + 1];
+    sz += src[4 * (long)i + 2];
+  }
+  sx = group_sum(sx, 64);
+  sy = group_sum(sy, 64);
+  sz = group_sum(sz, 64);
+  const int wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
+  if ((threadIdx.x & 63) == 0) {
+    red[3 * wave] = sx;
+    red[3 * wave + 1] = sy;
+    red[3 * wave + 2] = sz;
+  }
+  __syncthreads();
+  T t[3] = {0, 0, 0};
+  for (int w = 0; w < nw; ++w)
+    for (int k = 0; k < 3; ++k) t[k] += red[3 * w + k];
+  __syncthreads();   // the caller reuses red
+  for (int k = 0; k < 3; ++k) c[k] = n > 0 ? t[k] / (T)n : (T)0;
+}
+
+// Scale factors and origin of a pair's frame.
+template <typename T, bool ESP>
+__device__ __forceinline__ void score_frame(const T* __restrict__ A, int N, const T* __restrict__ Bp, int M,
+                                            T alpha, T lam, T* red, T c[3], T& s, T& sw) {
+  const T base = score_base<T>();
+  s = sqrt(alpha / 2 * base);
+  sw = ESP ? sqrt(base / lam) : (T)0;
+  score_origin<T>(A, N, Bp, M, red, c);
+}
+
+// Scaled 4-D point i of a cloud.
+template <typename T, bool ESP>
+__device__ __forceinline__ typename Vec4<T>::type score_point(const T* __restrict__ src, int i, const T c[3],
+                                                              T s, T sw) {
+  const T* p = src + 4 * (long)i;
+  return {(p[0] - c[0]) * s, (p[1] - c[1]) * s, (p[2] - c[2]) * s, ESP ? p[3] * sw : (T)0};
+}
+
+// W = -|p|^2 of a scaled point (the expanded form's per-point term).
+__device__ __forceinline__ float neg_sq(const float4& p) {
+  return -fmaf(p.x, p.x, fmaf(p.y, p.y, fmaf(p.z, p.z, p.w * p.w)));
+}
+
+// The expanded form's terms W_p, W_q and 2 p.q cancel to -|p - q|^2; their rounding grows with
+// |p|^2.  A pair takes it only while every scaled point has |p|^2 <= kScoreHMax (about 30 A from
+// the ref centroid at alpha 0.81); other pairs take the direct form.  Measured on 1024 surface +
+// ESP pairs of 300 points (|p|^2 up to 227) against the FP64 kernels: S within 1.6e-7, dS/dfit
+// within 4e-6 of its largest component, the same as the direct form's.
+constexpr float kScoreHMax = 512.f;
+
+// Whether a pair's clouds leave the expanded form's range (FP32; every thread calls it, two
+// barriers; the same answer staged and streamed, from the same points).
+template <typename T, bool ESP>
+__device__ __forceinline__ bool score_far(const T* __restrict__ A, int N, const T* __restrict__ Bp, int M,
+                                          const T c[3], T s, T sw, T* red) {
+  if constexpr (!score_expanded<T>()) {
+    return true;
+  } else {
+    float h = 0.f;
+    for (int i = threadIdx.x; i < N + M; i += blockDim.x)
+      h = fmaxf(h, -neg_sq(score_point<T, ESP>(i < N ? A : Bp, i < N ? i : i - N, c, s, sw)));
+    for (int m = 32; m >= 1; m >>= 1) h = fmaxf(h, __shfl_xor(h, m, 64));
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = h;
+    __syncthreads();
+    float hm = 0.f;
+    for (int w = 0; w < (int)(blockDim.x >> 6); ++w) hm = fmaxf(hm, red[w]);
+    __syncthreads();   // the caller reuses red
+    return !(hm <= kScoreHMax);   // NaN points take the direct form
   }
 }
 
-// Stages the pair's clouds (ref then fit) and returns their counts.
+// Stages the pair's clouds (ref then fit) in the frame of score_frame, with W (FP32) after them,
+// in one pass over HBM: the raw points go to LDS while the origin's sums run (the same threads
+// summing the same points in the same order as score_origin), then they are moved into the frame
+// in place, with score_far's test on the way (a maximum: the same in any order).
 template <typename T, bool ESP>
 __device__ __forceinline__ void stage_pair_clouds(const T* __restrict__ A, int N, const T* __restrict__ Bp, int M,
-                                                  T alpha, T lam, typename Vec4<T>::type* la,
-                                                  typename Vec4<T>::type* lb, T& s) {
+                                                  T alpha, T lam, T* red, typename Vec4<T>::type* la,
+                                                  typename Vec4<T>::type* lb, T* wa, T* wb, T& s, bool& far) {
+  using V = typename Vec4<T>::type;
+  const int n0 = N > 0 ? N : M;   // the cloud whose centroid is the origin comes first
+  T sx = 0, sy = 0, sz = 0;
+  for (int i = threadIdx.x; i < N + M; i += blockDim.x) {
+    const bool is_ref = i < N;
+    const T* q = (is_ref ? A : Bp) + 4 * (long)(is_ref ? i : i - N);
+    const V raw = {q[0], q[1], q[2], q[3]};
+    (is_ref ? la[i] : lb[i - N]) = raw;
+    if (i < n0) {
+      sx += raw.x;
+      sy += raw.y;
+      sz += raw.z;
+    }
+  }
+  sx = group_sum(sx, 64);
+  sy = group_sum(sy, 64);
+  sz = group_sum(sz, 64);
+  const int wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
+  if ((threadIdx.x & 63) == 0) {
+    red[3 * wave] = sx;
+    red[3 * wave + 1] = sy;
+    red[3 * wave + 2] = sz;
+  }
+  __syncthreads();   // also: the raw points are in LDS
+  T t[3] = {0, 0, 0};
+  for (int w = 0; w < nw; ++w)
+    for (int k = 0; k < 3; ++k) t[k] += red[3 * w + k];
+  T c[3];
+  for (int k = 0; k < 3; ++k) c[k] = n0 > 0 ? t[k] / (T)n0 : (T)0;
   const T base = score_base<T>();
   s = sqrt(alpha / 2 * base);
   const T sw = ESP ? sqrt(base / lam) : (T)0;
-  const T* o = N > 0 ? A : Bp;
-  const T c[3] = {N + M > 0 ? o[0] : (T)0, N + M > 0 ? o[1] : (T)0, N + M > 0 ? o[2] : (T)0};
-  stage_cloud<T, ESP>(A, N, c, s, sw, la);
-  stage_cloud<T, ESP>(Bp, M, c, s, sw, lb);
-  __syncthreads();
+  float h = 0.f;
+  for (int i = threadIdx.x; i < N + M; i += blockDim.x) {
+    const bool is_ref = i < N;
+    V& slot = is_ref ? la[i] : lb[i - N];
+    const V raw = slot;
+    const V p = {(raw.x - c[0]) * s, (raw.y - c[1]) * s, (raw.z - c[2]) * s, ESP ? raw.w * sw : (T)0};
+    slot = p;
+    if constexpr (score_expanded<T>()) {
+      const float w = neg_sq(p);
+      (is_ref ? wa[i] : wb[i - N]) = w;
+      h = fmaxf(h, -w);
+    }
+  }
+  if constexpr (score_expanded<T>()) {
+    for (int m = 32; m >= 1; m >>= 1) h = fmaxf(h, __shfl_xor(h, m, 64));
+    __syncthreads();   // every thread has read the origin's sums from red
+    if ((threadIdx.x & 63) == 0) red[wave] = h;
+    __syncthreads();   // also: the staged points are in LDS
+    float hm = 0.f;
+    for (int w = 0; w < nw; ++w) hm = fmaxf(hm, red[w]);
+    far = !(hm <= kScoreHMax);
+    __syncthreads();   // the caller reuses red
+  } else {
+    far = true;
+    __syncthreads();
+  }
 }
 
-// base^(-|q - p|^2) of one (column, row) point pair.
+// base^(-|q - p|^2) of one (column, row) point pair, direct form.
 template <typename T, bool ESP>
 __device__ __forceinline__ T pair_exp(const typename Vec4<T>::type& q, const typename Vec4<T>::type& p) {
   const T dx = q.x - p.x, dy = q.y - p.y, dz = q.z - p.z;
@@ -1222,9 +1356,19 @@ __device__ __forceinline__ T pair_exp(const typename Vec4<T>::type& q, const typ
   return neg_exp<T>(r2);
 }
 
+// -|q - p|^2 in the expanded form: W_q + W_p + 2 p.q, with p2 = 2 p.
+template <bool ESP>
+__device__ __forceinline__ float expo(const float4& q, float wq, const float4& p2, float wp) {
+  float g = wq + wp;
+  if (ESP) g = fmaf(q.w, p2.w, g);
+  g = fmaf(q.z, p2.z, g);
+  g = fmaf(q.y, p2.y, g);
+  return fmaf(q.x, p2.x, g);
+}
+
 // Sum over the columns [0, nc) of base^(-|q_c - p|^2); every lane reads the same column (LDS
 // broadcast).  Four running sums take the columns c % 4 (four loads in flight per step), added
-// in a fixed order at the end.
+// in a fixed order at the end.  Direct form (FP64).
 template <typename T, bool ESP>
 __device__ __forceinline__ T tile_sum(const typename Vec4<T>::type* __restrict__ cols, int nc,
                                       typename Vec4<T>::type p) {
@@ -1244,6 +1388,42 @@ __device__ __forceinline__ T tile_sum(const typename Vec4<T>::type* __restrict__
   return (e[0] + e[1]) + (e[2] + e[3]);
 }
 
+// tile_sum in the expanded form (FP32): the same columns in the same running sums.
+template <bool ESP>
+__device__ __forceinline__ float tile_sum_x(const float4* __restrict__ cols, const float* __restrict__ cw, int nc,
+                                            const float4& p2, float wp) {
+  float e[4] = {0.f, 0.f, 0.f, 0.f};
+  int c = 0;
+  for (; c + 3 < nc; c += 4) {
+    float4 q[4];
+    float w[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      q[k] = cols[c + k];
+      w[k] = cw[c + k];
+    }
+#pragma unroll
+    for (int k = 0; k < 4; ++k) e[k] += fast_exp2(expo<ESP>(q[k], w[k], p2, wp));
+  }
+#pragma unroll
+  for (int k = 0; k < 3; ++k)
+    if (c + k < nc) e[k] += fast_exp2(expo<ESP>(cols[c + k], cw[c + k], p2, wp));
+  return (e[0] + e[1]) + (e[2] + e[3]);
+}
+
+// One 64-column tile of a row against the tile's columns, either form.
+template <typename T, bool ESP>
+__device__ __forceinline__ T row_tile(const typename Vec4<T>::type* __restrict__ cols, const T* __restrict__ cw,
+                                      int nc, const typename Vec4<T>::type& p, T wp, bool far) {
+  if constexpr (score_expanded<T>()) {
+    if (!far) {
+      const float4 p2 = {2.f * p.x, 2.f * p.y, 2.f * p.z, 2.f * p.w};
+      return tile_sum_x<ESP>(cols, cw, nc, p2, wp);
+    }
+  }
+  return tile_sum<T, ESP>(cols, nc, p);
+}
+
 // Forward: VAB, VAA and VBB as 64 x 64 tiles (lane = row, columns broadcast from LDS), VAA and VBB
 // over the upper block triangle only (off-diagonal tiles count twice).  The tiles are dealt to the
 // wavefronts round-robin, so a pair's whole product is balanced over its workgroup.
@@ -1259,25 +1439,36 @@ __global__ __launch_bounds__(kScoreBlock) void k_score_fwd(const T* __restrict__
   const int N = rr[2 * pair + 1], M = fr[2 * pair + 1];
   V* la = reinterpret_cast<V*>(score_smem);
   V* lb = la + N;
+  T* wa = reinterpret_cast<T*>(lb + M);   // FP32 only
+  T* wb = wa + N;
   T s;
-  stage_pair_clouds<T, ESP>(ref + 4 * (long)rr[2 * pair], N, fit + 4 * (long)fr[2 * pair], M, alpha, lam, la, lb, s);
-  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, nw = blockDim.x >> 6;
+  bool far;
+  stage_pair_clouds<T, ESP>(ref + 4 * (long)rr[2 * pair], N, fit + 4 * (long)fr[2 * pair], M, alpha, lam, red,
+                            la, lb, wa, wb, s, far);
+  constexpr int nw = kScoreBlock / 64;   // wavefronts (the launch's block is kScoreBlock)
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   T acc[3] = {0, 0, 0};   // VAB, VAA, VBB sums
   int u = 0;
 #pragma unroll
   for (int t = 0; t < 3; ++t) {
     const V* rows = t == 1 ? la : lb;
     const V* cols = t == 2 ? lb : la;
+    const T* rw = t == 1 ? wa : wb;
+    const T* cw = t == 2 ? wb : wa;
     const int nr = t == 1 ? N : M, nc = t == 2 ? M : N;
     const int nbr = (nr + 63) / 64, nbc = (nc + 63) / 64;
     for (int i = 0; i < nbr; ++i) {
-      const int r = i * 64 + lane;
-      const V p = rows[r < nr ? r : nr - 1];
-      for (int j = t == 0 ? 0 : i; j < nbc; ++j, ++u) {
-        if (u % nw != wave) continue;
-        const T e = tile_sum<T, ESP>(cols + 64 * j, min(64, nc - 64 * j), p);
+      const int r = i * 64 + lane, rc = r < nr ? r : nr - 1;
+      const V p = rows[rc];
+      const T wp = score_expanded<T>() ? rw[rc] : (T)0;
+      // the row block's tiles j0 .. nbc - 1 are units u .. u + nbc - j0 - 1; this wavefront's
+      // are those with unit % nw == wave
+      const int j0 = t == 0 ? 0 : i;
+      for (int j = j0 + ((wave - u) & (nw - 1)); j < nbc; j += nw) {
+        const T e = row_tile<T, ESP>(cols + 64 * j, cw + 64 * j, min(64, nc - 64 * j), p, wp, far);
         acc[t] += r < nr ? (t != 0 && j > i ? 2 * e : e) : (T)0;
       }
+      u += nbc - j0;
     }
   }
 #pragma unroll
@@ -1299,12 +1490,13 @@ __global__ __launch_bounds__(kScoreBlock) void k_score_fwd(const T* __restrict__
 
 // Sum over the columns first, first + kScoreSlices, ... < n of base^(-|q - p|^2) (q - p) into two
 // running sums, a over the slice's even-numbered columns and b over its odd-numbered ones (two
-// loads in flight); the caller adds a + b after its last call.  A block of 64 columns holds an
-// even number of each slice's columns, so calls over consecutive 64-column blocks add the same
-// terms to the same sums in the same order as one call over the whole range.
+// loads in flight); the caller combines them after its last call (slice_total).  A block of 64
+// columns holds an even number of each slice's columns, so calls over consecutive 64-column
+// blocks add the same terms to the same sums in the same order as one call over the whole range.
+// Direct form (FP64): a[0..2] are the force sums.
 template <typename T, bool ESP>
 __device__ __forceinline__ void slice_force(const typename Vec4<T>::type* __restrict__ cols, int n, int first,
-                                            typename Vec4<T>::type p, T a[3], T b[3]) {
+                                            typename Vec4<T>::type p, T a[4], T b[4]) {
   using V = typename Vec4<T>::type;
   int c = first;
   for (; c + kScoreSlices < n; c += 2 * kScoreSlices) {
@@ -1330,7 +1522,72 @@ __device__ __forceinline__ void slice_force(const typename Vec4<T>::type* __rest
     a[0] += e * dx; a[1] += e * dy; a[2] += e * dz;
   }
 }
+
+// slice_force in the expanded form (FP32): a, b = (sum E, sum E q), the force sum E (q - p) formed
+// once at the end (slice_total).
+template <bool ESP>
+__device__ __forceinline__ void slice_force_x(const float4* __restrict__ cols, const float* __restrict__ cw, int n,
+                                              int first, const float4& p2, float wp, float a[4], float b[4]) {
+  int c = first;
+  // four columns per step (a, b, a, b): the same order per running sum as two per step
+  for (; c + 3 * kScoreSlices < n; c += 4 * kScoreSlices) {
+    float4 q[4];
+    float w[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      q[k] = cols[c + k * kScoreSlices];
+      w[k] = cw[c + k * kScoreSlices];
+    }
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      const float e = fast_exp2(expo<ESP>(q[k], w[k], p2, wp));
+      float* r = (k & 1) ? b : a;
+      r[0] += e; r[1] = fmaf(e, q[k].x, r[1]); r[2] = fmaf(e, q[k].y, r[2]); r[3] = fmaf(e, q[k].z, r[3]);
+    }
+  }
+  for (; c + kScoreSlices < n; c += 2 * kScoreSlices) {
+    const float4 q0 = cols[c], q1 = cols[c + kScoreSlices];
+    const float w0 = cw[c], w1 = cw[c + kScoreSlices];
+    const float e0 = fast_exp2(expo<ESP>(q0, w0, p2, wp)), e1 = fast_exp2(expo<ESP>(q1, w1, p2, wp));
+    a[0] += e0; a[1] = fmaf(e0, q0.x, a[1]); a[2] = fmaf(e0, q0.y, a[2]); a[3] = fmaf(e0, q0.z, a[3]);
+    b[0] += e1; b[1] = fmaf(e1, q1.x, b[1]); b[2] = fmaf(e1, q1.y, b[2]); b[3] = fmaf(e1, q1.z, b[3]);
+  }
+  if (c < n) {
+    const float4 q = cols[c];
+    const float e = fast_exp2(expo<ESP>(q, cw[c], p2, wp));
+    a[0] += e; a[1] = fmaf(e, q.x, a[1]); a[2] = fmaf(e, q.y, a[2]); a[3] = fmaf(e, q.z, a[3]);
+  }
+}
 static_assert(64 % (2 * kScoreSlices) == 0, "a 64-column block must hold an even count of each slice's columns");
+
+// One call of either form over columns [0, n) of a block.
+template <typename T, bool ESP>
+__device__ __forceinline__ void row_slice(const typename Vec4<T>::type* __restrict__ cols, const T* __restrict__ cw,
+                                          int n, int first, const typename Vec4<T>::type& p, T wp, bool far,
+                                          T a[4], T b[4]) {
+  if constexpr (score_expanded<T>()) {
+    if (!far) {
+      const float4 p2 = {2.f * p.x, 2.f * p.y, 2.f * p.z, 2.f * p.w};
+      slice_force_x<ESP>(cols, cw, n, first, p2, wp, a, b);
+      return;
+    }
+  }
+  slice_force<T, ESP>(cols, n, first, p, a, b);
+}
+
+// The force sum E (q - p) of a row from its two running sums (expanded form: (E, E q) sums).
+template <typename T>
+__device__ __forceinline__ void slice_total(const T a[4], const T b[4], const typename Vec4<T>::type& p, bool far,
+                                            T f[3]) {
+  if (score_expanded<T>() && !far) {
+    const T e = a[0] + b[0];
+    f[0] = (a[1] + b[1]) - p.x * e;
+    f[1] = (a[2] + b[2]) - p.y * e;
+    f[2] = (a[3] + b[3]) - p.z * e;
+  } else {
+    for (int k = 0; k < 3; ++k) f[k] = a[k] + b[k];
+  }
+}
 
 // Backward.  With U = VAA + VBB and D = U - VAB, for a fit point x_b
 //   dS/dx_b = K alpha / D^2 (U sum_a w E (a - x_b) - 2 VAB sum_b' w E (x_b' - x_b))
@@ -1346,34 +1603,40 @@ __global__ __launch_bounds__(kScoreBlock) void k_score_bwd(const T* __restrict__
                                                           T* d_fit, T* d_ref) {
   using V = typename Vec4<T>::type;
   extern __shared__ double4 score_smem[];
+  __shared__ T red[3 * (kScoreBlock / 64)];
   const int pair = blockIdx.x;
   const int N = rr[2 * pair + 1], M = fr[2 * pair + 1];
   V* la = reinterpret_cast<V*>(score_smem);
   V* lb = la + N;
+  T* wa = reinterpret_cast<T*>(lb + M);   // FP32 only
+  T* wb = wa + N;
   T s;
-  stage_pair_clouds<T, ESP>(ref + 4 * (long)rr[2 * pair], N, fit + 4 * (long)fr[2 * pair], M, alpha, lam, la, lb, s);
+  bool far;
+  stage_pair_clouds<T, ESP>(ref + 4 * (long)rr[2 * pair], N, fit + 4 * (long)fr[2 * pair], M, alpha, lam, red,
+                            la, lb, wa, wb, s, far);
   const T K = (T)pow(M_PI / (2.0 * (double)alpha), 1.5);
   const T vab = ov[4 * pair], U = ov[4 * pair + 1] + ov[4 * pair + 2], D = U - vab;
   const T g = gS[pair] * K * alpha / (D * D) / s;   // the force sums are in scaled coordinates
-  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, nw = blockDim.x >> 6;
+  constexpr int nw = kScoreBlock / 64;   // wavefronts (the launch's block is kScoreBlock)
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int sl = lane / kScoreRows, rl = lane % kScoreRows;
   const int nbf = (M + kScoreRows - 1) / kScoreRows, nba = d_ref ? (N + kScoreRows - 1) / kScoreRows : 0;
   for (int u = wave; u < nbf + nba; u += nw) {
     const bool fit_row = u < nbf;
     const V* own = fit_row ? lb : la;
     const V* oth = fit_row ? la : lb;
+    const T* ownw = fit_row ? wb : wa;
+    const T* othw = fit_row ? wa : wb;
     const int n_own = fit_row ? M : N, n_oth = fit_row ? N : M;
-    const int row = (fit_row ? u : u - nbf) * kScoreRows + rl;
-    const V p = own[row < n_own ? row : n_own - 1];
-    T oa[3] = {0, 0, 0}, ob[3] = {0, 0, 0}, sa[3] = {0, 0, 0}, sb[3] = {0, 0, 0};
-    slice_force<T, ESP>(oth, n_oth, sl, p, oa, ob);
-    slice_force<T, ESP>(own, n_own, sl, p, sa, sb);
+    const int row = (fit_row ? u : u - nbf) * kScoreRows + rl, rc = row < n_own ? row : n_own - 1;
+    const V p = own[rc];
+    const T wp = score_expanded<T>() ? ownw[rc] : (T)0;
+    T oa[4] = {0, 0, 0, 0}, ob[4] = {0, 0, 0, 0}, sa[4] = {0, 0, 0, 0}, sb[4] = {0, 0, 0, 0};
+    row_slice<T, ESP>(oth, othw, n_oth, sl, p, wp, far, oa, ob);
+    row_slice<T, ESP>(own, ownw, n_own, sl, p, wp, far, sa, sb);
     T fo[3], fs[3];
-#pragma unroll
-    for (int k = 0; k < 3; ++k) {
-      fo[k] = oa[k] + ob[k];
-      fs[k] = sa[k] + sb[k];
-    }
+    slice_total<T>(oa, ob, p, far, fo);
+    slice_total<T>(sa, sb, p, far, fs);
     T d[3];
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
@@ -1392,40 +1655,33 @@ __global__ __launch_bounds__(kScoreBlock) void k_score_bwd(const T* __restrict__
 // ------------------------------------------------------------------------------------------------
 // Clouds beyond LDS: the same forward and backward with each wavefront staging the 64 points of
 // the column block it works on (one per lane) into a buffer of its own, and the row points read
-// from HBM; the column partition per row (blocks of 64; slice c % 4 in the backward) is the staged
-// kernels', so the sums are taken in the same order.
+// from HBM; the frame, the column partition per row (blocks of 64; slice c % 4 in the backward)
+// and the per-point values are the staged kernels', so the sums are taken in the same order.
 // ------------------------------------------------------------------------------------------------
-// Scaled 4-D point i of a cloud, as stage_cloud writes it.
-template <typename T, bool ESP>
-__device__ __forceinline__ typename Vec4<T>::type score_point(const T* __restrict__ src, int i, const T c[3],
-                                                              T s, T sw) {
-  const T* p = src + 4 * (long)i;
-  return {(p[0] - c[0]) * s, (p[1] - c[1]) * s, (p[2] - c[2]) * s, ESP ? p[3] * sw : (T)0};
-}
-
-// Points [c0, c0 + n) of a cloud into the calling wavefront's buffer (lane k writes point c0 + k).
-// The wavefront-scope fences order the stores after the previous block's loads and before this
-// block's (a wavefront's LDS operations complete in issue order).
+// Points [c0, c0 + n) of a cloud into the calling wavefront's buffers (lane k writes point c0 + k
+// and, FP32, its W).  The wavefront-scope fences order the stores after the previous block's loads
+// and before this block's (a wavefront's LDS operations complete in issue order).
 template <typename T, bool ESP>
 __device__ __forceinline__ void stage_block(const T* __restrict__ src, int c0, int n, const T c[3], T s, T sw,
-                                            typename Vec4<T>::type* wbuf) {
+                                            typename Vec4<T>::type* wbuf, T* wbw) {
   const int lane = threadIdx.x & 63;
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
   __builtin_amdgcn_wave_barrier();
-  if (lane < n) wbuf[lane] = score_point<T, ESP>(src, c0 + lane, c, s, sw);
+  if (lane < n) {
+    const typename Vec4<T>::type p = score_point<T, ESP>(src, c0 + lane, c, s, sw);
+    wbuf[lane] = p;
+    if constexpr (score_expanded<T>()) wbw[lane] = neg_sq(p);
+  }
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
   __builtin_amdgcn_wave_barrier();
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-template <typename T, bool ESP>
-__device__ __forceinline__ void score_frame(const T* __restrict__ A, int N, const T* __restrict__ Bp, int M,
-                                            T alpha, T lam, T c[3], T& s, T& sw) {
-  const T base = score_base<T>();
-  s = sqrt(alpha / 2 * base);
-  sw = ESP ? sqrt(base / lam) : (T)0;
-  const T* o = N > 0 ? A : Bp;
-  for (int k = 0; k < 3; ++k) c[k] = N + M > 0 ? o[k] : (T)0;
+// W of a row point read from HBM (FP32; 0 for the direct form).
+template <typename T>
+__device__ __forceinline__ T row_w(const typename Vec4<T>::type& p) {
+  if constexpr (score_expanded<T>()) return neg_sq(p);
+  else return (T)0;
 }
 
 template <typename T, bool ESP>
@@ -1435,15 +1691,19 @@ __global__ __launch_bounds__(kScoreBlock) void k_score_fwd_stream(const T* __res
                                                                  T* S, T* ov) {
   using V = typename Vec4<T>::type;
   __shared__ V wbufs[kScoreBlock];
+  __shared__ T wbws[kScoreBlock];
   __shared__ T red[3 * (kScoreBlock / 64)];
   const int pair = blockIdx.x;
   const int N = rr[2 * pair + 1], M = fr[2 * pair + 1];
   const T* A = ref + 4 * (long)rr[2 * pair];
   const T* Bp = fit + 4 * (long)fr[2 * pair];
   T c[3], s, sw;
-  score_frame<T, ESP>(A, N, Bp, M, alpha, lam, c, s, sw);
-  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, nw = blockDim.x >> 6;
+  score_frame<T, ESP>(A, N, Bp, M, alpha, lam, red, c, s, sw);
+  const bool far = score_far<T, ESP>(A, N, Bp, M, c, s, sw, red);
+  constexpr int nw = kScoreBlock / 64;   // wavefronts (the launch's block is kScoreBlock)
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   V* wbuf = wbufs + 64 * wave;
+  T* wbw = wbws + 64 * wave;
   T acc[3] = {0, 0, 0};
   int u = 0;
 #pragma unroll
@@ -1455,13 +1715,15 @@ __global__ __launch_bounds__(kScoreBlock) void k_score_fwd_stream(const T* __res
     for (int i = 0; i < nbr; ++i) {
       const int r = i * 64 + lane;
       const V p = score_point<T, ESP>(rows, r < nr ? r : nr - 1, c, s, sw);
-      for (int j = t == 0 ? 0 : i; j < nbc; ++j, ++u) {
-        if (u % nw != wave) continue;
+      const T wp = row_w<T>(p);
+      const int j0 = t == 0 ? 0 : i;
+      for (int j = j0 + ((wave - u) & (nw - 1)); j < nbc; j += nw) {
         const int n = min(64, nc - 64 * j);
-        stage_block<T, ESP>(cols, 64 * j, n, c, s, sw, wbuf);
-        const T e = tile_sum<T, ESP>(wbuf, n, p);
+        stage_block<T, ESP>(cols, 64 * j, n, c, s, sw, wbuf, wbw);
+        const T e = row_tile<T, ESP>(wbuf, wbw, n, p, wp, far);
         acc[t] += r < nr ? (t != 0 && j > i ? 2 * e : e) : (T)0;
       }
+      u += nbc - j0;
     }
   }
 #pragma unroll
@@ -1489,17 +1751,22 @@ __global__ __launch_bounds__(kScoreBlock) void k_score_bwd_stream(const T* __res
                                                                  T* d_fit, T* d_ref) {
   using V = typename Vec4<T>::type;
   __shared__ V wbufs[kScoreBlock];
+  __shared__ T wbws[kScoreBlock];
+  __shared__ T red[3 * (kScoreBlock / 64)];
   const int pair = blockIdx.x;
   const int N = rr[2 * pair + 1], M = fr[2 * pair + 1];
   const T* A = ref + 4 * (long)rr[2 * pair];
   const T* Bp = fit + 4 * (long)fr[2 * pair];
   T c[3], s, sw;
-  score_frame<T, ESP>(A, N, Bp, M, alpha, lam, c, s, sw);
+  score_frame<T, ESP>(A, N, Bp, M, alpha, lam, red, c, s, sw);
+  const bool far = score_far<T, ESP>(A, N, Bp, M, c, s, sw, red);
   const T K = (T)pow(M_PI / (2.0 * (double)alpha), 1.5);
   const T vab = ov[4 * pair], U = ov[4 * pair + 1] + ov[4 * pair + 2], D = U - vab;
   const T g = gS[pair] * K * alpha / (D * D) / s;
-  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, nw = blockDim.x >> 6;
+  constexpr int nw = kScoreBlock / 64;   // wavefronts (the launch's block is kScoreBlock)
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   V* wbuf = wbufs + 64 * wave;
+  T* wbw = wbws + 64 * wave;
   const int sl = lane / kScoreRows, rl = lane % kScoreRows;
   const int nbf = (M + kScoreRows - 1) / kScoreRows, nba = d_ref ? (N + kScoreRows - 1) / kScoreRows : 0;
   for (int u = wave; u < nbf + nba; u += nw) {
@@ -1509,23 +1776,21 @@ __global__ __launch_bounds__(kScoreBlock) void k_score_bwd_stream(const T* __res
     const int n_own = fit_row ? M : N, n_oth = fit_row ? N : M;
     const int row = (fit_row ? u : u - nbf) * kScoreRows + rl;
     const V p = score_point<T, ESP>(own, row < n_own ? row : n_own - 1, c, s, sw);
-    T oa[3] = {0, 0, 0}, ob[3] = {0, 0, 0}, sa[3] = {0, 0, 0}, sb[3] = {0, 0, 0};
+    const T wp = row_w<T>(p);
+    T oa[4] = {0, 0, 0, 0}, ob[4] = {0, 0, 0, 0}, sa[4] = {0, 0, 0, 0}, sb[4] = {0, 0, 0, 0};
     for (int c0 = 0; c0 < n_oth; c0 += 64) {
       const int n = min(64, n_oth - c0);
-      stage_block<T, ESP>(oth, c0, n, c, s, sw, wbuf);
-      slice_force<T, ESP>(wbuf, n, sl, p, oa, ob);
+      stage_block<T, ESP>(oth, c0, n, c, s, sw, wbuf, wbw);
+      row_slice<T, ESP>(wbuf, wbw, n, sl, p, wp, far, oa, ob);
     }
     for (int c0 = 0; c0 < n_own; c0 += 64) {
       const int n = min(64, n_own - c0);
-      stage_block<T, ESP>(own, c0, n, c, s, sw, wbuf);
-      slice_force<T, ESP>(wbuf, n, sl, p, sa, sb);
+      stage_block<T, ESP>(own, c0, n, c, s, sw, wbuf, wbw);
+      row_slice<T, ESP>(wbuf, wbw, n, sl, p, wp, far, sa, sb);
     }
     T fo[3], fs[3];
-#pragma unroll
-    for (int k = 0; k < 3; ++k) {
-      fo[k] = oa[k] + ob[k];
-      fs[k] = sa[k] + sb[k];
-    }
+    slice_total<T>(oa, ob, p, far, fo);
+    slice_total<T>(sa, sb, p, far, fs);
     T d[3];
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
@@ -1600,12 +1865,13 @@ int score_check(const FssPairBatch* batch, const FssScoreParams* sp) {
   return FSS_OK;
 }
 
-// Both clouds of a pair are staged in LDS (4 values per point) when they fit beside the forward's
-// static reduction scratch (3 sums per wavefront); lds = 0 selects the streamed kernels (a
+// Both clouds of a pair are staged in LDS (4 values per point, and W in FP32) when they fit beside
+// the static reduction scratch (3 sums per wavefront); lds = 0 selects the streamed kernels (a
 // 64-point block per wavefront).
 int score_lds(const FssPairBatch* batch, size_t elem, size_t& lds) {
   if (batch->max_ref < 0 || batch->max_fit < 0) return set_error(FSS_E_INVALID, "negative max_ref / max_fit");
-  lds = ((size_t)batch->max_ref + (size_t)batch->max_fit) * 4 * elem;
+  // FP32: the float4 point and its W; FP64: the double4 point
+  lds = ((size_t)batch->max_ref + (size_t)batch->max_fit) * (elem == 4 ? 5 : 4) * elem;
   if (lds + 3 * (kScoreBlock / 64) * elem > max_lds_bytes()) lds = 0;
   return FSS_OK;
 }

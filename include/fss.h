@@ -327,8 +327,8 @@ int fss_seed_poses(const FssPairBatch* batch, int32_t flags, int32_t n_restarts,
  *        dVBB/dfit term; dS/dref likewise when d_ref != NULL.
  * Modes FSS_MODE_SHAPE and FSS_MODE_SHAPE_ESP.  The *_f64 variants take double clouds.
  * Both clouds of a pair are staged in LDS while max_ref + max_fit is at most the per-workgroup
- * LDS size / 16 (float) or / 32 (double) less 6 points of reduction scratch, i.e. 10234 / 5114
- * points on gfx950; larger batches run
+ * LDS size / 20 (float: the point and -|p|^2) or / 32 (double) less the reduction scratch, i.e.
+ * 8187 / 5114 points on gfx950; larger batches run
  * streamed kernels (each wavefront stages the 64-point block it works on), with the same sums in
  * the same order.  bwd is deterministic (fixed reduction order).
  * No reference counterpart for bwd w.r.t. coordinates; forward follows

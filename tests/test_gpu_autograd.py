@@ -110,7 +110,7 @@ def test_ragged_tile_edges_and_determinism(dev, N, M):
 
 @pytest.mark.parametrize("dtype,N,M", [(torch.float32, 6000, 4500), (torch.float64, 3000, 2500)])
 def test_clouds_beyond_lds_stream(dev, dtype, N, M):
-    """A pair whose clouds exceed LDS (FP32: 10234 points, FP64: 5114) runs the streamed kernels:
+    """A pair whose clouds exceed LDS (FP32: 8187 points, FP64: 5114) runs the streamed kernels:
     S and both gradients against autograd of the FP64 oracle (ESP weights on)."""
     print(f"case score-as-loss streamed {dtype} N={N} M={M}", flush=True)
     ref64, fit64, re64, fe64 = _clouds(1, N, M, 5, torch.float64)
@@ -171,3 +171,27 @@ def test_streamed_equals_staged_bitwise(dev):
     assert torch.isfinite(S2).all()
     assert torch.equal(S1[0], S2[0]) and torch.equal(ov1[0], ov2[0])
     assert torch.equal(df1, df2[:280]) and torch.equal(dr1, dr2[:300])
+
+
+@pytest.mark.parametrize("spread,shift", [(4.0, 0.0), (8.0, 0.0), (1.0, 30.0)])
+def test_far_points_take_direct_form(dev, spread, shift):
+    """Pairs reaching to and beyond the FP32 expanded form's range from the ref centroid (about
+    30 A: a wide pair inside it, a wider pair and a displaced fit beyond it, which run the direct
+    form): S and both gradients against autograd of the FP64 oracle, staged in LDS (300 x 280
+    points)."""
+    print(f"case score-as-loss far points spread={spread} shift={shift}", flush=True)
+    ref64, fit64, re64, fe64 = _clouds(2, 300, 280, 21, torch.float64)
+    ref64, fit64 = ref64 * spread, fit64 * spread
+    fit64 = fit64 + torch.tensor([shift, 0.0, 0.0], dtype=torch.float64)
+    x = fit64.clone().requires_grad_(True)
+    r = ref64.clone().requires_grad_(True)
+    S_ref = orc.tanimoto_esp(r, x, re64, fe64, 0.81, 0.5)
+    S_ref.sum().backward()
+    ref = ref64.float().to(dev).requires_grad_(True)
+    fit = fit64.float().to(dev).requires_grad_(True)
+    S = gaussian_similarity(ref, fit, re64.float().to(dev), fe64.float().to(dev), 0.81, 0.5)
+    S.sum().backward()
+    np.testing.assert_allclose(S.detach().cpu().numpy(), S_ref.detach().numpy(), atol=1e-5)
+    for got, want in ((fit.grad, x.grad), (ref.grad, r.grad)):
+        g, w = got.cpu().numpy(), want.numpy()
+        np.testing.assert_allclose(g, w, atol=1e-4 * np.abs(w).max() + 1e-12)
