@@ -1036,4 +1036,54 @@ __device__ __forceinline__ double pharm_value_f64(const float4* __restrict__ ran
   return v;
 }
 
+// FP32 counterpart of pharm_value_f64 (same rows, weights and extended points; exp2 with the
+// table's alpha/2 log2 e) for the re-scoring pre-pass that picks the restarts worth an FP64 score.
+__device__ __forceinline__ float pharm_value_f32(const float4* __restrict__ ranc,
+                                                 const float4* __restrict__ rvec,
+                                                 const int32_t* __restrict__ rtyp, int N,
+                                                 const float4* __restrict__ fanc,
+                                                 const float4* __restrict__ fvec,
+                                                 const int32_t* __restrict__ ftyp, int M, int j,
+                                                 int lpp, const float R[9], const float t[3],
+                                                 const PhTab& tab) {
+  float v = 0.f;
+  for (int b = j; b < M; b += lpp) {
+    const int tb = ftyp[b];
+    if (tb < 0 || tb >= FSS_N_PTYPES) continue;
+    const int cat = tab.cat[tb];
+    if (cat == 3) continue;
+    const float4 p = fanc[b];
+    const float4 u0 = fvec[b];
+    const float un = fmaxf(sqrtf(u0.x * u0.x + u0.y * u0.y + u0.z * u0.z), 1e-12f);
+    const float ux0 = u0.x / un, uy0 = u0.y / un, uz0 = u0.z / un;
+    const float bx = fmaf(R[0], p.x, fmaf(R[1], p.y, fmaf(R[2], p.z, t[0])));
+    const float by = fmaf(R[3], p.x, fmaf(R[4], p.y, fmaf(R[5], p.z, t[1])));
+    const float bz = fmaf(R[6], p.x, fmaf(R[7], p.y, fmaf(R[8], p.z, t[2])));
+    const float ux = R[0] * ux0 + R[1] * uy0 + R[2] * uz0;
+    const float uy = R[3] * ux0 + R[4] * uy0 + R[5] * uz0;
+    const float uz = R[6] * ux0 + R[7] * uy0 + R[8] * uz0;
+    const float kap = tab.kap[tb], kk = tab.k[tb];
+    for (int a = 0; a < N; ++a) {
+      if (rtyp[a] != tb) continue;
+      const float4 r = ranc[a];
+      const float4 rv0 = rvec[a];
+      const float rn = fmaxf(sqrtf(rv0.x * rv0.x + rv0.y * rv0.y + rv0.z * rv0.z), 1e-12f);
+      const float dx = r.x - bx, dy = r.y - by, dz = r.z - bz;
+      const float e = fast_exp2(-kap * fmaf(dx, dx, fmaf(dy, dy, dz * dz)));
+      const float D = (ux * rv0.x + uy * rv0.y + uz * rv0.z) / rn;
+      float w = 1.f;
+      if (cat == 1) w = (fminf(fmaxf(D, 0.f), 1.f) + 2.f) * (1.f / 3.f);
+      else if (cat == 2) w = (fabsf(D) + 2.f) * (1.f / 3.f);
+      else if (cat == 5) w = 0.f;
+      v = fmaf(kk * w, e, v);
+      if (cat >= 4) {   // extended points anchor + unit vector, isotropic
+        const float ex = (r.x + rv0.x / rn) - (bx + ux), ey = (r.y + rv0.y / rn) - (by + uy),
+                    ez = (r.z + rv0.z / rn) - (bz + uz);
+        v = fmaf(kk, fast_exp2(-kap * fmaf(ex, ex, fmaf(ey, ey, ez * ez))), v);
+      }
+    }
+  }
+  return v;
+}
+
 }  // namespace fss

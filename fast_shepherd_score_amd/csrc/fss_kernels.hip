@@ -1074,10 +1074,10 @@ __global__ __launch_bounds__(256) void k_pose_score(FssPairBatch pb, KScore ks, 
   if (j == 0) score[pose] = S;
 }
 
-// Re-scoring pre-pass of the shape modes: the FP32 similarity of every final pose (one wave per
-// pose, direct form), then per pair the threshold max - kRescoreMargin over its restarts.  Only
-// restarts at or above it get the FP64 score; the others keep the FP32 one, whose error (about
-// 1e-6) is far below the margin, so the FP64 argmax over the restarts is unchanged.
+// Re-scoring pre-pass: the FP32 similarity of every final pose (one wave per pose, direct form),
+// then per pair the threshold max - kRescoreMargin over its restarts.  Only restarts at or above
+// it get the FP64 score; the others keep the FP32 one, whose error (about 1e-6) is far below the
+// margin, so the FP64 argmax over the restarts is unchanged.  Every mode but ShaEP.
 constexpr double kRescoreMargin = 1e-3;
 
 template <int MODE>
@@ -1085,8 +1085,10 @@ __global__ __launch_bounds__(256) void k_pose_score_f32(FssPairBatch pb, KScore 
                                                         const double* __restrict__ self_ov,
                                                         const float* __restrict__ qin,
                                                         const float* __restrict__ tin, double* score) {
-  static_assert(MODE == FSS_MODE_SHAPE || MODE == FSS_MODE_SHAPE_ESP, "shape modes only");
+  static_assert(MODE != FSS_MODE_ESP_COMBO, "the ShaEP mode scores every pose in FP64");
+  constexpr bool SH = MODE != FSS_MODE_PHARM;
   constexpr bool ESP = MODE == FSS_MODE_SHAPE_ESP;
+  constexpr bool PH = MODE == FSS_MODE_PHARM || MODE == FSS_MODE_SHAPE_PHARM;
   const int pose = __builtin_amdgcn_readfirstlane(blockIdx.x * 4 + (threadIdx.x >> 6));
   const int j = threadIdx.x & 63;
   if (pose >= n_poses) return;
@@ -1098,13 +1100,25 @@ __global__ __launch_bounds__(256) void k_pose_score_f32(FssPairBatch pb, KScore 
 #pragma unroll
   for (int i = 0; i < 9; ++i) R[i] = (float)Rd[i];
   const float t[3] = {tin[3 * (long)pose], tin[3 * (long)pose + 1], tin[3 * (long)pose + 2]};
-  const float4* gref = reinterpret_cast<const float4*>(pb.ref_pts) + pb.ref_rng[2 * pair];
-  const float4* gfit = reinterpret_cast<const float4*>(pb.fit_pts) + pb.fit_rng[2 * pair];
-  float v = shape_value_f32<ESP>(gref, pb.ref_rng[2 * pair + 1], gfit, pb.fit_rng[2 * pair + 1], j, 64, R, t,
-                                 -ks.s * ks.s, ESP ? -ks.sc * ks.sc : 0.f);
-  v = group_sum(v, 64) * ks.K;
+  float v = 0.f, vp = 0.f;
+  if (SH) {
+    const float4* gref = reinterpret_cast<const float4*>(pb.ref_pts) + pb.ref_rng[2 * pair];
+    const float4* gfit = reinterpret_cast<const float4*>(pb.fit_pts) + pb.fit_rng[2 * pair];
+    v = shape_value_f32<ESP>(gref, pb.ref_rng[2 * pair + 1], gfit, pb.fit_rng[2 * pair + 1], j, 64, R, t,
+                             -ks.s * ks.s, ESP ? -ks.sc * ks.sc : 0.f);
+    v = group_sum(v, 64) * ks.K;
+  }
+  if (PH) {
+    const int ro = pb.ref_prng[2 * pair], fo = pb.fit_prng[2 * pair];
+    vp = pharm_value_f32(reinterpret_cast<const float4*>(pb.ref_anc) + ro,
+                         reinterpret_cast<const float4*>(pb.ref_vec) + ro, pb.ref_typ + ro, pb.ref_prng[2 * pair + 1],
+                         reinterpret_cast<const float4*>(pb.fit_anc) + fo,
+                         reinterpret_cast<const float4*>(pb.fit_vec) + fo, pb.fit_typ + fo, pb.fit_prng[2 * pair + 1],
+                         j, 64, R, t, ks.tab);
+    vp = group_sum(vp, 64);
+  }
   float ws, wp;
-  const float S = similarity<float>(ks, v, 0.f, self_ov + 4 * pair, true, false, ws, wp);
+  const float S = similarity<float>(ks, v, vp, self_ov + 4 * pair, SH, PH, ws, wp);
   if (j == 0) score[pose] = (double)S;
 }
 
@@ -2082,27 +2096,48 @@ __global__ __launch_bounds__(64) void k_seed_poses(FssPairBatch pb, int flags, i
   }
 }
 
-// Final scores and the best restart of every pair.  Shape modes: FP32 pre-pass over every final
-// pose, per-pair threshold (kept in best_score until k_align_finish overwrites it), FP64 for the
-// restarts at or above it.  The other modes score every final pose in FP64.
+// Final scores and the best restart of every pair.  Every mode but ShaEP: FP32 pre-pass over
+// every final pose, per-pair threshold (kept in best_score until k_align_finish overwrites it),
+// FP64 for the restarts at or above it.  ShaEP scores every final pose in FP64.
+template <int MODE>
+void launch_prepass(const FssPairBatch* batch, const KScore& ks, int n, int B, const double* self_ov,
+                    FssAlignOut* out, hipStream_t s) {
+  hipLaunchKernelGGL(k_pose_score_f32<MODE>, dim3((n + 3) / 4), dim3(256), 0, s, *batch, ks, n, B, self_ov,
+                     out->q, out->t, out->pose_score);
+}
+
 int align_finish_impl(const FssPairBatch* batch, const KScore& ks, int B, const double* self_ov,
                       FssAlignOut* out, hipStream_t s) {
   const int n = batch->n_pairs * B;
   int rc;
-  if (ks.mode == FSS_MODE_SHAPE || ks.mode == FSS_MODE_SHAPE_ESP) {
-    if (ks.mode == FSS_MODE_SHAPE)
-      hipLaunchKernelGGL(k_pose_score_f32<FSS_MODE_SHAPE>, dim3((n + 3) / 4), dim3(256), 0, s, *batch, ks, n, B,
-                         self_ov, out->q, out->t, out->pose_score);
-    else
-      hipLaunchKernelGGL(k_pose_score_f32<FSS_MODE_SHAPE_ESP>, dim3((n + 3) / 4), dim3(256), 0, s, *batch, ks, n, B,
-                         self_ov, out->q, out->t, out->pose_score);
+  if (ks.mode != FSS_MODE_ESP_COMBO) {
+    switch (ks.mode) {
+      case FSS_MODE_SHAPE: launch_prepass<FSS_MODE_SHAPE>(batch, ks, n, B, self_ov, out, s); break;
+      case FSS_MODE_SHAPE_ESP: launch_prepass<FSS_MODE_SHAPE_ESP>(batch, ks, n, B, self_ov, out, s); break;
+      case FSS_MODE_PHARM: launch_prepass<FSS_MODE_PHARM>(batch, ks, n, B, self_ov, out, s); break;
+      default: launch_prepass<FSS_MODE_SHAPE_PHARM>(batch, ks, n, B, self_ov, out, s); break;
+    }
     hipLaunchKernelGGL(k_rescore_threshold, dim3((batch->n_pairs + 255) / 256), dim3(256), 0, s, batch->n_pairs, B,
                        out->pose_score, out->best_score);
     if ((rc = check_launch("fss_align (re-scoring pre-pass)"))) return rc;
-    if (ks.mode == FSS_MODE_SHAPE)
-      launch_pose_score<FSS_MODE_SHAPE>(batch, ks, n, B, nullptr, self_ov, out->q, out->t, out->pose_score, s, out->best_score);
-    else
-      launch_pose_score<FSS_MODE_SHAPE_ESP>(batch, ks, n, B, nullptr, self_ov, out->q, out->t, out->pose_score, s, out->best_score);
+    switch (ks.mode) {
+      case FSS_MODE_SHAPE:
+        launch_pose_score<FSS_MODE_SHAPE>(batch, ks, n, B, nullptr, self_ov, out->q, out->t, out->pose_score, s,
+                                          out->best_score);
+        break;
+      case FSS_MODE_SHAPE_ESP:
+        launch_pose_score<FSS_MODE_SHAPE_ESP>(batch, ks, n, B, nullptr, self_ov, out->q, out->t, out->pose_score, s,
+                                              out->best_score);
+        break;
+      case FSS_MODE_PHARM:
+        launch_pose_score<FSS_MODE_PHARM>(batch, ks, n, B, nullptr, self_ov, out->q, out->t, out->pose_score, s,
+                                          out->best_score);
+        break;
+      default:
+        launch_pose_score<FSS_MODE_SHAPE_PHARM>(batch, ks, n, B, nullptr, self_ov, out->q, out->t, out->pose_score, s,
+                                                out->best_score);
+        break;
+    }
     rc = check_launch("fss_align (FP64 re-scoring)");
   } else {
     rc = pose_score_impl(batch, ks, n, nullptr, B, self_ov, out->q, out->t, out->pose_score, s);

@@ -146,9 +146,10 @@ typedef struct FssAlignOut {
   float*   q;                /* (n_pairs*B, 4) in: seeds, out: final raw quaternions         */
   float*   t;                /* (n_pairs*B, 3) in: seeds, out: final translations            */
   double*  pose_score;       /* (n_pairs*B) similarity of every final pose: FP64 sums for the
-                                restarts within 1e-3 of the pair's best (FP32 pre-pass; shape
-                                modes) and in every other mode; FP32 sums (error ~1e-6) for the
-                                rest.  The best restart is the FP64 argmax either way.         */
+                                restarts within 1e-3 of the pair's best (FP32 pre-pass; every
+                                mode but FSS_MODE_ESP_COMBO, which scores all in FP64); FP32 sums
+                                (error ~1e-6) for the rest.  The best restart is the FP64 argmax
+                                either way.                                                     */
   double*  best_score;       /* (n_pairs)                                                    */
   int32_t* best_idx;         /* (n_pairs) restart index of the best final pose               */
   float*   best_tf;          /* (n_pairs, 4, 4) SE(3) matrix of the best pose, row-major      */

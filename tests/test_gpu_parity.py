@@ -105,11 +105,12 @@ def test_persistent_align_matches_oracle(dev, mode, early_stop):
             assert nst[i] == steps
 
 
-@pytest.mark.parametrize("mode", [_lib.MODE_SHAPE, _lib.MODE_SHAPE_ESP])
+@pytest.mark.parametrize("mode", [_lib.MODE_SHAPE, _lib.MODE_SHAPE_ESP, _lib.MODE_PHARM, _lib.MODE_SHAPE_PHARM])
 def test_fp32_prepass_keeps_the_fp64_argmax(dev, mode):
-    """Shape modes re-score in FP64 only the restarts within 1e-3 of the pair's FP32 best: the best
-    restart and its score equal a full FP64 re-scoring of every restart (fss_pose_score), the
-    candidates' scores are bitwise the FP64 ones and the rest are within 1e-5 of FP64."""
+    """Every mode but ShaEP re-scores in FP64 only the restarts within 1e-3 of the pair's FP32
+    best: the best restart and its score equal a full FP64 re-scoring of every restart
+    (fss_pose_score), the candidates' scores are bitwise the FP64 ones and the rest are within
+    1e-5 of FP64."""
     print(f"case fp32 prepass mode={mode}", flush=True)
     n, B, steps = 8, 16, 60
     pb, spec, pairs = make_case(mode, n, dev, seed=21 + mode)
