@@ -193,10 +193,11 @@ def bench_c1(dev, reps: int = 3):
 
 
 def bench_c3(dev, n_pairs: int = 16384):
-    """Uneven shape + directional pharmacophore pairs in one persistent launch."""
+    """Uneven shape + directional pharmacophore pairs in one persistent launch, and the
+    hipGraph-captured stepped path on the same pairs."""
     import torch
     from fast_shepherd_score_amd import _lib, synth
-    from fast_shepherd_score_amd.align import OptimSpec, ScoreSpec, align, self_overlaps
+    from fast_shepherd_score_amd.align import OptimSpec, ScoreSpec, SteppedAligner, align, self_overlaps
     from fast_shepherd_score_amd.packing import CloudSet, PairBatch, PharmSet
     from fast_shepherd_score_amd.seeds import rotmat
     d = synth.pharm_shape_pairs(n_pairs, seed=4)
@@ -225,9 +226,30 @@ def bench_c3(dev, n_pairs: int = 16384):
                     for a, b in zip(d["ref_types"], d["fit_types"]))
     ph_terms = float(same_type) * B * 200
     kms = ev[0][0].elapsed_time(ev[0][1])
+    # the configuration's hipGraph-captured stepped path: 200 x (loss + grad, Adam) launches over
+    # every pose, recorded once and replayed
+    st = SteppedAligner(pb, spec, qd, td, opt, self_ov=self_overlaps(pb, spec))
+    st.capture()
+
+    def graph_run():
+        st.q.copy_(qd)
+        st.t.copy_(td)
+        st.m.zero_()
+        st.v.zero_()
+        return st.run(use_graph=True)
+
+    graph_run()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    rg = graph_run()
+    torch.cuda.synchronize()
+    dtg = time.perf_counter() - t0
     return {"workload": f"{n_pairs} uneven pairs: 5-60 atoms, 2-20 pharmacophores (8 types, directional), "
-                        "(1-w) shape + w pharm Tanimoto, w=0.5, 50 restarts x 200 steps, one persistent launch",
-            "pairs_per_s": n_pairs / dt, "kernel_ms": kms, "lanes_per_pose": res.extra["lanes_per_pose"],
+                        "(1-w) shape + w pharm Tanimoto, w=0.5, 50 restarts x 200 steps, one persistent launch; "
+                        "the hipGraph-captured stepped path on the same pairs and seeds beside it",
+            "pairs_per_s": n_pairs / dt, "graph_stepped_pairs_per_s": n_pairs / dtg,
+            "max_abs_score_diff_graph_vs_persistent": float((rg.score - res.score).abs().max()),
+            "kernel_ms": kms, "lanes_per_pose": res.extra["lanes_per_pose"],
             "restarts_per_wg": res.extra["restarts_per_wg"], "inner_loop_variant": res.extra["variant"],
             "achieved_tflops": (shape_terms * 16 + ph_terms * 36) / (kms * 1e-3) / 1e12,
             "pair_terms": {"shape": shape_terms, "pharm_same_type": ph_terms},

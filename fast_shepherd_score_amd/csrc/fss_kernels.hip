@@ -1825,12 +1825,18 @@ void launch_self(const FssPairBatch* pb, const KScore& ks, double* out, hipStrea
   hipLaunchKernelGGL(k_self_overlap<MODE>, dim3(pb->n_pairs, 2), dim3(kPoseBlock), 0, s, *pb, ks, out);
 }
 
+// Staged clouds whose fit rows (shape and pharmacophore) fit one wavefront run one wavefront per
+// pose: a lane owns the same fit rows as in four wavefronts, whose other lanes would only add
+// zeros, so the sums and their bits are the same and the pose needs no exchange across
+// wavefronts.  (The staging centroid is summed independently of the workgroup size.)
 template <int MODE, bool GRAD, int OUT>
 void launch_pose(const FssPairBatch* pb, const KScore& ks, LdsPlan lp, int n, const int32_t* pp,
                  const float* q, const float* t, const double* so, float invB, float* V, float* gR,
                  float* gT, float* gQ, float* loss, float* grad, hipStream_t s) {
-  hipLaunchKernelGGL((k_pose_grad<MODE, GRAD, OUT>), dim3(n), dim3(kPoseBlock), lp.end * sizeof(float4), s,
-                     *pb, ks, lp, pp, q, t, so, invB, V, gR, gT, gQ, loss, grad);
+  const bool one_wave = MODE != FSS_MODE_ESP_COMBO && lp.tile_ref == 0 && pb->max_fit <= 64 &&
+                        pb->max_fit_ph <= 64;
+  hipLaunchKernelGGL((k_pose_grad<MODE, GRAD, OUT>), dim3(n), dim3(one_wave ? 64 : kPoseBlock),
+                     lp.end * sizeof(float4), s, *pb, ks, lp, pp, q, t, so, invB, V, gR, gT, gQ, loss, grad);
 }
 
 // Workgroups of up to 512 threads get a 256-VGPR budget, larger ones 128.  Shape inner loop:
