@@ -457,18 +457,25 @@ class SteppedAligner:
 
 def finish(pb: PairBatch, spec: ScoreSpec, q: torch.Tensor, t: torch.Tensor,
            self_ov: torch.Tensor, n_steps: torch.Tensor, stream=None) -> AlignResult:
-    """FP64 similarity of every final pose, argmax over restarts, SE(3) matrix of the best."""
+    """Final scores and the best restart of every pair, as the persistent launch takes them
+    (fss_align_finish): the FP32 similarity of every final pose, FP64 for the restarts within
+    1e-3 of the pair's best (every restart in the ShaEP mode), the FP64 argmax over the
+    restarts and the SE(3) matrix of the best.  q (P, B, 4), t (P, B, 3) float32."""
     P, B = q.shape[0], q.shape[1]
-    ps = torch.empty((P, B), dtype=torch.float64, device=pb.device)
+    dev = pb.device
+    q, t = q.contiguous(), t.contiguous()
+    ps = torch.empty((P, B), dtype=torch.float64, device=dev)
+    best = torch.empty(P, dtype=torch.float64, device=dev)
+    bidx = torch.empty(P, dtype=torch.int32, device=dev)
+    tf = torch.empty((P, 4, 4), dtype=torch.float32, device=dev)
+    out = _lib.FssAlignOut()
+    out.q, out.t = _lib.ptr(q), _lib.ptr(t)
+    out.pose_score, out.best_score, out.best_idx = _lib.ptr(ps), _lib.ptr(best), _lib.ptr(bidx)
+    out.best_tf, out.n_steps = _lib.ptr(tf), _lib.ptr(n_steps)
     bs, sp = pb.struct(), spec.struct()
-    _lib.call("fss_pose_score", _lib.byref(bs), _lib.byref(sp), P * B, 0, B, _lib.ptr(self_ov),
-              _lib.ptr(q), _lib.ptr(t), _lib.ptr(ps), _stream(stream))
-    bidx = torch.argmax(ps, dim=1)
-    best = ps.gather(1, bidx[:, None])[:, 0]
-    qb = q[torch.arange(P, device=q.device), bidx].double()
-    tb = t[torch.arange(P, device=q.device), bidx]
-    tf = se3_matrix(qb, tb.double()).float()
-    return AlignResult(best, bidx.int(), tf, q, t, ps, n_steps)
+    _lib.call("fss_align_finish", _lib.byref(bs), _lib.byref(sp), B, _lib.ptr(self_ov), _lib.byref(out),
+              _stream(stream))
+    return AlignResult(best, bidx, tf, q, t, ps, n_steps)
 
 
 def align_stepped(pb, spec, q0, t0, opt: OptimSpec = OptimSpec(), use_graph: bool = True,
