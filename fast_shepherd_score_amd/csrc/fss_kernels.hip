@@ -1289,7 +1289,7 @@ __device__ __forceinline__ bool score_far(const T* __restrict__ A, int N, const 
     float hm = 0.f;
     for (int w = 0; w < (int)(blockDim.x >> 6); ++w) hm = fmaxf(hm, red[w]);
     __syncthreads();   // the caller reuses red
-    return !(hm <= kScoreHMax);   // NaN points take the direct form
+    return !(hm <= kScoreHMax);   // fmaxf skips NaN points (their sums are NaN in either form)
   }
 }
 
