@@ -1,8 +1,8 @@
 """fss_align_noalloc (no device allocation; self-overlaps and re-scoring buffer in the library's
 static per-GPU scratch) against fss_align with caller-allocated buffers (needs an MI355X).
 
-Two calls in a row through the C ABI, the second with a smaller batch (and another mode) than
-the first, so the second runs on scratch that still holds the first call's contents: each must
+Three calls in a row through the C ABI, each with a smaller batch (and another mode) than the
+one before, so each runs on scratch that still holds the earlier calls' contents: each must
 equal fss_align bit for bit (the same kernels on the same inputs).  Oversized batches and
 skip_finish are rejected before any launch.
 """
@@ -50,10 +50,11 @@ def noalloc(pb, spec, q0, t0, opt, lpp, rpw, variant, skip_finish=0):
 CALLS = [  # (mode, pairs, restarts, steps, early stop, lanes per pose, restarts per workgroup, variant)
     (_lib.MODE_SHAPE_ESP, 12, 10, 60, True, 16, 10, 2),
     (_lib.MODE_SHAPE, 5, 6, 40, False, 64, 2, 3),
+    (_lib.MODE_SHAPE_PHARM, 4, 5, 30, True, 1, 5, 2),
 ]
 
 
-def test_noalloc_matches_fss_align_twice_in_a_row(dev):
+def test_noalloc_matches_fss_align_calls_in_a_row(dev):
     for k, (mode, n, B, steps, es, lpp, rpw, var) in enumerate(CALLS):
         print(f"case noalloc call {k}: mode {mode}, {n} pairs x {B} restarts", flush=True)
         pb, spec, _ = make_case(mode, n, dev, seed=60 + k)
