@@ -131,9 +131,10 @@ def test_clouds_beyond_lds_stream(dev, dtype, N, M):
         np.testing.assert_allclose(g, w, atol=gtol * np.abs(w).max())
 
 
-def _score_ragged(clouds, lam, dev):
-    """fss_score_fwd / fss_score_bwd (FP32, ESP) through the C ABI on a ragged batch of
-    (ref (N, 3), fit (M, 3), ref_esp (N,), fit_esp (M,)) pairs: S, overlaps, dS/dfit, dS/dref."""
+def _score_ragged(clouds, lam, dev, esp=True):
+    """fss_score_fwd / fss_score_bwd (FP32; ESP-weighted or not) through the C ABI on a ragged
+    batch of (ref (N, 3), fit (M, 3), ref_esp (N,), fit_esp (M,)) pairs: S, overlaps, dS/dfit,
+    dS/dref."""
     from fast_shepherd_score_amd import _lib
     pack = lambda xs, ws: torch.cat([torch.cat([x, w[:, None]], 1) for x, w in zip(xs, ws)]).float().to(dev).contiguous()
     rng = lambda ns: torch.tensor([[sum(ns[:i]), n] for i, n in enumerate(ns)], dtype=torch.int32, device=dev)
@@ -144,7 +145,7 @@ def _score_ragged(clouds, lam, dev):
     pb.n_pairs, pb.ref_pts, pb.fit_pts, pb.ref_rng, pb.fit_rng = len(clouds), _lib.ptr(rp), _lib.ptr(fp), _lib.ptr(rr), _lib.ptr(fr)
     pb.max_ref, pb.max_fit = max(ns), max(ms)
     sp = _lib.FssScoreParams()
-    sp.mode, sp.alpha, sp.lam = _lib.MODE_SHAPE_ESP, 0.81, lam
+    sp.mode, sp.alpha, sp.lam = (_lib.MODE_SHAPE_ESP if esp else _lib.MODE_SHAPE), 0.81, lam
     P = len(clouds)
     S = torch.empty(P, device=dev)
     ov = torch.empty((P, 4), device=dev)
@@ -159,15 +160,16 @@ def _score_ragged(clouds, lam, dev):
     return S.cpu(), ov.cpu(), d_fit.cpu(), d_ref.cpu()
 
 
-def test_streamed_equals_staged_bitwise(dev):
+@pytest.mark.parametrize("esp", [False, True])
+def test_streamed_equals_staged_bitwise(dev, esp):
     """The streamed kernels take the staged kernels' sums in the same order: a small pair gives
     the same bits when its batch also holds a pair beyond LDS (which streams the whole batch)."""
-    print("case score-as-loss staged vs streamed, 300 x 280 beside 6000 x 4500", flush=True)
+    print(f"case score-as-loss staged vs streamed esp={esp}, 300 x 280 beside 6000 x 4500", flush=True)
     small = [x[0] for x in _clouds(1, 300, 280, 11, torch.float64)]
     big = [x[0] for x in _clouds(1, 6000, 4500, 12, torch.float64)]
     big[0], big[1] = big[0] * 6.0, big[1] * 6.0
-    S1, ov1, df1, dr1 = _score_ragged([small], 0.5, dev)
-    S2, ov2, df2, dr2 = _score_ragged([small, big], 0.5, dev)
+    S1, ov1, df1, dr1 = _score_ragged([small], 0.5, dev, esp)
+    S2, ov2, df2, dr2 = _score_ragged([small, big], 0.5, dev, esp)
     assert torch.isfinite(S2).all()
     assert torch.equal(S1[0], S2[0]) and torch.equal(ov1[0], ov2[0])
     assert torch.equal(df1, df2[:280]) and torch.equal(dr1, dr2[:300])
