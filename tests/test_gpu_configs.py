@@ -4,7 +4,7 @@
 The oracle's alignments of these pairs take minutes on a CPU, so tests/golden/make_config_vectors.py
 ran them once and stored inputs and results (config_*.npz).  Each test aligns the same pairs from
 the same seeds on the GPU and checks, per pair, that the best-restart score agrees with the
-oracle's within 1e-3.  Live checks on the GPU box: the FP32 per-pose loss and SE(3) gradient at the
+oracle's within 1e-3.  Live checks on the GPU: the FP32 per-pose loss and SE(3) gradient at the
 seeds against the oracle's (loss within 1e-4, gradient within 2e-4 of its largest component), and
 the FP64 re-scoring of the final HIP poses against the oracle's scoring restatement (1e-4).
 
