@@ -1,8 +1,8 @@
 """FP64 CPU oracle of the ShEPhERD scoring + alignment hot path.
 
 TEST INFRASTRUCTURE ONLY.  Only ``tests/``, ``__graft_entry__.smoke()`` and the ``cpu_baseline``
-leg of ``bench.py`` may import this module, and only as the checker -- never as the thing measured
-or shipped.  The product (``fast_shepherd_score_amd``) never imports it.
+leg of ``bench.py`` may import this module, and only to check results against -- never as the
+thing measured or shipped.  The product (``fast_shepherd_score_amd``) never imports it.
 
 A restatement, in torch float64 on the CPU, of:
   * score/analytical_gradients/_torch.py:28-1017  (R(q), dR/dq, projection, overlap + analytic
